@@ -1,0 +1,232 @@
+/*
+ * mld_ainv_hip.h -- C ABI of libmldainv_hip.so, the MI355X-native
+ * implementation of the MLD2P4 approximate-inverse sub-solvers
+ * (mld_d_ainv_solver, mld_d_invk_solver, mld_d_invt_solver).
+ *
+ * One handle = one solver object (the reference's `sv`).  Every entry point
+ * returns an int info code (0 = success, AINV_ERR_* otherwise, mirroring the
+ * PSBLAS error categories); ainv_last_error() returns the message of the
+ * last failure on the calling thread.  Matrices cross the boundary as CSR
+ * with int64 row pointers, int32 column indices and FP64 values, 0- or
+ * 1-based (index_base).  Vectors handed to the apply / Krylov calls are
+ * device pointers unless AINV_HOST_PTRS is set.  A block of k right-hand
+ * sides is stored row-interleaved: entry (i, c) at x[i*ldx + c], ldx >= k.
+ *
+ * Reference interfaces replaced (files of sfilippone/mld-ainv):
+ *   type-bound procedures of mld_d_base_ainv_solver_type
+ *     src/mld_d_base_ainv_mod.F90:45-68 and its extensions
+ *     src/mld_d_ainv_solver.F90:46-67, src/mld_d_invk_solver.f90:47-57,
+ *     src/mld_d_invt_solver.f90:46-58.
+ */
+#ifndef MLD_AINV_HIP_H
+#define MLD_AINV_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- info codes ---------------------------------------------------- */
+#define AINV_SUCCESS 0
+#define AINV_ERR_ARG 1      /* invalid argument (psb_err_iarg_invalid_i_) */
+#define AINV_ERR_ALG 2      /* invalid algorithm variant */
+#define AINV_ERR_ALLOC 3    /* allocation failure (psb_err_alloc_request_) */
+#define AINV_ERR_INTERNAL 4 /* internal error (psb_err_internal_error_) */
+#define AINV_ERR_PIVOT 5    /* zero pivot in the incomplete factorization */
+#define AINV_ERR_STATE 6    /* call not valid in the object's state */
+#define AINV_ERR_HIP 7      /* HIP runtime failure or no usable device */
+#define AINV_ERR_CAPACITY 8 /* a working set exceeded the kernel capacity */
+
+/* ---- solver kinds -------------------------------------------------- */
+#define AINV_KIND_AINV 1 /* mld_d_ainv_solver_type (biconjugation) */
+#define AINV_KIND_INVK 2 /* mld_d_invk_solver_type (inverse of ILU(k)) */
+#define AINV_KIND_INVT 3 /* mld_d_invt_solver_type (inverse of ILUT) */
+
+/* ---- AINV algorithm variants, in the order of the reference's ids
+ * (src/mld_base_ainv_mod.F90:40-60; the absolute values there are offsets
+ * from MLD2P4 constants, these are this library's own) ---------------- */
+#define AINV_ALG_LLK 1      /* 'LLK'      left-looking */
+#define AINV_ALG_S_LLK 2    /* 'SYM-LLK'  symmetric left-looking, W = Z */
+#define AINV_ALG_S_FT_LLK 3 /* 'STAB-LLK' stabilized left-looking */
+#define AINV_ALG_LLK_NOTH 4 /* left-looking without the alpha drop */
+#define AINV_ALG_MLK 5      /* 'MLK'      left-looking, list merge */
+
+/* ---- factor storage precision (the MI355X counterpart of the
+ * reference's amold/vmold re-mold, mld_d_ainv_solver_bld.f90:71-82) ---- */
+#define AINV_PREC_F64 0  /* FP64 values, FP64 d, FP64 intermediate */
+#define AINV_PREC_F32 1  /* FP32 values and d, FP64 accumulation */
+#define AINV_PREC_BF16 2 /* BF16 values, FP32 d, FP32 accumulation */
+
+/* ---- flags --------------------------------------------------------- */
+#define AINV_HOST_PTRS 1 /* x/y (or b/x) are host pointers */
+
+typedef struct ainv_solver *ainv_h;
+typedef struct ainv_matrix *ainv_mat_h;
+
+/* Last error message of the calling thread ("" if none). */
+const char *ainv_last_error(void);
+
+/* Library / device set-up: selects the HIP device used by handles created
+ * afterwards on this thread.  Returns AINV_ERR_HIP if no device is usable. */
+int ainv_init(int device);
+
+/* Create / destroy.  create also applies the defaults (d_*_solver_default,
+ * mld_d_ainv_solver.F90:206-220, mld_d_invk_solver.f90:243-256,
+ * mld_d_invt_solver.f90:258-272). free releases everything
+ * (mld_d_base_ainv_solver_free.f90). */
+int ainv_create(int kind, ainv_h *h);
+int ainv_free(ainv_h h);
+int ainv_default(ainv_h h);
+int ainv_get_kind(ainv_h h, int *kind);
+
+/* Setters keyed by case-insensitive strings (sv%cseti / sv%csetr /
+ * sv%csetc: mld_d_ainv_solver_cseti.f90:55-62, _csetr.f90:55-60,
+ * _csetc.f90:56-61, mld_d_invk_solver_cseti.f90:55-62,
+ * mld_d_invt_solver_cseti.f90:55-62, mld_d_invt_solver_csetr.f90:55-62).
+ * Keys: SUB_FILLIN, AINV_ALG, SUB_ILUTHRS, INV_FILLIN, INV_THRESH, plus
+ * FACTOR_PREC ('F64'|'F32'|'BF16' or AINV_PREC_*).  Unknown keys are
+ * passed to the base solver, which ignores them (info = 0). */
+int ainv_cseti(ainv_h h, const char *what, int val);
+int ainv_csetr(ainv_h h, const char *what, double val);
+int ainv_csetc(ainv_h h, const char *what, const char *val);
+/* Getters for the same keys (integer and real). */
+int ainv_geti(ainv_h h, const char *what, int *val);
+int ainv_getr(ainv_h h, const char *what, double *val);
+
+/* sv%check: coerce illegal parameters to their defaults
+ * (mld_d_ainv_solver_check.f90:53-56, mld_d_invk_solver_check.f90,
+ * mld_d_invt_solver_check.f90). */
+int ainv_check(ainv_h h);
+
+/* sv%descr: text description into buf (NUL-terminated, truncated to len)
+ * (mld_d_ainv_solver_descr.f90:63-66, mld_d_invk_solver_descr.f90:63-65,
+ * mld_d_invt_solver_descr.f90:63-67). */
+int ainv_descr(ainv_h h, char *buf, size_t len);
+
+/* AINV variant name <-> id (d_ainv_stringval / d_ainv_algname,
+ * mld_d_ainv_solver.F90:232-292). stringval returns -1 if unknown. */
+int ainv_stringval(const char *name);
+int ainv_algname(int alg, char *buf, size_t len);
+
+/* sv%build (mld_d_ainv_solver_bld.f90:68-82 -> mld_d_ainv_bld.F90,
+ * mld_d_invk_solver_bld.f90 -> mld_d_invk_bld.f90, mld_d_invt_solver_bld
+ * .f90 -> mld_d_invt_bld.f90).  The n x n local block in host CSR; the
+ * factors end up resident on the device in the storage precision set by
+ * FACTOR_PREC.  Calls check first. */
+int ainv_build_csr(ainv_h h, int32_t n, const int64_t *irp, const int32_t *ja,
+                   const double *val, int index_base);
+
+/* sv%cnv: re-mold the resident factors to another storage precision
+ * (mld_d_base_ainv_solver_cnv.f90:56-60). */
+int ainv_cnv(ainv_h h, int prec);
+
+/* sv%apply_v / sv%apply_a (mld_d_base_ainv_solver_apply_vect.f90:110-143,
+ * mld_d_base_ainv_solver_apply.f90:100-121):
+ *   trans 'N': y = alpha * Z * (d o (W * x)) + beta * y
+ *   trans 'T'/'C': y = alpha * W^T * (d o (Z^T * x)) + beta * y
+ * for k right-hand sides (1 <= k <= 64) at once.  beta == 0 overwrites y
+ * without reading it.  Asynchronous on `stream` (NULL = the handle's
+ * stream) unless AINV_HOST_PTRS is set. */
+int ainv_apply(ainv_h h, double alpha, const double *x, double beta, double *y,
+               char trans, int k, int64_t ldx, int64_t ldy, int flags,
+               void *stream);
+
+/* Accessors (mld_d_base_ainv_mod.F90:179-214). */
+int ainv_get_nzeros(ainv_h h, int64_t *nz);  /* n + nnz(W) + nnz(Z) */
+int ainv_sizeof(ainv_h h, int64_t *bytes);   /* resident bytes */
+int ainv_get_wrksz(ainv_h h, int *wrksz);    /* = 2 work vectors */
+int ainv_get_n(ainv_h h, int32_t *n);
+
+/* clone / clone_settings / clear_data (mld_d_ainv_solver_clone.f90:57-82,
+ * _clone_settings.f90, mld_d_base_ainv_solver_clear_data.f90:52-62). */
+int ainv_clone(ainv_h h, ainv_h *out);
+int ainv_clone_settings(ainv_h h, ainv_h out);
+int ainv_clear_data(ainv_h h);
+
+/* Factor export for parity checks and dumps (the data sv%dump writes,
+ * mld_d_base_ainv_solver_dmp.f90:77-106).  which: 'W' (stored W, i.e. W^T
+ * for AINV and L^{-1} for INVK/INVT), 'Z', or 'D'.  Call with NULL arrays
+ * to get n and nnz, then with arrays of that size (0-based, FP64). */
+int ainv_export(ainv_h h, char which, int32_t *n, int64_t *nnz, int64_t *irp,
+                int32_t *ja, double *val);
+
+/* Timing and layout report of the last build (JSON text). */
+int ainv_report(ainv_h h, char *buf, size_t len);
+
+/* ---- system matrix and Krylov driver (the PSBLAS psb_krylov call the
+ * reference drivers make, tests/pdegen/ppde3d.f90:280) ----------------- */
+int ainv_mat_create(int32_t n, const int64_t *irp, const int32_t *ja,
+                    const double *val, int index_base, ainv_mat_h *a);
+int ainv_mat_free(ainv_mat_h a);
+/* y = alpha*A*x + beta*y for k interleaved columns (device pointers). */
+int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
+                  double *y, int k, int64_t ldx, int64_t ldy, void *stream);
+
+/* method: "BICGSTAB" or "CG"; istop = 2 (||r||_2 <= eps ||b||_2); x holds
+ * x0 on entry.  k interleaved right-hand sides, each with its own
+ * recurrence and stopping test; iter[c] and err[c] (host arrays of k) get
+ * the iteration count and final ||r_c||/||b_c|| of column c.  prec may be
+ * NULL (no preconditioner). */
+int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
+                double *x, int k, double eps, int itmax, int istop,
+                int32_t *iter, double *err, int flags, void *stream);
+
+/* ---- synthetic matrices of the benchmark configurations (host CSR).
+ * Call with NULL arrays to get n and nnz, then with allocated arrays. ---- */
+/* 3-D convection-diffusion -div(grad u) + b.grad u, b = (1,1,1)/sqrt(3),
+ * 7-point centred differences on an N^3 interior grid; hconv 0 uses
+ * h = 1/(N+1), hconv 1 uses the reference generator's h = 1/(N+2)
+ * (tests/pdegen/genpde_mod.f90:141, :330-389). */
+int ainv_gen_pde3d(int32_t N, int hconv, int32_t *n, int64_t *nnz,
+                   int64_t *irp, int32_t *ja, double *val);
+/* 2-D anisotropic diffusion -(eps u_xx + u_yy), 5-point, h = 1/(N+1). */
+int ainv_gen_aniso2d(int32_t N, double eps, int32_t *n, int64_t *nnz,
+                     int64_t *irp, int32_t *ja, double *val);
+/* Random irregular nonsymmetric matrix: off-diagonal counts
+ * round(LogNormal(ln(median), sigma)) capped at cap, columns uniform in
+ * [i-band, i+band] plus nlong uniform long-range columns, N(0,1) values,
+ * diagonal = dom * row absolute sum. */
+int ainv_gen_random(int32_t nrows, uint64_t seed, double median, double sigma,
+                    int32_t cap, int32_t band, int32_t nlong, double dom,
+                    int32_t *n, int64_t *nnz, int64_t *irp, int32_t *ja,
+                    double *val);
+/* n*k N(0,1) values from the library RNG (xoshiro256** seeded by
+ * SplitMix64), so CPU and GPU paths see byte-identical inputs. */
+int ainv_gen_normal(uint64_t seed, int64_t count, double *out);
+
+/* ---- device plumbing for hosts without a GPU array library (Python via
+ * ctypes, Fortran via iso_c_binding).  No counterpart in the reference,
+ * where PSBLAS vectors own their storage. -------------------------------- */
+#define AINV_H2D 1
+#define AINV_D2H 2
+#define AINV_D2D 3
+int ainv_dev_alloc(size_t bytes, void **p);
+int ainv_dev_free(void *p);
+int ainv_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int ainv_memset(void *p, int value, size_t bytes);
+int ainv_device_sync(void);
+/* Runs `reps` applies of h on device vectors and returns the average
+ * duration (ms) of each of its two kernels, timed with HIP events on the
+ * stream they are launched on: ms_first = d o (W x) (or Z^T for 'T'),
+ * ms_second = alpha Z t + beta y. */
+int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
+                       double *ms_first, double *ms_second);
+/* Same for the system matrix product y = A x. */
+int ainv_mat_profile(ainv_mat_h a, const double *x, double *y, int k, int reps,
+                     double *ms);
+/* Average device time (ms) of `reps` Krylov solves from x0 = 0 with the
+ * last solve's iteration counts and errors (timed with HIP events). */
+int ainv_krylov_profile(const char *method, ainv_mat_h a, ainv_h prec,
+                        const double *b, double *x, int k, double eps,
+                        int itmax, int reps, int32_t *iter, double *err,
+                        double *ms);
+/* Layout / traffic facts of the resident factors: n, nnz(W), nnz(Z),
+ * padded entries of each HLL, value bytes. */
+int ainv_layout(ainv_h h, int64_t *n, int64_t *nnz_w, int64_t *nnz_z,
+                int64_t *pad_w, int64_t *pad_z, int *value_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,522 @@
+"""MI355X-native AINV / INVK / INVT preconditioners: Python host mirror.
+
+The objects here mirror the MLD2P4 solver objects of the reference plugin
+(``mld_d_ainv_solver_type``, ``mld_d_invk_solver_type``,
+``mld_d_invt_solver_type``; src/mld_d_ainv_solver.F90:46-67,
+src/mld_d_invk_solver.f90:47-57, src/mld_d_invt_solver.f90:46-58) with the
+same procedure names (build, apply, cseti/csetr/csetc/set, check, descr,
+clone, clone_settings, clear_data, free, sizeof, get_nzeros, get_wrksz,
+cnv, dump) and the same error behaviour: every call that the reference
+reports through ``info /= 0`` raises :class:`AinvError` carrying that code.
+
+All computation runs in ``lib/libmldainv_hip.so`` (HIP kernels for gfx950);
+this module only marshals arguments.  If the library is missing it raises
+at import of the first object, it never computes anything itself.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmldainv_hip.so")
+
+# info codes (include/mld_ainv_hip.h)
+SUCCESS, ERR_ARG, ERR_ALG, ERR_ALLOC, ERR_INTERNAL, ERR_PIVOT, ERR_STATE, \
+    ERR_HIP, ERR_CAPACITY = range(9)
+KIND_AINV, KIND_INVK, KIND_INVT = 1, 2, 3
+ALG_LLK, ALG_S_LLK, ALG_S_FT_LLK, ALG_LLK_NOTH, ALG_MLK = 1, 2, 3, 4, 5
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+HOST_PTRS = 1
+H2D, D2H, D2D = 1, 2, 3
+
+
+class AinvError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"info={code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library; fails loudly if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AinvError(ERR_STATE, f"{LIB_PATH} is not built; run "
+                        "__graft_entry__.build() or make -C mld-ainv_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    P, vp = C.POINTER, C.c_void_p
+    i64p, i32p, dp = P(C.c_int64), P(C.c_int32), P(C.c_double)
+    sig = {
+        "ainv_init": [C.c_int],
+        "ainv_create": [C.c_int, P(vp)],
+        "ainv_free": [vp],
+        "ainv_default": [vp],
+        "ainv_get_kind": [vp, P(C.c_int)],
+        "ainv_cseti": [vp, C.c_char_p, C.c_int],
+        "ainv_csetr": [vp, C.c_char_p, C.c_double],
+        "ainv_csetc": [vp, C.c_char_p, C.c_char_p],
+        "ainv_geti": [vp, C.c_char_p, P(C.c_int)],
+        "ainv_getr": [vp, C.c_char_p, dp],
+        "ainv_check": [vp],
+        "ainv_descr": [vp, C.c_char_p, C.c_size_t],
+        "ainv_algname": [C.c_int, C.c_char_p, C.c_size_t],
+        "ainv_build_csr": [vp, C.c_int32, i64p, i32p, dp, C.c_int],
+        "ainv_cnv": [vp, C.c_int],
+        "ainv_apply": [vp, C.c_double, vp, C.c_double, vp, C.c_char, C.c_int,
+                       C.c_int64, C.c_int64, C.c_int, vp],
+        "ainv_get_nzeros": [vp, i64p],
+        "ainv_sizeof": [vp, i64p],
+        "ainv_get_wrksz": [vp, P(C.c_int)],
+        "ainv_get_n": [vp, i32p],
+        "ainv_clone": [vp, P(vp)],
+        "ainv_clone_settings": [vp, vp],
+        "ainv_clear_data": [vp],
+        "ainv_export": [vp, C.c_char, i32p, i64p, i64p, i32p, dp],
+        "ainv_report": [vp, C.c_char_p, C.c_size_t],
+        "ainv_mat_create": [C.c_int32, i64p, i32p, dp, C.c_int, P(vp)],
+        "ainv_mat_free": [vp],
+        "ainv_mat_spmm": [vp, C.c_double, vp, C.c_double, vp, C.c_int,
+                          C.c_int64, C.c_int64, vp],
+        "ainv_krylov": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_double,
+                        C.c_int, C.c_int, i32p, dp, C.c_int, vp],
+        "ainv_gen_pde3d": [C.c_int32, C.c_int, i32p, i64p, i64p, i32p, dp],
+        "ainv_gen_aniso2d": [C.c_int32, C.c_double, i32p, i64p, i64p, i32p, dp],
+        "ainv_gen_random": [C.c_int32, C.c_uint64, C.c_double, C.c_double,
+                            C.c_int32, C.c_int32, C.c_int32, C.c_double, i32p,
+                            i64p, i64p, i32p, dp],
+        "ainv_gen_normal": [C.c_uint64, C.c_int64, dp],
+        "ainv_dev_alloc": [C.c_size_t, P(vp)],
+        "ainv_dev_free": [vp],
+        "ainv_memcpy": [vp, vp, C.c_size_t, C.c_int],
+        "ainv_memset": [vp, C.c_int, C.c_size_t],
+        "ainv_device_sync": [],
+        "ainv_apply_profile": [vp, vp, vp, C.c_int, C.c_int, dp, dp],
+        "ainv_mat_profile": [vp, vp, vp, C.c_int, C.c_int, dp],
+        "ainv_krylov_profile": [C.c_char_p, vp, vp, vp, vp, C.c_int,
+                                C.c_double, C.c_int, C.c_int, i32p, dp, dp],
+        "ainv_layout": [vp, i64p, i64p, i64p, i64p, i64p, P(C.c_int)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.ainv_last_error.argtypes = []
+    L.ainv_last_error.restype = C.c_char_p
+    L.ainv_stringval.argtypes = [C.c_char_p]
+    L.ainv_stringval.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ck(rc):
+    if rc != SUCCESS:
+        raise AinvError(rc, lib().ainv_last_error().decode(errors="replace"))
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+_initialized = False
+
+
+def init(device=0):
+    """Selects the HIP device (raises ERR_HIP when no GPU is usable)."""
+    global _initialized
+    _ck(lib().ainv_init(device))
+    _initialized = True
+
+
+def _ensure_init():
+    if not _initialized:
+        init(0)
+
+
+# ----------------------------------------------------------------- memory
+class DeviceArray:
+    """FP64 array in device memory (n rows x k interleaved columns)."""
+
+    def __init__(self, shape, dtype=np.float64):
+        _ensure_init()
+        self.shape = tuple(shape) if np.ndim(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _ck(lib().ainv_dev_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if d.nbytes:
+            _ck(lib().ainv_memcpy(d.ptr, a.ctypes.data_as(C.c_void_p), d.nbytes,
+                                  H2D))
+        return d
+
+    def to_numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            _ck(lib().ainv_memcpy(out.ctypes.data_as(C.c_void_p), self.ptr,
+                                  self.nbytes, D2H))
+        return out
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _ck(lib().ainv_memcpy(self.ptr, a.ctypes.data_as(C.c_void_p),
+                              self.nbytes, H2D))
+
+    def zero(self):
+        _ck(lib().ainv_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            _ck(lib().ainv_dev_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None) and _lib is not None:
+                _lib.ainv_dev_free(self.ptr)
+        except Exception:
+            pass
+
+
+def synchronize():
+    _ck(lib().ainv_device_sync())
+
+
+# ------------------------------------------------------------------- CSR in
+def _csr_args(a):
+    """(n, irp int64, ja int32, val f64) from scipy CSR or a tuple."""
+    if isinstance(a, tuple):
+        n, irp, ja, val = a
+    else:
+        a = a.tocsr()
+        n, irp, ja, val = a.shape[0], a.indptr, a.indices, a.data
+    irp = np.ascontiguousarray(irp, dtype=np.int64)
+    ja = np.ascontiguousarray(ja, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    return int(n), irp, ja, val
+
+
+def _ptrs(irp, ja, val):
+    return (irp.ctypes.data_as(C.POINTER(C.c_int64)),
+            ja.ctypes.data_as(C.POINTER(C.c_int32)),
+            val.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+# ----------------------------------------------------------- solver objects
+class _BaseAinvSolver:
+    """mld_d_base_ainv_solver_type (src/mld_d_base_ainv_mod.F90:45-68)."""
+
+    KIND = None
+
+    def __init__(self, _handle=None):
+        if _handle is None:
+            h = C.c_void_p()
+            _ck(lib().ainv_create(self.KIND, C.byref(h)))
+            _handle = h
+        self._h = _handle
+
+    # -- parameters -------------------------------------------------------
+    def cseti(self, what, val):
+        _ck(lib().ainv_cseti(self._h, _b(what), int(val)))
+
+    def csetr(self, what, val):
+        _ck(lib().ainv_csetr(self._h, _b(what), float(val)))
+
+    def csetc(self, what, val):
+        _ck(lib().ainv_csetc(self._h, _b(what), _b(val)))
+
+    def set(self, what, val):
+        """Generic set dispatching on the value's type, as MLD2P4 does."""
+        if isinstance(val, str):
+            self.csetc(what, val)
+        elif isinstance(val, (int, np.integer)) and not isinstance(val, bool):
+            self.cseti(what, val)
+        else:
+            self.csetr(what, val)
+
+    def geti(self, what):
+        v = C.c_int()
+        _ck(lib().ainv_geti(self._h, _b(what), C.byref(v)))
+        return v.value
+
+    def getr(self, what):
+        v = C.c_double()
+        _ck(lib().ainv_getr(self._h, _b(what), C.byref(v)))
+        return v.value
+
+    def default(self):
+        _ck(lib().ainv_default(self._h))
+
+    def check(self):
+        _ck(lib().ainv_check(self._h))
+
+    def descr(self):
+        buf = C.create_string_buffer(2048)
+        _ck(lib().ainv_descr(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    # -- build / apply ----------------------------------------------------
+    def build(self, a, index_base=0):
+        _ensure_init()
+        n, irp, ja, val = _csr_args(a)
+        _ck(lib().ainv_build_csr(self._h, n, *_ptrs(irp, ja, val), index_base))
+
+    def cnv(self, prec):
+        if isinstance(prec, str):
+            prec = {"F64": PREC_F64, "F32": PREC_F32, "BF16": PREC_BF16}[
+                prec.upper()]
+        _ck(lib().ainv_cnv(self._h, int(prec)))
+
+    def apply(self, x, y=None, alpha=1.0, beta=0.0, trans="N"):
+        """y = alpha * prec(x) + beta * y.  x, y: numpy (host, n or n x k)
+        or DeviceArray (device, interleaved n x k).  Returns y."""
+        n = self.n
+        if isinstance(x, DeviceArray):
+            k = 1 if len(x.shape) == 1 else x.shape[1]
+            if y is None:
+                y = DeviceArray(x.shape)
+            _ck(lib().ainv_apply(self._h, alpha, x.ptr, beta, y.ptr,
+                                 _b(trans[0]), k, k, k, 0, None))
+            synchronize()
+            return y
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        k = 1 if x.ndim == 1 else x.shape[1]
+        if x.shape[0] != n:
+            raise AinvError(ERR_ARG, "x has the wrong number of rows")
+        y = np.zeros_like(x) if y is None else np.ascontiguousarray(
+            y, dtype=np.float64)
+        _ck(lib().ainv_apply(self._h, alpha, x.ctypes.data_as(C.c_void_p),
+                             beta, y.ctypes.data_as(C.c_void_p),
+                             _b(trans[0]), k, k, k, HOST_PTRS, None))
+        return y
+
+    # -- accessors --------------------------------------------------------
+    @property
+    def n(self):
+        v = C.c_int32()
+        _ck(lib().ainv_get_n(self._h, C.byref(v)))
+        return v.value
+
+    def get_nzeros(self):
+        v = C.c_int64()
+        _ck(lib().ainv_get_nzeros(self._h, C.byref(v)))
+        return v.value
+
+    def sizeof(self):
+        v = C.c_int64()
+        _ck(lib().ainv_sizeof(self._h, C.byref(v)))
+        return v.value
+
+    def get_wrksz(self):
+        v = C.c_int()
+        _ck(lib().ainv_get_wrksz(self._h, C.byref(v)))
+        return v.value
+
+    def layout(self):
+        vals = [C.c_int64() for _ in range(5)]
+        vb = C.c_int()
+        _ck(lib().ainv_layout(self._h, *[C.byref(v) for v in vals],
+                              C.byref(vb)))
+        n, nw, nz, pw, pz = (v.value for v in vals)
+        return dict(n=n, nnz_w=nw, nnz_z=nz, pad_w=pw, pad_z=pz,
+                    value_bytes=vb.value)
+
+    def report(self):
+        import json
+        buf = C.create_string_buffer(8192)
+        _ck(lib().ainv_report(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    # -- lifecycle ----------------------------------------------------------
+    def clone(self):
+        h = C.c_void_p()
+        _ck(lib().ainv_clone(self._h, C.byref(h)))
+        return type(self)(_handle=h)
+
+    def clone_settings(self, other):
+        _ck(lib().ainv_clone_settings(self._h, other._h))
+
+    def clear_data(self):
+        _ck(lib().ainv_clear_data(self._h))
+
+    def free(self):
+        if self._h:
+            _ck(lib().ainv_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib is not None:
+                _lib.ainv_free(self._h)
+        except Exception:
+            pass
+
+    # -- factors ------------------------------------------------------------
+    def export(self, which):
+        """Returns the stored factor 'W' or 'Z' as scipy CSR, or 'D' as a
+        numpy vector (FP64, as built)."""
+        import scipy.sparse as sp
+        n = C.c_int32()
+        nnz = C.c_int64()
+        _ck(lib().ainv_export(self._h, _b(which), C.byref(n), C.byref(nnz),
+                              None, None, None))
+        if which.upper() == "D":
+            d = np.empty(n.value)
+            _ck(lib().ainv_export(self._h, b"D", None, None, None, None,
+                                  d.ctypes.data_as(C.POINTER(C.c_double))))
+            return d
+        irp = np.empty(n.value + 1, dtype=np.int64)
+        ja = np.empty(max(nnz.value, 1), dtype=np.int32)
+        val = np.empty(max(nnz.value, 1), dtype=np.float64)
+        _ck(lib().ainv_export(self._h, _b(which), None, None,
+                              *_ptrs(irp, ja, val)))
+        return sp.csr_matrix((val[:nnz.value], ja[:nnz.value], irp),
+                             shape=(n.value, n.value))
+
+    def dump(self, prefix):
+        """Writes W, D and Z as MatrixMarket files, the data the reference's
+        sv%dump writes (mld_d_base_ainv_solver_dmp.f90:77-106)."""
+        import scipy.io
+        import scipy.sparse as sp
+        scipy.io.mmwrite(prefix + "_w.mtx", self.export("W"))
+        scipy.io.mmwrite(prefix + "_z.mtx", self.export("Z"))
+        d = self.export("D")
+        scipy.io.mmwrite(prefix + "_d.mtx", sp.diags(d).tocsr())
+
+
+class MldDAinvSolver(_BaseAinvSolver):
+    """mld_d_ainv_solver_type: AINV by sparse biconjugation."""
+    KIND = KIND_AINV
+
+    @staticmethod
+    def stringval(name):
+        return lib().ainv_stringval(_b(name))
+
+    @staticmethod
+    def algname(alg):
+        buf = C.create_string_buffer(64)
+        _ck(lib().ainv_algname(int(alg), buf, len(buf)))
+        return buf.value.decode()
+
+
+class MldDInvkSolver(_BaseAinvSolver):
+    """mld_d_invk_solver_type: inverse of ILU(k) factors with fill level."""
+    KIND = KIND_INVK
+
+
+class MldDInvtSolver(_BaseAinvSolver):
+    """mld_d_invt_solver_type: inverse of ILUT factors with threshold."""
+    KIND = KIND_INVT
+
+
+SOLVERS = {"AINV": MldDAinvSolver, "INVK": MldDInvkSolver,
+           "INVT": MldDInvtSolver}
+
+
+# ---------------------------------------------------- system matrix, Krylov
+class SystemMatrix:
+    """The operator A of the Krylov solve, resident on the device (FP64)."""
+
+    def __init__(self, a, index_base=0):
+        _ensure_init()
+        n, irp, ja, val = _csr_args(a)
+        h = C.c_void_p()
+        _ck(lib().ainv_mat_create(n, *_ptrs(irp, ja, val), index_base,
+                                  C.byref(h)))
+        self._h = h
+        self.n = n
+
+    def spmm(self, x, y, alpha=1.0, beta=0.0):
+        k = 1 if len(x.shape) == 1 else x.shape[1]
+        _ck(lib().ainv_mat_spmm(self._h, alpha, x.ptr, beta, y.ptr, k, k, k,
+                                None))
+        synchronize()
+        return y
+
+    def free(self):
+        if self._h:
+            _ck(lib().ainv_mat_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib is not None:
+                _lib.ainv_mat_free(self._h)
+        except Exception:
+            pass
+
+
+def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2):
+    """psb_krylov(method, a, prec, b, x, eps, ...) for k interleaved
+    right-hand sides.  a: SystemMatrix; prec: solver object or None;
+    b, x: numpy (host) or DeviceArray.  Returns (x, iters[k], errs[k])."""
+    host = not isinstance(b, DeviceArray)
+    if host:
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        k = 1 if b.ndim == 1 else b.shape[1]
+        x = np.zeros_like(b) if x is None else np.ascontiguousarray(
+            x, dtype=np.float64).copy()
+        bp, xp, flags = (b.ctypes.data_as(C.c_void_p),
+                         x.ctypes.data_as(C.c_void_p), HOST_PTRS)
+    else:
+        k = 1 if len(b.shape) == 1 else b.shape[1]
+        if x is None:
+            x = DeviceArray(b.shape)
+            x.zero()
+        bp, xp, flags = b.ptr, x.ptr, 0
+    iters = np.zeros(k, dtype=np.int32)
+    errs = np.zeros(k, dtype=np.float64)
+    _ck(lib().ainv_krylov(_b(method), a._h, prec._h if prec else None, bp, xp,
+                          k, eps, itmax, istop,
+                          iters.ctypes.data_as(C.POINTER(C.c_int32)),
+                          errs.ctypes.data_as(C.POINTER(C.c_double)), flags,
+                          None))
+    if not host:
+        synchronize()
+    return x, iters, errs
+
+
+# --------------------------------------------------------------- generators
+def _gen(fn, *args):
+    import scipy.sparse as sp
+    n = C.c_int32()
+    nnz = C.c_int64()
+    _ck(fn(*args, C.byref(n), C.byref(nnz), None, None, None))
+    irp = np.empty(n.value + 1, dtype=np.int64)
+    ja = np.empty(nnz.value, dtype=np.int32)
+    val = np.empty(nnz.value, dtype=np.float64)
+    _ck(fn(*args, None, None, *_ptrs(irp, ja, val)))
+    return sp.csr_matrix((val, ja, irp), shape=(n.value, n.value))
+
+
+def gen_pde3d(N, hconv=0):
+    """3-D convection-diffusion, 7-point (tests/pdegen/genpde_mod.f90)."""
+    return _gen(lib().ainv_gen_pde3d, int(N), int(hconv))
+
+
+def gen_aniso2d(N, eps=1e-3):
+    return _gen(lib().ainv_gen_aniso2d, int(N), float(eps))
+
+
+def gen_random(n, seed=40, median=12.0, sigma=1.0, cap=400, band=5000,
+               nlong=2, dom=1.1):
+    return _gen(lib().ainv_gen_random, int(n), int(seed), float(median),
+                float(sigma), int(cap), int(band), int(nlong), float(dom))
+
+
+def gen_normal(seed, shape):
+    count = int(np.prod(shape))
+    out = np.empty(count, dtype=np.float64)
+    _ck(lib().ainv_gen_normal(int(seed), count,
+                              out.ctypes.data_as(C.POINTER(C.c_double))))
+    return out.reshape(shape)

@@ -1,0 +1,171 @@
+// Internal declarations shared by the translation units of
+// libmldainv_hip.so.  Not part of the public ABI (include/mld_ainv_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mld_ainv_hip.h"
+
+namespace ainv {
+
+// Error carrying an AINV_ERR_* code; converted to an info code at the
+// C-ABI boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define AINV_HIP_CHECK(expr)                                                \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      throw ::ainv::Error(AINV_ERR_HIP, std::string(#expr) + ": " +         \
+                                            hipGetErrorString(e_));         \
+  } while (0)
+
+// Host CSR, 0-based, rows sorted by column.
+struct HostCsr {
+  int32_t n = 0;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> idx;
+  std::vector<double> val;
+  int64_t nnz() const { return ptr.empty() ? 0 : ptr.back(); }
+};
+
+// Owned device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t b);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Device CSR, FP64 values (the build output kept for export / re-mold).
+struct DevCsr {
+  int32_t n = 0;
+  int64_t nnz = 0;
+  DevBuf ptr, idx, val;
+};
+
+// Sliced ELLPACK ("HLL"): slices of 64 consecutive rows, one per
+// wavefront; inside a slice entries are stored column-major
+// (entry j of row 64*s + l at sptr[s] + 64*j + l) so that the 64 lanes of a
+// wave read consecutive words.  Padding entries carry value 0 and the row's
+// own index as column.  rlen holds each row's true length.
+constexpr int kSlice = 64;
+struct Hll {
+  int32_t n = 0;
+  int32_t nslices = 0;
+  int64_t nnz = 0;     // true entries
+  int64_t padded = 0;  // stored entries including padding
+  int32_t max_width = 0;
+  int prec = AINV_PREC_F64;
+  DevBuf sptr;  // int64 [nslices + 1]
+  DevBuf rlen;  // int32 [n]
+  DevBuf col;   // int32 [padded]
+  DevBuf val;   // VT    [padded]
+  int64_t bytes() const {
+    return (int64_t)(sptr.bytes + rlen.bytes + col.bytes + val.bytes);
+  }
+};
+
+int value_bytes(int prec);
+
+// ---- device-side helpers (hll.hip) --------------------------------------
+void csr_upload(const HostCsr &h, DevCsr &d, hipStream_t s);
+void csr_download(const DevCsr &d, HostCsr &h, hipStream_t s);
+// Transpose of a device CSR (rows sorted by column in the result).
+void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s);
+// HLL from device CSR, values converted to `prec`.
+void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s);
+// Convert n doubles to the storage type of prec (d vector).
+void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
+                    hipStream_t s);
+
+// ---- kernels launched by the apply / Krylov code (apply.hip) -----------
+// out = d o (A x)          (EPI scale, out type by prec: F64 double, else
+// float) or
+// y   = alpha * A x + beta*y (EPI axpby)  for k interleaved columns.
+void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
+                    int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s);
+void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
+                    int64_t ldx, double beta, double *y, int64_t ldy, int k,
+                    hipStream_t s);
+
+// ---- GPU row inverses (invkt.hip) ----------------------------------------
+// Row-wise approximate inverse of a unit upper triangular matrix given by
+// its strictly upper part `u` (device CSR).  kind AINV_KIND_INVK uses the
+// level-of-fill rule with fill; AINV_KIND_INVT the threshold + cap rule.
+void gpu_sparse_inverse(int kind, const DevCsr &u, int32_t fill, double thresh,
+                        DevCsr &z, hipStream_t s, std::string *report);
+
+// ---- host builds (build_host.cpp) ----------------------------------------
+void host_iluk(const HostCsr &a, int32_t fill, HostCsr &l, HostCsr &u,
+               std::vector<double> &d);
+void host_ilut(const HostCsr &a, int32_t fill, double thresh, HostCsr &l,
+               HostCsr &u, std::vector<double> &d);
+// AINV: returns stored W (= W^T, CSR), Z (CSR) and d.
+void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
+               HostCsr &w, HostCsr &z, std::vector<double> &d);
+
+double now_seconds();
+
+// ---- generators (gen.cpp) ---------------------------------------------------
+void gen_pde3d(int32_t N, int hconv, HostCsr &a);
+void gen_aniso2d(int32_t N, double eps, HostCsr &a);
+void gen_random(int32_t n, uint64_t seed, double median, double sigma,
+                int32_t cap, int32_t band, int32_t nlong, double dom,
+                HostCsr &a);
+void gen_normal(uint64_t seed, int64_t count, double *out);
+
+// ---- error plumbing at the C ABI ------------------------------------------
+// Message of the last failure on this thread (capi.cpp).
+std::string &last_error();
+
+// Runs f, mapping exceptions to info codes and recording the message.
+template <class F> int guard(F &&f) {
+  try {
+    f();
+    last_error().clear();
+    return AINV_SUCCESS;
+  } catch (const Error &e) {
+    last_error() = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    last_error() = "host allocation failed";
+    return AINV_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    last_error() = e.what();
+    return AINV_ERR_INTERNAL;
+  } catch (...) {
+    last_error() = "unknown failure";
+    return AINV_ERR_INTERNAL;
+  }
+}
+
+}  // namespace ainv

@@ -1,0 +1,441 @@
+// Host-side factorizations of the build phase: ILU(k), ILUT and the sparse
+// biconjugation of AINV.  These are sequential row/column recurrences
+// (each row or column depends on all earlier ones), so they run on the
+// host cores; their outputs are uploaded once and everything the Krylov
+// loop touches (inverse factors, apply, vector ops) lives on the GPU.  The
+// INVK/INVT row inverses, which ARE independent per row, run on the GPU
+// (invkt.hip).
+//
+// Arithmetic follows the reference operation by operation (one rounding
+// per multiply and per add, no contraction: this file is compiled with
+// -ffp-contract=off) so that patterns and values agree bit for bit with a
+// faithful restatement of:
+//   ILU(k):  MLD2P4 mld_iluk_fact as called from
+//            src/impl/mld_d_invk_bld.f90:104 (Saad Alg. 10.5, IKJ form)
+//   ILUT:    MLD2P4 mld_ilut_fact with maxval scaling,
+//            src/impl/mld_d_invt_bld.f90:108 (Saad Alg. 10.6)
+//   AINV:    src/impl/mld_d_ainv_bld.F90:115-182,
+//            src/impl/mld_dsparse_biconjg_llk.F90:82-364,
+//            src/impl/mld_dsparse_biconjg_s_llk.F90,
+//            src/impl/mld_d_sparsify.f90:83-143
+// Ties among equal magnitudes at a fill cap are resolved by "smaller column
+// first" (the reference leaves them to PSBLAS heap order).
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <queue>
+
+#include "ainv_internal.h"
+
+namespace ainv {
+
+double now_seconds() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+const double kEpsTol = DBL_EPSILON;  // d_epstol
+
+// ascending integer heap with repeats (the psb_i_heap use pattern)
+using MinHeap =
+    std::priority_queue<int32_t, std::vector<int32_t>, std::greater<int32_t>>;
+
+struct Entry {
+  double v;
+  int32_t j;
+};
+
+// |v| descending, then column ascending
+inline bool before(const Entry &a, const Entry &b) {
+  double fa = std::fabs(a.v), fb = std::fabs(b.v);
+  if (fa != fb) return fa > fb;
+  return a.j < b.j;
+}
+
+// keep at most `keep` entries of e (largest first), sorted by column
+void keep_largest(std::vector<Entry> &e, int64_t keep) {
+  if (keep < 0) keep = 0;
+  if ((int64_t)e.size() > keep) {
+    std::nth_element(e.begin(), e.begin() + keep, e.end(), before);
+    e.resize(keep);
+  }
+  std::sort(e.begin(), e.end(),
+            [](const Entry &a, const Entry &b) { return a.j < b.j; });
+}
+
+// classic reference BLAS dnrm2
+double nrm2(const double *x, int64_t n) {
+  if (n < 1) return 0.0;
+  if (n == 1) return std::fabs(x[0]);
+  double scale = 0.0, ssq = 1.0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (x[i] == 0.0) continue;
+    double ax = std::fabs(x[i]);
+    if (scale < ax) {
+      double r = scale / ax;
+      ssq = 1.0 + ssq * (r * r);
+      scale = ax;
+    } else {
+      double r = ax / scale;
+      ssq = ssq + r * r;
+    }
+  }
+  return scale * std::sqrt(ssq);
+}
+
+double max_abs(const HostCsr &a) {
+  double m = 0.0;
+  for (double v : a.val) m = std::max(m, std::fabs(v));
+  return m;
+}
+
+void transpose(const HostCsr &a, HostCsr &t) {
+  const int32_t n = a.n;
+  t.n = n;
+  t.ptr.assign(n + 1, 0);
+  t.idx.resize(a.nnz());
+  t.val.resize(a.nnz());
+  for (int64_t k = 0; k < a.nnz(); ++k) t.ptr[a.idx[k] + 1]++;
+  for (int32_t i = 0; i < n; ++i) t.ptr[i + 1] += t.ptr[i];
+  std::vector<int64_t> pos(t.ptr.begin(), t.ptr.end() - 1);
+  for (int32_t i = 0; i < n; ++i)
+    for (int64_t k = a.ptr[i]; k < a.ptr[i + 1]; ++k) {
+      int64_t q = pos[a.idx[k]]++;
+      t.idx[q] = i;
+      t.val[q] = a.val[k];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- ILU(k)
+void host_iluk(const HostCsr &a, int32_t fill, HostCsr &l, HostCsr &u,
+               std::vector<double> &d) {
+  const int32_t n = a.n;
+  const int32_t untouched = -(n + 1);
+  std::vector<double> row(n, 0.0);
+  std::vector<int32_t> lev(n, untouched), seen;
+  std::vector<int32_t> ulev;  // level of each stored U entry
+  d.assign(n, 0.0);
+  l = HostCsr();
+  u = HostCsr();
+  l.n = u.n = n;
+  l.ptr.assign(n + 1, 0);
+  u.ptr.assign(n + 1, 0);
+  MinHeap heap;
+  for (int32_t i = 0; i < n; ++i) {
+    for (int64_t e = a.ptr[i]; e < a.ptr[i + 1]; ++e) {
+      row[a.idx[e]] = a.val[e];
+      lev[a.idx[e]] = 0;
+      heap.push(a.idx[e]);
+    }
+    seen.clear();
+    int32_t last = -1;
+    while (!heap.empty()) {
+      int32_t k = heap.top();
+      heap.pop();
+      if (k == last) continue;
+      last = k;
+      seen.push_back(k);
+      if (k >= i || row[k] == 0.0 || lev[k] > fill) continue;
+      // U is stored scaled by the inverted pivots: the unscaled row(k)
+      // is the multiplier of the update, L(i,k) = row(k) * d(k).
+      const double rk = row[k];
+      const int32_t lk = lev[k];
+      for (int64_t jj = u.ptr[k]; jj < u.ptr[k + 1]; ++jj) {
+        const int32_t j = u.idx[jj];
+        if (lev[j] < 0) {
+          heap.push(j);
+          lev[j] = -lev[j];
+        }
+        row[j] = row[j] - rk * u.val[jj];
+        lev[j] = std::min(lev[j], lk + ulev[jj] + 1);
+      }
+    }
+    double piv = 0.0;
+    const int64_t ubeg = (int64_t)u.idx.size();
+    for (int32_t j : seen) {
+      if (j < i) {
+        if (lev[j] <= fill) {
+          l.idx.push_back(j);
+          l.val.push_back(row[j] * d[j]);
+        }
+      } else if (j == i) {
+        piv = row[j];
+      } else if (lev[j] <= fill) {
+        u.idx.push_back(j);
+        u.val.push_back(row[j]);
+        ulev.push_back(lev[j]);
+      }
+      row[j] = 0.0;
+      lev[j] = untouched;
+    }
+    if (piv == 0.0)
+      throw Error(AINV_ERR_PIVOT,
+                  "ILU(k): zero pivot at row " + std::to_string(i + 1));
+    d[i] = 1.0 / piv;
+    for (int64_t e = ubeg; e < (int64_t)u.val.size(); ++e)
+      u.val[e] = u.val[e] * d[i];
+    l.ptr[i + 1] = (int64_t)l.idx.size();
+    u.ptr[i + 1] = (int64_t)u.idx.size();
+  }
+}
+
+// ------------------------------------------------------------------ ILUT
+void host_ilut(const HostCsr &a, int32_t fill, double thresh, HostCsr &l,
+               HostCsr &u, std::vector<double> &d) {
+  const int32_t n = a.n;
+  const double scale = 1.0 / max_abs(a);
+  std::vector<double> row(n, 0.0), rv;
+  std::vector<char> flag(n, 0);
+  std::vector<int32_t> kept;
+  std::vector<Entry> cand;
+  d.assign(n, 0.0);
+  l = HostCsr();
+  u = HostCsr();
+  l.n = u.n = n;
+  l.ptr.assign(n + 1, 0);
+  u.ptr.assign(n + 1, 0);
+  MinHeap heap;
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t nlw = 0, nup = 0;
+    rv.clear();
+    for (int64_t e = a.ptr[i]; e < a.ptr[i + 1]; ++e) {
+      const int32_t k = a.idx[e];
+      row[k] = a.val[e] * scale;
+      rv.push_back(row[k]);
+      heap.push(k);
+      flag[k] = 1;
+      nlw += (k < i);
+      nup += (k > i);
+    }
+    const double thr = thresh * nrm2(rv.data(), (int64_t)rv.size());
+    kept.clear();
+    int32_t last = -1;
+    while (!heap.empty()) {
+      const int32_t k = heap.top();
+      heap.pop();
+      if (k == last) continue;
+      last = k;
+      if (k < i) {
+        const double rk = row[k];  // multiplier for the scaled U
+        row[k] = row[k] * d[k];    // L(i,k), the value the drop rule sees
+        if (std::fabs(row[k]) < thr) {
+          row[k] = 0.0;
+          flag[k] = 0;
+          continue;
+        }
+        for (int64_t jj = u.ptr[k]; jj < u.ptr[k + 1]; ++jj) {
+          const int32_t j = u.idx[jj];
+          row[j] = row[j] - rk * u.val[jj];
+          if (!flag[j]) {
+            if (std::fabs(row[j]) < thr) {
+              row[j] = 0.0;
+            } else {
+              heap.push(j);
+              flag[j] = 1;
+            }
+          }
+        }
+      }
+      kept.push_back(k);
+      flag[k] = 0;
+    }
+    cand.clear();
+    double piv = 0.0;
+    for (int32_t j : kept) {
+      if (j < i && std::fabs(row[j]) >= thr) cand.push_back({row[j], j});
+      if (j == i) piv = row[j];
+    }
+    keep_largest(cand, (int64_t)nlw + fill);
+    for (const Entry &e : cand) {
+      l.idx.push_back(e.j);
+      l.val.push_back(e.v);
+    }
+    if (piv == 0.0)
+      throw Error(AINV_ERR_PIVOT,
+                  "ILUT: zero pivot at row " + std::to_string(i + 1));
+    d[i] = 1.0 / piv;
+    cand.clear();
+    for (int32_t j : kept)
+      if (j > i && std::fabs(row[j]) >= thr) cand.push_back({row[j], j});
+    keep_largest(cand, (int64_t)nup + fill);
+    for (const Entry &e : cand) {
+      u.idx.push_back(e.j);
+      u.val.push_back(e.v * d[i]);
+    }
+    for (int32_t j : kept) row[j] = 0.0;
+    l.ptr[i + 1] = (int64_t)l.idx.size();
+    u.ptr[i + 1] = (int64_t)u.idx.size();
+  }
+  for (int32_t i = 0; i < n; ++i) d[i] = d[i] * scale;
+}
+
+// ------------------------------------------------------------ AINV (LLK)
+namespace {
+
+// Column store of a factor under construction (CSC: column i = vector i).
+struct ColStore {
+  std::vector<int64_t> ptr{0};
+  std::vector<int32_t> idx;
+  std::vector<double> val;
+};
+
+// Work state of one biconjugation pass (Z or W).
+struct PassWork {
+  std::vector<double> zval;
+  std::vector<char> in_z;    // izkr: index already in the pattern of zval
+  std::vector<char> queued;  // izcr: row already queued for a dot product
+  MinHeap pat;               // indices of the nonzeros of zval
+  MinHeap rows;              // rows (or columns) still to be dotted
+  std::vector<Entry> cand;
+};
+
+// One column i of Z (dot = rows of A, seed/enqueue = columns of A) or of W
+// (the same with A^T).  dotm: matrix whose row j is dotted with zval;
+// seedm: matrix whose row i / row kr lists the candidates to queue.
+void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
+                 const ColStore &f, std::vector<double> &piv, int32_t nzrmax,
+                 double thresh, PassWork &w, ColStore &out) {
+  w.zval[i] = 1.0;
+  w.in_z[i] = 1;
+  w.pat.push(i);
+  for (int64_t e = seedm.ptr[i]; e < seedm.ptr[i + 1]; ++e) {
+    const int32_t r = seedm.idx[e];
+    if (r < i) {
+      w.rows.push(r);
+      w.queued[r] = 1;
+    }
+  }
+  int32_t lastj = -1;
+  while (!w.rows.empty()) {
+    const int32_t j = w.rows.top();
+    w.rows.pop();
+    if (j <= lastj) continue;
+    lastj = j;
+    w.queued[j] = 0;
+    if (j >= i) continue;
+    double s = 0.0;
+    for (int64_t e = dotm.ptr[j]; e < dotm.ptr[j + 1]; ++e)
+      s = s + dotm.val[e] * w.zval[dotm.idx[e]];
+    const double alpha = (-s / piv[j]);
+    if (!(std::fabs(alpha) > thresh)) continue;
+    for (int64_t e = f.ptr[j]; e < f.ptr[j + 1]; ++e) {
+      const int32_t kr = f.idx[e];
+      w.zval[kr] = w.zval[kr] + alpha * f.val[e];
+      if (w.in_z[kr]) continue;
+      w.pat.push(kr);
+      w.in_z[kr] = 1;
+      for (int64_t c = seedm.ptr[kr]; c < seedm.ptr[kr + 1]; ++c) {
+        const int32_t nj = seedm.idx[c];
+        if (!w.queued[nj] && nj > j && nj < i) {
+          w.rows.push(nj);
+          w.queued[nj] = 1;
+        }
+      }
+    }
+  }
+  {
+    double s = 0.0;
+    for (int64_t e = dotm.ptr[i]; e < dotm.ptr[i + 1]; ++e)
+      s = s + dotm.val[e] * w.zval[dotm.idx[e]];
+    piv[i] = (std::fabs(s) < kEpsTol) ? 1.0e-3 : s;
+  }
+  // sparsify: diagonal always, then the nzrmax-1 largest with |z| >= thresh
+  w.cand.clear();
+  double diag = 0.0;
+  int32_t last = -1;
+  while (!w.pat.empty()) {
+    const int32_t k = w.pat.top();
+    w.pat.pop();
+    if (k == last) continue;
+    last = k;
+    if (k == i)
+      diag = w.zval[k];
+    else if (std::fabs(w.zval[k]) >= thresh)
+      w.cand.push_back({w.zval[k], k});
+    w.zval[k] = 0.0;
+    w.in_z[k] = 0;
+  }
+  const int64_t keep =
+      nzrmax >= 1 ? (int64_t)nzrmax - 1 : (int64_t)w.cand.size();
+  if ((int64_t)w.cand.size() > keep) {
+    std::nth_element(w.cand.begin(), w.cand.begin() + keep, w.cand.end(),
+                     before);
+    w.cand.resize(keep);
+  }
+  w.cand.push_back({diag, i});
+  std::sort(w.cand.begin(), w.cand.end(),
+            [](const Entry &a, const Entry &b) { return a.j < b.j; });
+  for (const Entry &e : w.cand) {
+    out.idx.push_back(e.j);
+    out.val.push_back(e.v);
+  }
+  out.ptr.push_back((int64_t)out.idx.size());
+}
+
+}  // namespace
+
+void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
+               HostCsr &w, HostCsr &z, std::vector<double> &d) {
+  if (alg != AINV_ALG_LLK && alg != AINV_ALG_S_LLK)
+    throw Error(AINV_ERR_ALG,
+                "AINV variant " + std::to_string(alg) +
+                    " is not available in this build (LLK, SYM-LLK are)");
+  const int32_t n = a.n;
+  // maxval scaling (mld_d_ainv_bld.F90:121-124)
+  const double weight = 1.0 / max_abs(a);
+  HostCsr as = a;
+  for (double &v : as.val) v = v * weight;
+  HostCsr at;
+  transpose(as, at);
+  std::vector<double> p(n, 0.0), q(n, 0.0);
+  ColStore zc, wc;
+  PassWork pw;
+  pw.zval.assign(n, 0.0);
+  pw.in_z.assign(n, 0);
+  pw.queued.assign(n, 0);
+  if (n > 0) {
+    for (int64_t e = as.ptr[0]; e < as.ptr[1]; ++e)
+      if (as.idx[e] == 0) {
+        p[0] = as.val[e];
+        break;
+      }
+    if (std::fabs(p[0]) < kEpsTol) p[0] = 1.0e-3;
+    q[0] = p[0];
+    zc.idx.push_back(0);
+    zc.val.push_back(1.0);
+    zc.ptr.push_back(1);
+    wc.idx.push_back(0);
+    wc.val.push_back(1.0);
+    wc.ptr.push_back(1);
+  }
+  for (int32_t i = 1; i < n; ++i) {
+    bicg_column(i, as, at, zc, p, fill_in, thresh, pw, zc);
+    if (alg == AINV_ALG_LLK)
+      bicg_column(i, at, as, wc, q, fill_in, thresh, pw, wc);
+  }
+  if (alg == AINV_ALG_S_LLK) wc = zc;  // W := Z (s_llk.F90:246)
+  // Z CSC -> CSR; W CSC holds W^T as CSR (biconjg_mod.F90:131-135)
+  HostCsr zcsc;
+  zcsc.n = n;
+  zcsc.ptr = std::move(zc.ptr);
+  zcsc.idx = std::move(zc.idx);
+  zcsc.val = std::move(zc.val);
+  transpose(zcsc, z);
+  w.n = n;
+  w.ptr = std::move(wc.ptr);
+  w.idx = std::move(wc.idx);
+  w.val = std::move(wc.val);
+  d.assign(n, 0.0);
+  for (int32_t i = 0; i < n; ++i) {
+    double v = (std::fabs(p[i]) < kEpsTol) ? 1.0 : 1.0 / p[i];
+    d[i] = v * weight;
+  }
+}
+
+}  // namespace ainv

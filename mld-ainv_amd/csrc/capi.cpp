@@ -1,0 +1,936 @@
+// C ABI of the library: the solver-object API of the MLD2P4 AINV plugin
+// (build / apply / cseti / csetr / csetc / check / descr / clone / free /
+// ...), the system-matrix object and the Krylov driver.  Declarations and
+// the reference interface each call replaces: include/mld_ainv_hip.h.
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "objects.h"
+
+namespace ainv {
+
+std::string &last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+
+namespace {
+
+thread_local int g_device = 0;
+
+std::string upper_trim(const char *s) {
+  if (!s) throw Error(AINV_ERR_ARG, "NULL string argument");
+  std::string t(s);
+  size_t b = t.find_first_not_of(" \t\n\r");
+  size_t e = t.find_last_not_of(" \t\n\r");
+  t = (b == std::string::npos) ? std::string() : t.substr(b, e - b + 1);
+  for (char &c : t) c = (char)std::toupper((unsigned char)c);
+  return t;
+}
+
+ainv_solver *check_handle(ainv_h h) {
+  if (!h) throw Error(AINV_ERR_ARG, "NULL solver handle");
+  return h;
+}
+
+void set_device(int dev) { AINV_HIP_CHECK(hipSetDevice(dev)); }
+
+void apply_defaults(ainv_solver *h) {
+  // d_ainv_solver_default (mld_d_ainv_solver.F90:215-217),
+  // d_invk_solver_default (mld_d_invk_solver.f90:252-253),
+  // d_invt_solver_default (mld_d_invt_solver.f90:265-268)
+  h->alg = AINV_ALG_LLK;
+  h->fill_in = 0;
+  h->inv_fill = 0;
+  h->thresh = 0.0;
+  h->inv_thresh = 0.0;
+}
+
+// mld_check_def semantics: an illegal value is replaced by the default
+void check_params(ainv_solver *h) {
+  if (h->kind == AINV_KIND_AINV) {
+    if (!(h->fill_in >= 1)) h->fill_in = 1;          // is_positive_nz_min
+    if (!(h->thresh >= 0.0)) h->thresh = 0.0;        // is_legal_d_fact_thrs
+  } else {
+    if (!(h->fill_in >= 0)) h->fill_in = 0;          // is_int_non_negative
+    if (!(h->inv_fill >= 0)) h->inv_fill = 0;
+    if (h->kind == AINV_KIND_INVT) {
+      if (!(h->thresh >= 0.0)) h->thresh = 0.0;
+      if (!(h->inv_thresh >= 0.0)) h->inv_thresh = 0.0;
+    }
+  }
+}
+
+int prec_from_string(const std::string &v) {
+  if (v == "F64" || v == "FP64" || v == "DOUBLE") return AINV_PREC_F64;
+  if (v == "F32" || v == "FP32" || v == "SINGLE") return AINV_PREC_F32;
+  if (v == "BF16") return AINV_PREC_BF16;
+  throw Error(AINV_ERR_ARG, "unknown FACTOR_PREC '" + v + "'");
+}
+
+const char *prec_name(int p) {
+  switch (p) {
+    case AINV_PREC_F64: return "F64";
+    case AINV_PREC_F32: return "F32";
+    default: return "BF16";
+  }
+}
+
+int stringval(const std::string &v) {
+  // d_ainv_stringval (mld_d_ainv_solver.F90:232-258)
+  if (v == "LLK") return AINV_ALG_LLK;
+  if (v == "STAB-LLK") return AINV_ALG_S_FT_LLK;
+  if (v == "SYM-LLK") return AINV_ALG_S_LLK;
+  if (v == "MLK") return AINV_ALG_MLK;
+  if (v == "LLK-NOTH") return AINV_ALG_LLK_NOTH;
+  return -1;
+}
+
+const char *algname(int alg) {
+  // d_ainv_algname (mld_d_ainv_solver.F90:261-292)
+  switch (alg) {
+    case AINV_ALG_MLK: return "Left-looking, list merge ";
+    case AINV_ALG_LLK: return "Left-looking ";
+    case AINV_ALG_S_LLK: return "Symmetric Left-looking ";
+    case AINV_ALG_S_FT_LLK: return "Stabilized Left-looking ";
+    default: return "Unknown alg variant ";
+  }
+}
+
+// Validate and canonicalize a CSR block: 0-based, columns outside [0,n)
+// clipped away (a%csclip(imax=n,jmax=n), mld_d_ainv_bld.F90:115-116),
+// rows sorted by column, duplicates summed (PSBLAS assembly default).
+HostCsr canonical_csr(int32_t n, const int64_t *irp, const int32_t *ja,
+                      const double *val, int base) {
+  if (n < 1) throw Error(AINV_ERR_ARG, "matrix order must be positive");
+  if (!irp || !ja || !val) throw Error(AINV_ERR_ARG, "NULL CSR array");
+  if (base != 0 && base != 1) throw Error(AINV_ERR_ARG, "index_base must be 0 or 1");
+  HostCsr a;
+  a.n = n;
+  a.ptr.assign((size_t)n + 1, 0);
+  const int64_t p0 = irp[0];
+  if (p0 != base) throw Error(AINV_ERR_ARG, "irp[0] must equal index_base");
+  a.idx.reserve((size_t)(irp[n] - p0));
+  a.val.reserve((size_t)(irp[n] - p0));
+  std::vector<std::pair<int32_t, double>> row;
+  for (int32_t i = 0; i < n; ++i) {
+    const int64_t b = irp[i] - base, e = irp[i + 1] - base;
+    if (e < b) throw Error(AINV_ERR_ARG, "irp is not non-decreasing");
+    row.clear();
+    bool sorted = true;
+    for (int64_t t = b; t < e; ++t) {
+      int32_t c = ja[t] - base;
+      if (c < 0 || c >= n) continue;
+      if (!row.empty() && c <= row.back().first) sorted = false;
+      row.emplace_back(c, val[t]);
+    }
+    if (!sorted) {
+      std::stable_sort(row.begin(), row.end(),
+                       [](const std::pair<int32_t, double> &x,
+                          const std::pair<int32_t, double> &y) {
+                         return x.first < y.first;
+                       });
+    }
+    for (size_t t = 0; t < row.size(); ++t) {
+      if (t > 0 && row[t].first == a.idx.back() &&
+          (int64_t)a.idx.size() > a.ptr[i]) {
+        a.val.back() += row[t].second;
+      } else {
+        a.idx.push_back(row[t].first);
+        a.val.push_back(row[t].second);
+      }
+    }
+    a.ptr[i + 1] = (int64_t)a.idx.size();
+  }
+  return a;
+}
+
+void ensure_stream(ainv_solver *h) {
+  if (!h->stream) AINV_HIP_CHECK(hipStreamCreate(&h->stream));
+}
+
+void clear_built(ainv_solver *h) {
+  h->built = false;
+  h->have_t = false;
+  h->n = 0;
+  h->w = DevCsr();
+  h->z = DevCsr();
+  h->d64 = DevBuf();
+  h->hw = Hll();
+  h->hz = Hll();
+  h->hwt = Hll();
+  h->hzt = Hll();
+  h->dv = DevBuf();
+  h->tbuf = DevBuf();
+  h->stage_x = DevBuf();
+  h->stage_y = DevBuf();
+  h->inv_report.clear();
+}
+
+void build_layout(ainv_solver *h) {
+  const double t0 = now_seconds();
+  hll_from_csr(h->w, h->prec, h->hw, h->stream);
+  hll_from_csr(h->z, h->prec, h->hz, h->stream);
+  convert_vector(h->d64.as<double>(), h->n, h->prec, h->dv, h->stream);
+  h->have_t = false;
+  h->hwt = Hll();
+  h->hzt = Hll();
+  h->t_layout = now_seconds() - t0;
+}
+
+void ensure_transposed(ainv_solver *h) {
+  if (h->have_t) return;
+  DevCsr wt, zt;
+  csr_transpose(h->w, wt, h->stream);
+  csr_transpose(h->z, zt, h->stream);
+  hll_from_csr(wt, h->prec, h->hwt, h->stream);
+  hll_from_csr(zt, h->prec, h->hzt, h->stream);
+  h->have_t = true;
+}
+
+void do_build(ainv_solver *h, const HostCsr &a) {
+  const double t0 = now_seconds();
+  clear_built(h);
+  ensure_stream(h);
+  h->n = a.n;
+  std::vector<double> d;
+  if (h->kind == AINV_KIND_AINV) {
+    HostCsr w, z;
+    host_ainv(a, h->alg, h->fill_in, h->thresh, w, z, d);
+    h->t_factor = now_seconds() - t0;
+    csr_upload(w, h->w, h->stream);
+    csr_upload(z, h->z, h->stream);
+    h->t_inverse = 0.0;
+  } else {
+    HostCsr l, u;
+    if (h->kind == AINV_KIND_INVK)
+      host_iluk(a, h->fill_in, l, u, d);
+    else
+      host_ilut(a, h->fill_in, h->thresh, l, u, d);
+    const double t1 = now_seconds();
+    h->t_factor = t1 - t0;
+    // Z = U^{-1}; W = L^{-1} = ((L^T)^{-1})^T (mld_d_invk_bld.f90:119-124,
+    // mld_d_invt_bld.f90:131-136), the row inverses on the GPU
+    DevCsr du, dl, dlt, wt;
+    csr_upload(u, du, h->stream);
+    csr_upload(l, dl, h->stream);
+    const double thr = (h->kind == AINV_KIND_INVT) ? h->inv_thresh : 0.0;
+    std::string rep_u, rep_l;
+    gpu_sparse_inverse(h->kind, du, h->inv_fill, thr, h->z, h->stream, &rep_u);
+    csr_transpose(dl, dlt, h->stream);
+    gpu_sparse_inverse(h->kind, dlt, h->inv_fill, thr, wt, h->stream, &rep_l);
+    csr_transpose(wt, h->w, h->stream);
+    h->inv_report = "{\"upper\": " + rep_u + ", \"lower\": " + rep_l + "}";
+    h->t_inverse = now_seconds() - t1;
+  }
+  h->d64.alloc((size_t)h->n * sizeof(double));
+  AINV_HIP_CHECK(hipMemcpyAsync(h->d64.p, d.data(), h->d64.bytes,
+                                hipMemcpyHostToDevice, h->stream));
+  AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
+  build_layout(h);
+  h->built = true;
+  h->t_total = now_seconds() - t0;
+}
+
+void copy_settings(const ainv_solver *src, ainv_solver *dst) {
+  dst->kind = src->kind;
+  dst->alg = src->alg;
+  dst->fill_in = src->fill_in;
+  dst->inv_fill = src->inv_fill;
+  dst->thresh = src->thresh;
+  dst->inv_thresh = src->inv_thresh;
+  dst->prec = src->prec;
+}
+
+void copy_csr_dev(const DevCsr &a, DevCsr &b, hipStream_t s) {
+  b.n = a.n;
+  b.nnz = a.nnz;
+  b.ptr.alloc(a.ptr.bytes);
+  b.idx.alloc(a.idx.bytes);
+  b.val.alloc(a.val.bytes);
+  if (a.ptr.bytes)
+    AINV_HIP_CHECK(hipMemcpyAsync(b.ptr.p, a.ptr.p, a.ptr.bytes,
+                                  hipMemcpyDeviceToDevice, s));
+  if (a.idx.bytes)
+    AINV_HIP_CHECK(hipMemcpyAsync(b.idx.p, a.idx.p, a.idx.bytes,
+                                  hipMemcpyDeviceToDevice, s));
+  if (a.val.bytes)
+    AINV_HIP_CHECK(hipMemcpyAsync(b.val.p, a.val.p, a.val.bytes,
+                                  hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace
+
+void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
+                  double *y, char trans, int k, int64_t ldx, int64_t ldy,
+                  hipStream_t s) {
+  if (!h->built) throw Error(AINV_ERR_STATE, "apply before build");
+  const char t = (char)std::toupper((unsigned char)trans);
+  if (t != 'N' && t != 'T' && t != 'C')
+    throw Error(AINV_ERR_ARG,
+                std::string("invalid trans '") + trans + "'");  // apply.f90:63-69
+  if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
+  if (ldx < k || ldy < k) throw Error(AINV_ERR_ARG, "ldx and ldy must be >= k");
+  if (!x || !y) throw Error(AINV_ERR_ARG, "NULL vector");
+  const bool f32t = h->prec != AINV_PREC_F64;
+  const size_t need = (size_t)h->n * k * (f32t ? sizeof(float) : sizeof(double));
+  if (h->tbuf.bytes < need) h->tbuf.alloc(need);
+  const Hll *first = &h->hw, *second = &h->hz;
+  if (t != 'N') {
+    ensure_transposed(h);
+    first = &h->hzt;
+    second = &h->hwt;
+  }
+  hll_spmm_scale(*first, h->dv.p, x, false, ldx, h->tbuf.p, k, k, s);
+  hll_spmm_axpby(*second, alpha, h->tbuf.p, f32t, k, beta, y, ldy, k, s);
+}
+
+}  // namespace ainv
+
+using namespace ainv;
+
+extern "C" {
+
+const char *ainv_last_error(void) { return last_error().c_str(); }
+
+int ainv_init(int device) {
+  return guard([&] {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1)
+      throw Error(AINV_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= count)
+      throw Error(AINV_ERR_ARG, "device index out of range");
+    set_device(device);
+    g_device = device;
+  });
+}
+
+int ainv_create(int kind, ainv_h *out) {
+  return guard([&] {
+    if (!out) throw Error(AINV_ERR_ARG, "NULL output handle");
+    if (kind != AINV_KIND_AINV && kind != AINV_KIND_INVK &&
+        kind != AINV_KIND_INVT)
+      throw Error(AINV_ERR_ARG, "unknown solver kind");
+    auto *h = new ainv_solver();
+    h->kind = kind;
+    h->device = g_device;
+    apply_defaults(h);
+    *out = h;
+  });
+}
+
+int ainv_free(ainv_h h) {
+  return guard([&] {
+    if (!h) return;
+    {
+      std::lock_guard<std::mutex> lk(h->mu);
+      set_device(h->device);
+      if (h->stream) (void)hipStreamSynchronize(h->stream);
+      clear_built(h);
+      if (h->stream) (void)hipStreamDestroy(h->stream);
+      h->stream = nullptr;
+    }
+    delete h;
+  });
+}
+
+int ainv_default(ainv_h h) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    apply_defaults(h);
+  });
+}
+
+int ainv_get_kind(ainv_h h, int *kind) {
+  return guard([&] {
+    check_handle(h);
+    if (!kind) throw Error(AINV_ERR_ARG, "NULL output");
+    *kind = h->kind;
+  });
+}
+
+int ainv_cseti(ainv_h h, const char *what, int val) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    const std::string w = upper_trim(what);
+    if (w == "SUB_FILLIN") {
+      h->fill_in = val;
+    } else if (w == "AINV_ALG" && h->kind == AINV_KIND_AINV) {
+      h->alg = val;
+    } else if (w == "INV_FILLIN" && h->kind != AINV_KIND_AINV) {
+      h->inv_fill = val;
+    } else if (w == "FACTOR_PREC") {
+      if (val < AINV_PREC_F64 || val > AINV_PREC_BF16)
+        throw Error(AINV_ERR_ARG, "FACTOR_PREC out of range");
+      h->prec = val;
+    }
+    // other keys go to the base solver, which ignores them
+  });
+}
+
+int ainv_csetr(ainv_h h, const char *what, double val) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    const std::string w = upper_trim(what);
+    if (w == "SUB_ILUTHRS" && h->kind != AINV_KIND_INVK) {
+      h->thresh = val;
+    } else if (w == "INV_THRESH" && h->kind == AINV_KIND_INVT) {
+      h->inv_thresh = val;
+    }
+  });
+}
+
+int ainv_csetc(ainv_h h, const char *what, const char *val) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    const std::string w = upper_trim(what);
+    const std::string v = upper_trim(val);
+    if (w == "AINV_ALG" && h->kind == AINV_KIND_AINV) {
+      const int a = stringval(v);
+      // an unknown name yields an id no variant has; build reports it
+      h->alg = a < 0 ? 0 : a;
+    } else if (w == "FACTOR_PREC") {
+      h->prec = prec_from_string(v);
+    }
+  });
+}
+
+int ainv_geti(ainv_h h, const char *what, int *val) {
+  return guard([&] {
+    check_handle(h);
+    if (!val) throw Error(AINV_ERR_ARG, "NULL output");
+    const std::string w = upper_trim(what);
+    if (w == "SUB_FILLIN")
+      *val = h->fill_in;
+    else if (w == "AINV_ALG")
+      *val = h->alg;
+    else if (w == "INV_FILLIN")
+      *val = h->inv_fill;
+    else if (w == "FACTOR_PREC")
+      *val = h->prec;
+    else
+      throw Error(AINV_ERR_ARG, "unknown integer key " + w);
+  });
+}
+
+int ainv_getr(ainv_h h, const char *what, double *val) {
+  return guard([&] {
+    check_handle(h);
+    if (!val) throw Error(AINV_ERR_ARG, "NULL output");
+    const std::string w = upper_trim(what);
+    if (w == "SUB_ILUTHRS")
+      *val = h->thresh;
+    else if (w == "INV_THRESH")
+      *val = h->inv_thresh;
+    else
+      throw Error(AINV_ERR_ARG, "unknown real key " + w);
+  });
+}
+
+int ainv_check(ainv_h h) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    check_params(h);
+  });
+}
+
+int ainv_descr(ainv_h h, char *buf, size_t len) {
+  return guard([&] {
+    check_handle(h);
+    if (!buf || len == 0) throw Error(AINV_ERR_ARG, "NULL buffer");
+    char tmp[1024];
+    int m = 0;
+    if (h->kind == AINV_KIND_AINV) {
+      m = std::snprintf(
+          tmp, sizeof tmp,
+          "  AINV: Approximate Inverse with sparse biconjugation \n"
+          "  Algorithm variant      : %s\n"
+          "  Fill level             : %d\n"
+          "  Fill threshold         : %.16g\n",
+          algname(h->alg), h->fill_in, h->thresh);
+    } else if (h->kind == AINV_KIND_INVK) {
+      m = std::snprintf(tmp, sizeof tmp,
+                        "  INVK Approximate Inverse with ILU(N) \n"
+                        "  Fill level             :%d\n"
+                        "  Inverse fill level     :%d\n",
+                        h->fill_in, h->inv_fill);
+    } else {
+      m = std::snprintf(tmp, sizeof tmp,
+                        "  INVT Approximate Inverse with ILU(T,P) \n"
+                        "  Fill level             :%d\n"
+                        "  Fill threshold         :%.16g\n"
+                        "  Inverse fill level     :%d\n"
+                        "  Inverse fill threshold :%.16g\n",
+                        h->fill_in, h->thresh, h->inv_fill, h->inv_thresh);
+    }
+    std::snprintf(tmp + m, sizeof tmp - m, "  Factor storage         : %s\n",
+                  prec_name(h->prec));
+    std::snprintf(buf, len, "%s", tmp);
+  });
+}
+
+int ainv_stringval(const char *name) {
+  try {
+    return stringval(upper_trim(name));
+  } catch (...) {
+    return -1;
+  }
+}
+
+int ainv_algname(int alg, char *buf, size_t len) {
+  return guard([&] {
+    if (!buf || len == 0) throw Error(AINV_ERR_ARG, "NULL buffer");
+    std::snprintf(buf, len, "%s", algname(alg));
+  });
+}
+
+int ainv_build_csr(ainv_h h, int32_t n, const int64_t *irp, const int32_t *ja,
+                   const double *val, int index_base) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    check_params(h);
+    HostCsr a = canonical_csr(n, irp, ja, val, index_base);
+    do_build(h, a);
+  });
+}
+
+int ainv_cnv(ainv_h h, int prec) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    if (prec < AINV_PREC_F64 || prec > AINV_PREC_BF16)
+      throw Error(AINV_ERR_ARG, "unknown factor precision");
+    h->prec = prec;
+    if (h->built) build_layout(h);
+  });
+}
+
+int ainv_apply(ainv_h h, double alpha, const double *x, double beta, double *y,
+               char trans, int k, int64_t ldx, int64_t ldy, int flags,
+               void *stream) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    if (!h->built) throw Error(AINV_ERR_STATE, "apply before build");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    if (flags & AINV_HOST_PTRS) {
+      if (!x || !y) throw Error(AINV_ERR_ARG, "NULL vector");
+      const size_t bx = (size_t)h->n * ldx * sizeof(double);
+      const size_t by = (size_t)h->n * ldy * sizeof(double);
+      if (h->stage_x.bytes < bx) h->stage_x.alloc(bx);
+      if (h->stage_y.bytes < by) h->stage_y.alloc(by);
+      AINV_HIP_CHECK(hipMemcpyAsync(h->stage_x.p, x, bx, hipMemcpyHostToDevice, s));
+      if (beta != 0.0)
+        AINV_HIP_CHECK(hipMemcpyAsync(h->stage_y.p, y, by,
+                                      hipMemcpyHostToDevice, s));
+      solver_apply(h, alpha, h->stage_x.as<double>(), beta,
+                   h->stage_y.as<double>(), trans, k, ldx, ldy, s);
+      AINV_HIP_CHECK(hipMemcpyAsync(y, h->stage_y.p, by, hipMemcpyDeviceToHost, s));
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      solver_apply(h, alpha, x, beta, y, trans, k, ldx, ldy, s);
+    }
+  });
+}
+
+int ainv_get_nzeros(ainv_h h, int64_t *nz) {
+  return guard([&] {
+    check_handle(h);
+    if (!nz) throw Error(AINV_ERR_ARG, "NULL output");
+    // n + nnz(W) + nnz(Z) (mld_d_base_ainv_mod.F90:179-192)
+    *nz = h->built ? (int64_t)h->n + h->w.nnz + h->z.nnz : 0;
+  });
+}
+
+int ainv_sizeof(ainv_h h, int64_t *bytes) {
+  return guard([&] {
+    check_handle(h);
+    if (!bytes) throw Error(AINV_ERR_ARG, "NULL output");
+    int64_t b = (int64_t)sizeof(ainv_solver);
+    if (h->built) {
+      b += (int64_t)(h->w.ptr.bytes + h->w.idx.bytes + h->w.val.bytes);
+      b += (int64_t)(h->z.ptr.bytes + h->z.idx.bytes + h->z.val.bytes);
+      b += (int64_t)h->d64.bytes + (int64_t)h->dv.bytes;
+      b += h->hw.bytes() + h->hz.bytes() + h->hwt.bytes() + h->hzt.bytes();
+      b += (int64_t)h->tbuf.bytes;
+    }
+    *bytes = b;
+  });
+}
+
+int ainv_get_wrksz(ainv_h h, int *wrksz) {
+  return guard([&] {
+    check_handle(h);
+    if (!wrksz) throw Error(AINV_ERR_ARG, "NULL output");
+    *wrksz = 2;  // mld_d_base_ainv_mod.F90:209-214
+  });
+}
+
+int ainv_get_n(ainv_h h, int32_t *n) {
+  return guard([&] {
+    check_handle(h);
+    if (!n) throw Error(AINV_ERR_ARG, "NULL output");
+    *n = h->built ? h->n : 0;
+  });
+}
+
+int ainv_clone(ainv_h h, ainv_h *out) {
+  return guard([&] {
+    check_handle(h);
+    if (!out) throw Error(AINV_ERR_ARG, "NULL output handle");
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    auto *c = new ainv_solver();
+    try {
+      copy_settings(h, c);
+      c->device = h->device;
+      ensure_stream(c);
+      if (h->built) {
+        c->n = h->n;
+        copy_csr_dev(h->w, c->w, c->stream);
+        copy_csr_dev(h->z, c->z, c->stream);
+        c->d64.alloc(h->d64.bytes);
+        AINV_HIP_CHECK(hipMemcpyAsync(c->d64.p, h->d64.p, h->d64.bytes,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        AINV_HIP_CHECK(hipStreamSynchronize(c->stream));
+        build_layout(c);
+        c->built = true;
+        c->t_factor = h->t_factor;
+        c->t_inverse = h->t_inverse;
+        c->t_total = h->t_total;
+        c->inv_report = h->inv_report;
+      }
+    } catch (...) {
+      clear_built(c);
+      if (c->stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int ainv_clone_settings(ainv_h h, ainv_h out) {
+  return guard([&] {
+    check_handle(h);
+    check_handle(out);
+    if (h->kind != out->kind)
+      throw Error(AINV_ERR_ARG, "clone_settings between different kinds");
+    copy_settings(h, out);
+  });
+}
+
+int ainv_clear_data(ainv_h h) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    if (h->stream) AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
+    clear_built(h);
+  });
+}
+
+int ainv_export(ainv_h h, char which, int32_t *n, int64_t *nnz, int64_t *irp,
+                int32_t *ja, double *val) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->built) throw Error(AINV_ERR_STATE, "export before build");
+    set_device(h->device);
+    const char w = (char)std::toupper((unsigned char)which);
+    if (w == 'D') {
+      if (n) *n = h->n;
+      if (nnz) *nnz = h->n;
+      if (val)
+        AINV_HIP_CHECK(hipMemcpy(val, h->d64.p, h->d64.bytes,
+                                 hipMemcpyDeviceToHost));
+      return;
+    }
+    const DevCsr *m = (w == 'W') ? &h->w : (w == 'Z') ? &h->z : nullptr;
+    if (!m) throw Error(AINV_ERR_ARG, "which must be 'W', 'Z' or 'D'");
+    if (n) *n = m->n;
+    if (nnz) *nnz = m->nnz;
+    if (irp)
+      AINV_HIP_CHECK(hipMemcpy(irp, m->ptr.p, m->ptr.bytes, hipMemcpyDeviceToHost));
+    if (ja && m->nnz)
+      AINV_HIP_CHECK(hipMemcpy(ja, m->idx.p, m->idx.bytes, hipMemcpyDeviceToHost));
+    if (val && m->nnz)
+      AINV_HIP_CHECK(hipMemcpy(val, m->val.p, m->val.bytes, hipMemcpyDeviceToHost));
+  });
+}
+
+int ainv_report(ainv_h h, char *buf, size_t len) {
+  return guard([&] {
+    check_handle(h);
+    if (!buf || len == 0) throw Error(AINV_ERR_ARG, "NULL buffer");
+    std::string r = "{\"built\": " + std::string(h->built ? "true" : "false");
+    if (h->built) {
+      auto hll_json = [](const Hll &x) {
+        return "{\"nnz\": " + std::to_string(x.nnz) +
+               ", \"padded\": " + std::to_string(x.padded) +
+               ", \"slices\": " + std::to_string(x.nslices) +
+               ", \"max_width\": " + std::to_string(x.max_width) + "}";
+      };
+      char tb[256];
+      std::snprintf(tb, sizeof tb,
+                    ", \"t_factor\": %.6f, \"t_inverse\": %.6f, \"t_layout\": "
+                    "%.6f, \"t_total\": %.6f",
+                    h->t_factor, h->t_inverse, h->t_layout, h->t_total);
+      r += ", \"n\": " + std::to_string(h->n) + tb +
+           ", \"prec\": \"" + prec_name(h->prec) + "\"" +
+           ", \"w\": " + hll_json(h->hw) + ", \"z\": " + hll_json(h->hz);
+      if (!h->inv_report.empty()) r += ", \"inverse\": " + h->inv_report;
+    }
+    r += "}";
+    std::snprintf(buf, len, "%s", r.c_str());
+  });
+}
+
+int ainv_mat_create(int32_t n, const int64_t *irp, const int32_t *ja,
+                    const double *val, int index_base, ainv_mat_h *out) {
+  return guard([&] {
+    if (!out) throw Error(AINV_ERR_ARG, "NULL output handle");
+    set_device(g_device);
+    HostCsr a = canonical_csr(n, irp, ja, val, index_base);
+    auto *m = new ainv_matrix();
+    try {
+      m->device = g_device;
+      AINV_HIP_CHECK(hipStreamCreate(&m->stream));
+      m->n = n;
+      csr_upload(a, m->csr, m->stream);
+      hll_from_csr(m->csr, AINV_PREC_F64, m->hll, m->stream);
+    } catch (...) {
+      if (m->stream) (void)hipStreamDestroy(m->stream);
+      delete m;
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int ainv_mat_free(ainv_mat_h a) {
+  return guard([&] {
+    if (!a) return;
+    set_device(a->device);
+    if (a->stream) {
+      (void)hipStreamSynchronize(a->stream);
+      (void)hipStreamDestroy(a->stream);
+    }
+    delete a;
+  });
+}
+
+int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
+                  double *y, int k, int64_t ldx, int64_t ldy, void *stream) {
+  return guard([&] {
+    if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
+    if (!x || !y) throw Error(AINV_ERR_ARG, "NULL vector");
+    if (k < 1 || k > 64 || ldx < k || ldy < k)
+      throw Error(AINV_ERR_ARG, "bad k / leading dimensions");
+    set_device(a->device);
+    hipStream_t s = stream ? (hipStream_t)stream : a->stream;
+    hll_spmm_axpby(a->hll, alpha, x, false, ldx, beta, y, ldy, k, s);
+  });
+}
+
+int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
+                double *x, int k, double eps, int itmax, int istop,
+                int32_t *iter, double *err, int flags, void *stream) {
+  return guard([&] {
+    if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
+    if (!b || !x || !iter || !err) throw Error(AINV_ERR_ARG, "NULL argument");
+    if (istop != 2) throw Error(AINV_ERR_ARG, "only istop = 2 is supported");
+    if (itmax < 0) throw Error(AINV_ERR_ARG, "itmax must be >= 0");
+    const std::string m = upper_trim(method);
+    set_device(a->device);
+    std::unique_lock<std::mutex> lk;
+    if (prec) {
+      lk = std::unique_lock<std::mutex>(prec->mu);
+      if (!prec->built) throw Error(AINV_ERR_STATE, "preconditioner not built");
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : a->stream;
+    if (flags & AINV_HOST_PTRS) {
+      DevBuf db, dx;
+      const size_t bytes = (size_t)a->n * k * sizeof(double);
+      db.alloc(bytes);
+      dx.alloc(bytes);
+      AINV_HIP_CHECK(hipMemcpyAsync(db.p, b, bytes, hipMemcpyHostToDevice, s));
+      AINV_HIP_CHECK(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, s));
+      krylov_solve(m, a, prec, db.as<double>(), dx.as<double>(), k, eps, itmax,
+                   iter, err, s);
+      AINV_HIP_CHECK(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, s));
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, s);
+    }
+  });
+}
+
+// ---- device plumbing -----------------------------------------------------
+
+int ainv_dev_alloc(size_t bytes, void **p) {
+  return guard([&] {
+    if (!p) throw Error(AINV_ERR_ARG, "NULL output pointer");
+    set_device(g_device);
+    *p = nullptr;
+    if (bytes == 0) return;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess)
+      throw Error(e == hipErrorOutOfMemory ? AINV_ERR_ALLOC : AINV_ERR_HIP,
+                  std::string("hipMalloc: ") + hipGetErrorString(e));
+  });
+}
+
+int ainv_dev_free(void *p) {
+  return guard([&] {
+    if (p) AINV_HIP_CHECK(hipFree(p));
+  });
+}
+
+int ainv_memcpy(void *dst, const void *src, size_t bytes, int kind) {
+  return guard([&] {
+    if (bytes == 0) return;
+    if (!dst || !src) throw Error(AINV_ERR_ARG, "NULL pointer");
+    hipMemcpyKind k = kind == AINV_H2D   ? hipMemcpyHostToDevice
+                      : kind == AINV_D2H ? hipMemcpyDeviceToHost
+                      : kind == AINV_D2D ? hipMemcpyDeviceToDevice
+                                         : throw Error(AINV_ERR_ARG, "bad kind");
+    AINV_HIP_CHECK(hipMemcpy(dst, src, bytes, k));
+  });
+}
+
+int ainv_memset(void *p, int value, size_t bytes) {
+  return guard([&] {
+    if (bytes == 0) return;
+    if (!p) throw Error(AINV_ERR_ARG, "NULL pointer");
+    AINV_HIP_CHECK(hipMemset(p, value, bytes));
+  });
+}
+
+int ainv_device_sync(void) {
+  return guard([&] { AINV_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+namespace {
+struct Events {
+  std::vector<hipEvent_t> ev;
+  explicit Events(size_t n) : ev(n, nullptr) {
+    for (auto &e : ev) AINV_HIP_CHECK(hipEventCreate(&e));
+  }
+  ~Events() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  double ms(size_t a, size_t b) const {
+    float t = 0.f;
+    AINV_HIP_CHECK(hipEventElapsedTime(&t, ev[a], ev[b]));
+    return (double)t;
+  }
+};
+}  // namespace
+
+int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
+                       double *ms_first, double *ms_second) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    if (!h->built) throw Error(AINV_ERR_STATE, "profile before build");
+    if (reps < 1 || k < 1 || k > 64 || !x || !y || !ms_first || !ms_second)
+      throw Error(AINV_ERR_ARG, "bad profile arguments");
+    const bool f32t = h->prec != AINV_PREC_F64;
+    const size_t need = (size_t)h->n * k * (f32t ? sizeof(float) : sizeof(double));
+    if (h->tbuf.bytes < need) h->tbuf.alloc(need);
+    hipStream_t s = h->stream;
+    Events e(3 * (size_t)reps);
+    for (int r = 0; r < reps; ++r) {
+      AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r], s));
+      hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s);
+      AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 1], s));
+      hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s);
+      AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 2], s));
+    }
+    AINV_HIP_CHECK(hipStreamSynchronize(s));
+    double a = 0, b = 0;
+    for (int r = 0; r < reps; ++r) {
+      a += e.ms(3 * r, 3 * r + 1);
+      b += e.ms(3 * r + 1, 3 * r + 2);
+    }
+    *ms_first = a / reps;
+    *ms_second = b / reps;
+  });
+}
+
+int ainv_mat_profile(ainv_mat_h a, const double *x, double *y, int k, int reps,
+                     double *ms) {
+  return guard([&] {
+    if (!a || !x || !y || !ms || reps < 1 || k < 1 || k > 64)
+      throw Error(AINV_ERR_ARG, "bad profile arguments");
+    set_device(a->device);
+    Events e(2);
+    AINV_HIP_CHECK(hipEventRecord(e.ev[0], a->stream));
+    for (int r = 0; r < reps; ++r)
+      hll_spmm_axpby(a->hll, 1.0, x, false, k, 0.0, y, k, k, a->stream);
+    AINV_HIP_CHECK(hipEventRecord(e.ev[1], a->stream));
+    AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
+    *ms = e.ms(0, 1) / reps;
+  });
+}
+
+int ainv_krylov_profile(const char *method, ainv_mat_h a, ainv_h prec,
+                        const double *b, double *x, int k, double eps,
+                        int itmax, int reps, int32_t *iter, double *err,
+                        double *ms) {
+  return guard([&] {
+    if (!a || !b || !x || !iter || !err || !ms || reps < 1)
+      throw Error(AINV_ERR_ARG, "bad profile arguments");
+    const std::string m = upper_trim(method);
+    set_device(a->device);
+    std::unique_lock<std::mutex> lk;
+    if (prec) {
+      lk = std::unique_lock<std::mutex>(prec->mu);
+      if (!prec->built) throw Error(AINV_ERR_STATE, "preconditioner not built");
+    }
+    Events e(2);
+    double total = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      AINV_HIP_CHECK(hipMemsetAsync(x, 0, (size_t)a->n * k * sizeof(double),
+                                    a->stream));
+      AINV_HIP_CHECK(hipEventRecord(e.ev[0], a->stream));
+      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, a->stream);
+      AINV_HIP_CHECK(hipEventRecord(e.ev[1], a->stream));
+      AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
+      total += e.ms(0, 1);
+    }
+    *ms = total / reps;
+  });
+}
+
+int ainv_layout(ainv_h h, int64_t *n, int64_t *nnz_w, int64_t *nnz_z,
+                int64_t *pad_w, int64_t *pad_z, int *value_bytes_out) {
+  return guard([&] {
+    check_handle(h);
+    if (!h->built) throw Error(AINV_ERR_STATE, "layout before build");
+    if (n) *n = h->n;
+    if (nnz_w) *nnz_w = h->hw.nnz;
+    if (nnz_z) *nnz_z = h->hz.nnz;
+    if (pad_w) *pad_w = h->hw.padded;
+    if (pad_z) *pad_z = h->hz.padded;
+    if (value_bytes_out) *value_bytes_out = value_bytes(h->prec);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,308 @@
+// Device-resident sparse storage: owned buffers, CSR upload/download, a
+// deterministic CSR transpose (radix sort on (column,row) keys) and the
+// CSR -> sliced-ELL (HLL) conversion that lays factors out for the apply
+// kernels.
+#include <hipcub/hipcub.hpp>
+
+#include "ainv_internal.h"
+
+namespace ainv {
+
+void DevBuf::alloc(size_t b) {
+  release();
+  if (b == 0) return;
+  AINV_HIP_CHECK(hipMalloc(&p, b));
+  bytes = b;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+int value_bytes(int prec) {
+  switch (prec) {
+    case AINV_PREC_F64: return 8;
+    case AINV_PREC_F32: return 4;
+    case AINV_PREC_BF16: return 2;
+  }
+  throw Error(AINV_ERR_ARG, "unknown factor precision " + std::to_string(prec));
+}
+
+void csr_upload(const HostCsr &h, DevCsr &d, hipStream_t s) {
+  d.n = h.n;
+  d.nnz = h.nnz();
+  d.ptr.alloc((size_t)(h.n + 1) * sizeof(int64_t));
+  d.idx.alloc((size_t)d.nnz * sizeof(int32_t));
+  d.val.alloc((size_t)d.nnz * sizeof(double));
+  AINV_HIP_CHECK(hipMemcpyAsync(d.ptr.p, h.ptr.data(), d.ptr.bytes,
+                                hipMemcpyHostToDevice, s));
+  if (d.nnz) {
+    AINV_HIP_CHECK(hipMemcpyAsync(d.idx.p, h.idx.data(), d.idx.bytes,
+                                  hipMemcpyHostToDevice, s));
+    AINV_HIP_CHECK(hipMemcpyAsync(d.val.p, h.val.data(), d.val.bytes,
+                                  hipMemcpyHostToDevice, s));
+  }
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void csr_download(const DevCsr &d, HostCsr &h, hipStream_t s) {
+  h.n = d.n;
+  h.ptr.resize(d.n + 1);
+  h.idx.resize(d.nnz);
+  h.val.resize(d.nnz);
+  AINV_HIP_CHECK(hipMemcpyAsync(h.ptr.data(), d.ptr.p, d.ptr.bytes,
+                                hipMemcpyDeviceToHost, s));
+  if (d.nnz) {
+    AINV_HIP_CHECK(hipMemcpyAsync(h.idx.data(), d.idx.p, d.idx.bytes,
+                                  hipMemcpyDeviceToHost, s));
+    AINV_HIP_CHECK(hipMemcpyAsync(h.val.data(), d.val.p, d.val.bytes,
+                                  hipMemcpyDeviceToHost, s));
+  }
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(int64_t work, int per_block = kThreads) {
+  int64_t g = (work + per_block - 1) / per_block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+// key = column << 32 | row for every stored entry (one thread per row)
+__global__ void k_transpose_keys(int32_t n, const int64_t *__restrict__ ptr,
+                                 const int32_t *__restrict__ idx,
+                                 uint64_t *__restrict__ keys) {
+  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e)
+    keys[e] = ((uint64_t)(uint32_t)idx[e] << 32) | (uint32_t)r;
+}
+
+// row pointer of the transpose: first position whose column >= c
+__global__ void k_transpose_ptr(int32_t n, int64_t nnz,
+                                const uint64_t *__restrict__ keys,
+                                int64_t *__restrict__ tptr,
+                                int32_t *__restrict__ tidx) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nnz) tidx[t] = (int32_t)(uint32_t)(keys[t] & 0xffffffffu);
+  if (t <= n) {
+    int64_t lo = 0, hi = nnz;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)(keys[mid] >> 32) < t)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    tptr[t] = lo;
+  }
+}
+
+int bits_for(int64_t v) {
+  int b = 1;
+  while ((int64_t(1) << b) <= v && b < 62) ++b;
+  return b;
+}
+
+}  // namespace
+
+void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s) {
+  const int32_t n = a.n;
+  const int64_t nnz = a.nnz;
+  t.n = n;
+  t.nnz = nnz;
+  t.ptr.alloc((size_t)(n + 1) * sizeof(int64_t));
+  t.idx.alloc((size_t)nnz * sizeof(int32_t));
+  t.val.alloc((size_t)nnz * sizeof(double));
+  if (nnz == 0) {
+    AINV_HIP_CHECK(hipMemsetAsync(t.ptr.p, 0, t.ptr.bytes, s));
+    AINV_HIP_CHECK(hipStreamSynchronize(s));
+    return;
+  }
+  DevBuf kin, kout;
+  kin.alloc((size_t)nnz * sizeof(uint64_t));
+  kout.alloc((size_t)nnz * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_transpose_keys, dim3(grid_for(n)), dim3(kThreads), 0, s,
+                     n, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
+                     kin.as<uint64_t>());
+  AINV_HIP_CHECK(hipGetLastError());
+  const int end_bit = 32 + bits_for(n);
+  size_t tmp_bytes = 0;
+  AINV_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+      nullptr, tmp_bytes, kin.as<uint64_t>(), kout.as<uint64_t>(),
+      a.val.as<double>(), t.val.as<double>(), (int)nnz, 0, end_bit, s));
+  DevBuf tmp;
+  tmp.alloc(tmp_bytes ? tmp_bytes : 16);
+  AINV_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+      tmp.p, tmp_bytes, kin.as<uint64_t>(), kout.as<uint64_t>(),
+      a.val.as<double>(), t.val.as<double>(), (int)nnz, 0, end_bit, s));
+  hipLaunchKernelGGL(k_transpose_ptr, dim3(grid_for(std::max<int64_t>(nnz, n + 1))),
+                     dim3(kThreads), 0, s, n, nnz, kout.as<uint64_t>(),
+                     t.ptr.as<int64_t>(), t.idx.as<int32_t>());
+  AINV_HIP_CHECK(hipGetLastError());
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace {
+
+// per slice: width = max row length (one thread per slice); rlen per row
+__global__ void k_hll_width(int32_t n, int32_t nslices,
+                            const int64_t *__restrict__ ptr,
+                            int32_t *__restrict__ rlen,
+                            int64_t *__restrict__ w64) {
+  int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nslices) return;
+  if (s == nslices) {
+    w64[s] = 0;
+    return;
+  }
+  int32_t w = 0;
+  for (int l = 0; l < kSlice; ++l) {
+    int32_t r = s * kSlice + l;
+    if (r >= n) break;
+    int32_t len = (int32_t)(ptr[r + 1] - ptr[r]);
+    rlen[r] = len;
+    w = len > w ? len : w;
+  }
+  w64[s] = (int64_t)w * kSlice;
+}
+
+__device__ inline void store_val(double *p, double v) { *p = v; }
+__device__ inline void store_val(float *p, double v) { *p = (float)v; }
+// BF16: round-to-nearest-even of the FP32 rounding of v
+__device__ inline void store_val(uint16_t *p, double v) {
+  float f = (float)v;
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) {  // inf / nan
+    *p = (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    return;
+  }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  *p = (uint16_t)(u >> 16);
+}
+
+// one thread per (row) of the padded row space: writes its row column-major
+template <typename VT>
+__global__ void k_hll_fill(int32_t n, int32_t nslices,
+                           const int64_t *__restrict__ ptr,
+                           const int32_t *__restrict__ idx,
+                           const double *__restrict__ val,
+                           const int64_t *__restrict__ sptr,
+                           int32_t *__restrict__ col, VT *__restrict__ hv) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t s = (int32_t)(r / kSlice);
+  if (s >= nslices) return;
+  int lane = (int)(r % kSlice);
+  int64_t base = sptr[s];
+  int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+  int64_t beg = 0;
+  int32_t len = 0;
+  int32_t self = (int32_t)(r < n ? r : n - 1);
+  if (r < n) {
+    beg = ptr[r];
+    len = (int32_t)(ptr[r + 1] - beg);
+  }
+  for (int32_t j = 0; j < width; ++j) {
+    int64_t q = base + (int64_t)j * kSlice + lane;
+    if (j < len) {
+      col[q] = idx[beg + j];
+      store_val(hv + q, val[beg + j]);
+    } else {
+      col[q] = self;
+      store_val(hv + q, 0.0);
+    }
+  }
+}
+
+template <typename VT>
+__global__ void k_convert(int64_t n, const double *__restrict__ src,
+                          VT *__restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_val(dst + i, src[i]);
+}
+
+}  // namespace
+
+void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s) {
+  const int vb = value_bytes(prec);
+  h = Hll();
+  h.n = a.n;
+  h.prec = prec;
+  h.nnz = a.nnz;
+  h.nslices = (a.n + kSlice - 1) / kSlice;
+  h.sptr.alloc((size_t)(h.nslices + 1) * sizeof(int64_t));
+  h.rlen.alloc((size_t)std::max(a.n, 1) * sizeof(int32_t));
+  DevBuf w64;
+  w64.alloc((size_t)(h.nslices + 1) * sizeof(int64_t));
+  hipLaunchKernelGGL(k_hll_width, dim3(grid_for(h.nslices + 1)),
+                     dim3(kThreads), 0, s, a.n, h.nslices, a.ptr.as<int64_t>(),
+                     h.rlen.as<int32_t>(), w64.as<int64_t>());
+  AINV_HIP_CHECK(hipGetLastError());
+  size_t tmp_bytes = 0;
+  AINV_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(
+      nullptr, tmp_bytes, w64.as<int64_t>(), h.sptr.as<int64_t>(),
+      h.nslices + 1, s));
+  DevBuf tmp;
+  tmp.alloc(tmp_bytes ? tmp_bytes : 16);
+  AINV_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(
+      tmp.p, tmp_bytes, w64.as<int64_t>(), h.sptr.as<int64_t>(),
+      h.nslices + 1, s));
+  AINV_HIP_CHECK(hipMemcpyAsync(&h.padded, h.sptr.as<int64_t>() + h.nslices,
+                                sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  // widest slice, for reporting
+  std::vector<int64_t> hs(h.nslices + 1);
+  AINV_HIP_CHECK(hipMemcpyAsync(hs.data(), h.sptr.p, hs.size() * sizeof(int64_t),
+                                hipMemcpyDeviceToHost, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < h.nslices; ++i)
+    h.max_width = std::max<int32_t>(h.max_width,
+                                    (int32_t)((hs[i + 1] - hs[i]) / kSlice));
+  h.col.alloc((size_t)h.padded * sizeof(int32_t));
+  h.val.alloc((size_t)h.padded * vb);
+  if (h.padded > 0) {
+    dim3 g(grid_for((int64_t)h.nslices * kSlice));
+    switch (prec) {
+      case AINV_PREC_F64:
+        hipLaunchKernelGGL(k_hll_fill<double>, g, dim3(kThreads), 0, s, a.n,
+                           h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
+                           a.val.as<double>(), h.sptr.as<int64_t>(),
+                           h.col.as<int32_t>(), h.val.as<double>());
+        break;
+      case AINV_PREC_F32:
+        hipLaunchKernelGGL(k_hll_fill<float>, g, dim3(kThreads), 0, s, a.n,
+                           h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
+                           a.val.as<double>(), h.sptr.as<int64_t>(),
+                           h.col.as<int32_t>(), h.val.as<float>());
+        break;
+      default:
+        hipLaunchKernelGGL(k_hll_fill<uint16_t>, g, dim3(kThreads), 0, s, a.n,
+                           h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
+                           a.val.as<double>(), h.sptr.as<int64_t>(),
+                           h.col.as<int32_t>(), h.val.as<uint16_t>());
+    }
+    AINV_HIP_CHECK(hipGetLastError());
+  }
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
+                    hipStream_t s) {
+  // d is kept FP64 for F64 factors and FP32 otherwise
+  if (prec == AINV_PREC_F64) {
+    dst.alloc((size_t)n * sizeof(double));
+    AINV_HIP_CHECK(hipMemcpyAsync(dst.p, src, dst.bytes,
+                                  hipMemcpyDeviceToDevice, s));
+  } else {
+    dst.alloc((size_t)n * sizeof(float));
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for(n)), dim3(kThreads), 0,
+                       s, n, src, dst.as<float>());
+    AINV_HIP_CHECK(hipGetLastError());
+  }
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace ainv

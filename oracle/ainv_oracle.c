@@ -1,0 +1,1168 @@
+/*
+ * ainv_oracle.c -- CPU restatement (plain C, FP64, one thread) of the MLD2P4
+ * AINV plugin hot path: sparsify, left-looking biconjugation (LLK, S-LLK),
+ * the AINV/INVK/INVT factor builds, the ILU(k)/ILUT factorizations they
+ * consume, the preconditioner apply and the BiCGSTAB/CG drivers.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the oracle the parity tests check the
+ * HIP library against.  Nothing in the product links or loads it.
+ *
+ * Citations are to the reference tree (1-based Fortran); the code here is
+ * 0-based.  Where the reference's behaviour depends on code that is not in
+ * its tree (PSBLAS heaps, MLD2P4 ILU), the restatement fixes one published
+ * rule and says so:
+ *   - selection among equal |values| (PSBLAS heap tie order) uses the total
+ *     order "larger |v| first, then smaller index first";
+ *   - ILU(k)/ILUT follow Saad, Iterative Methods for Sparse Linear Systems,
+ *     2nd ed., Alg. 10.5 (ILU(p), IKJ) and Alg. 10.6 (ILUT), with the
+ *     conventions the plugin relies on (unit L, unit U stored strictly
+ *     upper, d = inverted pivots, maxval scaling for ILUT);
+ *   - dnrm2 is the classic reference-BLAS scaled sum of squares;
+ *   - d_epstol is DBL_EPSILON.
+ * Build with -ffp-contract=off so no multiply-add is fused.
+ */
+#include "ainv_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define D_EPSTOL DBL_EPSILON
+
+void or_free(void *p) { free(p); }
+
+void or_mat_free(or_mat *m) {
+  if (!m) return;
+  free(m->ptr);
+  free(m->idx);
+  free(m->val);
+  m->ptr = NULL;
+  m->idx = NULL;
+  m->val = NULL;
+  m->n = 0;
+  m->nnz = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* growable entry list used to assemble CSR/CSC outputs column by column */
+typedef struct {
+  int64_t len, cap;
+  int32_t *idx;
+  double *val;
+} vlist;
+
+static int vl_push(vlist *v, int32_t j, double x) {
+  if (v->len == v->cap) {
+    int64_t nc = v->cap ? v->cap * 2 : 1024;
+    int32_t *ni = (int32_t *)realloc(v->idx, (size_t)nc * sizeof(int32_t));
+    if (!ni) return OR_ERR_ALLOC;
+    v->idx = ni;
+    double *nv = (double *)realloc(v->val, (size_t)nc * sizeof(double));
+    if (!nv) return OR_ERR_ALLOC;
+    v->val = nv;
+    v->cap = nc;
+  }
+  v->idx[v->len] = j;
+  v->val[v->len] = x;
+  v->len++;
+  return OR_OK;
+}
+
+static void vl_to_mat(vlist *v, int32_t n, int64_t *ptr, or_mat *m) {
+  m->n = n;
+  m->nnz = v->len;
+  m->ptr = ptr;
+  m->idx = v->idx;
+  m->val = v->val;
+  if (!m->idx) m->idx = (int32_t *)malloc(sizeof(int32_t));
+  if (!m->val) m->val = (double *)malloc(sizeof(double));
+  v->idx = NULL;
+  v->val = NULL;
+}
+
+/* ------------------------------------------------------------------ */
+/* integer min-heap, duplicates allowed (the psb_i_heap usage pattern:
+ * insert / get_first, callers skip repeats themselves) */
+typedef struct {
+  int32_t *a;
+  int64_t sz, cap;
+} iheap;
+
+static int ih_push(iheap *h, int32_t x) {
+  if (h->sz == h->cap) {
+    int64_t nc = h->cap ? h->cap * 2 : 256;
+    int32_t *na = (int32_t *)realloc(h->a, (size_t)nc * sizeof(int32_t));
+    if (!na) return OR_ERR_ALLOC;
+    h->a = na;
+    h->cap = nc;
+  }
+  int64_t i = h->sz++;
+  while (i > 0) {
+    int64_t p = (i - 1) / 2;
+    if (h->a[p] <= x) break;
+    h->a[i] = h->a[p];
+    i = p;
+  }
+  h->a[i] = x;
+  return OR_OK;
+}
+
+/* returns 0 and the smallest element, or -1 if empty */
+static int ih_pop(iheap *h, int32_t *out) {
+  if (h->sz == 0) return -1;
+  *out = h->a[0];
+  int32_t x = h->a[--h->sz];
+  int64_t i = 0, n = h->sz;
+  for (;;) {
+    int64_t c = 2 * i + 1;
+    if (c >= n) break;
+    if (c + 1 < n && h->a[c + 1] < h->a[c]) c++;
+    if (h->a[c] >= x) break;
+    h->a[i] = h->a[c];
+    i = c;
+  }
+  if (n > 0) h->a[i] = x;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* (value, index) pairs ordered by |value| descending, then index ascending:
+ * the total order used wherever the reference pops a psb_d_idx_heap built
+ * with dir=psb_asort_down_ (mld_d_sparsify.f90:81, invt_copyout.f90:78). */
+typedef struct {
+  double v;
+  int32_t i;
+} vipair;
+
+static int vip_cmp(const void *pa, const void *pb) {
+  const vipair *a = (const vipair *)pa, *b = (const vipair *)pb;
+  double fa = fabs(a->v), fb = fabs(b->v);
+  if (fa > fb) return -1;
+  if (fa < fb) return 1;
+  return (a->i < b->i) ? -1 : (a->i > b->i);
+}
+
+static int vip_idx_cmp(const void *pa, const void *pb) {
+  const vipair *a = (const vipair *)pa, *b = (const vipair *)pb;
+  return (a->i < b->i) ? -1 : (a->i > b->i);
+}
+
+/* keep the first `keep` pairs of cand[0..m) in the |v|-descending order,
+ * then put them in ascending index order; returns the kept count */
+static int64_t select_top_sorted(vipair *cand, int64_t m, int64_t keep) {
+  if (m > keep) {
+    qsort(cand, (size_t)m, sizeof(vipair), vip_cmp);
+    m = keep;
+  }
+  if (m < 0) m = 0;
+  qsort(cand, (size_t)m, sizeof(vipair), vip_idx_cmp);
+  return m;
+}
+
+/* ------------------------------------------------------------------ */
+/* classic reference-BLAS dnrm2 (scaled sum of squares) */
+static double dnrm2(int64_t n, const double *x) {
+  if (n < 1) return 0.0;
+  if (n == 1) return fabs(x[0]);
+  double scale = 0.0, ssq = 1.0;
+  for (int64_t i = 0; i < n; i++) {
+    if (x[i] != 0.0) {
+      double ax = fabs(x[i]);
+      if (scale < ax) {
+        double r = scale / ax;
+        ssq = 1.0 + ssq * (r * r);
+        scale = ax;
+      } else {
+        double r = ax / scale;
+        ssq = ssq + r * r;
+      }
+    }
+  }
+  return scale * sqrt(ssq);
+}
+
+/* psb_spge_dot: sum over stored entries in stored order (used at
+ * mld_dsparse_biconjg_llk.F90:173, :218, :293, :340) */
+static double spge_dot(int64_t nz, const int32_t *ja, const double *v,
+                       const double *x) {
+  double s = 0.0;
+  for (int64_t k = 0; k < nz; k++) s = s + v[k] * x[ja[k]];
+  return s;
+}
+
+/* ------------------------------------------------------------------ */
+int or_transpose(const or_mat *a, or_mat *t) {
+  int32_t n = a->n;
+  int64_t nnz = a->ptr[n];
+  int64_t *tp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int32_t *ti = (int32_t *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
+  double *tv = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+  if (!tp || !ti || !tv) {
+    free(tp);
+    free(ti);
+    free(tv);
+    return OR_ERR_ALLOC;
+  }
+  for (int64_t k = 0; k < nnz; k++) tp[a->idx[k] + 1]++;
+  for (int32_t i = 0; i < n; i++) tp[i + 1] += tp[i];
+  int64_t *pos = (int64_t *)malloc((size_t)(n ? n : 1) * sizeof(int64_t));
+  if (!pos) {
+    free(tp);
+    free(ti);
+    free(tv);
+    return OR_ERR_ALLOC;
+  }
+  memcpy(pos, tp, (size_t)n * sizeof(int64_t));
+  for (int32_t i = 0; i < n; i++)
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++) {
+      int64_t q = pos[a->idx[k]]++;
+      ti[q] = i;
+      tv[q] = a->val[k];
+    }
+  free(pos);
+  t->n = n;
+  t->nnz = nnz;
+  t->ptr = tp;
+  t->idx = ti;
+  t->val = tv;
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* sparsify with an index heap (mld_d_sparsify.f90:83-143, iheap branch
+ * :88-106): pop candidates ascending, skip repeats; the diagonal goes to
+ * slot 1; others with |z| >= thresh are candidates; zw and ikr are cleared;
+ * then at most nzrmax entries in total are kept (:119-134; note the loop
+ * exits only when k == nzrmax, so nzrmax = 0 keeps every candidate). */
+static int sparsify_heap(int32_t idiag, int32_t nzrmax, double thresh,
+                         double *zw, iheap *heap, int32_t *ikr, vipair *cand,
+                         int32_t *iz, double *valz, int32_t *nzout) {
+  int32_t last = -1, i;
+  int64_t m = 0;
+  double diagv = 0.0;
+  int have_diag = 0;
+  while (ih_pop(heap, &i) == 0) {
+    if (i == last) continue;
+    last = i;
+    if (i == idiag) {
+      diagv = zw[i];
+      have_diag = 1;
+    } else if (fabs(zw[i]) >= thresh) {
+      cand[m].v = zw[i];
+      cand[m].i = i;
+      m++;
+    }
+    zw[i] = 0.0;
+    ikr[i] = 0;
+  }
+  /* slot 1 always holds the diagonal (xw(1) set when i == idiag) */
+  int64_t keep = (nzrmax >= 1) ? (int64_t)nzrmax - 1 : m;
+  if (m > keep) {
+    qsort(cand, (size_t)m, sizeof(vipair), vip_cmp);
+    m = keep;
+  }
+  cand[m].v = have_diag ? diagv : 0.0;
+  cand[m].i = idiag;
+  m++;
+  qsort(cand, (size_t)m, sizeof(vipair), vip_idx_cmp);
+  for (int64_t k = 0; k < m; k++) {
+    iz[k] = cand[k].i;
+    valz[k] = cand[k].v;
+  }
+  *nzout = (int32_t)m;
+  return OR_OK;
+}
+
+int32_t or_sparsify(int32_t idiag, int32_t nzrmax, double thresh, int32_t n,
+                    double *zw, int32_t istart, int32_t *iz, double *valz) {
+  /* stand-alone branch (:107-116): scan istart..n */
+  vipair *cand = (vipair *)malloc(((size_t)n + 1) * sizeof(vipair));
+  int64_t m = 0;
+  double diagv = 0.0;
+  if (istart < 0) istart = 0;
+  for (int32_t i = istart; i < n; i++) {
+    if (i == idiag)
+      diagv = zw[i];
+    else if (fabs(zw[i]) >= thresh) {
+      cand[m].v = zw[i];
+      cand[m].i = i;
+      m++;
+    }
+    zw[i] = 0.0;
+  }
+  int64_t keep = (nzrmax >= 1) ? (int64_t)nzrmax - 1 : m;
+  if (m > keep) {
+    qsort(cand, (size_t)m, sizeof(vipair), vip_cmp);
+    m = keep;
+  }
+  cand[m].v = diagv;
+  cand[m].i = idiag;
+  m++;
+  qsort(cand, (size_t)m, sizeof(vipair), vip_idx_cmp);
+  for (int64_t k = 0; k < m; k++) {
+    iz[k] = cand[k].i;
+    valz[k] = cand[k].v;
+  }
+  free(cand);
+  return (int32_t)m;
+}
+
+/* ------------------------------------------------------------------ */
+/* One biconjugation pass (the Z loop of mld_dsparse_biconjg_llk.F90:
+ * 125-241, or the W loop :248-362 with A and A^T exchanged).
+ *   rows    : "row access"  matrix used for the dot products (A for Z)
+ *   cols    : "column access" matrix used to seed/enqueue (A^T as CSR,
+ *             i.e. CSC of A, for Z)
+ *   f       : finished columns of the factor (CSC, column j = entries)
+ *   pv      : pivots (p for Z, q for W)
+ */
+typedef struct {
+  const or_mat *rows; /* dot products: row j of this matrix */
+  const or_mat *cols; /* seeding/enqueue: row i of this matrix */
+} pass_mats;
+
+static int biconjg_pass(int32_t i, const pass_mats *pm, const vlist *f,
+                        const int64_t *fptr, double *pv, int32_t nzrmax,
+                        double thresh, double *zval, int32_t *izkr,
+                        int32_t *izcr, iheap *heap, iheap *rheap,
+                        vipair *cand, int32_t *iz, double *vz, int32_t *nzz) {
+  int rc;
+  const or_mat *R = pm->rows, *C = pm->cols;
+  zval[i] = 1.0;
+  izkr[i] = 1;
+  heap->sz = 0;
+  if ((rc = ih_push(heap, i))) return rc;
+  rheap->sz = 0;
+  /* seed: {j < i : A(j,i) != 0} for Z (:132-138), {j < i: A(i,j) != 0}
+   * for W (:256-261) */
+  for (int64_t e = C->ptr[i]; e < C->ptr[i + 1]; e++) {
+    int32_t r = C->idx[e];
+    if (r < i) {
+      if ((rc = ih_push(rheap, r))) return rc;
+      izcr[r] = 1;
+    }
+  }
+  int32_t lastj = -1, j;
+  for (;;) {
+    int got = 0;
+    while (ih_pop(rheap, &j) == 0) {
+      if (j > lastj) {
+        lastj = j;
+        got = 1;
+        break;
+      }
+    }
+    if (!got) break;
+    izcr[j] = 0;
+    if (j >= i) continue;
+    /* dot with row j (trailing columns beyond n clipped, :162-171) */
+    int64_t ip1 = R->ptr[j], ip2 = R->ptr[j + 1];
+    double s = spge_dot(ip2 - ip1, R->idx + ip1, R->val + ip1, zval);
+    double alpha = (-s / pv[j]);
+    if (fabs(alpha) > thresh) {
+      for (int64_t k = fptr[j]; k < fptr[j + 1]; k++) {
+        int32_t kr = f->idx[k];
+        zval[kr] = zval[kr] + alpha * f->val[k];
+        if (izkr[kr] == 0) {
+          if ((rc = ih_push(heap, kr))) return rc;
+          izkr[kr] = 1;
+          /* enqueue rows j < r < i that touch kr (:193-203) */
+          for (int64_t kc = C->ptr[kr]; kc < C->ptr[kr + 1]; kc++) {
+            int32_t nextj = C->idx[kc];
+            if (izcr[nextj] == 0 && nextj > j && nextj < i) {
+              if ((rc = ih_push(rheap, nextj))) return rc;
+              izcr[nextj] = 1;
+            }
+          }
+        }
+      }
+    }
+  }
+  /* pivot: row i of A (Z pass, :214-218) / column i (W pass, :333-340) */
+  {
+    int64_t ip1 = R->ptr[i], ip2 = R->ptr[i + 1];
+    pv[i] = spge_dot(ip2 - ip1, R->idx + ip1, R->val + ip1, zval);
+    if (fabs(pv[i]) < D_EPSTOL) pv[i] = 1.0e-3;
+  }
+  return sparsify_heap(i, nzrmax, thresh, zval, heap, izkr, cand, iz, vz, nzz);
+}
+
+int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
+               double *p, or_mat *zcsc, or_mat *wcsc) {
+  if (alg != OR_ALG_LLK && alg != OR_ALG_S_LLK) return OR_ERR_ALG;
+  int32_t n = a->n;
+  int rc = OR_OK;
+  or_mat ac = {0};
+  if ((rc = or_transpose(a, &ac))) return rc; /* CSC copy of A (:65) */
+  double *zval = (double *)calloc((size_t)n + 1, sizeof(double));
+  double *q = (double *)calloc((size_t)n + 1, sizeof(double));
+  int32_t *izkr = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  int32_t *izcr = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  int32_t *iz = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  double *vz = (double *)malloc(((size_t)n + 1) * sizeof(double));
+  vipair *cand = (vipair *)malloc(((size_t)n + 1) * sizeof(vipair));
+  int64_t *zp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int64_t *wp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  vlist zl = {0}, wl = {0};
+  iheap heap = {0}, rheap = {0};
+  if (!zval || !q || !izkr || !izcr || !iz || !vz || !cand || !zp || !wp) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  /* p_1 = a_11 scanning row 1 (:84-92); z_1 = w_1 = e_1 (:97-110) */
+  p[0] = 0.0;
+  for (int64_t e = a->ptr[0]; e < a->ptr[1]; e++)
+    if (a->idx[e] == 0) {
+      p[0] = a->val[e];
+      break;
+    }
+  if (fabs(p[0]) < D_EPSTOL) p[0] = 1.0e-3;
+  q[0] = p[0];
+  vl_push(&zl, 0, 1.0);
+  zp[1] = 1;
+  vl_push(&wl, 0, 1.0);
+  wp[1] = 1;
+  pass_mats zpass = {a, &ac};  /* Z: dots with rows of A, seed column i */
+  pass_mats wpass = {&ac, a};  /* W: dots with columns of A, seed row i */
+  for (int32_t i = 1; i < n; i++) {
+    int32_t nz;
+    rc = biconjg_pass(i, &zpass, &zl, zp, p, nzrmax, thresh, zval, izkr, izcr,
+                      &heap, &rheap, cand, iz, vz, &nz);
+    if (rc) goto done;
+    for (int32_t k = 0; k < nz; k++)
+      if ((rc = vl_push(&zl, iz[k], vz[k]))) goto done;
+    zp[i + 1] = zl.len;
+    if (alg == OR_ALG_LLK) {
+      rc = biconjg_pass(i, &wpass, &wl, wp, q, nzrmax, thresh, zval, izkr,
+                        izcr, &heap, &rheap, cand, iz, vz, &nz);
+      if (rc) goto done;
+      for (int32_t k = 0; k < nz; k++)
+        if ((rc = vl_push(&wl, iz[k], vz[k]))) goto done;
+      wp[i + 1] = wl.len;
+    }
+  }
+  vl_to_mat(&zl, n, zp, zcsc);
+  zp = NULL;
+  if (alg == OR_ALG_S_LLK) {
+    /* W := Z (mld_dsparse_biconjg_s_llk.F90:246) */
+    wcsc->n = n;
+    wcsc->nnz = zcsc->nnz;
+    wcsc->ptr = (int64_t *)malloc(((size_t)n + 1) * sizeof(int64_t));
+    wcsc->idx = (int32_t *)malloc((size_t)(zcsc->nnz + 1) * sizeof(int32_t));
+    wcsc->val = (double *)malloc((size_t)(zcsc->nnz + 1) * sizeof(double));
+    memcpy(wcsc->ptr, zcsc->ptr, ((size_t)n + 1) * sizeof(int64_t));
+    memcpy(wcsc->idx, zcsc->idx, (size_t)zcsc->nnz * sizeof(int32_t));
+    memcpy(wcsc->val, zcsc->val, (size_t)zcsc->nnz * sizeof(double));
+  } else {
+    vl_to_mat(&wl, n, wp, wcsc);
+    wp = NULL;
+  }
+done:
+  or_mat_free(&ac);
+  free(zval);
+  free(q);
+  free(izkr);
+  free(izcr);
+  free(iz);
+  free(vz);
+  free(cand);
+  free(zp);
+  free(wp);
+  free(zl.idx);
+  free(zl.val);
+  free(wl.idx);
+  free(wl.val);
+  free(heap.a);
+  free(rheap.a);
+  return rc;
+}
+
+static double maxabs(const or_mat *a) {
+  double m = 0.0;
+  for (int64_t k = 0; k < a->ptr[a->n]; k++)
+    if (fabs(a->val[k]) > m) m = fabs(a->val[k]);
+  return m;
+}
+
+static int copy_scaled(const or_mat *a, double s, or_mat *b) {
+  int32_t n = a->n;
+  int64_t nnz = a->ptr[n];
+  b->n = n;
+  b->nnz = nnz;
+  b->ptr = (int64_t *)malloc(((size_t)n + 1) * sizeof(int64_t));
+  b->idx = (int32_t *)malloc((size_t)(nnz + 1) * sizeof(int32_t));
+  b->val = (double *)malloc((size_t)(nnz + 1) * sizeof(double));
+  if (!b->ptr || !b->idx || !b->val) return OR_ERR_ALLOC;
+  memcpy(b->ptr, a->ptr, ((size_t)n + 1) * sizeof(int64_t));
+  memcpy(b->idx, a->idx, (size_t)nnz * sizeof(int32_t));
+  for (int64_t k = 0; k < nnz; k++) b->val[k] = a->val[k] * s;
+  return OR_OK;
+}
+
+int or_ainv_bld(const or_mat *a, int alg, int32_t fill_in, double thresh,
+                or_mat *w, or_mat *z, double *d) {
+  int rc;
+  int32_t n = a->n;
+  /* maxval scaling (mld_d_ainv_bld.F90:121-124) */
+  double weight = maxabs(a);
+  weight = 1.0 / weight;
+  or_mat as = {0}, zc = {0}, wc = {0};
+  if ((rc = copy_scaled(a, weight, &as))) {
+    or_mat_free(&as);
+    return rc;
+  }
+  rc = or_biconjg(alg, &as, fill_in, thresh, d, &zc, &wc);
+  or_mat_free(&as);
+  if (rc) {
+    or_mat_free(&zc);
+    or_mat_free(&wc);
+    return rc;
+  }
+  /* Z: CSC -> CSR; W: CSC -> transpose -> CSR, i.e. the CSC arrays of W
+   * are the CSR arrays of W^T (mld_d_biconjg_mod.F90:131-135) */
+  rc = or_transpose(&zc, z);
+  or_mat_free(&zc);
+  *w = wc;
+  if (rc) return rc;
+  /* inverted pivots with epsilon guard, then unscale (:161-173) */
+  for (int32_t i = 0; i < n; i++) {
+    if (fabs(d[i]) < D_EPSTOL)
+      d[i] = 1.0;
+    else
+      d[i] = 1.0 / d[i];
+  }
+  for (int32_t i = 0; i < n; i++) d[i] = d[i] * weight;
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* ILU(k), IKJ row form (Saad Alg. 10.5) with the MLD2P4 conventions the
+ * plugin consumes (mld_d_invk_bld.f90:104): levels 0 for A's entries,
+ * lev(i,j) = min(lev(i,j), lev(i,k) + lev(k,j) + 1), entries above `fill`
+ * dropped, L(i,k) = row(k) * d(k), d(i) = 1/row(i), U(i,:) = row * d(i). */
+int or_iluk(const or_mat *a, int32_t fill, or_mat *l, or_mat *u, double *d) {
+  int32_t n = a->n;
+  int rc = OR_OK;
+  double *row = (double *)calloc((size_t)n + 1, sizeof(double));
+  int32_t *lev = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  int32_t *idxs = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  int64_t *lp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int64_t *up = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  vlist ll = {0}, ul = {0};
+  int32_t *ulev = NULL;
+  int64_t ulev_cap = 0;
+  iheap heap = {0};
+  if (!row || !lev || !idxs || !lp || !up) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  for (int32_t k = 0; k < n; k++) lev[k] = -(n + 1);
+  for (int32_t i = 0; i < n; i++) {
+    heap.sz = 0;
+    for (int64_t e = a->ptr[i]; e < a->ptr[i + 1]; e++) {
+      int32_t k = a->idx[e];
+      row[k] = a->val[e];
+      lev[k] = 0;
+      if ((rc = ih_push(&heap, k))) goto done;
+    }
+    int32_t nidx = 0, lastk = -1, k;
+    while (ih_pop(&heap, &k) == 0) {
+      if (k == lastk) continue;
+      lastk = k;
+      idxs[nidx++] = k;
+      if (k < i && row[k] != 0.0 && lev[k] <= fill) {
+        /* U is stored scaled by the inverted pivots, so the unscaled
+         * multiplier row(k) is the one that updates the row; L(i,k) =
+         * row(k) * d(k) is formed at copy-out */
+        double rwk = row[k];
+        int32_t lrwk = lev[k];
+        for (int64_t jj = up[k]; jj < up[k + 1]; jj++) {
+          int32_t j = ul.idx[jj];
+          if (lev[j] < 0) {
+            if ((rc = ih_push(&heap, j))) goto done;
+            lev[j] = -lev[j];
+          }
+          row[j] = row[j] - rwk * ul.val[jj];
+          int32_t nl = lrwk + ulev[jj] + 1;
+          if (nl < lev[j]) lev[j] = nl;
+        }
+      }
+    }
+    double piv = 0.0;
+    for (int32_t t = 0; t < nidx; t++) {
+      int32_t j = idxs[t];
+      if (j < i) {
+        if (lev[j] <= fill)
+          if ((rc = vl_push(&ll, j, row[j] * d[j]))) goto done;
+      } else if (j == i) {
+        piv = row[j];
+      } else if (lev[j] <= fill) {
+        if ((rc = vl_push(&ul, j, row[j]))) goto done;
+        if (ul.len > ulev_cap) {
+          int64_t nc = ulev_cap ? 2 * ulev_cap : 1024;
+          while (nc < ul.len) nc *= 2;
+          int32_t *nu = (int32_t *)realloc(ulev, (size_t)nc * sizeof(int32_t));
+          if (!nu) {
+            rc = OR_ERR_ALLOC;
+            goto done;
+          }
+          ulev = nu;
+          ulev_cap = nc;
+        }
+        ulev[ul.len - 1] = lev[j];
+      }
+      row[j] = 0.0;
+      lev[j] = -(n + 1);
+    }
+    if (piv == 0.0) {
+      rc = OR_ERR_PIVOT;
+      goto done;
+    }
+    d[i] = 1.0 / piv;
+    for (int64_t e = up[i]; e < ul.len; e++) ul.val[e] = ul.val[e] * d[i];
+    lp[i + 1] = ll.len;
+    up[i + 1] = ul.len;
+  }
+  vl_to_mat(&ll, n, lp, l);
+  lp = NULL;
+  vl_to_mat(&ul, n, up, u);
+  up = NULL;
+done:
+  free(row);
+  free(lev);
+  free(idxs);
+  free(lp);
+  free(up);
+  free(ll.idx);
+  free(ll.val);
+  free(ul.idx);
+  free(ul.val);
+  free(ulev);
+  free(heap.a);
+  return rc;
+}
+
+/* ILUT(p, tau) (Saad Alg. 10.6) with maxval scaling: entries whose
+ * magnitude falls below tau*||row i||_2 are dropped during elimination and
+ * at copy-out; L keeps the nlw+p and U the nup+p largest entries. */
+int or_ilut(const or_mat *a, int32_t fill, double thresh, or_mat *l,
+            or_mat *u, double *d) {
+  int32_t n = a->n;
+  int rc = OR_OK;
+  double scale = 1.0 / maxabs(a);
+  double *row = (double *)calloc((size_t)n + 1, sizeof(double));
+  int32_t *irwt = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  int32_t *idxs = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  double *rv = NULL;
+  int64_t rv_cap = 0;
+  vipair *cand = (vipair *)malloc(((size_t)n + 1) * sizeof(vipair));
+  int64_t *lp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int64_t *up = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  vlist ll = {0}, ul = {0};
+  iheap heap = {0};
+  if (!row || !irwt || !idxs || !cand || !lp || !up) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  for (int32_t i = 0; i < n; i++) {
+    heap.sz = 0;
+    int32_t nlw = 0, nup = 0;
+    int64_t nz = a->ptr[i + 1] - a->ptr[i];
+    if (nz > rv_cap) {
+      rv_cap = nz * 2;
+      double *nr = (double *)realloc(rv, (size_t)rv_cap * sizeof(double));
+      if (!nr) {
+        rc = OR_ERR_ALLOC;
+        goto done;
+      }
+      rv = nr;
+    }
+    for (int64_t e = a->ptr[i]; e < a->ptr[i + 1]; e++) {
+      int32_t k = a->idx[e];
+      row[k] = a->val[e] * scale;
+      rv[e - a->ptr[i]] = row[k];
+      if ((rc = ih_push(&heap, k))) goto done;
+      irwt[k] = 1;
+      if (k < i) nlw++;
+      if (k > i) nup++;
+    }
+    double nrmi = dnrm2(nz, rv);
+    double thr = thresh * nrmi;
+    int32_t nidx = 0, lastk = -1, k;
+    while (ih_pop(&heap, &k) == 0) {
+      if (k == lastk) continue;
+      lastk = k;
+      if (k < i) {
+        /* drop test on the multiplier L(i,k) = row(k) * d(k); the update
+         * uses the unscaled row(k) against the scaled U */
+        double rwk = row[k];
+        row[k] = row[k] * d[k];
+        if (fabs(row[k]) < thr) {
+          row[k] = 0.0;
+          irwt[k] = 0;
+          continue;
+        }
+        for (int64_t jj = up[k]; jj < up[k + 1]; jj++) {
+          int32_t j = ul.idx[jj];
+          row[j] = row[j] - rwk * ul.val[jj];
+          if (irwt[j] == 0) {
+            if (fabs(row[j]) < thr) {
+              row[j] = 0.0;
+            } else {
+              if ((rc = ih_push(&heap, j))) goto done;
+              irwt[j] = 1;
+            }
+          }
+        }
+      }
+      idxs[nidx++] = k;
+      irwt[k] = 0;
+    }
+    /* copy-out: lower part */
+    int64_t m = 0;
+    double piv = 0.0;
+    for (int32_t t = 0; t < nidx; t++) {
+      int32_t j = idxs[t];
+      if (j < i && fabs(row[j]) >= thr) {
+        cand[m].v = row[j];
+        cand[m].i = j;
+        m++;
+      } else if (j == i) {
+        piv = row[j];
+      }
+    }
+    m = select_top_sorted(cand, m, (int64_t)nlw + fill);
+    for (int64_t t = 0; t < m; t++)
+      if ((rc = vl_push(&ll, cand[t].i, cand[t].v))) goto done;
+    if (piv == 0.0) {
+      rc = OR_ERR_PIVOT;
+      goto done;
+    }
+    d[i] = 1.0 / piv;
+    /* upper part */
+    m = 0;
+    for (int32_t t = 0; t < nidx; t++) {
+      int32_t j = idxs[t];
+      if (j > i && fabs(row[j]) >= thr) {
+        cand[m].v = row[j];
+        cand[m].i = j;
+        m++;
+      }
+    }
+    m = select_top_sorted(cand, m, (int64_t)nup + fill);
+    for (int64_t t = 0; t < m; t++)
+      if ((rc = vl_push(&ul, cand[t].i, cand[t].v * d[i]))) goto done;
+    for (int32_t t = 0; t < nidx; t++) row[idxs[t]] = 0.0;
+    lp[i + 1] = ll.len;
+    up[i + 1] = ul.len;
+  }
+  /* undo the scaling on the pivots: A = (1/scale) * L D U */
+  for (int32_t i = 0; i < n; i++) d[i] = d[i] * scale;
+  vl_to_mat(&ll, n, lp, l);
+  lp = NULL;
+  vl_to_mat(&ul, n, up, u);
+  up = NULL;
+done:
+  free(row);
+  free(irwt);
+  free(idxs);
+  free(rv);
+  free(cand);
+  free(lp);
+  free(up);
+  free(ll.idx);
+  free(ll.val);
+  free(ul.idx);
+  free(ul.val);
+  free(heap.a);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* INVK row inverse (mld_dsparse_invk.f90:94-131 with mld_d_invk_copyin.f90
+ * :84-102, mld_d_invk_inv.f90:69-124 (uplevs = 0, :87), mld_d_invk_copyout
+ * .f90:62-96). */
+int or_sparse_invk(const or_mat *u, int32_t fill_in, or_mat *z) {
+  int32_t n = u->n;
+  int rc = OR_OK;
+  double *row = (double *)calloc((size_t)n + 1, sizeof(double));
+  int32_t *lev = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  int32_t *idxs = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  int64_t *zp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  vlist zl = {0};
+  iheap heap = {0};
+  if (!row || !lev || !idxs || !zp) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  for (int32_t k = 0; k < n; k++) lev[k] = -(n + 1);
+  for (int32_t i = 0; i < n - 1; i++) {
+    heap.sz = 0;
+    for (int64_t e = u->ptr[i]; e < u->ptr[i + 1]; e++) {
+      int32_t k = u->idx[e];
+      row[k] = -1.0 * u->val[e];
+      lev[k] = 0;
+      if ((rc = ih_push(&heap, k))) goto done;
+    }
+    row[i] = 1.0;
+    lev[i] = 0;
+    int32_t nidx = 1, lastk = i, k;
+    idxs[0] = i;
+    while (ih_pop(&heap, &k) == 0) {
+      if (k == lastk) continue;
+      lastk = k;
+      idxs[nidx++] = k;
+      if (row[k] != 0.0 && lev[k] <= fill_in) {
+        double rwk = row[k];
+        int32_t lrwk = lev[k];
+        for (int64_t jj = u->ptr[k]; jj < u->ptr[k + 1]; jj++) {
+          int32_t j = u->idx[jj];
+          if (j <= k) {
+            rc = OR_ERR_INTERNAL; /* inv.f90:100-103 */
+            goto done;
+          }
+          if (lev[j] < 0) {
+            if ((rc = ih_push(&heap, j))) goto done;
+            lev[j] = -lev[j];
+          }
+          row[j] = row[j] - rwk * u->val[jj];
+          int32_t nl = lrwk + 0 + 1;
+          if (nl < lev[j]) lev[j] = nl;
+        }
+      }
+    }
+    for (int32_t t = 0; t < nidx; t++) {
+      int32_t j = idxs[t];
+      if (j >= i) {
+        if (lev[j] <= fill_in)
+          if ((rc = vl_push(&zl, j, row[j]))) goto done;
+        row[j] = 0.0;
+        lev[j] = -(n + 1);
+      }
+    }
+    zp[i + 1] = zl.len;
+  }
+  if (n >= 1) {
+    /* last row is e_n (:120-124) */
+    if ((rc = vl_push(&zl, n - 1, 1.0))) goto done;
+    zp[n] = zl.len;
+  }
+  vl_to_mat(&zl, n, zp, z);
+  zp = NULL;
+done:
+  free(row);
+  free(lev);
+  free(idxs);
+  free(zp);
+  free(zl.idx);
+  free(zl.val);
+  free(heap.a);
+  return rc;
+}
+
+/* INVT row inverse (mld_dsparse_invt.f90:87-128, copyin.f90:82-104,
+ * inv.f90:69-137, copyout.f90:124-286 with the active `.true.` branch). */
+int or_sparse_invt(const or_mat *u, int32_t fill_in, double thresh,
+                   or_mat *z) {
+  int32_t n = u->n;
+  int rc = OR_OK;
+  double *row = (double *)calloc((size_t)n + 1, sizeof(double));
+  int32_t *irwt = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  int32_t *idxs = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  vipair *cand = (vipair *)malloc(((size_t)n + 1) * sizeof(vipair));
+  int64_t *zp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  vlist zl = {0};
+  iheap heap = {0};
+  if (!row || !irwt || !idxs || !cand || !zp) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  for (int32_t i = 0; i < n - 1; i++) {
+    heap.sz = 0;
+    int32_t nup = 0;
+    for (int64_t e = u->ptr[i]; e < u->ptr[i + 1]; e++) {
+      int32_t k = u->idx[e];
+      row[k] = -1.0 * u->val[e];
+      if ((rc = ih_push(&heap, k))) goto done;
+      irwt[k] = 1;
+      if (k > i) nup++;
+    }
+    double nrmi = dnrm2(u->ptr[i + 1] - u->ptr[i], u->val + u->ptr[i]);
+    row[i] = 1.0;
+    /* norm of the row with the unit diagonal included (:100-104) */
+    if (nrmi < 1.0)
+      nrmi = sqrt(1.0 + nrmi * nrmi);
+    else
+      nrmi = nrmi * sqrt(1.0 + 1.0 / (nrmi * nrmi));
+    double thr = thresh * nrmi;
+    int32_t nidx = 1, lastk = i, k;
+    idxs[0] = i;
+    irwt[i] = 1;
+    while (ih_pop(&heap, &k) == 0) {
+      if (k == lastk) continue;
+      lastk = k;
+      double rwk = row[k];
+      if (fabs(rwk) < thr) {
+        row[k] = 0.0;
+        irwt[k] = 0;
+        continue;
+      }
+      for (int64_t jj = u->ptr[k]; jj < u->ptr[k + 1]; jj++) {
+        int32_t j = u->idx[jj];
+        if (j <= k) {
+          rc = OR_ERR_INTERNAL; /* inv.f90:101-104 */
+          goto done;
+        }
+        row[j] = row[j] - rwk * u->val[jj];
+        if (irwt[j] == 0) {
+          if (fabs(row[j]) < thr) {
+            row[j] = 0.0;
+          } else {
+            if ((rc = ih_push(&heap, j))) goto done;
+            irwt[j] = 1;
+          }
+        }
+      }
+      idxs[nidx++] = k;
+      irwt[k] = 0;
+    }
+    irwt[i] = 0;
+    /* copy-out: threshold (diagonal exempt), running-min filter once the
+     * cap nup+fill is exceeded, then keep the largest nup+fill (the
+     * diagonal is not protected), ascending index order */
+    int64_t cap = (int64_t)nup + fill_in;
+    int64_t m = 0;
+    double wmin = DBL_MAX;
+    for (int32_t t = 0; t < nidx; t++) {
+      int32_t widx = idxs[t];
+      double witem = row[widx];
+      if (widx != i && fabs(witem) < thr) continue;
+      if (m > cap && fabs(witem) < wmin) continue;
+      if (fabs(witem) < wmin) wmin = fabs(witem);
+      cand[m].v = witem;
+      cand[m].i = widx;
+      m++;
+    }
+    m = select_top_sorted(cand, m, cap);
+    for (int64_t t = 0; t < m; t++)
+      if ((rc = vl_push(&zl, cand[t].i, cand[t].v))) goto done;
+    for (int32_t t = 0; t < nidx; t++) row[idxs[t]] = 0.0;
+    zp[i + 1] = zl.len;
+  }
+  if (n >= 1) {
+    if ((rc = vl_push(&zl, n - 1, 1.0))) goto done;
+    zp[n] = zl.len;
+  }
+  vl_to_mat(&zl, n, zp, z);
+  zp = NULL;
+done:
+  free(row);
+  free(irwt);
+  free(idxs);
+  free(cand);
+  free(zp);
+  free(zl.idx);
+  free(zl.val);
+  free(heap.a);
+  return rc;
+}
+
+/* W = L^{-1} through (inverse of L^T)^T (mld_d_invk_bld.f90:119-124) */
+static int lower_inverse(const or_mat *l, int32_t fill, double thresh,
+                         int is_t, or_mat *w) {
+  or_mat lt = {0}, wt = {0};
+  int rc = or_transpose(l, &lt);
+  if (!rc)
+    rc = is_t ? or_sparse_invt(&lt, fill, thresh, &wt)
+              : or_sparse_invk(&lt, fill, &wt);
+  if (!rc) rc = or_transpose(&wt, w);
+  or_mat_free(&lt);
+  or_mat_free(&wt);
+  return rc;
+}
+
+int or_invk_bld(const or_mat *a, int32_t fill1, int32_t fill2, or_mat *w,
+                or_mat *z, double *d) {
+  or_mat l = {0}, u = {0};
+  int rc = or_iluk(a, fill1, &l, &u, d);
+  if (!rc) rc = or_sparse_invk(&u, fill2, z);
+  if (!rc) rc = lower_inverse(&l, fill2, 0.0, 0, w);
+  or_mat_free(&l);
+  or_mat_free(&u);
+  return rc;
+}
+
+int or_invt_bld(const or_mat *a, int32_t fill, double thresh, int32_t invfill,
+                double invthresh, or_mat *w, or_mat *z, double *d) {
+  or_mat l = {0}, u = {0};
+  int rc = or_ilut(a, fill, thresh, &l, &u, d);
+  if (!rc) rc = or_sparse_invt(&u, invfill, invthresh, z);
+  if (!rc) rc = lower_inverse(&l, invfill, invthresh, 1, w);
+  or_mat_free(&l);
+  or_mat_free(&u);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ */
+void or_spmv(char trans, double alpha, const or_mat *a, const double *x,
+             double beta, double *y) {
+  int32_t n = a->n;
+  if (trans == 'N' || trans == 'n') {
+    for (int32_t i = 0; i < n; i++) {
+      double acc = 0.0;
+      for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
+        acc = acc + a->val[k] * x[a->idx[k]];
+      y[i] = (beta == 0.0) ? alpha * acc : alpha * acc + beta * y[i];
+    }
+  } else {
+    double *acc = (double *)calloc((size_t)n + 1, sizeof(double));
+    for (int32_t i = 0; i < n; i++)
+      for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
+        acc[a->idx[k]] = acc[a->idx[k]] + a->val[k] * x[i];
+    for (int32_t i = 0; i < n; i++)
+      y[i] = (beta == 0.0) ? alpha * acc[i] : alpha * acc[i] + beta * y[i];
+    free(acc);
+  }
+}
+
+int or_apply(char trans, double alpha, const or_mat *w, const or_mat *z,
+             const double *d, const double *x, double beta, double *y) {
+  int32_t n = w->n;
+  char t = trans;
+  if (t == 'n') t = 'N';
+  if (t == 't') t = 'T';
+  if (t == 'c') t = 'C';
+  if (t != 'N' && t != 'T' && t != 'C') return OR_ERR_ARG;
+  double *tx = (double *)calloc((size_t)n + 1, sizeof(double));
+  if (!tx) return OR_ERR_ALLOC;
+  if (t == 'N') {
+    or_spmv('N', 1.0, w, x, 0.0, tx);
+    for (int32_t i = 0; i < n; i++) tx[i] = d[i] * tx[i];
+    or_spmv('N', alpha, z, tx, beta, y);
+  } else {
+    or_spmv('T', 1.0, z, x, 0.0, tx);
+    for (int32_t i = 0; i < n; i++) tx[i] = d[i] * tx[i];
+    or_spmv('T', alpha, w, tx, beta, y);
+  }
+  free(tx);
+  return OR_OK;
+}
+
+static double dot(int32_t n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int32_t i = 0; i < n; i++) s = s + a[i] * b[i];
+  return s;
+}
+
+/* BiCGSTAB (van der Vorst 1992) in the PSBLAS psb_dbicgstab arrangement:
+ * r0 = b - A x0, rtilde = r0, convergence tested on the recursively updated
+ * residual after each full step, istop = 2 (||r||_2 / ||b||_2). */
+int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
+                const double *d, const double *b, double *x, double eps,
+                int32_t itmax, int32_t *iter, double *err) {
+  int32_t n = a->n;
+  size_t sz = ((size_t)n + 1) * sizeof(double);
+  double *r = (double *)malloc(sz), *q = (double *)malloc(sz);
+  double *p = (double *)calloc(1, sz), *v = (double *)calloc(1, sz);
+  double *f = (double *)malloc(sz), *s = (double *)malloc(sz);
+  double *zz = (double *)malloc(sz), *t = (double *)malloc(sz);
+  double rho = 0.0, rho_old, alpha = 0.0, omega = 0.0, beta, sigma, tau;
+  memcpy(r, b, (size_t)n * sizeof(double));
+  or_spmv('N', -1.0, a, x, 1.0, r);
+  memcpy(q, r, (size_t)n * sizeof(double));
+  double bn2 = sqrt(dot(n, b, b));
+  double rn2 = sqrt(dot(n, r, r));
+  *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+  int32_t it = 0;
+  if (*err <= eps) goto out;
+  while (it < itmax) {
+    it++;
+    rho_old = rho;
+    rho = dot(n, q, r);
+    if (rho == 0.0) break;
+    if (it == 1) {
+      memcpy(p, r, (size_t)n * sizeof(double));
+    } else {
+      beta = (rho / rho_old) * (alpha / omega);
+      for (int32_t i = 0; i < n; i++) p[i] = p[i] + (-omega) * v[i];
+      for (int32_t i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
+    }
+    or_apply('N', 1.0, w, z, d, p, 0.0, f);
+    or_spmv('N', 1.0, a, f, 0.0, v);
+    sigma = dot(n, q, v);
+    if (sigma == 0.0) break;
+    alpha = rho / sigma;
+    for (int32_t i = 0; i < n; i++) s[i] = r[i] + (-alpha) * v[i];
+    or_apply('N', 1.0, w, z, d, s, 0.0, zz);
+    or_spmv('N', 1.0, a, zz, 0.0, t);
+    sigma = dot(n, t, t);
+    if (sigma == 0.0) break;
+    tau = dot(n, t, s);
+    omega = tau / sigma;
+    if (omega == 0.0) break;
+    for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * f[i];
+    for (int32_t i = 0; i < n; i++) x[i] = x[i] + omega * zz[i];
+    for (int32_t i = 0; i < n; i++) r[i] = s[i] + (-omega) * t[i];
+    rn2 = sqrt(dot(n, r, r));
+    *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+    if (*err <= eps) break;
+  }
+out:
+  *iter = it;
+  free(r);
+  free(q);
+  free(p);
+  free(v);
+  free(f);
+  free(s);
+  free(zz);
+  free(t);
+  return OR_OK;
+}
+
+/* Preconditioned CG (Hestenes-Stiefel) in the PSBLAS psb_dcg arrangement. */
+int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
+          const double *b, double *x, double eps, int32_t itmax,
+          int32_t *iter, double *err) {
+  int32_t n = a->n;
+  size_t sz = ((size_t)n + 1) * sizeof(double);
+  double *r = (double *)malloc(sz), *zz = (double *)malloc(sz);
+  double *p = (double *)calloc(1, sz), *q = (double *)malloc(sz);
+  double rho = 0.0, rho_old, beta, sigma, alpha;
+  memcpy(r, b, (size_t)n * sizeof(double));
+  or_spmv('N', -1.0, a, x, 1.0, r);
+  double bn2 = sqrt(dot(n, b, b));
+  double rn2 = sqrt(dot(n, r, r));
+  *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+  int32_t it = 0;
+  if (*err <= eps) goto out;
+  while (it < itmax) {
+    it++;
+    or_apply('N', 1.0, w, z, d, r, 0.0, zz);
+    rho_old = rho;
+    rho = dot(n, r, zz);
+    if (it == 1) {
+      memcpy(p, zz, (size_t)n * sizeof(double));
+    } else {
+      beta = rho / rho_old;
+      for (int32_t i = 0; i < n; i++) p[i] = zz[i] + beta * p[i];
+    }
+    or_spmv('N', 1.0, a, p, 0.0, q);
+    sigma = dot(n, p, q);
+    if (sigma == 0.0) break;
+    alpha = rho / sigma;
+    for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * p[i];
+    for (int32_t i = 0; i < n; i++) r[i] = r[i] + (-alpha) * q[i];
+    rn2 = sqrt(dot(n, r, r));
+    *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+    if (*err <= eps) break;
+  }
+out:
+  *iter = it;
+  free(r);
+  free(zz);
+  free(p);
+  free(q);
+  return OR_OK;
+}

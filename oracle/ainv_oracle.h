@@ -1,0 +1,112 @@
+/*
+ * ainv_oracle.h -- CPU restatement of the approximate-inverse preconditioner
+ * algorithms of the MLD2P4 AINV plugin (AINV / INVK / INVT), FP64, single
+ * thread, reference loop order.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests,
+ * the smoke check and the CPU-baseline leg of the benchmark call it.  The
+ * product library (mld-ainv_amd/) never links, loads or calls it.
+ *
+ * Conventions: 0-based indices, CSR/CSC with int64 pointers and int32
+ * indices.  Every routine cites the reference file:line (1-based Fortran)
+ * it restates.  Outputs are malloc'ed and released with or_free().
+ */
+#ifndef AINV_ORACLE_H
+#define AINV_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes (mirror the reference's categories) */
+#define OR_OK 0
+#define OR_ERR_ARG 1      /* invalid argument (psb_err_iarg_invalid_i_) */
+#define OR_ERR_ALG 2      /* invalid alg (biconjg_mod.F90:119-122) */
+#define OR_ERR_ALLOC 3
+#define OR_ERR_INTERNAL 4 /* e.g. non-upper entry inside the row inverse */
+#define OR_ERR_PIVOT 5    /* zero pivot in ILU */
+
+/* algorithm ids for biconjugation; names from mld_d_ainv_solver.F90:240-257 */
+#define OR_ALG_LLK 0
+#define OR_ALG_S_LLK 1
+
+/* A sparse matrix handed out by the oracle (CSR unless stated otherwise). */
+typedef struct {
+  int32_t n;
+  int64_t nnz;
+  int64_t *ptr;
+  int32_t *idx;
+  double *val;
+} or_mat;
+
+void or_free(void *p);
+void or_mat_free(or_mat *m);
+
+/* sparsify (mld_d_sparsify.f90:73-143), stand-alone form without iheap
+ * (:107-116): scans zw[istart..n), keeps idiag plus at most nzrmax-1 entries
+ * with |z| >= thresh, largest |z| first (ties: smaller index first), output
+ * sorted by index.  zw is zeroed on return.  Returns the count. */
+int32_t or_sparsify(int32_t idiag, int32_t nzrmax, double thresh, int32_t n,
+                    double *zw, int32_t istart, int32_t *iz, double *valz);
+
+/* Sparse biconjugation (mld_dsparse_biconjg_llk.F90 / _s_llk.F90).  Input A
+ * CSR (already scaled).  Output Z (CSC, n x n, unit upper), W (CSC), p. */
+int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
+               double *p, or_mat *zcsc, or_mat *wcsc);
+
+/* AINV factor build (mld_d_ainv_bld.F90:115-182, maxval scaling as
+ * requested by mld_d_ainv_solver_bld.f90:68-69).  Outputs CSR: w = W^T
+ * (unit lower), z = Z (unit upper), d (inverted pivots, unscaled). */
+int or_ainv_bld(const or_mat *a, int alg, int32_t fill_in, double thresh,
+                or_mat *w, or_mat *z, double *d);
+
+/* ILU(k) (MLD2P4 mld_iluk_fact restated; unit L strictly lower, unit U
+ * strictly upper scaled by inverted pivots, d = inverted pivots). */
+int or_iluk(const or_mat *a, int32_t fill, or_mat *l, or_mat *u, double *d);
+/* ILUT(fill, thresh) with maxval scaling (MLD2P4 mld_ilut_fact restated). */
+int or_ilut(const or_mat *a, int32_t fill, double thresh, or_mat *l,
+            or_mat *u, double *d);
+
+/* Row-wise sparse inverse of a unit upper U given strictly upper
+ * (mld_dsparse_invk.f90:66-131 / mld_dsparse_invt.f90:66-128). Output CSR
+ * with the diagonal stored. */
+int or_sparse_invk(const or_mat *u, int32_t fill_in, or_mat *z);
+int or_sparse_invt(const or_mat *u, int32_t fill_in, double thresh,
+                   or_mat *z);
+
+/* INVK / INVT builds (mld_d_invk_bld.f90:104-124, mld_d_invt_bld.f90:
+ * 108-136): w = L^{-1} (stored, lower incl. diagonal), z = U^{-1}, d. */
+int or_invk_bld(const or_mat *a, int32_t fill1, int32_t fill2, or_mat *w,
+                or_mat *z, double *d);
+int or_invt_bld(const or_mat *a, int32_t fill, double thresh, int32_t invfill,
+                double invthresh, or_mat *w, or_mat *z, double *d);
+
+/* Transpose of a CSR matrix (psb transp + cscnv). */
+int or_transpose(const or_mat *a, or_mat *t);
+
+/* y = alpha*A*x + beta*y (trans 'N') or alpha*A^T*x + beta*y ('T'). */
+void or_spmv(char trans, double alpha, const or_mat *a, const double *x,
+             double beta, double *y);
+
+/* Preconditioner apply (mld_d_base_ainv_solver_apply_vect.f90:112-127):
+ * 'N': y = alpha*Z*(d o (W*x)) + beta*y;  'T': y = alpha*W^T*(d o (Z^T x))
+ * + beta*y.  Returns OR_ERR_ARG for any other trans (:63-69). */
+int or_apply(char trans, double alpha, const or_mat *w, const or_mat *z,
+             const double *d, const double *x, double beta, double *y);
+
+/* Preconditioned Krylov solvers with the istop=2 test ||r||/||b|| <= eps,
+ * restating PSBLAS psb_dbicgstab / psb_dcg (called from ppde3d.f90:280).
+ * The preconditioner is (w, z, d) applied with trans 'N'.  Returns the
+ * iteration count in *iter (-1 on breakdown) and the last estimate. */
+int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
+                const double *d, const double *b, double *x, double eps,
+                int32_t itmax, int32_t *iter, double *err);
+int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
+          const double *b, double *x, double eps, int32_t itmax,
+          int32_t *iter, double *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

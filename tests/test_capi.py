@@ -1,0 +1,86 @@
+"""C-ABI surface on the CPU: the library loads, exports every entry point
+declared in include/*.h, and the host-side logic of the solver objects
+(setters, check, descr, defaults, error codes) behaves as the reference's
+(no GPU needed: none of these calls touches the device)."""
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[\w\s\*]+?\b(ainv_\w+)\s*\(",
+                             text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_every_declared_symbol_is_exported(mld):
+    names = declared_functions()
+    assert len(names) > 40
+    out = subprocess.run(["nm", "-D", "--defined-only", mld.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = sorted(names - exported)
+    assert not missing, missing
+    lib = mld.lib()
+    for n in names:
+        getattr(lib, n)
+
+
+def test_defaults_setters_and_check(mld):
+    a = mld.MldDAinvSolver()
+    # d_ainv_solver_default: alg LLK, fill_in 0, thresh 0
+    assert a.geti("AINV_ALG") == mld.ALG_LLK
+    assert a.geti("SUB_FILLIN") == 0 and a.getr("SUB_ILUTHRS") == 0.0
+    a.set("ainv_alg", "sym-llk")  # keys and values are case-insensitive
+    assert a.geti("AINV_ALG") == mld.ALG_S_LLK
+    a.set("AINV_ALG", "STAB-LLK")
+    assert a.geti("AINV_ALG") == mld.ALG_S_FT_LLK
+    a.set("SUB_ILUTHRS", -1.0)
+    a.check()  # fill_in 0 -> 1 (is_positive_nz_min), thresh -1 -> 0
+    assert a.geti("SUB_FILLIN") == 1 and a.getr("SUB_ILUTHRS") == 0.0
+    a.set("SOME_UNKNOWN_KEY", 3)  # passed to the base solver, ignored
+    assert "Stabilized Left-looking" in a.descr()
+    k = mld.MldDInvkSolver()
+    k.set("SUB_FILLIN", -2)
+    k.set("INV_FILLIN", 3)
+    k.check()
+    assert k.geti("SUB_FILLIN") == 0 and k.geti("INV_FILLIN") == 3
+    assert "INVK Approximate Inverse with ILU(N)" in k.descr()
+    t = mld.MldDInvtSolver()
+    t.set("SUB_ILUTHRS", 1e-2)
+    t.set("INV_THRESH", 5e-2)
+    t.set("FACTOR_PREC", "bf16")
+    assert t.getr("INV_THRESH") == 5e-2
+    assert t.geti("FACTOR_PREC") == mld.PREC_BF16
+    assert "Inverse fill threshold" in t.descr()
+    c = mld.MldDInvtSolver()
+    t.clone_settings(c)
+    assert c.getr("INV_THRESH") == 5e-2
+    assert mld.MldDAinvSolver.stringval("MLK") == mld.ALG_MLK
+    assert mld.MldDAinvSolver.stringval("nope") == -1
+    assert mld.MldDAinvSolver.algname(mld.ALG_LLK).strip() == "Left-looking"
+
+
+def test_argument_errors_without_device(mld):
+    t = mld.MldDInvtSolver()
+    with pytest.raises(mld.AinvError) as e:
+        t.set("FACTOR_PREC", "FP8")
+    assert e.value.code == mld.ERR_ARG
+    with pytest.raises(mld.AinvError) as e:
+        t.geti("NOT_A_KEY")
+    assert e.value.code == mld.ERR_ARG
+    assert t.get_wrksz() == 2
+    assert t.get_nzeros() == 0  # not built
+    with pytest.raises(mld.AinvError) as e:
+        mld.MldDInvkSolver().export("W")
+    assert e.value.code == mld.ERR_STATE

@@ -1,0 +1,227 @@
+"""GPU parity of the HIP library against the CPU oracle, through the C ABI.
+
+Bars: factor patterns and values bit-exact (the build kernels reproduce the
+reference's operation order with no contraction); FP64 apply within 1e-12
+relative; FP32 / BF16 factor storage within the stated tolerances of the FP64
+apply; Krylov iteration counts within +-1 of the oracle at the same stopping
+tolerance.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def same_csr(a, b):
+    a, b = a.tocsr(), b.tocsr()
+    a.sort_indices()
+    b.sort_indices()
+    return (a.shape == b.shape and np.array_equal(a.indptr, b.indptr)
+            and np.array_equal(a.indices, b.indices)
+            and np.array_equal(a.data, b.data))
+
+
+def build(mld, kind, a, **params):
+    sv = mld.SOLVERS[kind]()
+    for k, v in params.items():
+        sv.set(k, v)
+    sv.build(a)
+    return sv
+
+
+def check_factors(sv, ref):
+    w_ref, z_ref, d_ref = ref
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    assert same_csr(w, w_ref), "W differs from the oracle"
+    assert same_csr(z, z_ref), "Z differs from the oracle"
+    assert np.array_equal(d, d_ref), "d differs from the oracle"
+
+
+@pytest.mark.parametrize("fill1,fill2", [(0, 0), (0, 1), (1, 1), (1, 2), (2, 1)])
+def test_invk_factors_bit_exact(gpu, O, fill1, fill2):
+    a = gpu.gen_pde3d(9)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=fill1, INV_FILLIN=fill2)
+    check_factors(sv, O.invk_bld(a, fill1, fill2))
+
+
+@pytest.mark.parametrize("fill,thr,ifill,ithr", [
+    (10, 1e-2, 10, 1e-2), (0, 1e-1, 0, 1e-1), (3, 1e-3, 2, 1e-2),
+    (10, 0.0, 5, 0.0)])
+def test_invt_factors_bit_exact(gpu, O, fill, thr, ifill, ithr):
+    a = gpu.gen_pde3d(9)
+    sv = build(gpu, "INVT", a, SUB_FILLIN=fill, SUB_ILUTHRS=thr,
+               INV_FILLIN=ifill, INV_THRESH=ithr)
+    check_factors(sv, O.invt_bld(a, fill, thr, ifill, ithr))
+
+
+def test_invt_random_matrix_bit_exact(gpu, O):
+    a = gpu.gen_random(3000, seed=11, median=6.0, band=200)
+    sv = build(gpu, "INVT", a, SUB_FILLIN=5, SUB_ILUTHRS=1e-2, INV_FILLIN=4,
+               INV_THRESH=5e-2)
+    check_factors(sv, O.invt_bld(a, 5, 1e-2, 4, 5e-2))
+
+
+def test_invk_global_tier_exact_inverse(gpu, O):
+    """Rows of the exact inverse exceed the LDS capacity (512 pending
+    entries) and go through the global-memory tier."""
+    a = gpu.gen_pde3d(10)
+    n = a.shape[0]
+    sv = build(gpu, "INVK", a, SUB_FILLIN=0, INV_FILLIN=n)
+    rep = sv.report()
+    assert rep["inverse"]["upper"]["lds_overflow_rows"] > 0
+    check_factors(sv, O.invk_bld(a, 0, n))
+
+
+@pytest.mark.parametrize("alg,thr,fill", [("LLK", 0.1, 8), ("LLK", 0.0, 4),
+                                          ("LLK", 0.05, 0)])
+def test_ainv_llk_bit_exact(gpu, O, alg, thr, fill):
+    a = gpu.gen_pde3d(8)
+    sv = build(gpu, "AINV", a, AINV_ALG=alg, SUB_ILUTHRS=thr, SUB_FILLIN=fill)
+    # build calls check: fill 0 is coerced to 1 as the reference does
+    check_factors(sv, O.ainv_bld(a, alg, max(fill, 1), thr))
+
+
+def test_ainv_sym_llk_bit_exact(gpu, O):
+    a = gpu.gen_aniso2d(30, 1e-3)
+    sv = build(gpu, "AINV", a, AINV_ALG="SYM-LLK", SUB_ILUTHRS=5e-2,
+               SUB_FILLIN=8)
+    check_factors(sv, O.ainv_bld(a, "SYM-LLK", 8, 5e-2))
+
+
+def test_tiny_and_diagonal_matrices(gpu, O):
+    for a in [sp.csr_matrix(np.array([[4.0]])),
+              sp.csr_matrix(np.diag([2.0, -3.0, 5.0])),
+              sp.csr_matrix(np.array([[4.0, 1.0], [-1.0, 3.0]]))]:
+        for kind, ref in [("INVK", lambda m: O.invk_bld(m, 1, 1)),
+                          ("INVT", lambda m: O.invt_bld(m, 2, 0.0, 2, 0.0))]:
+            sv = build(gpu, kind, a, SUB_FILLIN=1, INV_FILLIN=1) \
+                if kind == "INVK" else build(gpu, kind, a, SUB_FILLIN=2,
+                                             INV_FILLIN=2)
+            check_factors(sv, ref(a))
+            x = np.arange(1.0, a.shape[0] + 1)
+            assert np.allclose(sv.apply(x), np.linalg.solve(a.toarray(), x),
+                               rtol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["INVK", "INVT", "AINV"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_apply_fp64_matches_oracle(gpu, O, kind, trans):
+    a = gpu.gen_pde3d(11)
+    params = {"INVK": dict(SUB_FILLIN=1, INV_FILLIN=1),
+              "INVT": dict(SUB_FILLIN=10, SUB_ILUTHRS=1e-2, INV_FILLIN=10,
+                           INV_THRESH=1e-2),
+              "AINV": dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=0.1)}
+    sv = build(gpu, kind, a, **params[kind])
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    n = a.shape[0]
+    x = gpu.gen_normal(3, (n,))
+    y0 = gpu.gen_normal(4, (n,))
+    for alpha, beta in [(1.0, 0.0), (-2.5, 0.75)]:
+        y = sv.apply(x, y=y0.copy(), alpha=alpha, beta=beta, trans=trans)
+        ref = O.apply(w, z, d, x, alpha=alpha, beta=beta, y=y0, trans=trans)
+        rel = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+        assert rel < 1e-12, rel
+
+
+@pytest.mark.parametrize("k", [2, 3, 16, 33, 64])
+def test_block_apply_matches_columns(gpu, O, k):
+    a = gpu.gen_pde3d(10)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    n = a.shape[0]
+    x = gpu.gen_normal(5, (n, k))
+    y = sv.apply(x)
+    for c in range(k):
+        ref = O.apply(w, z, d, x[:, c])
+        assert np.linalg.norm(y[:, c] - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("prec,tol", [("F32", 2e-6), ("BF16", 2e-2)])
+def test_reduced_precision_factors(gpu, O, prec, tol):
+    """FP32 / BF16 factor storage: error against the FP64 apply bounded by
+    the storage precision (the factor values are the only rounded input)."""
+    a = gpu.gen_pde3d(12)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC=prec)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    x = gpu.gen_normal(6, (a.shape[0], 4))
+    y = sv.apply(x)
+    for c in range(4):
+        ref = O.apply(w, z, d, x[:, c])
+        rel = np.linalg.norm(y[:, c] - ref) / np.linalg.norm(ref)
+        assert rel < tol, (prec, rel)
+    sv.cnv("F64")
+    y64 = sv.apply(x[:, 0])
+    ref = O.apply(w, z, d, x[:, 0])
+    assert np.linalg.norm(y64 - ref) < 1e-12 * np.linalg.norm(ref)
+
+
+def test_krylov_bicgstab_iterations(gpu, O):
+    a = gpu.gen_pde3d(14)
+    n = a.shape[0]
+    b = a @ np.ones(n)
+    A = gpu.SystemMatrix(a)
+    for kind, params, ref in [
+            ("INVK", dict(SUB_FILLIN=1, INV_FILLIN=1),
+             lambda: O.invk_bld(a, 1, 1)),
+            ("AINV", dict(SUB_FILLIN=8, SUB_ILUTHRS=0.1),
+             lambda: O.ainv_bld(a, "LLK", 8, 0.1))]:
+        sv = build(gpu, kind, a, **params)
+        x, it, err = gpu.krylov("BICGSTAB", A, sv, b, eps=1e-8)
+        w, z, d = ref()
+        _, it_ref, _ = O.krylov("BICGSTAB", a, w, z, d, b, 1e-8)
+        assert abs(int(it[0]) - it_ref) <= 1, (kind, it, it_ref)
+        true = np.linalg.norm(b - a @ x) / np.linalg.norm(b)
+        assert err[0] <= 1e-8 and true < 1e-7
+
+
+def test_krylov_cg_block(gpu, O):
+    a = gpu.gen_aniso2d(40, 1e-3)
+    n = a.shape[0]
+    k = 4
+    B = gpu.gen_normal(3, (n, k))
+    A = gpu.SystemMatrix(a)
+    sv = build(gpu, "AINV", a, AINV_ALG="SYM-LLK", SUB_FILLIN=8,
+               SUB_ILUTHRS=5e-2)
+    X, it, err = gpu.krylov("CG", A, sv, B, eps=1e-8)
+    w, z, d = O.ainv_bld(a, "SYM-LLK", 8, 5e-2)
+    for c in range(k):
+        _, it_ref, _ = O.krylov("CG", a, w, z, d, B[:, c], 1e-8)
+        assert abs(int(it[c]) - it_ref) <= 1, (c, it[c], it_ref)
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c])
+
+
+def test_error_paths(gpu):
+    sv = gpu.MldDInvkSolver()
+    x = np.ones(10)
+    with pytest.raises(gpu.AinvError) as e:
+        sv.apply(x)
+    assert e.value.code == gpu.ERR_STATE
+    a = gpu.gen_pde3d(4)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    with pytest.raises(gpu.AinvError) as e:
+        sv.apply(np.ones(a.shape[0]), trans="X")
+    assert e.value.code == gpu.ERR_ARG
+    bad = gpu.MldDAinvSolver()
+    bad.set("AINV_ALG", "NO-SUCH-ALG")
+    with pytest.raises(gpu.AinvError) as e:
+        bad.build(a)
+    assert e.value.code == gpu.ERR_ALG
+
+
+def test_clone_clear_and_accessors(gpu, O):
+    a = gpu.gen_pde3d(8)
+    sv = build(gpu, "INVT", a, SUB_FILLIN=4, SUB_ILUTHRS=1e-2, INV_FILLIN=4,
+               INV_THRESH=1e-2)
+    c = sv.clone()
+    x = gpu.gen_normal(9, (a.shape[0],))
+    assert np.array_equal(sv.apply(x), c.apply(x))
+    w, z, d = O.invt_bld(a, 4, 1e-2, 4, 1e-2)
+    assert sv.get_nzeros() == a.shape[0] + w.nnz + z.nnz
+    assert sv.get_wrksz() == 2
+    assert sv.sizeof() > 0
+    sv.clear_data()
+    with pytest.raises(gpu.AinvError):
+        sv.apply(x)
+    c.free()

@@ -1,0 +1,313 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+The reference ships no golden vectors (its tests are sample drivers that
+print timings), so every oracle function is pinned here by what the
+mathematics fixes -- exact inverses and factorizations when nothing is
+dropped, biconjugation invariants, graph-distance patterns -- and by
+independent dense restatements (tests/refcheck.py) on small inputs, plus the
+committed fixtures of tests/golden/.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import refcheck as R
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rand_sparse(n, density, seed, dom=1.5, sym=False):
+    rng = np.random.default_rng(seed)
+    m = sp.random(n, n, density=density, random_state=rng,
+                  data_rvs=lambda k: rng.standard_normal(k)).tocsr()
+    if sym:
+        m = m + m.T
+    d = np.asarray(abs(m).sum(axis=1)).ravel() * dom + 1.0
+    m = (m + sp.diags(d)).tocsr()
+    m.sort_indices()
+    return m
+
+
+def as_dict_rows(m):
+    return R.rows_of(m)
+
+
+# ------------------------------------------------------------- generator
+def test_generator_matches_independent_numpy(mld):
+    for N in (3, 6):
+        a = mld.gen_pde3d(N)
+        b = R.pde3d_numpy(N)
+        assert (a != b).nnz == 0
+        assert a.nnz == 7 * N ** 3 - 6 * N ** 2
+    # genpde's own spacing h = 1/(N+2)
+    a = mld.gen_pde3d(4, hconv=1)
+    assert np.allclose(a.diagonal(), 6.0 * 36.0)
+
+
+def test_aniso_and_random_generators(mld):
+    a = mld.gen_aniso2d(16, 1e-3)
+    n = 256
+    assert a.nnz == 5 * n - 4 * 16
+    assert abs(a - a.T).max() == 0.0
+    assert np.all(np.linalg.eigvalsh(a.toarray()) > 0)
+    r = mld.gen_random(5000, seed=3, median=12.0, sigma=1.0, band=100)
+    off = r - sp.diags(r.diagonal())
+    rowabs = np.asarray(abs(off).sum(axis=1)).ravel()
+    assert np.allclose(r.diagonal(), np.where(rowabs > 0, 1.1 * rowabs, 1.0))
+    lens = np.diff(r.indptr) - 1
+    assert 10 < np.median(lens - 2) < 14  # lognormal median 12 (+2 long)
+    r2 = mld.gen_random(5000, seed=3, median=12.0, sigma=1.0, band=100)
+    assert (r != r2).nnz == 0  # deterministic
+
+
+# ------------------------------------------------------------- sparsify
+def test_sparsify_brute_force(O):
+    rng = np.random.default_rng(0)
+    for trial in range(50):
+        n = int(rng.integers(5, 40))
+        z = rng.standard_normal(n) * (rng.random(n) < 0.6)
+        idiag = int(rng.integers(0, n))
+        nzrmax = int(rng.integers(1, 8))
+        thr = float(rng.choice([0.0, 0.1, 0.5]))
+        iz, vz = O.sparsify(z, idiag, nzrmax, thr)
+        cand = [j for j in range(n) if j != idiag and abs(z[j]) >= thr]
+        cand.sort(key=lambda j: -abs(z[j]))
+        keep = sorted(cand[:nzrmax - 1] + [idiag])
+        assert list(iz) == keep
+        assert np.array_equal(vz, z[keep])
+    # nzrmax = 0 never hits the exit test: every candidate is kept
+    z = np.array([0.5, 0.0, 2.0, -0.1])
+    iz, _ = O.sparsify(z, 1, 0, 0.2)
+    assert list(iz) == [0, 1, 2]
+
+
+# ------------------------------------------------------------- biconjugation
+@pytest.mark.parametrize("alg", ["LLK", "SYM-LLK"])
+def test_biconjg_no_drop_is_exact_inverse(O, alg):
+    a = R.pde3d_numpy(4) if alg == "LLK" else rand_sparse(60, 0.05, 1, sym=True)
+    n = a.shape[0]
+    w, z, d = O.ainv_bld(a, alg, n, 0.0)
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    assert np.abs(M @ a.toarray() - np.eye(n)).max() < 1e-10
+    assert np.allclose(sp.triu(z).toarray(), z.toarray())
+    assert np.all(z.diagonal() == 1.0)  # zval(i) is never updated
+    # invariant of the biconjugation: W^T A Z diagonal, = weight * diag(p)
+    p, zc, wc = O.biconjg(a / abs(a).max(), alg, n, 0.0)
+    G = wc.toarray().T @ (a / abs(a).max()).toarray() @ zc.toarray()
+    assert np.abs(G - np.diag(np.diag(G))).max() < 1e-10
+    assert np.allclose(np.diag(G), p)
+
+
+@pytest.mark.parametrize("seed,nzrmax,thr", [(0, 3, 0.1), (1, 5, 0.05),
+                                             (2, 2, 0.0), (3, 0, 0.2)])
+def test_biconjg_matches_dense_restatement(O, seed, nzrmax, thr):
+    a = rand_sparse(50, 0.06, seed)
+    a = a / abs(a).max()
+    p, zc, wc = O.biconjg(a, "LLK", nzrmax, thr)
+    Z, W, pd = R.dense_llk(a, nzrmax, thr)
+    assert np.array_equal(zc.toarray(), Z)
+    assert np.array_equal(wc.toarray(), W)
+    assert np.array_equal(p, pd)
+    # pattern including stored zeros
+    zc = zc.tocsc()
+    for i in range(a.shape[0]):
+        col = set(zc.indices[zc.indptr[i]:zc.indptr[i + 1]])
+        assert i in col
+
+
+def test_sym_llk_matches_dense_restatement(O):
+    a = rand_sparse(50, 0.06, 9, sym=True)
+    a = a / abs(a).max()
+    p, zc, wc = O.biconjg(a, "SYM-LLK", 4, 0.05)
+    Z, W, pd = R.dense_llk(a, 4, 0.05, sym=True)
+    assert np.array_equal(zc.toarray(), Z)
+    assert np.array_equal(wc.toarray(), Z)
+    assert np.array_equal(p, pd)
+
+
+def test_ainv_bld_scaling_and_pivots(O):
+    """d = weight / p with weight = 1/max|a| (mld_d_ainv_bld.F90:121-173):
+    scaling A by any c > 0 leaves W, Z unchanged and scales d by 1/c."""
+    a = R.pde3d_numpy(4)
+    w1, z1, d1 = O.ainv_bld(a, "LLK", 6, 0.05)
+    w2, z2, d2 = O.ainv_bld(a * 8.0, "LLK", 6, 0.05)
+    assert (w1 != w2).nnz == 0 and (z1 != z2).nnz == 0
+    assert np.allclose(d1, 8.0 * d2, rtol=1e-15)
+
+
+# ------------------------------------------------------------- ILU
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_iluk_full_fill_is_lu(O, seed):
+    a = rand_sparse(40, 0.08, seed)
+    n = a.shape[0]
+    l, u, d = O.iluk(a, n)
+    L = l.toarray() + np.eye(n)
+    U = u.toarray() + np.eye(n)
+    assert np.abs(L @ np.diag(1 / d) @ U - a.toarray()).max() < 1e-12
+    # pivots of textbook Gaussian elimination without pivoting
+    G = a.toarray().copy()
+    for k in range(n):
+        G[k + 1:, k] /= G[k, k]
+        G[k + 1:, k + 1:] -= np.outer(G[k + 1:, k], G[k, k + 1:])
+    assert np.allclose(1 / d, np.diag(G), rtol=1e-12)
+
+
+@pytest.mark.parametrize("fill", [0, 1, 2, 3])
+def test_iluk_pattern_matches_kij_levels(O, fill):
+    a = rand_sparse(45, 0.05, 10 + fill)
+    l, u, d = O.iluk(a, fill)
+    keep = R.dense_iluk_levels(a, fill)
+    n = a.shape[0]
+    pat = np.zeros((n, n), dtype=bool)
+    pat[np.nonzero((l + u).toarray())] = True
+    lu = (l + u).tocoo()
+    pat[:] = False
+    pat[lu.row, lu.col] = True
+    np.fill_diagonal(keep, False)
+    assert np.array_equal(pat, keep)
+
+
+def test_ilut_no_drop_is_lu_and_drops_shrink(O):
+    a = rand_sparse(40, 0.1, 4)
+    n = a.shape[0]
+    l, u, d = O.ilut(a, n, 0.0)
+    L = l.toarray() + np.eye(n)
+    U = u.toarray() + np.eye(n)
+    assert np.abs(L @ np.diag(1 / d) @ U - a.toarray()).max() < 1e-12
+    l2, u2, d2 = O.ilut(a, 1, 0.05)
+    assert l2.nnz + u2.nnz < l.nnz + u.nnz
+    # fill cap: at most (original count) + fill per row and triangle
+    al = sp.tril(a, -1).tocsr()
+    au = sp.triu(a, 1).tocsr()
+    assert np.all(np.diff(l2.indptr) <= np.diff(al.indptr) + 1)
+    assert np.all(np.diff(u2.indptr) <= np.diff(au.indptr) + 1)
+
+
+# ------------------------------------------------------------- INVK / INVT
+@pytest.mark.parametrize("fill", [0, 1, 2])
+def test_invk_matches_dense_scan_and_bfs(O, fill):
+    a = rand_sparse(60, 0.05, 20 + fill)
+    l, u, d = O.iluk(a, 1)
+    z = O.sparse_invk(u, fill)
+    assert R.rows_of(z) == R.dense_row_inverse_invk(u, fill)
+    pat = [sorted(r) for r in R.rows_of(z)]
+    assert pat == R.invk_pattern_bfs(u, fill)
+
+
+def test_invk_full_fill_is_exact_inverse(O):
+    a = R.pde3d_numpy(4)
+    n = a.shape[0]
+    l, u, d = O.iluk(a, 1)
+    z = O.sparse_invk(u, n)
+    U = u.toarray() + np.eye(n)
+    ref = sla.solve_triangular(U, np.eye(n), unit_diagonal=True)
+    assert np.abs(z.toarray() - ref).max() < 1e-12
+    w, z, d = O.invk_bld(a, n, n)
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    assert np.abs(M - np.linalg.inv(a.toarray())).max() < 1e-12 * \
+        np.abs(np.linalg.inv(a.toarray())).max() * 100
+
+
+@pytest.mark.parametrize("fill,thr", [(0, 0.1), (2, 0.05), (5, 0.01),
+                                      (1, 0.0)])
+def test_invt_matches_dense_scan(O, fill, thr):
+    a = rand_sparse(60, 0.06, 30 + fill)
+    l, u, d = O.ilut(a, 3, 1e-3)
+    z = O.sparse_invt(u, fill, thr)
+    assert R.rows_of(z) == R.dense_row_inverse_invt(u, fill, thr)
+
+
+def test_invt_no_drop_is_exact_inverse(O):
+    a = R.pde3d_numpy(4)
+    n = a.shape[0]
+    w, z, d = O.invt_bld(a, n, 0.0, n, 0.0)
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    inv = np.linalg.inv(a.toarray())
+    assert np.abs(M - inv).max() < 1e-10 * np.abs(inv).max()
+
+
+def test_invt_diagonal_is_not_protected(O):
+    """The copy-out keeps the nup+fill largest entries with no exemption for
+    the unit diagonal (invt_copyout.f90:240-248)."""
+    u = sp.csr_matrix(np.array([[0, 3.0, 0], [0, 0, 2.0], [0, 0, 0]]))
+    z = O.sparse_invt(u, 0, 0.0)
+    row0 = R.rows_of(z)[0]
+    # row 0 of U^{-1} = [1, -3, 6]; cap = nup + 0 = 1 keeps only the 6
+    assert row0 == {2: 6.0}
+
+
+# ------------------------------------------------------------- apply
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_apply_closed_form(O, trans):
+    a = R.pde3d_numpy(4)
+    w, z, d = O.invk_bld(a, 1, 1)
+    x = np.random.default_rng(1).standard_normal(a.shape[0])
+    y0 = np.random.default_rng(2).standard_normal(a.shape[0])
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    if trans == "T":
+        M = M.T
+    y = O.apply(w, z, d, x, alpha=-1.5, beta=0.25, y=y0, trans=trans)
+    assert np.allclose(y, -1.5 * M @ x + 0.25 * y0, rtol=1e-13, atol=0)
+    with pytest.raises(O.OracleError):
+        O.apply(w, z, d, x, trans="X")
+
+
+# ------------------------------------------------------------- Krylov
+def _scipy_iters(method, a, M, b, rtol):
+    count = [0]
+
+    def cb(_):
+        count[0] += 1
+    f = spla.bicgstab if method == "BICGSTAB" else spla.cg
+    x, info = f(a, b, rtol=rtol, atol=0.0, M=M, maxiter=2000, callback=cb)
+    assert info == 0
+    return count[0], x
+
+
+@pytest.mark.parametrize("method,kind", [("BICGSTAB", "INVK"),
+                                         ("BICGSTAB", "AINV"),
+                                         ("CG", "SYM")])
+def test_krylov_against_scipy(O, method, kind):
+    if kind == "SYM":
+        a = sp.csr_matrix(R.pde3d_numpy(6))
+        a = ((a + a.T) * 0.5).tocsr()
+        a.sort_indices()
+        w, z, d = O.ainv_bld(a, "SYM-LLK", 8, 0.05)
+    elif kind == "INVK":
+        a = R.pde3d_numpy(7)
+        w, z, d = O.invk_bld(a, 1, 1)
+    else:
+        a = R.pde3d_numpy(7)
+        w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
+    n = a.shape[0]
+    b = a @ np.ones(n)
+    M = spla.LinearOperator((n, n), matvec=lambda v: O.apply(w, z, d, v))
+    it_sc, x_sc = _scipy_iters(method, a, M, b, 1e-8)
+    x, it, err = O.krylov(method, a, w, z, d, b, 1e-8)
+    assert abs(it - it_sc) <= 1, (it, it_sc)
+    assert err <= 1e-8
+    assert np.linalg.norm(b - a @ x) <= 1e-7 * np.linalg.norm(b)
+
+
+# ------------------------------------------------------------- fixtures
+def test_golden_fixtures(O):
+    """Committed vectors (tests/golden/make_golden.py): exact inverses from
+    numpy and iteration counts from scipy at the config-C1 settings on a
+    small grid."""
+    g = np.load(os.path.join(GOLDEN, "golden_small.npz"))
+    a = sp.csr_matrix((g["a_val"], g["a_ja"], g["a_irp"]),
+                      shape=(int(g["n"]), int(g["n"])))
+    n = a.shape[0]
+    w, z, d = O.ainv_bld(a, "LLK", n, 0.0)
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    assert np.abs(M - g["ainv"]).max() < 1e-10 * np.abs(g["ainv"]).max()
+    w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
+    _, it, _ = O.krylov("BICGSTAB", a, w, z, d, np.ones(n), 1e-9)
+    assert abs(it - int(g["scipy_bicgstab_iters_ainv"])) <= 1
+    w, z, d = O.invk_bld(a, 1, 1)
+    _, it, _ = O.krylov("BICGSTAB", a, w, z, d, a @ np.ones(n), 1e-8)
+    assert abs(it - int(g["scipy_bicgstab_iters_invk"])) <= 1
