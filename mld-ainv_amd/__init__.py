@@ -291,7 +291,7 @@ class _BaseAinvSolver:
             return y
         x = np.ascontiguousarray(x, dtype=np.float64)
         k = 1 if x.ndim == 1 else x.shape[1]
-        if x.shape[0] != n:
+        if n and x.shape[0] != n:  # an unbuilt solver reports ERR_STATE
             raise AinvError(ERR_ARG, "x has the wrong number of rows")
         y = np.zeros_like(x) if y is None else np.ascontiguousarray(
             y, dtype=np.float64)

@@ -15,9 +15,13 @@
 //   ILUT:    MLD2P4 mld_ilut_fact with maxval scaling,
 //            src/impl/mld_d_invt_bld.f90:108 (Saad Alg. 10.6)
 //   AINV:    src/impl/mld_d_ainv_bld.F90:115-182,
-//            src/impl/mld_dsparse_biconjg_llk.F90:82-364,
-//            src/impl/mld_dsparse_biconjg_s_llk.F90,
-//            src/impl/mld_d_sparsify.f90:83-143
+//            src/impl/mld_dsparse_biconjg_llk.F90:82-364 (LLK),
+//            src/impl/mld_dsparse_biconjg_s_llk.F90 (SYM-LLK),
+//            src/impl/mld_dsparse_biconjg_s_ft_llk.F90 (STAB-LLK),
+//            src/impl/mld_dsparse_biconjg_llk_noth.F90 (LLK-NOTH),
+//            src/impl/mld_dsparse_biconjg_mlk.F90 (MLK: same visiting set
+//            and arithmetic as LLK through sorted lists),
+//            src/impl/mld_d_sparsify.f90:83-143, :152-230
 // Ties among equal magnitudes at a fill cap are resolved by "smaller column
 // first" (the reference leaves them to PSBLAS heap order).
 #include <algorithm>
@@ -293,14 +297,59 @@ struct PassWork {
   MinHeap pat;               // indices of the nonzeros of zval
   MinHeap rows;              // rows (or columns) still to be dotted
   std::vector<Entry> cand;
+  // sparse accumulator for the STAB-LLK products
+  std::vector<double> acc;
+  std::vector<char> acc_on;
+  std::vector<int32_t> acc_idx;
 };
+
+// Variant switches of the left-looking family.
+struct Variant {
+  bool no_alpha_test = false;  // LLK-NOTH (_llk_noth.F90:117, :234)
+  bool list_sparsify = false;  // MLK: sparsify_list keeps 1 if nzrmax < 1
+  bool stabilized = false;     // STAB-LLK (_s_ft_llk.F90)
+};
+
+// ww = sum_t v_t M(k_t,:) accumulated per output in entry order, touched
+// outputs returned ascending (psb_d_spvspm / psb_d_spmspv,
+// mld_d_biconjg_mod.F90:145-296).
+void combine_rows(const HostCsr &M, const int32_t *ix, const double *vx,
+                  int64_t nx, PassWork &w) {
+  w.acc_idx.clear();
+  for (int64_t t = 0; t < nx; ++t) {
+    const int32_t k = ix[t];
+    for (int64_t e = M.ptr[k]; e < M.ptr[k + 1]; ++e) {
+      const int32_t c = M.idx[e];
+      const double term = vx[t] * M.val[e];
+      if (!w.acc_on[c]) {
+        w.acc_on[c] = 1;
+        w.acc_idx.push_back(c);
+        w.acc[c] = term;
+      } else {
+        w.acc[c] = term + w.acc[c];
+      }
+    }
+  }
+  std::sort(w.acc_idx.begin(), w.acc_idx.end());
+}
+
+void clear_acc(PassWork &w) {
+  for (int32_t c : w.acc_idx) {
+    w.acc[c] = 0.0;
+    w.acc_on[c] = 0;
+  }
+  w.acc_idx.clear();
+}
 
 // One column i of Z (dot = rows of A, seed/enqueue = columns of A) or of W
 // (the same with A^T).  dotm: matrix whose row j is dotted with zval;
 // seedm: matrix whose row i / row kr lists the candidates to queue.
+// With v.stabilized the dot of step j is (g_j^T M) . zval, g_j being
+// column j of the other factor `g` and M = `comb`, and no pivot is formed.
 void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
                  const ColStore &f, std::vector<double> &piv, int32_t nzrmax,
-                 double thresh, PassWork &w, ColStore &out) {
+                 double thresh, const Variant &v, const ColStore *g,
+                 const HostCsr *comb, PassWork &w, ColStore &out) {
   w.zval[i] = 1.0;
   w.in_z[i] = 1;
   w.pat.push(i);
@@ -320,10 +369,18 @@ void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
     w.queued[j] = 0;
     if (j >= i) continue;
     double s = 0.0;
-    for (int64_t e = dotm.ptr[j]; e < dotm.ptr[j + 1]; ++e)
-      s = s + dotm.val[e] * w.zval[dotm.idx[e]];
+    if (v.stabilized) {
+      const int64_t b = g->ptr[j];
+      combine_rows(*comb, g->idx.data() + b, g->val.data() + b,
+                   g->ptr[j + 1] - b, w);
+      for (int32_t c : w.acc_idx) s = s + w.acc[c] * w.zval[c];
+      clear_acc(w);
+    } else {
+      for (int64_t e = dotm.ptr[j]; e < dotm.ptr[j + 1]; ++e)
+        s = s + dotm.val[e] * w.zval[dotm.idx[e]];
+    }
     const double alpha = (-s / piv[j]);
-    if (!(std::fabs(alpha) > thresh)) continue;
+    if (!v.no_alpha_test && !(std::fabs(alpha) > thresh)) continue;
     for (int64_t e = f.ptr[j]; e < f.ptr[j + 1]; ++e) {
       const int32_t kr = f.idx[e];
       w.zval[kr] = w.zval[kr] + alpha * f.val[e];
@@ -339,12 +396,13 @@ void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
       }
     }
   }
-  {
+  if (!v.stabilized) {
     double s = 0.0;
     for (int64_t e = dotm.ptr[i]; e < dotm.ptr[i + 1]; ++e)
       s = s + dotm.val[e] * w.zval[dotm.idx[e]];
     piv[i] = (std::fabs(s) < kEpsTol) ? 1.0e-3 : s;
   }
+  if (v.list_sparsify && nzrmax < 1) nzrmax = 1;  // sparsify.f90:214-222
   // sparsify: diagonal always, then the nzrmax-1 largest with |z| >= thresh
   w.cand.clear();
   double diag = 0.0;
@@ -382,10 +440,13 @@ void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
 
 void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
                HostCsr &w, HostCsr &z, std::vector<double> &d) {
-  if (alg != AINV_ALG_LLK && alg != AINV_ALG_S_LLK)
-    throw Error(AINV_ERR_ALG,
-                "AINV variant " + std::to_string(alg) +
-                    " is not available in this build (LLK, SYM-LLK are)");
+  // the dispatcher's select (mld_d_biconjg_mod.F90:107-122)
+  if (alg < AINV_ALG_LLK || alg > AINV_ALG_MLK)
+    throw Error(AINV_ERR_ALG, "Invalid alg " + std::to_string(alg));
+  Variant var;
+  var.no_alpha_test = (alg == AINV_ALG_LLK_NOTH);
+  var.list_sparsify = (alg == AINV_ALG_MLK);
+  var.stabilized = (alg == AINV_ALG_S_FT_LLK);
   const int32_t n = a.n;
   // maxval scaling (mld_d_ainv_bld.F90:121-124)
   const double weight = 1.0 / max_abs(a);
@@ -399,6 +460,10 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
   pw.zval.assign(n, 0.0);
   pw.in_z.assign(n, 0);
   pw.queued.assign(n, 0);
+  if (var.stabilized) {
+    pw.acc.assign(n, 0.0);
+    pw.acc_on.assign(n, 0);
+  }
   if (n > 0) {
     for (int64_t e = as.ptr[0]; e < as.ptr[1]; ++e)
       if (as.idx[e] == 0) {
@@ -415,9 +480,29 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
     wc.ptr.push_back(1);
   }
   for (int32_t i = 1; i < n; ++i) {
-    bicg_column(i, as, at, zc, p, fill_in, thresh, pw, zc);
-    if (alg == AINV_ALG_LLK)
-      bicg_column(i, at, as, wc, q, fill_in, thresh, pw, wc);
+    // Z: dots with rows of A (STAB: w_j^T A), seeds from column i of A
+    bicg_column(i, as, at, zc, p, fill_in, thresh, var, &wc, &as, pw, zc);
+    if (alg == AINV_ALG_S_LLK) continue;
+    // W: dots with columns of A (STAB: A z_j), seeds from row i of A
+    bicg_column(i, at, as, wc, q, fill_in, thresh, var, &zc, &at, pw, wc);
+    if (var.stabilized) {
+      // q(i) = w_i^T (A z_i), merge dot over sorted indices
+      // (mld_dsparse_biconjg_s_ft_llk.F90:377-411)
+      const int64_t zb = zc.ptr[i];
+      combine_rows(at, zc.idx.data() + zb, zc.val.data() + zb,
+                   zc.ptr[i + 1] - zb, pw);
+      double t = 0.0;
+      size_t u = 0;
+      for (int64_t e = wc.ptr[i]; e < wc.ptr[i + 1]; ++e) {
+        const int32_t c = wc.idx[e];
+        while (u < pw.acc_idx.size() && pw.acc_idx[u] < c) ++u;
+        if (u < pw.acc_idx.size() && pw.acc_idx[u] == c)
+          t = t + pw.acc[c] * wc.val[e];
+      }
+      clear_acc(pw);
+      q[i] = (std::fabs(t) < kEpsTol) ? 1.0e-3 : t;
+      p[i] = q[i];
+    }
   }
   if (alg == AINV_ALG_S_LLK) wc = zc;  // W := Z (s_llk.F90:246)
   // Z CSC -> CSR; W CSC holds W^T as CSR (biconjg_mod.F90:131-135)

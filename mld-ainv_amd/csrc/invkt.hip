@@ -356,13 +356,18 @@ __global__ __launch_bounds__(kWave) void k_inverse_global(
 
 // last row of the inverse is e_n (invk.f90:120-124, invt.f90:117-121)
 __global__ void k_last_row(int32_t n, int32_t *__restrict__ out_idx,
-                           double *__restrict__ out_val,
+                           double *__restrict__ out_val, int64_t pool_cap,
                            unsigned long long *__restrict__ pool_top,
                            int64_t *__restrict__ row_start,
                            int32_t *__restrict__ row_cnt,
                            int32_t *__restrict__ status) {
   if (threadIdx.x != 0 || blockIdx.x != 0 || n < 1) return;
   unsigned long long b = atomicAdd(pool_top, 1ull);
+  if ((int64_t)b >= pool_cap) {
+    status[n - 1] = ST_POOL;
+    row_cnt[n - 1] = 0;
+    return;
+  }
   out_idx[b] = n - 1;
   out_val[b] = 1.0;
   row_start[n - 1] = (int64_t)b;
@@ -438,7 +443,7 @@ void gpu_sparse_inverse(int kind, const DevCsr &u, int32_t fill, double thresh,
       AINV_HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_last_row, dim3(1), dim3(64), 0, s, n,
-                       pidx.as<int32_t>(), pval.as<double>(),
+                       pidx.as<int32_t>(), pval.as<double>(), pool_cap,
                        top.as<unsigned long long>(), row_start.as<int64_t>(),
                        row_cnt.as<int32_t>(), status.as<int32_t>());
     AINV_HIP_CHECK(hipGetLastError());

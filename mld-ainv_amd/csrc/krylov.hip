@@ -347,20 +347,16 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   // r = b - A x
   AINV_HIP_CHECK(hipMemcpyAsync(v.r, b, vb, hipMemcpyDeviceToDevice, s));
   hll_spmm_axpby(a->hll, -1.0, x, false, k, 1.0, v.r, k, k, s);
+  // column state starts all-active so the init pass runs every column
+  std::vector<Col> init(kMaxCols);
+  for (auto &c0 : init) c0 = Col{0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
+  AINV_HIP_CHECK(hipMemcpyAsync(col, init.data(), sizeof(Col) * kMaxCols,
+                                hipMemcpyHostToDevice, s));
   if (!cg) {
     AINV_HIP_CHECK(hipMemcpyAsync(v.q, v.r, vb, hipMemcpyDeviceToDevice, s));
-    // column state starts all-active so the init pass runs every column
-    std::vector<Col> init(kMaxCols);
-    for (auto &c0 : init) c0 = Col{0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
-    AINV_HIP_CHECK(hipMemcpyAsync(col, init.data(), sizeof(Col) * kMaxCols,
-                                  hipMemcpyHostToDevice, s));
     vec_op<OP_INIT_BICG>(n, k, v, col, 0, pt, s);
     fin(FIN_INIT, n, k, 0, eps, pt, col, na, s);
   } else {
-    std::vector<Col> init(kMaxCols);
-    for (auto &c0 : init) c0 = Col{0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
-    AINV_HIP_CHECK(hipMemcpyAsync(col, init.data(), sizeof(Col) * kMaxCols,
-                                  hipMemcpyHostToDevice, s));
     vec_op<OP_INIT_CG>(n, k, v, col, 0, pt, s);
     fin(FIN_INIT_CG, n, k, 0, eps, pt, col, na, s);
   }

@@ -1,0 +1,108 @@
+! Sample driver in the style of the reference's tests/pdegen/ppde3d.f90:
+! generate the 3-D convection-diffusion matrix, build an AINV / INVK / INVT
+! preconditioner through the Fortran solver objects, solve with BiCGSTAB
+! or CG on the GPU and print the same quantities (build time, solve time,
+! iterations, error estimate, preconditioner nonzeros).
+!
+! stdin deck, one value per line (text after the value is ignored):
+!   method (BICGSTAB|CG), preconditioner (AINV|INVK|INVT), grid size N,
+!   eps, itmax, sub_fillin, inv_fillin, sub_iluthrs, inv_thresh,
+!   ainv_alg, factor storage (F64|F32|BF16)
+program ppde3d_hip
+  use iso_c_binding
+  use mld_ainv_hip_cbind, only: ainv_gen_pde3d, last_error
+  use mld_d_ainv_hip_mod
+  implicit none
+  character(len=20) :: method, ptype, alg, fprec
+  integer :: idim, itmax, fill1, fill2, info, iter, i
+  real(c_double) :: eps, thr1, thr2, err
+  integer(c_int32_t) :: n
+  integer(c_int64_t) :: nnz
+  integer(c_int64_t), allocatable, target :: irp(:)
+  integer(c_int32_t), allocatable, target :: ja(:)
+  real(c_double), allocatable, target :: val(:)
+  real(c_double), allocatable :: b(:), x(:), ones(:)
+  class(mld_d_base_ainv_solver_type), allocatable :: prec
+  integer(8) :: t0, t1, t2, rate
+
+  read(*, *) method
+  read(*, *) ptype
+  read(*, *) idim
+  read(*, *) eps
+  read(*, *) itmax
+  read(*, *) fill1
+  read(*, *) fill2
+  read(*, *) thr1
+  read(*, *) thr2
+  read(*, *) alg
+  read(*, *) fprec
+
+  call mld_ainv_init(0, info)
+  if (info /= 0) call fail('init')
+  info = ainv_gen_pde3d(int(idim, c_int32_t), 0_c_int, n, nnz, c_null_ptr, &
+       & c_null_ptr, c_null_ptr)
+  if (info /= 0) call fail('generate')
+  allocate(irp(n + 1), ja(nnz), val(nnz))
+  info = ainv_gen_pde3d(int(idim, c_int32_t), 0_c_int, n, nnz, c_loc(irp), &
+       & c_loc(ja), c_loc(val))
+  if (info /= 0) call fail('generate')
+  ! the library generates 0-based CSR; the Fortran side works 1-based
+  irp = irp + 1
+  ja = ja + 1
+
+  select case (trim(ptype))
+  case ('INVK')
+    allocate(mld_d_invk_solver_type :: prec)
+  case ('INVT')
+    allocate(mld_d_invt_solver_type :: prec)
+  case default
+    allocate(mld_d_ainv_solver_type :: prec)
+  end select
+  call prec%set('SUB_FILLIN', fill1, info)
+  call prec%set('INV_FILLIN', fill2, info)
+  call prec%set('SUB_ILUTHRS', thr1, info)
+  call prec%set('INV_THRESH', thr2, info)
+  if (trim(ptype) == 'AINV') call prec%set('AINV_ALG', trim(alg), info)
+  call prec%set('FACTOR_PREC', trim(fprec), info)
+  call prec%check(info)
+
+  call system_clock(t0, rate)
+  call prec%build(int(n), irp, ja, val, info)
+  if (info /= 0) call fail('build')
+  call system_clock(t1)
+
+  ! b = A * 1 (the df_sample.f90 convention when no rhs is given)
+  allocate(ones(n), b(n), x(n))
+  ones = 1.0d0
+  b = 0.0d0
+  do i = 1, n
+    b(i) = sum(val(irp(i):irp(i + 1) - 1) * ones(ja(irp(i):irp(i + 1) - 1)))
+  end do
+  x = 0.0d0
+  call mld_krylov(trim(method), int(n), irp, ja, val, prec, b, x, eps, &
+       & itmax, iter, err, info)
+  if (info /= 0) call fail('krylov')
+  call system_clock(t2)
+
+  call prec%descr(info)
+  write(*, '(a,i0)') 'Matrix order                 : ', n
+  write(*, '(a,es12.5)') 'Preconditioner time (s)      : ', &
+       & real(t1 - t0, 8) / real(rate, 8)
+  write(*, '(a,es12.5)') 'Time to solve system (s)     : ', &
+       & real(t2 - t1, 8) / real(rate, 8)
+  write(*, '(a,i0)') 'Number of iterations         : ', iter
+  write(*, '(a,es12.5)') 'Convergence indicator on exit: ', err
+  write(*, '(a,i0)') 'Total nonzeros for PREC      : ', prec%get_nzeros()
+  write(*, '(a,es12.5)') 'Max error |x - 1|            : ', maxval(abs(x - 1.0d0))
+  call prec%free(info)
+
+contains
+
+  subroutine fail(what)
+    character(len=*), intent(in) :: what
+    write(*, '(a,a,a,i0,a,a)') 'ERROR in ', what, ': info=', info, ' ', &
+         & trim(last_error())
+    stop 1
+  end subroutine
+
+end program ppde3d_hip

@@ -320,7 +320,49 @@ int32_t or_sparsify(int32_t idiag, int32_t nzrmax, double thresh, int32_t n,
 typedef struct {
   const or_mat *rows; /* dot products: row j of this matrix */
   const or_mat *cols; /* seeding/enqueue: row i of this matrix */
+  int noth;           /* LLK-NOTH: no |alpha| > thresh test (_noth.F90:117) */
+  int list_sparsify;  /* MLK: sparsify_list rule for nzrmax < 1 */
+  /* STAB-LLK: the dot of step j is (other_j^T A) . zval, other_j being
+   * column j of the other factor, and no pivot is formed here */
+  const vlist *other;
+  const int64_t *optr;
+  const or_mat *comb; /* matrix whose rows are combined: A for the Z pass
+                         (w_j^T A), A^T for the W pass (A z_j) */
+  double *acc;        /* dense accumulator, zero outside a call */
+  int32_t *accflag;
+  int32_t *acclist;
 } pass_mats;
+
+static int cmp_i32(const void *a, const void *b) {
+  int32_t x = *(const int32_t *)a, y = *(const int32_t *)b;
+  return (x > y) - (x < y);
+}
+
+/* ww = sum_t v_t * M(k_t, :) over the entries (k_t, v_t) in stored order,
+ * each output accumulated in that order ((0 + c1) + c2 ...), the sparse
+ * sparse products psb_d_spvspm / psb_d_spmspv of mld_d_biconjg_mod.F90:
+ * 145-296; returns the touched indices ascending (the merge order) */
+static int64_t sparse_combine(const or_mat *M, int64_t nx, const int32_t *ix,
+                              const double *vx, double *acc, int32_t *flag,
+                              int32_t *list) {
+  int64_t m = 0;
+  for (int64_t t = 0; t < nx; t++) {
+    int32_t k = ix[t];
+    for (int64_t e = M->ptr[k]; e < M->ptr[k + 1]; e++) {
+      int32_t c = M->idx[e];
+      double term = vx[t] * M->val[e];
+      if (!flag[c]) {
+        flag[c] = 1;
+        list[m++] = c;
+        acc[c] = term;
+      } else {
+        acc[c] = term + acc[c];
+      }
+    }
+  }
+  qsort(list, (size_t)m, sizeof(int32_t), cmp_i32);
+  return m;
+}
 
 static int biconjg_pass(int32_t i, const pass_mats *pm, const vlist *f,
                         const int64_t *fptr, double *pv, int32_t nzrmax,
@@ -356,11 +398,27 @@ static int biconjg_pass(int32_t i, const pass_mats *pm, const vlist *f,
     if (!got) break;
     izcr[j] = 0;
     if (j >= i) continue;
-    /* dot with row j (trailing columns beyond n clipped, :162-171) */
-    int64_t ip1 = R->ptr[j], ip2 = R->ptr[j + 1];
-    double s = spge_dot(ip2 - ip1, R->idx + ip1, R->val + ip1, zval);
+    double s;
+    if (pm->other) {
+      /* STAB-LLK (s_ft_llk.F90:160-166 / :286-296) */
+      int64_t o1 = pm->optr[j], o2 = pm->optr[j + 1];
+      int64_t m = sparse_combine(pm->comb, o2 - o1, pm->other->idx + o1,
+                                 pm->other->val + o1, pm->acc, pm->accflag,
+                                 pm->acclist);
+      s = 0.0;
+      for (int64_t t = 0; t < m; t++) {
+        int32_t c = pm->acclist[t];
+        s = s + pm->acc[c] * zval[c];
+        pm->acc[c] = 0.0;
+        pm->accflag[c] = 0;
+      }
+    } else {
+      /* dot with row j (trailing columns beyond n clipped, :162-171) */
+      int64_t ip1 = R->ptr[j], ip2 = R->ptr[j + 1];
+      s = spge_dot(ip2 - ip1, R->idx + ip1, R->val + ip1, zval);
+    }
     double alpha = (-s / pv[j]);
-    if (fabs(alpha) > thresh) {
+    if (pm->noth || fabs(alpha) > thresh) {
       for (int64_t k = fptr[j]; k < fptr[j + 1]; k++) {
         int32_t kr = f->idx[k];
         zval[kr] = zval[kr] + alpha * f->val[k];
@@ -379,18 +437,22 @@ static int biconjg_pass(int32_t i, const pass_mats *pm, const vlist *f,
       }
     }
   }
-  /* pivot: row i of A (Z pass, :214-218) / column i (W pass, :333-340) */
-  {
+  /* pivot: row i of A (Z pass, :214-218) / column i (W pass, :333-340);
+   * STAB-LLK forms it after both columns (s_ft_llk.F90:377-411) */
+  if (!pm->other) {
     int64_t ip1 = R->ptr[i], ip2 = R->ptr[i + 1];
     pv[i] = spge_dot(ip2 - ip1, R->idx + ip1, R->val + ip1, zval);
     if (fabs(pv[i]) < D_EPSTOL) pv[i] = 1.0e-3;
   }
-  return sparsify_heap(i, nzrmax, thresh, zval, heap, izkr, cand, iz, vz, nzz);
+  /* MLK's sparsify_list keeps max(1, min(count, nzrmax)) entries
+   * (mld_d_sparsify.f90:214-222): only the diagonal when nzrmax < 1 */
+  int32_t nzr = (pm->list_sparsify && nzrmax < 1) ? 1 : nzrmax;
+  return sparsify_heap(i, nzr, thresh, zval, heap, izkr, cand, iz, vz, nzz);
 }
 
 int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
                double *p, or_mat *zcsc, or_mat *wcsc) {
-  if (alg != OR_ALG_LLK && alg != OR_ALG_S_LLK) return OR_ERR_ALG;
+  if (alg < OR_ALG_LLK || alg > OR_ALG_MLK) return OR_ERR_ALG;
   int32_t n = a->n;
   int rc = OR_OK;
   or_mat ac = {0};
@@ -406,7 +468,11 @@ int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
   int64_t *wp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   vlist zl = {0}, wl = {0};
   iheap heap = {0}, rheap = {0};
-  if (!zval || !q || !izkr || !izcr || !iz || !vz || !cand || !zp || !wp) {
+  double *acc = (double *)calloc((size_t)n + 1, sizeof(double));
+  int32_t *accflag = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  int32_t *acclist = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  if (!zval || !q || !izkr || !izcr || !iz || !vz || !cand || !zp || !wp ||
+      !acc || !accflag || !acclist) {
     rc = OR_ERR_ALLOC;
     goto done;
   }
@@ -423,8 +489,13 @@ int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
   zp[1] = 1;
   vl_push(&wl, 0, 1.0);
   wp[1] = 1;
-  pass_mats zpass = {a, &ac};  /* Z: dots with rows of A, seed column i */
-  pass_mats wpass = {&ac, a};  /* W: dots with columns of A, seed row i */
+  const int stab = (alg == OR_ALG_S_FT_LLK);
+  /* Z: dots with rows of A, seed column i; W: dots with columns of A,
+   * seed row i */
+  pass_mats zpass = {a, &ac, alg == OR_ALG_LLK_NOTH, alg == OR_ALG_MLK,
+                     stab ? &wl : NULL, wp, a, acc, accflag, acclist};
+  pass_mats wpass = {&ac, a, alg == OR_ALG_LLK_NOTH, alg == OR_ALG_MLK,
+                     stab ? &zl : NULL, zp, &ac, acc, accflag, acclist};
   for (int32_t i = 1; i < n; i++) {
     int32_t nz;
     rc = biconjg_pass(i, &zpass, &zl, zp, p, nzrmax, thresh, zval, izkr, izcr,
@@ -433,13 +504,32 @@ int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
     for (int32_t k = 0; k < nz; k++)
       if ((rc = vl_push(&zl, iz[k], vz[k]))) goto done;
     zp[i + 1] = zl.len;
-    if (alg == OR_ALG_LLK) {
+    if (alg != OR_ALG_S_LLK) {
       rc = biconjg_pass(i, &wpass, &wl, wp, q, nzrmax, thresh, zval, izkr,
                         izcr, &heap, &rheap, cand, iz, vz, &nz);
       if (rc) goto done;
       for (int32_t k = 0; k < nz; k++)
         if ((rc = vl_push(&wl, iz[k], vz[k]))) goto done;
       wp[i + 1] = wl.len;
+    }
+    if (stab) {
+      /* q(i) = w_i^T (A z_i), sorted sparse dot (s_ft_llk.F90:377-411) */
+      int64_t z1 = zp[i], z2 = zp[i + 1];
+      int64_t m = sparse_combine(&ac, z2 - z1, zl.idx + z1, zl.val + z1, acc,
+                                 accflag, acclist);
+      double t = 0.0;
+      int64_t w1 = wp[i], w2 = wp[i + 1], u = 0;
+      for (int64_t e = w1; e < w2; e++) {
+        int32_t c = wl.idx[e];
+        while (u < m && acclist[u] < c) u++;
+        if (u < m && acclist[u] == c) t = t + acc[c] * wl.val[e];
+      }
+      for (int64_t v = 0; v < m; v++) {
+        acc[acclist[v]] = 0.0;
+        accflag[acclist[v]] = 0;
+      }
+      q[i] = (fabs(t) < D_EPSTOL) ? 1.0e-3 : t;
+      p[i] = q[i];
     }
   }
   vl_to_mat(&zl, n, zp, zcsc);
@@ -475,6 +565,9 @@ done:
   free(wl.val);
   free(heap.a);
   free(rheap.a);
+  free(acc);
+  free(accflag);
+  free(acclist);
   return rc;
 }
 

@@ -30,6 +30,9 @@ extern "C" {
 /* algorithm ids for biconjugation; names from mld_d_ainv_solver.F90:240-257 */
 #define OR_ALG_LLK 0
 #define OR_ALG_S_LLK 1
+#define OR_ALG_S_FT_LLK 2 /* 'STAB-LLK' */
+#define OR_ALG_LLK_NOTH 3
+#define OR_ALG_MLK 4
 
 /* A sparse matrix handed out by the oracle (CSR unless stated otherwise). */
 typedef struct {
@@ -50,8 +53,9 @@ void or_mat_free(or_mat *m);
 int32_t or_sparsify(int32_t idiag, int32_t nzrmax, double thresh, int32_t n,
                     double *zw, int32_t istart, int32_t *iz, double *valz);
 
-/* Sparse biconjugation (mld_dsparse_biconjg_llk.F90 / _s_llk.F90).  Input A
- * CSR (already scaled).  Output Z (CSC, n x n, unit upper), W (CSC), p. */
+/* Sparse biconjugation (mld_dsparse_biconjg_llk.F90, _s_llk.F90,
+ * _s_ft_llk.F90, _llk_noth.F90, _mlk.F90).  Input A CSR (already scaled).
+ * Output Z (CSC, n x n, unit upper), W (CSC), p. */
 int or_biconjg(int alg, const or_mat *a, int32_t nzrmax, double thresh,
                double *p, or_mat *zcsc, or_mat *wcsc);
 

@@ -17,7 +17,7 @@ import scipy.sparse as sp
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
-ALG = {"LLK": 0, "SYM-LLK": 1}
+ALG = {"LLK": 0, "SYM-LLK": 1, "STAB-LLK": 2, "LLK-NOTH": 3, "MLK": 4}
 
 
 class OrMat(C.Structure):

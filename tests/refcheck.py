@@ -41,11 +41,13 @@ def key(v, j):
     return (-abs(v), j)
 
 
-def dense_llk(a, nzrmax, thresh, sym=False):
+def dense_llk(a, nzrmax, thresh, sym=False, noth=False):
     """Left-looking biconjugation with dense work vectors, visiting every
     earlier column j (the sparse code visits only those whose row overlaps
     the current pattern; for the others the dot product is exactly 0 and
-    |alpha| = 0 is never > thresh).  Input a already scaled.  Returns dense
+    |alpha| = 0 is never > thresh; with noth (LLK-NOTH, no alpha test) the
+    zero update only touches entries that sparsify then drops, so thresh
+    must be > 0 for the two to agree).  Input a already scaled.  Returns dense
     Z, W (columns) and p."""
     a = sp.csr_matrix(a)
     a.sort_indices()
@@ -76,7 +78,7 @@ def dense_llk(a, nzrmax, thresh, sym=False):
         for j in range(i):
             s = rowdot(dotm, j, z)
             alpha = -s / piv[j]
-            if abs(alpha) > thresh:
+            if noth or abs(alpha) > thresh:
                 for k, v in sorted(cols[j].items()):
                     z[k] = z[k] + alpha * v
                     touched.add(k)

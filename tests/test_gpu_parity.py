@@ -73,8 +73,9 @@ def test_invk_global_tier_exact_inverse(gpu, O):
     check_factors(sv, O.invk_bld(a, 0, n))
 
 
-@pytest.mark.parametrize("alg,thr,fill", [("LLK", 0.1, 8), ("LLK", 0.0, 4),
-                                          ("LLK", 0.05, 0)])
+@pytest.mark.parametrize("alg,thr,fill", [
+    ("LLK", 0.1, 8), ("LLK", 0.0, 4), ("LLK", 0.05, 0), ("STAB-LLK", 0.1, 8),
+    ("LLK-NOTH", 0.1, 6), ("MLK", 0.1, 8)])
 def test_ainv_llk_bit_exact(gpu, O, alg, thr, fill):
     a = gpu.gen_pde3d(8)
     sv = build(gpu, "AINV", a, AINV_ALG=alg, SUB_ILUTHRS=thr, SUB_FILLIN=fill)

@@ -86,9 +86,11 @@ def test_sparsify_brute_force(O):
 
 
 # ------------------------------------------------------------- biconjugation
-@pytest.mark.parametrize("alg", ["LLK", "SYM-LLK"])
+@pytest.mark.parametrize("alg", ["LLK", "SYM-LLK", "STAB-LLK", "LLK-NOTH",
+                                 "MLK"])
 def test_biconjg_no_drop_is_exact_inverse(O, alg):
-    a = R.pde3d_numpy(4) if alg == "LLK" else rand_sparse(60, 0.05, 1, sym=True)
+    a = rand_sparse(60, 0.05, 1, sym=True) if alg == "SYM-LLK" else \
+        R.pde3d_numpy(4)
     n = a.shape[0]
     w, z, d = O.ainv_bld(a, alg, n, 0.0)
     M = z.toarray() @ np.diag(d) @ w.toarray()
@@ -117,6 +119,46 @@ def test_biconjg_matches_dense_restatement(O, seed, nzrmax, thr):
     for i in range(a.shape[0]):
         col = set(zc.indices[zc.indptr[i]:zc.indptr[i + 1]])
         assert i in col
+
+
+@pytest.mark.parametrize("seed,nzrmax,thr", [(5, 3, 0.1), (6, 6, 0.02)])
+def test_llk_noth_matches_dense_restatement(O, seed, nzrmax, thr):
+    a = rand_sparse(50, 0.06, seed)
+    a = a / abs(a).max()
+    p, zc, wc = O.biconjg(a, "LLK-NOTH", nzrmax, thr)
+    Z, W, pd = R.dense_llk(a, nzrmax, thr, noth=True)
+    assert np.array_equal(zc.toarray(), Z)
+    assert np.array_equal(wc.toarray(), W)
+    assert np.array_equal(p, pd)
+
+
+def test_mlk_is_llk_except_for_nzrmax_zero(O):
+    """MLK visits the same rows in the same order with sorted lists instead
+    of heaps (mld_dsparse_biconjg_mlk.F90:116-440), so it reproduces LLK;
+    its sparsify_list keeps max(1, min(count, nzrmax)) entries
+    (mld_d_sparsify.f90:214-222), only the diagonal when nzrmax = 0."""
+    a = rand_sparse(60, 0.06, 8)
+    a = a / abs(a).max()
+    p1, z1, w1 = O.biconjg(a, "LLK", 4, 0.05)
+    p2, z2, w2 = O.biconjg(a, "MLK", 4, 0.05)
+    assert (z1 != z2).nnz == 0 and (w1 != w2).nnz == 0
+    assert np.array_equal(p1, p2)
+    _, z0, w0 = O.biconjg(a, "MLK", 0, 0.05)
+    assert z0.nnz == a.shape[0] and w0.nnz == a.shape[0]
+    assert np.array_equal(z0.toarray(), np.eye(a.shape[0]))
+
+
+def test_stab_llk_pivots_are_w_a_z(O):
+    """STAB-LLK forms p_i = q_i = w_i^T A z_i after both columns
+    (mld_dsparse_biconjg_s_ft_llk.F90:377-411): with dropping, the pivots
+    equal the diagonal of W^T A Z computed densely."""
+    a = rand_sparse(50, 0.06, 12)
+    a = a / abs(a).max()
+    p, zc, wc = O.biconjg(a, "STAB-LLK", 5, 0.05)
+    G = wc.toarray().T @ a.toarray() @ zc.toarray()
+    d = np.diag(G)
+    assert np.allclose(p[1:], np.where(np.abs(d[1:]) < 2.2e-16, 1e-3, d[1:]),
+                       rtol=1e-13, atol=0)
 
 
 def test_sym_llk_matches_dense_restatement(O):
