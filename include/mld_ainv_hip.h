@@ -60,6 +60,8 @@ extern "C" {
 
 /* ---- flags --------------------------------------------------------- */
 #define AINV_HOST_PTRS 1 /* x/y (or b/x) are host pointers */
+#define AINV_KRYLOV_EAGER 2 /* Krylov: launch kernels one by one instead of
+                               replaying the captured iteration graph */
 
 typedef struct ainv_solver *ainv_h;
 typedef struct ainv_matrix *ainv_mat_h;
@@ -221,6 +223,12 @@ int ainv_krylov_profile(const char *method, ainv_mat_h a, ainv_h prec,
                         const double *b, double *x, int k, double eps,
                         int itmax, int reps, int32_t *iter, double *err,
                         double *ms);
+/* STREAM-style copy of `bytes` (multiple of 16) from src to dst, `reps`
+ * times; *ms = average duration of one copy kernel (HIP events).  The
+ * achievable-bandwidth reference and the calibration kernel for hardware
+ * traffic counters. */
+int ainv_stream_copy(void *dst, const void *src, size_t bytes, int reps,
+                     double *ms);
 /* Layout / traffic facts of the resident factors: n, nnz(W), nnz(Z),
  * padded entries of each HLL, value bytes. */
 int ainv_layout(ainv_h h, int64_t *n, int64_t *nnz_w, int64_t *nnz_z,

@@ -100,6 +100,7 @@ def lib():
         "ainv_krylov_profile": [C.c_char_p, vp, vp, vp, vp, C.c_int,
                                 C.c_double, C.c_int, C.c_int, i32p, dp, dp],
         "ainv_layout": [vp, i64p, i64p, i64p, i64p, i64p, P(C.c_int)],
+        "ainv_stream_copy": [vp, vp, C.c_size_t, C.c_int, dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -190,6 +191,17 @@ class DeviceArray:
 
 def synchronize():
     _ck(lib().ainv_device_sync())
+
+
+def stream_copy_gbs(nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth of a plain copy kernel (read + write bytes
+    over the kernel time), GB/s, and the per-copy time in ms."""
+    src = DeviceArray((nbytes // 8,))
+    dst = DeviceArray((nbytes // 8,))
+    src.zero()
+    ms = C.c_double()
+    _ck(lib().ainv_stream_copy(dst.ptr, src.ptr, nbytes, reps, C.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
 
 # ------------------------------------------------------------------- CSR in
@@ -456,10 +468,16 @@ class SystemMatrix:
             pass
 
 
-def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2):
+KRYLOV_EAGER = 2
+
+
+def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2,
+           eager=False):
     """psb_krylov(method, a, prec, b, x, eps, ...) for k interleaved
     right-hand sides.  a: SystemMatrix; prec: solver object or None;
-    b, x: numpy (host) or DeviceArray.  Returns (x, iters[k], errs[k])."""
+    b, x: numpy (host) or DeviceArray.  eager=True launches the kernels of
+    each iteration one by one instead of replaying the captured graph.
+    Returns (x, iters[k], errs[k])."""
     host = not isinstance(b, DeviceArray)
     if host:
         b = np.ascontiguousarray(b, dtype=np.float64)
@@ -474,6 +492,8 @@ def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2):
             x = DeviceArray(b.shape)
             x.zero()
         bp, xp, flags = b.ptr, x.ptr, 0
+    if eager:
+        flags |= KRYLOV_EAGER
     iters = np.zeros(k, dtype=np.int32)
     errs = np.zeros(k, dtype=np.float64)
     _ck(lib().ainv_krylov(_b(method), a._h, prec._h if prec else None, bp, xp,

@@ -103,6 +103,8 @@ void csr_download(const DevCsr &d, HostCsr &h, hipStream_t s);
 void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s);
 // HLL from device CSR, values converted to `prec`.
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s);
+// Plain 16-byte-vector copy kernel (bandwidth reference).
+void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Convert n doubles to the storage type of prec (d vector).
 void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
                     hipStream_t s);
@@ -116,6 +118,15 @@ void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
 void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, double *y, int64_t ldy, int k,
                     hipStream_t s);
+// Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *
+// kMaxCols + c] for pair q, block b, column c.
+constexpr int kRedBlocks = 1024;
+constexpr int kMaxCols = 64;
+// y = A x (FP64 HLL, k interleaved columns, ld = k) fused with the dot
+// products of its result: np = 1 gives (w, y), np = 2 gives (y, y) and
+// (y, w), per column, as block partials.  Returns the number of blocks.
+int hll_spmv_dot(const Hll &a, const double *x, double *y, int k,
+                 const double *w, int np, double *part, hipStream_t s);
 
 // ---- GPU row inverses (invkt.hip) ----------------------------------------
 // Row-wise approximate inverse of a unit upper triangular matrix given by

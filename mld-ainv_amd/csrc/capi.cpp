@@ -3,6 +3,7 @@
 // ...), the system-matrix object and the Krylov driver.  Declarations and
 // the reference interface each call replaces: include/mld_ainv_hip.h.
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cstdio>
 #include <cstring>
@@ -15,6 +16,11 @@ namespace ainv {
 std::string &last_error() {
   static thread_local std::string msg;
   return msg;
+}
+
+uint64_t next_generation() {
+  static std::atomic<uint64_t> g{1};
+  return g.fetch_add(1);
 }
 
 namespace {
@@ -153,6 +159,7 @@ void ensure_stream(ainv_solver *h) {
 }
 
 void clear_built(ainv_solver *h) {
+  h->generation = next_generation();
   h->built = false;
   h->have_t = false;
   h->n = 0;
@@ -175,6 +182,7 @@ void build_layout(ainv_solver *h) {
   hll_from_csr(h->w, h->prec, h->hw, h->stream);
   hll_from_csr(h->z, h->prec, h->hz, h->stream);
   convert_vector(h->d64.as<double>(), h->n, h->prec, h->dv, h->stream);
+  h->generation = next_generation();
   h->have_t = false;
   h->hwt = Hll();
   h->hzt = Hll();
@@ -263,6 +271,18 @@ void copy_csr_dev(const DevCsr &a, DevCsr &b, hipStream_t s) {
 }
 
 }  // namespace
+
+size_t solver_t_bytes(const ainv_solver *h, int k) {
+  return (size_t)h->n * k *
+         (h->prec != AINV_PREC_F64 ? sizeof(float) : sizeof(double));
+}
+
+void solver_apply_into(const ainv_solver *h, const double *x, double *y,
+                       int k, void *t, hipStream_t s) {
+  const bool f32t = h->prec != AINV_PREC_F64;
+  hll_spmm_scale(h->hw, h->dv.p, x, false, k, t, k, k, s);
+  hll_spmm_axpby(h->hz, 1.0, t, f32t, k, 0.0, y, k, k, s);
+}
 
 void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
                   double *y, char trans, int k, int64_t ldx, int64_t ldy,
@@ -770,11 +790,11 @@ int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
       AINV_HIP_CHECK(hipMemcpyAsync(db.p, b, bytes, hipMemcpyHostToDevice, s));
       AINV_HIP_CHECK(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, s));
       krylov_solve(m, a, prec, db.as<double>(), dx.as<double>(), k, eps, itmax,
-                   iter, err, s);
+                   iter, err, s, flags);
       AINV_HIP_CHECK(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, s));
       AINV_HIP_CHECK(hipStreamSynchronize(s));
     } else {
-      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, s);
+      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, s, flags);
     }
   });
 }
@@ -910,12 +930,36 @@ int ainv_krylov_profile(const char *method, ainv_mat_h a, ainv_h prec,
       AINV_HIP_CHECK(hipMemsetAsync(x, 0, (size_t)a->n * k * sizeof(double),
                                     a->stream));
       AINV_HIP_CHECK(hipEventRecord(e.ev[0], a->stream));
-      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, a->stream);
+      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, a->stream, 0);
       AINV_HIP_CHECK(hipEventRecord(e.ev[1], a->stream));
       AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
       total += e.ms(0, 1);
     }
     *ms = total / reps;
+  });
+}
+
+int ainv_stream_copy(void *dst, const void *src, size_t bytes, int reps,
+                     double *ms) {
+  return guard([&] {
+    if (!dst || !src || !ms || reps < 1 || bytes % 16)
+      throw Error(AINV_ERR_ARG, "bad stream_copy arguments");
+    set_device(g_device);
+    hipStream_t s = nullptr;
+    AINV_HIP_CHECK(hipStreamCreate(&s));
+    try {
+      Events e(2);
+      stream_copy(dst, src, bytes, s);  // warm-up
+      AINV_HIP_CHECK(hipEventRecord(e.ev[0], s));
+      for (int r = 0; r < reps; ++r) stream_copy(dst, src, bytes, s);
+      AINV_HIP_CHECK(hipEventRecord(e.ev[1], s));
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+      *ms = e.ms(0, 1) / reps;
+    } catch (...) {
+      (void)hipStreamDestroy(s);
+      throw;
+    }
+    AINV_HIP_CHECK(hipStreamDestroy(s));
   });
 }
 

@@ -289,6 +289,25 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s) {
   AINV_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+namespace {
+__global__ __launch_bounds__(256) void k_stream_copy(
+    int64_t n16, const uint4 *__restrict__ src, uint4 *__restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  const int64_t n16 = (int64_t)(bytes / 16);
+  int64_t g = (n16 + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)g), dim3(256), 0, s, n16,
+                     static_cast<const uint4 *>(src), static_cast<uint4 *>(dst));
+  AINV_HIP_CHECK(hipGetLastError());
+}
+
 void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
                     hipStream_t s) {
   // d is kept FP64 for F64 factors and FP32 otherwise

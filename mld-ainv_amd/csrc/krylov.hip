@@ -4,27 +4,36 @@
 // reference drivers call them (tests/pdegen/ppde3d.f90:280,
 // tests/fileread/df_sample.f90:368-369).
 //
-// Everything stays FP64 and on the device: vector updates are fused with
-// the dot products that follow them (one pass over the vectors per fused
-// step), per-column scalars (rho, alpha, omega, ...) are computed by a
-// one-block kernel from deterministic block partial sums and read back by
-// the next kernel from device memory.  The host only reads the number of
-// still-active columns once per iteration to decide whether to continue.
+// Everything stays FP64 and on the device.  Vector updates are fused with
+// the dot products that follow them, and the dot products of A*v with the
+// system-matrix product itself (one pass each).  Per-column scalars (rho,
+// alpha, omega, the iteration count, eps) live in device memory and are
+// updated by one block per column from deterministic block partial sums,
+// so no kernel argument changes from one iteration to the next: one
+// iteration is captured once as a hipGraph and replayed.  The host reads
+// the number of still-active columns through pinned memory one iteration
+// behind the device, so the GPU never waits for it; the iteration launched
+// after the last column converged changes nothing (every kernel skips
+// inactive columns and their counts stop).
+//
+// Workspaces and captured graphs are cached per (matrix, method,
+// preconditioner, k) and rebuilt when the preconditioner's factors change.
+#include <algorithm>
 #include <cmath>
 
 #include "objects.h"
 
 namespace ainv {
+
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRedBlocks = 1024;
 constexpr int kMaxPairs = 3;
-constexpr int kMaxCols = 64;
+constexpr int kPlansPerMatrix = 8;
 
-// per-column scalars of the recurrences
+// per-column state of the recurrences
 struct Col {
-  double rho, rho_old, rho_next, alpha, omega, beta, bn, err;
+  double rho, rho_old, rho_next, alpha, omega, beta, bn, err, eps;
   int32_t active, iter;
 };
 
@@ -33,29 +42,21 @@ struct Vecs {
 };
 
 enum Op {
-  // BiCGSTAB
   OP_INIT_BICG,  // dots (b,b) (r,r) (q,r)
-  OP_P_BICG,     // p = r (it 1) or r + beta (p - omega v)
-  OP_DOT_QV,     // dot (q,v)
+  OP_P_BICG,     // p = r (first step) or r + beta (p - omega v)
   OP_S_BICG,     // s = r - alpha v
-  OP_DOT_TT_TS,  // dots (t,t) (t,s)
   OP_XR_BICG,    // x += alpha f + omega z ; r = s - omega t ; (r,r) (q,r)
-  // CG
-  OP_INIT_CG,  // dots (b,b) (r,r)
-  OP_DOT_RZ,   // dot (r,z)
-  OP_P_CG,     // p = z (it 1) or z + beta p
-  OP_DOT_PQ,   // dot (p,q)   (q holds A p)
-  OP_XR_CG,    // x += alpha p ; r -= alpha q ; (r,r)
+  OP_INIT_CG,    // dots (b,b) (r,r)
+  OP_DOT_RZ,     // dot (r,z)
+  OP_P_CG,       // p = z (first step) or z + beta p
+  OP_XR_CG,      // x += alpha p ; r -= alpha q ; (r,r)
 };
 
 template <int OP> struct NPairs { static constexpr int v = 0; };
 template <> struct NPairs<OP_INIT_BICG> { static constexpr int v = 3; };
-template <> struct NPairs<OP_DOT_QV> { static constexpr int v = 1; };
-template <> struct NPairs<OP_DOT_TT_TS> { static constexpr int v = 2; };
 template <> struct NPairs<OP_XR_BICG> { static constexpr int v = 2; };
 template <> struct NPairs<OP_INIT_CG> { static constexpr int v = 2; };
 template <> struct NPairs<OP_DOT_RZ> { static constexpr int v = 1; };
-template <> struct NPairs<OP_DOT_PQ> { static constexpr int v = 1; };
 template <> struct NPairs<OP_XR_CG> { static constexpr int v = 1; };
 
 // One fused pass over the n x k vectors.  Thread (rr, c) of a block works on
@@ -64,7 +65,6 @@ template <> struct NPairs<OP_XR_CG> { static constexpr int v = 1; };
 template <int OP, int K>
 __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
                                                   const Col *__restrict__ col,
-                                                  int it,
                                                   double *__restrict__ part) {
   constexpr int NP = NPairs<OP>::v;
   constexpr int R = kThreads / K;
@@ -84,17 +84,12 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         acc[1] += v.r[e] * v.r[e];
         acc[2] += v.q[e] * v.r[e];
       } else if constexpr (OP == OP_P_BICG) {
-        if (it == 1)
+        if (cs.iter == 1)
           v.p[e] = v.r[e];
         else
           v.p[e] = v.r[e] + cs.beta * (v.p[e] + (-cs.omega) * v.v[e]);
-      } else if constexpr (OP == OP_DOT_QV) {
-        acc[0] += v.q[e] * v.v[e];
       } else if constexpr (OP == OP_S_BICG) {
         v.s[e] = v.r[e] + (-cs.alpha) * v.v[e];
-      } else if constexpr (OP == OP_DOT_TT_TS) {
-        acc[0] += v.t[e] * v.t[e];
-        acc[1] += v.t[e] * v.s[e];
       } else if constexpr (OP == OP_XR_BICG) {
         v.x[e] = v.x[e] + cs.alpha * v.f[e] + cs.omega * v.z[e];
         const double rn = v.s[e] + (-cs.omega) * v.t[e];
@@ -107,12 +102,10 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
       } else if constexpr (OP == OP_DOT_RZ) {
         acc[0] += v.r[e] * v.z[e];
       } else if constexpr (OP == OP_P_CG) {
-        if (it == 1)
+        if (cs.iter == 1)
           v.p[e] = v.z[e];
         else
           v.p[e] = v.z[e] + cs.beta * v.p[e];
-      } else if constexpr (OP == OP_DOT_PQ) {
-        acc[0] += v.p[e] * v.q[e];
       } else if constexpr (OP == OP_XR_CG) {
         v.x[e] = v.x[e] + cs.alpha * v.p[e];
         const double rn = v.r[e] + (-cs.alpha) * v.q[e];
@@ -143,114 +136,116 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
   }
 }
 
-// Scalar step of the recurrences: thread c sums its column's partials over
-// the blocks in order and updates the column state.
+// Scalar step of the recurrences, one block per column: the block sums its
+// column's partials (strided, then a fixed tree) and thread 0 updates the
+// column state.  nactive: zeroed at the start of an iteration (RHO steps),
+// incremented by every column still active at its end (INIT / CONV).
 enum Fin {
-  FIN_INIT,        // bn, err, active, rho_next (BiCGSTAB) from init dots
-  FIN_RHO_BICG,    // rho = rho_next, beta, breakdown on rho == 0
+  FIN_INIT,        // bn, err, active, rho_next (BiCGSTAB)
+  FIN_RHO_BICG,    // iter++, rho = rho_next, beta; breakdown on rho == 0
   FIN_ALPHA_BICG,  // alpha = rho / (q,v)
   FIN_OMEGA,       // omega = (t,s)/(t,t)
   FIN_CONV_BICG,   // err, convergence, rho_next = (q,r)
   FIN_INIT_CG,
-  FIN_RHO_CG,    // rho = (r,z), beta
+  FIN_RHO_CG,    // iter++, rho = (r,z), beta
   FIN_ALPHA_CG,  // alpha = rho / (p,q)
   FIN_CONV_CG,
 };
 
-__device__ double sum_part(const double *part, int q, int c, int nblk) {
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b)
-    s += part[((int64_t)q * kRedBlocks + b) * kMaxCols + c];
-  return s;
+__host__ __device__ inline int fin_pairs(int fin) {
+  switch (fin) {
+    case FIN_INIT: return 3;
+    case FIN_INIT_CG:
+    case FIN_OMEGA:
+    case FIN_CONV_BICG: return 2;
+    case FIN_RHO_BICG: return 0;
+    default: return 1;
+  }
 }
 
-__global__ void k_fin(int fin, int k, int nblk, int it, double eps,
-                      const double *__restrict__ part, Col *__restrict__ col,
-                      int32_t *__restrict__ nactive) {
-  __shared__ int32_t cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  const int c = threadIdx.x;
-  if (c < k) {
-    Col cs = col[c];
-    if (fin == FIN_INIT || fin == FIN_INIT_CG) {
-      const double bb = sum_part(part, 0, c, nblk);
-      const double rr = sum_part(part, 1, c, nblk);
-      cs.bn = sqrt(bb);
-      const double rn = sqrt(rr);
-      cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
-      cs.active = !(cs.err <= eps);
-      cs.iter = 0;
-      cs.rho = 0.0;
-      cs.rho_old = 0.0;
-      cs.alpha = cs.omega = cs.beta = 0.0;
-      if (fin == FIN_INIT) cs.rho_next = sum_part(part, 2, c, nblk);
-    } else if (cs.active) {
-      switch (fin) {
-        case FIN_RHO_BICG:
-          cs.iter = it;
-          cs.rho_old = cs.rho;
-          cs.rho = cs.rho_next;
-          if (cs.rho == 0.0) {
-            cs.active = 0;
-          } else if (it > 1) {
-            cs.beta = (cs.rho / cs.rho_old) * (cs.alpha / cs.omega);
-          }
-          break;
-        case FIN_ALPHA_BICG: {
-          const double sigma = sum_part(part, 0, c, nblk);
-          if (sigma == 0.0)
-            cs.active = 0;
-          else
-            cs.alpha = cs.rho / sigma;
-          break;
-        }
-        case FIN_OMEGA: {
-          const double tt = sum_part(part, 0, c, nblk);
-          const double ts = sum_part(part, 1, c, nblk);
-          if (tt == 0.0) {
-            cs.active = 0;
-          } else {
-            cs.omega = ts / tt;
-            if (cs.omega == 0.0) cs.active = 0;
-          }
-          break;
-        }
-        case FIN_CONV_BICG: {
-          const double rn = sqrt(sum_part(part, 0, c, nblk));
-          cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
-          cs.rho_next = sum_part(part, 1, c, nblk);
-          if (cs.err <= eps) cs.active = 0;
-          break;
-        }
-        case FIN_RHO_CG: {
-          cs.iter = it;
-          cs.rho_old = cs.rho;
-          cs.rho = sum_part(part, 0, c, nblk);
-          if (it > 1) cs.beta = cs.rho / cs.rho_old;
-          break;
-        }
-        case FIN_ALPHA_CG: {
-          const double sigma = sum_part(part, 0, c, nblk);
-          if (sigma == 0.0)
-            cs.active = 0;
-          else
-            cs.alpha = cs.rho / sigma;
-          break;
-        }
-        case FIN_CONV_CG: {
-          const double rn = sqrt(sum_part(part, 0, c, nblk));
-          cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
-          if (cs.err <= eps) cs.active = 0;
-          break;
-        }
-      }
-    }
-    col[c] = cs;
-    if (cs.active) atomicAdd(&cnt, 1);
+__global__ __launch_bounds__(kThreads) void k_fin(
+    int fin, int nblk, const double *__restrict__ part, Col *__restrict__ col,
+    int32_t *__restrict__ nactive) {
+  const int c = blockIdx.x;
+  const int np = fin_pairs(fin);
+  __shared__ double red[kMaxPairs][kThreads];
+  for (int q = 0; q < np; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kThreads)
+      s += part[((int64_t)q * kRedBlocks + b) * kMaxCols + c];
+    red[q][threadIdx.x] = s;
   }
   __syncthreads();
-  if (threadIdx.x == 0) *nactive = cnt;
+  for (int st = kThreads / 2; st >= 1; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int q = 0; q < np; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if ((fin == FIN_RHO_BICG || fin == FIN_RHO_CG) && c == 0) *nactive = 0;
+  Col cs = col[c];
+  const double s0 = red[0][0], s1 = red[1][0], s2 = red[2][0];
+  if (fin == FIN_INIT || fin == FIN_INIT_CG) {
+    cs.bn = sqrt(s0);
+    const double rn = sqrt(s1);
+    cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
+    cs.active = !(cs.err <= cs.eps);
+    cs.iter = 0;
+    cs.rho = cs.rho_old = 0.0;
+    cs.alpha = cs.omega = cs.beta = 0.0;
+    cs.rho_next = (fin == FIN_INIT) ? s2 : 0.0;
+  } else if (cs.active) {
+    switch (fin) {
+      case FIN_RHO_BICG:
+        cs.iter += 1;
+        cs.rho_old = cs.rho;
+        cs.rho = cs.rho_next;
+        if (cs.rho == 0.0)
+          cs.active = 0;
+        else if (cs.iter > 1)
+          cs.beta = (cs.rho / cs.rho_old) * (cs.alpha / cs.omega);
+        break;
+      case FIN_ALPHA_BICG:
+      case FIN_ALPHA_CG:
+        if (s0 == 0.0)
+          cs.active = 0;
+        else
+          cs.alpha = cs.rho / s0;
+        break;
+      case FIN_OMEGA:
+        if (s0 == 0.0) {
+          cs.active = 0;
+        } else {
+          cs.omega = s1 / s0;
+          if (cs.omega == 0.0) cs.active = 0;
+        }
+        break;
+      case FIN_CONV_BICG: {
+        const double rn = sqrt(s0);
+        cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
+        cs.rho_next = s1;
+        if (cs.err <= cs.eps) cs.active = 0;
+        break;
+      }
+      case FIN_RHO_CG:
+        cs.iter += 1;
+        cs.rho_old = cs.rho;
+        cs.rho = s0;
+        if (cs.iter > 1) cs.beta = cs.rho / cs.rho_old;
+        break;
+      case FIN_CONV_CG: {
+        const double rn = sqrt(s0);
+        cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
+        if (cs.err <= cs.eps) cs.active = 0;
+        break;
+      }
+    }
+  }
+  col[c] = cs;
+  if ((fin == FIN_INIT || fin == FIN_INIT_CG || fin == FIN_CONV_BICG ||
+       fin == FIN_CONV_CG) &&
+      cs.active)
+    atomicAdd(nactive, 1);
 }
 
 int pow2_at_least(int k) {
@@ -259,8 +254,8 @@ int pow2_at_least(int k) {
   return K;
 }
 
-int blocks_for(int32_t n, int K) {
-  int64_t R = kThreads / K;
+int vec_blocks(int32_t n, int k) {
+  int64_t R = kThreads / pow2_at_least(k);
   int64_t g = (n + R - 1) / R;
   if (g > kRedBlocks) g = kRedBlocks;
   if (g < 1) g = 1;
@@ -268,13 +263,13 @@ int blocks_for(int32_t n, int K) {
 }
 
 template <int OP>
-void vec_op(int32_t n, int k, const Vecs &v, const Col *col, int it,
-            double *part, hipStream_t s) {
+int vec_op(int32_t n, int k, const Vecs &v, const Col *col, double *part,
+           hipStream_t s) {
   const int K = pow2_at_least(k);
-  const int g = blocks_for(n, K);
+  const int g = vec_blocks(n, k);
 #define AINV_VL(KK)                                                        \
   hipLaunchKernelGGL((k_vec<OP, KK>), dim3(g), dim3(kThreads), 0, s, n, k, \
-                     v, col, it, part)
+                     v, col, part)
   switch (K) {
     case 1: AINV_VL(1); break;
     case 2: AINV_VL(2); break;
@@ -286,30 +281,148 @@ void vec_op(int32_t n, int k, const Vecs &v, const Col *col, int it,
   }
 #undef AINV_VL
   AINV_HIP_CHECK(hipGetLastError());
+  return g;
 }
 
-void fin(int which, int32_t n, int k, int it, double eps, const double *part,
-         Col *col, int32_t *nact, hipStream_t s) {
-  const int g = blocks_for(n, pow2_at_least(k));
-  hipLaunchKernelGGL(k_fin, dim3(1), dim3(kMaxCols), 0, s, which, k, g, it,
-                     eps, part, col, nact);
+void fin(int which, int k, int nblk, const double *part, Col *col,
+         int32_t *nact, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin, dim3(k), dim3(kThreads), 0, s, which, nblk, part,
+                     col, nact);
   AINV_HIP_CHECK(hipGetLastError());
 }
 
-void precond(ainv_solver *m, const double *x, double *y, int32_t n, int k,
-             hipStream_t s) {
-  if (m)
-    solver_apply(m, 1.0, x, 0.0, y, 'N', k, k, k, s);
+}  // namespace
+
+struct KrylovPlan {
+  bool cg = false;
+  const ainv_solver *m = nullptr;
+  uint64_t gen = 0;
+  int k = 0;
+  int32_t n = 0;
+  DevBuf work, colbuf, part, nact, tbuf;
+  Vecs v{};
+  Col *col = nullptr;
+  double *pt = nullptr;
+  int32_t *na = nullptr;
+  int32_t *pinned = nullptr;  // host copies of nactive, two slots
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void release_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~KrylovPlan() {
+    release_graph();
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (pinned) (void)hipHostFree(pinned);
+  }
+};
+
+void destroy_plan(KrylovPlan *p) { delete p; }
+
+namespace {
+
+KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k) {
+  for (size_t i = 0; i < a->plans.size(); ++i) {
+    KrylovPlan *p = a->plans[i].get();
+    if (p->cg == cg && p->m == m && p->k == k) {
+      if (m && p->gen != m->generation) {
+        p->release_graph();  // factors changed: recapture
+        if (p->tbuf.bytes < solver_t_bytes(m, k))
+          p->tbuf.alloc(solver_t_bytes(m, k));
+        p->gen = m->generation;
+      }
+      if (i != 0) std::rotate(a->plans.begin(), a->plans.begin() + i,
+                              a->plans.begin() + i + 1);
+      return a->plans.front().get();
+    }
+  }
+  std::unique_ptr<KrylovPlan, PlanDeleter> up(new KrylovPlan());
+  KrylovPlan *p = up.get();
+  p->cg = cg;
+  p->m = m;
+  p->gen = m ? m->generation : 0;
+  p->k = k;
+  p->n = a->n;
+  const int64_t nk = (int64_t)a->n * k;
+  const int nvec = cg ? 6 : 10;  // x, b + recurrence vectors
+  p->work.alloc((size_t)nk * nvec * sizeof(double));
+  double *w0 = p->work.as<double>();
+  p->v.x = w0;
+  p->v.b = w0 + nk;
+  p->v.r = w0 + 2 * nk;
+  p->v.p = w0 + 3 * nk;
+  p->v.z = w0 + 4 * nk;
+  p->v.q = w0 + 5 * nk;  // CG: A p ; BiCGSTAB: rtilde
+  if (!cg) {
+    p->v.v = w0 + 6 * nk;
+    p->v.f = w0 + 7 * nk;
+    p->v.s = w0 + 8 * nk;
+    p->v.t = w0 + 9 * nk;
+  }
+  p->colbuf.alloc(sizeof(Col) * kMaxCols);
+  p->part.alloc(sizeof(double) * kMaxPairs * kRedBlocks * kMaxCols);
+  p->nact.alloc(sizeof(int32_t));
+  if (m) p->tbuf.alloc(solver_t_bytes(m, k));
+  p->col = p->colbuf.as<Col>();
+  p->pt = p->part.as<double>();
+  p->na = p->nact.as<int32_t>();
+  AINV_HIP_CHECK(hipHostMalloc((void **)&p->pinned, 2 * sizeof(int32_t),
+                               hipHostMallocDefault));
+  for (auto &e : p->ev)
+    AINV_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  a->plans.insert(a->plans.begin(), std::move(up));
+  if ((int)a->plans.size() > kPlansPerMatrix) a->plans.pop_back();
+  return p;
+}
+
+void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
+  if (p->m)
+    solver_apply_into(p->m, x, y, p->k, p->tbuf.p, s);
   else
-    AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)n * k * sizeof(double),
+    AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->k * sizeof(double),
                                   hipMemcpyDeviceToDevice, s));
+}
+
+// One full iteration: the kernel sequence that is captured and replayed.
+void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
+  const int32_t n = p->n;
+  const int k = p->k;
+  const Vecs &v = p->v;
+  if (!p->cg) {
+    fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
+    vec_op<OP_P_BICG>(n, k, v, p->col, p->pt, s);
+    precond(p, v.p, v.f, s);
+    int g = hll_spmv_dot(a->hll, v.f, v.v, k, v.q, 1, p->pt, s);
+    fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
+    vec_op<OP_S_BICG>(n, k, v, p->col, p->pt, s);
+    precond(p, v.s, v.z, s);
+    g = hll_spmv_dot(a->hll, v.z, v.t, k, v.s, 2, p->pt, s);
+    fin(FIN_OMEGA, k, g, p->pt, p->col, p->na, s);
+    g = vec_op<OP_XR_BICG>(n, k, v, p->col, p->pt, s);
+    fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
+  } else {
+    precond(p, v.r, v.z, s);
+    int g = vec_op<OP_DOT_RZ>(n, k, v, p->col, p->pt, s);
+    fin(FIN_RHO_CG, k, g, p->pt, p->col, p->na, s);
+    vec_op<OP_P_CG>(n, k, v, p->col, p->pt, s);
+    g = hll_spmv_dot(a->hll, v.p, v.q, k, v.p, 1, p->pt, s);
+    fin(FIN_ALPHA_CG, k, g, p->pt, p->col, p->na, s);
+    g = vec_op<OP_XR_CG>(n, k, v, p->col, p->pt, s);
+    fin(FIN_CONV_CG, k, g, p->pt, p->col, p->na, s);
+  }
 }
 
 }  // namespace
 
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                   const double *b, double *x, int k, double eps, int itmax,
-                  int32_t *iter, double *err, hipStream_t s) {
+                  int32_t *iter, double *err, hipStream_t s, int flags) {
   const int32_t n = a->n;
   if (k < 1 || k > kMaxCols) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   if (m && m->n != n)
@@ -318,91 +431,74 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   if (!cg && method != "BICGSTAB")
     throw Error(AINV_ERR_ARG, "unknown Krylov method " + method +
                                   " (BICGSTAB, CG available)");
+  KrylovPlan *p = get_plan(a, cg, m, k);
   const size_t vb = (size_t)n * k * sizeof(double);
-  DevBuf work, colbuf, part, nact;
-  const int nvec = cg ? 4 : 8;
-  work.alloc(vb * nvec);
-  colbuf.alloc(sizeof(Col) * kMaxCols);
-  part.alloc(sizeof(double) * kMaxPairs * kRedBlocks * kMaxCols);
-  nact.alloc(sizeof(int32_t));
-  AINV_HIP_CHECK(hipMemsetAsync(colbuf.p, 0, colbuf.bytes, s));
-  double *w0 = work.as<double>();
-  const int64_t nk = (int64_t)n * k;
-  Vecs v{};
-  v.x = x;
-  v.b = const_cast<double *>(b);
-  v.r = w0;
-  v.p = w0 + nk;
-  v.z = w0 + 2 * nk;
-  v.q = w0 + 3 * nk;  // CG: q = A p ; BiCGSTAB: rtilde
-  if (!cg) {
-    v.v = w0 + 4 * nk;
-    v.f = w0 + 5 * nk;
-    v.s = w0 + 6 * nk;
-    v.t = w0 + 7 * nk;
-  }
-  Col *col = colbuf.as<Col>();
-  double *pt = part.as<double>();
-  int32_t *na = nact.as<int32_t>();
-  // r = b - A x
-  AINV_HIP_CHECK(hipMemcpyAsync(v.r, b, vb, hipMemcpyDeviceToDevice, s));
-  hll_spmm_axpby(a->hll, -1.0, x, false, k, 1.0, v.r, k, k, s);
-  // column state starts all-active so the init pass runs every column
+  const Vecs &v = p->v;
+  AINV_HIP_CHECK(hipMemcpyAsync(v.b, b, vb, hipMemcpyDeviceToDevice, s));
+  AINV_HIP_CHECK(hipMemcpyAsync(v.x, x, vb, hipMemcpyDeviceToDevice, s));
+  // column state: all active for the init pass, eps per column
   std::vector<Col> init(kMaxCols);
-  for (auto &c0 : init) c0 = Col{0, 0, 0, 0, 0, 0, 0, 0, 1, 0};
-  AINV_HIP_CHECK(hipMemcpyAsync(col, init.data(), sizeof(Col) * kMaxCols,
+  for (auto &c0 : init) {
+    c0 = Col{};
+    c0.active = 1;
+    c0.eps = eps;
+  }
+  AINV_HIP_CHECK(hipMemcpyAsync(p->col, init.data(), sizeof(Col) * kMaxCols,
                                 hipMemcpyHostToDevice, s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  // r = b - A x
+  AINV_HIP_CHECK(hipMemcpyAsync(v.r, v.b, vb, hipMemcpyDeviceToDevice, s));
+  hll_spmm_axpby(a->hll, -1.0, v.x, false, k, 1.0, v.r, k, k, s);
+  int g;
   if (!cg) {
     AINV_HIP_CHECK(hipMemcpyAsync(v.q, v.r, vb, hipMemcpyDeviceToDevice, s));
-    vec_op<OP_INIT_BICG>(n, k, v, col, 0, pt, s);
-    fin(FIN_INIT, n, k, 0, eps, pt, col, na, s);
+    g = vec_op<OP_INIT_BICG>(n, k, v, p->col, p->pt, s);
+    fin(FIN_INIT, k, g, p->pt, p->col, p->na, s);
   } else {
-    vec_op<OP_INIT_CG>(n, k, v, col, 0, pt, s);
-    fin(FIN_INIT_CG, n, k, 0, eps, pt, col, na, s);
+    g = vec_op<OP_INIT_CG>(n, k, v, p->col, p->pt, s);
+    fin(FIN_INIT_CG, k, g, p->pt, p->col, p->na, s);
   }
-  int32_t active = 0;
-  AINV_HIP_CHECK(hipMemcpyAsync(&active, na, sizeof(int32_t),
+  AINV_HIP_CHECK(hipMemcpyAsync(p->pinned, p->na, sizeof(int32_t),
                                 hipMemcpyDeviceToHost, s));
   AINV_HIP_CHECK(hipStreamSynchronize(s));
-  int it = 0;
-  while (active > 0 && it < itmax) {
-    ++it;
-    if (!cg) {
-      fin(FIN_RHO_BICG, n, k, it, eps, pt, col, na, s);
-      vec_op<OP_P_BICG>(n, k, v, col, it, pt, s);
-      precond(m, v.p, v.f, n, k, s);
-      hll_spmm_axpby(a->hll, 1.0, v.f, false, k, 0.0, v.v, k, k, s);
-      vec_op<OP_DOT_QV>(n, k, v, col, it, pt, s);
-      fin(FIN_ALPHA_BICG, n, k, it, eps, pt, col, na, s);
-      vec_op<OP_S_BICG>(n, k, v, col, it, pt, s);
-      precond(m, v.s, v.z, n, k, s);
-      hll_spmm_axpby(a->hll, 1.0, v.z, false, k, 0.0, v.t, k, k, s);
-      vec_op<OP_DOT_TT_TS>(n, k, v, col, it, pt, s);
-      fin(FIN_OMEGA, n, k, it, eps, pt, col, na, s);
-      vec_op<OP_XR_BICG>(n, k, v, col, it, pt, s);
-      fin(FIN_CONV_BICG, n, k, it, eps, pt, col, na, s);
-    } else {
-      precond(m, v.r, v.z, n, k, s);
-      vec_op<OP_DOT_RZ>(n, k, v, col, it, pt, s);
-      fin(FIN_RHO_CG, n, k, it, eps, pt, col, na, s);
-      vec_op<OP_P_CG>(n, k, v, col, it, pt, s);
-      hll_spmm_axpby(a->hll, 1.0, v.p, false, k, 0.0, v.q, k, k, s);
-      vec_op<OP_DOT_PQ>(n, k, v, col, it, pt, s);
-      fin(FIN_ALPHA_CG, n, k, it, eps, pt, col, na, s);
-      vec_op<OP_XR_CG>(n, k, v, col, it, pt, s);
-      fin(FIN_CONV_CG, n, k, it, eps, pt, col, na, s);
+  const bool eager = (flags & AINV_KRYLOV_EAGER) != 0;
+  if (p->pinned[0] > 0 && itmax > 0 && !eager && !p->exec) {
+    AINV_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_iteration(p, a, s);
+    } catch (...) {
+      hipGraph_t g0 = nullptr;
+      (void)hipStreamEndCapture(s, &g0);
+      if (g0) (void)hipGraphDestroy(g0);
+      throw;
     }
-    AINV_HIP_CHECK(hipMemcpyAsync(&active, na, sizeof(int32_t),
-                                  hipMemcpyDeviceToHost, s));
-    AINV_HIP_CHECK(hipStreamSynchronize(s));
+    AINV_HIP_CHECK(hipStreamEndCapture(s, &p->graph));
+    AINV_HIP_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
   }
+  if (p->pinned[0] > 0) {
+    // launch iteration it, then look at the count of iteration it - 1
+    for (int it = 1; it <= itmax; ++it) {
+      if (eager)
+        enqueue_iteration(p, a, s);
+      else
+        AINV_HIP_CHECK(hipGraphLaunch(p->exec, s));
+      const int slot = it & 1;
+      AINV_HIP_CHECK(hipMemcpyAsync(p->pinned + slot, p->na, sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+      AINV_HIP_CHECK(hipEventRecord(p->ev[slot], s));
+      if (it >= 2) {
+        AINV_HIP_CHECK(hipEventSynchronize(p->ev[slot ^ 1]));
+        if (p->pinned[slot ^ 1] == 0) break;
+      }
+    }
+  }
+  AINV_HIP_CHECK(hipMemcpyAsync(x, v.x, vb, hipMemcpyDeviceToDevice, s));
   std::vector<Col> hc(k);
-  AINV_HIP_CHECK(hipMemcpyAsync(hc.data(), col, sizeof(Col) * k,
+  AINV_HIP_CHECK(hipMemcpyAsync(hc.data(), p->col, sizeof(Col) * k,
                                 hipMemcpyDeviceToHost, s));
   AINV_HIP_CHECK(hipStreamSynchronize(s));
   for (int c = 0; c < k; ++c) {
-    // a column still active at itmax ran every iteration
-    iter[c] = hc[c].active ? it : hc[c].iter;
+    iter[c] = hc[c].iter;
     err[c] = hc[c].err;
   }
 }

@@ -2,9 +2,22 @@
 // ABI (capi.cpp) and the Krylov engine (krylov.hip).
 #pragma once
 
+#include <memory>
 #include <mutex>
 
 #include "ainv_internal.h"
+
+namespace ainv {
+// Process-wide counter: every (re)build, re-mold or release of a solver's
+// factors takes a new value, so cached Krylov plans (which hold device
+// addresses of the factors inside captured graphs) can tell they are stale.
+uint64_t next_generation();
+struct KrylovPlan;
+void destroy_plan(KrylovPlan *p);
+struct PlanDeleter {
+  void operator()(KrylovPlan *p) const { destroy_plan(p); }
+};
+}  // namespace ainv
 
 struct ainv_solver {
   // ---- parameters (mld_d_ainv_solver.F90:46-67, mld_d_invk_solver.f90:
@@ -23,6 +36,7 @@ struct ainv_solver {
   std::mutex mu;  // one call at a time per handle
 
   // ---- built data
+  uint64_t generation = 0;
   bool built = false;
   int32_t n = 0;
   ainv::DevCsr w, z;  // FP64 factors: stored W and Z
@@ -45,6 +59,8 @@ struct ainv_matrix {
   int32_t n = 0;
   ainv::DevCsr csr;
   ainv::Hll hll;
+  // Krylov workspaces and captured iteration graphs, most recent first
+  std::vector<std::unique_ptr<ainv::KrylovPlan, ainv::PlanDeleter>> plans;
 };
 
 namespace ainv {
@@ -54,9 +70,16 @@ namespace ainv {
 void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
                   double *y, char trans, int k, int64_t ldx, int64_t ldy,
                   hipStream_t s);
+// y = M x ('N', beta = 0, ld = k) with a caller-owned intermediate buffer t
+// of n*k words of the factor's intermediate type: no allocation, so it may
+// be captured in a graph.
+void solver_apply_into(const ainv_solver *h, const double *x, double *y,
+                       int k, void *t, hipStream_t s);
+size_t solver_t_bytes(const ainv_solver *h, int k);
 
+// flags: AINV_KRYLOV_EAGER disables graph capture (used to check replays)
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                   const double *b, double *x, int k, double eps, int itmax,
-                  int32_t *iter, double *err, hipStream_t s);
+                  int32_t *iter, double *err, hipStream_t s, int flags);
 
 }  // namespace ainv
