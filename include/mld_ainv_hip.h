@@ -87,8 +87,11 @@ int ainv_get_kind(ainv_h h, int *kind);
  * _csetc.f90:56-61, mld_d_invk_solver_cseti.f90:55-62,
  * mld_d_invt_solver_cseti.f90:55-62, mld_d_invt_solver_csetr.f90:55-62).
  * Keys: SUB_FILLIN, AINV_ALG, SUB_ILUTHRS, INV_FILLIN, INV_THRESH, plus
- * FACTOR_PREC ('F64'|'F32'|'BF16' or AINV_PREC_*).  Unknown keys are
- * passed to the base solver, which ignores them (info = 0). */
+ * FACTOR_PREC ('F64'|'F32'|'BF16' or AINV_PREC_*) and APPLY_MAP
+ * ('AUTO'|'SPLIT'|'ROW'|'COOP' or -1/0/1/2: kernel mapping of block applies
+ * with a power-of-two leading dimension; AUTO times each once per block
+ * width and keeps the fastest).  Unknown keys are passed to the base solver, which
+ * ignores them (info = 0). */
 int ainv_cseti(ainv_h h, const char *what, int val);
 int ainv_csetr(ainv_h h, const char *what, double val);
 int ainv_csetc(ainv_h h, const char *what, const char *val);
@@ -196,6 +199,34 @@ int ainv_gen_random(int32_t nrows, uint64_t seed, double median, double sigma,
 /* n*k N(0,1) values from the library RNG (xoshiro256** seeded by
  * SplitMix64), so CPU and GPU paths see byte-identical inputs. */
 int ainv_gen_normal(uint64_t seed, int64_t count, double *out);
+
+/* ---- serving loop: requests on many matrices, continuous batching ------
+ * A server keeps each registered matrix's factors resident on the device
+ * after first touch, under an HBM budget with LRU eviction of idle matrices.
+ * Worker threads (one HIP stream each) batch the pending right-hand sides of
+ * one matrix into up to 64 column slots; a converged column's slot is
+ * refilled by the next pending request between two Krylov iterations.
+ * No counterpart in the reference (its drivers solve one system at a time);
+ * serving is the deployment form of the apply + Krylov path. */
+typedef struct ainv_server *ainv_server_h;
+int ainv_server_create(int device, int64_t hbm_budget_bytes, int workers,
+                       ainv_server_h *srv);
+/* kind: AINV_KIND_*; params: "KEY=VALUE;..." with the keys of
+ * ainv_cseti/csetr/csetc plus METHOD=BICGSTAB|CG.  Host CSR is copied. */
+int ainv_server_add_matrix(ainv_server_h srv, int32_t n, const int64_t *irp,
+                           const int32_t *ja, const double *val,
+                           int index_base, int kind, const char *params,
+                           int32_t *mat_id);
+/* b: host, n x k interleaved (1 <= k <= 64), x0 = 0, istop = 2. */
+int ainv_server_submit(ainv_server_h srv, int32_t mat_id, int k,
+                       const double *b, double eps, int itmax,
+                       int64_t *req_id);
+/* Blocks until the request completes; x: host n x k; latency from submit to
+ * completion.  Releases the request. */
+int ainv_server_wait(ainv_server_h srv, int64_t req_id, double *x,
+                     int32_t *iters, double *errs, double *latency_ms);
+int ainv_server_stats(ainv_server_h srv, char *json, size_t len);
+int ainv_server_free(ainv_server_h srv);
 
 /* ---- device plumbing for hosts without a GPU array library (Python via
  * ctypes, Fortran via iso_c_binding).  No counterpart in the reference,

@@ -101,6 +101,14 @@ def lib():
                                 C.c_double, C.c_int, C.c_int, i32p, dp, dp],
         "ainv_layout": [vp, i64p, i64p, i64p, i64p, i64p, P(C.c_int)],
         "ainv_stream_copy": [vp, vp, C.c_size_t, C.c_int, dp],
+        "ainv_server_create": [C.c_int, C.c_int64, C.c_int, P(vp)],
+        "ainv_server_add_matrix": [vp, C.c_int32, i64p, i32p, dp, C.c_int,
+                                   C.c_int, C.c_char_p, i32p],
+        "ainv_server_submit": [vp, C.c_int32, C.c_int, dp, C.c_double, C.c_int,
+                               i64p],
+        "ainv_server_wait": [vp, C.c_int64, dp, i32p, dp, dp],
+        "ainv_server_stats": [vp, C.c_char_p, C.c_size_t],
+        "ainv_server_free": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -540,3 +548,74 @@ def gen_normal(seed, shape):
     _ck(lib().ainv_gen_normal(int(seed), count,
                               out.ctypes.data_as(C.POINTER(C.c_double))))
     return out.reshape(shape)
+
+
+# ----------------------------------------------------------------- serving
+class Server:
+    """Serving loop (ainv_server_*): matrices registered once, factors built
+    on first touch and kept resident under an HBM budget with LRU eviction,
+    requests of 1..64 right-hand sides batched per matrix with continuous
+    admission into freed column slots."""
+
+    KINDS = {"AINV": KIND_AINV, "INVK": KIND_INVK, "INVT": KIND_INVT}
+
+    def __init__(self, hbm_budget_bytes=64 << 30, workers=2, device=0):
+        _ensure_init()
+        h = C.c_void_p()
+        _ck(lib().ainv_server_create(device, int(hbm_budget_bytes), workers,
+                                     C.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def add_matrix(self, a, kind="INVK", **params):
+        """params: solver keys (SUB_FILLIN=1, FACTOR_PREC='F32', ...) and
+        METHOD='BICGSTAB'|'CG'.  Returns the matrix id."""
+        n, irp, ja, val = _csr_args(a)
+        spec = ";".join(f"{k}={v}" for k, v in params.items())
+        mid = C.c_int32()
+        _ck(lib().ainv_server_add_matrix(self._h, n, *_ptrs(irp, ja, val), 0,
+                                         self.KINDS[kind], spec.encode(),
+                                         C.byref(mid)))
+        self._n[mid.value] = n
+        return mid.value
+
+    def submit(self, mat_id, b, eps=1e-8, itmax=1000):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        k = 1 if b.ndim == 1 else b.shape[1]
+        rid = C.c_int64()
+        _ck(lib().ainv_server_submit(self._h, mat_id, k,
+                                     b.ctypes.data_as(C.POINTER(C.c_double)),
+                                     eps, itmax, C.byref(rid)))
+        return (rid.value, mat_id, k, b.ndim)
+
+    def wait(self, ticket):
+        rid, mat_id, k, nd = ticket
+        n = self._n[mat_id]
+        x = np.empty((n, k))
+        it = np.empty(k, dtype=np.int32)
+        err = np.empty(k)
+        lat = C.c_double()
+        _ck(lib().ainv_server_wait(self._h, rid,
+                                   x.ctypes.data_as(C.POINTER(C.c_double)),
+                                   it.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   err.ctypes.data_as(C.POINTER(C.c_double)),
+                                   C.byref(lat)))
+        return (x[:, 0] if nd == 1 else x), it, err, lat.value
+
+    def stats(self):
+        import json
+        buf = C.create_string_buffer(4096)
+        _ck(lib().ainv_server_stats(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def free(self):
+        if self._h:
+            _ck(lib().ainv_server_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and _lib is not None:
+                _lib.ainv_server_free(self._h)
+        except Exception:
+            pass

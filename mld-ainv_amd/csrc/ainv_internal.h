@@ -113,11 +113,21 @@ void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
 // out = d o (A x)          (EPI scale, out type by prec: F64 double, else
 // float) or
 // y   = alpha * A x + beta*y (EPI axpby)  for k interleaved columns.
+// map: AINV_MAP_SPLIT (lanes split rows x columns, any ld),
+// AINV_MAP_ROW (one row per lane) or AINV_MAP_COOP (the lanes of a row
+// group load each x row together); ROW and COOP need ldx = ldo = K = pow2
+// >= k, k <= 32 and 16-byte aligned vectors, otherwise the split mapping
+// runs.
+constexpr int AINV_MAP_SPLIT = 0;
+constexpr int AINV_MAP_ROW = 1;
+constexpr int AINV_MAP_COOP = 2;
+constexpr int kNumMaps = 3;
 void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
-                    int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s);
+                    int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s,
+                    int map = AINV_MAP_SPLIT);
 void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, double *y, int64_t ldy, int k,
-                    hipStream_t s);
+                    hipStream_t s, int map = AINV_MAP_SPLIT);
 // Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *
 // kMaxCols + c] for pair q, block b, column c.
 constexpr int kRedBlocks = 1024;
@@ -125,8 +135,9 @@ constexpr int kMaxCols = 64;
 // y = A x (FP64 HLL, k interleaved columns, ld = k) fused with the dot
 // products of its result: np = 1 gives (w, y), np = 2 gives (y, y) and
 // (y, w), per column, as block partials.  Returns the number of blocks.
-int hll_spmv_dot(const Hll &a, const double *x, double *y, int k,
-                 const double *w, int np, double *part, hipStream_t s);
+int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
+                 const double *w, int np, double *part, int map,
+                 hipStream_t s);
 
 // ---- GPU row inverses (invkt.hip) ----------------------------------------
 // Row-wise approximate inverse of a unit upper triangular matrix given by

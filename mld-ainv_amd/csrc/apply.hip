@@ -10,6 +10,11 @@
 // word is a broadcast.  Epilogues:
 //   SCALE: out[r, c] = d[r] * acc      (first SpMV fused with d o .)
 //   AXPBY: y[r, c] = alpha * acc + beta * y[r, c], y not read if beta == 0
+// A second mapping ("row"): one row per lane for any K, each lane loading
+// the K values x[col*K .. col*K+K) of an entry as 16-byte vectors (ld = K,
+// K a power of two).  It issues about K/2 times fewer load instructions per
+// entry than the split mapping at the cost of K accumulators per lane; the
+// autotuner picks the faster of the two per factor and K.
 // The algorithmic traffic per launch is (nnz)(value + 4) bytes of factor,
 // 4 bytes of row length per row, the gathered x and the output; no kernel
 // here re-reads a factor word.
@@ -20,6 +25,55 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWavesPerBlock * kSlice;
+
+// Workgroups are dispatched round-robin over the XCDs (workgroup i runs on
+// XCD i mod 8 on MI355X), each XCD with its own L2.  Neighbouring slices
+// gather mostly the same x rows (the factor patterns are banded), so the
+// kernels renumber workgroups to give each XCD a contiguous range of
+// slices: an x row is then fetched into one L2 rather than into all eight.
+#ifndef AINV_XCD_REMAP
+#define AINV_XCD_REMAP 1
+#endif
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t g) {
+#if AINV_XCD_REMAP
+  const int32_t x = b % kXcds, l = b / kXcds;
+  const int32_t per = g / kXcds, rem = g % kXcds;
+  return x < rem ? x * (per + 1) + l : rem * (per + 1) + (x - rem) * per + l;
+#else
+  (void)g;
+  return b;
+#endif
+}
+
+// Persistent kernels (fixed grid, a multiple of kXcds blocks): the slice
+// groups of kWavesPerBlock slices are split into kXcds contiguous ranges;
+// the blocks of XCD x stride through range x together.
+struct GroupRange {
+  int32_t first, end, stride;
+};
+__device__ __forceinline__ GroupRange xcd_groups(int32_t nslices) {
+  const int32_t ng = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+#if AINV_XCD_REMAP
+  const int32_t x = blockIdx.x % kXcds, nb = gridDim.x / kXcds;
+  return GroupRange{(int32_t)((int64_t)ng * x / kXcds) +
+                        (int32_t)(blockIdx.x / kXcds),
+                    (int32_t)((int64_t)ng * (x + 1) / kXcds), nb};
+#else
+  return GroupRange{(int32_t)blockIdx.x, ng, (int32_t)gridDim.x};
+#endif
+}
+
+int persistent_grid(int32_t nslices) {
+  int g = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (g > kRedBlocks) g = kRedBlocks;
+  if (g < 1) g = 1;
+#if AINV_XCD_REMAP
+  g = (g + kXcds - 1) / kXcds * kXcds;  // kRedBlocks is a multiple of 8
+#endif
+  return g;
+}
 
 template <typename VT> struct Val;
 template <> struct Val<double> {
@@ -120,7 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(
   using A = typename Val<VT>::acc;
   constexpr int G = kSlice / K;  // rows per pass
   const int lane = threadIdx.x & (kSlice - 1);
-  const int32_t s = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int32_t s = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                    (threadIdx.x >> 6);
   if (s >= nslices) return;
   const int64_t base = sptr[s];
   const int c = (K == 1) ? 0 : (lane % K);
@@ -142,6 +197,186 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(
       *yp = (OT)v;
     }
   }
+}
+
+// K values of one x row as 16-byte vectors when the row is 16-byte aligned
+template <typename XT, int K, typename A>
+__device__ __forceinline__ void load_xrow(const XT *__restrict__ p, A *out) {
+  constexpr int B = K * (int)sizeof(XT);
+  if constexpr (B % 16 == 0) {
+    constexpr int per = 16 / (int)sizeof(XT);
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int t = 0; t < B / 16; ++t) {
+      const uint4 w = q[t];
+      XT e[per];
+      __builtin_memcpy(e, &w, 16);
+#pragma unroll
+      for (int u = 0; u < per; ++u) out[t * per + u] = (A)e[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < K; ++u) out[u] = (A)p[u];
+  }
+}
+
+// Entries of one lane's row, walked over the slice's padded width (a
+// wave-uniform trip count: padding entries are (own row, 0) and add
+// nothing), in groups of P entries: the x rows of group g are requested
+// before the column / value words of group g+1, so each wave keeps the
+// gathers of one group and the index loads of the next in flight together
+// instead of waiting for two round trips per group.  P keeps the gathered
+// values of a group within about 32 registers per lane.
+// Waves per SIMD the row kernels are compiled for: at most 4 lets a lane
+// hold a whole group of gathered rows (up to 128 registers).
+#ifndef AINV_ROW_WAVES
+#define AINV_ROW_WAVES 4
+#endif
+#if AINV_ROW_WAVES > 0
+#define AINV_ROW_ATTR \
+  __attribute__((amdgpu_waves_per_eu(RowWaves<K>::v, RowWaves<K>::v)))
+#else
+#define AINV_ROW_ATTR
+#endif
+#ifndef AINV_ROW_PIPE
+#define AINV_ROW_PIPE 1
+#endif
+
+template <typename A, int K> struct RowGroup {
+  static constexpr int raw = 128 / (K * (int)sizeof(A));
+  static constexpr int v = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
+};
+// K >= 32 needs 2K accumulator registers per lane: compile for 2 waves per
+// SIMD (256 registers) instead of spilling.
+template <int K> struct RowWaves {
+  static constexpr int v = K >= 32 ? 2 : AINV_ROW_WAVES;
+};
+
+template <typename VT, typename XT, int K>
+__device__ __forceinline__ void row_acc(int64_t p0, int32_t width,
+                                        const int32_t *__restrict__ col,
+                                        const VT *__restrict__ val,
+                                        const XT *__restrict__ x,
+                                        typename Val<VT>::acc (&acc)[K]) {
+  using A = typename Val<VT>::acc;
+  constexpr int P = RowGroup<A, K>::v;
+  const int32_t nfull = width / P * P;
+  if (nfull > 0) {
+    int32_t ci[P];
+    A vi[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      ci[u] = col[p0 + (int64_t)u * kSlice];
+      vi[u] = Val<VT>::get(val[p0 + (int64_t)u * kSlice]);
+#if AINV_ROW_PIPE
+      // opaque copies: otherwise the compiler merges these loads with the
+      // prefetches of the loop into one load at the top of each iteration,
+      // which undoes the pipelining
+      asm volatile("" : "+v"(ci[u]));
+      asm volatile("" : "+v"(vi[u]));
+#endif
+    }
+    for (int32_t j = 0;;) {
+      A xv[P][K];
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+        load_xrow<XT, K>(x + (int64_t)ci[u] * K, xv[u]);
+      const int32_t jn = j + P;
+      const bool more = jn < nfull;
+      // last group: re-read its own words rather than branch around the load
+      const int64_t q = p0 + (int64_t)(more ? jn : j) * kSlice;
+      int32_t cn[P];
+      VT vn[P];  // converted after the sums: no wait on them before
+#pragma unroll
+      for (int u = 0; u < P; ++u) {
+        cn[u] = col[q + (int64_t)u * kSlice];
+        vn[u] = val[q + (int64_t)u * kSlice];
+      }
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[c] += vi[u] * xv[u][c];
+      if (!more) break;
+      j = jn;
+#pragma unroll
+      for (int u = 0; u < P; ++u) {
+        ci[u] = cn[u];
+        vi[u] = Val<VT>::get(vn[u]);
+      }
+    }
+  }
+  for (int32_t j = nfull; j < width; ++j) {
+    const int64_t q = p0 + (int64_t)j * kSlice;
+    const A v = Val<VT>::get(val[q]);
+    A xv[K];
+    load_xrow<XT, K>(x + (int64_t)col[q] * K, xv);
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] += v * xv[c];
+  }
+}
+
+// Row mapping: lane l of the wave owns row 64 s + l and all K columns.
+template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
+__global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_row(
+    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta) {
+  using A = typename Val<VT>::acc;
+  const int lane = threadIdx.x & (kSlice - 1);
+  // wave-uniform: scalar loads of the slice bounds, scalar loop control
+  const int32_t s = __builtin_amdgcn_readfirstlane(
+      xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6));
+  if (s >= nslices) return;
+  const int64_t base = sptr[s];
+  const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+  A acc[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) acc[u] = 0;
+  row_acc<VT, XT, K>(base + lane, width, col, val, x, acc);
+  const int32_t r = s * kSlice + lane;
+  if (r >= n) return;
+  OT *o = out + (int64_t)r * K;
+  if constexpr (EPI == EPI_SCALE) {
+    const double d = (double)dvec[r];
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (u < k) o[u] = (OT)(d * (double)acc[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      if (u >= k) continue;
+      double v = alpha * (double)acc[u];
+      if (beta != 0.0) v += beta * (double)o[u];
+      o[u] = (OT)v;
+    }
+  }
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int EPI>
+void launch_row(const Hll &a, const XT *x, int k, const DT *d, OT *out,
+                double alpha, double beta, hipStream_t s) {
+  if (a.nslices == 0) return;
+  dim3 grid((a.nslices + kWavesPerBlock - 1) / kWavesPerBlock), block(kBlock);
+  const int64_t *sp = a.sptr.as<int64_t>();
+  const int32_t *cl = a.col.as<int32_t>();
+  const VT *vl = a.val.as<VT>();
+#define AINV_LAUNCH(KK)                                                      \
+  hipLaunchKernelGGL((k_hll_spmm_row<VT, XT, DT, OT, KK, EPI>), grid, block, \
+                     0, s, a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha,  \
+                     beta)
+  if (k <= 2)
+    AINV_LAUNCH(2);
+  else if (k <= 4)
+    AINV_LAUNCH(4);
+  else if (k <= 8)
+    AINV_LAUNCH(8);
+  else if (k <= 16)
+    AINV_LAUNCH(16);
+  else
+    AINV_LAUNCH(32);
+#undef AINV_LAUNCH
+  AINV_HIP_CHECK(hipGetLastError());
 }
 
 template <typename VT, typename XT, typename DT, typename OT, int EPI>
@@ -185,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ rlen, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ x, int k,
-    double *__restrict__ y, const double *__restrict__ w,
+    int64_t ld, double *__restrict__ y, const double *__restrict__ w,
     double *__restrict__ part) {
   constexpr int G = kSlice / K;
   const int lane = threadIdx.x & (kSlice - 1);
@@ -194,8 +429,10 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
   const int g = (K == 1) ? lane : (lane / K);
   const bool col_ok = c < k;
   double d0 = 0.0, d1 = 0.0;
-  for (int32_t s = blockIdx.x * kWavesPerBlock + wave; s < nslices;
-       s += gridDim.x * kWavesPerBlock) {
+  const GroupRange gr = xcd_groups(nslices);
+  for (int32_t grp = gr.first; grp < gr.end; grp += gr.stride) {
+    const int32_t s = grp * kWavesPerBlock + wave;
+    if (s >= nslices) break;
     const int64_t base = sptr[s];
     for (int pass = 0; pass < K; ++pass) {
       const int ri = pass * G + g;
@@ -203,9 +440,9 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
       if (r >= n) break;
       const int32_t len = rlen[r];
       const double yv =
-          row_dot<double, double>(base + ri, len, col, val, x, k, c, col_ok);
+          row_dot<double, double>(base + ri, len, col, val, x, ld, c, col_ok);
       if (!col_ok) continue;
-      const int64_t e = (int64_t)r * k + c;
+      const int64_t e = (int64_t)r * ld + c;
       y[e] = yv;
       if constexpr (NP == 1) {
         d0 += w[e] * yv;
@@ -239,23 +476,390 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
   }
 }
 
+// Row mapping of the fused SpMV + dots (ld = K): each lane keeps K
+// partial sums for its rows and the wave reduces them once at the end.
+template <int K, int NP>
+__global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_row(
+    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ x, int k, double *__restrict__ y,
+    const double *__restrict__ w, double *__restrict__ part) {
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int wave = threadIdx.x >> 6;
+  double d0[K], d1[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) d0[u] = d1[u] = 0.0;
+  const GroupRange gr = xcd_groups(nslices);
+  for (int32_t grp = gr.first; grp < gr.end; grp += gr.stride) {
+    const int32_t s =
+        __builtin_amdgcn_readfirstlane(grp * kWavesPerBlock + wave);
+    if (s >= nslices) break;
+    const int64_t base = sptr[s];
+    const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+    double acc[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) acc[u] = 0.0;
+    row_acc<double, double, K>(base + lane, width, col, val, x, acc);
+    const int32_t r = s * kSlice + lane;
+    if (r >= n) continue;
+    double wr[K];
+    load_xrow<double, K>(w + (int64_t)r * K, wr);
+    double *yo = y + (int64_t)r * K;
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      if (u >= k) continue;
+      yo[u] = acc[u];
+      if constexpr (NP == 1) {
+        d0[u] += wr[u] * acc[u];
+      } else {
+        d0[u] += acc[u] * acc[u];
+        d1[u] += acc[u] * wr[u];
+      }
+    }
+  }
+  __shared__ double red[2][kWavesPerBlock][K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) {
+    double a0 = d0[u], a1 = d1[u];
+    for (int off = 1; off < kSlice; off <<= 1) {
+      a0 += __shfl_xor(a0, off);
+      if constexpr (NP == 2) a1 += __shfl_xor(a1, off);
+    }
+    if (lane == 0) {
+      red[0][wave][u] = a0;
+      red[1][wave][u] = a1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < K && (int)threadIdx.x < k) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+      t0 += red[0][wv][threadIdx.x];
+      t1 += red[1][wv][threadIdx.x];
+    }
+    part[((int64_t)0 * kRedBlocks + blockIdx.x) * kMaxCols + threadIdx.x] = t0;
+    if constexpr (NP == 2)
+      part[((int64_t)1 * kRedBlocks + blockIdx.x) * kMaxCols + threadIdx.x] =
+          t1;
+  }
+}
+
+// Cooperative row mapping: the L lanes of a row group load one x row
+// together, lane (row g, chunk c) the VB bytes at offset c * VB, so the four
+// lanes of a quad read one contiguous 64-byte piece (one cache line) instead
+// of four pieces of four different rows: the L1 looks up about a quarter of
+// the lines the row mapping does per gathered byte.  The column and value
+// words of a slice position are loaded once per wave (coalesced, lane l =
+// row l) and handed to the lanes of each row group by shuffles, which run on
+// the LDS pipe, not the L1.  Lane (g, c) accumulates columns
+// c*CPL .. c*CPL+CPL of rows g, g + R, ..., g + (L-1) R of the slice.
+template <typename XT, int K> struct Coop {
+  static constexpr int RB = K * (int)sizeof(XT);    // bytes of one x row
+  static constexpr int VB = RB < 16 ? RB : 16;      // bytes per lane load
+  static constexpr int L = RB / VB;                 // lanes per row
+  static constexpr int CPL = VB / (int)sizeof(XT);  // columns per lane
+  static constexpr int R = kSlice / L;              // rows per pass
+};
+
+template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
+  if constexpr (sizeof(T) == 2) {
+    return (T)__shfl((int)v, src);
+  } else {
+    return __shfl(v, src);
+  }
+}
+
+template <typename VT, typename XT, int K>
+__device__ __forceinline__ void coop_acc(
+    int64_t p0, int32_t width, int lane, int sub, int rg,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x,
+    typename Val<VT>::acc (&acc)[Coop<XT, K>::L][Coop<XT, K>::CPL]) {
+  using A = typename Val<VT>::acc;
+  using CP = Coop<XT, K>;
+  constexpr int L = CP::L, CPL = CP::CPL, R = CP::R;
+  constexpr int P = RowGroup<A, K>::v;
+  const int32_t nfull = width / P * P;
+  const int64_t pl = p0 + lane;
+  const XT *xs = x + sub * CPL;
+  if (nfull > 0) {
+    int32_t ci[P];
+    VT vi[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u) {
+      ci[u] = col[pl + (int64_t)u * kSlice];
+      vi[u] = val[pl + (int64_t)u * kSlice];
+#if AINV_ROW_PIPE
+      asm volatile("" : "+v"(ci[u]));
+#endif
+    }
+    for (int32_t j = 0;;) {
+      A xv[P][L][CPL];
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+#pragma unroll
+        for (int q = 0; q < L; ++q) {
+          const int32_t c = __shfl(ci[u], q * R + rg);
+          load_xrow<XT, CPL>(xs + (int64_t)c * K, xv[u][q]);
+        }
+      const int32_t jn = j + P;
+      const bool more = jn < nfull;
+      const int64_t qn = pl + (int64_t)(more ? jn : j) * kSlice;
+      int32_t cn[P];
+      VT vn[P];
+#pragma unroll
+      for (int u = 0; u < P; ++u) {
+        cn[u] = col[qn + (int64_t)u * kSlice];
+        vn[u] = val[qn + (int64_t)u * kSlice];
+      }
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+#pragma unroll
+        for (int q = 0; q < L; ++q) {
+          const A v = Val<VT>::get(shfl_raw(vi[u], q * R + rg));
+#pragma unroll
+          for (int c = 0; c < CPL; ++c) acc[q][c] += v * xv[u][q][c];
+        }
+      if (!more) break;
+      j = jn;
+#pragma unroll
+      for (int u = 0; u < P; ++u) {
+        ci[u] = cn[u];
+        vi[u] = vn[u];
+      }
+    }
+  }
+  for (int32_t j = nfull; j < width; ++j) {
+    const int32_t c0 = col[pl + (int64_t)j * kSlice];
+    const VT v0 = val[pl + (int64_t)j * kSlice];
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      const int32_t c = __shfl(c0, q * R + rg);
+      const A v = Val<VT>::get(shfl_raw(v0, q * R + rg));
+      A xv[CPL];
+      load_xrow<XT, CPL>(xs + (int64_t)c * K, xv);
+#pragma unroll
+      for (int u = 0; u < CPL; ++u) acc[q][u] += v * xv[u];
+    }
+  }
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
+__global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_coop(
+    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta) {
+  using A = typename Val<VT>::acc;
+  using CP = Coop<XT, K>;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int sub = lane % CP::L, rg = lane / CP::L;
+  const int32_t s = __builtin_amdgcn_readfirstlane(
+      xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6));
+  if (s >= nslices) return;
+  const int64_t base = sptr[s];
+  const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+  A acc[CP::L][CP::CPL];
+#pragma unroll
+  for (int q = 0; q < CP::L; ++q)
+#pragma unroll
+    for (int u = 0; u < CP::CPL; ++u) acc[q][u] = 0;
+  coop_acc<VT, XT, K>(base, width, lane, sub, rg, col, val, x, acc);
+  const int c0 = sub * CP::CPL;
+#pragma unroll
+  for (int q = 0; q < CP::L; ++q) {
+    const int32_t r = s * kSlice + q * CP::R + rg;
+    if (r >= n) continue;
+    OT *o = out + (int64_t)r * K + c0;
+    if constexpr (EPI == EPI_SCALE) {
+      const double d = (double)dvec[r];
+#pragma unroll
+      for (int u = 0; u < CP::CPL; ++u)
+        if (c0 + u < k) o[u] = (OT)(d * (double)acc[q][u]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < CP::CPL; ++u) {
+        if (c0 + u >= k) continue;
+        double v = alpha * (double)acc[q][u];
+        if (beta != 0.0) v += beta * (double)o[u];
+        o[u] = (OT)v;
+      }
+    }
+  }
+}
+
+// Fused system-matrix product + dots in the cooperative mapping (ld = K).
+// Lanes with the same chunk hold partial sums of the same columns; they are
+// reduced by shuffles across the row groups, then across the waves in LDS.
+template <int K, int NP>
+__global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_coop(
+    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ x, int k, double *__restrict__ y,
+    const double *__restrict__ w, double *__restrict__ part) {
+  using CP = Coop<double, K>;
+  constexpr int L = CP::L, CPL = CP::CPL, R = CP::R;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane % L, rg = lane / L;
+  const int c0 = sub * CPL;
+  double d0[CPL], d1[CPL];
+#pragma unroll
+  for (int u = 0; u < CPL; ++u) d0[u] = d1[u] = 0.0;
+  const GroupRange gr = xcd_groups(nslices);
+  for (int32_t grp = gr.first; grp < gr.end; grp += gr.stride) {
+    const int32_t s =
+        __builtin_amdgcn_readfirstlane(grp * kWavesPerBlock + wave);
+    if (s >= nslices) break;
+    const int64_t base = sptr[s];
+    const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+    double acc[L][CPL];
+#pragma unroll
+    for (int q = 0; q < L; ++q)
+#pragma unroll
+      for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
+    coop_acc<double, double, K>(base, width, lane, sub, rg, col, val, x, acc);
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      const int32_t r = s * kSlice + q * R + rg;
+      if (r >= n) continue;
+      double wr[CPL];
+      load_xrow<double, CPL>(w + (int64_t)r * K + c0, wr);
+      double *yo = y + (int64_t)r * K + c0;
+#pragma unroll
+      for (int u = 0; u < CPL; ++u) {
+        if (c0 + u >= k) continue;
+        yo[u] = acc[q][u];
+        if constexpr (NP == 1) {
+          d0[u] += wr[u] * acc[q][u];
+        } else {
+          d0[u] += acc[q][u] * acc[q][u];
+          d1[u] += acc[q][u] * wr[u];
+        }
+      }
+    }
+  }
+  __shared__ double red[2][kWavesPerBlock][K];
+#pragma unroll
+  for (int u = 0; u < CPL; ++u) {
+    double a0 = d0[u], a1 = d1[u];
+    for (int off = L; off < kSlice; off <<= 1) {
+      a0 += __shfl_xor(a0, off);
+      if constexpr (NP == 2) a1 += __shfl_xor(a1, off);
+    }
+    if (rg == 0) {
+      red[0][wave][c0 + u] = a0;
+      red[1][wave][c0 + u] = a1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < K && (int)threadIdx.x < k) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+      t0 += red[0][wv][threadIdx.x];
+      t1 += red[1][wv][threadIdx.x];
+    }
+    part[((int64_t)0 * kRedBlocks + blockIdx.x) * kMaxCols + threadIdx.x] = t0;
+    if constexpr (NP == 2)
+      part[((int64_t)1 * kRedBlocks + blockIdx.x) * kMaxCols + threadIdx.x] =
+          t1;
+  }
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int EPI>
+void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
+                 double alpha, double beta, hipStream_t s) {
+  if (a.nslices == 0) return;
+  dim3 grid((a.nslices + kWavesPerBlock - 1) / kWavesPerBlock), block(kBlock);
+  const int64_t *sp = a.sptr.as<int64_t>();
+  const int32_t *cl = a.col.as<int32_t>();
+  const VT *vl = a.val.as<VT>();
+#define AINV_LAUNCH(KK)                                                       \
+  hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, EPI>), grid, block, \
+                     0, s, a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha,   \
+                     beta)
+  if (k <= 2)
+    AINV_LAUNCH(2);
+  else if (k <= 4)
+    AINV_LAUNCH(4);
+  else if (k <= 8)
+    AINV_LAUNCH(8);
+  else if (k <= 16)
+    AINV_LAUNCH(16);
+  else
+    AINV_LAUNCH(32);
+#undef AINV_LAUNCH
+  AINV_HIP_CHECK(hipGetLastError());
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int EPI>
+void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
+                 OT *out, double alpha, double beta, hipStream_t s) {
+  if (map == AINV_MAP_COOP)
+    launch_coop<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
+  else
+    launch_row<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
+}
+
+// The row kernels read whole x rows as 16-byte vectors.
+bool aligned16(const void *p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
 }  // namespace
 
-int hll_spmv_dot(const Hll &a, const double *x, double *y, int k,
-                 const double *w, int np, double *part, hipStream_t s) {
+int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
+                 const double *w, int np, double *part, int map,
+                 hipStream_t s) {
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
-  int g = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
-  if (g > kRedBlocks) g = kRedBlocks;
-  if (g < 1) g = 1;
+  const int g = persistent_grid(a.nslices);
   const int64_t *sp = a.sptr.as<int64_t>();
   const int32_t *rl = a.rlen.as<int32_t>();
   const int32_t *cl = a.col.as<int32_t>();
   const double *vl = a.val.as<double>();
+  if ((map == AINV_MAP_ROW || map == AINV_MAP_COOP) && k > 1 && k <= 32) {
+    int K = 2;
+    while (K < k) K <<= 1;
+    if (ld != K || !aligned16(x) || !aligned16(y) || !aligned16(w))
+      throw Error(AINV_ERR_INTERNAL,
+                  "row mapping needs ld = K and 16-byte aligned vectors");
+#define AINV_LR(KK, NPP)                                                     \
+  do {                                                                       \
+    if (map == AINV_MAP_COOP)                                                \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, NPP>), dim3(g),            \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part);                                     \
+    else                                                                     \
+      hipLaunchKernelGGL((k_hll_spmv_dot_row<KK, NPP>), dim3(g),             \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part);                                     \
+  } while (0)
+#define AINV_LRK(NPP)   \
+  if (K == 2)           \
+    AINV_LR(2, NPP);    \
+  else if (K == 4)      \
+    AINV_LR(4, NPP);    \
+  else if (K == 8)      \
+    AINV_LR(8, NPP);    \
+  else if (K == 16)     \
+    AINV_LR(16, NPP);   \
+  else                  \
+    AINV_LR(32, NPP);
+    if (np == 1) {
+      AINV_LRK(1)
+    } else {
+      AINV_LRK(2)
+    }
+#undef AINV_LRK
+#undef AINV_LR
+    AINV_HIP_CHECK(hipGetLastError());
+    return g;
+  }
 #define AINV_LD(KK, NPP)                                                     \
   hipLaunchKernelGGL((k_hll_spmv_dot<KK, NPP>), dim3(g), dim3(kBlock), 0, s, \
-                     a.n, a.nslices, sp, rl, cl, vl, x, k, y, w, part)
+                     a.n, a.nslices, sp, rl, cl, vl, x, k, ld, y, w, part)
 #define AINV_LDK(NPP)        \
   if (k <= 1)                \
     AINV_LD(1, NPP);         \
@@ -282,11 +886,42 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k,
   return g;
 }
 
+// Row mapping needs ld = K (a power of two) and 16-byte aligned x and
+// output.
+static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,
+                    const void *o) {
+  if ((map != AINV_MAP_ROW && map != AINV_MAP_COOP) || k < 2 || k > 32)
+    return false;
+  int K = 2;
+  while (K < k) K <<= 1;
+  return ldx == K && ldo == K && aligned16(x) && aligned16(o);
+}
+
 void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
-                    int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s) {
+                    int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s,
+                    int map) {
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   if (x_is_f32) throw Error(AINV_ERR_INTERNAL, "scale pass reads FP64 x");
   const double *xd = static_cast<const double *>(x);
+  if (use_row(map, k, ldx, ldo, x, out)) {
+    switch (a.prec) {
+      case AINV_PREC_F64:
+        launch_lane<double, double, double, double, EPI_SCALE>(
+            map, a, xd, k, static_cast<const double *>(d), static_cast<double *>(out),
+            1.0, 0.0, s);
+        break;
+      case AINV_PREC_F32:
+        launch_lane<float, double, float, float, EPI_SCALE>(
+            map, a, xd, k, static_cast<const float *>(d), static_cast<float *>(out),
+            1.0, 0.0, s);
+        break;
+      default:
+        launch_lane<uint16_t, double, float, float, EPI_SCALE>(
+            map, a, xd, k, static_cast<const float *>(d), static_cast<float *>(out),
+            1.0, 0.0, s);
+    }
+    return;
+  }
   switch (a.prec) {
     case AINV_PREC_F64:
       launch_k<double, double, double, double, EPI_SCALE>(
@@ -307,8 +942,38 @@ void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
 
 void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, double *y, int64_t ldy, int k,
-                    hipStream_t s) {
+                    hipStream_t s, int map) {
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
+  if (use_row(map, k, ldx, ldy, x, y)) {
+    switch (a.prec) {
+      case AINV_PREC_F64:
+        if (x_is_f32) throw Error(AINV_ERR_INTERNAL, "F64 factor with FP32 t");
+        launch_lane<double, double, double, double, EPI_AXPBY>(
+            map, a, static_cast<const double *>(x), k, (const double *)nullptr, y,
+            alpha, beta, s);
+        break;
+      case AINV_PREC_F32:
+        if (x_is_f32)
+          launch_lane<float, float, float, double, EPI_AXPBY>(
+              map, a, static_cast<const float *>(x), k, (const float *)nullptr, y,
+              alpha, beta, s);
+        else
+          launch_lane<float, double, float, double, EPI_AXPBY>(
+              map, a, static_cast<const double *>(x), k, (const float *)nullptr, y,
+              alpha, beta, s);
+        break;
+      default:
+        if (x_is_f32)
+          launch_lane<uint16_t, float, float, double, EPI_AXPBY>(
+              map, a, static_cast<const float *>(x), k, (const float *)nullptr, y,
+              alpha, beta, s);
+        else
+          launch_lane<uint16_t, double, float, double, EPI_AXPBY>(
+              map, a, static_cast<const double *>(x), k, (const float *)nullptr, y,
+              alpha, beta, s);
+    }
+    return;
+  }
   switch (a.prec) {
     case AINV_PREC_F64:
       if (x_is_f32) throw Error(AINV_ERR_INTERNAL, "F64 factor with FP32 t");

@@ -6,10 +6,15 @@
 #include <atomic>
 #include <cctype>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
 #include "objects.h"
+
+extern "C" int ainv_cseti(ainv_h h, const char *what, int val);
+extern "C" int ainv_csetr(ainv_h h, const char *what, double val);
+extern "C" int ainv_csetc(ainv_h h, const char *what, const char *val);
 
 namespace ainv {
 
@@ -182,6 +187,7 @@ void build_layout(ainv_solver *h) {
   hll_from_csr(h->w, h->prec, h->hw, h->stream);
   hll_from_csr(h->z, h->prec, h->hz, h->stream);
   convert_vector(h->d64.as<double>(), h->n, h->prec, h->dv, h->stream);
+  reset_tuning(h);
   h->generation = next_generation();
   h->have_t = false;
   h->hwt = Hll();
@@ -251,6 +257,29 @@ void copy_settings(const ainv_solver *src, ainv_solver *dst) {
   dst->thresh = src->thresh;
   dst->inv_thresh = src->inv_thresh;
   dst->prec = src->prec;
+  dst->map_mode = src->map_mode;
+}
+
+int map_from_string(const std::string &v) {
+  if (v == "AUTO") return -1;
+  if (v == "SPLIT") return AINV_MAP_SPLIT;
+  if (v == "ROW") return AINV_MAP_ROW;
+  if (v == "COOP") return AINV_MAP_COOP;
+  throw Error(AINV_ERR_ARG,
+              "unknown APPLY_MAP '" + v + "' (AUTO, SPLIT, ROW, COOP)");
+}
+
+const char *map_name(int m) {
+  return m == AINV_MAP_ROW ? "row" : m == AINV_MAP_COOP ? "coop" : "split";
+}
+
+void set_map_mode(ainv_solver *h, int mode) {
+  if (mode < -1 || mode >= kNumMaps)
+    throw Error(AINV_ERR_ARG, "APPLY_MAP out of range");
+  if (mode == h->map_mode) return;
+  h->map_mode = mode;
+  // cached Krylov graphs hold the old kernels
+  if (h->built) h->generation = next_generation();
 }
 
 void copy_csr_dev(const DevCsr &a, DevCsr &b, hipStream_t s) {
@@ -272,16 +301,94 @@ void copy_csr_dev(const DevCsr &a, DevCsr &b, hipStream_t s) {
 
 }  // namespace
 
+// ---- entry points for the serving loop (serve.cpp) ----------------------
+HostCsr canonical_csr_public(int32_t n, const int64_t *irp, const int32_t *ja,
+                             const double *val, int base) {
+  return canonical_csr(n, irp, ja, val, base);
+}
+
+void build_solver(ainv_solver *h, const HostCsr &a) {
+  set_device(h->device);
+  check_params(h);
+  do_build(h, a);
+}
+
+ainv_matrix *make_matrix(const HostCsr &a, int device) {
+  set_device(device);
+  auto *m = new ainv_matrix();
+  try {
+    m->device = device;
+    AINV_HIP_CHECK(hipStreamCreate(&m->stream));
+    m->n = a.n;
+    csr_upload(a, m->csr, m->stream);
+    hll_from_csr(m->csr, AINV_PREC_F64, m->hll, m->stream);
+  } catch (...) {
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    throw;
+  }
+  return m;
+}
+
+// key = value with the setters' semantics: integers through cseti, reals
+// through csetr, anything else through csetc
+void apply_param(ainv_solver *h, const std::string &key,
+                 const std::string &val) {
+  char *end = nullptr;
+  const long iv = std::strtol(val.c_str(), &end, 10);
+  int rc;
+  if (end && *end == '\0' && !val.empty()) {
+    rc = ainv_cseti(h, key.c_str(), (int)iv);
+  } else {
+    end = nullptr;
+    const double dv = std::strtod(val.c_str(), &end);
+    if (end && *end == '\0' && !val.empty())
+      rc = ainv_csetr(h, key.c_str(), dv);
+    else
+      rc = ainv_csetc(h, key.c_str(), val.c_str());
+  }
+  if (rc != AINV_SUCCESS)
+    throw Error(rc, "parameter " + key + "=" + val + ": " + last_error());
+}
+
+int64_t solver_device_bytes(const ainv_solver *h) {
+  int64_t b = 0;
+  b += (int64_t)(h->w.ptr.bytes + h->w.idx.bytes + h->w.val.bytes);
+  b += (int64_t)(h->z.ptr.bytes + h->z.idx.bytes + h->z.val.bytes);
+  b += (int64_t)h->d64.bytes + (int64_t)h->dv.bytes;
+  b += h->hw.bytes() + h->hz.bytes() + h->hwt.bytes() + h->hzt.bytes();
+  return b;
+}
+
 size_t solver_t_bytes(const ainv_solver *h, int k) {
   return (size_t)h->n * k *
          (h->prec != AINV_PREC_F64 ? sizeof(float) : sizeof(double));
 }
 
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
-                       int k, void *t, hipStream_t s) {
+                       int k, int64_t ld, void *t, hipStream_t s, int map) {
   const bool f32t = h->prec != AINV_PREC_F64;
-  hll_spmm_scale(h->hw, h->dv.p, x, false, k, t, k, k, s);
-  hll_spmm_axpby(h->hz, 1.0, t, f32t, k, 0.0, y, k, k, s);
+  hll_spmm_scale(h->hw, h->dv.p, x, false, ld, t, ld, k, s, map);
+  hll_spmm_axpby(h->hz, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map);
+}
+
+int log2_pow2(int K) {
+  int i = 0;
+  while ((1 << i) < K && i < 7) ++i;
+  return i;
+}
+
+int tuned_apply_map(const ainv_solver *h, int K) {
+  if (h->map_mode >= 0) return h->map_mode;
+  const int m = h->apply_map[log2_pow2(K)];
+  return m < 0 ? AINV_MAP_SPLIT : m;
+}
+
+void reset_tuning(ainv_solver *h) {
+  for (int i = 0; i < 8; ++i) {
+    h->apply_map[i] = -1;
+    h->apply_ms[i][0] = h->apply_ms[i][1] = 0.0;
+  }
 }
 
 void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
@@ -304,8 +411,18 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
     first = &h->hzt;
     second = &h->hwt;
   }
-  hll_spmm_scale(*first, h->dv.p, x, false, ldx, h->tbuf.p, k, k, s);
-  hll_spmm_axpby(*second, alpha, h->tbuf.p, f32t, k, beta, y, ldy, k, s);
+  // the tuned mapping needs x, t and y with leading dimension K = pow2 >= k
+  int K = 1;
+  while (K < k) K <<= 1;
+  const int map = (t == 'N' && ldx == K && ldy == K) ? tuned_apply_map(h, K)
+                                                     : AINV_MAP_SPLIT;
+  const int64_t ldt = (map == AINV_MAP_ROW) ? K : k;
+  const size_t need2 = (size_t)h->n * ldt *
+                       (f32t ? sizeof(float) : sizeof(double));
+  if (h->tbuf.bytes < need2) h->tbuf.alloc(need2);
+  hll_spmm_scale(*first, h->dv.p, x, false, ldx, h->tbuf.p, ldt, k, s, map);
+  hll_spmm_axpby(*second, alpha, h->tbuf.p, f32t, ldt, beta, y, ldy, k, s,
+                 map);
 }
 
 }  // namespace ainv
@@ -389,6 +506,8 @@ int ainv_cseti(ainv_h h, const char *what, int val) {
       if (val < AINV_PREC_F64 || val > AINV_PREC_BF16)
         throw Error(AINV_ERR_ARG, "FACTOR_PREC out of range");
       h->prec = val;
+    } else if (w == "APPLY_MAP") {
+      set_map_mode(h, val);
     }
     // other keys go to the base solver, which ignores them
   });
@@ -419,6 +538,8 @@ int ainv_csetc(ainv_h h, const char *what, const char *val) {
       h->alg = a < 0 ? 0 : a;
     } else if (w == "FACTOR_PREC") {
       h->prec = prec_from_string(v);
+    } else if (w == "APPLY_MAP") {
+      set_map_mode(h, map_from_string(v));
     }
   });
 }
@@ -436,6 +557,8 @@ int ainv_geti(ainv_h h, const char *what, int *val) {
       *val = h->inv_fill;
     else if (w == "FACTOR_PREC")
       *val = h->prec;
+    else if (w == "APPLY_MAP")
+      *val = h->map_mode;
     else
       throw Error(AINV_ERR_ARG, "unknown integer key " + w);
   });
@@ -713,6 +836,22 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
            ", \"prec\": \"" + prec_name(h->prec) + "\"" +
            ", \"w\": " + hll_json(h->hw) + ", \"z\": " + hll_json(h->hz);
       if (!h->inv_report.empty()) r += ", \"inverse\": " + h->inv_report;
+      // autotuner: per block width K, ms of the split / row apply, choice
+      std::string tu;
+      for (int i = 0; i < 8; ++i) {
+        if (h->apply_map[i] < 0) continue;
+        char b2[200];
+        std::snprintf(b2, sizeof b2,
+                      "%s{\"K\": %d, \"split_ms\": %.6f, \"row_ms\": %.6f, "
+                      "\"coop_ms\": %.6f, \"map\": \"%s\"}",
+                      tu.empty() ? "" : ", ", 1 << i, h->apply_ms[i][0],
+                      h->apply_ms[i][1], h->apply_ms[i][2],
+                      map_name(h->apply_map[i]));
+        tu += b2;
+      }
+      r += ", \"apply_map\": \"" +
+           std::string(h->map_mode < 0 ? "auto" : map_name(h->map_mode)) +
+           "\", \"tuning\": [" + tu + "]";
     }
     r += "}";
     std::snprintf(buf, len, "%s", r.c_str());
@@ -876,11 +1015,16 @@ int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
     if (h->tbuf.bytes < need) h->tbuf.alloc(need);
     hipStream_t s = h->stream;
     Events e(3 * (size_t)reps);
+    // the mapping the apply would use (forced or tuned; split if neither)
+    const int map = tuned_apply_map(h, k);
+    // one untimed pass first: first touches of the buffers stay out
+    hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, map);
+    hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, map);
     for (int r = 0; r < reps; ++r) {
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r], s));
-      hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s);
+      hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, map);
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 1], s));
-      hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s);
+      hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, map);
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 2], s));
     }
     AINV_HIP_CHECK(hipStreamSynchronize(s));
@@ -901,6 +1045,7 @@ int ainv_mat_profile(ainv_mat_h a, const double *x, double *y, int k, int reps,
       throw Error(AINV_ERR_ARG, "bad profile arguments");
     set_device(a->device);
     Events e(2);
+    hll_spmm_axpby(a->hll, 1.0, x, false, k, 0.0, y, k, k, a->stream);
     AINV_HIP_CHECK(hipEventRecord(e.ev[0], a->stream));
     for (int r = 0; r < reps; ++r)
       hll_spmm_axpby(a->hll, 1.0, x, false, k, 0.0, y, k, k, a->stream);

@@ -20,6 +20,9 @@
 // preconditioner, k) and rebuilt when the preconditioner's factors change.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
 
 #include "objects.h"
 
@@ -78,7 +81,7 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
     const Col cs = col[c];
     for (int64_t r = (int64_t)blockIdx.x * R + rr; r < n;
          r += (int64_t)gridDim.x * R) {
-      const int64_t e = r * k + c;
+      const int64_t e = r * K + c;
       if constexpr (OP == OP_INIT_BICG) {
         acc[0] += v.b[e] * v.b[e];
         acc[1] += v.r[e] * v.r[e];
@@ -165,8 +168,10 @@ __host__ __device__ inline int fin_pairs(int fin) {
 
 __global__ __launch_bounds__(kThreads) void k_fin(
     int fin, int nblk, const double *__restrict__ part, Col *__restrict__ col,
-    int32_t *__restrict__ nactive) {
+    int32_t *__restrict__ nactive, const int32_t *__restrict__ mask) {
   const int c = blockIdx.x;
+  // an admission initializes only the columns it fills
+  if (mask && !mask[c]) return;
   const int np = fin_pairs(fin);
   __shared__ double red[kMaxPairs][kThreads];
   for (int q = 0; q < np; ++q) {
@@ -285,10 +290,40 @@ int vec_op(int32_t n, int k, const Vecs &v, const Col *col, double *part,
 }
 
 void fin(int which, int k, int nblk, const double *part, Col *col,
-         int32_t *nact, hipStream_t s) {
+         int32_t *nact, hipStream_t s, const int32_t *mask = nullptr) {
   hipLaunchKernelGGL(k_fin, dim3(k), dim3(kThreads), 0, s, which, nblk, part,
-                     col, nact);
+                     col, nact, mask);
   AINV_HIP_CHECK(hipGetLastError());
+}
+
+// Slot admission: columns [c0, c0+kc) of the K-column recurrence vectors
+// take the right-hand sides src (n x kc, interleaved) with x0 = 0, so
+// r0 = b and (BiCGSTAB) rtilde = r0.
+__global__ void k_admit(int32_t n, int K, int c0, int kc,
+                        const double *__restrict__ src, double *__restrict__ b,
+                        double *__restrict__ x, double *__restrict__ r,
+                        double *__restrict__ q) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * kc) return;
+  const int64_t row = t / kc;
+  const int j = (int)(t % kc);
+  const int64_t e = row * K + c0 + j;
+  const double v = src[t];
+  b[e] = v;
+  x[e] = 0.0;
+  r[e] = v;
+  if (q) q[e] = v;
+}
+
+// Copy columns [c0, c0+kc) of a K-column vector to dst (n x kc).
+__global__ void k_extract(int32_t n, int K, int c0, int kc,
+                          const double *__restrict__ x,
+                          double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * kc) return;
+  const int64_t row = t / kc;
+  const int j = (int)(t % kc);
+  dst[t] = x[row * K + c0 + j];
 }
 
 }  // namespace
@@ -297,7 +332,9 @@ struct KrylovPlan {
   bool cg = false;
   const ainv_solver *m = nullptr;
   uint64_t gen = 0;
-  int k = 0;
+  int k = 0;       // columns in use
+  int K = 1;       // leading dimension of the workspace, pow2 >= k
+  int apply_map = AINV_MAP_SPLIT, spmv_map = AINV_MAP_SPLIT;
   int32_t n = 0;
   DevBuf work, colbuf, part, nact, tbuf;
   Vecs v{};
@@ -308,6 +345,9 @@ struct KrylovPlan {
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // slot batches: per-column states copied back each step, admission mask
+  Col *pinned_cols = nullptr;  // two slots of kMaxCols states
+  DevBuf mask;
 
   void release_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -320,6 +360,7 @@ struct KrylovPlan {
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
     if (pinned) (void)hipHostFree(pinned);
+    if (pinned_cols) (void)hipHostFree(pinned_cols);
   }
 };
 
@@ -327,14 +368,15 @@ void destroy_plan(KrylovPlan *p) { delete p; }
 
 namespace {
 
-KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k) {
+KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k,
+                     hipStream_t s) {
   for (size_t i = 0; i < a->plans.size(); ++i) {
     KrylovPlan *p = a->plans[i].get();
     if (p->cg == cg && p->m == m && p->k == k) {
       if (m && p->gen != m->generation) {
         p->release_graph();  // factors changed: recapture
-        if (p->tbuf.bytes < solver_t_bytes(m, k))
-          p->tbuf.alloc(solver_t_bytes(m, k));
+        if (p->tbuf.bytes < solver_t_bytes(m, p->K))
+          p->tbuf.alloc(solver_t_bytes(m, p->K));
         p->gen = m->generation;
       }
       if (i != 0) std::rotate(a->plans.begin(), a->plans.begin() + i,
@@ -348,10 +390,14 @@ KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k) {
   p->m = m;
   p->gen = m ? m->generation : 0;
   p->k = k;
+  p->K = 1;
+  while (p->K < k) p->K <<= 1;
   p->n = a->n;
-  const int64_t nk = (int64_t)a->n * k;
+  const int64_t nk = (int64_t)a->n * p->K;
   const int nvec = cg ? 6 : 10;  // x, b + recurrence vectors
   p->work.alloc((size_t)nk * nvec * sizeof(double));
+  // padding columns (k..K) are read by the row-mapped kernels: keep them 0
+  AINV_HIP_CHECK(hipMemsetAsync(p->work.p, 0, p->work.bytes, s));
   double *w0 = p->work.as<double>();
   p->v.x = w0;
   p->v.b = w0 + nk;
@@ -368,7 +414,7 @@ KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k) {
   p->colbuf.alloc(sizeof(Col) * kMaxCols);
   p->part.alloc(sizeof(double) * kMaxPairs * kRedBlocks * kMaxCols);
   p->nact.alloc(sizeof(int32_t));
-  if (m) p->tbuf.alloc(solver_t_bytes(m, k));
+  if (m) p->tbuf.alloc(solver_t_bytes(m, p->K));
   p->col = p->colbuf.as<Col>();
   p->pt = p->part.as<double>();
   p->na = p->nact.as<int32_t>();
@@ -383,26 +429,86 @@ KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k) {
 
 void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
   if (p->m)
-    solver_apply_into(p->m, x, y, p->k, p->tbuf.p, s);
+    solver_apply_into(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map);
   else
-    AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->k * sizeof(double),
+    AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->K * sizeof(double),
                                   hipMemcpyDeviceToDevice, s));
+}
+
+// Autotuner: time the kernel mappings (split, row, coop) of the preconditioner
+// apply and of the fused system-matrix product on this plan's buffers and
+// keep the faster; results are cached on the solver / matrix per K.
+double time_ms(hipStream_t s, const std::function<void()> &f, int reps) {
+  hipEvent_t e0, e1;
+  AINV_HIP_CHECK(hipEventCreate(&e0));
+  AINV_HIP_CHECK(hipEventCreate(&e1));
+  f();  // warm-up
+  AINV_HIP_CHECK(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; ++r) f();
+  AINV_HIP_CHECK(hipEventRecord(e1, s));
+  AINV_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  AINV_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)ms / reps;
+}
+
+void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
+  const int K = p->K;
+  const int li = log2_pow2(K);
+  // a mapping forced on the preconditioner (APPLY_MAP) also holds for the
+  // system-matrix product of the solves it is used in
+  const int forced = m ? m->map_mode : -1;
+  if (K == 1 || p->k < 2 || K > 32) {
+    p->apply_map = p->spmv_map = AINV_MAP_SPLIT;
+    return;
+  }
+  if (forced >= 0) {
+    p->apply_map = p->spmv_map = forced;
+    return;
+  }
+  const Vecs &v = p->v;
+  if (m) {
+    if (m->apply_map[li] < 0) {
+      int best = 0;
+      for (int map = 0; map < kNumMaps; ++map) {
+        m->apply_ms[li][map] = time_ms(s, [&] {
+          solver_apply_into(m, v.b, v.z, p->k, K, p->tbuf.p, s, map);
+        }, 3);
+        if (m->apply_ms[li][map] < m->apply_ms[li][best]) best = map;
+      }
+      m->apply_map[li] = (int8_t)best;
+    }
+    p->apply_map = m->apply_map[li];
+  }
+  if (a->spmv_map[li] < 0) {
+    int best = 0;
+    for (int map = 0; map < kNumMaps; ++map) {
+      a->spmv_ms[li][map] = time_ms(s, [&] {
+        hll_spmv_dot(a->hll, v.b, v.z, p->k, K, v.r, 1, p->pt, map, s);
+      }, 3);
+      if (a->spmv_ms[li][map] < a->spmv_ms[li][best]) best = map;
+    }
+    a->spmv_map[li] = (int8_t)best;
+  }
+  p->spmv_map = a->spmv_map[li];
 }
 
 // One full iteration: the kernel sequence that is captured and replayed.
 void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   const int32_t n = p->n;
-  const int k = p->k;
+  const int k = p->k, K = p->K, sm = p->spmv_map;
   const Vecs &v = p->v;
   if (!p->cg) {
     fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
     vec_op<OP_P_BICG>(n, k, v, p->col, p->pt, s);
     precond(p, v.p, v.f, s);
-    int g = hll_spmv_dot(a->hll, v.f, v.v, k, v.q, 1, p->pt, s);
+    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s);
     fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_S_BICG>(n, k, v, p->col, p->pt, s);
     precond(p, v.s, v.z, s);
-    g = hll_spmv_dot(a->hll, v.z, v.t, k, v.s, 2, p->pt, s);
+    g = hll_spmv_dot(a->hll, v.z, v.t, k, K, v.s, 2, p->pt, sm, s);
     fin(FIN_OMEGA, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_BICG>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
@@ -411,14 +517,168 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     int g = vec_op<OP_DOT_RZ>(n, k, v, p->col, p->pt, s);
     fin(FIN_RHO_CG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_P_CG>(n, k, v, p->col, p->pt, s);
-    g = hll_spmv_dot(a->hll, v.p, v.q, k, v.p, 1, p->pt, s);
+    g = hll_spmv_dot(a->hll, v.p, v.q, k, K, v.p, 1, p->pt, sm, s);
     fin(FIN_ALPHA_CG, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_CG>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_CG, k, g, p->pt, p->col, p->na, s);
   }
 }
 
+// user vectors (ld k) <-> workspace columns (ld K); padding columns zero
+__global__ void k_pack(int32_t n, int k, int K, const double *__restrict__ src,
+                       double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * K) return;
+  const int64_t row = t / K;
+  const int c = (int)(t % K);
+  dst[t] = c < k ? src[row * k + c] : 0.0;
+}
+
+__global__ void k_unpack(int32_t n, int k, int K, const double *__restrict__ src,
+                         double *__restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * k) return;
+  const int64_t row = t / k;
+  const int c = (int)(t % k);
+  dst[t] = src[row * K + c];
+}
+
+void capture(KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
+  if (p->exec) return;
+  tune_plan(p, a, const_cast<ainv_solver *>(p->m), s);
+  AINV_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue_iteration(p, a, s);
+  } catch (...) {
+    hipGraph_t g0 = nullptr;
+    (void)hipStreamEndCapture(s, &g0);
+    if (g0) (void)hipGraphDestroy(g0);
+    throw;
+  }
+  AINV_HIP_CHECK(hipStreamEndCapture(s, &p->graph));
+  AINV_HIP_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+}
+
 }  // namespace
+
+// ---------------------------------------------------------- slot batches
+struct SlotBatch {
+  ainv_matrix *a = nullptr;
+  KrylovPlan *p = nullptr;
+  hipStream_t s = nullptr;
+  int K = 0;
+  int64_t steps = 0;
+  std::vector<int32_t> hmask;
+};
+
+SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, bool cg, int K,
+                      hipStream_t s) {
+  if (K < 1 || K > kMaxCols) throw Error(AINV_ERR_ARG, "slots must be 1..64");
+  if (m && m->n != a->n)
+    throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
+  KrylovPlan *p = get_plan(a, cg, m, K, s);
+  if (!p->pinned_cols)
+    AINV_HIP_CHECK(hipHostMalloc((void **)&p->pinned_cols,
+                                 2 * kMaxCols * sizeof(Col),
+                                 hipHostMallocDefault));
+  if (!p->mask.p) p->mask.alloc(kMaxCols * sizeof(int32_t));
+  // every slot empty: inactive columns are skipped by every kernel
+  std::vector<Col> init(kMaxCols);
+  for (auto &c0 : init) c0 = Col{};
+  AINV_HIP_CHECK(hipMemcpyAsync(p->col, init.data(), sizeof(Col) * kMaxCols,
+                                hipMemcpyHostToDevice, s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->work.p, 0, p->work.bytes, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  capture(p, a, s);
+  auto *b = new SlotBatch();
+  b->a = a;
+  b->p = p;
+  b->s = s;
+  b->K = p->K;  // slot count and leading dimension (pow2 >= K)
+  b->hmask.assign(kMaxCols, 0);
+  return b;
+}
+
+void batch_close(SlotBatch *b) {
+  if (b && b->s) (void)hipStreamSynchronize(b->s);
+  delete b;
+}
+
+void batch_admit(SlotBatch *b, int c0, int kc, const double *src, double eps) {
+  KrylovPlan *p = b->p;
+  if (c0 < 0 || kc < 1 || c0 + kc > b->K)
+    throw Error(AINV_ERR_ARG, "admission outside the slot range");
+  const int32_t n = p->n;
+  const int64_t tot = (int64_t)n * kc;
+  hipLaunchKernelGGL(k_admit, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                     0, b->s, n, b->K, c0, kc, src, p->v.b, p->v.x, p->v.r,
+                     p->cg ? nullptr : p->v.q);
+  AINV_HIP_CHECK(hipGetLastError());
+  std::vector<Col> cs(kc);
+  for (auto &c1 : cs) {
+    c1 = Col{};
+    c1.active = 1;  // so that the init pass computes its dots
+    c1.eps = eps;
+  }
+  std::fill(b->hmask.begin(), b->hmask.end(), 0);
+  for (int j = 0; j < kc; ++j) b->hmask[c0 + j] = 1;
+  AINV_HIP_CHECK(hipMemcpyAsync(p->col + c0, cs.data(), sizeof(Col) * kc,
+                                hipMemcpyHostToDevice, b->s));
+  AINV_HIP_CHECK(hipMemcpyAsync(p->mask.p, b->hmask.data(),
+                                sizeof(int32_t) * kMaxCols,
+                                hipMemcpyHostToDevice, b->s));
+  int g;
+  if (!p->cg) {
+    g = vec_op<OP_INIT_BICG>(n, b->K, p->v, p->col, p->pt, b->s);
+    fin(FIN_INIT, b->K, g, p->pt, p->col, p->na, b->s, p->mask.as<int32_t>());
+  } else {
+    g = vec_op<OP_INIT_CG>(n, b->K, p->v, p->col, p->pt, b->s);
+    fin(FIN_INIT_CG, b->K, g, p->pt, p->col, p->na, b->s,
+        p->mask.as<int32_t>());
+  }
+  // the host buffers above must outlive the copies
+  AINV_HIP_CHECK(hipStreamSynchronize(b->s));
+}
+
+void batch_step(SlotBatch *b) {
+  KrylovPlan *p = b->p;
+  AINV_HIP_CHECK(hipGraphLaunch(p->exec, b->s));
+  const int slot = (int)(b->steps & 1);
+  AINV_HIP_CHECK(hipMemcpyAsync(p->pinned_cols + slot * kMaxCols, p->col,
+                                sizeof(Col) * b->K, hipMemcpyDeviceToHost,
+                                b->s));
+  AINV_HIP_CHECK(hipEventRecord(p->ev[slot], b->s));
+  ++b->steps;
+}
+
+// States after the step before the most recent one (no wait on the most
+// recent, which may still run); with wait_last, after the most recent.
+int64_t batch_next_step(const SlotBatch *b) { return b->steps; }
+
+int64_t batch_poll(SlotBatch *b, bool wait_last, std::vector<SlotState> &out) {
+  KrylovPlan *p = b->p;
+  out.assign(b->K, SlotState{});
+  int64_t which = wait_last ? b->steps - 1 : b->steps - 2;
+  if (which < 0) return -1;
+  const int slot = (int)(which & 1);
+  AINV_HIP_CHECK(hipEventSynchronize(p->ev[slot]));
+  const Col *c = p->pinned_cols + slot * kMaxCols;
+  for (int j = 0; j < b->K; ++j) {
+    out[j].active = c[j].active;
+    out[j].iter = c[j].iter;
+    out[j].err = c[j].err;
+    out[j].valid = true;
+  }
+  return which;
+}
+
+void batch_extract(SlotBatch *b, int c0, int kc, double *dst) {
+  const int64_t tot = (int64_t)b->p->n * kc;
+  hipLaunchKernelGGL(k_extract, dim3((unsigned)((tot + 255) / 256)), dim3(256),
+                     0, b->s, b->p->n, b->K, c0, kc, b->p->v.x, dst);
+  AINV_HIP_CHECK(hipGetLastError());
+}
 
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                   const double *b, double *x, int k, double eps, int itmax,
@@ -431,11 +691,14 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   if (!cg && method != "BICGSTAB")
     throw Error(AINV_ERR_ARG, "unknown Krylov method " + method +
                                   " (BICGSTAB, CG available)");
-  KrylovPlan *p = get_plan(a, cg, m, k);
-  const size_t vb = (size_t)n * k * sizeof(double);
+  KrylovPlan *p = get_plan(a, cg, m, k, s);
+  const int K = p->K;
+  const size_t vb = (size_t)n * K * sizeof(double);
   const Vecs &v = p->v;
-  AINV_HIP_CHECK(hipMemcpyAsync(v.b, b, vb, hipMemcpyDeviceToDevice, s));
-  AINV_HIP_CHECK(hipMemcpyAsync(v.x, x, vb, hipMemcpyDeviceToDevice, s));
+  const unsigned gp = (unsigned)(((int64_t)n * K + 255) / 256);
+  hipLaunchKernelGGL(k_pack, dim3(gp), dim3(256), 0, s, n, k, K, b, v.b);
+  hipLaunchKernelGGL(k_pack, dim3(gp), dim3(256), 0, s, n, k, K, x, v.x);
+  AINV_HIP_CHECK(hipGetLastError());
   // column state: all active for the init pass, eps per column
   std::vector<Col> init(kMaxCols);
   for (auto &c0 : init) {
@@ -448,7 +711,7 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
   // r = b - A x
   AINV_HIP_CHECK(hipMemcpyAsync(v.r, v.b, vb, hipMemcpyDeviceToDevice, s));
-  hll_spmm_axpby(a->hll, -1.0, v.x, false, k, 1.0, v.r, k, k, s);
+  hll_spmm_axpby(a->hll, -1.0, v.x, false, K, 1.0, v.r, K, k, s);
   int g;
   if (!cg) {
     AINV_HIP_CHECK(hipMemcpyAsync(v.q, v.r, vb, hipMemcpyDeviceToDevice, s));
@@ -462,19 +725,10 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                                 hipMemcpyDeviceToHost, s));
   AINV_HIP_CHECK(hipStreamSynchronize(s));
   const bool eager = (flags & AINV_KRYLOV_EAGER) != 0;
-  if (p->pinned[0] > 0 && itmax > 0 && !eager && !p->exec) {
-    AINV_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      enqueue_iteration(p, a, s);
-    } catch (...) {
-      hipGraph_t g0 = nullptr;
-      (void)hipStreamEndCapture(s, &g0);
-      if (g0) (void)hipGraphDestroy(g0);
-      throw;
-    }
-    AINV_HIP_CHECK(hipStreamEndCapture(s, &p->graph));
-    AINV_HIP_CHECK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
-  }
+  // eager and captured runs use the same kernel mappings, so that they
+  // compute the same sums in the same order
+  if (p->pinned[0] > 0 && itmax > 0) tune_plan(p, a, m, s);
+  if (p->pinned[0] > 0 && itmax > 0 && !eager) capture(p, a, s);
   if (p->pinned[0] > 0) {
     // launch iteration it, then look at the count of iteration it - 1
     for (int it = 1; it <= itmax; ++it) {
@@ -492,7 +746,9 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
       }
     }
   }
-  AINV_HIP_CHECK(hipMemcpyAsync(x, v.x, vb, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)(((int64_t)n * k + 255) / 256)),
+                     dim3(256), 0, s, n, k, K, v.x, x);
+  AINV_HIP_CHECK(hipGetLastError());
   std::vector<Col> hc(k);
   AINV_HIP_CHECK(hipMemcpyAsync(hc.data(), p->col, sizeof(Col) * k,
                                 hipMemcpyDeviceToHost, s));

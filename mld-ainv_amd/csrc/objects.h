@@ -29,6 +29,9 @@ struct ainv_solver {
   double thresh = 0.0;
   double inv_thresh = 0.0;
   int prec = AINV_PREC_F64;
+  // kernel mapping of the apply: -1 autotuned, else AINV_MAP_SPLIT / ROW /
+  // COOP
+  int map_mode = -1;
 
   // ---- device / stream
   int device = 0;
@@ -48,6 +51,10 @@ struct ainv_solver {
   ainv::DevBuf tbuf;   // intermediate t = d o (W x)
   ainv::DevBuf stage_x, stage_y;  // host-pointer staging
 
+  // ---- autotuned kernel mapping per block width K = 2^i (-1: not tuned)
+  int8_t apply_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  double apply_ms[8][3] = {};  // split, row, coop
+
   // ---- build report
   double t_factor = 0, t_inverse = 0, t_layout = 0, t_total = 0;
   std::string inv_report;
@@ -59,6 +66,8 @@ struct ainv_matrix {
   int32_t n = 0;
   ainv::DevCsr csr;
   ainv::Hll hll;
+  int8_t spmv_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  double spmv_ms[8][3] = {};
   // Krylov workspaces and captured iteration graphs, most recent first
   std::vector<std::unique_ptr<ainv::KrylovPlan, ainv::PlanDeleter>> plans;
 };
@@ -74,8 +83,38 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
 // of n*k words of the factor's intermediate type: no allocation, so it may
 // be captured in a graph.
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
-                       int k, void *t, hipStream_t s);
+                       int k, int64_t ld, void *t, hipStream_t s,
+                       int map = 0);
+// Index of K = 2^i and the mapping the tuner chose for it (split if none).
+int log2_pow2(int K);
+int tuned_apply_map(const ainv_solver *h, int K);
+void reset_tuning(ainv_solver *h);
 size_t solver_t_bytes(const ainv_solver *h, int k);
+
+// Continuous batching over K column slots of one (matrix, preconditioner):
+// admit right-hand sides into free slots between iterations, step one
+// captured iteration at a time, read the per-column states one step behind
+// the device, extract finished columns.
+struct SlotBatch;
+struct SlotState {
+  bool valid = false;
+  int32_t active = 0, iter = 0;
+  double err = 0.0;
+};
+SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, bool cg, int K,
+                      hipStream_t s);
+void batch_close(SlotBatch *b);
+// src: device, n x kc interleaved; x0 = 0
+void batch_admit(SlotBatch *b, int c0, int kc, const double *src, double eps);
+void batch_step(SlotBatch *b);
+// Index of the next step batch_step will launch (0 for the first).
+int64_t batch_next_step(const SlotBatch *b);
+// Fills out with the states after step `which` and returns `which` (-1 and
+// no valid state before the first step).  Columns admitted at or after that
+// step still show the states of their slots' previous occupants.
+int64_t batch_poll(SlotBatch *b, bool wait_last, std::vector<SlotState> &out);
+// dst: device, n x kc interleaved
+void batch_extract(SlotBatch *b, int c0, int kc, double *dst);
 
 // flags: AINV_KRYLOV_EAGER disables graph capture (used to check replays)
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,

@@ -1,0 +1,509 @@
+// Serving loop: solves on many matrices arrive as requests (1..64
+// right-hand sides each); each matrix's factors are built on first touch and
+// stay resident on the device under an HBM budget with LRU eviction of idle
+// matrices.  Worker threads (one HIP stream each, so batches on different
+// matrices overlap on the device) take the matrix with the oldest pending
+// request and run a slot batch on it: requests are admitted into free
+// column slots, one captured Krylov iteration advances every active column,
+// and a column that converges frees its slot for the next pending request
+// of the same matrix between two iterations (continuous batching).
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "objects.h"
+
+extern "C" int ainv_free(ainv_h h);
+extern "C" int ainv_mat_free(ainv_mat_h a);
+
+namespace ainv {
+// defined in capi.cpp
+HostCsr canonical_csr_public(int32_t n, const int64_t *irp, const int32_t *ja,
+                             const double *val, int base);
+void build_solver(ainv_solver *h, const HostCsr &a);
+ainv_matrix *make_matrix(const HostCsr &a, int device);
+void apply_param(ainv_solver *h, const std::string &key,
+                 const std::string &val);
+int64_t solver_device_bytes(const ainv_solver *h);
+}  // namespace ainv
+
+namespace {
+
+using namespace ainv;
+
+struct Request {
+  int64_t id = 0;
+  int32_t mat = 0;
+  int k = 1;
+  double eps = 1e-8;
+  int itmax = 1000;
+  std::vector<double> b, x;
+  std::vector<int32_t> iters;
+  std::vector<double> errs;
+  double t_submit = 0, t_start = 0, t_done = 0;
+  bool done = false;
+  int status = AINV_SUCCESS;
+  std::string msg;
+};
+
+struct ServedMatrix {
+  int32_t id = 0;
+  HostCsr a;
+  std::vector<std::pair<std::string, std::string>> params;
+  int kind = AINV_KIND_INVK;
+  bool cg = false;
+  ainv_matrix *A = nullptr;
+  ainv_solver *M = nullptr;
+  int64_t bytes = 0;
+  uint64_t last_use = 0;
+  bool busy = false;
+  std::deque<int64_t> pending;
+  int builds = 0;
+};
+
+}  // namespace
+
+struct ainv_server {
+  int device = 0;
+  int64_t budget = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::unordered_map<int32_t, std::unique_ptr<ServedMatrix>> mats;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+  int32_t next_mat = 1;
+  int64_t next_req = 1;
+  uint64_t clock = 0;
+  int64_t resident = 0;
+  bool stop = false;
+  std::vector<std::thread> workers;
+  // statistics
+  int64_t builds = 0, evictions = 0, completed = 0, admitted_midflight = 0;
+  int64_t steps = 0, batches = 0;
+  double build_seconds = 0;
+  int64_t peak_resident = 0;
+};
+
+namespace {
+
+int64_t plan_bytes(int32_t n, int K, bool cg) {
+  return (int64_t)n * K * (cg ? 6 : 10) * 8 + (int64_t)n * K * 8;
+}
+
+void evict_locked(ainv_server *S, ServedMatrix *m) {
+  // caller holds the lock and m is idle
+  if (m->M) ainv_free(m->M);
+  if (m->A) ainv_mat_free(m->A);
+  m->M = nullptr;
+  m->A = nullptr;
+  S->resident -= m->bytes;
+  m->bytes = 0;
+  S->evictions++;
+}
+
+// Make m resident, evicting least-recently used idle matrices while the
+// budget would be exceeded.  Called with m busy (owned by this worker).
+void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
+  if (m->A && m->M) return;
+  // estimate: A (value + column) and two factors of about 4x the nonzeros
+  const int64_t est = m->a.nnz() * 12 * 9 + plan_bytes(m->a.n, K, m->cg);
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (S->resident + est > S->budget) {
+      ServedMatrix *victim = nullptr;
+      for (auto &kv : S->mats) {
+        ServedMatrix *c = kv.second.get();
+        if (c == m || c->busy || !c->A) continue;
+        if (!victim || c->last_use < victim->last_use) victim = c;
+      }
+      if (!victim) break;  // nothing idle to evict: go over budget
+      evict_locked(S, victim);
+    }
+  }
+  const double t0 = now_seconds();
+  ainv_solver *M = nullptr;
+  ainv_matrix *A = nullptr;
+  try {
+    A = make_matrix(m->a, S->device);
+    M = new ainv_solver();
+    M->kind = m->kind;
+    M->device = S->device;
+    for (auto &kv : m->params) apply_param(M, kv.first, kv.second);
+    build_solver(M, m->a);
+  } catch (...) {
+    if (M) ainv_free(M);
+    if (A) ainv_mat_free(A);
+    throw;
+  }
+  const double dt = now_seconds() - t0;
+  std::unique_lock<std::mutex> lk(S->mu);
+  m->A = A;
+  m->M = M;
+  m->bytes = A->hll.bytes() + (int64_t)(A->csr.ptr.bytes + A->csr.idx.bytes +
+                                        A->csr.val.bytes) +
+             solver_device_bytes(M) + plan_bytes(m->a.n, K, m->cg);
+  S->resident += m->bytes;
+  S->peak_resident = std::max(S->peak_resident, S->resident);
+  S->builds++;
+  S->build_seconds += dt;
+  m->builds++;
+}
+
+void finish_request(ainv_server *S, Request *r, int status,
+                    const std::string &msg) {
+  std::unique_lock<std::mutex> lk(S->mu);
+  r->status = status;
+  r->msg = msg;
+  r->t_done = now_seconds();
+  r->done = true;
+  S->completed++;
+  S->cv_done.notify_all();
+}
+
+struct SlotUse {
+  int64_t req = -1;
+  int c0 = 0, kc = 0;
+  int64_t first_step = 0;  // first step that runs with this request's columns
+};
+
+void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
+                  DevBuf &stage) {
+  int K = 1;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    int cols = 0;
+    for (int64_t id : m->pending) cols += S->reqs[id]->k;
+    cols = std::min(cols, kMaxCols);
+    while (K < cols) K <<= 1;
+  }
+  ensure_resident(S, m, K);
+  const int32_t n = m->a.n;
+  const size_t need = (size_t)n * kMaxCols * sizeof(double);
+  if (stage.bytes < need) stage.alloc(need);
+  SlotBatch *b = batch_open(m->A, m->M, m->cg, K, s);
+  std::vector<SlotUse> uses;  // admitted requests
+  std::vector<int> owner(K, -1);
+  std::vector<SlotState> st;
+  bool first = true;
+  try {
+    for (;;) {
+      // admit whole requests into free contiguous slot runs, FIFO
+      std::vector<Request *> adm;
+      std::vector<int> adm_c0;
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        while (!m->pending.empty()) {
+          Request *r = S->reqs[m->pending.front()].get();
+          int c0 = -1;
+          for (int c = 0; c + r->k <= K && c0 < 0; ++c) {
+            bool free = true;
+            for (int j = 0; j < r->k && free; ++j) free = owner[c + j] < 0;
+            if (free) c0 = c;
+          }
+          if (c0 < 0) break;
+          m->pending.pop_front();
+          for (int j = 0; j < r->k; ++j) owner[c0 + j] = (int)uses.size();
+          uses.push_back(SlotUse{r->id, c0, r->k, batch_next_step(b)});
+          adm.push_back(r);
+          adm_c0.push_back(c0);
+          if (!first) S->admitted_midflight++;
+        }
+      }
+      for (size_t t = 0; t < adm.size(); ++t) {
+        Request *r = adm[t];
+        r->t_start = now_seconds();
+        AINV_HIP_CHECK(hipMemcpyAsync(stage.p, r->b.data(),
+                                      r->b.size() * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+        batch_admit(b, adm_c0[t], r->k, stage.as<double>(), r->eps);
+      }
+      first = false;
+      bool any = false;
+      for (int c = 0; c < K; ++c) any |= owner[c] >= 0;
+      if (!any) break;
+      batch_step(b);
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->steps++;
+      }
+      const int64_t seen = batch_poll(b, false, st);
+      if (seen >= 0) {
+        // complete requests whose columns all stopped (or hit itmax); a
+        // snapshot older than a request's admission shows the previous
+        // occupants of its slots and says nothing about it
+        for (size_t u = 0; u < uses.size(); ++u) {
+          SlotUse &su = uses[u];
+          if (su.req < 0 || seen < su.first_step) continue;
+          Request *r;
+          {
+            std::unique_lock<std::mutex> lk(S->mu);
+            r = S->reqs[su.req].get();
+          }
+          bool alldone = true;
+          for (int j = 0; j < su.kc; ++j) {
+            const SlotState &x = st[su.c0 + j];
+            if (x.active && x.iter < r->itmax) alldone = false;
+          }
+          if (!alldone) continue;
+          batch_extract(b, su.c0, su.kc, stage.as<double>());
+          r->x.resize((size_t)n * su.kc);
+          AINV_HIP_CHECK(hipMemcpyAsync(r->x.data(), stage.p,
+                                        r->x.size() * sizeof(double),
+                                        hipMemcpyDeviceToHost, s));
+          AINV_HIP_CHECK(hipStreamSynchronize(s));
+          r->iters.resize(su.kc);
+          r->errs.resize(su.kc);
+          for (int j = 0; j < su.kc; ++j) {
+            r->iters[j] = st[su.c0 + j].iter;
+            r->errs[j] = st[su.c0 + j].err;
+            owner[su.c0 + j] = -1;
+          }
+          su.req = -1;
+          finish_request(S, r, AINV_SUCCESS, "");
+        }
+      }
+    }
+  } catch (const std::exception &e) {
+    batch_close(b);
+    std::string msg = e.what();
+    for (auto &su : uses)
+      if (su.req >= 0) {
+        Request *r;
+        {
+          std::unique_lock<std::mutex> lk(S->mu);
+          r = S->reqs[su.req].get();
+        }
+        finish_request(S, r, AINV_ERR_INTERNAL, msg);
+      }
+    throw;
+  }
+  batch_close(b);
+  std::unique_lock<std::mutex> lk(S->mu);
+  S->batches++;
+}
+
+void worker_main(ainv_server *S) {
+  hipStream_t s = nullptr;
+  if (hipSetDevice(S->device) != hipSuccess || hipStreamCreate(&s) != hipSuccess)
+    return;
+  DevBuf stage;
+  for (;;) {
+    ServedMatrix *m = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      for (;;) {
+        if (S->stop) {
+          lk.unlock();
+          stage.release();
+          (void)hipStreamDestroy(s);
+          return;
+        }
+        double oldest = 0;
+        for (auto &kv : S->mats) {
+          ServedMatrix *c = kv.second.get();
+          if (c->busy || c->pending.empty()) continue;
+          const double t = S->reqs[c->pending.front()]->t_submit;
+          if (!m || t < oldest) {
+            m = c;
+            oldest = t;
+          }
+        }
+        if (m) break;
+        S->cv_work.wait(lk);
+      }
+      m->busy = true;
+    }
+    try {
+      serve_matrix(S, m, s, stage);
+    } catch (const std::exception &e) {
+      // fail the requests still queued for this matrix
+      std::vector<Request *> fail;
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        while (!m->pending.empty()) {
+          fail.push_back(S->reqs[m->pending.front()].get());
+          m->pending.pop_front();
+        }
+      }
+      for (Request *r : fail) finish_request(S, r, AINV_ERR_INTERNAL, e.what());
+    }
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      m->busy = false;
+      m->last_use = ++S->clock;
+    }
+    S->cv_work.notify_all();
+  }
+}
+
+std::vector<std::pair<std::string, std::string>> parse_params(const char *p) {
+  std::vector<std::pair<std::string, std::string>> out;
+  if (!p) return out;
+  std::string s(p);
+  size_t i = 0;
+  while (i < s.size()) {
+    size_t j = s.find(';', i);
+    if (j == std::string::npos) j = s.size();
+    std::string item = s.substr(i, j - i);
+    size_t e = item.find('=');
+    if (e != std::string::npos) {
+      auto trim = [](std::string t) {
+        size_t b = t.find_first_not_of(" \t"), f = t.find_last_not_of(" \t");
+        return b == std::string::npos ? std::string() : t.substr(b, f - b + 1);
+      };
+      out.emplace_back(trim(item.substr(0, e)), trim(item.substr(e + 1)));
+    }
+    i = j + 1;
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ainv_server_create(int device, int64_t hbm_budget, int workers,
+                       ainv_server_h *out) {
+  return guard([&] {
+    if (!out) throw Error(AINV_ERR_ARG, "NULL output handle");
+    if (workers < 1 || workers > 16)
+      throw Error(AINV_ERR_ARG, "workers must be 1..16");
+    if (hbm_budget <= 0) throw Error(AINV_ERR_ARG, "budget must be positive");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 ||
+        device >= count)
+      throw Error(AINV_ERR_HIP, "no such HIP device");
+    auto *S = new ainv_server();
+    S->device = device;
+    S->budget = hbm_budget;
+    for (int w = 0; w < workers; ++w) S->workers.emplace_back(worker_main, S);
+    *out = S;
+  });
+}
+
+int ainv_server_add_matrix(ainv_server_h S, int32_t n, const int64_t *irp,
+                           const int32_t *ja, const double *val,
+                           int index_base, int kind, const char *params,
+                           int32_t *mat_id) {
+  return guard([&] {
+    if (!S || !mat_id) throw Error(AINV_ERR_ARG, "NULL argument");
+    if (kind != AINV_KIND_AINV && kind != AINV_KIND_INVK &&
+        kind != AINV_KIND_INVT)
+      throw Error(AINV_ERR_ARG, "unknown solver kind");
+    auto m = std::make_unique<ServedMatrix>();
+    m->a = canonical_csr_public(n, irp, ja, val, index_base);
+    m->kind = kind;
+    for (auto &kv : parse_params(params)) {
+      std::string k = kv.first;
+      for (char &c : k) c = (char)std::toupper((unsigned char)c);
+      if (k == "METHOD") {
+        std::string v = kv.second;
+        for (char &c : v) c = (char)std::toupper((unsigned char)c);
+        if (v != "CG" && v != "BICGSTAB")
+          throw Error(AINV_ERR_ARG, "METHOD must be BICGSTAB or CG");
+        m->cg = (v == "CG");
+      } else {
+        m->params.emplace_back(k, kv.second);
+      }
+    }
+    std::unique_lock<std::mutex> lk(S->mu);
+    m->id = S->next_mat++;
+    *mat_id = m->id;
+    S->mats[m->id] = std::move(m);
+  });
+}
+
+int ainv_server_submit(ainv_server_h S, int32_t mat_id, int k, const double *b,
+                       double eps, int itmax, int64_t *req_id) {
+  return guard([&] {
+    if (!S || !b || !req_id) throw Error(AINV_ERR_ARG, "NULL argument");
+    if (k < 1 || k > kMaxCols) throw Error(AINV_ERR_ARG, "k must be 1..64");
+    if (itmax < 1) throw Error(AINV_ERR_ARG, "itmax must be >= 1");
+    auto r = std::make_unique<Request>();
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->mats.find(mat_id);
+    if (it == S->mats.end()) throw Error(AINV_ERR_ARG, "unknown matrix id");
+    const int32_t n = it->second->a.n;
+    r->b.assign(b, b + (size_t)n * k);
+    r->id = S->next_req++;
+    r->mat = mat_id;
+    r->k = k;
+    r->eps = eps;
+    r->itmax = itmax;
+    r->t_submit = now_seconds();
+    *req_id = r->id;
+    it->second->pending.push_back(r->id);
+    S->reqs[r->id] = std::move(r);
+    S->cv_work.notify_one();
+  });
+}
+
+int ainv_server_wait(ainv_server_h S, int64_t req_id, double *x,
+                     int32_t *iters, double *errs, double *latency_ms) {
+  int code = AINV_SUCCESS;
+  int rc = guard([&] {
+    if (!S) throw Error(AINV_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(req_id);
+    if (it == S->reqs.end()) throw Error(AINV_ERR_ARG, "unknown request id");
+    Request *r = it->second.get();
+    S->cv_done.wait(lk, [&] { return r->done; });
+    if (r->status != AINV_SUCCESS) {
+      code = r->status;
+      std::string msg = r->msg;
+      S->reqs.erase(it);
+      throw Error(code, "request failed: " + msg);
+    }
+    if (x) std::memcpy(x, r->x.data(), r->x.size() * sizeof(double));
+    if (iters) std::memcpy(iters, r->iters.data(), r->k * sizeof(int32_t));
+    if (errs) std::memcpy(errs, r->errs.data(), r->k * sizeof(double));
+    if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
+    S->reqs.erase(it);
+  });
+  return rc;
+}
+
+int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
+  return guard([&] {
+    if (!S || !buf || len == 0) throw Error(AINV_ERR_ARG, "NULL argument");
+    std::unique_lock<std::mutex> lk(S->mu);
+    int resident_mats = 0;
+    for (auto &kv : S->mats) resident_mats += kv.second->A != nullptr;
+    std::snprintf(buf, len,
+                  "{\"matrices\": %zu, \"resident_matrices\": %d, "
+                  "\"resident_bytes\": %lld, \"peak_resident_bytes\": %lld, "
+                  "\"budget_bytes\": %lld, \"builds\": %lld, "
+                  "\"build_seconds\": %.6f, \"evictions\": %lld, "
+                  "\"completed\": %lld, \"batches\": %lld, \"steps\": %lld, "
+                  "\"admitted_midflight\": %lld, \"workers\": %zu}",
+                  S->mats.size(), resident_mats, (long long)S->resident,
+                  (long long)S->peak_resident, (long long)S->budget,
+                  (long long)S->builds, S->build_seconds,
+                  (long long)S->evictions, (long long)S->completed,
+                  (long long)S->batches, (long long)S->steps,
+                  (long long)S->admitted_midflight, S->workers.size());
+  });
+}
+
+int ainv_server_free(ainv_server_h S) {
+  return guard([&] {
+    if (!S) return;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto &t : S->workers) t.join();
+    for (auto &kv : S->mats) {
+      if (kv.second->M) ainv_free(kv.second->M);
+      if (kv.second->A) ainv_mat_free(kv.second->A);
+    }
+    delete S;
+  });
+}
+
+}  // extern "C"

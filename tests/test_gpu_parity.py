@@ -138,6 +138,76 @@ def test_block_apply_matches_columns(gpu, O, k):
         assert np.linalg.norm(y[:, c] - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("prec,tol", [("F64", 1e-12), ("F32", 2e-6),
+                                      ("BF16", 2e-2)])
+@pytest.mark.parametrize("k", [2, 4, 8, 16, 32])
+@pytest.mark.parametrize("mapping", ["ROW", "COOP"])
+def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
+    """The row-per-lane and cooperative kernels (leading dimension a
+    power of two) against the oracle column by column, alpha / beta
+    included, and against the split mapping."""
+    a = gpu.gen_pde3d(10)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC=prec,
+               APPLY_MAP=mapping)
+    assert sv.geti("APPLY_MAP") == {"ROW": 1, "COOP": 2}[mapping]
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    n = a.shape[0]
+    x = gpu.gen_normal(7, (n, k))
+    y0 = gpu.gen_normal(8, (n, k))
+    y = sv.apply(x, y=y0.copy(), alpha=-1.5, beta=0.5)
+    for c in range(k):
+        ref = O.apply(w, z, d, x[:, c], alpha=-1.5, beta=0.5, y=y0[:, c])
+        rel = np.linalg.norm(y[:, c] - ref) / np.linalg.norm(ref)
+        assert rel < tol, (c, rel)
+    sv.set("APPLY_MAP", "SPLIT")
+    ys = sv.apply(x, y=y0.copy(), alpha=-1.5, beta=0.5)
+    assert np.linalg.norm(ys - y) <= 4 * tol * np.linalg.norm(ys)
+
+
+@pytest.mark.parametrize("method,k,mapping", [
+    ("BICGSTAB", 8, "ROW"), ("BICGSTAB", 5, "ROW"), ("CG", 16, "ROW"),
+    ("BICGSTAB", 8, "COOP"), ("BICGSTAB", 3, "COOP"), ("CG", 32, "COOP")])
+def test_row_mapping_krylov(gpu, O, method, k, mapping):
+    """Krylov solves with the row-mapped apply and system-matrix product
+    (a padded leading dimension when k is not a power of two): iteration
+    counts within +-1 of the oracle per column."""
+    if method == "CG":
+        a = gpu.gen_aniso2d(40, 1e-2)
+        sv = build(gpu, "AINV", a, AINV_ALG="SYM-LLK", SUB_FILLIN=8,
+                   SUB_ILUTHRS=5e-2, APPLY_MAP=mapping)
+        w, z, d = O.ainv_bld(a, "SYM-LLK", 8, 5e-2)
+    else:
+        a = gpu.gen_pde3d(12)
+        sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1,
+                   APPLY_MAP=mapping)
+        w, z, d = O.invk_bld(a, 1, 1)
+    n = a.shape[0]
+    B = gpu.gen_normal(11, (n, k))
+    A = gpu.SystemMatrix(a)
+    X, it, err = gpu.krylov(method, A, sv, B, eps=1e-8)
+    for c in range(k):
+        _, it_ref, _ = O.krylov(method, a, w, z, d, B[:, c], 1e-8)
+        assert abs(int(it[c]) - it_ref) <= 1, (c, it[c], it_ref)
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c])
+
+
+def test_autotuner_report(gpu):
+    """AUTO mapping: a block solve tunes the apply once per block width
+    and reports both timings and the choice."""
+    a = gpu.gen_pde3d(16)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(12, (a.shape[0], 8))
+    gpu.krylov("BICGSTAB", A, sv, B, eps=1e-6)
+    rep = sv.report()
+    assert rep["apply_map"] == "auto"
+    (t,) = [e for e in rep["tuning"] if e["K"] == 8]
+    ms = {m: t[m + "_ms"] for m in ("split", "row", "coop")}
+    assert min(ms.values()) > 0
+    assert t["map"] == min(ms, key=ms.get)
+
+
 @pytest.mark.parametrize("prec,tol", [("F32", 2e-6), ("BF16", 2e-2)])
 def test_reduced_precision_factors(gpu, O, prec, tol):
     """FP32 / BF16 factor storage: error against the FP64 apply bounded by

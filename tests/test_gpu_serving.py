@@ -1,0 +1,78 @@
+"""Serving loop on the GPU: each result is the same alone and in the mix
+(within the Krylov tolerance; iteration counts within +-1), requests that
+arrive while others run are admitted into freed column slots mid-batch, and
+an HBM budget smaller than the pool forces LRU evictions without changing
+results."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(mld, a, kind, params, b, eps, method="BICGSTAB"):
+    sv = mld.SOLVERS[kind]()
+    for k, v in params.items():
+        sv.set(k, v)
+    sv.build(a)
+    x, it, err = mld.krylov(method, mld.SystemMatrix(a), sv, b, eps=eps)
+    return x, it
+
+
+def pool(mld):
+    mats = [
+        (mld.gen_pde3d(12), "INVK", dict(SUB_FILLIN=1, INV_FILLIN=1,
+                                          FACTOR_PREC="F32")),
+        (mld.gen_pde3d(9), "INVT", dict(SUB_FILLIN=5, SUB_ILUTHRS=1e-2,
+                                         INV_FILLIN=5, INV_THRESH=1e-2)),
+        (mld.gen_random(4000, seed=8, median=8.0, band=300), "AINV",
+         dict(SUB_FILLIN=8, SUB_ILUTHRS=5e-2, FACTOR_PREC="F32")),
+    ]
+    return mats
+
+
+def test_alone_vs_mix_and_midflight_admission(gpu):
+    mats = pool(gpu)
+    srv = gpu.Server(hbm_budget_bytes=8 << 30, workers=2)
+    ids = [srv.add_matrix(a, kind, **p) for a, kind, p in mats]
+    rng = np.random.default_rng(0)
+    tickets = []
+    for r in range(30):
+        m = r % 3
+        n = mats[m][0].shape[0]
+        k = int(rng.integers(1, 9))
+        b = gpu.gen_normal(100 + r, (n, k))
+        tickets.append((m, b, srv.submit(ids[m], b, eps=1e-9)))
+    for m, b, t in tickets:
+        x, it, err, lat = srv.wait(t)
+        a, kind, p = mats[m]
+        assert np.all(err <= 1e-9)
+        xa, ita = alone(gpu, a, kind, p, b, 1e-9)
+        for c in range(b.shape[1]):
+            assert abs(int(it[c]) - int(ita[c])) <= 1, (m, c, it, ita)
+            tr = np.linalg.norm(b[:, c] - a @ x[:, c]) / np.linalg.norm(b[:, c])
+            assert tr < 1e-7, (m, c, tr)
+            # both iterates have relative residual <= 1e-9; the pool's
+            # condition numbers are below 1.1e3, so they agree to ~1e-6
+            rel = np.linalg.norm(x[:, c] - xa[:, c]) / np.linalg.norm(xa[:, c])
+            assert rel < 2e-5, (m, c, rel)
+        assert lat > 0
+    st = srv.stats()
+    assert st["completed"] == 30
+    assert st["admitted_midflight"] >= 1
+    assert st["builds"] == 3 and st["evictions"] == 0
+    srv.free()
+
+
+def test_lru_eviction_under_budget(gpu):
+    mats = pool(gpu)[:2]
+    srv = gpu.Server(hbm_budget_bytes=1, workers=1)  # room for none: evict
+    ids = [srv.add_matrix(a, kind, **p) for a, kind, p in mats]
+    for rnd in range(3):
+        for m in range(2):
+            n = mats[m][0].shape[0]
+            b = gpu.gen_normal(7 + rnd, (n, 2))
+            x, it, err, _ = srv.wait(srv.submit(ids[m], b, eps=1e-8))
+            assert np.all(err <= 1e-8)
+    st = srv.stats()
+    assert st["evictions"] >= 4 and st["builds"] >= 5
+    srv.free()
