@@ -1,20 +1,26 @@
 """Benchmark: preconditioned solves/s on one MI355X.
 
-Workload (the configuration the metric is quoted on): the 3-D
-convection-diffusion operator on a 160^3 grid (n = 4,096,000, 28.5 M
-nonzeros), INVK preconditioner (ILU(1) factors, inverse fill level 1),
-BiCGSTAB to 1e-8 relative residual in FP64, right-hand side A*1 (first
-column; further columns A*x_c with x_c ~ N(0,1) from a fixed seed), x0 = 0.
+Workload: the 3-D convection-diffusion operator on a 160^3 grid
+(n = 4,096,000, 28.5 M nonzeros), INVK preconditioner (ILU(1) factors,
+inverse fill level 1), BiCGSTAB to 1e-8 relative residual with FP64
+vectors, dots and residuals.  Right-hand sides b_j = A x_j: x_0 = 1 (the
+reference driver's convention), x_j ~ N(0,1) from a fixed seed for j >= 1.
+All right-hand sides are resident in HBM before any timed region.
 
-A step is one batched preconditioned solve of --k right-hand sides with
-FP32-stored factors (FP64 Krylov recurrence, FP64 dot products and
-residuals).  Inputs are resident in HBM before the timed region.  Also
-reported: the plain FP64 single-RHS GPU solve, the roofline of the
-dominant apply kernel (HIP events on the stream it runs on), and the CPU
-baseline (the oracle's BiCGSTAB iterations, single thread, on the same
-matrix and factors).
+A step is one pass of the served form over a queue of --queue right-hand
+sides: FP32-stored factors, --slots column slots of one captured BiCGSTAB
+iteration, a slot refilled from the queue as soon as its column converges
+(continuous batching).  value = right-hand sides solved per second.
+
+Also reported, on the same right-hand sides: the plain FP64 single-RHS
+GPU solve (one right-hand side at a time, FP64 factors) and the speedup
+over it; per-solve latency p50 / p99 in the served form; the roofline of
+the dominant kernel (the first apply pass t = d o (W x) at the served block
+width; its average duration measured with HIP events on the solver's
+stream); the CPU baseline (the oracle's BiCGSTAB, single thread).
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -26,53 +32,100 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM traffic of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_w.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def apply_bytes(lay, k, st):
-    """Algorithmic bytes of the two apply kernels for k columns:
-    factor words (value + int32 column), one row length per row, d, the x
-    read, the t write / read and the y write (beta = 0)."""
+def apply_pass_bytes(lay, k, first):
+    """Algorithmic bytes of one apply pass for k columns (padding
+    excluded): factor words (value + int32 column), one row-extent word per
+    row, and for the first pass d, the FP64 x read and the t write; for the
+    second pass the t read and the FP64 y write (beta = 0).  t is FP32 for
+    FP32 / BF16 factors, FP64 otherwise."""
     n, vb = lay["n"], lay["value_bytes"]
-    sd = 8 if vb == 8 else 4
-    first = lay["nnz_w"] * (vb + 4) + n * 4 + n * sd + k * n * 8 + k * n * st
-    second = lay["nnz_z"] * (vb + 4) + n * 4 + k * n * st + k * n * 8
-    return first, second
+    st = 8 if vb == 8 else 4
+    if first:
+        return lay["nnz_w"] * (vb + 4) + n * 4 + n * st + k * n * 8 + k * n * st
+    return lay["nnz_z"] * (vb + 4) + n * 4 + k * n * st + k * n * 8
 
 
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--grid", type=int, default=160)
-    p.add_argument("--k", type=int, default=32)
+    p.add_argument("--queue", type=int, default=192)
+    p.add_argument("--slots", type=int, default=32)
+    p.add_argument("--single", type=int, default=6,
+                   help="right-hand sides 1..single solved one at a time "
+                        "in FP64 (the baseline sample)")
     p.add_argument("--eps", type=float, default=1e-8)
+    p.add_argument("--prec", default="F32")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-iters", type=int, default=4)
+    p.add_argument("--apply-only", action="store_true",
+                   help="profiling mode: build, then only the two apply "
+                        "kernels at the served block width (and the copy "
+                        "kernel that calibrates the traffic counters)")
+    p.add_argument("--map", default="AUTO",
+                   help="APPLY_MAP for --apply-only (AUTO, SPLIT, ROW, COOP)")
     args = p.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the "
-              "path shards by row blocks with no communication in the "
-              "apply, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): every "
+              "GPU box of this setup has a single GPU; DESIGN.md describes "
+              "how the path shards", file=sys.stderr)
         sys.exit(2)
 
     import importlib
     m = importlib.import_module("mld-ainv_amd")
     m.init(0)
-    N, k = args.grid, args.k
+    lib = m.lib()
+    N, M, slots = args.grid, args.queue, args.slots
+    copy_gbs, _ = m.stream_copy_gbs(1 << 30, 10)
     t0 = time.time()
     a = m.gen_pde3d(N)
     n = a.shape[0]
-    log(f"matrix {N}^3: n={n} nnz={a.nnz} ({time.time() - t0:.1f}s)")
-    X = m.gen_normal(5, (n, k))
-    X[:, 0] = 1.0
-    B = np.ascontiguousarray(np.column_stack([a @ X[:, c] for c in range(k)]))
-    b1 = np.ascontiguousarray(B[:, 0])
+    log(f"matrix {N}^3: n={n} nnz={a.nnz} ({time.time() - t0:.1f}s); "
+        f"copy kernel {copy_gbs:.0f} GB/s")
+    if args.apply_only:
+        sv = m.MldDInvkSolver()
+        sv.set("SUB_FILLIN", 1)
+        sv.set("INV_FILLIN", 1)
+        sv.set("FACTOR_PREC", args.prec)
+        sv.set("APPLY_MAP", args.map)
+        sv.build(a)
+        lay = sv.layout()
+        ms_w, ms_z = profile_apply(m, sv, n, slots, 10)
+        bw = apply_pass_bytes(lay, slots, True)
+        print(json.dumps({"apply_only": True, "prec": args.prec,
+                          "map": args.map, "k": slots, "ms_w": ms_w,
+                          "ms_z": ms_z, "bytes_w": bw,
+                          "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
+        return
     A = m.SystemMatrix(a)
+
+    # ---- right-hand sides b_j = A x_j, resident on the device
+    t0 = time.time()
+    X = m.gen_normal(5, (M, n))
+    X[0] = 1.0
+    dXt = m.DeviceArray.from_numpy(X)
+    del X
+    dB = m.DeviceArray((M, n))
+    for j in range(M):
+        off = j * n * 8
+        m._ck(lib.ainv_mat_spmm(A._h, 1.0, C.c_void_p(dXt.ptr.value + off),
+                                0.0, C.c_void_p(dB.ptr.value + off), 1, 1, 1,
+                                None))
+    m.synchronize()
+    dXt.free()
+    log(f"{M} right-hand sides on the device ({time.time() - t0:.1f}s)")
+
     sv = m.MldDInvkSolver()
     sv.set("SUB_FILLIN", 1)
     sv.set("INV_FILLIN", 1)
@@ -80,76 +133,104 @@ def main():
     t0 = time.time()
     sv.build(a)
     t_build = time.time() - t0
-    rep = sv.report()
-    log(f"INVK build {t_build:.2f}s  report {json.dumps(rep)}")
+    log(f"INVK build {t_build:.2f}s")
 
-    dB = m.DeviceArray.from_numpy(B)
-    dX = m.DeviceArray((n, k))
-    db1 = m.DeviceArray.from_numpy(b1)
+    # ---- plain FP64 single-RHS GPU solves of the first right-hand sides
     dx1 = m.DeviceArray((n,))
+    Bh = dB.to_numpy() if args.single > 0 else None
 
-    def solve(bvec, xvec, kk):
-        xvec.zero()
-        x, it, err = m.krylov("BICGSTAB", A, sv, bvec, xvec, eps=args.eps,
+    def single(j):
+        db1 = m.DeviceArray.from_numpy(Bh[j])
+        dx1.zero()
+        m.synchronize()
+        ts = time.time()
+        _, it, err = m.krylov("BICGSTAB", A, sv, db1, dx1, eps=args.eps,
                               itmax=2000)
-        return it, err
+        m.synchronize()
+        dt = time.time() - ts
+        db1.free()
+        return dt, int(it[0]), float(err[0])
 
-    # ---- plain FP64 single-RHS GPU solve (the comparison point)
-    solve(db1, dx1, 1)
-    t0 = time.time()
-    it64, err64 = solve(db1, dx1, 1)
-    t_fp64 = time.time() - t0
-    x1 = dx1.to_numpy()
-    true64 = float(np.linalg.norm(b1 - a @ x1) / np.linalg.norm(b1))
+    single(0)  # warm-up (plan, graph capture)
+    t64, it64 = [], []
+    # the sample skips b_0 = A*1, which needs about 1.6x the iterations of
+    # the N(0,1) right-hand sides that make up the rest of the queue
+    for j in range(1, args.single + 1):
+        dt, it, err = single(j)
+        t64.append(dt)
+        it64.append(it)
+    x64 = dx1.to_numpy()
+    true64 = float(np.linalg.norm(Bh[args.single] - a @ x64) /
+                   np.linalg.norm(Bh[args.single]))
     lay64 = sv.layout()
     ms_w64, ms_z64 = profile_apply(m, sv, n, 1, 20)
-    log(f"FP64 1-RHS: {it64[0]} its, {t_fp64 * 1e3:.1f} ms, true res "
-        f"{true64:.2e}; apply kernels {ms_w64:.3f}+{ms_z64:.3f} ms")
+    gbs64 = ((apply_pass_bytes(lay64, 1, True) +
+              apply_pass_bytes(lay64, 1, False)) /
+             ((ms_w64 + ms_z64) * 1e-3) / 1e9)
+    fp64_rate = len(t64) / sum(t64)
+    log(f"FP64 single: {fp64_rate:.2f} solves/s, iterations {it64}, "
+        f"true residual {true64:.2e}, apply {ms_w64:.3f}+{ms_z64:.3f} ms "
+        f"({gbs64:.0f} GB/s)")
 
-    # ---- FP32 factors, batched k RHS (the timed workload)
-    sv.cnv("F32")
+    # ---- served form with reduced-precision factors (the timed workload)
+    sv.cnv(args.prec)
     lay = sv.layout()
+    dX = m.DeviceArray((M, n))
+    warm = m.DeviceArray.from_numpy(Bh[:slots]) if Bh is not None else dB
     for _ in range(args.warmup):
-        solve(dB, dX, k)
+        m.krylov_stream("BICGSTAB", A, sv, warm, slots=slots, eps=args.eps,
+                        itmax=2000)
     m.synchronize()
-    lat = []
-    iters = None
+    lat, iters, errs = [], None, None
     t_start = time.time()
     for _ in range(args.steps):
-        ts = time.time()
-        iters, errs = solve(dB, dX, k)
-        m.synchronize()
-        lat.append(time.time() - ts)
+        _, iters, errs, l = m.krylov_stream("BICGSTAB", A, sv, dB, dX,
+                                            slots=slots, eps=args.eps,
+                                            itmax=2000)
+        lat.extend(l.tolist())
     m.synchronize()
     elapsed = time.time() - t_start
+    rate = args.steps * M / elapsed
     Xs = dX.to_numpy()
-    true = [float(np.linalg.norm(B[:, c] - a @ Xs[:, c]) /
-                  np.linalg.norm(B[:, c])) for c in range(k)]
-    solves_per_s = args.steps * k / elapsed
-    # single-RHS FP32 for the iteration delta against FP64
-    solve(db1, dx1, 1)
-    it32, _ = solve(db1, dx1, 1)
+    true = max(float(np.linalg.norm(Bh[j] - a @ Xs[j]) / np.linalg.norm(Bh[j]))
+               for j in range(0, M, max(1, M // 16)))
+    rep = sv.report()
+    tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
+    kmap = tuned[0]["map"] if tuned else "split"
+    # same right-hand sides solved singly at reduced precision: iterations
+    it_red = [int(v) for v in iters[1:args.single + 1]]
+    log(f"served {args.prec} slots={slots}: {rate:.2f} solves/s, "
+        f"{elapsed / args.steps:.2f} s/step, iterations mean "
+        f"{float(np.mean(iters)):.1f} max {int(np.max(iters))}, tuning {tuned}")
 
-    ms_w, ms_z = profile_apply(m, sv, n, k, 10)
-    bw, bz = apply_bytes(lay, k, 4)
+    ms_w, ms_z = profile_apply(m, sv, n, slots, 10)
+    bw = apply_pass_bytes(lay, slots, True)
     gbs_w = bw / (ms_w * 1e-3) / 1e9
-    gbs_apply = (bw + bz) / ((ms_w + ms_z) * 1e-3) / 1e9
-    bw64, bz64 = apply_bytes(lay64, 1, 8)
-    gbs64 = (bw64 + bz64) / ((ms_w64 + ms_z64) * 1e-3) / 1e9
-    log(f"FP32 k={k}: {elapsed / args.steps * 1e3:.1f} ms/step, "
-        f"{solves_per_s:.1f} solves/s, iters {list(iters)}, apply W "
-        f"{ms_w:.3f} ms ({gbs_w:.0f} GB/s) Z {ms_z:.3f} ms")
+    kernel = {"split": "k_hll_spmm", "row": "k_hll_spmm_row",
+              "coop": "k_hll_spmm_coop"}[kmap]
+    vt = {"F64": "double", "F32": "float", "BF16": "unsigned short"}[args.prec]
+    tt = "double" if args.prec == "F64" else "float"
+    kname = f"{kernel}<{vt}, double, {tt}, {tt}, {slots}, 0>"
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == kname and tr.get("grid") == N:
+            traffic = tr.get("bytes_per_launch")
+    log(f"apply at k={slots}: W {ms_w:.3f} ms ({gbs_w:.0f} GB/s), "
+        f"Z {ms_z:.3f} ms")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sv, a, b1, args.cpu_iters, int(it64[0]))
+        cpu = cpu_baseline(sv, a, Bh[0], args.cpu_iters,
+                           float(np.mean(it64)))
         log(f"cpu baseline {cpu}")
 
-    lat_ms = np.array(lat) * 1e3
+    lat_ms = np.array(lat)
     out = {
         "metric": "preconditioned solves/s + apply HBM GB/s (% roofline); "
                   "p50/p99 solve latency",
-        "value": solves_per_s,
+        "value": rate,
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -161,53 +242,61 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"pde3d-{N}^3 convection-diffusion, INVK(1,1), "
-                        f"BiCGSTAB eps={args.eps:g}, FP32-stored factors, "
-                        f"FP64 Krylov, batch of {k} RHS per solve step",
-            "n": n, "nnz": int(a.nnz), "rhs_per_step": k,
-            "parallelism": "1 GPU",
+            "workload": f"pde3d {N}^3 convection-diffusion, INVK(1,1), "
+                        f"BiCGSTAB rtol {args.eps:g}; served form: queue of "
+                        f"{M} right-hand sides through {slots} slots, "
+                        f"{args.prec}-stored factors, FP64 Krylov",
+            "n": n, "nnz": int(a.nnz), "queue": M, "slots": slots,
+            "factor_prec": args.prec, "parallelism": "1 GPU",
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
-                       "p99": float(np.percentile(lat_ms, 99))},
-        "iterations": {"fp32_batch": [int(v) for v in iters],
-                       "fp32_single": int(it32[0]),
-                       "fp64_single": int(it64[0])},
-        "true_residual_max": max(true),
+                       "p99": float(np.percentile(lat_ms, 99)),
+                       "note": "admission to completion in the served form, "
+                               "queue fully loaded"},
+        "iterations": {"served_mean": float(np.mean(iters)),
+                       "served_max": int(np.max(iters)),
+                       "served_sample": it_red,
+                       "fp64_single_sample": it64,
+                       "sample": f"right-hand sides 1..{args.single}"},
+        "true_residual_max": true,
         "fp64_single_rhs": {
-            "solves_per_s": 1.0 / t_fp64, "ms": t_fp64 * 1e3,
-            "iterations": int(it64[0]), "true_residual": true64,
+            "solves_per_s": fp64_rate, "ms_mean": 1e3 * float(np.mean(t64)),
+            "iterations": it64, "true_residual": true64,
             "apply_GBs": gbs64, "apply_roofline_frac": gbs64 / HBM_PEAK_GBS,
         },
-        "speedup_vs_fp64_single": solves_per_s * t_fp64,
+        "speedup_vs_fp64_single": rate / fp64_rate,
         "roofline": {
-            "kernel": "k_hll_spmm (first apply pass: t = d o (W x), FP32 "
-                      f"factors, k={k})",
             "bound": "hbm", "achieved": gbs_w, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": gbs_w / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": gbs_w / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": kname,
             "bytes_per_launch": bw, "ms_per_launch": ms_w,
-            "apply_both_kernels_GBs": gbs_apply,
+            "achievable_copy_GBs": copy_gbs,
+            "second_pass_ms": ms_z,
+            "second_pass_GBs": apply_pass_bytes(lay, slots, False) /
+            (ms_z * 1e-3) / 1e9,
         },
-        "build": {"seconds": t_build, "report": rep},
+        "build": {"seconds": t_build},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
 def profile_apply(m, sv, n, k, reps):
-    import ctypes as C
     x = m.DeviceArray.from_numpy(m.gen_normal(11, (n, k)) if k > 1
                                  else m.gen_normal(11, (n,)))
     y = m.DeviceArray((n, k) if k > 1 else (n,))
     a, b = C.c_double(), C.c_double()
     m._ck(m.lib().ainv_apply_profile(sv._h, x.ptr, y.ptr, k, reps,
                                      C.byref(a), C.byref(b)))
+    x.free()
+    y.free()
     return a.value, b.value
 
 
 def cpu_baseline(sv, a, b, iters, gpu_iters):
     """Oracle BiCGSTAB (single thread) for `iters` iterations on the same
-    160^3 system with the same FP64 INVK factors, extrapolated to one full
-    solve at the GPU's iteration count."""
+    system with the same FP64 INVK factors, extrapolated to one solve at the
+    FP64 GPU path's mean iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sv.cnv("F64")
@@ -219,8 +308,9 @@ def cpu_baseline(sv, a, b, iters, gpu_iters):
     return {"value": 1.0 / (per_iter * gpu_iters), "unit": "solves/s",
             "cores": 1, "kind": "port",
             "sample": f"{it} BiCGSTAB iterations of the oracle on the same "
-                      f"matrix and FP64 factors ({dt:.1f} s), extrapolated "
-                      f"to {gpu_iters} iterations per solve",
+                      f"matrix, right-hand side A*1 and FP64 factors "
+                      f"({dt:.1f} s), extrapolated to {gpu_iters:.1f} "
+                      f"iterations per solve",
             "seconds_per_iteration": per_iter}
 
 

@@ -177,6 +177,20 @@ int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
                 double *x, int k, double eps, int itmax, int istop,
                 int32_t *iter, double *err, int flags, void *stream);
 
+/* Served form of the same solve (no counterpart in the reference, whose
+ * Krylov driver takes one right-hand side at a time): a device-resident
+ * queue of nrhs right-hand sides b (n-vectors one after another) is solved
+ * through `slots` (1..64) column slots of one captured BiCGSTAB / CG
+ * iteration; a slot is refilled from the queue between two iterations as
+ * soon as its column stops (continuous batching).  x (device, nrhs
+ * n-vectors) gets the solutions, x0 = 0.  iter, err, lat_ms are host arrays
+ * of nrhs: iterations, final ||r||/||b|| and milliseconds from admission to
+ * completion (lat_ms may be NULL).  istop = 2 semantics as above. */
+int ainv_krylov_stream(const char *method, ainv_mat_h a, ainv_h prec,
+                       const double *b, double *x, int nrhs, int slots,
+                       double eps, int itmax, int32_t *iter, double *err,
+                       double *lat_ms, void *stream);
+
 /* ---- synthetic matrices of the benchmark configurations (host CSR).
  * Call with NULL arrays to get n and nnz, then with allocated arrays. ---- */
 /* 3-D convection-diffusion -div(grad u) + b.grad u, b = (1,1,1)/sqrt(3),

@@ -82,6 +82,8 @@ def lib():
         "ainv_mat_free": [vp],
         "ainv_mat_spmm": [vp, C.c_double, vp, C.c_double, vp, C.c_int,
                           C.c_int64, C.c_int64, vp],
+        "ainv_krylov_stream": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_int,
+                               C.c_double, C.c_int, vp, vp, vp, vp],
         "ainv_krylov": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_double,
                         C.c_int, C.c_int, i32p, dp, C.c_int, vp],
         "ainv_gen_pde3d": [C.c_int32, C.c_int, i32p, i64p, i64p, i32p, dp],
@@ -512,6 +514,29 @@ def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2,
     if not host:
         synchronize()
     return x, iters, errs
+
+
+def krylov_stream(method, a, prec, b, x=None, slots=32, eps=1e-8,
+                  itmax=1000):
+    """Served form: solve a device-resident queue of right-hand sides
+    (DeviceArray b of shape (nrhs, n), one right-hand side per row) through
+    `slots` continuously refilled column slots.  Returns (x, iters[nrhs],
+    errs[nrhs], latency_ms[nrhs]) with x a DeviceArray like b."""
+    if not isinstance(b, DeviceArray) or len(b.shape) != 2:
+        raise AinvError(ERR_ARG, "krylov_stream takes a (nrhs, n) DeviceArray")
+    nrhs = b.shape[0]
+    if x is None:
+        x = DeviceArray(b.shape)
+    iters = np.zeros(nrhs, dtype=np.int32)
+    errs = np.zeros(nrhs, dtype=np.float64)
+    lat = np.zeros(nrhs, dtype=np.float64)
+    _ck(lib().ainv_krylov_stream(
+        _b(method), a._h, prec._h if prec else None, b.ptr, x.ptr, nrhs,
+        slots, eps, itmax, iters.ctypes.data_as(C.c_void_p),
+        errs.ctypes.data_as(C.c_void_p), lat.ctypes.data_as(C.c_void_p),
+        None))
+    synchronize()
+    return x, iters, errs, lat
 
 
 # --------------------------------------------------------------- generators

@@ -242,8 +242,11 @@ __device__ __forceinline__ void load_xrow(const XT *__restrict__ p, A *out) {
 #define AINV_ROW_PIPE 1
 #endif
 
+#ifndef AINV_ROW_GROUP_BYTES
+#define AINV_ROW_GROUP_BYTES 128
+#endif
 template <typename A, int K> struct RowGroup {
-  static constexpr int raw = 128 / (K * (int)sizeof(A));
+  static constexpr int raw = AINV_ROW_GROUP_BYTES / (K * (int)sizeof(A));
   static constexpr int v = raw < 1 ? 1 : (raw > 8 ? 8 : raw);
 };
 // K >= 32 needs 2K accumulator registers per lane: compile for 2 waves per

@@ -938,6 +938,28 @@ int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
   });
 }
 
+int ainv_krylov_stream(const char *method, ainv_mat_h a, ainv_h prec,
+                       const double *b, double *x, int nrhs, int slots,
+                       double eps, int itmax, int32_t *iter, double *err,
+                       double *lat_ms, void *stream) {
+  return guard([&] {
+    if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
+    if (nrhs > 0 && (!b || !x || !iter || !err))
+      throw Error(AINV_ERR_ARG, "NULL argument");
+    if (itmax < 0) throw Error(AINV_ERR_ARG, "itmax must be >= 0");
+    const std::string m = upper_trim(method);
+    set_device(a->device);
+    std::unique_lock<std::mutex> lk;
+    if (prec) {
+      lk = std::unique_lock<std::mutex>(prec->mu);
+      if (!prec->built) throw Error(AINV_ERR_STATE, "preconditioner not built");
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : a->stream;
+    krylov_stream(m, a, prec, b, x, nrhs, slots, eps, itmax, iter, err, lat_ms,
+                  s);
+  });
+}
+
 // ---- device plumbing -----------------------------------------------------
 
 int ainv_dev_alloc(size_t bytes, void **p) {

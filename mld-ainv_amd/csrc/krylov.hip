@@ -296,23 +296,57 @@ void fin(int which, int k, int nblk, const double *part, Col *col,
   AINV_HIP_CHECK(hipGetLastError());
 }
 
-// Slot admission: columns [c0, c0+kc) of the K-column recurrence vectors
-// take the right-hand sides src (n x kc, interleaved) with x0 = 0, so
-// r0 = b and (BiCGSTAB) rtilde = r0.
-__global__ void k_admit(int32_t n, int K, int c0, int kc,
-                        const double *__restrict__ src, double *__restrict__ b,
+// Slot admission.  Entry j of the list fills slot slot[j] with the
+// right-hand side src[j][row * stride[j]], x0 = 0, so r0 = b and
+// (BiCGSTAB) rtilde = r0: one thread per row writes every admitted slot of
+// its row, so each row's cache lines are written once per admission round.
+struct AdmitList {
+  int cnt;
+  int slot[kMaxCols];
+  int stride[kMaxCols];
+  const double *src[kMaxCols];
+};
+
+__global__ void k_admit(int32_t n, int K, AdmitList L, double *__restrict__ b,
                         double *__restrict__ x, double *__restrict__ r,
                         double *__restrict__ q) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n * kc) return;
-  const int64_t row = t / kc;
-  const int j = (int)(t % kc);
-  const int64_t e = row * K + c0 + j;
-  const double v = src[t];
-  b[e] = v;
-  x[e] = 0.0;
-  r[e] = v;
-  if (q) q[e] = v;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int j = 0; j < L.cnt; ++j) {
+    const double v = L.src[j][row * L.stride[j]];
+    const int64_t e = row * K + L.slot[j];
+    b[e] = v;
+    x[e] = 0.0;
+    r[e] = v;
+    if (q) q[e] = v;
+  }
+}
+
+// With x0 = 0 every dot of the init step of an admitted column is ||b||^2:
+// block partial sums of it for entry blockIdx.y, written as the np pairs
+// the init step reads (fixed-order tree, as the other reductions).
+__global__ __launch_bounds__(kThreads) void k_admit_norms(
+    int32_t n, AdmitList L, int np, double *__restrict__ part) {
+  const int j = blockIdx.y;
+  const double *src = L.src[j];
+  const int64_t st = L.stride[j];
+  double acc = 0.0;
+  for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < n;
+       row += (int64_t)gridDim.x * kThreads) {
+    const double v = src[row * st];
+    acc += v * v;
+  }
+  __shared__ double red[kThreads];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kThreads / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < np; ++q)
+      part[((int64_t)q * kRedBlocks + blockIdx.x) * kMaxCols + L.slot[j]] =
+          red[0];
 }
 
 // Copy columns [c0, c0+kc) of a K-column vector to dst (n x kc).
@@ -605,40 +639,70 @@ void batch_close(SlotBatch *b) {
   delete b;
 }
 
-void batch_admit(SlotBatch *b, int c0, int kc, const double *src, double eps) {
+namespace {
+// Writes the admitted columns, their column states and admission mask, and
+// runs the init step (bn, err, active; BiCGSTAB rho) for them only.
+void admit(SlotBatch *b, const AdmitList &L, double eps) {
   KrylovPlan *p = b->p;
-  if (c0 < 0 || kc < 1 || c0 + kc > b->K)
-    throw Error(AINV_ERR_ARG, "admission outside the slot range");
   const int32_t n = p->n;
-  const int64_t tot = (int64_t)n * kc;
-  hipLaunchKernelGGL(k_admit, dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                     0, b->s, n, b->K, c0, kc, src, p->v.b, p->v.x, p->v.r,
+  if (L.cnt == 0) return;
+  hipLaunchKernelGGL(k_admit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     b->s, n, b->K, L, p->v.b, p->v.x, p->v.r,
                      p->cg ? nullptr : p->v.q);
   AINV_HIP_CHECK(hipGetLastError());
-  std::vector<Col> cs(kc);
-  for (auto &c1 : cs) {
-    c1 = Col{};
-    c1.active = 1;  // so that the init pass computes its dots
-    c1.eps = eps;
-  }
+  const int g = vec_blocks(n, 1);
+  const int fin_id = p->cg ? FIN_INIT_CG : FIN_INIT;
+  hipLaunchKernelGGL(k_admit_norms, dim3(g, L.cnt), dim3(kThreads), 0, b->s,
+                     n, L, fin_pairs(fin_id), p->pt);
+  AINV_HIP_CHECK(hipGetLastError());
+  Col c1 = Col{};
+  c1.active = 1;
+  c1.eps = eps;
+  std::vector<Col> cs(L.cnt, c1);
   std::fill(b->hmask.begin(), b->hmask.end(), 0);
-  for (int j = 0; j < kc; ++j) b->hmask[c0 + j] = 1;
-  AINV_HIP_CHECK(hipMemcpyAsync(p->col + c0, cs.data(), sizeof(Col) * kc,
-                                hipMemcpyHostToDevice, b->s));
+  for (int j = 0; j < L.cnt; ++j) {
+    b->hmask[L.slot[j]] = 1;
+    AINV_HIP_CHECK(hipMemcpyAsync(p->col + L.slot[j], &cs[j], sizeof(Col),
+                                  hipMemcpyHostToDevice, b->s));
+  }
   AINV_HIP_CHECK(hipMemcpyAsync(p->mask.p, b->hmask.data(),
                                 sizeof(int32_t) * kMaxCols,
                                 hipMemcpyHostToDevice, b->s));
-  int g;
-  if (!p->cg) {
-    g = vec_op<OP_INIT_BICG>(n, b->K, p->v, p->col, p->pt, b->s);
-    fin(FIN_INIT, b->K, g, p->pt, p->col, p->na, b->s, p->mask.as<int32_t>());
-  } else {
-    g = vec_op<OP_INIT_CG>(n, b->K, p->v, p->col, p->pt, b->s);
-    fin(FIN_INIT_CG, b->K, g, p->pt, p->col, p->na, b->s,
-        p->mask.as<int32_t>());
-  }
+  fin(fin_id, b->K, g, p->pt, p->col, p->na, b->s, p->mask.as<int32_t>());
   // the host buffers above must outlive the copies
   AINV_HIP_CHECK(hipStreamSynchronize(b->s));
+}
+}  // namespace
+
+void batch_admit(SlotBatch *b, int c0, int kc, const double *src, double eps) {
+  if (c0 < 0 || kc < 1 || c0 + kc > b->K)
+    throw Error(AINV_ERR_ARG, "admission outside the slot range");
+  AdmitList L{};
+  L.cnt = kc;
+  for (int j = 0; j < kc; ++j) {
+    L.slot[j] = c0 + j;
+    L.stride[j] = kc;
+    L.src[j] = src + j;
+  }
+  admit(b, L, eps);
+}
+
+void batch_admit_cols(SlotBatch *b, const std::vector<int> &slots,
+                      const std::vector<const double *> &srcs, double eps) {
+  if (slots.size() != srcs.size())
+    throw Error(AINV_ERR_INTERNAL, "admission lists differ in length");
+  if ((int)slots.size() > b->K)
+    throw Error(AINV_ERR_ARG, "more admissions than slots");
+  AdmitList L{};
+  L.cnt = (int)slots.size();
+  for (int j = 0; j < L.cnt; ++j) {
+    if (slots[j] < 0 || slots[j] >= b->K)
+      throw Error(AINV_ERR_ARG, "admission outside the slot range");
+    L.slot[j] = slots[j];
+    L.stride[j] = 1;
+    L.src[j] = srcs[j];
+  }
+  admit(b, L, eps);
 }
 
 void batch_step(SlotBatch *b) {
@@ -757,6 +821,67 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
     iter[c] = hc[c].iter;
     err[c] = hc[c].err;
   }
+}
+
+void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
+                   const double *b, double *x, int nrhs, int slots, double eps,
+                   int itmax, int32_t *iter, double *err, double *lat_ms,
+                   hipStream_t s) {
+  if (nrhs < 0) throw Error(AINV_ERR_ARG, "nrhs must be >= 0");
+  if (slots < 1 || slots > kMaxCols)
+    throw Error(AINV_ERR_ARG, "slots must be in 1..64");
+  if (m && m->n != a->n)
+    throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
+  const bool cg = (method == "CG");
+  if (!cg && method != "BICGSTAB")
+    throw Error(AINV_ERR_ARG, "unknown Krylov method " + method +
+                                  " (BICGSTAB, CG available)");
+  if (nrhs == 0) return;
+  const int64_t n = a->n;
+  SlotBatch *bt = batch_open(a, m, cg, slots, s);
+  const int K = bt->K;
+  std::vector<int> owner(K, -1);
+  std::vector<int64_t> first(K, 0);
+  std::vector<double> t_adm(K, 0.0);
+  std::vector<SlotState> st;
+  int next = 0, done = 0;
+  try {
+    while (done < nrhs) {
+      // fill every free slot (up to the requested count) from the queue
+      std::vector<int> adm;
+      std::vector<const double *> src;
+      for (int c = 0; c < slots && next < nrhs; ++c) {
+        if (owner[c] >= 0) continue;
+        owner[c] = next;
+        first[c] = batch_next_step(bt);
+        t_adm[c] = now_seconds();
+        adm.push_back(c);
+        src.push_back(b + (int64_t)next * n);
+        ++next;
+      }
+      batch_admit_cols(bt, adm, src, eps);
+      batch_step(bt);
+      const int64_t seen = batch_poll(bt, false, st);
+      if (seen < 0) continue;
+      const double t = now_seconds();
+      for (int c = 0; c < K; ++c) {
+        if (owner[c] < 0 || seen < first[c]) continue;
+        if (st[c].active && st[c].iter < itmax) continue;
+        const int j = owner[c];
+        batch_extract(bt, c, 1, x + (int64_t)j * n);
+        iter[j] = st[c].iter;
+        err[j] = st[c].err;
+        if (lat_ms) lat_ms[j] = (t - t_adm[c]) * 1e3;
+        owner[c] = -1;
+        ++done;
+      }
+    }
+    AINV_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    batch_close(bt);
+    throw;
+  }
+  batch_close(bt);
 }
 
 }  // namespace ainv

@@ -106,6 +106,9 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, bool cg, int K,
 void batch_close(SlotBatch *b);
 // src: device, n x kc interleaved; x0 = 0
 void batch_admit(SlotBatch *b, int c0, int kc, const double *src, double eps);
+// one right-hand side (a contiguous device n-vector) per listed slot
+void batch_admit_cols(SlotBatch *b, const std::vector<int> &slots,
+                      const std::vector<const double *> &srcs, double eps);
 void batch_step(SlotBatch *b);
 // Index of the next step batch_step will launch (0 for the first).
 int64_t batch_next_step(const SlotBatch *b);
@@ -115,6 +118,15 @@ int64_t batch_next_step(const SlotBatch *b);
 int64_t batch_poll(SlotBatch *b, bool wait_last, std::vector<SlotState> &out);
 // dst: device, n x kc interleaved
 void batch_extract(SlotBatch *b, int c0, int kc, double *dst);
+
+// Continuous batching over a device-resident queue: b and x hold nrhs
+// n-vectors one after another; `slots` columns run at a time and a slot is
+// refilled from the queue as soon as its column stops.  iter, err and
+// lat_ms (admission to completion seen by the host) are per right-hand side.
+void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
+                   const double *b, double *x, int nrhs, int slots, double eps,
+                   int itmax, int32_t *iter, double *err, double *lat_ms,
+                   hipStream_t s);
 
 // flags: AINV_KRYLOV_EAGER disables graph capture (used to check replays)
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,

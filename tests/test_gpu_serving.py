@@ -76,3 +76,39 @@ def test_lru_eviction_under_budget(gpu):
     st = srv.stats()
     assert st["evictions"] >= 4 and st["builds"] >= 5
     srv.free()
+
+
+@pytest.mark.parametrize("method,slots,nrhs", [("BICGSTAB", 8, 21),
+                                               ("BICGSTAB", 5, 3),
+                                               ("CG", 16, 40)])
+def test_krylov_stream_matches_single_solves(gpu, method, slots, nrhs):
+    """The served form (a device queue through continuously refilled slots)
+    gives each right-hand side the iterations (+-1) and solution of solving
+    it alone; latencies are positive."""
+    if method == "CG":
+        a = gpu.gen_aniso2d(40, 1e-2)
+        params = dict(AINV_ALG="SYM-LLK", SUB_FILLIN=8, SUB_ILUTHRS=5e-2,
+                      FACTOR_PREC="F32")
+        kind = "AINV"
+    else:
+        a = gpu.gen_pde3d(12)
+        params = dict(SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC="F32")
+        kind = "INVK"
+    n = a.shape[0]
+    sv = gpu.SOLVERS[kind]()
+    for k, v in params.items():
+        sv.set(k, v)
+    sv.build(a)
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(31, (nrhs, n))
+    X, it, err, lat = gpu.krylov_stream(method, A, sv,
+                                        gpu.DeviceArray.from_numpy(B),
+                                        slots=slots, eps=1e-9)
+    Xh = X.to_numpy()
+    assert np.all(err <= 1e-9) and np.all(lat > 0)
+    for j in range(nrhs):
+        xs, its, _ = gpu.krylov(method, A, sv, B[j], eps=1e-9)
+        assert abs(int(it[j]) - int(its[0])) <= 1, (j, it[j], its)
+        tr = np.linalg.norm(B[j] - a @ Xh[j]) / np.linalg.norm(B[j])
+        assert tr < 1e-8, (j, tr)
+        assert np.linalg.norm(Xh[j] - xs) <= 1e-6 * np.linalg.norm(xs)
