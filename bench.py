@@ -196,7 +196,7 @@ def main():
                for j in range(0, M, max(1, M // 16)))
     rep = sv.report()
     tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
-    kmap = tuned[0]["map"] if tuned else "split"
+    kmap = tuned[0]["map_w"] if tuned else "split"
     # same right-hand sides solved singly at reduced precision: iterations
     it_red = [int(v) for v in iters[1:args.single + 1]]
     log(f"served {args.prec} slots={slots}: {rate:.2f} solves/s, "
@@ -206,11 +206,13 @@ def main():
     ms_w, ms_z = profile_apply(m, sv, n, slots, 10)
     bw = apply_pass_bytes(lay, slots, True)
     gbs_w = bw / (ms_w * 1e-3) / 1e9
-    kernel = {"split": "k_hll_spmm", "row": "k_hll_spmm_row",
-              "coop": "k_hll_spmm_coop"}[kmap]
+    kernel, extra = {"split": ("k_hll_spmm", ""),
+                     "row": ("k_hll_spmm_row", ""),
+                     "coop": ("k_hll_spmm_coop", " 256,"),
+                     "coop128": ("k_hll_spmm_coop", " 128,")}[kmap]
     vt = {"F64": "double", "F32": "float", "BF16": "unsigned short"}[args.prec]
     tt = "double" if args.prec == "F64" else "float"
-    kname = f"{kernel}<{vt}, double, {tt}, {tt}, {slots}, 0>"
+    kname = f"{kernel}<{vt}, double, {tt}, {tt}, {slots},{extra} 0>"
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:

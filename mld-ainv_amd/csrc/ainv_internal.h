@@ -114,14 +114,20 @@ void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
 // float) or
 // y   = alpha * A x + beta*y (EPI axpby)  for k interleaved columns.
 // map: AINV_MAP_SPLIT (lanes split rows x columns, any ld),
-// AINV_MAP_ROW (one row per lane) or AINV_MAP_COOP (the lanes of a row
-// group load each x row together); ROW and COOP need ldx = ldo = K = pow2
-// >= k, k <= 32 and 16-byte aligned vectors, otherwise the split mapping
-// runs.
+// AINV_MAP_ROW (one row per lane, k <= 32) or AINV_MAP_COOP /
+// AINV_MAP_COOP128 (the lanes of a row group load each x row together;
+// blocks whose x rows exceed 256 / 128 bytes run as column chunks, k <= 64);
+// all but split need ldx = ldo = K = pow2 >= k and 16-byte aligned vectors,
+// otherwise the split mapping runs.
 constexpr int AINV_MAP_SPLIT = 0;
 constexpr int AINV_MAP_ROW = 1;
 constexpr int AINV_MAP_COOP = 2;
-constexpr int kNumMaps = 3;
+constexpr int AINV_MAP_COOP128 = 3;
+constexpr int kNumMaps = 4;
+// The two apply passes may use different mappings: a packed pair.
+inline int map_pair(int w, int z) { return w | (z << 4); }
+inline int map_w(int pair) { return pair & 15; }
+inline int map_z(int pair) { return (pair >> 4) & 15; }
 void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
                     int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s,
                     int map = AINV_MAP_SPLIT);

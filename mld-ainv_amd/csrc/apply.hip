@@ -564,6 +564,27 @@ template <typename XT, int K> struct Coop {
   static constexpr int R = kSlice / L;              // rows per pass
 };
 
+// Column chunks.  A block of K columns whose x rows are wider than
+// CB bytes (AINV_MAP_COOP: 256, AINV_MAP_COOP128: 128) is processed as NCH chunks of KC columns by separate
+// workgroups, each XCD taking one chunk for a contiguous range of slices: the
+// x rows an XCD gathers between the first and the last use of a row (the
+// matrix bandwidth times the row width) then fit in its L2, at the price of
+// reading the factor words once per chunk.
+template <typename XT, int K, int CB> struct CoopChunk {
+  static constexpr int cap = CB / (int)sizeof(XT);
+  static constexpr int KC = K < cap ? K : cap;
+  static constexpr int NCH = K / KC;
+  // KC >= 32 accumulators per lane: 2 waves per SIMD instead of spilling
+  static constexpr int waves = KC >= 32 ? 2 : AINV_ROW_WAVES;
+};
+#if AINV_ROW_WAVES > 0
+#define AINV_COOP_ATTR(XT)                                                  \
+  __attribute__((amdgpu_waves_per_eu(CoopChunk<XT, K, CB>::waves,           \
+                                     CoopChunk<XT, K, CB>::waves)))
+#else
+#define AINV_COOP_ATTR(XT)
+#endif
+
 template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
   if constexpr (sizeof(T) == 2) {
     return (T)__shfl((int)v, src);
@@ -572,19 +593,19 @@ template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
   }
 }
 
-template <typename VT, typename XT, int K>
+// x rows are K apart; xs points at this lane's first column of the chunk.
+template <typename VT, typename XT, int K, int KC>
 __device__ __forceinline__ void coop_acc(
-    int64_t p0, int32_t width, int lane, int sub, int rg,
+    int64_t p0, int32_t width, int lane, int rg,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
-    const XT *__restrict__ x,
-    typename Val<VT>::acc (&acc)[Coop<XT, K>::L][Coop<XT, K>::CPL]) {
+    const XT *__restrict__ xs,
+    typename Val<VT>::acc (&acc)[Coop<XT, KC>::L][Coop<XT, KC>::CPL]) {
   using A = typename Val<VT>::acc;
-  using CP = Coop<XT, K>;
+  using CP = Coop<XT, KC>;
   constexpr int L = CP::L, CPL = CP::CPL, R = CP::R;
-  constexpr int P = RowGroup<A, K>::v;
+  constexpr int P = RowGroup<A, KC>::v;
   const int32_t nfull = width / P * P;
   const int64_t pl = p0 + lane;
-  const XT *xs = x + sub * CPL;
   if (nfull > 0) {
     int32_t ci[P];
     VT vi[P];
@@ -647,18 +668,26 @@ __device__ __forceinline__ void coop_acc(
   }
 }
 
-template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
-__global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_coop(
+template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
+          int EPI>
+__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
     OT *__restrict__ out, double alpha, double beta) {
   using A = typename Val<VT>::acc;
-  using CP = Coop<XT, K>;
+  using CH = CoopChunk<XT, K, CB>;
+  constexpr int KC = CH::KC;
+  using CP = Coop<XT, KC>;
   const int lane = threadIdx.x & (kSlice - 1);
   const int sub = lane % CP::L, rg = lane / CP::L;
-  const int32_t s = __builtin_amdgcn_readfirstlane(
-      xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6));
+  // grid = NCH x slice groups; each XCD runs one chunk of a slice range
+  const int32_t ngrp = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int32_t lb = __builtin_amdgcn_readfirstlane(
+      xcd_block(blockIdx.x, gridDim.x));
+  const int chunk = lb / ngrp;
+  const int32_t s = lb % ngrp * kWavesPerBlock +
+                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= nslices) return;
   const int64_t base = sptr[s];
   const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
@@ -667,8 +696,8 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_coop(
   for (int q = 0; q < CP::L; ++q)
 #pragma unroll
     for (int u = 0; u < CP::CPL; ++u) acc[q][u] = 0;
-  coop_acc<VT, XT, K>(base, width, lane, sub, rg, col, val, x, acc);
-  const int c0 = sub * CP::CPL;
+  const int c0 = chunk * KC + sub * CP::CPL;  // first column of this lane
+  coop_acc<VT, XT, K, KC>(base, width, lane, rg, col, val, x + c0, acc);
 #pragma unroll
   for (int q = 0; q < CP::L; ++q) {
     const int32_t r = s * kSlice + q * CP::R + rg;
@@ -692,25 +721,47 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_coop(
 }
 
 // Fused system-matrix product + dots in the cooperative mapping (ld = K).
-// Lanes with the same chunk hold partial sums of the same columns; they are
-// reduced by shuffles across the row groups, then across the waves in LDS.
-template <int K, int NP>
-__global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_coop(
-    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
-    const int32_t *__restrict__ col, const double *__restrict__ val,
-    const double *__restrict__ x, int k, double *__restrict__ y,
-    const double *__restrict__ w, double *__restrict__ part) {
-  using CP = Coop<double, K>;
+// With NCH chunks, the blocks of XCD x work on chunk x / (8 / NCH) and
+// stride through a contiguous part of the slice groups.  Lanes with the same
+// lane chunk hold partial sums of the same columns; they are reduced by
+// shuffles across the row groups, then across the waves in LDS; a block
+// writes zero partial sums for the columns of the other chunks.
+template <int K, int CB, int NP>
+__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(double) void
+k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
+                    const int64_t *__restrict__ sptr,
+                    const int32_t *__restrict__ col,
+                    const double *__restrict__ val,
+                    const double *__restrict__ x, int k,
+                    double *__restrict__ y, const double *__restrict__ w,
+                    double *__restrict__ part) {
+  using CH = CoopChunk<double, K, CB>;
+  constexpr int KC = CH::KC, NCH = CH::NCH;
+  using CP = Coop<double, KC>;
   constexpr int L = CP::L, CPL = CP::CPL, R = CP::R;
   const int lane = threadIdx.x & (kSlice - 1);
   const int wave = threadIdx.x >> 6;
   const int sub = lane % L, rg = lane / L;
-  const int c0 = sub * CPL;
+  // chunk and slice-group range of this block
+  const int32_t ng = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+#if AINV_XCD_REMAP
+  constexpr int per = kXcds / NCH;  // XCDs per chunk (grid: multiple of 8)
+  const int xcd = blockIdx.x % kXcds;
+  const int chunk = xcd / per, sx = xcd % per;
+  const int32_t g_first = (int32_t)((int64_t)ng * sx / per) +
+                          (int32_t)(blockIdx.x / kXcds);
+  const int32_t g_end = (int32_t)((int64_t)ng * (sx + 1) / per);
+  const int32_t g_stride = gridDim.x / kXcds;
+#else
+  const int chunk = blockIdx.x % NCH;
+  const int32_t g_first = blockIdx.x / NCH, g_end = ng;
+  const int32_t g_stride = gridDim.x / NCH;
+#endif
+  const int c0 = chunk * KC + sub * CPL;
   double d0[CPL], d1[CPL];
 #pragma unroll
   for (int u = 0; u < CPL; ++u) d0[u] = d1[u] = 0.0;
-  const GroupRange gr = xcd_groups(nslices);
-  for (int32_t grp = gr.first; grp < gr.end; grp += gr.stride) {
+  for (int32_t grp = g_first; grp < g_end; grp += g_stride) {
     const int32_t s =
         __builtin_amdgcn_readfirstlane(grp * kWavesPerBlock + wave);
     if (s >= nslices) break;
@@ -721,7 +772,8 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_coop(
     for (int q = 0; q < L; ++q)
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
-    coop_acc<double, double, K>(base, width, lane, sub, rg, col, val, x, acc);
+    coop_acc<double, double, K, KC>(base, width, lane, rg, col, val, x + c0,
+                                    acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
       const int32_t r = s * kSlice + q * R + rg;
@@ -742,7 +794,7 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_coop(
       }
     }
   }
-  __shared__ double red[2][kWavesPerBlock][K];
+  __shared__ double red[2][kWavesPerBlock][KC];
 #pragma unroll
   for (int u = 0; u < CPL; ++u) {
     double a0 = d0[u], a1 = d1[u];
@@ -751,17 +803,19 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_coop(
       if constexpr (NP == 2) a1 += __shfl_xor(a1, off);
     }
     if (rg == 0) {
-      red[0][wave][c0 + u] = a0;
-      red[1][wave][c0 + u] = a1;
+      red[0][wave][sub * CPL + u] = a0;
+      red[1][wave][sub * CPL + u] = a1;
     }
   }
   __syncthreads();
   if (threadIdx.x < K && (int)threadIdx.x < k) {
+    const int cl = (int)threadIdx.x - chunk * KC;  // column within the chunk
     double t0 = 0.0, t1 = 0.0;
-    for (int wv = 0; wv < kWavesPerBlock; ++wv) {
-      t0 += red[0][wv][threadIdx.x];
-      t1 += red[1][wv][threadIdx.x];
-    }
+    if (cl >= 0 && cl < KC)
+      for (int wv = 0; wv < kWavesPerBlock; ++wv) {
+        t0 += red[0][wv][cl];
+        t1 += red[1][wv][cl];
+      }
     part[((int64_t)0 * kRedBlocks + blockIdx.x) * kMaxCols + threadIdx.x] = t0;
     if constexpr (NP == 2)
       part[((int64_t)1 * kRedBlocks + blockIdx.x) * kMaxCols + threadIdx.x] =
@@ -769,18 +823,19 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_coop(
   }
 }
 
-template <typename VT, typename XT, typename DT, typename OT, int EPI>
+template <typename VT, typename XT, typename DT, typename OT, int CB, int EPI>
 void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
                  double alpha, double beta, hipStream_t s) {
   if (a.nslices == 0) return;
-  dim3 grid((a.nslices + kWavesPerBlock - 1) / kWavesPerBlock), block(kBlock);
+  const int ngrp = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
   const int64_t *sp = a.sptr.as<int64_t>();
   const int32_t *cl = a.col.as<int32_t>();
   const VT *vl = a.val.as<VT>();
-#define AINV_LAUNCH(KK)                                                       \
-  hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, EPI>), grid, block, \
-                     0, s, a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha,   \
-                     beta)
+#define AINV_LAUNCH(KK)                                                     \
+  hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, CB, EPI>),        \
+                     dim3(ngrp * CoopChunk<XT, KK, CB>::NCH), dim3(kBlock), 0, \
+                     s,                                                       \
+                     a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha, beta)
   if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)
@@ -789,8 +844,10 @@ void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
     AINV_LAUNCH(8);
   else if (k <= 16)
     AINV_LAUNCH(16);
-  else
+  else if (k <= 32)
     AINV_LAUNCH(32);
+  else
+    AINV_LAUNCH(64);
 #undef AINV_LAUNCH
   AINV_HIP_CHECK(hipGetLastError());
 }
@@ -799,7 +856,9 @@ template <typename VT, typename XT, typename DT, typename OT, int EPI>
 void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
                  OT *out, double alpha, double beta, hipStream_t s) {
   if (map == AINV_MAP_COOP)
-    launch_coop<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
+    launch_coop<VT, XT, DT, OT, 256, EPI>(a, x, k, d, out, alpha, beta, s);
+  else if (map == AINV_MAP_COOP128)
+    launch_coop<VT, XT, DT, OT, 128, EPI>(a, x, k, d, out, alpha, beta, s);
   else
     launch_row<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
 }
@@ -822,7 +881,9 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
   const int32_t *rl = a.rlen.as<int32_t>();
   const int32_t *cl = a.col.as<int32_t>();
   const double *vl = a.val.as<double>();
-  if ((map == AINV_MAP_ROW || map == AINV_MAP_COOP) && k > 1 && k <= 32) {
+  const bool coop = map == AINV_MAP_COOP || map == AINV_MAP_COOP128;
+  if ((map == AINV_MAP_ROW && k > 1 && k <= 32) ||
+      (coop && k > 1 && k <= 64)) {
     int K = 2;
     while (K < k) K <<= 1;
     if (ld != K || !aligned16(x) || !aligned16(y) || !aligned16(w))
@@ -831,11 +892,27 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
 #define AINV_LR(KK, NPP)                                                     \
   do {                                                                       \
     if (map == AINV_MAP_COOP)                                                \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, NPP>), dim3(g),            \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 256, NPP>), dim3(g),       \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part);                                     \
+    else if (map == AINV_MAP_COOP128)                                        \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part);                                     \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_row<KK, NPP>), dim3(g),             \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part);                                     \
+  } while (0)
+// K = 64 exists only in the cooperative mapping
+#define AINV_LC(KK, NPP)                                                     \
+  do {                                                                       \
+    if (map == AINV_MAP_COOP128)                                             \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 128, NPP>), dim3(g),       \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part);                                     \
+    else                                                                     \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 256, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part);                                     \
   } while (0)
@@ -848,14 +925,17 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
     AINV_LR(8, NPP);    \
   else if (K == 16)     \
     AINV_LR(16, NPP);   \
+  else if (K == 32)     \
+    AINV_LR(32, NPP);   \
   else                  \
-    AINV_LR(32, NPP);
+    AINV_LC(64, NPP);
     if (np == 1) {
       AINV_LRK(1)
     } else {
       AINV_LRK(2)
     }
 #undef AINV_LRK
+#undef AINV_LC
 #undef AINV_LR
     AINV_HIP_CHECK(hipGetLastError());
     return g;
@@ -891,9 +971,15 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
 
 // Row mapping needs ld = K (a power of two) and 16-byte aligned x and
 // output.
+static bool is_coop(int map) {
+  return map == AINV_MAP_COOP || map == AINV_MAP_COOP128;
+}
+
 static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,
                     const void *o) {
-  if ((map != AINV_MAP_ROW && map != AINV_MAP_COOP) || k < 2 || k > 32)
+  // row: k <= 32; coop: k <= 64 (column chunks)
+  if ((map != AINV_MAP_ROW && !is_coop(map)) || k < 2 ||
+      k > (is_coop(map) ? 64 : 32))
     return false;
   int K = 2;
   while (K < k) K <<= 1;

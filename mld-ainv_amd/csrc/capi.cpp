@@ -265,12 +265,16 @@ int map_from_string(const std::string &v) {
   if (v == "SPLIT") return AINV_MAP_SPLIT;
   if (v == "ROW") return AINV_MAP_ROW;
   if (v == "COOP") return AINV_MAP_COOP;
+  if (v == "COOP128") return AINV_MAP_COOP128;
   throw Error(AINV_ERR_ARG,
-              "unknown APPLY_MAP '" + v + "' (AUTO, SPLIT, ROW, COOP)");
+              "unknown APPLY_MAP '" + v + "' (AUTO, SPLIT, ROW, COOP, COOP128)");
 }
 
 const char *map_name(int m) {
-  return m == AINV_MAP_ROW ? "row" : m == AINV_MAP_COOP ? "coop" : "split";
+  return m == AINV_MAP_ROW       ? "row"
+         : m == AINV_MAP_COOP    ? "coop"
+         : m == AINV_MAP_COOP128 ? "coop128"
+                                 : "split";
 }
 
 void set_map_mode(ainv_solver *h, int mode) {
@@ -368,8 +372,8 @@ size_t solver_t_bytes(const ainv_solver *h, int k) {
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
                        int k, int64_t ld, void *t, hipStream_t s, int map) {
   const bool f32t = h->prec != AINV_PREC_F64;
-  hll_spmm_scale(h->hw, h->dv.p, x, false, ld, t, ld, k, s, map);
-  hll_spmm_axpby(h->hz, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map);
+  hll_spmm_scale(h->hw, h->dv.p, x, false, ld, t, ld, k, s, map_w(map));
+  hll_spmm_axpby(h->hz, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map));
 }
 
 int log2_pow2(int K) {
@@ -379,15 +383,16 @@ int log2_pow2(int K) {
 }
 
 int tuned_apply_map(const ainv_solver *h, int K) {
-  if (h->map_mode >= 0) return h->map_mode;
+  if (h->map_mode >= 0) return map_pair(h->map_mode, h->map_mode);
   const int m = h->apply_map[log2_pow2(K)];
-  return m < 0 ? AINV_MAP_SPLIT : m;
+  return m < 0 ? map_pair(AINV_MAP_SPLIT, AINV_MAP_SPLIT) : m;
 }
 
 void reset_tuning(ainv_solver *h) {
   for (int i = 0; i < 8; ++i) {
     h->apply_map[i] = -1;
-    h->apply_ms[i][0] = h->apply_ms[i][1] = 0.0;
+    for (auto &pass : h->apply_ms[i])
+      for (double &v : pass) v = 0.0;
   }
 }
 
@@ -414,15 +419,16 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
   // the tuned mapping needs x, t and y with leading dimension K = pow2 >= k
   int K = 1;
   while (K < k) K <<= 1;
-  const int map = (t == 'N' && ldx == K && ldy == K) ? tuned_apply_map(h, K)
-                                                     : AINV_MAP_SPLIT;
-  const int64_t ldt = (map == AINV_MAP_ROW) ? K : k;
+  const bool tuned = (t == 'N' && ldx == K && ldy == K);
+  const int map = tuned ? tuned_apply_map(h, K) : 0;
+  const int64_t ldt = tuned ? K : k;
   const size_t need2 = (size_t)h->n * ldt *
                        (f32t ? sizeof(float) : sizeof(double));
   if (h->tbuf.bytes < need2) h->tbuf.alloc(need2);
-  hll_spmm_scale(*first, h->dv.p, x, false, ldx, h->tbuf.p, ldt, k, s, map);
+  hll_spmm_scale(*first, h->dv.p, x, false, ldx, h->tbuf.p, ldt, k, s,
+                 map_w(map));
   hll_spmm_axpby(*second, alpha, h->tbuf.p, f32t, ldt, beta, y, ldy, k, s,
-                 map);
+                 map_z(map));
 }
 
 }  // namespace ainv
@@ -836,18 +842,25 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
            ", \"prec\": \"" + prec_name(h->prec) + "\"" +
            ", \"w\": " + hll_json(h->hw) + ", \"z\": " + hll_json(h->hz);
       if (!h->inv_report.empty()) r += ", \"inverse\": " + h->inv_report;
-      // autotuner: per block width K, ms of the split / row apply, choice
+      // autotuner: per block width K, ms of each pass in each mapping
       std::string tu;
       for (int i = 0; i < 8; ++i) {
         if (h->apply_map[i] < 0) continue;
-        char b2[200];
-        std::snprintf(b2, sizeof b2,
-                      "%s{\"K\": %d, \"split_ms\": %.6f, \"row_ms\": %.6f, "
-                      "\"coop_ms\": %.6f, \"map\": \"%s\"}",
-                      tu.empty() ? "" : ", ", 1 << i, h->apply_ms[i][0],
-                      h->apply_ms[i][1], h->apply_ms[i][2],
-                      map_name(h->apply_map[i]));
-        tu += b2;
+        std::string e = std::string(tu.empty() ? "" : ", ") +
+                        "{\"K\": " + std::to_string(1 << i);
+        for (int pass = 0; pass < 2; ++pass) {
+          e += pass ? ", \"z_ms\": {" : ", \"w_ms\": {";
+          for (int m = 0; m < kNumMaps; ++m) {
+            char b2[64];
+            std::snprintf(b2, sizeof b2, "%s\"%s\": %.6f", m ? ", " : "",
+                          map_name(m), h->apply_ms[i][pass][m]);
+            e += b2;
+          }
+          e += "}";
+        }
+        e += std::string(", \"map_w\": \"") + map_name(map_w(h->apply_map[i])) +
+             "\", \"map_z\": \"" + map_name(map_z(h->apply_map[i])) + "\"}";
+        tu += e;
       }
       r += ", \"apply_map\": \"" +
            std::string(h->map_mode < 0 ? "auto" : map_name(h->map_mode)) +
@@ -1040,13 +1053,14 @@ int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
     // the mapping the apply would use (forced or tuned; split if neither)
     const int map = tuned_apply_map(h, k);
     // one untimed pass first: first touches of the buffers stay out
-    hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, map);
-    hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, map);
+    const int mw = map_w(map), mz = map_z(map);
+    hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, mw);
+    hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, mz);
     for (int r = 0; r < reps; ++r) {
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r], s));
-      hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, map);
+      hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, mw);
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 1], s));
-      hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, map);
+      hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, mz);
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 2], s));
     }
     AINV_HIP_CHECK(hipStreamSynchronize(s));

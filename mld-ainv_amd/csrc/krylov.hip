@@ -368,7 +368,8 @@ struct KrylovPlan {
   uint64_t gen = 0;
   int k = 0;       // columns in use
   int K = 1;       // leading dimension of the workspace, pow2 >= k
-  int apply_map = AINV_MAP_SPLIT, spmv_map = AINV_MAP_SPLIT;
+  int apply_map = 0;  // packed pair of the two passes' mappings
+  int spmv_map = AINV_MAP_SPLIT;
   int32_t n = 0;
   DevBuf work, colbuf, part, nact, tbuf;
   Vecs v{};
@@ -469,9 +470,10 @@ void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
                                   hipMemcpyDeviceToDevice, s));
 }
 
-// Autotuner: time the kernel mappings (split, row, coop) of the preconditioner
-// apply and of the fused system-matrix product on this plan's buffers and
-// keep the faster; results are cached on the solver / matrix per K.
+// Autotuner: time the kernel mappings (split, row, coop, coop128) of each
+// apply pass and of the fused system-matrix product on this plan's buffers
+// and keep the fastest of each; results are cached on the solver / matrix
+// per K.
 double time_ms(hipStream_t s, const std::function<void()> &f, int reps) {
   hipEvent_t e0, e1;
   AINV_HIP_CHECK(hipEventCreate(&e0));
@@ -494,25 +496,35 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   // a mapping forced on the preconditioner (APPLY_MAP) also holds for the
   // system-matrix product of the solves it is used in
   const int forced = m ? m->map_mode : -1;
-  if (K == 1 || p->k < 2 || K > 32) {
-    p->apply_map = p->spmv_map = AINV_MAP_SPLIT;
+  if (K == 1 || p->k < 2) {
+    p->apply_map = map_pair(AINV_MAP_SPLIT, AINV_MAP_SPLIT);
+    p->spmv_map = AINV_MAP_SPLIT;
     return;
   }
   if (forced >= 0) {
-    p->apply_map = p->spmv_map = forced;
+    p->apply_map = map_pair(forced, forced);
+    p->spmv_map = forced;
     return;
   }
   const Vecs &v = p->v;
   if (m) {
     if (m->apply_map[li] < 0) {
-      int best = 0;
+      // the two passes are timed separately; t keeps leading dimension K
+      const bool f32t = m->prec != AINV_PREC_F64;
+      int bw = 0, bz = 0;
       for (int map = 0; map < kNumMaps; ++map) {
-        m->apply_ms[li][map] = time_ms(s, [&] {
-          solver_apply_into(m, v.b, v.z, p->k, K, p->tbuf.p, s, map);
+        m->apply_ms[li][0][map] = time_ms(s, [&] {
+          hll_spmm_scale(m->hw, m->dv.p, v.b, false, K, p->tbuf.p, K, p->k, s,
+                         map);
         }, 3);
-        if (m->apply_ms[li][map] < m->apply_ms[li][best]) best = map;
+        m->apply_ms[li][1][map] = time_ms(s, [&] {
+          hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, K, 0.0, v.z, K, p->k, s,
+                         map);
+        }, 3);
+        if (m->apply_ms[li][0][map] < m->apply_ms[li][0][bw]) bw = map;
+        if (m->apply_ms[li][1][map] < m->apply_ms[li][1][bz]) bz = map;
       }
-      m->apply_map[li] = (int8_t)best;
+      m->apply_map[li] = (int16_t)map_pair(bw, bz);
     }
     p->apply_map = m->apply_map[li];
   }
@@ -529,7 +541,6 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   p->spmv_map = a->spmv_map[li];
 }
 
-// One full iteration: the kernel sequence that is captured and replayed.
 void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   const int32_t n = p->n;
   const int k = p->k, K = p->K, sm = p->spmv_map;

@@ -30,7 +30,7 @@ struct ainv_solver {
   double inv_thresh = 0.0;
   int prec = AINV_PREC_F64;
   // kernel mapping of the apply: -1 autotuned, else AINV_MAP_SPLIT / ROW /
-  // COOP
+  // COOP / COOP128 for both passes
   int map_mode = -1;
 
   // ---- device / stream
@@ -52,8 +52,9 @@ struct ainv_solver {
   ainv::DevBuf stage_x, stage_y;  // host-pointer staging
 
   // ---- autotuned kernel mapping per block width K = 2^i (-1: not tuned)
-  int8_t apply_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-  double apply_ms[8][3] = {};  // split, row, coop
+  // (packed pair: first and second pass, map_pair)
+  int16_t apply_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  double apply_ms[8][2][ainv::kNumMaps] = {};  // [K][pass][map]
 
   // ---- build report
   double t_factor = 0, t_inverse = 0, t_layout = 0, t_total = 0;
@@ -67,7 +68,7 @@ struct ainv_matrix {
   ainv::DevCsr csr;
   ainv::Hll hll;
   int8_t spmv_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-  double spmv_ms[8][3] = {};
+  double spmv_ms[8][ainv::kNumMaps] = {};
   // Krylov workspaces and captured iteration graphs, most recent first
   std::vector<std::unique_ptr<ainv::KrylovPlan, ainv::PlanDeleter>> plans;
 };
@@ -79,13 +80,15 @@ namespace ainv {
 void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
                   double *y, char trans, int k, int64_t ldx, int64_t ldy,
                   hipStream_t s);
-// y = M x ('N', beta = 0, ld = k) with a caller-owned intermediate buffer t
-// of n*k words of the factor's intermediate type: no allocation, so it may
-// be captured in a graph.
+// y = M x ('N', beta = 0, x, t and y with leading dimension ld) with a
+// caller-owned intermediate buffer t of n*ld words of the factor's
+// intermediate type: no allocation, so it may be captured in a graph.  map:
+// packed pair of the two passes' mappings (map_pair).
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
                        int k, int64_t ld, void *t, hipStream_t s,
                        int map = 0);
-// Index of K = 2^i and the mapping the tuner chose for it (split if none).
+// Index of K = 2^i and the mapping pair the tuner chose for it (split if
+// none, the forced mapping for both passes under APPLY_MAP).
 int log2_pow2(int K);
 int tuned_apply_map(const ainv_solver *h, int K);
 void reset_tuning(ainv_solver *h);

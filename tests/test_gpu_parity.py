@@ -140,8 +140,8 @@ def test_block_apply_matches_columns(gpu, O, k):
 
 @pytest.mark.parametrize("prec,tol", [("F64", 1e-12), ("F32", 2e-6),
                                       ("BF16", 2e-2)])
-@pytest.mark.parametrize("k", [2, 4, 8, 16, 32])
-@pytest.mark.parametrize("mapping", ["ROW", "COOP"])
+@pytest.mark.parametrize("k", [2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("mapping", ["ROW", "COOP", "COOP128"])
 def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
     """The row-per-lane and cooperative kernels (leading dimension a
     power of two) against the oracle column by column, alpha / beta
@@ -149,7 +149,7 @@ def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
     a = gpu.gen_pde3d(10)
     sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC=prec,
                APPLY_MAP=mapping)
-    assert sv.geti("APPLY_MAP") == {"ROW": 1, "COOP": 2}[mapping]
+    assert sv.geti("APPLY_MAP") == {"ROW": 1, "COOP": 2, "COOP128": 3}[mapping]
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
     n = a.shape[0]
     x = gpu.gen_normal(7, (n, k))
@@ -166,7 +166,9 @@ def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
 
 @pytest.mark.parametrize("method,k,mapping", [
     ("BICGSTAB", 8, "ROW"), ("BICGSTAB", 5, "ROW"), ("CG", 16, "ROW"),
-    ("BICGSTAB", 8, "COOP"), ("BICGSTAB", 3, "COOP"), ("CG", 32, "COOP")])
+    ("BICGSTAB", 8, "COOP"), ("BICGSTAB", 3, "COOP"), ("CG", 32, "COOP"),
+    ("BICGSTAB", 40, "COOP"), ("BICGSTAB", 64, "COOP128"),
+    ("CG", 32, "COOP128")])
 def test_row_mapping_krylov(gpu, O, method, k, mapping):
     """Krylov solves with the row-mapped apply and system-matrix product
     (a padded leading dimension when k is not a power of two): iteration
@@ -203,9 +205,11 @@ def test_autotuner_report(gpu):
     rep = sv.report()
     assert rep["apply_map"] == "auto"
     (t,) = [e for e in rep["tuning"] if e["K"] == 8]
-    ms = {m: t[m + "_ms"] for m in ("split", "row", "coop")}
-    assert min(ms.values()) > 0
-    assert t["map"] == min(ms, key=ms.get)
+    for pass_, key in (("w_ms", "map_w"), ("z_ms", "map_z")):
+        ms = t[pass_]
+        assert set(ms) == {"split", "row", "coop", "coop128"}
+        assert min(ms.values()) > 0
+        assert t[key] == min(ms, key=ms.get)
 
 
 @pytest.mark.parametrize("prec,tol", [("F32", 2e-6), ("BF16", 2e-2)])
