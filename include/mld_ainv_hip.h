@@ -156,6 +156,35 @@ int ainv_clear_data(ainv_h h);
 int ainv_export(ainv_h h, char which, int32_t *n, int64_t *nnz, int64_t *irp,
                 int32_t *ja, double *val);
 
+/* sv%dump (mld_d_base_ainv_solver_dmp.f90:56-106): with solver != 0 writes
+ * the stored W, d and Z as MatrixMarket files named
+ * <prefix>_p000_l<level>_{wmat,diag,zmat}.mtx (prefix NULL: "dump_ainv_d";
+ * rank 000: one process per library instance), W and Z as 1-based
+ * coordinate files, d as a dense array, values with 17 significant digits;
+ * head (may be NULL) becomes a comment line.  solver == 0 writes nothing, as
+ * in the reference. */
+int ainv_dump(ainv_h h, const char *prefix, int level, int solver,
+              const char *head);
+
+/* Factor exchange: make h apply factors built elsewhere (for instance the
+ * files of a reference sv%dump): stored W and Z as n x n CSR (index_base 0
+ * or 1) and the inverted pivots d.  No counterpart in the reference, whose
+ * factors only come from its own build; the kind and build parameters of h
+ * are left as they are.  The factors are stored in h's FACTOR_PREC. */
+int ainv_import_factors(ainv_h h, int32_t n, const int64_t *w_irp,
+                        const int32_t *w_ja, const double *w_val,
+                        const int64_t *z_irp, const int32_t *z_ja,
+                        const double *z_val, const double *d, int index_base);
+
+/* MatrixMarket readers (the input path of tests/fileread/df_sample.f90:
+ * 156-175): a square coordinate file (real, integer or pattern; general,
+ * symmetric or skew-symmetric) as 0-based CSR with sorted columns and
+ * duplicates summed; a dense single-column array as a vector.  Call with
+ * NULL arrays to get the sizes first. */
+int ainv_mm_read(const char *path, int32_t *n, int64_t *nnz, int64_t *irp,
+                 int32_t *ja, double *val);
+int ainv_mm_read_vector(const char *path, int64_t *n, double *val);
+
 /* Timing and layout report of the last build (JSON text). */
 int ainv_report(ainv_h h, char *buf, size_t len);
 

@@ -82,6 +82,12 @@ def lib():
         "ainv_mat_free": [vp],
         "ainv_mat_spmm": [vp, C.c_double, vp, C.c_double, vp, C.c_int,
                           C.c_int64, C.c_int64, vp],
+        "ainv_dump": [vp, C.c_char_p, C.c_int, C.c_int, C.c_char_p],
+        "ainv_import_factors": [vp, C.c_int32, i64p, i32p, dp, i64p, i32p, dp,
+                                dp, C.c_int],
+        "ainv_mm_read": [C.c_char_p, P(C.c_int32), P(C.c_int64), i64p, i32p,
+                         dp],
+        "ainv_mm_read_vector": [C.c_char_p, P(C.c_int64), dp],
         "ainv_krylov_stream": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_int,
                                C.c_double, C.c_int, vp, vp, vp, vp],
         "ainv_krylov": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_double,
@@ -405,15 +411,29 @@ class _BaseAinvSolver:
         return sp.csr_matrix((val[:nnz.value], ja[:nnz.value], irp),
                              shape=(n.value, n.value))
 
-    def dump(self, prefix):
-        """Writes W, D and Z as MatrixMarket files, the data the reference's
-        sv%dump writes (mld_d_base_ainv_solver_dmp.f90:77-106)."""
-        import scipy.io
-        import scipy.sparse as sp
-        scipy.io.mmwrite(prefix + "_w.mtx", self.export("W"))
-        scipy.io.mmwrite(prefix + "_z.mtx", self.export("Z"))
-        d = self.export("D")
-        scipy.io.mmwrite(prefix + "_d.mtx", sp.diags(d).tocsr())
+    def dump(self, level=1, prefix=None, head=None, solver=True):
+        """sv%dump (mld_d_base_ainv_solver_dmp.f90:56-106): stored W, d and
+        Z as MatrixMarket files <prefix>_p000_l<level>_{wmat,diag,zmat}.mtx
+        (prefix default "dump_ainv_d"); nothing unless solver is true.
+        Returns the three file names."""
+        _ck(lib().ainv_dump(self._h, _b(prefix) if prefix else None, level,
+                            1 if solver else 0, _b(head) if head else None))
+        base = f"{prefix or 'dump_ainv_d'}_p000_l{level:03d}_"
+        return [base + s + ".mtx" for s in ("wmat", "diag", "zmat")]
+
+    def import_factors(self, w, z, d):
+        """Applies factors built elsewhere (stored W and Z as scipy CSR,
+        inverted pivots d), e.g. read back from a dump."""
+        _ensure_init()
+        n, wi, wj, wv = _csr_args(w)
+        n2, zi, zj, zv = _csr_args(z)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        if n2 != n or d.shape != (n,):
+            raise AinvError(ERR_ARG, "W, Z and d sizes differ")
+        _ck(lib().ainv_import_factors(self._h, n, *_ptrs(wi, wj, wv),
+                                      *_ptrs(zi, zj, zv),
+                                      d.ctypes.data_as(C.POINTER(C.c_double)),
+                                      0))
 
 
 class MldDAinvSolver(_BaseAinvSolver):
@@ -537,6 +557,31 @@ def krylov_stream(method, a, prec, b, x=None, slots=32, eps=1e-8,
         None))
     synchronize()
     return x, iters, errs, lat
+
+
+def mm_read(path):
+    """MatrixMarket coordinate file -> scipy CSR (square; duplicates summed;
+    symmetric / skew-symmetric storage expanded)."""
+    import scipy.sparse as sp
+    n = C.c_int32()
+    nnz = C.c_int64()
+    _ck(lib().ainv_mm_read(_b(path), C.byref(n), C.byref(nnz), None, None,
+                           None))
+    irp = np.empty(n.value + 1, dtype=np.int64)
+    ja = np.empty(nnz.value, dtype=np.int32)
+    val = np.empty(nnz.value, dtype=np.float64)
+    _ck(lib().ainv_mm_read(_b(path), None, None, *_ptrs(irp, ja, val)))
+    return sp.csr_matrix((val, ja, irp), shape=(n.value, n.value))
+
+
+def mm_read_vector(path):
+    """MatrixMarket dense single-column array -> numpy vector."""
+    n = C.c_int64()
+    _ck(lib().ainv_mm_read_vector(_b(path), C.byref(n), None))
+    v = np.empty(n.value, dtype=np.float64)
+    _ck(lib().ainv_mm_read_vector(_b(path), None,
+                                  v.ctypes.data_as(C.POINTER(C.c_double))))
+    return v
 
 
 # --------------------------------------------------------------- generators

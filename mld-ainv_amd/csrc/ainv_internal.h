@@ -99,6 +99,13 @@ int value_bytes(int prec);
 // ---- device-side helpers (hll.hip) --------------------------------------
 void csr_upload(const HostCsr &h, DevCsr &d, hipStream_t s);
 void csr_download(const DevCsr &d, HostCsr &h, hipStream_t s);
+
+// MatrixMarket text I/O (mmio.cpp)
+void mm_write_csr(const std::string &path, const HostCsr &a, const char *head);
+void mm_write_vector(const std::string &path, const std::vector<double> &v,
+                     const char *head);
+HostCsr mm_read_csr(const std::string &path);
+std::vector<double> mm_read_vector(const std::string &path);
 // Transpose of a device CSR (rows sorted by column in the result).
 void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s);
 // HLL from device CSR, values converted to `prec`.

@@ -821,6 +821,58 @@ int ainv_export(ainv_h h, char which, int32_t *n, int64_t *nnz, int64_t *irp,
   });
 }
 
+int ainv_dump(ainv_h h, const char *prefix, int level, int solver,
+              const char *head) {
+  return guard([&] {
+    check_handle(h);
+    if (!solver) return;  // the reference dumps the solver only on request
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->built) throw Error(AINV_ERR_STATE, "dump before build");
+    if (level < 0 || level > 999) throw Error(AINV_ERR_ARG, "level out of range");
+    set_device(h->device);
+    const std::string pre = prefix && *prefix ? prefix : "dump_ainv_d";
+    char tail[64];
+    std::snprintf(tail, sizeof tail, "_p000_l%03d_", level);
+    HostCsr w, z;
+    csr_download(h->w, w, h->stream);
+    csr_download(h->z, z, h->stream);
+    std::vector<double> d((size_t)h->n);
+    AINV_HIP_CHECK(hipMemcpy(d.data(), h->d64.p, h->d64.bytes,
+                             hipMemcpyDeviceToHost));
+    mm_write_csr(pre + tail + "wmat.mtx", w, head);
+    mm_write_vector(pre + tail + "diag.mtx", d, head);
+    mm_write_csr(pre + tail + "zmat.mtx", z, head);
+  });
+}
+
+int ainv_import_factors(ainv_h h, int32_t n, const int64_t *w_irp,
+                        const int32_t *w_ja, const double *w_val,
+                        const int64_t *z_irp, const int32_t *z_ja,
+                        const double *z_val, const double *d, int index_base) {
+  return guard([&] {
+    check_handle(h);
+    if (!d) throw Error(AINV_ERR_ARG, "NULL d");
+    HostCsr w = canonical_csr(n, w_irp, w_ja, w_val, index_base);
+    HostCsr z = canonical_csr(n, z_irp, z_ja, z_val, index_base);
+    std::lock_guard<std::mutex> lk(h->mu);
+    set_device(h->device);
+    const double t0 = now_seconds();
+    clear_built(h);
+    ensure_stream(h);
+    h->n = n;
+    csr_upload(w, h->w, h->stream);
+    csr_upload(z, h->z, h->stream);
+    h->d64.alloc((size_t)n * sizeof(double));
+    AINV_HIP_CHECK(hipMemcpyAsync(h->d64.p, d, h->d64.bytes,
+                                  hipMemcpyHostToDevice, h->stream));
+    AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
+    build_layout(h);
+    h->built = true;
+    h->t_factor = h->t_inverse = 0.0;
+    h->t_total = now_seconds() - t0;
+  });
+}
+
 int ainv_report(ainv_h h, char *buf, size_t len) {
   return guard([&] {
     check_handle(h);

@@ -134,6 +134,14 @@ module mld_ainv_hip_cbind
       type(c_ptr), value :: h
       integer(c_int) :: ainv_clear_data
     end function
+    function ainv_dump(h, prefix, level, solver, head) &
+         & bind(c, name='ainv_dump')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char) :: prefix(*), head(*)
+      integer(c_int), value :: level, solver
+      integer(c_int) :: ainv_dump
+    end function
     function ainv_mat_create(n, irp, ja, val, base, a) &
          & bind(c, name='ainv_mat_create')
       import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double

@@ -41,6 +41,7 @@ module mld_d_ainv_hip_mod
     procedure, pass(sv) :: clone_settings => base_clone_settings
     procedure, pass(sv) :: clear_data => base_clear_data
     procedure, pass(sv) :: cnv => base_cnv
+    procedure, pass(sv) :: dump => base_dump
     procedure, pass(sv) :: sizeof => base_sizeof
     procedure, pass(sv) :: get_nzeros => base_get_nzeros
     procedure, nopass :: get_wrksz => base_get_wrksz
@@ -218,6 +219,32 @@ contains
     info = 0
     if (c_associated(sv%h)) info = ainv_clear_data(sv%h)
     sv%n = 0
+  end subroutine
+
+  ! sv%dump (mld_d_base_ainv_solver_dmp.f90:56-106): W, d and Z as
+  ! MatrixMarket files <prefix>_p000_l<level>_{wmat,diag,zmat}.mtx when
+  ! solver is present and true
+  subroutine base_dump(sv, level, info, prefix, head, solver)
+    class(mld_d_base_ainv_solver_type), intent(in) :: sv
+    integer, intent(in) :: level
+    integer, intent(out) :: info
+    character(len=*), intent(in), optional :: prefix, head
+    logical, intent(in), optional :: solver
+    character(len=80) :: prefix_
+    character(len=120) :: head_
+    integer(c_int) :: solver_
+    prefix_ = 'dump_ainv_d'
+    if (present(prefix)) prefix_ = prefix
+    head_ = ' '
+    if (present(head)) head_ = head
+    solver_ = 0
+    if (present(solver)) then
+      if (solver) solver_ = 1
+    end if
+    info = 0
+    if (c_associated(sv%h)) &
+         & info = ainv_dump(sv%h, cstr(prefix_), int(level, c_int), solver_, &
+         & cstr(head_))
   end subroutine
 
   ! re-mold the resident factors: 'F64', 'F32' or 'BF16'

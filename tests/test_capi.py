@@ -7,6 +7,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -101,3 +102,37 @@ def test_fortran_layer_is_built_and_binds_the_abi():
     for sym in ("ainv_build_csr", "ainv_krylov", "ainv_cseti", "ainv_descr",
                 "ainv_gen_pde3d"):
         assert sym in out
+
+
+def test_matrix_market_reader(mld, tmp_path):
+    """ainv_mm_read / ainv_mm_read_vector against scipy's writer: general,
+    symmetric (expanded), pattern and integer fields, duplicates summed;
+    malformed files are errors, not crashes."""
+    import scipy.io
+    import scipy.sparse as sp
+    rng = np.random.default_rng(7)
+    a = sp.random(40, 40, density=0.1, random_state=3, format="csr")
+    a = a + sp.eye(40)
+    scipy.io.mmwrite(str(tmp_path / "g.mtx"), a)
+    b = mld.mm_read(str(tmp_path / "g.mtx"))
+    assert abs(b - a).max() == 0
+    s = (a + a.T).tocsr()
+    scipy.io.mmwrite(str(tmp_path / "s.mtx"), sp.tril(s), symmetry="symmetric")
+    assert abs(mld.mm_read(str(tmp_path / "s.mtx")) - s).max() == 0
+    pat = (a != 0).astype(np.int64)
+    scipy.io.mmwrite(str(tmp_path / "p.mtx"), pat, field="pattern")
+    assert abs(mld.mm_read(str(tmp_path / "p.mtx")) - pat).max() == 0
+    (tmp_path / "d.mtx").write_text(
+        "%%MatrixMarket matrix coordinate real general\n% dup\n"
+        "2 2 3\n1 1 1.5\n1 1 2.0\n2 2 -1\n")
+    dd = mld.mm_read(str(tmp_path / "d.mtx")).toarray()
+    assert dd.tolist() == [[3.5, 0.0], [0.0, -1.0]]
+    v = rng.standard_normal(17)
+    scipy.io.mmwrite(str(tmp_path / "v.mtx"), v.reshape(-1, 1))
+    assert np.array_equal(mld.mm_read_vector(str(tmp_path / "v.mtx")), v)
+    (tmp_path / "bad.mtx").write_text(
+        "%%MatrixMarket matrix coordinate real general\n3 3 2\n1 1 1\n")
+    with pytest.raises(mld.AinvError):
+        mld.mm_read(str(tmp_path / "bad.mtx"))
+    with pytest.raises(mld.AinvError):
+        mld.mm_read(str(tmp_path / "missing.mtx"))
