@@ -1,23 +1,33 @@
 """Benchmark: preconditioned solves/s on one MI355X.
 
-Workload: the 3-D convection-diffusion operator on a 160^3 grid
-(n = 4,096,000, 28.5 M nonzeros), INVK preconditioner (ILU(1) factors,
-inverse fill level 1), BiCGSTAB to 1e-8 relative residual with FP64
-vectors, dots and residuals.  Right-hand sides b_j = A x_j: x_0 = 1 (the
-reference driver's convention), x_j ~ N(0,1) from a fixed seed for j >= 1.
-All right-hand sides are resident in HBM before any timed region.
+Default workload (--config C2, the configuration the metric is quoted on):
+the 3-D convection-diffusion operator on a 160^3 grid (n = 4,096,000,
+28.5 M nonzeros), INVK preconditioner (ILU(1) factors, inverse fill level
+1), BiCGSTAB to 1e-8 relative residual with FP64 vectors, dots and
+residuals.  Right-hand sides b_j = A x_j: x_0 = 1 (the reference driver's
+convention), x_j ~ N(0,1) from a fixed seed for j >= 1.  All right-hand
+sides are resident in HBM before any timed region.
 
 A step is one pass of the served form over a queue of --queue right-hand
-sides: FP32-stored factors, --slots column slots of one captured BiCGSTAB
+sides: FP32-stored factors, --slots column slots of one captured Krylov
 iteration, a slot refilled from the queue as soon as its column converges
 (continuous batching).  value = right-hand sides solved per second.
 
-Also reported, on the same right-hand sides: the plain FP64 single-RHS
-GPU solve (one right-hand side at a time, FP64 factors) and the speedup
-over it; per-solve latency p50 / p99 in the served form; the roofline of
-the dominant kernel (the first apply pass t = d o (W x) at the served block
-width; its average duration measured with HIP events on the solver's
-stream); the CPU baseline (the oracle's BiCGSTAB, single thread).
+Also reported, on right-hand sides of the same kind: the plain FP64
+single-RHS GPU solve (one right-hand side at a time, FP64 factors) and the
+speedup over it; per-solve latency p50 / p99 in the served form; the
+roofline of the dominant kernel (the first apply pass t = d o (W x) at the
+served block width; its average duration measured with HIP events on the
+solver's stream); the CPU baseline (the oracle's Krylov iterations, single
+thread, same factors).
+
+Other configurations (--config): C1 pde3d 40^3 with AINV-LLK (fill 8,
+thresh 0.1), BiCGSTAB 1e-9, b = 1; C3 2-D anisotropic diffusion 2048^2
+(eps = 1e-3) with AINV SYM-LLK (fill 8, thresh 5e-2), CG 1e-8, 32 slots,
+N(0,1) right-hand sides (seed 3); C4 random nonsymmetric n = 10^6
+(LogNormal(ln 12, 1.0) off-diagonals capped at 400, band 5000, 2
+long-range columns, diagonal 1.1 x row sum) with AINV-LLK (fill 8, thresh
+5e-2), BiCGSTAB 1e-8, 16 slots, N(0,1) right-hand sides (seed 4).
 """
 import argparse
 import ctypes as C
@@ -36,9 +46,41 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_w.json")
 
+# name: (description, solver kind, parameters, method, rtol, slots, queue,
+#        right-hand sides ("Ax": x_0 = 1, x_j ~ N(0,1); "ones+N": b_0 = 1,
+#        b_j ~ N(0,1); "N": all N(0,1)), seed)
+CONFIGS = {
+    "C1": ("pde3d 40^3 convection-diffusion, AINV-LLK fill 8 thresh 0.1, "
+           "BiCGSTAB rtol 1e-9", "AINV",
+           dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=0.1), "BICGSTAB",
+           1e-9, 32, 192, "ones+N", 1),
+    "C2": ("pde3d 160^3 convection-diffusion, INVK(1,1), BiCGSTAB rtol 1e-8",
+           "INVK", dict(SUB_FILLIN=1, INV_FILLIN=1), "BICGSTAB", 1e-8, 32,
+           192, "Ax", 5),
+    "C3": ("aniso2d 2048^2 eps 1e-3, AINV SYM-LLK fill 8 thresh 5e-2, "
+           "CG rtol 1e-8", "AINV",
+           dict(AINV_ALG="SYM-LLK", SUB_FILLIN=8, SUB_ILUTHRS=5e-2), "CG",
+           1e-8, 32, 96, "N", 3),
+    "C4": ("random nonsymmetric n=10^6 (LogNormal sigma 1.0), AINV-LLK fill 8 "
+           "thresh 5e-2, BiCGSTAB rtol 1e-8", "AINV",
+           dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=5e-2), "BICGSTAB",
+           1e-8, 16, 96, "N", 4),
+}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def make_matrix(m, name, grid):
+    if name == "C1":
+        return m.gen_pde3d(grid or 40)
+    if name == "C2":
+        return m.gen_pde3d(grid or 160)
+    if name == "C3":
+        return m.gen_aniso2d(grid or 2048, 1e-3)
+    return m.gen_random(grid or 1000000, seed=40, median=12.0, sigma=1.0,
+                        cap=400, band=5000, nlong=2, dom=1.1)
 
 
 def apply_pass_bytes(lay, k, first):
@@ -59,13 +101,14 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--grid", type=int, default=160)
-    p.add_argument("--queue", type=int, default=192)
-    p.add_argument("--slots", type=int, default=32)
+    p.add_argument("--config", default="C2", choices=sorted(CONFIGS))
+    p.add_argument("--grid", type=int, default=0,
+                   help="override the grid size (C1-C3) or order (C4)")
+    p.add_argument("--queue", type=int, default=0)
+    p.add_argument("--slots", type=int, default=0)
     p.add_argument("--single", type=int, default=6,
                    help="right-hand sides 1..single solved one at a time "
                         "in FP64 (the baseline sample)")
-    p.add_argument("--eps", type=float, default=1e-8)
     p.add_argument("--prec", default="F32")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-iters", type=int, default=4)
@@ -74,7 +117,8 @@ def main():
                         "kernels at the served block width (and the copy "
                         "kernel that calibrates the traffic counters)")
     p.add_argument("--map", default="AUTO",
-                   help="APPLY_MAP for --apply-only (AUTO, SPLIT, ROW, COOP)")
+                   help="APPLY_MAP for --apply-only (AUTO, SPLIT, ROW, COOP, "
+                        "COOP128)")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): every "
@@ -86,82 +130,95 @@ def main():
     m = importlib.import_module("mld-ainv_amd")
     m.init(0)
     lib = m.lib()
-    N, M, slots = args.grid, args.queue, args.slots
+    desc, kind, params, method, eps, slots, M, rhs_kind, seed = \
+        CONFIGS[args.config]
+    slots = args.slots or slots
+    M = args.queue or M
     copy_gbs, _ = m.stream_copy_gbs(1 << 30, 10)
     t0 = time.time()
-    a = m.gen_pde3d(N)
+    a = make_matrix(m, args.config, args.grid)
     n = a.shape[0]
-    log(f"matrix {N}^3: n={n} nnz={a.nnz} ({time.time() - t0:.1f}s); "
+    log(f"{args.config}: n={n} nnz={a.nnz} ({time.time() - t0:.1f}s); "
         f"copy kernel {copy_gbs:.0f} GB/s")
+
+    def new_solver(prec):
+        sv = m.SOLVERS[kind]()
+        for key, v in params.items():
+            sv.set(key, v)
+        sv.set("FACTOR_PREC", prec)
+        return sv
+
     if args.apply_only:
-        sv = m.MldDInvkSolver()
-        sv.set("SUB_FILLIN", 1)
-        sv.set("INV_FILLIN", 1)
-        sv.set("FACTOR_PREC", args.prec)
+        sv = new_solver(args.prec)
         sv.set("APPLY_MAP", args.map)
         sv.build(a)
         lay = sv.layout()
         ms_w, ms_z = profile_apply(m, sv, n, slots, 10)
         bw = apply_pass_bytes(lay, slots, True)
-        print(json.dumps({"apply_only": True, "prec": args.prec,
-                          "map": args.map, "k": slots, "ms_w": ms_w,
-                          "ms_z": ms_z, "bytes_w": bw,
+        print(json.dumps({"apply_only": True, "config": args.config,
+                          "prec": args.prec, "map": args.map, "k": slots,
+                          "ms_w": ms_w, "ms_z": ms_z, "bytes_w": bw,
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
         return
     A = m.SystemMatrix(a)
 
-    # ---- right-hand sides b_j = A x_j, resident on the device
+    # ---- right-hand sides, resident on the device: M n-vectors
     t0 = time.time()
-    X = m.gen_normal(5, (M, n))
-    X[0] = 1.0
-    dXt = m.DeviceArray.from_numpy(X)
-    del X
-    dB = m.DeviceArray((M, n))
-    for j in range(M):
-        off = j * n * 8
-        m._ck(lib.ainv_mat_spmm(A._h, 1.0, C.c_void_p(dXt.ptr.value + off),
-                                0.0, C.c_void_p(dB.ptr.value + off), 1, 1, 1,
-                                None))
-    m.synchronize()
-    dXt.free()
+    if rhs_kind == "Ax":
+        X = m.gen_normal(seed, (M, n))
+        X[0] = 1.0
+        dXt = m.DeviceArray.from_numpy(X)
+        del X
+        dB = m.DeviceArray((M, n))
+        for j in range(M):
+            off = j * n * 8
+            m._ck(lib.ainv_mat_spmm(A._h, 1.0,
+                                    C.c_void_p(dXt.ptr.value + off), 0.0,
+                                    C.c_void_p(dB.ptr.value + off), 1, 1, 1,
+                                    None))
+        m.synchronize()
+        dXt.free()
+    else:
+        Bn = m.gen_normal(seed, (M, n))
+        if rhs_kind == "ones+N":
+            Bn[0] = 1.0
+        dB = m.DeviceArray.from_numpy(Bn)
+        del Bn
     log(f"{M} right-hand sides on the device ({time.time() - t0:.1f}s)")
 
-    sv = m.MldDInvkSolver()
-    sv.set("SUB_FILLIN", 1)
-    sv.set("INV_FILLIN", 1)
-    sv.set("FACTOR_PREC", "F64")
+    sv = new_solver("F64")
     t0 = time.time()
     sv.build(a)
     t_build = time.time() - t0
-    log(f"INVK build {t_build:.2f}s")
+    log(f"{kind} build {t_build:.2f}s")
 
-    # ---- plain FP64 single-RHS GPU solves of the first right-hand sides
+    # ---- plain FP64 single-RHS GPU solves of a sample of the queue
     dx1 = m.DeviceArray((n,))
-    Bh = dB.to_numpy() if args.single > 0 else None
+    Bh = dB.to_numpy()
+    # the sample skips b_0 (A*1 or 1, the reference driver's right-hand
+    # side; it appears once in the queue)
+    sample = list(range(1, min(args.single, M - 1) + 1))
 
     def single(j):
         db1 = m.DeviceArray.from_numpy(Bh[j])
         dx1.zero()
         m.synchronize()
         ts = time.time()
-        _, it, err = m.krylov("BICGSTAB", A, sv, db1, dx1, eps=args.eps,
-                              itmax=2000)
+        _, it, err = m.krylov(method, A, sv, db1, dx1, eps=eps, itmax=5000)
         m.synchronize()
         dt = time.time() - ts
         db1.free()
         return dt, int(it[0]), float(err[0])
 
-    single(0)  # warm-up (plan, graph capture)
+    single(sample[0])  # warm-up (plan, graph capture)
     t64, it64 = [], []
-    # the sample skips b_0 = A*1, which needs about 1.6x the iterations of
-    # the N(0,1) right-hand sides that make up the rest of the queue
-    for j in range(1, args.single + 1):
+    for j in sample:
         dt, it, err = single(j)
         t64.append(dt)
         it64.append(it)
     x64 = dx1.to_numpy()
-    true64 = float(np.linalg.norm(Bh[args.single] - a @ x64) /
-                   np.linalg.norm(Bh[args.single]))
+    true64 = float(np.linalg.norm(Bh[sample[-1]] - a @ x64) /
+                   np.linalg.norm(Bh[sample[-1]]))
     lay64 = sv.layout()
     ms_w64, ms_z64 = profile_apply(m, sv, n, 1, 20)
     gbs64 = ((apply_pass_bytes(lay64, 1, True) +
@@ -176,17 +233,16 @@ def main():
     sv.cnv(args.prec)
     lay = sv.layout()
     dX = m.DeviceArray((M, n))
-    warm = m.DeviceArray.from_numpy(Bh[:slots]) if Bh is not None else dB
+    warm = m.DeviceArray.from_numpy(Bh[:min(slots, M)])
     for _ in range(args.warmup):
-        m.krylov_stream("BICGSTAB", A, sv, warm, slots=slots, eps=args.eps,
-                        itmax=2000)
+        m.krylov_stream(method, A, sv, warm, slots=slots, eps=eps, itmax=5000)
     m.synchronize()
+    warm.free()
     lat, iters, errs = [], None, None
     t_start = time.time()
     for _ in range(args.steps):
-        _, iters, errs, l = m.krylov_stream("BICGSTAB", A, sv, dB, dX,
-                                            slots=slots, eps=args.eps,
-                                            itmax=2000)
+        _, iters, errs, l = m.krylov_stream(method, A, sv, dB, dX,
+                                            slots=slots, eps=eps, itmax=5000)
         lat.extend(l.tolist())
     m.synchronize()
     elapsed = time.time() - t_start
@@ -197,8 +253,7 @@ def main():
     rep = sv.report()
     tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
     kmap = tuned[0]["map_w"] if tuned else "split"
-    # same right-hand sides solved singly at reduced precision: iterations
-    it_red = [int(v) for v in iters[1:args.single + 1]]
+    it_red = [int(iters[j]) for j in sample]
     log(f"served {args.prec} slots={slots}: {rate:.2f} solves/s, "
         f"{elapsed / args.steps:.2f} s/step, iterations mean "
         f"{float(np.mean(iters)):.1f} max {int(np.max(iters))}, tuning {tuned}")
@@ -217,14 +272,15 @@ def main():
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if tr.get("kernel") == kname and tr.get("grid") == N:
+        if (tr.get("kernel") == kname and tr.get("config") == args.config
+                and tr.get("n") == n):
             traffic = tr.get("bytes_per_launch")
     log(f"apply at k={slots}: W {ms_w:.3f} ms ({gbs_w:.0f} GB/s), "
         f"Z {ms_z:.3f} ms")
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sv, a, Bh[0], args.cpu_iters,
+        cpu = cpu_baseline(sv, a, Bh[sample[0]], method, args.cpu_iters,
                            float(np.mean(it64)))
         log(f"cpu baseline {cpu}")
 
@@ -244,12 +300,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"pde3d {N}^3 convection-diffusion, INVK(1,1), "
-                        f"BiCGSTAB rtol {args.eps:g}; served form: queue of "
-                        f"{M} right-hand sides through {slots} slots, "
+            "workload": f"{args.config}: {desc}; served form: queue of {M} "
+                        f"right-hand sides through {slots} slots, "
                         f"{args.prec}-stored factors, FP64 Krylov",
-            "n": n, "nnz": int(a.nnz), "queue": M, "slots": slots,
-            "factor_prec": args.prec, "parallelism": "1 GPU",
+            "name": args.config, "n": n, "nnz": int(a.nnz), "queue": M,
+            "slots": slots, "factor_prec": args.prec, "parallelism": "1 GPU",
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99)),
@@ -259,7 +314,8 @@ def main():
                        "served_max": int(np.max(iters)),
                        "served_sample": it_red,
                        "fp64_single_sample": it64,
-                       "sample": f"right-hand sides 1..{args.single}"},
+                       "sample": f"right-hand sides {sample[0]}.."
+                                 f"{sample[-1]}"},
         "true_residual_max": true,
         "fp64_single_rhs": {
             "solves_per_s": fp64_rate, "ms_mean": 1e3 * float(np.mean(t64)),
@@ -295,24 +351,24 @@ def profile_apply(m, sv, n, k, reps):
     return a.value, b.value
 
 
-def cpu_baseline(sv, a, b, iters, gpu_iters):
-    """Oracle BiCGSTAB (single thread) for `iters` iterations on the same
-    system with the same FP64 INVK factors, extrapolated to one solve at the
-    FP64 GPU path's mean iteration count."""
+def cpu_baseline(sv, a, b, method, iters, gpu_iters):
+    """The oracle's Krylov method (single thread) for `iters` iterations on
+    the same system with the same FP64 factors, extrapolated to one solve at
+    the FP64 GPU path's mean iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sv.cnv("F64")
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
     t0 = time.time()
-    _, it, _ = O.krylov("BICGSTAB", a, w, z, d, b, 1e-30, itmax=iters)
+    _, it, _ = O.krylov(method, a, w, z, d, b, 1e-30, itmax=iters)
     dt = time.time() - t0
     per_iter = dt / max(it, 1)
     return {"value": 1.0 / (per_iter * gpu_iters), "unit": "solves/s",
             "cores": 1, "kind": "port",
-            "sample": f"{it} BiCGSTAB iterations of the oracle on the same "
-                      f"matrix, right-hand side A*1 and FP64 factors "
-                      f"({dt:.1f} s), extrapolated to {gpu_iters:.1f} "
-                      f"iterations per solve",
+            "sample": f"{it} {method} iterations of the oracle on the same "
+                      f"matrix, one right-hand side of the queue and the "
+                      f"FP64 factors ({dt:.1f} s), extrapolated to "
+                      f"{gpu_iters:.1f} iterations per solve",
             "seconds_per_iteration": per_iter}
 
 
