@@ -21,6 +21,16 @@ served block width; its average duration measured with HIP events on the
 solver's stream); the CPU baseline (the oracle's Krylov iterations, single
 thread, same factors).
 
+C5 (--config C5) measures the serving engine instead: a pool of 24
+matrices (12 pde3d with N ~ U{20..120}, INVK(1,1); 12 random as C4 with
+n ~ logU[1e4, --c5-nmax], AINV-LLK 5e-2; FP32 factors, BiCGSTAB 1e-8)
+registered with one server; requests of k ~ U{1..64} N(0,1) columns on a
+uniformly drawn matrix.  After one warm-up request per matrix (first-touch
+builds), a closed-loop burst measures the capacity, then requests arrive
+as a Poisson process at 0.8x that capacity; value = right-hand sides solved
+per second in the open-loop phase, latency = submission to completion per
+request.
+
 Other configurations (--config): C1 pde3d 40^3 with AINV-LLK (fill 8,
 thresh 0.1), BiCGSTAB 1e-9, b = 1; C3 2-D anisotropic diffusion 2048^2
 (eps = 1e-3) with AINV SYM-LLK (fill 8, thresh 5e-2), CG 1e-8, 32 slots,
@@ -101,7 +111,11 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--config", default="C2", choices=sorted(CONFIGS))
+    p.add_argument("--config", default="C2", choices=sorted(CONFIGS) + ["C5"])
+    p.add_argument("--c5-nmax", type=float, default=2e5,
+                   help="largest random matrix of the C5 pool")
+    p.add_argument("--c5-requests", type=int, default=96)
+    p.add_argument("--workers", type=int, default=4)
     p.add_argument("--grid", type=int, default=0,
                    help="override the grid size (C1-C3) or order (C4)")
     p.add_argument("--queue", type=int, default=0)
@@ -130,6 +144,9 @@ def main():
     m = importlib.import_module("mld-ainv_amd")
     m.init(0)
     lib = m.lib()
+    if args.config == "C5":
+        serve_bench(m, args)
+        return
     desc, kind, params, method, eps, slots, M, rhs_kind, seed = \
         CONFIGS[args.config]
     slots = args.slots or slots
@@ -335,6 +352,125 @@ def main():
         },
         "build": {"seconds": t_build},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def serve_bench(m, args):
+    import threading
+    rng = np.random.default_rng(5)
+    pool = []
+    for i in range(12):
+        N = int(rng.integers(20, 121))
+        pool.append((f"pde3d N={N}", m.gen_pde3d(N), "INVK",
+                     dict(SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC="F32",
+                          METHOD="BICGSTAB")))
+    for i in range(12):
+        n = int(np.exp(rng.uniform(np.log(1e4), np.log(args.c5_nmax))))
+        pool.append((f"random n={n}",
+                     m.gen_random(n, seed=100 + i, median=12.0, sigma=1.0,
+                                  cap=400, band=5000, nlong=2, dom=1.1),
+                     "AINV", dict(AINV_ALG="LLK", SUB_FILLIN=8,
+                                  SUB_ILUTHRS=5e-2, FACTOR_PREC="F32",
+                                  METHOD="BICGSTAB")))
+    sizes = [a.shape[0] for _, a, _, _ in pool]
+    log(f"C5 pool: {len(pool)} matrices, n from {min(sizes)} to {max(sizes)}")
+    srv = m.Server(hbm_budget_bytes=200 << 30, workers=args.workers)
+    ids = [srv.add_matrix(a, kind, **prm) for _, a, kind, prm in pool]
+    blocks = [m.gen_normal(500 + j, (sizes[j], 64)) for j in range(len(pool))]
+    t0 = time.time()
+    warm = [srv.submit(ids[j], np.ascontiguousarray(blocks[j][:, :1]),
+                       eps=1e-8) for j in range(len(pool))]
+    for t in warm:
+        srv.wait(t)
+    t_warm = time.time() - t0
+    log(f"first-touch builds + one solve per matrix: {t_warm:.1f}s")
+
+    def requests(count, seed):
+        rr = np.random.default_rng(seed)
+        out = []
+        for _ in range(count):
+            j = int(rr.integers(0, len(pool)))
+            k = int(rr.integers(1, 65))
+            out.append((j, k, np.ascontiguousarray(blocks[j][:, :k])))
+        return out
+
+    def check(res):
+        worst = max(float(np.max(e)) for _, _, e, _ in res)
+        if worst > 1e-8:
+            raise SystemExit(f"C5: a request ended at ||r||/||b|| = {worst}")
+
+    # closed-loop burst: capacity
+    burst = requests(max(24, args.c5_requests // 2), 6)
+    t0 = time.time()
+    tk = [srv.submit(ids[j], b, eps=1e-8, itmax=2000) for j, _, b in burst]
+    res = [srv.wait(t) for t in tk]
+    t_burst = time.time() - t0
+    check(res)
+    cap_req = len(burst) / t_burst
+    cap_cols = sum(k for _, k, _ in burst) / t_burst
+    log(f"capacity: {cap_req:.2f} requests/s, {cap_cols:.1f} columns/s")
+
+    # open loop: Poisson arrivals at 0.8x capacity
+    reqs = requests(args.c5_requests, 7)
+    lam = 0.8 * cap_req
+    arr = np.cumsum(np.random.default_rng(8).exponential(1.0 / lam,
+                                                         len(reqs)))
+    tickets, results = [], []
+    cv = threading.Condition()
+
+    def waiter():
+        for i in range(len(reqs)):
+            with cv:
+                while len(tickets) <= i:
+                    cv.wait()
+                t = tickets[i]
+            results.append(srv.wait(t))
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t_start = time.time()
+    for (j, _, b), ta in zip(reqs, arr):
+        delay = t_start + ta - time.time()
+        if delay > 0:
+            time.sleep(delay)
+        t = srv.submit(ids[j], b, eps=1e-8, itmax=2000)
+        with cv:
+            tickets.append(t)
+            cv.notify()
+    th.join()
+    elapsed = time.time() - t_start
+    check(results)
+    cols = sum(k for _, k, _ in reqs)
+    lat = np.array([r[3] for r in results])
+    st = srv.stats()
+    srv.free()
+    out = {
+        "metric": "preconditioned solves/s + apply HBM GB/s (% roofline); "
+                  "p50/p99 solve latency",
+        "value": cols / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": elapsed * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {
+            "workload": "C5: serving pool of 24 matrices (12 pde3d N~U{20..120} "
+                        "INVK(1,1); 12 random n~logU[1e4, "
+                        f"{args.c5_nmax:.0e}] AINV-LLK 5e-2), FP32 factors, "
+                        "BiCGSTAB 1e-8, requests of U{1..64} columns, Poisson "
+                        "arrivals at 0.8x the measured capacity",
+            "name": "C5", "requests": len(reqs), "columns": cols,
+            "workers": args.workers, "parallelism": "1 GPU"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                       "p99": float(np.percentile(lat, 99)),
+                       "note": "submission to completion per request "
+                               "(host buffers, queueing included)"},
+        "capacity": {"requests_per_s": cap_req, "columns_per_s": cap_cols},
+        "arrival_rate_per_s": lam,
+        "first_touch_seconds": t_warm,
+        "server_stats": st,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
 
