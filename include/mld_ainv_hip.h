@@ -185,6 +185,14 @@ int ainv_mm_read(const char *path, int32_t *n, int64_t *nnz, int64_t *irp,
                  int32_t *ja, double *val);
 int ainv_mm_read_vector(const char *path, int64_t *n, double *val);
 
+/* sv%update_a (mld_d_base_ainv_update_a.f90:53-79): adapt the built
+ * preconditioner to the shifted operator A + diag(x) by updating the
+ * inverted pivots, d_i = 1 / (1/d_i + sum_k W(i,k) x_k Z(k,i)) (stored W by
+ * rows, Z by columns); W and Z are kept.  x has nx >= n entries (fewer:
+ * AINV_ERR_ARG, the reference's "Wrong size in update"); device pointer, or
+ * host pointer with flags = AINV_HOST_PTRS. */
+int ainv_update_a(ainv_h h, const double *x, int64_t nx, int flags);
+
 /* Timing and layout report of the last build (JSON text). */
 int ainv_report(ainv_h h, char *buf, size_t len);
 

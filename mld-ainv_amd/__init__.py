@@ -83,6 +83,7 @@ def lib():
         "ainv_mat_spmm": [vp, C.c_double, vp, C.c_double, vp, C.c_int,
                           C.c_int64, C.c_int64, vp],
         "ainv_dump": [vp, C.c_char_p, C.c_int, C.c_int, C.c_char_p],
+        "ainv_update_a": [vp, vp, C.c_int64, C.c_int],
         "ainv_import_factors": [vp, C.c_int32, i64p, i32p, dp, i64p, i32p, dp,
                                 dp, C.c_int],
         "ainv_mm_read": [C.c_char_p, P(C.c_int32), P(C.c_int64), i64p, i32p,
@@ -420,6 +421,17 @@ class _BaseAinvSolver:
                             1 if solver else 0, _b(head) if head else None))
         base = f"{prefix or 'dump_ainv_d'}_p000_l{level:03d}_"
         return [base + s + ".mtx" for s in ("wmat", "diag", "zmat")]
+
+    def update_a(self, x):
+        """sv%update_a: inverted pivots for the shifted operator A + diag(x)
+        (mld_d_base_ainv_update_a.f90:53-79); x: numpy (host) or
+        DeviceArray."""
+        if isinstance(x, DeviceArray):
+            _ck(lib().ainv_update_a(self._h, x.ptr, x.shape[0], 0))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            _ck(lib().ainv_update_a(self._h, x.ctypes.data_as(C.c_void_p),
+                                    x.shape[0], HOST_PTRS))
 
     def import_factors(self, w, z, d):
         """Applies factors built elsewhere (stored W and Z as scipy CSR,

@@ -99,6 +99,9 @@ int value_bytes(int prec);
 // ---- device-side helpers (hll.hip) --------------------------------------
 void csr_upload(const HostCsr &h, DevCsr &d, hipStream_t s);
 void csr_download(const DevCsr &d, HostCsr &h, hipStream_t s);
+// d_i = 1 / (1/d_i + sum_k W(i,k) x_k Z(k,i)) with zt = Z^T (hll.hip)
+void update_pivots(const DevCsr &w, const DevCsr &zt, const double *x,
+                   double *d, hipStream_t s);
 
 // MatrixMarket text I/O (mmio.cpp)
 void mm_write_csr(const std::string &path, const HostCsr &a, const char *head);

@@ -873,6 +873,34 @@ int ainv_import_factors(ainv_h h, int32_t n, const int64_t *w_irp,
   });
 }
 
+int ainv_update_a(ainv_h h, const double *x, int64_t nx, int flags) {
+  return guard([&] {
+    check_handle(h);
+    if (!x) throw Error(AINV_ERR_ARG, "NULL x");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->built) throw Error(AINV_ERR_STATE, "update_a before build");
+    if (nx < h->n)
+      throw Error(AINV_ERR_ARG, "wrong size in update: " + std::to_string(nx) +
+                                    " < " + std::to_string(h->n));
+    set_device(h->device);
+    DevBuf dx;
+    const double *xd = x;
+    if (flags & AINV_HOST_PTRS) {
+      dx.alloc((size_t)h->n * sizeof(double));
+      AINV_HIP_CHECK(hipMemcpyAsync(dx.p, x, dx.bytes, hipMemcpyHostToDevice,
+                                    h->stream));
+      xd = dx.as<double>();
+    }
+    DevCsr zt;
+    csr_transpose(h->z, zt, h->stream);
+    update_pivots(h->w, zt, xd, h->d64.as<double>(), h->stream);
+    AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
+    // d in storage precision is reallocated: captured graphs are stale
+    convert_vector(h->d64.as<double>(), h->n, h->prec, h->dv, h->stream);
+    h->generation = next_generation();
+  });
+}
+
 int ainv_report(ainv_h h, char *buf, size_t len) {
   return guard([&] {
     check_handle(h);

@@ -308,6 +308,55 @@ void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s) {
   AINV_HIP_CHECK(hipGetLastError());
 }
 
+namespace {
+// sv%update_a pivots (mld_d_base_ainv_update_a.f90:53-79): one thread per
+// row merges row i of the stored W with row i of Z^T (= column i of Z),
+// both sorted, summing W(i,k) * (x_k * Z(k,i)) in ascending k; then
+// d_i = 1 / (1/d_i + ee).  Compiled without contraction: same roundings as
+// the sequential restatement.
+__global__ void k_update_pivots(int32_t n, const int64_t *__restrict__ wp,
+                                const int32_t *__restrict__ wi,
+                                const double *__restrict__ wv,
+                                const int64_t *__restrict__ zp,
+                                const int32_t *__restrict__ zi,
+                                const double *__restrict__ zv,
+                                const double *__restrict__ x,
+                                double *__restrict__ d) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  double ee = 0.0;
+  int64_t p = wp[i], q = zp[i];
+  const int64_t pe = wp[i + 1], qe = zp[i + 1];
+  while (p < pe && q < qe) {
+    const int32_t a = wi[p], b = zi[q];
+    if (a == b) {
+      const double sz = x[b] * zv[q];
+      ee = ee + wv[p] * sz;
+      ++p;
+      ++q;
+    } else if (a < b) {
+      ++p;
+    } else {
+      ++q;
+    }
+  }
+  double dd = 1.0 / d[i];
+  dd = dd + ee;
+  d[i] = 1.0 / dd;
+}
+}  // namespace
+
+void update_pivots(const DevCsr &w, const DevCsr &zt, const double *x,
+                   double *d, hipStream_t s) {
+  if (w.n == 0) return;
+  hipLaunchKernelGGL(k_update_pivots, dim3((unsigned)((w.n + 255) / 256)),
+                     dim3(256), 0, s, w.n, w.ptr.as<int64_t>(),
+                     w.idx.as<int32_t>(), w.val.as<double>(),
+                     zt.ptr.as<int64_t>(), zt.idx.as<int32_t>(),
+                     zt.val.as<double>(), x, d);
+  AINV_HIP_CHECK(hipGetLastError());
+}
+
 void convert_vector(const double *src, int64_t n, int prec, DevBuf &dst,
                     hipStream_t s) {
   // d is kept FP64 for F64 factors and FP32 otherwise

@@ -12,6 +12,7 @@ module mld_ainv_hip_cbind
   integer(c_int), parameter :: ainv_prec_f64 = 0, ainv_prec_f32 = 1, &
        & ainv_prec_bf16 = 2
   integer(c_int), parameter :: ainv_host_ptrs = 1
+  integer(c_int), parameter :: ainv_err_state = 6
 
   interface
     function ainv_init(device) bind(c, name='ainv_init')
@@ -133,6 +134,14 @@ module mld_ainv_hip_cbind
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int) :: ainv_clear_data
+    end function
+    function ainv_update_a(h, x, nx, flags) bind(c, name='ainv_update_a')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: h
+      real(c_double) :: x(*)
+      integer(c_int64_t), value :: nx
+      integer(c_int), value :: flags
+      integer(c_int) :: ainv_update_a
     end function
     function ainv_dump(h, prefix, level, solver, head) &
          & bind(c, name='ainv_dump')

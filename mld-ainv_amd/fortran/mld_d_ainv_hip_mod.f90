@@ -42,6 +42,7 @@ module mld_d_ainv_hip_mod
     procedure, pass(sv) :: clear_data => base_clear_data
     procedure, pass(sv) :: cnv => base_cnv
     procedure, pass(sv) :: dump => base_dump
+    procedure, pass(sv) :: update_a => base_update_a
     procedure, pass(sv) :: sizeof => base_sizeof
     procedure, pass(sv) :: get_nzeros => base_get_nzeros
     procedure, nopass :: get_wrksz => base_get_wrksz
@@ -245,6 +246,21 @@ contains
     if (c_associated(sv%h)) &
          & info = ainv_dump(sv%h, cstr(prefix_), int(level, c_int), solver_, &
          & cstr(head_))
+  end subroutine
+
+  ! sv%update_a (mld_d_base_ainv_update_a.f90:53-79): pivots of the
+  ! shifted operator A + diag(x) with the same W and Z
+  subroutine base_update_a(sv, x, info)
+    class(mld_d_base_ainv_solver_type), intent(inout) :: sv
+    real(c_double), intent(in) :: x(:)
+    integer, intent(out) :: info
+    real(c_double), allocatable :: xc(:)
+    if (.not. c_associated(sv%h)) then
+      info = ainv_err_state
+      return
+    end if
+    xc = x
+    info = ainv_update_a(sv%h, xc, int(size(x), c_int64_t), ainv_host_ptrs)
   end subroutine
 
   ! re-mold the resident factors: 'F64', 'F32' or 'BF16'

@@ -1142,6 +1142,37 @@ int or_apply(char trans, double alpha, const or_mat *w, const or_mat *z,
   return OR_OK;
 }
 
+int or_update_a(const or_mat *w, const or_mat *z, const double *x, double *d) {
+  /* column i of Z = row i of Z^T; the reference scales Z's rows by x first
+   * (ac%scal(ee, side='L'), :68) and then merges (:70-76) */
+  or_mat zt;
+  int rc = or_transpose(z, &zt);
+  if (rc != OR_OK) return rc;
+  for (int32_t i = 0; i < w->n; i++) {
+    double ee = 0.0;
+    int64_t p = w->ptr[i], pe = w->ptr[i + 1];
+    int64_t q = zt.ptr[i], qe = zt.ptr[i + 1];
+    while (p < pe && q < qe) {
+      const int32_t a = w->idx[p], b = zt.idx[q];
+      if (a == b) {
+        const double sz = x[b] * zt.val[q];
+        ee = ee + w->val[p] * sz;
+        p++;
+        q++;
+      } else if (a < b) {
+        p++;
+      } else {
+        q++;
+      }
+    }
+    double dd = 1.0 / d[i]; /* :77-79 */
+    dd = dd + ee;
+    d[i] = 1.0 / dd;
+  }
+  or_mat_free(&zt);
+  return OR_OK;
+}
+
 static double dot(int32_t n, const double *a, const double *b) {
   double s = 0.0;
   for (int32_t i = 0; i < n; i++) s = s + a[i] * b[i];

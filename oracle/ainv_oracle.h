@@ -93,6 +93,13 @@ int or_transpose(const or_mat *a, or_mat *t);
 void or_spmv(char trans, double alpha, const or_mat *a, const double *x,
              double beta, double *y);
 
+/* sv%update_a (mld_d_base_ainv_update_a.f90:53-79) for the shifted
+ * operator A + diag(x): ee_i = sum_k W(i,k) * (x_k * Z(k,i)) over the common
+ * indices of row i of the stored W and column i of Z, in ascending k
+ * (psb_spdot_srtd), then d_i = 1 / (1/d_i + ee_i), in place.  Rows of w
+ * and z sorted. */
+int or_update_a(const or_mat *w, const or_mat *z, const double *x, double *d);
+
 /* Preconditioner apply (mld_d_base_ainv_solver_apply_vect.f90:112-127):
  * 'N': y = alpha*Z*(d o (W*x)) + beta*y;  'T': y = alpha*W^T*(d o (Z^T x))
  * + beta*y.  Returns OR_ERR_ARG for any other trans (:63-69). */

@@ -57,6 +57,8 @@ def lib():
                             P(C.c_double)],
             "or_invt_bld": [P(OrMat), C.c_int32, C.c_double, C.c_int32,
                             C.c_double, P(OrMat), P(OrMat), P(C.c_double)],
+            "or_update_a": [P(OrMat), P(OrMat), P(C.c_double),
+                            P(C.c_double)],
             "or_apply": [C.c_char, C.c_double, P(OrMat), P(OrMat),
                          P(C.c_double), P(C.c_double), C.c_double, P(C.c_double)],
             "or_bicgstab": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
@@ -199,6 +201,15 @@ def invt_bld(a, fill, thresh, invfill, invthresh):
     _check(lib().or_invt_bld(A.ref, fill, thresh, invfill, invthresh,
                              C.byref(w), C.byref(z), _dp(d)), "invt_bld")
     return _out_to_csr(w), _out_to_csr(z), d
+
+
+def update_a(w, z, d, x):
+    """sv%update_a: inverted pivots for A + diag(x) with the same W, Z."""
+    W, Z = _In(w), _In(z)
+    d = np.array(d, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    _check(lib().or_update_a(W.ref, Z.ref, _dp(x), _dp(d)), "update_a")
+    return d
 
 
 def apply(w, z, d, x, alpha=1.0, beta=0.0, y=None, trans="N"):

@@ -357,3 +357,42 @@ def test_dump_and_import_roundtrip(gpu, O, tmp_path):
     ref = O.apply(w, z, d, x)
     assert np.linalg.norm(sv2.apply(x) - ref) <= 1e-12 * np.linalg.norm(ref)
     assert np.array_equal(sv2.apply(x, trans="T"), sv.apply(x, trans="T"))
+
+
+@pytest.mark.parametrize("kind", ["INVK", "AINV"])
+def test_update_a_matches_oracle(gpu, O, kind):
+    """sv%update_a on the GPU: the updated pivots equal the oracle's bit for
+    bit, the factors stay, the apply uses the new pivots (also through a
+    captured Krylov graph, recaptured after the update), and a short x is
+    rejected."""
+    a = gpu.gen_pde3d(10)
+    n = a.shape[0]
+    params = (dict(SUB_FILLIN=1, INV_FILLIN=1) if kind == "INVK" else
+              dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=0.1))
+    sv = build(gpu, kind, a, **params)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    x = gpu.gen_normal(12, (n,)) * 0.1 + 0.5
+    A = gpu.SystemMatrix(a)
+    b = a @ np.ones(n)
+    gpu.krylov("BICGSTAB", A, sv, b, eps=1e-8)      # capture a graph first
+    sv.update_a(x)
+    d_ref = O.update_a(w, z, d, x)
+    assert np.array_equal(sv.export("D"), d_ref)
+    assert same_csr(sv.export("W"), w) and same_csr(sv.export("Z"), z)
+    v = gpu.gen_normal(13, (n,))
+    ref = O.apply(w, z, d_ref, v)
+    assert np.linalg.norm(sv.apply(v) - ref) <= 1e-12 * np.linalg.norm(ref)
+    # the shifted system solved with the updated preconditioner
+    ash = (a + sp.diags(x)).tocsr()
+    Ash = gpu.SystemMatrix(ash)
+    bsh = ash @ np.ones(n)
+    xs, it, err = gpu.krylov("BICGSTAB", Ash, sv, bsh, eps=1e-8)
+    _, it_ref, _ = O.krylov("BICGSTAB", ash, w, z, d_ref, bsh, 1e-8)
+    assert abs(int(it[0]) - it_ref) <= 1
+    assert np.linalg.norm(bsh - ash @ xs) <= 1e-7 * np.linalg.norm(bsh)
+    # the graph captured before the update is recaptured with the new d
+    _, it2, _ = gpu.krylov("BICGSTAB", A, sv, b, eps=1e-8)
+    _, it2_ref, _ = O.krylov("BICGSTAB", a, w, z, d_ref, b, 1e-8)
+    assert abs(int(it2[0]) - it2_ref) <= 1
+    with pytest.raises(gpu.AinvError):
+        sv.update_a(x[:-1])

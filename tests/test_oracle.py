@@ -298,6 +298,44 @@ def test_apply_closed_form(O, trans):
         O.apply(w, z, d, x, trans="X")
 
 
+# ------------------------------------------------------------- update_a
+def test_update_a_matrix_form(O):
+    """Arbitrary sparse W, Z: d_new = 1 / (1/d + diag(W diag(x) Z)) (the
+    stored W by rows, Z by columns; a transposed operand fails this)."""
+    rng = np.random.default_rng(4)
+    n = 30
+    w = sp.random(n, n, density=0.2, random_state=1, format="csr") + sp.eye(n)
+    z = sp.random(n, n, density=0.2, random_state=2, format="csr") + sp.eye(n)
+    d = rng.uniform(0.5, 2.0, n)
+    x = rng.standard_normal(n)
+    got = O.update_a(w, z, d, x)
+    ee = np.diag(w.toarray() @ np.diag(x) @ z.toarray())
+    assert np.allclose(got, 1.0 / (1.0 / d + ee), rtol=1e-13, atol=0)
+    assert not np.allclose(
+        got, 1.0 / (1.0 / d + np.diag(w.toarray().T @ np.diag(x) @ z.toarray())))
+    # x = 0 leaves d as 1/(1/d)
+    assert np.allclose(O.update_a(w, z, d, np.zeros(n)), d, rtol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["INVK", "AINV"])
+def test_update_a_exact_factors_give_shifted_pivots(O, kind):
+    """With exact factors (no dropping) W A Z = diag(1/d); the update must
+    give the pivots of the same W, Z against A + diag(x):
+    d_new = 1 / diag(W (A + X) Z)."""
+    a = R.pde3d_numpy(3)
+    n = a.shape[0]
+    if kind == "INVK":
+        w, z, d = O.invk_bld(a, n, n)
+    else:
+        w, z, d = O.ainv_bld(a, "LLK", n, 0.0)
+    W, Z = w.toarray(), z.toarray()
+    assert np.allclose(W @ a.toarray() @ Z, np.diag(1.0 / d), atol=1e-10)
+    x = np.random.default_rng(5).uniform(0.1, 1.0, n)
+    got = O.update_a(w, z, d, x)
+    ref = 1.0 / np.diag(W @ (a.toarray() + np.diag(x)) @ Z)
+    assert np.allclose(got, ref, rtol=1e-10, atol=0)
+
+
 # ------------------------------------------------------------- Krylov
 def _scipy_iters(method, a, M, b, rtol):
     count = [0]
