@@ -205,7 +205,9 @@ int ainv_mat_free(ainv_mat_h a);
 int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
                   double *y, int k, int64_t ldx, int64_t ldy, void *stream);
 
-/* method: "BICGSTAB" or "CG"; istop = 2 (||r||_2 <= eps ||b||_2); x holds
+/* method: "BICGSTAB", "CG" or "CGS" (psb_krylov's names; CGS in the
+ * preconditioned form of Barrett et al., Templates, fig. 2.9); istop = 2
+ * (||r||_2 <= eps ||b||_2); x holds
  * x0 on entry.  k interleaved right-hand sides, each with its own
  * recurrence and stopping test; iter[c] and err[c] (host arrays of k) get
  * the iteration count and final ||r_c||/||b_c|| of column c.  prec may be

@@ -25,6 +25,8 @@
 // Ties among equal magnitudes at a fill cap are resolved by "smaller column
 // first" (the reference leaves them to PSBLAS heap order).
 #include <algorithm>
+#include <exception>
+#include <thread>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -479,7 +481,37 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
     wc.val.push_back(1.0);
     wc.ptr.push_back(1);
   }
+  if (!var.stabilized && alg != AINV_ALG_S_LLK && n > 1) {
+    // Without stabilization the Z pass (pivots p) and the W pass (pivots q)
+    // share nothing but scratch arrays: they run on two host threads, each
+    // with its own scratch, performing exactly the operations of the
+    // interleaved loop below.
+    PassWork pw2;
+    pw2.zval.assign(n, 0.0);
+    pw2.in_z.assign(n, 0);
+    pw2.queued.assign(n, 0);
+    std::exception_ptr err;
+    std::thread tw([&] {
+      try {
+        for (int32_t i = 1; i < n; ++i)
+          bicg_column(i, at, as, wc, q, fill_in, thresh, var, &zc, &at, pw2,
+                      wc);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    });
+    try {
+      for (int32_t i = 1; i < n; ++i)
+        bicg_column(i, as, at, zc, p, fill_in, thresh, var, &wc, &as, pw, zc);
+    } catch (...) {
+      tw.join();
+      throw;
+    }
+    tw.join();
+    if (err) std::rethrow_exception(err);
+  }
   for (int32_t i = 1; i < n; ++i) {
+    if (!var.stabilized && alg != AINV_ALG_S_LLK) break;  // done above
     // Z: dots with rows of A (STAB: w_j^T A), seeds from column i of A
     bicg_column(i, as, at, zc, p, fill_in, thresh, var, &wc, &as, pw, zc);
     if (alg == AINV_ALG_S_LLK) continue;

@@ -53,6 +53,9 @@ enum Op {
   OP_DOT_RZ,     // dot (r,z)
   OP_P_CG,       // p = z (first step) or z + beta p
   OP_XR_CG,      // x += alpha p ; r -= alpha q ; (r,r)
+  OP_U_CGS,      // u = r + beta q ; p = u + beta (q + beta p) (first: r)
+  OP_Q_CGS,      // q = u - alpha vhat ; z = u + q
+  OP_XR_CGS,     // x += alpha uhat ; r -= alpha qhat ; (r,r) (rtilde,r)
 };
 
 template <int OP> struct NPairs { static constexpr int v = 0; };
@@ -61,6 +64,7 @@ template <> struct NPairs<OP_XR_BICG> { static constexpr int v = 2; };
 template <> struct NPairs<OP_INIT_CG> { static constexpr int v = 2; };
 template <> struct NPairs<OP_DOT_RZ> { static constexpr int v = 1; };
 template <> struct NPairs<OP_XR_CG> { static constexpr int v = 1; };
+template <> struct NPairs<OP_XR_CGS> { static constexpr int v = 2; };
 
 // One fused pass over the n x k vectors.  Thread (rr, c) of a block works on
 // column c of rows rr, rr + R, ...; partial sums per column are reduced in
@@ -114,6 +118,26 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         const double rn = v.r[e] + (-cs.alpha) * v.q[e];
         v.r[e] = rn;
         acc[0] += rn * rn;
+      } else if constexpr (OP == OP_U_CGS) {
+        // u in s, the q of CGS in t
+        if (cs.iter == 1) {
+          v.s[e] = v.r[e];
+          v.p[e] = v.r[e];
+        } else {
+          const double u = v.r[e] + cs.beta * v.t[e];
+          v.s[e] = u;
+          v.p[e] = u + cs.beta * (v.t[e] + cs.beta * v.p[e]);
+        }
+      } else if constexpr (OP == OP_Q_CGS) {
+        const double qn = v.s[e] + (-cs.alpha) * v.v[e];
+        v.t[e] = qn;
+        v.z[e] = v.s[e] + qn;
+      } else if constexpr (OP == OP_XR_CGS) {
+        v.x[e] = v.x[e] + cs.alpha * v.f[e];
+        const double rn = v.r[e] + (-cs.alpha) * v.v[e];
+        v.r[e] = rn;
+        acc[0] += rn * rn;
+        acc[1] += v.q[e] * rn;
       }
     }
   }
@@ -153,6 +177,7 @@ enum Fin {
   FIN_RHO_CG,    // iter++, rho = (r,z), beta
   FIN_ALPHA_CG,  // alpha = rho / (p,q)
   FIN_CONV_CG,
+  FIN_RHO_CGS,   // iter++, rho = rho_next, beta = rho / rho_old
 };
 
 __host__ __device__ inline int fin_pairs(int fin) {
@@ -161,7 +186,8 @@ __host__ __device__ inline int fin_pairs(int fin) {
     case FIN_INIT_CG:
     case FIN_OMEGA:
     case FIN_CONV_BICG: return 2;
-    case FIN_RHO_BICG: return 0;
+    case FIN_RHO_BICG:
+    case FIN_RHO_CGS: return 0;
     default: return 1;
   }
 }
@@ -187,7 +213,9 @@ __global__ __launch_bounds__(kThreads) void k_fin(
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
-  if ((fin == FIN_RHO_BICG || fin == FIN_RHO_CG) && c == 0) *nactive = 0;
+  if ((fin == FIN_RHO_BICG || fin == FIN_RHO_CG || fin == FIN_RHO_CGS) &&
+      c == 0)
+    *nactive = 0;
   Col cs = col[c];
   const double s0 = red[0][0], s1 = red[1][0], s2 = red[2][0];
   if (fin == FIN_INIT || fin == FIN_INIT_CG) {
@@ -232,6 +260,15 @@ __global__ __launch_bounds__(kThreads) void k_fin(
         if (cs.err <= cs.eps) cs.active = 0;
         break;
       }
+      case FIN_RHO_CGS:
+        cs.iter += 1;
+        cs.rho_old = cs.rho;
+        cs.rho = cs.rho_next;
+        if (cs.rho == 0.0)
+          cs.active = 0;
+        else if (cs.iter > 1)
+          cs.beta = cs.rho / cs.rho_old;
+        break;
       case FIN_RHO_CG:
         cs.iter += 1;
         cs.rho_old = cs.rho;
@@ -363,7 +400,8 @@ __global__ void k_extract(int32_t n, int K, int c0, int kc,
 }  // namespace
 
 struct KrylovPlan {
-  bool cg = false;
+  int method = KM_BICGSTAB;
+  bool cg = false;  // CG's vector set and init; otherwise BiCGSTAB's
   const ainv_solver *m = nullptr;
   uint64_t gen = 0;
   int k = 0;       // columns in use
@@ -403,11 +441,11 @@ void destroy_plan(KrylovPlan *p) { delete p; }
 
 namespace {
 
-KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k,
+KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
                      hipStream_t s) {
   for (size_t i = 0; i < a->plans.size(); ++i) {
     KrylovPlan *p = a->plans[i].get();
-    if (p->cg == cg && p->m == m && p->k == k) {
+    if (p->method == method && p->m == m && p->k == k) {
       if (m && p->gen != m->generation) {
         p->release_graph();  // factors changed: recapture
         if (p->tbuf.bytes < solver_t_bytes(m, p->K))
@@ -421,7 +459,9 @@ KrylovPlan *get_plan(ainv_matrix *a, bool cg, const ainv_solver *m, int k,
   }
   std::unique_ptr<KrylovPlan, PlanDeleter> up(new KrylovPlan());
   KrylovPlan *p = up.get();
-  p->cg = cg;
+  p->method = method;
+  p->cg = method == KM_CG;
+  const bool cg = p->cg;
   p->m = m;
   p->gen = m ? m->generation : 0;
   p->k = k;
@@ -545,7 +585,20 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   const int32_t n = p->n;
   const int k = p->k, K = p->K, sm = p->spmv_map;
   const Vecs &v = p->v;
-  if (!p->cg) {
+  if (p->method == KM_CGS) {
+    // vectors: q = rtilde, s = u, t = q of CGS, f = phat then uhat,
+    // v = vhat then qhat, z = u + q
+    fin(FIN_RHO_CGS, k, 0, p->pt, p->col, p->na, s);
+    vec_op<OP_U_CGS>(n, k, v, p->col, p->pt, s);
+    precond(p, v.p, v.f, s);
+    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s);
+    fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
+    vec_op<OP_Q_CGS>(n, k, v, p->col, p->pt, s);
+    precond(p, v.z, v.f, s);
+    hll_spmm_axpby(a->hll, 1.0, v.f, false, K, 0.0, v.v, K, k, s, sm);
+    g = vec_op<OP_XR_CGS>(n, k, v, p->col, p->pt, s);
+    fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
+  } else if (!p->cg) {
     fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
     vec_op<OP_P_BICG>(n, k, v, p->col, p->pt, s);
     precond(p, v.p, v.f, s);
@@ -606,6 +659,14 @@ void capture(KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
 
 }  // namespace
 
+int krylov_method_id(const std::string &name) {
+  if (name == "BICGSTAB") return KM_BICGSTAB;
+  if (name == "CG") return KM_CG;
+  if (name == "CGS") return KM_CGS;
+  throw Error(AINV_ERR_ARG, "unknown Krylov method " + name +
+                                " (BICGSTAB, CG, CGS available)");
+}
+
 // ---------------------------------------------------------- slot batches
 struct SlotBatch {
   ainv_matrix *a = nullptr;
@@ -616,12 +677,12 @@ struct SlotBatch {
   std::vector<int32_t> hmask;
 };
 
-SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, bool cg, int K,
+SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s) {
   if (K < 1 || K > kMaxCols) throw Error(AINV_ERR_ARG, "slots must be 1..64");
   if (m && m->n != a->n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
-  KrylovPlan *p = get_plan(a, cg, m, K, s);
+  KrylovPlan *p = get_plan(a, method, m, K, s);
   if (!p->pinned_cols)
     AINV_HIP_CHECK(hipHostMalloc((void **)&p->pinned_cols,
                                  2 * kMaxCols * sizeof(Col),
@@ -762,11 +823,9 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   if (k < 1 || k > kMaxCols) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   if (m && m->n != n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
-  const bool cg = (method == "CG");
-  if (!cg && method != "BICGSTAB")
-    throw Error(AINV_ERR_ARG, "unknown Krylov method " + method +
-                                  " (BICGSTAB, CG available)");
-  KrylovPlan *p = get_plan(a, cg, m, k, s);
+  const int mid = krylov_method_id(method);
+  const bool cg = mid == KM_CG;
+  KrylovPlan *p = get_plan(a, mid, m, k, s);
   const int K = p->K;
   const size_t vb = (size_t)n * K * sizeof(double);
   const Vecs &v = p->v;
@@ -843,13 +902,10 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
     throw Error(AINV_ERR_ARG, "slots must be in 1..64");
   if (m && m->n != a->n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
-  const bool cg = (method == "CG");
-  if (!cg && method != "BICGSTAB")
-    throw Error(AINV_ERR_ARG, "unknown Krylov method " + method +
-                                  " (BICGSTAB, CG available)");
+  const int mid = krylov_method_id(method);
   if (nrhs == 0) return;
   const int64_t n = a->n;
-  SlotBatch *bt = batch_open(a, m, cg, slots, s);
+  SlotBatch *bt = batch_open(a, m, mid, slots, s);
   const int K = bt->K;
   std::vector<int> owner(K, -1);
   std::vector<int64_t> first(K, 0);

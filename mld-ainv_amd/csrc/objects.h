@@ -104,7 +104,10 @@ struct SlotState {
   int32_t active = 0, iter = 0;
   double err = 0.0;
 };
-SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, bool cg, int K,
+// Krylov methods of the engine (psb_krylov's names)
+enum { KM_BICGSTAB = 0, KM_CG = 1, KM_CGS = 2 };
+int krylov_method_id(const std::string &name);  // throws for others
+SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s);
 void batch_close(SlotBatch *b);
 // src: device, n x kc interleaved; x0 = 0

@@ -56,7 +56,7 @@ struct ServedMatrix {
   HostCsr a;
   std::vector<std::pair<std::string, std::string>> params;
   int kind = AINV_KIND_INVK;
-  bool cg = false;
+  int method = KM_BICGSTAB;
   ainv_matrix *A = nullptr;
   ainv_solver *M = nullptr;
   int64_t bytes = 0;
@@ -90,8 +90,8 @@ struct ainv_server {
 
 namespace {
 
-int64_t plan_bytes(int32_t n, int K, bool cg) {
-  return (int64_t)n * K * (cg ? 6 : 10) * 8 + (int64_t)n * K * 8;
+int64_t plan_bytes(int32_t n, int K, int method) {
+  return (int64_t)n * K * (method == KM_CG ? 6 : 10) * 8 + (int64_t)n * K * 8;
 }
 
 void evict_locked(ainv_server *S, ServedMatrix *m) {
@@ -110,7 +110,7 @@ void evict_locked(ainv_server *S, ServedMatrix *m) {
 void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
   if (m->A && m->M) return;
   // estimate: A (value + column) and two factors of about 4x the nonzeros
-  const int64_t est = m->a.nnz() * 12 * 9 + plan_bytes(m->a.n, K, m->cg);
+  const int64_t est = m->a.nnz() * 12 * 9 + plan_bytes(m->a.n, K, m->method);
   {
     std::unique_lock<std::mutex> lk(S->mu);
     while (S->resident + est > S->budget) {
@@ -145,7 +145,7 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
   m->M = M;
   m->bytes = A->hll.bytes() + (int64_t)(A->csr.ptr.bytes + A->csr.idx.bytes +
                                         A->csr.val.bytes) +
-             solver_device_bytes(M) + plan_bytes(m->a.n, K, m->cg);
+             solver_device_bytes(M) + plan_bytes(m->a.n, K, m->method);
   S->resident += m->bytes;
   S->peak_resident = std::max(S->peak_resident, S->resident);
   S->builds++;
@@ -184,7 +184,7 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
   const int32_t n = m->a.n;
   const size_t need = (size_t)n * kMaxCols * sizeof(double);
   if (stage.bytes < need) stage.alloc(need);
-  SlotBatch *b = batch_open(m->A, m->M, m->cg, K, s);
+  SlotBatch *b = batch_open(m->A, m->M, m->method, K, s);
   std::vector<SlotUse> uses;  // admitted requests
   std::vector<int> owner(K, -1);
   std::vector<SlotState> st;
@@ -403,9 +403,7 @@ int ainv_server_add_matrix(ainv_server_h S, int32_t n, const int64_t *irp,
       if (k == "METHOD") {
         std::string v = kv.second;
         for (char &c : v) c = (char)std::toupper((unsigned char)c);
-        if (v != "CG" && v != "BICGSTAB")
-          throw Error(AINV_ERR_ARG, "METHOD must be BICGSTAB or CG");
-        m->cg = (v == "CG");
+        m->method = krylov_method_id(v);
       } else {
         m->params.emplace_back(k, kv.second);
       }

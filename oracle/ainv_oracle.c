@@ -1245,6 +1245,74 @@ out:
   return OR_OK;
 }
 
+/* CGS (Sonneveld 1989; the preconditioned form of Barrett et al.,
+ * Templates, 1994, fig. 2.9) with the same istop = 2 test after each full
+ * step: rtilde = r0; rho = (rtilde, r); u = r + beta q;
+ * p = u + beta (q + beta p); phat = M p; vhat = A phat;
+ * alpha = rho / (rtilde, vhat); q = u - alpha vhat; uhat = M (u + q);
+ * x += alpha uhat; r -= alpha A uhat. */
+int or_cgs(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
+           const double *b, double *x, double eps, int32_t itmax,
+           int32_t *iter, double *err) {
+  int32_t n = a->n;
+  size_t sz = ((size_t)n + 1) * sizeof(double);
+  double *r = (double *)malloc(sz), *rt = (double *)malloc(sz);
+  double *u = (double *)malloc(sz), *p = (double *)calloc(1, sz);
+  double *qq = (double *)calloc(1, sz), *ph = (double *)malloc(sz);
+  double *vh = (double *)malloc(sz), *uq = (double *)malloc(sz);
+  double *uh = (double *)malloc(sz), *qh = (double *)malloc(sz);
+  double rho = 0.0, rho_old, beta, sigma, alpha;
+  memcpy(r, b, (size_t)n * sizeof(double));
+  or_spmv('N', -1.0, a, x, 1.0, r);
+  memcpy(rt, r, (size_t)n * sizeof(double));
+  double bn2 = sqrt(dot(n, b, b));
+  double rn2 = sqrt(dot(n, r, r));
+  *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+  int32_t it = 0;
+  if (*err <= eps) goto out;
+  while (it < itmax) {
+    it++;
+    rho_old = rho;
+    rho = dot(n, rt, r);
+    if (rho == 0.0) break;
+    if (it == 1) {
+      memcpy(u, r, (size_t)n * sizeof(double));
+      memcpy(p, u, (size_t)n * sizeof(double));
+    } else {
+      beta = rho / rho_old;
+      for (int32_t i = 0; i < n; i++) u[i] = r[i] + beta * qq[i];
+      for (int32_t i = 0; i < n; i++) p[i] = u[i] + beta * (qq[i] + beta * p[i]);
+    }
+    or_apply('N', 1.0, w, z, d, p, 0.0, ph);
+    or_spmv('N', 1.0, a, ph, 0.0, vh);
+    sigma = dot(n, rt, vh);
+    if (sigma == 0.0) break;
+    alpha = rho / sigma;
+    for (int32_t i = 0; i < n; i++) qq[i] = u[i] + (-alpha) * vh[i];
+    for (int32_t i = 0; i < n; i++) uq[i] = u[i] + qq[i];
+    or_apply('N', 1.0, w, z, d, uq, 0.0, uh);
+    or_spmv('N', 1.0, a, uh, 0.0, qh);
+    for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * uh[i];
+    for (int32_t i = 0; i < n; i++) r[i] = r[i] + (-alpha) * qh[i];
+    rn2 = sqrt(dot(n, r, r));
+    *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+    if (*err <= eps) break;
+  }
+out:
+  *iter = it;
+  free(r);
+  free(rt);
+  free(u);
+  free(p);
+  free(qq);
+  free(ph);
+  free(vh);
+  free(uq);
+  free(uh);
+  free(qh);
+  return OR_OK;
+}
+
 /* Preconditioned CG (Hestenes-Stiefel) in the PSBLAS psb_dcg arrangement. */
 int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
           const double *b, double *x, double eps, int32_t itmax,

@@ -267,7 +267,27 @@ def test_krylov_cg_block(gpu, O):
             1e-7 * np.linalg.norm(B[:, c])
 
 
-@pytest.mark.parametrize("method", ["BICGSTAB", "CG"])
+@pytest.mark.parametrize("k", [1, 5])
+def test_krylov_cgs(gpu, O, k):
+    """CGS on the GPU: per-column iteration counts within +-1 of the
+    oracle's CGS (itself checked against scipy's cgs), true residuals."""
+    a = gpu.gen_pde3d(12)
+    n = a.shape[0]
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    w, z, d = O.invk_bld(a, 1, 1)
+    B = gpu.gen_normal(15, (n, k)) if k > 1 else a @ np.ones(n)
+    A = gpu.SystemMatrix(a)
+    X, it, err = gpu.krylov("CGS", A, sv, B, eps=1e-8)
+    Bc = B.reshape(n, -1)
+    Xc = X.reshape(n, -1)
+    for c in range(k):
+        _, it_ref, _ = O.krylov("CGS", a, w, z, d, Bc[:, c], 1e-8)
+        assert abs(int(it[c]) - it_ref) <= 1, (c, it[c], it_ref)
+        assert np.linalg.norm(Bc[:, c] - a @ Xc[:, c]) <= \
+            1e-7 * np.linalg.norm(Bc[:, c])
+
+
+@pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS"])
 def test_graph_replays_match_eager(gpu, method):
     """The captured iteration graph, replayed solve after solve with eager
     work and synchronizations in between, gives bit-identical iterates and

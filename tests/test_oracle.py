@@ -342,7 +342,7 @@ def _scipy_iters(method, a, M, b, rtol):
 
     def cb(_):
         count[0] += 1
-    f = spla.bicgstab if method == "BICGSTAB" else spla.cg
+    f = {"BICGSTAB": spla.bicgstab, "CG": spla.cg, "CGS": spla.cgs}[method]
     x, info = f(a, b, rtol=rtol, atol=0.0, M=M, maxiter=2000, callback=cb)
     assert info == 0
     return count[0], x
@@ -350,7 +350,8 @@ def _scipy_iters(method, a, M, b, rtol):
 
 @pytest.mark.parametrize("method,kind", [("BICGSTAB", "INVK"),
                                          ("BICGSTAB", "AINV"),
-                                         ("CG", "SYM")])
+                                         ("CG", "SYM"), ("CGS", "INVK"),
+                                         ("CGS", "AINV")])
 def test_krylov_against_scipy(O, method, kind):
     if kind == "SYM":
         a = sp.csr_matrix(R.pde3d_numpy(6))
