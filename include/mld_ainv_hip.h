@@ -205,8 +205,10 @@ int ainv_mat_free(ainv_mat_h a);
 int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
                   double *y, int k, int64_t ldx, int64_t ldy, void *stream);
 
-/* method: "BICGSTAB", "CG" or "CGS" (psb_krylov's names; CGS in the
- * preconditioned form of Barrett et al., Templates, fig. 2.9); istop = 2
+/* method: "BICGSTAB", "CG", "CGS" or "BICG" (psb_krylov's names; CGS and
+ * BiCG in the preconditioned forms of Barrett et al., Templates, figs. 2.9
+ * and 2.8; BiCG applies A^T and M^T through transposed layouts built on
+ * its first call); istop = 2
  * (||r||_2 <= eps ||b||_2); x holds
  * x0 on entry.  k interleaved right-hand sides, each with its own
  * recurrence and stopping test; iter[c] and err[c] (host arrays of k) get
@@ -219,8 +221,8 @@ int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
 /* Served form of the same solve (no counterpart in the reference, whose
  * Krylov driver takes one right-hand side at a time): a device-resident
  * queue of nrhs right-hand sides b (n-vectors one after another) is solved
- * through `slots` (1..64) column slots of one captured BiCGSTAB / CG
- * iteration; a slot is refilled from the queue between two iterations as
+ * through `slots` (1..64) column slots of one captured Krylov
+ * iteration (any method above); a slot is refilled from the queue between two iterations as
  * soon as its column stops (continuous batching).  x (device, nrhs
  * n-vectors) gets the solutions, x0 = 0.  iter, err, lat_ms are host arrays
  * of nrhs: iterations, final ||r||/||b|| and milliseconds from admission to
@@ -265,7 +267,8 @@ typedef struct ainv_server *ainv_server_h;
 int ainv_server_create(int device, int64_t hbm_budget_bytes, int workers,
                        ainv_server_h *srv);
 /* kind: AINV_KIND_*; params: "KEY=VALUE;..." with the keys of
- * ainv_cseti/csetr/csetc plus METHOD=BICGSTAB|CG.  Host CSR is copied. */
+ * ainv_cseti/csetr/csetc plus METHOD=BICGSTAB|CG|CGS|BICG.  Host CSR is
+ * copied. */
 int ainv_server_add_matrix(ainv_server_h srv, int32_t n, const int64_t *irp,
                            const int32_t *ja, const double *val,
                            int index_base, int kind, const char *params,

@@ -651,7 +651,7 @@ class Server:
 
     def add_matrix(self, a, kind="INVK", **params):
         """params: solver keys (SUB_FILLIN=1, FACTOR_PREC='F32', ...) and
-        METHOD='BICGSTAB'|'CG'.  Returns the matrix id."""
+        METHOD='BICGSTAB'|'CG'|'CGS'|'BICG'.  Returns the matrix id."""
         n, irp, ja, val = _csr_args(a)
         spec = ";".join(f"{k}={v}" for k, v in params.items())
         mid = C.c_int32()

@@ -376,6 +376,20 @@ void solver_apply_into(const ainv_solver *h, const double *x, double *y,
   hll_spmm_axpby(h->hz, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map));
 }
 
+void solver_ensure_transposed(ainv_solver *h) {
+  ensure_transposed(h);
+  AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
+}
+
+// y = M^T x with the transposed layout ('T' apply, beta = 0), no allocation
+void solver_apply_into_t(const ainv_solver *h, const double *x, double *y,
+                         int k, int64_t ld, void *t, hipStream_t s, int map) {
+  if (!h->have_t) throw Error(AINV_ERR_INTERNAL, "transposed layout missing");
+  const bool f32t = h->prec != AINV_PREC_F64;
+  hll_spmm_scale(h->hzt, h->dv.p, x, false, ld, t, ld, k, s, map_w(map));
+  hll_spmm_axpby(h->hwt, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map));
+}
+
 int log2_pow2(int K) {
   int i = 0;
   while ((1 << i) < K && i < 7) ++i;

@@ -56,6 +56,9 @@ enum Op {
   OP_U_CGS,      // u = r + beta q ; p = u + beta (q + beta p) (first: r)
   OP_Q_CGS,      // q = u - alpha vhat ; z = u + q
   OP_XR_CGS,     // x += alpha uhat ; r -= alpha qhat ; (r,r) (rtilde,r)
+  OP_DOT_ZRT,    // BiCG: (z, rtilde)
+  OP_P_BICGM,    // BiCG: p = z + beta p ; ptilde = ztilde + beta ptilde
+  OP_XR_BICGM,   // BiCG: x += alpha p ; r -= alpha q ; rtilde -= alpha qt
 };
 
 template <int OP> struct NPairs { static constexpr int v = 0; };
@@ -65,6 +68,8 @@ template <> struct NPairs<OP_INIT_CG> { static constexpr int v = 2; };
 template <> struct NPairs<OP_DOT_RZ> { static constexpr int v = 1; };
 template <> struct NPairs<OP_XR_CG> { static constexpr int v = 1; };
 template <> struct NPairs<OP_XR_CGS> { static constexpr int v = 2; };
+template <> struct NPairs<OP_DOT_ZRT> { static constexpr int v = 1; };
+template <> struct NPairs<OP_XR_BICGM> { static constexpr int v = 1; };
 
 // One fused pass over the n x k vectors.  Thread (rr, c) of a block works on
 // column c of rows rr, rr + R, ...; partial sums per column are reduced in
@@ -132,6 +137,24 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         const double qn = v.s[e] + (-cs.alpha) * v.v[e];
         v.t[e] = qn;
         v.z[e] = v.s[e] + qn;
+      } else if constexpr (OP == OP_DOT_ZRT) {
+        acc[0] += v.z[e] * v.q[e];
+      } else if constexpr (OP == OP_P_BICGM) {
+        // ztilde in f, ptilde in s
+        if (cs.iter == 1) {
+          v.p[e] = v.z[e];
+          v.s[e] = v.f[e];
+        } else {
+          v.p[e] = v.z[e] + cs.beta * v.p[e];
+          v.s[e] = v.f[e] + cs.beta * v.s[e];
+        }
+      } else if constexpr (OP == OP_XR_BICGM) {
+        // q of BiCG in v, qtilde in t, rtilde in q
+        v.x[e] = v.x[e] + cs.alpha * v.p[e];
+        const double rn = v.r[e] + (-cs.alpha) * v.v[e];
+        v.r[e] = rn;
+        v.q[e] = v.q[e] + (-cs.alpha) * v.t[e];
+        acc[0] += rn * rn;
       } else if constexpr (OP == OP_XR_CGS) {
         v.x[e] = v.x[e] + cs.alpha * v.f[e];
         const double rn = v.r[e] + (-cs.alpha) * v.v[e];
@@ -178,6 +201,7 @@ enum Fin {
   FIN_ALPHA_CG,  // alpha = rho / (p,q)
   FIN_CONV_CG,
   FIN_RHO_CGS,   // iter++, rho = rho_next, beta = rho / rho_old
+  FIN_RHO_BICGM, // BiCG: iter++, rho = (z, rtilde), beta = rho / rho_old
 };
 
 __host__ __device__ inline int fin_pairs(int fin) {
@@ -188,6 +212,7 @@ __host__ __device__ inline int fin_pairs(int fin) {
     case FIN_CONV_BICG: return 2;
     case FIN_RHO_BICG:
     case FIN_RHO_CGS: return 0;
+    case FIN_RHO_BICGM: return 1;
     default: return 1;
   }
 }
@@ -213,7 +238,8 @@ __global__ __launch_bounds__(kThreads) void k_fin(
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
-  if ((fin == FIN_RHO_BICG || fin == FIN_RHO_CG || fin == FIN_RHO_CGS) &&
+  if ((fin == FIN_RHO_BICG || fin == FIN_RHO_CG || fin == FIN_RHO_CGS ||
+       fin == FIN_RHO_BICGM) &&
       c == 0)
     *nactive = 0;
   Col cs = col[c];
@@ -264,6 +290,15 @@ __global__ __launch_bounds__(kThreads) void k_fin(
         cs.iter += 1;
         cs.rho_old = cs.rho;
         cs.rho = cs.rho_next;
+        if (cs.rho == 0.0)
+          cs.active = 0;
+        else if (cs.iter > 1)
+          cs.beta = cs.rho / cs.rho_old;
+        break;
+      case FIN_RHO_BICGM:
+        cs.iter += 1;
+        cs.rho_old = cs.rho;
+        cs.rho = s0;
         if (cs.rho == 0.0)
           cs.active = 0;
         else if (cs.iter > 1)
@@ -510,6 +545,29 @@ void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
                                   hipMemcpyDeviceToDevice, s));
 }
 
+void precond_t(const KrylovPlan *p, const double *x, double *y,
+               hipStream_t s) {
+  if (p->m)
+    solver_apply_into_t(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map);
+  else
+    AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->K * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+}
+
+// Layouts a method needs besides the plan's own (built outside any capture):
+// BiCG applies M^T and A^T.
+void prepare_method(KrylovPlan *p, ainv_matrix *a, ainv_solver *m) {
+  if (p->method != KM_BICG) return;
+  if (m) solver_ensure_transposed(m);
+  if (!a->have_t) {
+    DevCsr at;
+    csr_transpose(a->csr, at, a->stream);
+    hll_from_csr(at, AINV_PREC_F64, a->hll_t, a->stream);
+    AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
+    a->have_t = true;
+  }
+}
+
 // Autotuner: time the kernel mappings (split, row, coop, coop128) of each
 // apply pass and of the fused system-matrix product on this plan's buffers
 // and keep the fastest of each; results are cached on the solver / matrix
@@ -585,7 +643,19 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   const int32_t n = p->n;
   const int k = p->k, K = p->K, sm = p->spmv_map;
   const Vecs &v = p->v;
-  if (p->method == KM_CGS) {
+  if (p->method == KM_BICG) {
+    // vectors: q = rtilde, f = ztilde, s = ptilde, v = A p, t = A^T ptilde
+    precond(p, v.r, v.z, s);
+    precond_t(p, v.q, v.f, s);
+    int g = vec_op<OP_DOT_ZRT>(n, k, v, p->col, p->pt, s);
+    fin(FIN_RHO_BICGM, k, g, p->pt, p->col, p->na, s);
+    vec_op<OP_P_BICGM>(n, k, v, p->col, p->pt, s);
+    g = hll_spmv_dot(a->hll, v.p, v.v, k, K, v.s, 1, p->pt, sm, s);
+    hll_spmm_axpby(a->hll_t, 1.0, v.s, false, K, 0.0, v.t, K, k, s, sm);
+    fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
+    g = vec_op<OP_XR_BICGM>(n, k, v, p->col, p->pt, s);
+    fin(FIN_CONV_CG, k, g, p->pt, p->col, p->na, s);
+  } else if (p->method == KM_CGS) {
     // vectors: q = rtilde, s = u, t = q of CGS, f = phat then uhat,
     // v = vhat then qhat, z = u + q
     fin(FIN_RHO_CGS, k, 0, p->pt, p->col, p->na, s);
@@ -663,8 +733,9 @@ int krylov_method_id(const std::string &name) {
   if (name == "BICGSTAB") return KM_BICGSTAB;
   if (name == "CG") return KM_CG;
   if (name == "CGS") return KM_CGS;
+  if (name == "BICG") return KM_BICG;
   throw Error(AINV_ERR_ARG, "unknown Krylov method " + name +
-                                " (BICGSTAB, CG, CGS available)");
+                                " (BICGSTAB, CG, CGS, BICG available)");
 }
 
 // ---------------------------------------------------------- slot batches
@@ -683,6 +754,7 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
   if (m && m->n != a->n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
   KrylovPlan *p = get_plan(a, method, m, K, s);
+  prepare_method(p, a, m);
   if (!p->pinned_cols)
     AINV_HIP_CHECK(hipHostMalloc((void **)&p->pinned_cols,
                                  2 * kMaxCols * sizeof(Col),
@@ -826,6 +898,7 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   const int mid = krylov_method_id(method);
   const bool cg = mid == KM_CG;
   KrylovPlan *p = get_plan(a, mid, m, k, s);
+  prepare_method(p, a, m);
   const int K = p->K;
   const size_t vb = (size_t)n * K * sizeof(double);
   const Vecs &v = p->v;

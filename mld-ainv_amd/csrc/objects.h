@@ -67,6 +67,8 @@ struct ainv_matrix {
   int32_t n = 0;
   ainv::DevCsr csr;
   ainv::Hll hll;
+  bool have_t = false;  // A^T layout (BiCG), built on first use
+  ainv::Hll hll_t;
   int8_t spmv_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   double spmv_ms[8][ainv::kNumMaps] = {};
   // Krylov workspaces and captured iteration graphs, most recent first
@@ -87,6 +89,12 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
                        int k, int64_t ld, void *t, hipStream_t s,
                        int map = 0);
+// Transposed layout for 'T' applies (built on first use; call before any
+// graph capture) and y = M^T x into a caller buffer t, like
+// solver_apply_into.
+void solver_ensure_transposed(ainv_solver *h);
+void solver_apply_into_t(const ainv_solver *h, const double *x, double *y,
+                         int k, int64_t ld, void *t, hipStream_t s, int map);
 // Index of K = 2^i and the mapping pair the tuner chose for it (split if
 // none, the forced mapping for both passes under APPLY_MAP).
 int log2_pow2(int K);
@@ -105,7 +113,7 @@ struct SlotState {
   double err = 0.0;
 };
 // Krylov methods of the engine (psb_krylov's names)
-enum { KM_BICGSTAB = 0, KM_CG = 1, KM_CGS = 2 };
+enum { KM_BICGSTAB = 0, KM_CG = 1, KM_CGS = 2, KM_BICG = 3 };
 int krylov_method_id(const std::string &name);  // throws for others
 SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s);

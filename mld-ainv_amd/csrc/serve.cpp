@@ -146,6 +146,9 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
   m->bytes = A->hll.bytes() + (int64_t)(A->csr.ptr.bytes + A->csr.idx.bytes +
                                         A->csr.val.bytes) +
              solver_device_bytes(M) + plan_bytes(m->a.n, K, m->method);
+  // BiCG builds the transposed layouts of A and of both factors on first use
+  if (m->method == KM_BICG)
+    m->bytes += A->hll.bytes() + M->hw.bytes() + M->hz.bytes();
   S->resident += m->bytes;
   S->peak_resident = std::max(S->peak_resident, S->resident);
   S->builds++;

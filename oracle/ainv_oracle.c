@@ -1313,6 +1313,70 @@ out:
   return OR_OK;
 }
 
+/* BiCG (Fletcher 1976; the preconditioned form of Barrett et al.,
+ * Templates, 1994, fig. 2.8) with the istop = 2 test after each full step:
+ * rtilde = r0; z = M r; ztilde = M^T rtilde; rho = (z, rtilde);
+ * p = z + beta p; ptilde = ztilde + beta ptilde; q = A p;
+ * qtilde = A^T ptilde; alpha = rho / (ptilde, q); x += alpha p;
+ * r -= alpha q; rtilde -= alpha qtilde. */
+int or_bicg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
+            const double *b, double *x, double eps, int32_t itmax,
+            int32_t *iter, double *err) {
+  int32_t n = a->n;
+  size_t sz = ((size_t)n + 1) * sizeof(double);
+  double *r = (double *)malloc(sz), *rt = (double *)malloc(sz);
+  double *zz = (double *)malloc(sz), *zt = (double *)malloc(sz);
+  double *p = (double *)calloc(1, sz), *pt = (double *)calloc(1, sz);
+  double *q = (double *)malloc(sz), *qt = (double *)malloc(sz);
+  double rho = 0.0, rho_old, beta, sigma, alpha;
+  memcpy(r, b, (size_t)n * sizeof(double));
+  or_spmv('N', -1.0, a, x, 1.0, r);
+  memcpy(rt, r, (size_t)n * sizeof(double));
+  double bn2 = sqrt(dot(n, b, b));
+  double rn2 = sqrt(dot(n, r, r));
+  *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+  int32_t it = 0;
+  if (*err <= eps) goto out;
+  while (it < itmax) {
+    it++;
+    or_apply('N', 1.0, w, z, d, r, 0.0, zz);
+    or_apply('T', 1.0, w, z, d, rt, 0.0, zt);
+    rho_old = rho;
+    rho = dot(n, zz, rt);
+    if (rho == 0.0) break;
+    if (it == 1) {
+      memcpy(p, zz, (size_t)n * sizeof(double));
+      memcpy(pt, zt, (size_t)n * sizeof(double));
+    } else {
+      beta = rho / rho_old;
+      for (int32_t i = 0; i < n; i++) p[i] = zz[i] + beta * p[i];
+      for (int32_t i = 0; i < n; i++) pt[i] = zt[i] + beta * pt[i];
+    }
+    or_spmv('N', 1.0, a, p, 0.0, q);
+    or_spmv('T', 1.0, a, pt, 0.0, qt);
+    sigma = dot(n, pt, q);
+    if (sigma == 0.0) break;
+    alpha = rho / sigma;
+    for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * p[i];
+    for (int32_t i = 0; i < n; i++) r[i] = r[i] + (-alpha) * q[i];
+    for (int32_t i = 0; i < n; i++) rt[i] = rt[i] + (-alpha) * qt[i];
+    rn2 = sqrt(dot(n, r, r));
+    *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+    if (*err <= eps) break;
+  }
+out:
+  *iter = it;
+  free(r);
+  free(rt);
+  free(zz);
+  free(zt);
+  free(p);
+  free(pt);
+  free(q);
+  free(qt);
+  return OR_OK;
+}
+
 /* Preconditioned CG (Hestenes-Stiefel) in the PSBLAS psb_dcg arrangement. */
 int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
           const double *b, double *x, double eps, int32_t itmax,

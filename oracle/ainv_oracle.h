@@ -118,6 +118,11 @@ int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
 int or_cgs(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
            const double *b, double *x, double eps, int32_t itmax,
            int32_t *iter, double *err);
+/* BiCG, preconditioned as in Barrett et al., Templates (1994), fig. 2.8
+ * (M^T = the trans 'T' apply), with the istop = 2 test. */
+int or_bicg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
+            const double *b, double *x, double eps, int32_t itmax,
+            int32_t *iter, double *err);
 int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
           const double *b, double *x, double eps, int32_t itmax,
           int32_t *iter, double *err);

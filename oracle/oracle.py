@@ -64,6 +64,9 @@ def lib():
             "or_bicgstab": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
                             P(C.c_double), P(C.c_double), C.c_double, C.c_int32,
                             P(C.c_int32), P(C.c_double)],
+            "or_bicg": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
+                        P(C.c_double), P(C.c_double), C.c_double, C.c_int32,
+                        P(C.c_int32), P(C.c_double)],
             "or_cgs": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
                        P(C.c_double), P(C.c_double), C.c_double, C.c_int32,
                        P(C.c_int32), P(C.c_double)],
@@ -242,7 +245,7 @@ def krylov(method, a, w, z, d, b, eps, itmax=1000, x0=None):
     it = C.c_int32(0)
     err = C.c_double(0.0)
     f = {"BICGSTAB": lib().or_bicgstab, "CG": lib().or_cg,
-         "CGS": lib().or_cgs}[method.upper()]
+         "CGS": lib().or_cgs, "BICG": lib().or_bicg}[method.upper()]
     _check(f(A.ref, W.ref, Z.ref, _dp(d), _dp(b), _dp(x), eps, itmax,
              C.byref(it), C.byref(err)), method)
     return x, it.value, err.value

@@ -287,7 +287,38 @@ def test_krylov_cgs(gpu, O, k):
             1e-7 * np.linalg.norm(Bc[:, c])
 
 
-@pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS"])
+@pytest.mark.parametrize("kind,k", [("INVK", 1), ("INVK", 5), ("AINV", 3),
+                                    ("NONE", 2)])
+def test_krylov_bicg(gpu, O, kind, k):
+    """BiCG on the GPU (A^T and M^T through the transposed layouts):
+    per-column iteration counts within +-1 of the oracle's BiCG (itself
+    checked against scipy's bicg), true residuals."""
+    a = gpu.gen_pde3d(12)
+    n = a.shape[0]
+    if kind == "INVK":
+        sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+        w, z, d = O.invk_bld(a, 1, 1)
+    elif kind == "AINV":
+        sv = build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
+                   SUB_ILUTHRS=0.1)
+        w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
+    else:
+        sv = None
+        w = z = sp.identity(n, format="csr")
+        d = np.ones(n)
+    B = gpu.gen_normal(17, (n, k)) if k > 1 else a @ np.ones(n)
+    A = gpu.SystemMatrix(a)
+    X, it, err = gpu.krylov("BICG", A, sv, B, eps=1e-8, itmax=400)
+    Bc = B.reshape(n, -1)
+    Xc = X.reshape(n, -1)
+    for c in range(k):
+        _, it_ref, _ = O.krylov("BICG", a, w, z, d, Bc[:, c], 1e-8, 400)
+        assert abs(int(it[c]) - it_ref) <= 1, (c, it[c], it_ref)
+        assert np.linalg.norm(Bc[:, c] - a @ Xc[:, c]) <= \
+            1e-7 * np.linalg.norm(Bc[:, c])
+
+
+@pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS", "BICG"])
 def test_graph_replays_match_eager(gpu, method):
     """The captured iteration graph, replayed solve after solve with eager
     work and synchronizations in between, gives bit-identical iterates and

@@ -81,7 +81,8 @@ def test_lru_eviction_under_budget(gpu):
 @pytest.mark.parametrize("method,slots,nrhs", [("BICGSTAB", 8, 21),
                                                ("BICGSTAB", 5, 3),
                                                ("CG", 16, 40),
-                                               ("CGS", 8, 12)])
+                                               ("CGS", 8, 12),
+                                               ("BICG", 6, 10)])
 def test_krylov_stream_matches_single_solves(gpu, method, slots, nrhs):
     """The served form (a device queue through continuously refilled slots)
     gives each right-hand side the iterations (+-1) and solution of solving

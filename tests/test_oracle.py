@@ -342,7 +342,8 @@ def _scipy_iters(method, a, M, b, rtol):
 
     def cb(_):
         count[0] += 1
-    f = {"BICGSTAB": spla.bicgstab, "CG": spla.cg, "CGS": spla.cgs}[method]
+    f = {"BICGSTAB": spla.bicgstab, "CG": spla.cg, "CGS": spla.cgs,
+         "BICG": spla.bicg}[method]
     x, info = f(a, b, rtol=rtol, atol=0.0, M=M, maxiter=2000, callback=cb)
     assert info == 0
     return count[0], x
@@ -351,7 +352,8 @@ def _scipy_iters(method, a, M, b, rtol):
 @pytest.mark.parametrize("method,kind", [("BICGSTAB", "INVK"),
                                          ("BICGSTAB", "AINV"),
                                          ("CG", "SYM"), ("CGS", "INVK"),
-                                         ("CGS", "AINV")])
+                                         ("CGS", "AINV"), ("BICG", "INVK"),
+                                         ("BICG", "AINV")])
 def test_krylov_against_scipy(O, method, kind):
     if kind == "SYM":
         a = sp.csr_matrix(R.pde3d_numpy(6))
@@ -366,7 +368,8 @@ def test_krylov_against_scipy(O, method, kind):
         w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
     n = a.shape[0]
     b = a @ np.ones(n)
-    M = spla.LinearOperator((n, n), matvec=lambda v: O.apply(w, z, d, v))
+    M = spla.LinearOperator((n, n), matvec=lambda v: O.apply(w, z, d, v),
+                            rmatvec=lambda v: O.apply(w, z, d, v, trans="T"))
     it_sc, x_sc = _scipy_iters(method, a, M, b, 1e-8)
     x, it, err = O.krylov(method, a, w, z, d, b, 1e-8)
     assert abs(it - it_sc) <= 1, (it, it_sc)
