@@ -125,7 +125,8 @@ def main():
                         "in FP64 (the baseline sample)")
     p.add_argument("--prec", default="F32")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-iters", type=int, default=4)
+    p.add_argument("--cpu-iters", type=int, default=None,
+                   help="oracle iterations per timed run (default: ~2 s)")
     p.add_argument("--apply-only", action="store_true",
                    help="profiling mode: build, then only the two apply "
                         "kernels at the served block width (and the copy "
@@ -487,26 +488,67 @@ def profile_apply(m, sv, n, k, reps):
     return a.value, b.value
 
 
-def cpu_baseline(sv, a, b, method, iters, gpu_iters):
-    """The oracle's Krylov method (single thread) for `iters` iterations on
-    the same system with the same FP64 factors, extrapolated to one solve at
-    the FP64 GPU path's mean iteration count."""
+def _host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(sv, a, b, method, iters, gpu_iters, threads=None):
+    """The oracle's Krylov method on the same system with the same FP64
+    factors: one warm-up, then 5 timed runs of `iters` iterations each
+    (`iters` = None: as many as take about 2 s), median time per iteration,
+    extrapolated to one solve at the FP64 GPU path's mean iteration count.
+    Returns the single-thread figure (the reference's one rank, no threads)
+    and, beside it, the same loops row-parallel on `threads` host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sv.cnv("F64")
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
-    t0 = time.time()
-    _, it, _ = O.krylov(method, a, w, z, d, b, 1e-30, itmax=iters)
-    dt = time.time() - t0
-    per_iter = dt / max(it, 1)
-    return {"value": 1.0 / (per_iter * gpu_iters), "unit": "solves/s",
-            "cores": 1, "kind": "port",
-            "sample": f"{it} {method} iterations of the oracle on the same "
-                      f"matrix, one right-hand side of the queue and the "
-                      f"FP64 factors ({dt:.1f} s), extrapolated to "
-                      f"{gpu_iters:.1f} iterations per solve",
-            "seconds_per_iteration": per_iter}
+    A, W, Z = O.wrap(a), O.wrap(w), O.wrap(z)
+    model, nproc = _host_info()
+    if threads is None:
+        threads = min(16, nproc)
 
+    def timed(nt, m):
+        O.set_threads(nt)
+        t0 = time.perf_counter()
+        _, it, _ = O.krylov(method, A, W, Z, d, b, 1e-30, itmax=m)
+        return (time.perf_counter() - t0) / max(it, 1), it
+
+    def leg(nt):
+        per0, _ = timed(nt, 1)  # warm-up, also sizes the runs
+        m = iters or max(1, int(round(2.0 / max(per0, 1e-6))))
+        runs = [timed(nt, m) for _ in range(5)]
+        per = float(np.median([r[0] for r in runs]))
+        return {"value": 1.0 / (per * gpu_iters), "unit": "solves/s",
+                "cores": nt, "kind": "port",
+                "sample": f"5 runs of {runs[0][1]} {method} iterations of the "
+                          f"oracle on the same matrix, one right-hand side of "
+                          f"the queue and the FP64 factors, after one warm-up; "
+                          f"median {per:.4f} s per iteration, extrapolated "
+                          f"to {gpu_iters:.1f} iterations per solve",
+                "seconds_per_iteration": per}
+
+    try:
+        one = leg(1)
+        one["host"] = {"cpu_model": model, "nproc": nproc,
+                       "compiler": O.build_info()}
+        if threads > 1:
+            mt = leg(threads)
+            mt["note"] = ("beyond the reference (one rank, no threads): the "
+                          "same loops row-parallel with OpenMP")
+            one["threaded"] = mt
+    finally:
+        O.set_threads(1)
+    return one
 
 if __name__ == "__main__":
     main()

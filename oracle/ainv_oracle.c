@@ -30,6 +30,24 @@
 
 #define D_EPSTOL DBL_EPSILON
 
+/* Threads for the row-parallel loops of the products, dots and vector
+ * updates (1 = the sequential order of the reference's single rank; more
+ * threads change only the summation order of the dots). */
+static int or_threads = 1;
+
+void or_set_threads(int t) { or_threads = t < 1 ? 1 : t; }
+
+const char *or_build_info(void) {
+#ifdef __VERSION__
+  return "gcc " __VERSION__ " -O2 -ffp-contract=off -fopenmp";
+#else
+  return "-O2 -ffp-contract=off -fopenmp";
+#endif
+}
+
+#define OR_PAR \
+  _Pragma("omp parallel for schedule(static) num_threads(or_threads) if (or_threads > 1)")
+
 void or_free(void *p) { free(p); }
 
 void or_mat_free(or_mat *m) {
@@ -1102,6 +1120,7 @@ void or_spmv(char trans, double alpha, const or_mat *a, const double *x,
              double beta, double *y) {
   int32_t n = a->n;
   if (trans == 'N' || trans == 'n') {
+    OR_PAR
     for (int32_t i = 0; i < n; i++) {
       double acc = 0.0;
       for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
@@ -1131,6 +1150,7 @@ int or_apply(char trans, double alpha, const or_mat *w, const or_mat *z,
   if (!tx) return OR_ERR_ALLOC;
   if (t == 'N') {
     or_spmv('N', 1.0, w, x, 0.0, tx);
+    OR_PAR
     for (int32_t i = 0; i < n; i++) tx[i] = d[i] * tx[i];
     or_spmv('N', alpha, z, tx, beta, y);
   } else {
@@ -1175,6 +1195,7 @@ int or_update_a(const or_mat *w, const or_mat *z, const double *x, double *d) {
 
 static double dot(int32_t n, const double *a, const double *b) {
   double s = 0.0;
+  _Pragma("omp parallel for schedule(static) reduction(+ : s) num_threads(or_threads) if (or_threads > 1)")
   for (int32_t i = 0; i < n; i++) s = s + a[i] * b[i];
   return s;
 }
@@ -1209,7 +1230,9 @@ int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
       memcpy(p, r, (size_t)n * sizeof(double));
     } else {
       beta = (rho / rho_old) * (alpha / omega);
+      OR_PAR
       for (int32_t i = 0; i < n; i++) p[i] = p[i] + (-omega) * v[i];
+      OR_PAR
       for (int32_t i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
     }
     or_apply('N', 1.0, w, z, d, p, 0.0, f);
@@ -1217,6 +1240,7 @@ int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
     sigma = dot(n, q, v);
     if (sigma == 0.0) break;
     alpha = rho / sigma;
+    OR_PAR
     for (int32_t i = 0; i < n; i++) s[i] = r[i] + (-alpha) * v[i];
     or_apply('N', 1.0, w, z, d, s, 0.0, zz);
     or_spmv('N', 1.0, a, zz, 0.0, t);
@@ -1225,8 +1249,11 @@ int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
     tau = dot(n, t, s);
     omega = tau / sigma;
     if (omega == 0.0) break;
+    OR_PAR
     for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * f[i];
+    OR_PAR
     for (int32_t i = 0; i < n; i++) x[i] = x[i] + omega * zz[i];
+    OR_PAR
     for (int32_t i = 0; i < n; i++) r[i] = s[i] + (-omega) * t[i];
     rn2 = sqrt(dot(n, r, r));
     *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
@@ -1402,13 +1429,16 @@ int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
       memcpy(p, zz, (size_t)n * sizeof(double));
     } else {
       beta = rho / rho_old;
+      OR_PAR
       for (int32_t i = 0; i < n; i++) p[i] = zz[i] + beta * p[i];
     }
     or_spmv('N', 1.0, a, p, 0.0, q);
     sigma = dot(n, p, q);
     if (sigma == 0.0) break;
     alpha = rho / sigma;
+    OR_PAR
     for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * p[i];
+    OR_PAR
     for (int32_t i = 0; i < n; i++) r[i] = r[i] + (-alpha) * q[i];
     rn2 = sqrt(dot(n, r, r));
     *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;

@@ -89,6 +89,12 @@ int or_invt_bld(const or_mat *a, int32_t fill, double thresh, int32_t invfill,
 /* Transpose of a CSR matrix (psb transp + cscnv). */
 int or_transpose(const or_mat *a, or_mat *t);
 
+/* Threads for the row-parallel loops (products 'N', dots, vector updates of
+ * BiCGSTAB and CG); default 1, the reference's single-rank order. */
+void or_set_threads(int t);
+/* Compiler and flags this library was built with. */
+const char *or_build_info(void);
+
 /* y = alpha*A*x + beta*y (trans 'N') or alpha*A^T*x + beta*y ('T'). */
 void or_spmv(char trans, double alpha, const or_mat *a, const double *x,
              double beta, double *y);

@@ -80,6 +80,10 @@ def lib():
         L.or_spmv.argtypes = [C.c_char, C.c_double, P(OrMat), P(C.c_double),
                               C.c_double, P(C.c_double)]
         L.or_spmv.restype = None
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = None
+        L.or_build_info.argtypes = []
+        L.or_build_info.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -236,9 +240,23 @@ def spmv(a, x, alpha=1.0, beta=0.0, y=None, trans="N"):
     return y
 
 
+def set_threads(t):
+    """Threads of the row-parallel loops (1 = sequential, the default)."""
+    lib().or_set_threads(int(t))
+
+
+def build_info():
+    return lib().or_build_info().decode()
+
+
+def wrap(a):
+    """A matrix converted once for repeated calls (krylov, apply)."""
+    return a if isinstance(a, _In) else _In(a)
+
+
 def krylov(method, a, w, z, d, b, eps, itmax=1000, x0=None):
     """Returns (x, iterations, last ||r||/||b||)."""
-    A, W, Z = _In(a), _In(w), _In(z)
+    A, W, Z = wrap(a), wrap(w), wrap(z)
     d = np.ascontiguousarray(d, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)

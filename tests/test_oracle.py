@@ -377,6 +377,33 @@ def test_krylov_against_scipy(O, method, kind):
     assert np.linalg.norm(b - a @ x) <= 1e-7 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("method", ["BICGSTAB", "CG"])
+def test_threaded_loops_same_solve(O, method):
+    """The row-parallel loops (CPU baseline on all host cores) change only
+    the summation order of the dots: same iterations +-1, same solution to
+    the tolerance, and one thread reproduces the sequential result bit for
+    bit."""
+    a = R.pde3d_numpy(9)
+    if method == "CG":
+        a = sp.csr_matrix(((a + a.T) * 0.5))
+        a.sort_indices()
+        w, z, d = O.ainv_bld(a, "SYM-LLK", 8, 0.05)
+    else:
+        w, z, d = O.invk_bld(a, 1, 1)
+    b = a @ np.ones(a.shape[0])
+    x1, it1, _ = O.krylov(method, a, w, z, d, b, 1e-9)
+    try:
+        O.set_threads(4)
+        x4, it4, e4 = O.krylov(method, a, w, z, d, b, 1e-9)
+        O.set_threads(1)
+        x1b, it1b, _ = O.krylov(method, a, w, z, d, b, 1e-9)
+    finally:
+        O.set_threads(1)
+    assert abs(it4 - it1) <= 1 and e4 <= 1e-9
+    assert np.linalg.norm(b - a @ x4) <= 1e-8 * np.linalg.norm(b)
+    assert it1b == it1 and np.array_equal(x1b, x1)
+
+
 # ------------------------------------------------------------- fixtures
 def test_golden_fixtures(O):
     """Committed vectors (tests/golden/make_golden.py): exact inverses from
