@@ -1404,6 +1404,113 @@ out:
   return OR_OK;
 }
 
+/* Restarted, right-preconditioned GMRES (Saad, alg. 9.5 with the MGS
+ * Arnoldi of alg. 6.2 and the Givens update of sec. 6.5.3).  The rotation
+ * of step i is c = h_ii / g, s = h_{i+1,i} / g with g = sqrt(h_ii^2 +
+ * h_{i+1,i}^2); v_{i+1} = (1 / h_{i+1,i}) w.  h is column-major
+ * (m+1) x m. */
+int or_gmres(const or_mat *a, const or_mat *w, const or_mat *z,
+             const double *d, const double *b, double *x, double eps,
+             int32_t itmax, int32_t irst, int32_t *iter, double *err) {
+  const int32_t n = a->n;
+  const int32_t m = irst < 1 ? 10 : irst;
+  const size_t sz = ((size_t)n + 1) * sizeof(double);
+  double *v = (double *)malloc(sz * (size_t)(m + 1));
+  double *wv = (double *)malloc(sz), *zv = (double *)malloc(sz);
+  double *h = (double *)calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = (double *)calloc((size_t)m, sizeof(double));
+  double *sn = (double *)calloc((size_t)m, sizeof(double));
+  double *rs = (double *)calloc((size_t)m + 1, sizeof(double));
+  double *y = (double *)calloc((size_t)m, sizeof(double));
+  if (!v || !wv || !zv || !h || !cs || !sn || !rs || !y) {
+    free(v); free(wv); free(zv); free(h); free(cs); free(sn); free(rs);
+    free(y);
+    return OR_ERR_ALLOC;
+  }
+#define VV(j) (v + (size_t)(j) * ((size_t)n + 1))
+#define H(r, c) h[(size_t)(c) * (m + 1) + (r)]
+  const double bn2 = sqrt(dot(n, b, b));
+  /* r0 = b - A x0 into v_0 */
+  memcpy(VV(0), b, (size_t)n * sizeof(double));
+  or_spmv('N', -1.0, a, x, 1.0, VV(0));
+  double beta = sqrt(dot(n, VV(0), VV(0)));
+  *err = (bn2 > 0.0) ? beta / bn2 : beta;
+  int32_t it = 0;
+  while (*err > eps && it < itmax) {
+    /* start of a cycle: v_0 = r / beta, rs = beta e_1 */
+    const double sc0 = 1.0 / beta;
+    for (int32_t i = 0; i < n; i++) VV(0)[i] = sc0 * VV(0)[i];
+    for (int32_t j = 0; j <= m; j++) rs[j] = 0.0;
+    rs[0] = beta;
+    int32_t len = 0, conv = 0, brk = 0;
+    for (int32_t i = 0; i < m && it < itmax; i++) {
+      it++;
+      or_apply('N', 1.0, w, z, d, VV(i), 0.0, zv);
+      or_spmv('N', 1.0, a, zv, 0.0, wv);
+      for (int32_t j = 0; j <= i; j++) {
+        const double hj = dot(n, VV(j), wv);
+        H(j, i) = hj;
+        for (int32_t e = 0; e < n; e++) wv[e] = wv[e] + (-hj) * VV(j)[e];
+      }
+      const double hn = sqrt(dot(n, wv, wv));
+      H(i + 1, i) = hn;
+      if (hn != 0.0) {
+        const double sc = 1.0 / hn;
+        for (int32_t e = 0; e < n; e++) VV(i + 1)[e] = sc * wv[e];
+      }
+      for (int32_t j = 0; j < i; j++) {
+        const double t0 = H(j, i), t1 = H(j + 1, i);
+        H(j, i) = cs[j] * t0 + sn[j] * t1;
+        H(j + 1, i) = -sn[j] * t0 + cs[j] * t1;
+      }
+      const double g = sqrt(H(i, i) * H(i, i) + hn * hn);
+      len = i + 1;
+      if (g == 0.0) { /* breakdown: nothing more to gain in this space */
+        brk = 1;
+        len = i;
+        break;
+      }
+      cs[i] = H(i, i) / g;
+      sn[i] = hn / g;
+      H(i, i) = g;
+      H(i + 1, i) = 0.0;
+      rs[i + 1] = -sn[i] * rs[i];
+      rs[i] = cs[i] * rs[i];
+      *err = (bn2 > 0.0) ? fabs(rs[i + 1]) / bn2 : fabs(rs[i + 1]);
+      if (*err <= eps) {
+        conv = 1;
+        break;
+      }
+    }
+    /* y = H(0:len, 0:len)^-1 rs ; x += M (V y) */
+    for (int32_t j = len - 1; j >= 0; j--) {
+      double t = rs[j];
+      for (int32_t l = j + 1; l < len; l++) t = t - H(j, l) * y[l];
+      y[j] = t / H(j, j);
+    }
+    if (len > 0) {
+      for (int32_t e = 0; e < n; e++) {
+        double u = 0.0;
+        for (int32_t j = 0; j < len; j++) u = u + y[j] * VV(j)[e];
+        wv[e] = u;
+      }
+      or_apply('N', 1.0, w, z, d, wv, 0.0, zv);
+      for (int32_t e = 0; e < n; e++) x[e] = x[e] + zv[e];
+    }
+    if (conv || brk) break;
+    /* true residual of the restart */
+    memcpy(VV(0), b, (size_t)n * sizeof(double));
+    or_spmv('N', -1.0, a, x, 1.0, VV(0));
+    beta = sqrt(dot(n, VV(0), VV(0)));
+    *err = (bn2 > 0.0) ? beta / bn2 : beta;
+  }
+#undef VV
+#undef H
+  *iter = it;
+  free(v); free(wv); free(zv); free(h); free(cs); free(sn); free(rs); free(y);
+  return OR_OK;
+}
+
 /* Preconditioned CG (Hestenes-Stiefel) in the PSBLAS psb_dcg arrangement. */
 int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
           const double *b, double *x, double eps, int32_t itmax,

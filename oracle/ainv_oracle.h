@@ -124,6 +124,17 @@ int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
 int or_cgs(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
            const double *b, double *x, double eps, int32_t itmax,
            int32_t *iter, double *err);
+/* Restarted GMRES(irst) with right preconditioning (Saad, Iterative Methods
+ * for Sparse Linear Systems, 2nd ed., alg. 9.5), the Arnoldi basis by
+ * modified Gram-Schmidt (alg. 6.2) and the Hessenberg least-squares problem
+ * by Givens rotations (sec. 6.5.3); psb_krylov's 'RGMRES' with restart
+ * irst (tests/fileread/runs/dfs.inp:10).  One iteration = one Arnoldi step;
+ * a step stops on the rotated residual |rs_{i+1}| <= eps ||b||; at the end
+ * of a cycle x += M (V y) and, unless converged, the true residual starts
+ * the next cycle (and is tested with the same istop = 2 rule). */
+int or_gmres(const or_mat *a, const or_mat *w, const or_mat *z,
+             const double *d, const double *b, double *x, double eps,
+             int32_t itmax, int32_t irst, int32_t *iter, double *err);
 /* BiCG, preconditioned as in Barrett et al., Templates (1994), fig. 2.8
  * (M^T = the trans 'T' apply), with the istop = 2 test. */
 int or_bicg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,

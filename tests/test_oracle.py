@@ -377,6 +377,79 @@ def test_krylov_against_scipy(O, method, kind):
     assert np.linalg.norm(b - a @ x) <= 1e-7 * np.linalg.norm(b)
 
 
+@pytest.mark.parametrize("m", [5, 30])
+def test_gmres_unpreconditioned_against_scipy(O, m):
+    """With M = I right and left preconditioning coincide: iteration counts
+    of scipy's restarted gmres (inner steps, 'pr_norm' callback)."""
+    a = R.pde3d_numpy(7)
+    n = a.shape[0]
+    w = z = sp.identity(n, format="csr")
+    d = np.ones(n)
+    b = a @ np.ones(n)
+    count = [0]
+
+    def cb(_):
+        count[0] += 1
+    x_sc, info = spla.gmres(a, b, rtol=1e-8, atol=0.0, restart=m,
+                            maxiter=500, callback=cb, callback_type="pr_norm")
+    assert info == 0
+    x, it, err = O.krylov("RGMRES", a, w, z, d, b, 1e-8, 2000, irst=m)
+    assert abs(it - count[0]) <= 1, (it, count[0])
+    assert np.linalg.norm(b - a @ x) <= 1e-7 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("kind", ["INVK", "AINV"])
+def test_gmres_minimal_residual(O, kind):
+    """Before a restart, step i of right-preconditioned GMRES reaches the
+    least-squares minimum of ||b - A M u|| over the Krylov space
+    K_i(A M, b) (x0 = 0), computed here with numpy from the explicit
+    Krylov basis."""
+    a = R.pde3d_numpy(5)
+    n = a.shape[0]
+    if kind == "INVK":
+        w, z, d = O.invk_bld(a, 1, 1)
+    else:
+        w, z, d = O.ainv_bld(a, "LLK", 4, 0.1)
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    AM = a.toarray() @ M
+    b = np.random.default_rng(8).standard_normal(n)
+    basis = [b / np.linalg.norm(b)]
+    for i in range(1, 7):
+        x, it, err = O.krylov("RGMRES", a, w, z, d, b, 1e-30, i, irst=50)
+        assert it == i
+        K = np.stack(basis, axis=1)
+        c, *_ = np.linalg.lstsq(AM @ K, b, rcond=None)
+        rmin = np.linalg.norm(b - AM @ K @ c)
+        assert abs(err * np.linalg.norm(b) - rmin) <= 1e-7 * rmin, (i, err)
+        assert np.linalg.norm(b - a @ x) == pytest.approx(rmin, rel=1e-7)
+        nxt = AM @ basis[-1]
+        for q in basis:  # orthonormal basis keeps the lstsq well posed
+            nxt = nxt - (q @ nxt) * q
+        basis.append(nxt / np.linalg.norm(nxt))
+
+
+def test_gmres_finite_termination_and_restart(O):
+    """Without restart GMRES solves an n x n system in at most n steps; a
+    short restart still converges on a convection-diffusion matrix."""
+    rng = np.random.default_rng(9)
+    n = 9
+    a = sp.csr_matrix(rng.standard_normal((n, n)) + 4 * np.eye(n))
+    i1 = sp.identity(n, format="csr")
+    b = rng.standard_normal(n)
+    x, it, err = O.krylov("RGMRES", a, i1, i1, np.ones(n), b, 1e-12, 100,
+                          irst=n)
+    assert it <= n and err <= 1e-12
+    assert np.linalg.norm(b - a @ x) <= 1e-11 * np.linalg.norm(b)
+    a = R.pde3d_numpy(6)
+    w, z, d = O.invk_bld(a, 1, 1)
+    b = a @ np.ones(a.shape[0])
+    x2, it2, err2 = O.krylov("RGMRES", a, w, z, d, b, 1e-8, 2000, irst=2)
+    x30, it30, _ = O.krylov("RGMRES", a, w, z, d, b, 1e-8, 2000, irst=30)
+    assert err2 <= 1e-8 and it2 >= it30
+    for xx in (x2, x30):
+        assert np.linalg.norm(b - a @ xx) <= 1e-7 * np.linalg.norm(b)
+
+
 @pytest.mark.parametrize("method", ["BICGSTAB", "CG"])
 def test_threaded_loops_same_solve(O, method):
     """The row-parallel loops (CPU baseline on all host cores) change only
