@@ -205,17 +205,21 @@ int ainv_mat_free(ainv_mat_h a);
 int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
                   double *y, int k, int64_t ldx, int64_t ldy, void *stream);
 
-/* method: "BICGSTAB", "CG", "CGS" or "BICG" (psb_krylov's names; CGS and
- * BiCG in the preconditioned forms of Barrett et al., Templates, figs. 2.9
- * and 2.8; BiCG applies A^T and M^T through transposed layouts built on
- * its first call); istop = 2
+/* method: "BICGSTAB", "CG", "CGS", "BICG" or "RGMRES" (psb_krylov's names,
+ * the methods of tests/pdegen/runs/mld3d.inp:1; CGS and BiCG in the
+ * preconditioned forms of Barrett et al., Templates, figs. 2.9 and 2.8;
+ * BiCG applies A^T and M^T through transposed layouts built on its first
+ * call; RGMRES is right-preconditioned GMRES restarted every irst steps,
+ * irst <= 0 meaning 10, as psb_krylov's irst argument,
+ * tests/fileread/df_sample.f90:368-369; one RGMRES iteration is one
+ * Arnoldi step and its in-cycle test uses the rotated residual); istop = 2
  * (||r||_2 <= eps ||b||_2); x holds
  * x0 on entry.  k interleaved right-hand sides, each with its own
  * recurrence and stopping test; iter[c] and err[c] (host arrays of k) get
  * the iteration count and final ||r_c||/||b_c|| of column c.  prec may be
  * NULL (no preconditioner). */
 int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
-                double *x, int k, double eps, int itmax, int istop,
+                double *x, int k, double eps, int itmax, int istop, int irst,
                 int32_t *iter, double *err, int flags, void *stream);
 
 /* Served form of the same solve (no counterpart in the reference, whose

@@ -92,7 +92,7 @@ def lib():
         "ainv_krylov_stream": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_int,
                                C.c_double, C.c_int, vp, vp, vp, vp],
         "ainv_krylov": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_double,
-                        C.c_int, C.c_int, i32p, dp, C.c_int, vp],
+                        C.c_int, C.c_int, C.c_int, i32p, dp, C.c_int, vp],
         "ainv_gen_pde3d": [C.c_int32, C.c_int, i32p, i64p, i64p, i32p, dp],
         "ainv_gen_aniso2d": [C.c_int32, C.c_double, i32p, i64p, i64p, i32p, dp],
         "ainv_gen_random": [C.c_int32, C.c_uint64, C.c_double, C.c_double,
@@ -514,12 +514,12 @@ KRYLOV_EAGER = 2
 
 
 def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2,
-           eager=False):
+           eager=False, irst=10):
     """psb_krylov(method, a, prec, b, x, eps, ...) for k interleaved
     right-hand sides.  a: SystemMatrix; prec: solver object or None;
-    b, x: numpy (host) or DeviceArray.  eager=True launches the kernels of
-    each iteration one by one instead of replaying the captured graph.
-    Returns (x, iters[k], errs[k])."""
+    b, x: numpy (host) or DeviceArray; irst: restart of RGMRES.  eager=True
+    launches the kernels of each iteration one by one instead of replaying
+    the captured graphs.  Returns (x, iters[k], errs[k])."""
     host = not isinstance(b, DeviceArray)
     if host:
         b = np.ascontiguousarray(b, dtype=np.float64)
@@ -539,7 +539,7 @@ def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2,
     iters = np.zeros(k, dtype=np.int32)
     errs = np.zeros(k, dtype=np.float64)
     _ck(lib().ainv_krylov(_b(method), a._h, prec._h if prec else None, bp, xp,
-                          k, eps, itmax, istop,
+                          k, eps, itmax, istop, irst,
                           iters.ctypes.data_as(C.POINTER(C.c_int32)),
                           errs.ctypes.data_as(C.POINTER(C.c_double)), flags,
                           None))

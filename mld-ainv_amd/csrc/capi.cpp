@@ -1013,7 +1013,7 @@ int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
 }
 
 int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
-                double *x, int k, double eps, int itmax, int istop,
+                double *x, int k, double eps, int itmax, int istop, int irst,
                 int32_t *iter, double *err, int flags, void *stream) {
   return guard([&] {
     if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
@@ -1036,11 +1036,11 @@ int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
       AINV_HIP_CHECK(hipMemcpyAsync(db.p, b, bytes, hipMemcpyHostToDevice, s));
       AINV_HIP_CHECK(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, s));
       krylov_solve(m, a, prec, db.as<double>(), dx.as<double>(), k, eps, itmax,
-                   iter, err, s, flags);
+                   iter, err, s, flags, irst);
       AINV_HIP_CHECK(hipMemcpyAsync(x, dx.p, bytes, hipMemcpyDeviceToHost, s));
       AINV_HIP_CHECK(hipStreamSynchronize(s));
     } else {
-      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, s, flags);
+      krylov_solve(m, a, prec, b, x, k, eps, itmax, iter, err, s, flags, irst);
     }
   });
 }

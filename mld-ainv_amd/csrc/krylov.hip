@@ -456,12 +456,21 @@ struct KrylovPlan {
   // slot batches: per-column states copied back each step, admission mask
   Col *pinned_cols = nullptr;  // two slots of kMaxCols states
   DevBuf mask;
+  // RGMRES: restart length, Arnoldi basis (irst + 1 vectors of n x K),
+  // per-column states and small dense arrays, one graph per Arnoldi step
+  // of a cycle plus one for the cycle's end
+  int irst = 0;
+  DevBuf basis, gmcol, gmarr;
+  std::vector<hipGraphExec_t> gm_exec;
 
   void release_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+    for (auto &e : gm_exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    gm_exec.clear();
   }
   ~KrylovPlan() {
     release_graph();
@@ -477,10 +486,10 @@ void destroy_plan(KrylovPlan *p) { delete p; }
 namespace {
 
 KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
-                     hipStream_t s) {
+                     hipStream_t s, int irst = 0) {
   for (size_t i = 0; i < a->plans.size(); ++i) {
     KrylovPlan *p = a->plans[i].get();
-    if (p->method == method && p->m == m && p->k == k) {
+    if (p->method == method && p->m == m && p->k == k && p->irst == irst) {
       if (m && p->gen != m->generation) {
         p->release_graph();  // factors changed: recapture
         if (p->tbuf.bytes < solver_t_bytes(m, p->K))
@@ -495,6 +504,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   std::unique_ptr<KrylovPlan, PlanDeleter> up(new KrylovPlan());
   KrylovPlan *p = up.get();
   p->method = method;
+  p->irst = irst;
   p->cg = method == KM_CG;
   const bool cg = p->cg;
   p->m = m;
@@ -504,7 +514,8 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   while (p->K < k) p->K <<= 1;
   p->n = a->n;
   const int64_t nk = (int64_t)a->n * p->K;
-  const int nvec = cg ? 6 : 10;  // x, b + recurrence vectors
+  // x, b + recurrence vectors (RGMRES: x, b, w, z and the basis apart)
+  const int nvec = (method == KM_GMRES || cg) ? 6 : 10;
   p->work.alloc((size_t)nk * nvec * sizeof(double));
   // padding columns (k..K) are read by the row-mapped kernels: keep them 0
   AINV_HIP_CHECK(hipMemsetAsync(p->work.p, 0, p->work.bytes, s));
@@ -515,7 +526,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   p->v.p = w0 + 3 * nk;
   p->v.z = w0 + 4 * nk;
   p->v.q = w0 + 5 * nk;  // CG: A p ; BiCGSTAB: rtilde
-  if (!cg) {
+  if (!cg && method != KM_GMRES) {
     p->v.v = w0 + 6 * nk;
     p->v.f = w0 + 7 * nk;
     p->v.s = w0 + 8 * nk;
@@ -734,8 +745,9 @@ int krylov_method_id(const std::string &name) {
   if (name == "CG") return KM_CG;
   if (name == "CGS") return KM_CGS;
   if (name == "BICG") return KM_BICG;
+  if (name == "RGMRES" || name == "GMRES") return KM_GMRES;
   throw Error(AINV_ERR_ARG, "unknown Krylov method " + name +
-                                " (BICGSTAB, CG, CGS, BICG available)");
+                                " (BICGSTAB, CG, CGS, BICG, RGMRES available)");
 }
 
 // ---------------------------------------------------------- slot batches
@@ -751,6 +763,10 @@ struct SlotBatch {
 SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s) {
   if (K < 1 || K > kMaxCols) throw Error(AINV_ERR_ARG, "slots must be 1..64");
+  if (method == KM_GMRES)
+    throw Error(AINV_ERR_ARG,
+                "the served form runs BICGSTAB, CG, CGS and BICG (RGMRES "
+                "restarts in cycles)");
   if (m && m->n != a->n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
   KrylovPlan *p = get_plan(a, method, m, K, s);
@@ -888,14 +904,23 @@ void batch_extract(SlotBatch *b, int c0, int kc, double *dst) {
   AINV_HIP_CHECK(hipGetLastError());
 }
 
+void gmres_solve(ainv_matrix *a, ainv_solver *m, const double *b, double *x,
+                 int k, double eps, int itmax, int irst, int32_t *iter,
+                 double *err, hipStream_t s, int flags);
+
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                   const double *b, double *x, int k, double eps, int itmax,
-                  int32_t *iter, double *err, hipStream_t s, int flags) {
+                  int32_t *iter, double *err, hipStream_t s, int flags,
+                  int irst) {
   const int32_t n = a->n;
   if (k < 1 || k > kMaxCols) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   if (m && m->n != n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
   const int mid = krylov_method_id(method);
+  if (mid == KM_GMRES) {
+    gmres_solve(a, m, b, x, k, eps, itmax, irst, iter, err, s, flags);
+    return;
+  }
   const bool cg = mid == KM_CG;
   KrylovPlan *p = get_plan(a, mid, m, k, s);
   prepare_method(p, a, m);
@@ -958,6 +983,440 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   AINV_HIP_CHECK(hipGetLastError());
   std::vector<Col> hc(k);
   AINV_HIP_CHECK(hipMemcpyAsync(hc.data(), p->col, sizeof(Col) * k,
+                                hipMemcpyDeviceToHost, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < k; ++c) {
+    iter[c] = hc[c].iter;
+    err[c] = hc[c].err;
+  }
+}
+
+// ------------------------------------------------------------------ RGMRES
+// Restarted GMRES with right preconditioning (Saad, Iterative Methods for
+// Sparse Linear Systems, 2nd ed., alg. 9.5), modified Gram-Schmidt Arnoldi
+// and Givens rotations, psb_krylov's 'RGMRES' with restart irst
+// (tests/fileread/runs/dfs.inp:10), for k interleaved columns.  The basis
+// v_0..v_m lives in HBM as m+1 blocks of n x K; the Hessenberg columns,
+// rotations, rotated right-hand side and solution coefficients of each
+// column are small per-column arrays updated by one block per column.
+// Step i of a cycle is one captured graph: z = M v_i, w = A z with
+// (w, v_0) fused, i+1 passes w -= h_ji v_j each fused with the next dot
+// ((v_{j+1}, w) or (w, w)), the rotations, v_{i+1} = w / h_{i+1,i}.  The
+// end graph solves H y = rs per column, x += M (V y) and forms the next
+// cycle's true residual.
+namespace {
+
+struct GmCol {
+  double bn, beta, err, eps, hn;
+  int32_t active, done, iter, len;
+};
+
+enum GmOp {
+  GM_INIT,      // dots (b,b) (v0,v0)
+  GM_SCALE0,    // v0 *= 1 / beta (active)
+  GM_AXPY_DOT,  // w -= h_ji v_j ; (v_{j+1}, w) or, at j == i, (w, w)
+  GM_SCALE,     // v_{i+1} = (1 / h_{i+1,i}) w (active)
+  GM_COMBINE,   // w = sum_{j < len} y_j v_j (len > 0)
+  GM_XUPD,      // x += z (len > 0)
+  GM_RCOPY,     // v0 = b
+  GM_RNORM,     // (v0, v0) for the columns that restart
+};
+
+struct GmArgs {
+  double *x, *b, *w, *z, *v;  // v: basis vector 0
+  int64_t vs;                 // elements from one basis vector to the next
+  const GmCol *col;
+  const double *arr;          // per-column arrays, stride astr
+  int astr, m, j, i;
+};
+
+// per-column array offsets: H column-major (m+1) x m, cs, sn, rs, y
+__host__ __device__ inline int gm_h(int m, int r, int c) { return c * (m + 1) + r; }
+__host__ __device__ inline int gm_cs(int m) { return (m + 1) * m; }
+__host__ __device__ inline int gm_sn(int m) { return (m + 1) * m + m; }
+__host__ __device__ inline int gm_rs(int m) { return (m + 1) * m + 2 * m; }
+__host__ __device__ inline int gm_y(int m) { return (m + 1) * m + 3 * m + 1; }
+inline int gm_stride(int m) { return ((m + 1) * m + 4 * m + 1 + 7) & ~7; }
+
+template <int OP> struct GmPairs { static constexpr int v = 0; };
+template <> struct GmPairs<GM_INIT> { static constexpr int v = 2; };
+template <> struct GmPairs<GM_AXPY_DOT> { static constexpr int v = 1; };
+template <> struct GmPairs<GM_RNORM> { static constexpr int v = 1; };
+
+template <int OP, int K>
+__global__ __launch_bounds__(kThreads) void k_gm_vec(int32_t n, int k, GmArgs g,
+                                                     double *__restrict__ part) {
+  constexpr int NP = GmPairs<OP>::v;
+  constexpr int R = kThreads / K;
+  const int c = threadIdx.x % K;
+  const int rr = threadIdx.x / K;
+  bool cok = c < k;
+  GmCol cs{};
+  if (cok) {
+    cs = g.col[c];
+    if constexpr (OP == GM_SCALE0 || OP == GM_AXPY_DOT || OP == GM_SCALE)
+      cok = cs.active != 0;
+    else if constexpr (OP == GM_COMBINE || OP == GM_XUPD)
+      cok = cs.len > 0;
+    else if constexpr (OP == GM_RNORM)
+      cok = cs.len > 0 && !cs.done;
+  }
+  double acc[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int q = 0; q < (NP > 0 ? NP : 1); ++q) acc[q] = 0.0;
+  if (cok) {
+    const double *ar = g.arr + (int64_t)c * g.astr;
+    double sc = 0.0;
+    if constexpr (OP == GM_SCALE0) sc = 1.0 / cs.beta;
+    if constexpr (OP == GM_SCALE) sc = 1.0 / cs.hn;
+    if constexpr (OP == GM_AXPY_DOT) sc = ar[gm_h(g.m, g.j, g.i)];
+    const double *vj = g.v + (int64_t)g.j * g.vs;
+    for (int64_t r = (int64_t)blockIdx.x * R + rr; r < n;
+         r += (int64_t)gridDim.x * R) {
+      const int64_t e = r * K + c;
+      if constexpr (OP == GM_INIT) {
+        acc[0] += g.b[e] * g.b[e];
+        acc[1] += g.v[e] * g.v[e];
+      } else if constexpr (OP == GM_SCALE0) {
+        g.v[e] = sc * g.v[e];
+      } else if constexpr (OP == GM_AXPY_DOT) {
+        const double wn = g.w[e] + (-sc) * vj[e];
+        g.w[e] = wn;
+        acc[0] += (g.j < g.i ? vj[g.vs + e] : wn) * wn;
+      } else if constexpr (OP == GM_SCALE) {
+        if (cs.hn != 0.0) g.v[(int64_t)(g.i + 1) * g.vs + e] = sc * g.w[e];
+      } else if constexpr (OP == GM_COMBINE) {
+        const double *y = ar + gm_y(g.m);
+        double u = 0.0;
+        for (int j = 0; j < cs.len; ++j) u = u + y[j] * g.v[(int64_t)j * g.vs + e];
+        g.w[e] = u;
+      } else if constexpr (OP == GM_XUPD) {
+        g.x[e] = g.x[e] + g.z[e];
+      } else if constexpr (OP == GM_RCOPY) {
+        g.v[e] = g.b[e];
+      } else if constexpr (OP == GM_RNORM) {
+        acc[0] += g.v[e] * g.v[e];
+      }
+    }
+  }
+  if constexpr (NP > 0) {
+    __shared__ double red[NP][kThreads];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int st = R / 2; st >= 1; st >>= 1) {
+      if (rr < st) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+          red[q][threadIdx.x] += red[q][threadIdx.x + st * K];
+      }
+      __syncthreads();
+    }
+    if (rr == 0 && c < k) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        part[((int64_t)q * kRedBlocks + blockIdx.x) * kMaxCols + c] =
+            red[q][threadIdx.x];
+    }
+  }
+}
+
+template <int OP>
+int gm_vec(int32_t n, int k, int K, const GmArgs &g, double *part,
+           hipStream_t s) {
+  const int nb = vec_blocks(n, k);
+#define AINV_GL(KK)                                                         \
+  hipLaunchKernelGGL((k_gm_vec<OP, KK>), dim3(nb), dim3(kThreads), 0, s, n, \
+                     k, g, part)
+  switch (K) {
+    case 1: AINV_GL(1); break;
+    case 2: AINV_GL(2); break;
+    case 4: AINV_GL(4); break;
+    case 8: AINV_GL(8); break;
+    case 16: AINV_GL(16); break;
+    case 32: AINV_GL(32); break;
+    default: AINV_GL(64); break;
+  }
+#undef AINV_GL
+  AINV_HIP_CHECK(hipGetLastError());
+  return nb;
+}
+
+enum GmFin {
+  FG_INIT,     // bn, beta, err, active / done, rs = beta e_1
+  FG_H,        // h_ji = sum (j == 0 resets the active count)
+  FG_NORM,     // h_{i+1,i}, rotations, rs, err, convergence
+  FG_SOLVE,    // y = H^-1 rs (len > 0); resets the active count
+  FG_RESTART,  // beta, err of the true residual; rs = beta e_1; len = 0
+};
+
+__global__ __launch_bounds__(kThreads) void k_gm_fin(
+    int fin, int j, int i, int nblk, const double *__restrict__ part,
+    GmCol *__restrict__ col, double *__restrict__ arr, int astr, int m,
+    int32_t *__restrict__ nactive) {
+  const int c = blockIdx.x;
+  const int np = fin == FG_INIT ? 2 : (fin == FG_SOLVE ? 0 : 1);
+  __shared__ double red[2][kThreads];
+  for (int q = 0; q < np; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kThreads)
+      s += part[((int64_t)q * kRedBlocks + b) * kMaxCols + c];
+    red[q][threadIdx.x] = s;
+  }
+  __syncthreads();
+  for (int st = kThreads / 2; st >= 1; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int q = 0; q < np; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (((fin == FG_H && j == 0) || fin == FG_SOLVE) && c == 0) *nactive = 0;
+  GmCol cs = col[c];
+  double *ar = arr + (int64_t)c * astr;
+  double *rs = ar + gm_rs(m);
+  const double s0 = red[0][0], s1 = red[1][0];
+  switch (fin) {
+    case FG_INIT:
+      cs.bn = sqrt(s0);
+      cs.beta = sqrt(s1);
+      cs.err = (cs.bn > 0.0) ? cs.beta / cs.bn : cs.beta;
+      cs.iter = 0;
+      cs.len = 0;
+      cs.hn = 0.0;
+      cs.done = cs.err <= cs.eps;
+      cs.active = !cs.done;
+      for (int q = 0; q <= m; ++q) rs[q] = 0.0;
+      rs[0] = cs.beta;
+      break;
+    case FG_H:
+      if (cs.active) ar[gm_h(m, j, i)] = s0;
+      break;
+    case FG_NORM: {
+      if (!cs.active) break;
+      cs.iter += 1;
+      double *cv = ar + gm_cs(m), *sv = ar + gm_sn(m);
+      const double hn = sqrt(s0);
+      cs.hn = hn;
+      ar[gm_h(m, i + 1, i)] = hn;
+      for (int q = 0; q < i; ++q) {
+        const double t0 = ar[gm_h(m, q, i)], t1 = ar[gm_h(m, q + 1, i)];
+        ar[gm_h(m, q, i)] = cv[q] * t0 + sv[q] * t1;
+        ar[gm_h(m, q + 1, i)] = -sv[q] * t0 + cv[q] * t1;
+      }
+      const double hii = ar[gm_h(m, i, i)];
+      const double gg = sqrt(hii * hii + hn * hn);
+      if (gg == 0.0) {  // breakdown: stop with the steps so far
+        cs.len = i;
+        cs.active = 0;
+        cs.done = 1;
+        break;
+      }
+      cv[i] = hii / gg;
+      sv[i] = hn / gg;
+      ar[gm_h(m, i, i)] = gg;
+      ar[gm_h(m, i + 1, i)] = 0.0;
+      rs[i + 1] = -sv[i] * rs[i];
+      rs[i] = cv[i] * rs[i];
+      cs.len = i + 1;
+      cs.err = (cs.bn > 0.0) ? fabs(rs[i + 1]) / cs.bn : fabs(rs[i + 1]);
+      if (cs.err <= cs.eps) {
+        cs.active = 0;
+        cs.done = 1;
+      }
+      break;
+    }
+    case FG_SOLVE: {
+      double *y = ar + gm_y(m);
+      for (int q = cs.len - 1; q >= 0; --q) {
+        double t = rs[q];
+        for (int l = q + 1; l < cs.len; ++l) t = t - ar[gm_h(m, q, l)] * y[l];
+        y[q] = t / ar[gm_h(m, q, q)];
+      }
+      break;
+    }
+    case FG_RESTART:
+      if (cs.len > 0 && !cs.done) {
+        cs.beta = sqrt(s0);
+        cs.err = (cs.bn > 0.0) ? cs.beta / cs.bn : cs.beta;
+        cs.done = cs.err <= cs.eps;
+        cs.active = !cs.done;
+        for (int q = 0; q <= m; ++q) rs[q] = 0.0;
+        rs[0] = cs.beta;
+      }
+      cs.len = 0;
+      break;
+  }
+  col[c] = cs;
+  if ((fin == FG_INIT || fin == FG_NORM || fin == FG_RESTART) && cs.active)
+    atomicAdd(nactive, 1);
+}
+
+void gm_fin(const KrylovPlan *p, int which, int j, int i, int nblk,
+            hipStream_t s) {
+  hipLaunchKernelGGL(k_gm_fin, dim3(p->k), dim3(kThreads), 0, s, which, j, i,
+                     nblk, p->pt, p->gmcol.as<GmCol>(), p->gmarr.as<double>(),
+                     gm_stride(p->irst), p->irst, p->na);
+  AINV_HIP_CHECK(hipGetLastError());
+}
+
+GmArgs gm_args(const KrylovPlan *p, int j, int i) {
+  GmArgs g{};
+  g.x = p->v.x;
+  g.b = p->v.b;
+  g.w = p->v.r;
+  g.z = p->v.z;
+  g.v = p->basis.as<double>();
+  g.vs = (int64_t)p->n * p->K;
+  g.col = p->gmcol.as<GmCol>();
+  g.arr = p->gmarr.as<double>();
+  g.astr = gm_stride(p->irst);
+  g.m = p->irst;
+  g.j = j;
+  g.i = i;
+  return g;
+}
+
+void gm_step(const KrylovPlan *p, ainv_matrix *a, int i, hipStream_t s) {
+  const int32_t n = p->n;
+  const int k = p->k, K = p->K;
+  double *v0 = p->basis.as<double>();
+  const int64_t vs = (int64_t)n * K;
+  precond(p, v0 + i * vs, p->v.z, s);
+  int g = hll_spmv_dot(a->hll, p->v.z, p->v.r, k, K, v0, 1, p->pt,
+                       p->spmv_map, s);
+  gm_fin(p, FG_H, 0, i, g, s);
+  for (int j = 0; j <= i; ++j) {
+    g = gm_vec<GM_AXPY_DOT>(n, k, K, gm_args(p, j, i), p->pt, s);
+    gm_fin(p, j < i ? FG_H : FG_NORM, j + 1, i, g, s);
+  }
+  gm_vec<GM_SCALE>(n, k, K, gm_args(p, 0, i), p->pt, s);
+}
+
+// the true residual of x into v_0 and its norm (columns that restart)
+void gm_residual(const KrylovPlan *p, ainv_matrix *a, hipStream_t s,
+                 bool init) {
+  const int32_t n = p->n;
+  const int k = p->k, K = p->K;
+  const GmArgs g0 = gm_args(p, 0, 0);
+  gm_vec<GM_RCOPY>(n, k, K, g0, p->pt, s);
+  hll_spmm_axpby(a->hll, -1.0, p->v.x, false, K, 1.0, g0.v, K, k, s,
+                 p->spmv_map);
+  const int g = init ? gm_vec<GM_INIT>(n, k, K, g0, p->pt, s)
+                     : gm_vec<GM_RNORM>(n, k, K, g0, p->pt, s);
+  gm_fin(p, init ? FG_INIT : FG_RESTART, 0, 0, g, s);
+  gm_vec<GM_SCALE0>(n, k, K, g0, p->pt, s);
+}
+
+void gm_end(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
+  const int32_t n = p->n;
+  const int k = p->k, K = p->K;
+  const GmArgs g0 = gm_args(p, 0, 0);
+  gm_fin(p, FG_SOLVE, 0, 0, 0, s);
+  gm_vec<GM_COMBINE>(n, k, K, g0, p->pt, s);
+  precond(p, p->v.r, p->v.z, s);
+  gm_vec<GM_XUPD>(n, k, K, g0, p->pt, s);
+  gm_residual(p, a, s, false);
+}
+
+void gm_capture(KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
+  if (!p->gm_exec.empty()) return;
+  for (int i = 0; i <= p->irst; ++i) {
+    hipGraph_t gr = nullptr;
+    AINV_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      if (i < p->irst)
+        gm_step(p, a, i, s);
+      else
+        gm_end(p, a, s);
+    } catch (...) {
+      hipGraph_t g0 = nullptr;
+      (void)hipStreamEndCapture(s, &g0);
+      if (g0) (void)hipGraphDestroy(g0);
+      throw;
+    }
+    AINV_HIP_CHECK(hipStreamEndCapture(s, &gr));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    AINV_HIP_CHECK(e);
+    p->gm_exec.push_back(ex);
+  }
+}
+
+}  // namespace
+
+void gmres_solve(ainv_matrix *a, ainv_solver *m, const double *b, double *x,
+                 int k, double eps, int itmax, int irst, int32_t *iter,
+                 double *err, hipStream_t s, int flags) {
+  const int32_t n = a->n;
+  const int mr = irst <= 0 ? 10 : irst;
+  if (mr > 500) throw Error(AINV_ERR_ARG, "irst must be <= 500");
+  KrylovPlan *p = get_plan(a, KM_GMRES, m, k, s, mr);
+  const int K = p->K;
+  const int64_t vs = (int64_t)n * K;
+  if (!p->basis.p) {
+    p->basis.alloc((size_t)(mr + 1) * vs * sizeof(double));
+    // padding columns are read by the row-mapped kernels: keep them 0
+    AINV_HIP_CHECK(hipMemsetAsync(p->basis.p, 0, p->basis.bytes, s));
+    p->gmcol.alloc(sizeof(GmCol) * kMaxCols);
+    p->gmarr.alloc(sizeof(double) * (size_t)gm_stride(mr) * kMaxCols);
+  }
+  const unsigned gp = (unsigned)((vs + 255) / 256);
+  hipLaunchKernelGGL(k_pack, dim3(gp), dim3(256), 0, s, n, k, K, b, p->v.b);
+  hipLaunchKernelGGL(k_pack, dim3(gp), dim3(256), 0, s, n, k, K, x, p->v.x);
+  AINV_HIP_CHECK(hipGetLastError());
+  std::vector<GmCol> init(kMaxCols);
+  for (auto &c0 : init) {
+    c0 = GmCol{};
+    c0.eps = eps;
+  }
+  AINV_HIP_CHECK(hipMemcpyAsync(p->gmcol.p, init.data(),
+                                sizeof(GmCol) * kMaxCols,
+                                hipMemcpyHostToDevice, s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  const bool eager = (flags & AINV_KRYLOV_EAGER) != 0;
+  // the init step uses the tuned system-matrix mapping, as the graphs do
+  tune_plan(p, a, m, s);
+  gm_residual(p, a, s, true);
+  AINV_HIP_CHECK(hipMemcpyAsync(p->pinned, p->na, sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  int active = p->pinned[0];
+  if (active > 0 && itmax > 0 && !eager) gm_capture(p, a, s);
+  int it = 0;
+  while (active > 0 && it < itmax) {
+    // one cycle: launch step i, then look at the count after step i - 1
+    for (int i = 0; i < mr && it < itmax; ++i, ++it) {
+      if (eager)
+        gm_step(p, a, i, s);
+      else
+        AINV_HIP_CHECK(hipGraphLaunch(p->gm_exec[i], s));
+      const int slot = i & 1;
+      AINV_HIP_CHECK(hipMemcpyAsync(p->pinned + slot, p->na, sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+      AINV_HIP_CHECK(hipEventRecord(p->ev[slot], s));
+      if (i >= 1) {
+        AINV_HIP_CHECK(hipEventSynchronize(p->ev[slot ^ 1]));
+        if (p->pinned[slot ^ 1] == 0) {
+          ++it;
+          break;
+        }
+      }
+    }
+    if (eager)
+      gm_end(p, a, s);
+    else
+      AINV_HIP_CHECK(hipGraphLaunch(p->gm_exec[mr], s));
+    AINV_HIP_CHECK(hipMemcpyAsync(p->pinned, p->na, sizeof(int32_t),
+                                  hipMemcpyDeviceToHost, s));
+    AINV_HIP_CHECK(hipStreamSynchronize(s));
+    active = p->pinned[0];
+  }
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)(((int64_t)n * k + 255) / 256)),
+                     dim3(256), 0, s, n, k, K, p->v.x, x);
+  AINV_HIP_CHECK(hipGetLastError());
+  std::vector<GmCol> hc(k);
+  AINV_HIP_CHECK(hipMemcpyAsync(hc.data(), p->gmcol.p, sizeof(GmCol) * k,
                                 hipMemcpyDeviceToHost, s));
   AINV_HIP_CHECK(hipStreamSynchronize(s));
   for (int c = 0; c < k; ++c) {

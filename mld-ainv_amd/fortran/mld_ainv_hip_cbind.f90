@@ -167,13 +167,13 @@ module mld_ainv_hip_cbind
       type(c_ptr), value :: a
       integer(c_int) :: ainv_mat_free
     end function
-    function ainv_krylov(method, a, prec, b, x, k, eps, itmax, istop, iter, &
-         & err, flags, stream) bind(c, name='ainv_krylov')
+    function ainv_krylov(method, a, prec, b, x, k, eps, itmax, istop, irst, &
+         & iter, err, flags, stream) bind(c, name='ainv_krylov')
       import :: c_int, c_ptr, c_char, c_double, c_int32_t
       character(kind=c_char) :: method(*)
       type(c_ptr), value :: a, prec
       real(c_double) :: b(*), x(*)
-      integer(c_int), value :: k, itmax, istop, flags
+      integer(c_int), value :: k, itmax, istop, irst, flags
       real(c_double), value :: eps
       integer(c_int32_t) :: iter(*)
       real(c_double) :: err(*)

@@ -303,11 +303,12 @@ contains
   end function
 
   ! psb_krylov(method, a, prec, b, x, eps, ...) with istop = 2 on the local
-  ! block (1-based CSR) and host vectors
+  ! block (1-based CSR) and host vectors; irst: restart of RGMRES
   subroutine mld_krylov(method, n, irp, ja, val, prec, b, x, eps, itmax, &
-       & iter, err, info)
+       & iter, err, info, irst)
     character(len=*), intent(in) :: method
     integer, intent(in) :: n, itmax
+    integer, intent(in), optional :: irst
     integer(c_int64_t), intent(in) :: irp(:)
     integer(c_int32_t), intent(in) :: ja(:)
     real(c_double), intent(in) :: val(:), b(:), eps
@@ -319,12 +320,15 @@ contains
     integer(c_int32_t) :: it(1)
     real(c_double) :: er(1), bc(size(b))
     integer :: i2
+    integer(c_int) :: nl
+    nl = 10
+    if (present(irst)) nl = irst
     a = c_null_ptr
     info = ainv_mat_create(int(n, c_int32_t), irp, ja, val, 1_c_int, a)
     if (info /= 0) return
     bc = b
     info = ainv_krylov(cstr(method), a, prec%h, bc, x, 1_c_int, eps, &
-         & int(itmax, c_int), 2_c_int, it, er, ainv_host_ptrs, c_null_ptr)
+         & int(itmax, c_int), 2_c_int, nl, it, er, ainv_host_ptrs, c_null_ptr)
     iter = it(1)
     err = er(1)
     i2 = ainv_mat_free(a)

@@ -318,7 +318,64 @@ def test_krylov_bicg(gpu, O, kind, k):
             1e-7 * np.linalg.norm(Bc[:, c])
 
 
-@pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS", "BICG"])
+@pytest.mark.parametrize("kind,k,irst", [("INVK", 1, 10), ("INVK", 5, 2),
+                                         ("AINV", 3, 30), ("NONE", 2, 5),
+                                         ("INVK", 33, 8)])
+def test_krylov_rgmres(gpu, O, kind, k, irst):
+    """Restarted GMRES on the GPU: per-column iteration counts within +-1 of
+    the oracle's RGMRES (itself pinned by the least-squares minimum over the
+    Krylov space and by scipy's gmres), true residuals."""
+    a = gpu.gen_pde3d(12)
+    n = a.shape[0]
+    if kind == "INVK":
+        sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+        w, z, d = O.invk_bld(a, 1, 1)
+    elif kind == "AINV":
+        sv = build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
+                   SUB_ILUTHRS=0.1)
+        w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
+    else:
+        sv = None
+        w = z = sp.identity(n, format="csr")
+        d = np.ones(n)
+    B = gpu.gen_normal(19, (n, k)) if k > 1 else a @ np.ones(n)
+    A = gpu.SystemMatrix(a)
+    X, it, err = gpu.krylov("RGMRES", A, sv, B, eps=1e-8, itmax=600,
+                            irst=irst)
+    Bc = B.reshape(n, -1)
+    Xc = X.reshape(n, -1)
+    for c in range(k):
+        _, it_ref, _ = O.krylov("RGMRES", a, w, z, d, Bc[:, c], 1e-8, 600,
+                                irst=irst)
+        assert abs(int(it[c]) - it_ref) <= 1, (c, it[c], it_ref)
+        assert err[c] <= 1e-8
+        assert np.linalg.norm(Bc[:, c] - a @ Xc[:, c]) <= \
+            1e-7 * np.linalg.norm(Bc[:, c])
+
+
+def test_rgmres_itmax_and_served_form(gpu, O):
+    """RGMRES stops at itmax mid-cycle with x updated from the steps taken
+    (the oracle's x to rounding); the served form refuses it."""
+    a = gpu.gen_pde3d(10)
+    n = a.shape[0]
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    w, z, d = O.invk_bld(a, 1, 1)
+    b = a @ np.ones(n)
+    A = gpu.SystemMatrix(a)
+    x, it, err = gpu.krylov("RGMRES", A, sv, b, eps=1e-14, itmax=7, irst=5)
+    xr, itr, errr = O.krylov("RGMRES", a, w, z, d, b, 1e-14, 7, irst=5)
+    assert int(it[0]) == itr == 7
+    assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
+    assert err[0] == pytest.approx(errr, rel=1e-6)
+    with pytest.raises(gpu.AinvError) as e:
+        gpu.krylov_stream("RGMRES", A, sv,
+                          gpu.DeviceArray.from_numpy(np.stack([b, b])),
+                          slots=2)
+    assert "RGMRES" in str(e.value)
+
+
+@pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS", "BICG",
+                                    "RGMRES"])
 def test_graph_replays_match_eager(gpu, method):
     """The captured iteration graph, replayed solve after solve with eager
     work and synchronizations in between, gives bit-identical iterates and
