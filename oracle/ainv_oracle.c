@@ -1511,6 +1511,115 @@ int or_gmres(const or_mat *a, const or_mat *w, const or_mat *z,
   return OR_OK;
 }
 
+/* BiCGStab(l), Sleijpen-Fokkema alg. 3.1 on B = A M.  rhat_j, uhat_j are
+ * rows of r, u (j = 0..l); tau is l x l (tau[i][j], i < j), sig, gp (gamma'),
+ * g (gamma), gpp (gamma'') are indexed 1..l. */
+int or_bicgstabl(const or_mat *a, const or_mat *w, const or_mat *z,
+                 const double *d, const double *b, double *x, double eps,
+                 int32_t itmax, int32_t l, int32_t *iter, double *err) {
+  const int32_t n = a->n;
+  if (l < 1) l = 2;
+  const size_t ln = (size_t)n + 1;
+  double *r = (double *)calloc(ln * (size_t)(l + 1), sizeof(double));
+  double *u = (double *)calloc(ln * (size_t)(l + 1), sizeof(double));
+  double *rt = (double *)malloc(ln * sizeof(double));
+  double *yh = (double *)calloc(ln, sizeof(double));
+  double *tz = (double *)malloc(ln * sizeof(double));
+  double *tau = (double *)calloc((size_t)(l + 1) * (l + 1), sizeof(double));
+  double *sig = (double *)calloc((size_t)l + 1, sizeof(double));
+  double *gp = (double *)calloc((size_t)l + 1, sizeof(double));
+  double *g = (double *)calloc((size_t)l + 1, sizeof(double));
+  double *gpp = (double *)calloc((size_t)l + 1, sizeof(double));
+  if (!r || !u || !rt || !yh || !tz || !tau || !sig || !gp || !g || !gpp) {
+    free(r); free(u); free(rt); free(yh); free(tz); free(tau); free(sig);
+    free(gp); free(g); free(gpp);
+    return OR_ERR_ALLOC;
+  }
+#define RR(j) (r + (size_t)(j) * ln)
+#define UU(j) (u + (size_t)(j) * ln)
+#define TAU(i, j) tau[(size_t)(i) * (l + 1) + (j)]
+  memcpy(RR(0), b, (size_t)n * sizeof(double));
+  or_spmv('N', -1.0, a, x, 1.0, RR(0));
+  memcpy(rt, RR(0), (size_t)n * sizeof(double));
+  const double bn2 = sqrt(dot(n, b, b));
+  double rn2 = sqrt(dot(n, RR(0), RR(0)));
+  *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+  double rho0 = 1.0, alpha = 0.0, omega = 1.0;
+  int32_t it = 0, brk = 0;
+  while (*err > eps && it < itmax && !brk) {
+    rho0 = -omega * rho0;
+    for (int32_t j = 0; j < l && !brk; j++) { /* BiCG part */
+      const double rho1 = dot(n, RR(j), rt);
+      if (rho0 == 0.0) { brk = 1; break; }
+      const double beta = alpha * rho1 / rho0;
+      rho0 = rho1;
+      for (int32_t i = 0; i <= j; i++)
+        for (int32_t e = 0; e < n; e++) UU(i)[e] = RR(i)[e] - beta * UU(i)[e];
+      or_apply('N', 1.0, w, z, d, UU(j), 0.0, tz);
+      or_spmv('N', 1.0, a, tz, 0.0, UU(j + 1));
+      const double gam = dot(n, UU(j + 1), rt);
+      if (gam == 0.0) { brk = 1; break; }
+      alpha = rho0 / gam;
+      for (int32_t i = 0; i <= j; i++)
+        for (int32_t e = 0; e < n; e++)
+          RR(i)[e] = RR(i)[e] - alpha * UU(i + 1)[e];
+      or_apply('N', 1.0, w, z, d, RR(j), 0.0, tz);
+      or_spmv('N', 1.0, a, tz, 0.0, RR(j + 1));
+      for (int32_t e = 0; e < n; e++) yh[e] = yh[e] + alpha * UU(0)[e];
+    }
+    if (brk) break;
+    for (int32_t j = 1; j <= l; j++) { /* MR part */
+      for (int32_t i = 1; i < j; i++) {
+        TAU(i, j) = dot(n, RR(j), RR(i)) / sig[i];
+        for (int32_t e = 0; e < n; e++)
+          RR(j)[e] = RR(j)[e] - TAU(i, j) * RR(i)[e];
+      }
+      sig[j] = dot(n, RR(j), RR(j));
+      if (sig[j] == 0.0) { brk = 1; break; }
+      gp[j] = dot(n, RR(0), RR(j)) / sig[j];
+    }
+    if (brk) break;
+    g[l] = gp[l];
+    omega = g[l];
+    for (int32_t j = l - 1; j >= 1; j--) {
+      double t = gp[j];
+      for (int32_t i = j + 1; i <= l; i++) t = t - TAU(j, i) * g[i];
+      g[j] = t;
+    }
+    for (int32_t j = 1; j <= l - 1; j++) {
+      double t = g[j + 1];
+      for (int32_t i = j + 1; i <= l - 1; i++) t = t + TAU(j, i) * g[i + 1];
+      gpp[j] = t;
+    }
+    for (int32_t e = 0; e < n; e++) {
+      double yv = yh[e] + g[1] * RR(0)[e];
+      double rv = RR(0)[e] - gp[l] * RR(l)[e];
+      double uv = UU(0)[e] - g[l] * UU(l)[e];
+      for (int32_t j = 1; j <= l - 1; j++) {
+        uv = uv - g[j] * UU(j)[e];
+        yv = yv + gpp[j] * RR(j)[e];
+        rv = rv - gp[j] * RR(j)[e];
+      }
+      yh[e] = yv;
+      RR(0)[e] = rv;
+      UU(0)[e] = uv;
+    }
+    it += l;
+    rn2 = sqrt(dot(n, RR(0), RR(0)));
+    *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
+  }
+  /* x = x0 + M yhat */
+  or_apply('N', 1.0, w, z, d, yh, 0.0, tz);
+  for (int32_t e = 0; e < n; e++) x[e] = x[e] + tz[e];
+#undef RR
+#undef UU
+#undef TAU
+  *iter = brk ? -1 : it;
+  free(r); free(u); free(rt); free(yh); free(tz); free(tau); free(sig);
+  free(gp); free(g); free(gpp);
+  return OR_OK;
+}
+
 /* Preconditioned CG (Hestenes-Stiefel) in the PSBLAS psb_dcg arrangement. */
 int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
           const double *b, double *x, double eps, int32_t itmax,

@@ -135,6 +135,14 @@ int or_cgs(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
 int or_gmres(const or_mat *a, const or_mat *w, const or_mat *z,
              const double *d, const double *b, double *x, double eps,
              int32_t itmax, int32_t irst, int32_t *iter, double *err);
+/* BiCGStab(l) (Sleijpen and Fokkema, ETNA 1 (1993) 11-32, alg. 3.1) with
+ * right preconditioning (operator A M, x = x0 + M yhat at exit); psb_krylov's
+ * 'BICGSTABL' with l = irst (tests/pdegen/runs/mld3d.inp:1, :10).  A cycle
+ * is l BiCG steps then the minimal-residual step by modified Gram-Schmidt;
+ * it counts l iterations and the istop = 2 test follows each cycle. */
+int or_bicgstabl(const or_mat *a, const or_mat *w, const or_mat *z,
+                 const double *d, const double *b, double *x, double eps,
+                 int32_t itmax, int32_t l, int32_t *iter, double *err);
 /* BiCG, preconditioned as in Barrett et al., Templates (1994), fig. 2.8
  * (M^T = the trans 'T' apply), with the istop = 2 test. */
 int or_bicg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,

@@ -73,6 +73,10 @@ def lib():
             "or_gmres": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
                          P(C.c_double), P(C.c_double), C.c_double, C.c_int32,
                          C.c_int32, P(C.c_int32), P(C.c_double)],
+            "or_bicgstabl": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
+                             P(C.c_double), P(C.c_double), C.c_double,
+                             C.c_int32, C.c_int32, P(C.c_int32),
+                             P(C.c_double)],
             "or_cg": [P(OrMat), P(OrMat), P(OrMat), P(C.c_double),
                       P(C.c_double), P(C.c_double), C.c_double, C.c_int32,
                       P(C.c_int32), P(C.c_double)],
@@ -259,7 +263,7 @@ def wrap(a):
 
 def krylov(method, a, w, z, d, b, eps, itmax=1000, x0=None, irst=10):
     """Returns (x, iterations, last ||r||/||b||).  irst: restart of
-    RGMRES."""
+    RGMRES, l of BICGSTABL."""
     A, W, Z = wrap(a), wrap(w), wrap(z)
     d = np.ascontiguousarray(d, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
@@ -267,10 +271,10 @@ def krylov(method, a, w, z, d, b, eps, itmax=1000, x0=None, irst=10):
     it = C.c_int32(0)
     err = C.c_double(0.0)
     meth = method.upper()
-    if meth == "RGMRES":
-        _check(lib().or_gmres(A.ref, W.ref, Z.ref, _dp(d), _dp(b), _dp(x),
-                              eps, itmax, irst, C.byref(it), C.byref(err)),
-               method)
+    if meth in ("RGMRES", "BICGSTABL"):
+        f = lib().or_gmres if meth == "RGMRES" else lib().or_bicgstabl
+        _check(f(A.ref, W.ref, Z.ref, _dp(d), _dp(b), _dp(x), eps, itmax,
+                 irst, C.byref(it), C.byref(err)), method)
         return x, it.value, err.value
     f = {"BICGSTAB": lib().or_bicgstab, "CG": lib().or_cg,
          "CGS": lib().or_cgs, "BICG": lib().or_bicg}[meth]

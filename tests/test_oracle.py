@@ -450,6 +450,76 @@ def test_gmres_finite_termination_and_restart(O):
         assert np.linalg.norm(b - a @ xx) <= 1e-7 * np.linalg.norm(b)
 
 
+def _bicgstabl_lstsq(A, M, b, l, cycles):
+    """BiCGStab(l) on B = A M with the minimal-residual step solved by
+    least squares (min ||r_0 - [r_1 .. r_l] c||), not by the Gram-Schmidt
+    gamma / gamma' / gamma'' recurrences; returns (x, ||r||) per cycle."""
+    B = A @ M
+    r = [b.copy()] + [np.zeros_like(b) for _ in range(l)]
+    u = [np.zeros_like(b) for _ in range(l + 1)]
+    rt = b.copy()
+    y = np.zeros_like(b)
+    rho0, alpha, omega = 1.0, 0.0, 1.0
+    out = []
+    for _ in range(cycles):
+        rho0 = -omega * rho0
+        for j in range(l):
+            rho1 = r[j] @ rt
+            beta = alpha * rho1 / rho0
+            rho0 = rho1
+            for i in range(j + 1):
+                u[i] = r[i] - beta * u[i]
+            u[j + 1] = B @ u[j]
+            alpha = rho0 / (u[j + 1] @ rt)
+            for i in range(j + 1):
+                r[i] = r[i] - alpha * u[i + 1]
+            r[j + 1] = B @ r[j]
+            y = y + alpha * u[0]
+        R = np.stack(r[1:], axis=1)
+        c, *_ = np.linalg.lstsq(R, r[0], rcond=None)
+        y = y + sum(c[j - 1] * r[j - 1] for j in range(1, l + 1))
+        u[0] = u[0] - sum(c[j - 1] * u[j] for j in range(1, l + 1))
+        r[0] = r[0] - R @ c
+        omega = c[l - 1]
+        out.append((M @ y, np.linalg.norm(r[0])))
+    return out
+
+
+@pytest.mark.parametrize("l", [2, 4])
+def test_bicgstabl_against_least_squares_form(O, l):
+    """The oracle's Gram-Schmidt minimal-residual step equals the least
+    squares one of an independent dense restatement, cycle by cycle."""
+    a = R.pde3d_numpy(5)
+    n = a.shape[0]
+    w, z, d = O.invk_bld(a, 1, 1)
+    M = z.toarray() @ np.diag(d) @ w.toarray()
+    b = np.random.default_rng(10).standard_normal(n)
+    ref = _bicgstabl_lstsq(a.toarray(), M, b, l, 3)
+    for cyc in range(1, 4):
+        x, it, err = O.krylov("BICGSTABL", a, w, z, d, b, 1e-30, cyc * l,
+                              irst=l)
+        xr, rn = ref[cyc - 1]
+        assert it == cyc * l
+        assert err * np.linalg.norm(b) == pytest.approx(rn, rel=1e-7)
+        assert np.linalg.norm(x - xr) <= 1e-7 * np.linalg.norm(xr)
+
+
+def test_bicgstabl_one_is_bicgstab_and_converges(O):
+    """BiCGStab(1) is BiCGStab (same iterations +-1, same solution to the
+    tolerance); l = 2, 4, 8 converge to true residuals at the tolerance."""
+    a = R.pde3d_numpy(7)
+    n = a.shape[0]
+    for w, z, d in (O.invk_bld(a, 1, 1), O.ainv_bld(a, "LLK", 8, 0.1)):
+        b = a @ np.ones(n)
+        x1, it1, _ = O.krylov("BICGSTABL", a, w, z, d, b, 1e-9, irst=1)
+        xs, its, _ = O.krylov("BICGSTAB", a, w, z, d, b, 1e-9)
+        assert abs(it1 - its) <= 1, (it1, its)
+        for l in (1, 2, 4, 8):
+            x, it, err = O.krylov("BICGSTABL", a, w, z, d, b, 1e-9, irst=l)
+            assert it > 0 and it % l == 0 and err <= 1e-9
+            assert np.linalg.norm(b - a @ x) <= 1e-8 * np.linalg.norm(b)
+
+
 @pytest.mark.parametrize("method", ["BICGSTAB", "CG"])
 def test_threaded_loops_same_solve(O, method):
     """The row-parallel loops (CPU baseline on all host cores) change only
