@@ -205,14 +205,17 @@ int ainv_mat_free(ainv_mat_h a);
 int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
                   double *y, int k, int64_t ldx, int64_t ldy, void *stream);
 
-/* method: "BICGSTAB", "CG", "CGS", "BICG" or "RGMRES" (psb_krylov's names,
- * the methods of tests/pdegen/runs/mld3d.inp:1; CGS and BiCG in the
- * preconditioned forms of Barrett et al., Templates, figs. 2.9 and 2.8;
- * BiCG applies A^T and M^T through transposed layouts built on its first
- * call; RGMRES is right-preconditioned GMRES restarted every irst steps,
- * irst <= 0 meaning 10, as psb_krylov's irst argument,
- * tests/fileread/df_sample.f90:368-369; one RGMRES iteration is one
- * Arnoldi step and its in-cycle test uses the rotated residual); istop = 2
+/* method: "BICGSTAB", "CG", "CGS", "BICG", "RGMRES" or "BICGSTABL"
+ * (psb_krylov's names, the methods of tests/pdegen/runs/mld3d.inp:1; CGS
+ * and BiCG in the preconditioned forms of Barrett et al., Templates, figs.
+ * 2.9 and 2.8; BiCG applies A^T and M^T through transposed layouts built on
+ * its first call; RGMRES is right-preconditioned GMRES restarted every irst
+ * steps, one iteration per Arnoldi step, its in-cycle test on the rotated
+ * residual; BICGSTABL is right-preconditioned BiCGStab(l) of Sleijpen and
+ * Fokkema with l = irst, counting l iterations per cycle and testing after
+ * each cycle; irst as psb_krylov's argument,
+ * tests/fileread/df_sample.f90:368-369, <= 0 meaning 10 for RGMRES and 2
+ * for BICGSTABL); istop = 2
  * (||r||_2 <= eps ||b||_2); x holds
  * x0 on entry.  k interleaved right-hand sides, each with its own
  * recurrence and stopping test; iter[c] and err[c] (host arrays of k) get

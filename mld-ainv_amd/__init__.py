@@ -514,10 +514,11 @@ KRYLOV_EAGER = 2
 
 
 def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2,
-           eager=False, irst=10):
+           eager=False, irst=0):
     """psb_krylov(method, a, prec, b, x, eps, ...) for k interleaved
     right-hand sides.  a: SystemMatrix; prec: solver object or None;
-    b, x: numpy (host) or DeviceArray; irst: restart of RGMRES.  eager=True
+    b, x: numpy (host) or DeviceArray; irst: restart of RGMRES / l of
+    BICGSTABL (0: 10 / 2).  eager=True
     launches the kernels of each iteration one by one instead of replaying
     the captured graphs.  Returns (x, iters[k], errs[k])."""
     host = not isinstance(b, DeviceArray)

@@ -515,7 +515,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   p->n = a->n;
   const int64_t nk = (int64_t)a->n * p->K;
   // x, b + recurrence vectors (RGMRES: x, b, w, z and the basis apart)
-  const int nvec = (method == KM_GMRES || cg) ? 6 : 10;
+  const int nvec = (method == KM_GMRES || method == KM_BICGSTABL || cg) ? 6 : 10;
   p->work.alloc((size_t)nk * nvec * sizeof(double));
   // padding columns (k..K) are read by the row-mapped kernels: keep them 0
   AINV_HIP_CHECK(hipMemsetAsync(p->work.p, 0, p->work.bytes, s));
@@ -526,7 +526,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   p->v.p = w0 + 3 * nk;
   p->v.z = w0 + 4 * nk;
   p->v.q = w0 + 5 * nk;  // CG: A p ; BiCGSTAB: rtilde
-  if (!cg && method != KM_GMRES) {
+  if (!cg && method != KM_GMRES && method != KM_BICGSTABL) {
     p->v.v = w0 + 6 * nk;
     p->v.f = w0 + 7 * nk;
     p->v.s = w0 + 8 * nk;
@@ -746,8 +746,10 @@ int krylov_method_id(const std::string &name) {
   if (name == "CGS") return KM_CGS;
   if (name == "BICG") return KM_BICG;
   if (name == "RGMRES" || name == "GMRES") return KM_GMRES;
+  if (name == "BICGSTABL") return KM_BICGSTABL;
   throw Error(AINV_ERR_ARG, "unknown Krylov method " + name +
-                                " (BICGSTAB, CG, CGS, BICG, RGMRES available)");
+                                " (BICGSTAB, CG, CGS, BICG, RGMRES, BICGSTABL "
+                                "available)");
 }
 
 // ---------------------------------------------------------- slot batches
@@ -763,10 +765,10 @@ struct SlotBatch {
 SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s) {
   if (K < 1 || K > kMaxCols) throw Error(AINV_ERR_ARG, "slots must be 1..64");
-  if (method == KM_GMRES)
+  if (method == KM_GMRES || method == KM_BICGSTABL)
     throw Error(AINV_ERR_ARG,
-                "the served form runs BICGSTAB, CG, CGS and BICG (RGMRES "
-                "restarts in cycles)");
+                "the served form runs BICGSTAB, CG, CGS and BICG (RGMRES and "
+                "BICGSTABL run in cycles of several steps)");
   if (m && m->n != a->n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
   KrylovPlan *p = get_plan(a, method, m, K, s);
@@ -907,6 +909,9 @@ void batch_extract(SlotBatch *b, int c0, int kc, double *dst) {
 void gmres_solve(ainv_matrix *a, ainv_solver *m, const double *b, double *x,
                  int k, double eps, int itmax, int irst, int32_t *iter,
                  double *err, hipStream_t s, int flags);
+void bicgstabl_solve(ainv_matrix *a, ainv_solver *m, const double *b,
+                     double *x, int k, double eps, int itmax, int l,
+                     int32_t *iter, double *err, hipStream_t s, int flags);
 
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                   const double *b, double *x, int k, double eps, int itmax,
@@ -919,6 +924,10 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   const int mid = krylov_method_id(method);
   if (mid == KM_GMRES) {
     gmres_solve(a, m, b, x, k, eps, itmax, irst, iter, err, s, flags);
+    return;
+  }
+  if (mid == KM_BICGSTABL) {
+    bicgstabl_solve(a, m, b, x, k, eps, itmax, irst, iter, err, s, flags);
     return;
   }
   const bool cg = mid == KM_CG;
@@ -1421,6 +1430,460 @@ void gmres_solve(ainv_matrix *a, ainv_solver *m, const double *b, double *x,
   AINV_HIP_CHECK(hipStreamSynchronize(s));
   for (int c = 0; c < k; ++c) {
     iter[c] = hc[c].iter;
+    err[c] = hc[c].err;
+  }
+}
+
+// --------------------------------------------------------------- BiCGStab(l)
+// Sleijpen and Fokkema, ETNA 1 (1993) 11-32, alg. 3.1, on the right-
+// preconditioned operator A M (x = x0 + M yhat at exit), psb_krylov's
+// 'BICGSTABL' with l = irst (tests/pdegen/runs/mld3d.inp:1, :10), for k
+// interleaved columns.  rhat_0..l and uhat_0..l are 2 (l + 1) blocks of
+// n x K; a whole cycle (l BiCG steps, the Gram-Schmidt minimal-residual
+// step, the update and the istop = 2 test) has a fixed structure and is one
+// captured graph.  The dots of each A M product ride on it
+// (hll_spmv_dot), every Gram-Schmidt update carries the next dot.
+namespace {
+
+struct BlCol {
+  double bn, err, eps, rho0, rho_next, alpha, omega, beta;
+  int32_t active, iter, brk, pad;
+};
+
+enum BlOp {
+  BL_INIT,   // rt = r0 ; yhat = 0 ; u0 = 0 ; dots (b,b) (r0,r0) (r0,rt)
+  BL_U,      // u_i = r_i - beta u_i, i <= j
+  BL_R,      // r_i -= alpha u_{i+1}, i <= j ; yhat += alpha u_0
+  BL_DOT,    // (r_j, r_i)
+  BL_MGS,    // r_j -= tau_ij r_i ; (r_j, r_{i+1}) or, at i + 1 == j,
+             // (r_j, r_j) (r_0, r_j)
+  BL_SIG1,   // (r_1, r_1) (r_0, r_1)
+  BL_FINAL,  // yhat, r_0, u_0 updates ; (r_0, r_0) (r_0, rt)
+  BL_XUPD,   // x += z
+};
+
+struct BlArgs {
+  double *x, *b, *rt, *y, *z, *r, *u;  // r, u: blocks 0 of rhat, uhat
+  int64_t vs;
+  const BlCol *col;
+  const double *arr;
+  int astr, l, i, j;
+};
+
+// per-column arrays: tau (l+1) x (l+1) row-major, sig, gp, g, gpp (l+1 each)
+__host__ __device__ inline int bl_tau(int l, int i, int j) { return i * (l + 1) + j; }
+__host__ __device__ inline int bl_sig(int l) { return (l + 1) * (l + 1); }
+__host__ __device__ inline int bl_gp(int l) { return (l + 1) * (l + 2); }
+__host__ __device__ inline int bl_g(int l) { return (l + 1) * (l + 3); }
+__host__ __device__ inline int bl_gpp(int l) { return (l + 1) * (l + 4); }
+inline int bl_stride(int l) { return ((l + 1) * (l + 5) + 7) & ~7; }
+
+template <int OP> struct BlPairs { static constexpr int v = 0; };
+template <> struct BlPairs<BL_INIT> { static constexpr int v = 3; };
+template <> struct BlPairs<BL_DOT> { static constexpr int v = 1; };
+template <> struct BlPairs<BL_MGS> { static constexpr int v = 2; };
+template <> struct BlPairs<BL_SIG1> { static constexpr int v = 2; };
+template <> struct BlPairs<BL_FINAL> { static constexpr int v = 2; };
+
+template <int OP, int K>
+__global__ __launch_bounds__(kThreads) void k_bl_vec(int32_t n, int k, BlArgs g,
+                                                     double *__restrict__ part) {
+  constexpr int NP = BlPairs<OP>::v;
+  constexpr int R = kThreads / K;
+  const int c = threadIdx.x % K;
+  const int rr = threadIdx.x / K;
+  bool cok = c < k;
+  BlCol cs{};
+  if (cok) {
+    cs = g.col[c];
+    if constexpr (OP != BL_INIT && OP != BL_XUPD) cok = cs.active != 0;
+  }
+  double acc[NP > 0 ? NP : 1];
+#pragma unroll
+  for (int q = 0; q < (NP > 0 ? NP : 1); ++q) acc[q] = 0.0;
+  if (cok) {
+    const double *ar = g.arr + (int64_t)c * g.astr;
+    const int l = g.l, i0 = g.i, j0 = g.j;
+    double tau = 0.0;
+    if constexpr (OP == BL_MGS) tau = ar[bl_tau(l, i0, j0)];
+    for (int64_t r = (int64_t)blockIdx.x * R + rr; r < n;
+         r += (int64_t)gridDim.x * R) {
+      const int64_t e = r * K + c;
+      if constexpr (OP == BL_INIT) {
+        const double r0 = g.r[e];
+        g.rt[e] = r0;
+        g.y[e] = 0.0;
+        g.u[e] = 0.0;
+        acc[0] += g.b[e] * g.b[e];
+        acc[1] += r0 * r0;
+        acc[2] += r0 * r0;
+      } else if constexpr (OP == BL_U) {
+        for (int i = 0; i <= j0; ++i) {
+          const int64_t o = (int64_t)i * g.vs + e;
+          g.u[o] = g.r[o] - cs.beta * g.u[o];
+        }
+      } else if constexpr (OP == BL_R) {
+        for (int i = 0; i <= j0; ++i) {
+          const int64_t o = (int64_t)i * g.vs + e;
+          g.r[o] = g.r[o] - cs.alpha * g.u[o + g.vs];
+        }
+        g.y[e] = g.y[e] + cs.alpha * g.u[e];
+      } else if constexpr (OP == BL_DOT) {
+        acc[0] += g.r[(int64_t)j0 * g.vs + e] * g.r[(int64_t)i0 * g.vs + e];
+      } else if constexpr (OP == BL_MGS) {
+        const int64_t oj = (int64_t)j0 * g.vs + e;
+        const double rj = g.r[oj] - tau * g.r[(int64_t)i0 * g.vs + e];
+        g.r[oj] = rj;
+        if (i0 + 1 < j0) {
+          acc[0] += rj * g.r[(int64_t)(i0 + 1) * g.vs + e];
+        } else {
+          acc[0] += rj * rj;
+          acc[1] += g.r[e] * rj;
+        }
+      } else if constexpr (OP == BL_SIG1) {
+        const double r1 = g.r[g.vs + e];
+        acc[0] += r1 * r1;
+        acc[1] += g.r[e] * r1;
+      } else if constexpr (OP == BL_FINAL) {
+        const double *gp = ar + bl_gp(l), *gg = ar + bl_g(l),
+                     *gpp = ar + bl_gpp(l);
+        double yv = g.y[e] + gg[1] * g.r[e];
+        double rv = g.r[e] - gp[l] * g.r[(int64_t)l * g.vs + e];
+        double uv = g.u[e] - gg[l] * g.u[(int64_t)l * g.vs + e];
+        for (int jj = 1; jj <= l - 1; ++jj) {
+          const int64_t o = (int64_t)jj * g.vs + e;
+          uv = uv - gg[jj] * g.u[o];
+          yv = yv + gpp[jj] * g.r[o];
+          rv = rv - gp[jj] * g.r[o];
+        }
+        g.y[e] = yv;
+        g.r[e] = rv;
+        g.u[e] = uv;
+        acc[0] += rv * rv;
+        acc[1] += rv * g.rt[e];
+      } else if constexpr (OP == BL_XUPD) {
+        g.x[e] = g.x[e] + g.z[e];
+      }
+    }
+  }
+  if constexpr (NP > 0) {
+    __shared__ double red[NP][kThreads];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int st = R / 2; st >= 1; st >>= 1) {
+      if (rr < st) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+          red[q][threadIdx.x] += red[q][threadIdx.x + st * K];
+      }
+      __syncthreads();
+    }
+    if (rr == 0 && c < k) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        part[((int64_t)q * kRedBlocks + blockIdx.x) * kMaxCols + c] =
+            red[q][threadIdx.x];
+    }
+  }
+}
+
+template <int OP>
+int bl_vec(int32_t n, int k, int K, const BlArgs &g, double *part,
+           hipStream_t s) {
+  const int nb = vec_blocks(n, k);
+#define AINV_BL(KK)                                                         \
+  hipLaunchKernelGGL((k_bl_vec<OP, KK>), dim3(nb), dim3(kThreads), 0, s, n, \
+                     k, g, part)
+  switch (K) {
+    case 1: AINV_BL(1); break;
+    case 2: AINV_BL(2); break;
+    case 4: AINV_BL(4); break;
+    case 8: AINV_BL(8); break;
+    case 16: AINV_BL(16); break;
+    case 32: AINV_BL(32); break;
+    default: AINV_BL(64); break;
+  }
+#undef AINV_BL
+  AINV_HIP_CHECK(hipGetLastError());
+  return nb;
+}
+
+enum BlFin {
+  FB_INIT,   // bn, err, active; rho0 = 1, alpha = 0, omega = 1
+  FB_BETA,   // (j == 0: rho0 = -omega rho0) rho1, beta, rho0 = rho1
+  FB_ALPHA,  // alpha = rho0 / (u_{j+1}, rt)
+  FB_TAU,    // tau_ij = (r_j, r_i) / sig_i
+  FB_SIG,    // sig_j, gamma'_j
+  FB_GAMMA,  // gamma, gamma'', omega
+  FB_CONV,   // iter += l, err, convergence, rho for the next cycle
+};
+
+__global__ __launch_bounds__(kThreads) void k_bl_fin(
+    int fin, int i, int j, int nblk, const double *__restrict__ part,
+    BlCol *__restrict__ col, double *__restrict__ arr, int astr, int l,
+    int32_t *__restrict__ nactive) {
+  const int c = blockIdx.x;
+  int np = 1;
+  if (fin == FB_INIT) np = 3;
+  else if (fin == FB_SIG || fin == FB_CONV) np = 2;
+  else if (fin == FB_GAMMA || (fin == FB_BETA && j == 0)) np = 0;
+  __shared__ double red[3][kThreads];
+  for (int q = 0; q < np; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kThreads)
+      s += part[((int64_t)q * kRedBlocks + b) * kMaxCols + c];
+    red[q][threadIdx.x] = s;
+  }
+  __syncthreads();
+  for (int st = kThreads / 2; st >= 1; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int q = 0; q < np; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (fin == FB_BETA && j == 0 && c == 0) *nactive = 0;
+  BlCol cs = col[c];
+  double *ar = arr + (int64_t)c * astr;
+  const double s0 = np > 0 ? red[0][0] : 0.0, s1 = np > 1 ? red[1][0] : 0.0,
+               s2 = np > 2 ? red[2][0] : 0.0;
+  if (fin == FB_INIT) {
+    cs.bn = sqrt(s0);
+    const double rn = sqrt(s1);
+    cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
+    cs.active = !(cs.err <= cs.eps);
+    cs.iter = 0;
+    cs.brk = 0;
+    cs.rho0 = 1.0;
+    cs.alpha = 0.0;
+    cs.omega = 1.0;
+    cs.beta = 0.0;
+    cs.rho_next = s2;
+  } else if (cs.active) {
+    double *sig = ar + bl_sig(l), *gp = ar + bl_gp(l), *gg = ar + bl_g(l),
+           *gpp = ar + bl_gpp(l);
+    switch (fin) {
+      case FB_BETA: {
+        if (j == 0) cs.rho0 = -cs.omega * cs.rho0;
+        const double rho1 = j == 0 ? cs.rho_next : s0;
+        if (cs.rho0 == 0.0) {
+          cs.active = 0;
+          cs.brk = 1;
+          break;
+        }
+        cs.beta = cs.alpha * rho1 / cs.rho0;
+        cs.rho0 = rho1;
+        break;
+      }
+      case FB_ALPHA:
+        if (s0 == 0.0) {
+          cs.active = 0;
+          cs.brk = 1;
+        } else {
+          cs.alpha = cs.rho0 / s0;
+        }
+        break;
+      case FB_TAU:
+        ar[bl_tau(l, i, j)] = s0 / sig[i];
+        break;
+      case FB_SIG:
+        sig[j] = s0;
+        if (s0 == 0.0) {
+          cs.active = 0;
+          cs.brk = 1;
+        } else {
+          gp[j] = s1 / s0;
+        }
+        break;
+      case FB_GAMMA: {
+        gg[l] = gp[l];
+        cs.omega = gg[l];
+        for (int jj = l - 1; jj >= 1; --jj) {
+          double t = gp[jj];
+          for (int ii = jj + 1; ii <= l; ++ii) t = t - ar[bl_tau(l, jj, ii)] * gg[ii];
+          gg[jj] = t;
+        }
+        for (int jj = 1; jj <= l - 1; ++jj) {
+          double t = gg[jj + 1];
+          for (int ii = jj + 1; ii <= l - 1; ++ii)
+            t = t + ar[bl_tau(l, jj, ii)] * gg[ii + 1];
+          gpp[jj] = t;
+        }
+        break;
+      }
+      case FB_CONV: {
+        cs.iter += l;
+        const double rn = sqrt(s0);
+        cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
+        cs.rho_next = s1;
+        if (cs.err <= cs.eps) cs.active = 0;
+        break;
+      }
+    }
+  }
+  col[c] = cs;
+  if ((fin == FB_INIT || fin == FB_CONV) && cs.active) atomicAdd(nactive, 1);
+}
+
+void bl_fin(const KrylovPlan *p, int which, int i, int j, int nblk,
+            hipStream_t s) {
+  hipLaunchKernelGGL(k_bl_fin, dim3(p->k), dim3(kThreads), 0, s, which, i, j,
+                     nblk, p->pt, p->gmcol.as<BlCol>(), p->gmarr.as<double>(),
+                     bl_stride(p->irst), p->irst, p->na);
+  AINV_HIP_CHECK(hipGetLastError());
+}
+
+BlArgs bl_args(const KrylovPlan *p, int i, int j) {
+  BlArgs g{};
+  g.x = p->v.x;
+  g.b = p->v.b;
+  g.rt = p->v.q;
+  g.y = p->v.p;
+  g.z = p->v.z;
+  const int64_t vs = (int64_t)p->n * p->K;
+  g.r = p->basis.as<double>();
+  g.u = g.r + (int64_t)(p->irst + 1) * vs;
+  g.vs = vs;
+  g.col = p->gmcol.as<BlCol>();
+  g.arr = p->gmarr.as<double>();
+  g.astr = bl_stride(p->irst);
+  g.l = p->irst;
+  g.i = i;
+  g.j = j;
+  return g;
+}
+
+void bl_cycle(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
+  const int32_t n = p->n;
+  const int k = p->k, K = p->K, l = p->irst, sm = p->spmv_map;
+  const BlArgs g0 = bl_args(p, 0, 0);
+  const int64_t vs = g0.vs;
+  int g = 0;
+  for (int j = 0; j < l; ++j) {  // BiCG part
+    bl_fin(p, FB_BETA, 0, j, g, s);
+    bl_vec<BL_U>(n, k, K, bl_args(p, 0, j), p->pt, s);
+    precond(p, g0.u + j * vs, p->v.z, s);
+    g = hll_spmv_dot(a->hll, p->v.z, g0.u + (j + 1) * vs, k, K, g0.rt, 1,
+                     p->pt, sm, s);
+    bl_fin(p, FB_ALPHA, 0, j, g, s);
+    bl_vec<BL_R>(n, k, K, bl_args(p, 0, j), p->pt, s);
+    precond(p, g0.r + j * vs, p->v.z, s);
+    g = hll_spmv_dot(a->hll, p->v.z, g0.r + (j + 1) * vs, k, K, g0.rt, 1,
+                     p->pt, sm, s);
+  }
+  for (int j = 1; j <= l; ++j) {  // minimal-residual part
+    if (j == 1) {
+      g = bl_vec<BL_SIG1>(n, k, K, g0, p->pt, s);
+      bl_fin(p, FB_SIG, 0, 1, g, s);
+      continue;
+    }
+    g = bl_vec<BL_DOT>(n, k, K, bl_args(p, 1, j), p->pt, s);
+    bl_fin(p, FB_TAU, 1, j, g, s);
+    for (int i = 1; i < j; ++i) {
+      g = bl_vec<BL_MGS>(n, k, K, bl_args(p, i, j), p->pt, s);
+      if (i + 1 < j)
+        bl_fin(p, FB_TAU, i + 1, j, g, s);
+      else
+        bl_fin(p, FB_SIG, 0, j, g, s);
+    }
+  }
+  bl_fin(p, FB_GAMMA, 0, 0, 0, s);
+  g = bl_vec<BL_FINAL>(n, k, K, g0, p->pt, s);
+  bl_fin(p, FB_CONV, 0, 0, g, s);
+}
+
+}  // namespace
+
+void bicgstabl_solve(ainv_matrix *a, ainv_solver *m, const double *b,
+                     double *x, int k, double eps, int itmax, int l,
+                     int32_t *iter, double *err, hipStream_t s, int flags) {
+  const int32_t n = a->n;
+  const int ll = l <= 0 ? 2 : l;
+  if (ll > 16) throw Error(AINV_ERR_ARG, "BICGSTABL: l must be <= 16");
+  KrylovPlan *p = get_plan(a, KM_BICGSTABL, m, k, s, ll);
+  const int K = p->K;
+  const int64_t vs = (int64_t)n * K;
+  if (!p->basis.p) {
+    p->basis.alloc((size_t)2 * (ll + 1) * vs * sizeof(double));
+    // padding columns are read by the row-mapped kernels: keep them 0
+    AINV_HIP_CHECK(hipMemsetAsync(p->basis.p, 0, p->basis.bytes, s));
+    p->gmcol.alloc(sizeof(BlCol) * kMaxCols);
+    p->gmarr.alloc(sizeof(double) * (size_t)bl_stride(ll) * kMaxCols);
+  }
+  const unsigned gp = (unsigned)((vs + 255) / 256);
+  hipLaunchKernelGGL(k_pack, dim3(gp), dim3(256), 0, s, n, k, K, b, p->v.b);
+  hipLaunchKernelGGL(k_pack, dim3(gp), dim3(256), 0, s, n, k, K, x, p->v.x);
+  AINV_HIP_CHECK(hipGetLastError());
+  std::vector<BlCol> init(kMaxCols);
+  for (auto &c0 : init) {
+    c0 = BlCol{};
+    c0.eps = eps;
+  }
+  AINV_HIP_CHECK(hipMemcpyAsync(p->gmcol.p, init.data(),
+                                sizeof(BlCol) * kMaxCols,
+                                hipMemcpyHostToDevice, s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  tune_plan(p, a, m, s);
+  const BlArgs g0 = bl_args(p, 0, 0);
+  // r0 = b - A x ; rt = r0 ; yhat = 0 ; uhat_0 = 0
+  AINV_HIP_CHECK(hipMemcpyAsync(g0.r, p->v.b, (size_t)vs * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+  hll_spmm_axpby(a->hll, -1.0, p->v.x, false, K, 1.0, g0.r, K, k, s,
+                 p->spmv_map);
+  int g = bl_vec<BL_INIT>(n, k, K, g0, p->pt, s);
+  bl_fin(p, FB_INIT, 0, 0, g, s);
+  AINV_HIP_CHECK(hipMemcpyAsync(p->pinned, p->na, sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  const bool eager = (flags & AINV_KRYLOV_EAGER) != 0;
+  if (p->pinned[0] > 0 && itmax > 0 && !eager && p->gm_exec.empty()) {
+    hipGraph_t gr = nullptr;
+    AINV_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      bl_cycle(p, a, s);
+    } catch (...) {
+      hipGraph_t gx = nullptr;
+      (void)hipStreamEndCapture(s, &gx);
+      if (gx) (void)hipGraphDestroy(gx);
+      throw;
+    }
+    AINV_HIP_CHECK(hipStreamEndCapture(s, &gr));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    AINV_HIP_CHECK(e);
+    p->gm_exec.push_back(ex);
+  }
+  if (p->pinned[0] > 0) {
+    // launch cycle c, then look at the count after cycle c - 1
+    int c = 0;
+    for (int it = 0; it < itmax; it += ll, ++c) {
+      if (eager)
+        bl_cycle(p, a, s);
+      else
+        AINV_HIP_CHECK(hipGraphLaunch(p->gm_exec[0], s));
+      const int slot = c & 1;
+      AINV_HIP_CHECK(hipMemcpyAsync(p->pinned + slot, p->na, sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+      AINV_HIP_CHECK(hipEventRecord(p->ev[slot], s));
+      if (c >= 1) {
+        AINV_HIP_CHECK(hipEventSynchronize(p->ev[slot ^ 1]));
+        if (p->pinned[slot ^ 1] == 0) break;
+      }
+    }
+  }
+  // x = x0 + M yhat
+  precond(p, g0.y, p->v.z, s);
+  bl_vec<BL_XUPD>(n, k, K, g0, p->pt, s);
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)(((int64_t)n * k + 255) / 256)),
+                     dim3(256), 0, s, n, k, K, p->v.x, x);
+  AINV_HIP_CHECK(hipGetLastError());
+  std::vector<BlCol> hc(k);
+  AINV_HIP_CHECK(hipMemcpyAsync(hc.data(), p->gmcol.p, sizeof(BlCol) * k,
+                                hipMemcpyDeviceToHost, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < k; ++c) {
+    iter[c] = hc[c].brk ? -1 : hc[c].iter;
     err[c] = hc[c].err;
   }
 }

@@ -113,7 +113,14 @@ struct SlotState {
   double err = 0.0;
 };
 // Krylov methods of the engine (psb_krylov's names)
-enum { KM_BICGSTAB = 0, KM_CG = 1, KM_CGS = 2, KM_BICG = 3, KM_GMRES = 4 };
+enum {
+  KM_BICGSTAB = 0,
+  KM_CG = 1,
+  KM_CGS = 2,
+  KM_BICG = 3,
+  KM_GMRES = 4,
+  KM_BICGSTABL = 5
+};
 int krylov_method_id(const std::string &name);  // throws for others
 SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s);
@@ -143,7 +150,8 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
                    hipStream_t s);
 
 // flags: AINV_KRYLOV_EAGER disables graph capture (used to check replays)
-// irst: restart length of RGMRES (<= 0: 10), ignored by the other methods
+// irst: restart length of RGMRES (<= 0: 10) and l of BICGSTABL (<= 0: 2),
+// ignored by the other methods
 void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
                   const double *b, double *x, int k, double eps, int itmax,
                   int32_t *iter, double *err, hipStream_t s, int flags,

@@ -303,7 +303,8 @@ contains
   end function
 
   ! psb_krylov(method, a, prec, b, x, eps, ...) with istop = 2 on the local
-  ! block (1-based CSR) and host vectors; irst: restart of RGMRES
+  ! block (1-based CSR) and host vectors; irst: restart of RGMRES, l of
+  ! BICGSTABL (absent: 10 and 2)
   subroutine mld_krylov(method, n, irp, ja, val, prec, b, x, eps, itmax, &
        & iter, err, info, irst)
     character(len=*), intent(in) :: method
@@ -321,7 +322,7 @@ contains
     real(c_double) :: er(1), bc(size(b))
     integer :: i2
     integer(c_int) :: nl
-    nl = 10
+    nl = 0
     if (present(irst)) nl = irst
     a = c_null_ptr
     info = ainv_mat_create(int(n, c_int32_t), irp, ja, val, 1_c_int, a)

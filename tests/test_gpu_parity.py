@@ -353,6 +353,42 @@ def test_krylov_rgmres(gpu, O, kind, k, irst):
             1e-7 * np.linalg.norm(Bc[:, c])
 
 
+@pytest.mark.parametrize("kind,k,l", [("INVK", 1, 1), ("INVK", 5, 2),
+                                      ("AINV", 3, 4), ("NONE", 2, 2),
+                                      ("INVK", 17, 3)])
+def test_krylov_bicgstabl(gpu, O, kind, k, l):
+    """BiCGStab(l) on the GPU: per-column iteration counts within one cycle
+    (+-l) of the oracle's BiCGStab(l) (itself pinned by BiCGStab(1) =
+    BiCGStab and a least-squares restatement), true residuals."""
+    a = gpu.gen_pde3d(12)
+    n = a.shape[0]
+    if kind == "INVK":
+        sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+        w, z, d = O.invk_bld(a, 1, 1)
+    elif kind == "AINV":
+        sv = build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
+                   SUB_ILUTHRS=0.1)
+        w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
+    else:
+        sv = None
+        w = z = sp.identity(n, format="csr")
+        d = np.ones(n)
+    B = gpu.gen_normal(23, (n, k)) if k > 1 else a @ np.ones(n)
+    A = gpu.SystemMatrix(a)
+    X, it, err = gpu.krylov("BICGSTABL", A, sv, B, eps=1e-8, itmax=800,
+                            irst=l)
+    Bc = B.reshape(n, -1)
+    Xc = X.reshape(n, -1)
+    for c in range(k):
+        _, it_ref, _ = O.krylov("BICGSTABL", a, w, z, d, Bc[:, c], 1e-8, 800,
+                                irst=l)
+        assert it[c] > 0 and it[c] % l == 0
+        assert abs(int(it[c]) - it_ref) <= l, (c, it[c], it_ref)
+        assert err[c] <= 1e-8
+        assert np.linalg.norm(Bc[:, c] - a @ Xc[:, c]) <= \
+            1e-7 * np.linalg.norm(Bc[:, c])
+
+
 def test_rgmres_itmax_and_served_form(gpu, O):
     """RGMRES stops at itmax mid-cycle with x updated from the steps taken
     (the oracle's x to rounding); the served form refuses it."""
@@ -375,7 +411,7 @@ def test_rgmres_itmax_and_served_form(gpu, O):
 
 
 @pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS", "BICG",
-                                    "RGMRES"])
+                                    "RGMRES", "BICGSTABL"])
 def test_graph_replays_match_eager(gpu, method):
     """The captured iteration graph, replayed solve after solve with eager
     work and synchronizations in between, gives bit-identical iterates and
