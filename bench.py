@@ -26,10 +26,11 @@ matrices (12 pde3d with N ~ U{20..120}, INVK(1,1); 12 random as C4 with
 n ~ logU[1e4, --c5-nmax], AINV-LLK 5e-2; FP32 factors, BiCGSTAB 1e-8)
 registered with one server; requests of k ~ U{1..64} N(0,1) columns on a
 uniformly drawn matrix.  After one warm-up request per matrix (first-touch
-builds), a closed-loop burst measures the capacity, then requests arrive
-as a Poisson process at 0.8x that capacity; value = right-hand sides solved
-per second in the open-loop phase, latency = submission to completion per
-request.
+builds), a burst (all requests at once) and a closed loop keeping
+--c5-window requests outstanding run; the closed loop's sustained rate is
+the capacity, and requests then arrive as a Poisson process at 0.8x that
+capacity; value = right-hand sides solved per second in the open-loop
+phase, latency = submission to completion per request.
 
 Other configurations (--config): C1 pde3d 40^3 with AINV-LLK (fill 8,
 thresh 0.1), BiCGSTAB 1e-9, b = 1; C3 2-D anisotropic diffusion 2048^2
@@ -115,6 +116,8 @@ def main():
     p.add_argument("--c5-nmax", type=float, default=2e5,
                    help="largest random matrix of the C5 pool")
     p.add_argument("--c5-requests", type=int, default=96)
+    p.add_argument("--c5-window", type=int, default=16,
+                   help="outstanding requests of the closed-loop capacity run")
     p.add_argument("--workers", type=int, default=4)
     p.add_argument("--grid", type=int, default=0,
                    help="override the grid size (C1-C3) or order (C4)")
@@ -386,6 +389,7 @@ def serve_bench(m, args):
         srv.wait(t)
     t_warm = time.time() - t0
     log(f"first-touch builds + one solve per matrix: {t_warm:.1f}s")
+    st_warm = srv.stats()
 
     def requests(count, seed):
         rr = np.random.default_rng(seed)
@@ -401,18 +405,53 @@ def serve_bench(m, args):
         if worst > 1e-8:
             raise SystemExit(f"C5: a request ended at ||r||/||b|| = {worst}")
 
-    # closed-loop burst: capacity
+    # burst: every request submitted at once (the most batching there is)
     burst = requests(max(24, args.c5_requests // 2), 6)
     t0 = time.time()
     tk = [srv.submit(ids[j], b, eps=1e-8, itmax=2000) for j, _, b in burst]
     res = [srv.wait(t) for t in tk]
     t_burst = time.time() - t0
     check(res)
-    cap_req = len(burst) / t_burst
-    cap_cols = sum(k for _, k, _ in burst) / t_burst
-    log(f"capacity: {cap_req:.2f} requests/s, {cap_cols:.1f} columns/s")
+    burst_req = len(burst) / t_burst
+    burst_cols = sum(k for _, k, _ in burst) / t_burst
+    log(f"burst: {burst_req:.2f} requests/s, {burst_cols:.1f} columns/s")
+    st_burst = srv.stats()
 
-    # open loop: Poisson arrivals at 0.8x capacity
+    # capacity: closed loop keeping `window` requests outstanding, each new
+    # one submitted when one completes (sustained throughput)
+    closed = requests(args.c5_requests, 9)
+    window = args.c5_window
+    lock = threading.Lock()
+    nxt = [0]
+    done_res = []
+
+    def client():
+        while True:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= len(closed):
+                return
+            j, _, b = closed[i]
+            r = srv.wait(srv.submit(ids[j], b, eps=1e-8, itmax=2000))
+            with lock:
+                done_res.append(r)
+
+    t0 = time.time()
+    cl = [threading.Thread(target=client) for _ in range(window)]
+    for t in cl:
+        t.start()
+    for t in cl:
+        t.join()
+    t_closed = time.time() - t0
+    check(done_res)
+    cap_req = len(closed) / t_closed
+    cap_cols = sum(k for _, k, _ in closed) / t_closed
+    log(f"capacity (closed loop, {window} outstanding): {cap_req:.2f} "
+        f"requests/s, {cap_cols:.1f} columns/s")
+    st_closed = srv.stats()
+
+    # open loop: Poisson arrivals at 0.8x the sustained capacity
     reqs = requests(args.c5_requests, 7)
     lam = 0.8 * cap_req
     arr = np.cumsum(np.random.default_rng(8).exponential(1.0 / lam,
@@ -446,6 +485,17 @@ def serve_bench(m, args):
     lat = np.array([r[3] for r in results])
     st = srv.stats()
     srv.free()
+
+    def phase(a, b):
+        return {k: (round(b[k] - a[k], 6) if isinstance(b[k], float)
+                    else b[k] - a[k])
+                for k in ("batches", "steps", "admitted_midflight",
+                          "open_seconds", "admit_seconds", "step_seconds",
+                          "extract_seconds", "queue_seconds",
+                          "service_seconds", "completed")}
+    log(f"burst phase {phase(st_warm, st_burst)}")
+    log(f"closed-loop phase {phase(st_burst, st_closed)}")
+    log(f"open-loop phase {phase(st_closed, st)}")
     out = {
         "metric": "preconditioned solves/s + apply HBM GB/s (% roofline); "
                   "p50/p99 solve latency",
@@ -460,17 +510,24 @@ def serve_bench(m, args):
                         "INVK(1,1); 12 random n~logU[1e4, "
                         f"{args.c5_nmax:.0e}] AINV-LLK 5e-2), FP32 factors, "
                         "BiCGSTAB 1e-8, requests of U{1..64} columns, Poisson "
-                        "arrivals at 0.8x the measured capacity",
+                        "arrivals at 0.8x the closed-loop capacity",
             "name": "C5", "requests": len(reqs), "columns": cols,
             "workers": args.workers, "parallelism": "1 GPU"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)),
                        "p99": float(np.percentile(lat, 99)),
                        "note": "submission to completion per request "
                                "(host buffers, queueing included)"},
-        "capacity": {"requests_per_s": cap_req, "columns_per_s": cap_cols},
+        "capacity": {"requests_per_s": cap_req, "columns_per_s": cap_cols,
+                     "note": f"closed loop, {window} requests outstanding, "
+                             f"{len(closed)} requests"},
+        "burst": {"requests_per_s": burst_req, "columns_per_s": burst_cols,
+                  "requests": len(burst)},
         "arrival_rate_per_s": lam,
         "first_touch_seconds": t_warm,
         "server_stats": st,
+        "phases": {"burst": phase(st_warm, st_burst),
+                   "closed_loop": phase(st_burst, st_closed),
+                   "open_loop": phase(st_closed, st)},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))

@@ -185,6 +185,23 @@ int ainv_mm_read(const char *path, int32_t *n, int64_t *nnz, int64_t *irp,
                  int32_t *ja, double *val);
 int ainv_mm_read_vector(const char *path, int64_t *n, double *val);
 
+/* The AINV factorization alone, on the host (no device involved):
+ * mld_d_ainv_bld (src/impl/mld_d_ainv_bld.F90:35-194) with the
+ * biconjugation variant alg (AINV_ALG_*), fill cap fill_in and drop
+ * threshold thresh taken as given (no check coercion), on `threads` host
+ * threads (0: AINV_BUILD_THREADS, else the CPUs available, at most 16;
+ * bit-identical results for any count).  Outputs (0-based CSR, FP64): the
+ * stored W (W^T, as the reference keeps it) and Z with row pointers of n+1
+ * entries and at most `cap` entries each (larger: AINV_ERR_ARG with the
+ * sizes in *w_nnz, *z_nnz), and the n entries of d. */
+int ainv_host_ainv_bld(int32_t n, const int64_t *irp, const int32_t *ja,
+                       const double *val, int index_base, int alg,
+                       int32_t fill_in, double thresh, int threads,
+                       int64_t cap, int64_t *w_irp, int32_t *w_ja,
+                       double *w_val, int64_t *w_nnz, int64_t *z_irp,
+                       int32_t *z_ja, double *z_val, int64_t *z_nnz,
+                       double *d);
+
 /* sv%update_a (mld_d_base_ainv_update_a.f90:53-79): adapt the built
  * preconditioner to the shifted operator A + diag(x) by updating the
  * inverted pivots, d_i = 1 / (1/d_i + sum_k W(i,k) x_k Z(k,i)) (stored W by

@@ -89,6 +89,10 @@ def lib():
         "ainv_mm_read": [C.c_char_p, P(C.c_int32), P(C.c_int64), i64p, i32p,
                          dp],
         "ainv_mm_read_vector": [C.c_char_p, P(C.c_int64), dp],
+        "ainv_host_ainv_bld": [C.c_int32, i64p, i32p, dp, C.c_int, C.c_int,
+                               C.c_int32, C.c_double, C.c_int, C.c_int64,
+                               i64p, i32p, dp, P(C.c_int64), i64p, i32p, dp,
+                               P(C.c_int64), dp],
         "ainv_krylov_stream": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_int,
                                C.c_double, C.c_int, vp, vp, vp, vp],
         "ainv_krylov": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_double,
@@ -595,6 +599,40 @@ def mm_read_vector(path):
     _ck(lib().ainv_mm_read_vector(_b(path), None,
                                   v.ctypes.data_as(C.POINTER(C.c_double))))
     return v
+
+
+# AINV_ALG_* of include/mld_ainv_hip.h by the reference's names
+ALGS = {"LLK": 1, "SYM-LLK": 2, "STAB-LLK": 3, "LLK-NOTH": 4, "MLK": 5}
+
+
+def host_ainv_bld(a, alg="LLK", fill_in=8, thresh=0.1, threads=0):
+    """mld_d_ainv_bld on the host (ainv_host_ainv_bld): returns
+    (W_stored, Z, d) as scipy CSR / numpy.  threads = 0: the default."""
+    import scipy.sparse as sp
+    a = sp.csr_matrix(a)
+    a.sort_indices()
+    n = a.shape[0]
+    irp = np.ascontiguousarray(a.indptr, dtype=np.int64)
+    ja = np.ascontiguousarray(a.indices, dtype=np.int32)
+    val = np.ascontiguousarray(a.data, dtype=np.float64)
+    cap = int(n) * max(int(fill_in), 1) + 1 if fill_in >= 1 else \
+        int(n) * int(n)
+    outs = []
+    for _ in range(2):
+        outs.append((np.empty(n + 1, dtype=np.int64),
+                     np.empty(cap, dtype=np.int32),
+                     np.empty(cap, dtype=np.float64)))
+    wn, zn = C.c_int64(), C.c_int64()
+    d = np.empty(n, dtype=np.float64)
+    (wi, wj, wv), (zi, zj, zv) = outs
+    _ck(lib().ainv_host_ainv_bld(
+        n, *_ptrs(irp, ja, val), 0, ALGS[alg], int(fill_in), float(thresh),
+        int(threads), cap, *_ptrs(wi, wj, wv), C.byref(wn),
+        *_ptrs(zi, zj, zv), C.byref(zn),
+        d.ctypes.data_as(C.POINTER(C.c_double))))
+    w = sp.csr_matrix((wv[:wn.value], wj[:wn.value], wi), shape=(n, n))
+    z = sp.csr_matrix((zv[:zn.value], zj[:zn.value], zi), shape=(n, n))
+    return w, z, d
 
 
 # --------------------------------------------------------------- generators

@@ -168,8 +168,12 @@ void host_iluk(const HostCsr &a, int32_t fill, HostCsr &l, HostCsr &u,
 void host_ilut(const HostCsr &a, int32_t fill, double thresh, HostCsr &l,
                HostCsr &u, std::vector<double> &d);
 // AINV: returns stored W (= W^T, CSR), Z (CSR) and d.
+// Host threads of the build phase: AINV_BUILD_THREADS, else the CPUs this
+// process may run on (at most 16).
+int build_threads();
 void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
-               HostCsr &w, HostCsr &z, std::vector<double> &d);
+               HostCsr &w, HostCsr &z, std::vector<double> &d,
+               int threads = 0);
 
 double now_seconds();
 

@@ -24,14 +24,20 @@
 //            src/impl/mld_d_sparsify.f90:83-143, :152-230
 // Ties among equal magnitudes at a fill cap are resolved by "smaller column
 // first" (the reference leaves them to PSBLAS heap order).
+#include <sched.h>
+
 #include <algorithm>
-#include <exception>
-#include <thread>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <exception>
 #include <functional>
+#include <memory>
+#include <mutex>
 #include <queue>
+#include <thread>
 
 #include "ainv_internal.h"
 
@@ -343,15 +349,90 @@ void clear_acc(PassWork &w) {
   w.acc_idx.clear();
 }
 
+// Finished columns of the factor under construction, appended in column
+// order by the one thread that builds them.
+struct SeqCols {
+  ColStore &c;
+  void wait(int32_t) const {}
+  int64_t size(int32_t j) const { return c.ptr[j + 1] - c.ptr[j]; }
+  const int32_t *idx(int32_t j) const { return c.idx.data() + c.ptr[j]; }
+  const double *val(int32_t j) const { return c.val.data() + c.ptr[j]; }
+  void emit(int32_t, const std::vector<Entry> &e) {
+    for (const Entry &x : e) {
+      c.idx.push_back(x.j);
+      c.val.push_back(x.v);
+    }
+    c.ptr.push_back((int64_t)c.idx.size());
+  }
+};
+
+// Columns built concurrently by several threads that claim column indices
+// in increasing order: column j lives in its own slot of `cap` entries and
+// is published (with its pivot, written before) by a release store of
+// ready[j]; a reader of column j < i waits for that flag.  Every column
+// below the smallest unfinished one is final and that one's builder only
+// waits on smaller columns, so the wait always ends.
+struct SlotCols {
+  int32_t cap = 0;
+  std::vector<int32_t> ix;
+  std::vector<double> vx;
+  std::vector<int32_t> cnt;
+  std::unique_ptr<std::atomic<int32_t>[]> ready;
+  std::atomic<bool> *abort = nullptr;
+  SlotCols(int32_t n, int32_t c, std::atomic<bool> *ab)
+      : cap(c), ix((size_t)n * c), vx((size_t)n * c), cnt(n, 0),
+        ready(new std::atomic<int32_t>[n]), abort(ab) {
+    for (int32_t j = 0; j < n; ++j) ready[j].store(0, std::memory_order_relaxed);
+  }
+  void wait(int32_t j) const {
+    int spins = 0;
+    while (!ready[j].load(std::memory_order_acquire)) {
+      if (abort->load(std::memory_order_relaxed))
+        throw Error(AINV_ERR_INTERNAL, "biconjugation aborted");
+      if (++spins > 64) {
+        std::this_thread::yield();
+        spins = 0;
+      }
+    }
+  }
+  int64_t size(int32_t j) const { return cnt[j]; }
+  const int32_t *idx(int32_t j) const { return ix.data() + (size_t)j * cap; }
+  const double *val(int32_t j) const { return vx.data() + (size_t)j * cap; }
+  void emit(int32_t i, const std::vector<Entry> &e) {
+    if ((int64_t)e.size() > cap)
+      throw Error(AINV_ERR_INTERNAL, "biconjugation column over capacity");
+    for (size_t t = 0; t < e.size(); ++t) {
+      ix[(size_t)i * cap + t] = e[t].j;
+      vx[(size_t)i * cap + t] = e[t].v;
+    }
+    cnt[i] = (int32_t)e.size();
+    ready[i].store(1, std::memory_order_release);
+  }
+  void to_colstore(ColStore &c) const {
+    const int32_t n = (int32_t)cnt.size();
+    c.ptr.assign(1, 0);
+    c.idx.clear();
+    c.val.clear();
+    for (int32_t j = 0; j < n; ++j) {
+      c.idx.insert(c.idx.end(), idx(j), idx(j) + cnt[j]);
+      c.val.insert(c.val.end(), val(j), val(j) + cnt[j]);
+      c.ptr.push_back((int64_t)c.idx.size());
+    }
+  }
+};
+
 // One column i of Z (dot = rows of A, seed/enqueue = columns of A) or of W
 // (the same with A^T).  dotm: matrix whose row j is dotted with zval;
 // seedm: matrix whose row i / row kr lists the candidates to queue.
 // With v.stabilized the dot of step j is (g_j^T M) . zval, g_j being
 // column j of the other factor `g` and M = `comb`, and no pivot is formed.
+// f gives the finished columns (and receives column i); column j and
+// piv[j] are read only after f.wait(j).
+template <class Cols>
 void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
-                 const ColStore &f, std::vector<double> &piv, int32_t nzrmax,
+                 Cols &f, std::vector<double> &piv, int32_t nzrmax,
                  double thresh, const Variant &v, const ColStore *g,
-                 const HostCsr *comb, PassWork &w, ColStore &out) {
+                 const HostCsr *comb, PassWork &w) {
   w.zval[i] = 1.0;
   w.in_z[i] = 1;
   w.pat.push(i);
@@ -381,11 +462,15 @@ void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
       for (int64_t e = dotm.ptr[j]; e < dotm.ptr[j + 1]; ++e)
         s = s + dotm.val[e] * w.zval[dotm.idx[e]];
     }
+    f.wait(j);
     const double alpha = (-s / piv[j]);
     if (!v.no_alpha_test && !(std::fabs(alpha) > thresh)) continue;
-    for (int64_t e = f.ptr[j]; e < f.ptr[j + 1]; ++e) {
-      const int32_t kr = f.idx[e];
-      w.zval[kr] = w.zval[kr] + alpha * f.val[e];
+    const int32_t *fi = f.idx(j);
+    const double *fv = f.val(j);
+    const int64_t fn = f.size(j);
+    for (int64_t e = 0; e < fn; ++e) {
+      const int32_t kr = fi[e];
+      w.zval[kr] = w.zval[kr] + alpha * fv[e];
       if (w.in_z[kr]) continue;
       w.pat.push(kr);
       w.in_z[kr] = 1;
@@ -431,17 +516,74 @@ void bicg_column(int32_t i, const HostCsr &dotm, const HostCsr &seedm,
   w.cand.push_back({diag, i});
   std::sort(w.cand.begin(), w.cand.end(),
             [](const Entry &a, const Entry &b) { return a.j < b.j; });
-  for (const Entry &e : w.cand) {
-    out.idx.push_back(e.j);
-    out.val.push_back(e.v);
+  f.emit(i, w.cand);
+}
+
+void init_work(PassWork &w, int32_t n, bool stabilized) {
+  w.zval.assign(n, 0.0);
+  w.in_z.assign(n, 0);
+  w.queued.assign(n, 0);
+  if (stabilized) {
+    w.acc.assign(n, 0.0);
+    w.acc_on.assign(n, 0);
   }
-  out.ptr.push_back((int64_t)out.idx.size());
+}
+
+// Columns 1..n-1 of one pass on `threads` threads claiming columns in
+// increasing order (SlotCols).  Results are bit-identical to the
+// sequential loop: each column performs the same operations on the same
+// finished columns, whichever thread runs it.
+void pipelined_pass(int32_t n, const HostCsr &dotm, const HostCsr &seedm,
+                    std::vector<double> &piv, int32_t nzrmax, double thresh,
+                    const Variant &v, const ColStore &first, int threads,
+                    ColStore &out) {
+  const int32_t cap = std::max<int32_t>(nzrmax, 1);
+  std::atomic<bool> abort{false};
+  SlotCols cols(n, cap, &abort);
+  cols.emit(0, {{first.val[0], first.idx[0]}});
+  std::atomic<int32_t> next{1};
+  std::exception_ptr err;
+  std::mutex err_mu;
+  auto worker = [&] {
+    try {
+      PassWork w;
+      init_work(w, n, false);
+      for (;;) {
+        const int32_t i = next.fetch_add(1, std::memory_order_relaxed);
+        if (i >= n || abort.load(std::memory_order_relaxed)) break;
+        bicg_column(i, dotm, seedm, cols, piv, nzrmax, thresh, v, nullptr,
+                    nullptr, w);
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(err_mu);
+      if (!err) err = std::current_exception();
+      abort.store(true);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
+  if (err) std::rethrow_exception(err);
+  cols.to_colstore(out);
 }
 
 }  // namespace
 
+int build_threads() {
+  if (const char *e = std::getenv("AINV_BUILD_THREADS")) {
+    const int t = std::atoi(e);
+    if (t >= 1) return std::min(t, 64);
+  }
+  cpu_set_t set;
+  int cpus = 0;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) cpus = CPU_COUNT(&set);
+  if (cpus < 1) cpus = (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(cpus, 16));
+}
+
 void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
-               HostCsr &w, HostCsr &z, std::vector<double> &d) {
+               HostCsr &w, HostCsr &z, std::vector<double> &d, int threads) {
   // the dispatcher's select (mld_d_biconjg_mod.F90:107-122)
   if (alg < AINV_ALG_LLK || alg > AINV_ALG_MLK)
     throw Error(AINV_ERR_ALG, "Invalid alg " + std::to_string(alg));
@@ -459,13 +601,7 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
   std::vector<double> p(n, 0.0), q(n, 0.0);
   ColStore zc, wc;
   PassWork pw;
-  pw.zval.assign(n, 0.0);
-  pw.in_z.assign(n, 0);
-  pw.queued.assign(n, 0);
-  if (var.stabilized) {
-    pw.acc.assign(n, 0.0);
-    pw.acc_on.assign(n, 0);
-  }
+  init_work(pw, n, var.stabilized);
   if (n > 0) {
     for (int64_t e = as.ptr[0]; e < as.ptr[1]; ++e)
       if (as.idx[e] == 0) {
@@ -481,42 +617,43 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
     wc.val.push_back(1.0);
     wc.ptr.push_back(1);
   }
-  if (!var.stabilized && alg != AINV_ALG_S_LLK && n > 1) {
+  if (threads <= 0) threads = build_threads();
+  // bounded columns (a fill cap) can be built by the pipelined passes
+  const bool bounded = fill_in >= 1 || var.list_sparsify;
+  const bool two = alg != AINV_ALG_S_LLK;
+  bool done = false;
+  if (!var.stabilized && n > 1 && threads >= 2 && bounded) {
     // Without stabilization the Z pass (pivots p) and the W pass (pivots q)
-    // share nothing but scratch arrays: they run on two host threads, each
-    // with its own scratch, performing exactly the operations of the
-    // interleaved loop below.
-    PassWork pw2;
-    pw2.zval.assign(n, 0.0);
-    pw2.in_z.assign(n, 0);
-    pw2.queued.assign(n, 0);
+    // share nothing: each runs pipelined on its share of the threads.
+    const int tz = two ? std::max(1, (threads + 1) / 2) : threads;
+    const int tw = std::max(1, threads - tz);
     std::exception_ptr err;
-    std::thread tw([&] {
-      try {
-        for (int32_t i = 1; i < n; ++i)
-          bicg_column(i, at, as, wc, q, fill_in, thresh, var, &zc, &at, pw2,
-                      wc);
-      } catch (...) {
-        err = std::current_exception();
-      }
-    });
+    std::thread wth;
+    if (two)
+      wth = std::thread([&] {
+        try {
+          pipelined_pass(n, at, as, q, fill_in, thresh, var, wc, tw, wc);
+        } catch (...) {
+          err = std::current_exception();
+        }
+      });
     try {
-      for (int32_t i = 1; i < n; ++i)
-        bicg_column(i, as, at, zc, p, fill_in, thresh, var, &wc, &as, pw, zc);
+      pipelined_pass(n, as, at, p, fill_in, thresh, var, zc, tz, zc);
     } catch (...) {
-      tw.join();
+      if (wth.joinable()) wth.join();
       throw;
     }
-    tw.join();
+    if (wth.joinable()) wth.join();
     if (err) std::rethrow_exception(err);
+    done = true;
   }
-  for (int32_t i = 1; i < n; ++i) {
-    if (!var.stabilized && alg != AINV_ALG_S_LLK) break;  // done above
+  SeqCols zs{zc}, ws{wc};
+  for (int32_t i = 1; i < n && !done; ++i) {
     // Z: dots with rows of A (STAB: w_j^T A), seeds from column i of A
-    bicg_column(i, as, at, zc, p, fill_in, thresh, var, &wc, &as, pw, zc);
+    bicg_column(i, as, at, zs, p, fill_in, thresh, var, &wc, &as, pw);
     if (alg == AINV_ALG_S_LLK) continue;
     // W: dots with columns of A (STAB: A z_j), seeds from row i of A
-    bicg_column(i, at, as, wc, q, fill_in, thresh, var, &zc, &at, pw, wc);
+    bicg_column(i, at, as, ws, q, fill_in, thresh, var, &zc, &at, pw);
     if (var.stabilized) {
       // q(i) = w_i^T (A z_i), merge dot over sorted indices
       // (mld_dsparse_biconjg_s_ft_llk.F90:377-411)

@@ -1012,6 +1012,38 @@ int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
   });
 }
 
+int ainv_host_ainv_bld(int32_t n, const int64_t *irp, const int32_t *ja,
+                       const double *val, int index_base, int alg,
+                       int32_t fill_in, double thresh, int threads,
+                       int64_t cap, int64_t *w_irp, int32_t *w_ja,
+                       double *w_val, int64_t *w_nnz, int64_t *z_irp,
+                       int32_t *z_ja, double *z_val, int64_t *z_nnz,
+                       double *d) {
+  return guard([&] {
+    if (!w_irp || !w_ja || !w_val || !w_nnz || !z_irp || !z_ja || !z_val ||
+        !z_nnz || !d)
+      throw Error(AINV_ERR_ARG, "NULL output");
+    if (threads < 0) throw Error(AINV_ERR_ARG, "threads must be >= 0");
+    HostCsr a = canonical_csr(n, irp, ja, val, index_base);
+    HostCsr w, z;
+    std::vector<double> dv;
+    host_ainv(a, alg, fill_in, thresh, w, z, dv, threads);
+    *w_nnz = w.nnz();
+    *z_nnz = z.nnz();
+    if (w.nnz() > cap || z.nnz() > cap)
+      throw Error(AINV_ERR_ARG, "factor larger than the output capacity (" +
+                                    std::to_string(std::max(w.nnz(), z.nnz())) +
+                                    " entries)");
+    std::copy(w.ptr.begin(), w.ptr.end(), w_irp);
+    std::copy(w.idx.begin(), w.idx.end(), w_ja);
+    std::copy(w.val.begin(), w.val.end(), w_val);
+    std::copy(z.ptr.begin(), z.ptr.end(), z_irp);
+    std::copy(z.idx.begin(), z.idx.end(), z_ja);
+    std::copy(z.val.begin(), z.val.end(), z_val);
+    std::copy(dv.begin(), dv.end(), d);
+  });
+}
+
 int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
                 double *x, int k, double eps, int itmax, int istop, int irst,
                 int32_t *iter, double *err, int flags, void *stream) {

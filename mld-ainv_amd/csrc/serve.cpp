@@ -85,6 +85,9 @@ struct ainv_server {
   int64_t builds = 0, evictions = 0, completed = 0, admitted_midflight = 0;
   int64_t steps = 0, batches = 0;
   double build_seconds = 0;
+  // host seconds per phase summed over workers, and request waits
+  double open_seconds = 0, admit_seconds = 0, step_seconds = 0,
+         extract_seconds = 0, queue_seconds = 0, service_seconds = 0;
   int64_t peak_resident = 0;
 };
 
@@ -164,6 +167,10 @@ void finish_request(ainv_server *S, Request *r, int status,
   r->t_done = now_seconds();
   r->done = true;
   S->completed++;
+  if (r->t_start > 0) {
+    S->queue_seconds += r->t_start - r->t_submit;
+    S->service_seconds += r->t_done - r->t_start;
+  }
   S->cv_done.notify_all();
 }
 
@@ -186,8 +193,14 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
   ensure_resident(S, m, K);
   const int32_t n = m->a.n;
   const size_t need = (size_t)n * kMaxCols * sizeof(double);
+  const double t_open = now_seconds();
   if (stage.bytes < need) stage.alloc(need);
   SlotBatch *b = batch_open(m->A, m->M, m->method, K, s);
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->open_seconds += now_seconds() - t_open;
+  }
+  double t_admit = 0, t_step = 0, t_extract = 0;
   std::vector<SlotUse> uses;  // admitted requests
   std::vector<int> owner(K, -1);
   std::vector<SlotState> st;
@@ -216,6 +229,7 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
           if (!first) S->admitted_midflight++;
         }
       }
+      const double t0 = now_seconds();
       for (size_t t = 0; t < adm.size(); ++t) {
         Request *r = adm[t];
         r->t_start = now_seconds();
@@ -224,6 +238,8 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
                                       hipMemcpyHostToDevice, s));
         batch_admit(b, adm_c0[t], r->k, stage.as<double>(), r->eps);
       }
+      const double t1 = now_seconds();
+      t_admit += t1 - t0;
       first = false;
       bool any = false;
       for (int c = 0; c < K; ++c) any |= owner[c] >= 0;
@@ -234,6 +250,8 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
         S->steps++;
       }
       const int64_t seen = batch_poll(b, false, st);
+      const double t2 = now_seconds();
+      t_step += t2 - t1;
       if (seen >= 0) {
         // complete requests whose columns all stopped (or hit itmax); a
         // snapshot older than a request's admission shows the previous
@@ -269,6 +287,7 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
           finish_request(S, r, AINV_SUCCESS, "");
         }
       }
+      t_extract += now_seconds() - t2;
     }
   } catch (const std::exception &e) {
     batch_close(b);
@@ -287,6 +306,9 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
   batch_close(b);
   std::unique_lock<std::mutex> lk(S->mu);
   S->batches++;
+  S->admit_seconds += t_admit;
+  S->step_seconds += t_step;
+  S->extract_seconds += t_extract;
 }
 
 void worker_main(ainv_server *S) {
@@ -480,13 +502,18 @@ int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
                   "\"budget_bytes\": %lld, \"builds\": %lld, "
                   "\"build_seconds\": %.6f, \"evictions\": %lld, "
                   "\"completed\": %lld, \"batches\": %lld, \"steps\": %lld, "
-                  "\"admitted_midflight\": %lld, \"workers\": %zu}",
+                  "\"admitted_midflight\": %lld, \"workers\": %zu, "
+                  "\"open_seconds\": %.6f, \"admit_seconds\": %.6f, "
+                  "\"step_seconds\": %.6f, \"extract_seconds\": %.6f, "
+                  "\"queue_seconds\": %.6f, \"service_seconds\": %.6f}",
                   S->mats.size(), resident_mats, (long long)S->resident,
                   (long long)S->peak_resident, (long long)S->budget,
                   (long long)S->builds, S->build_seconds,
                   (long long)S->evictions, (long long)S->completed,
                   (long long)S->batches, (long long)S->steps,
-                  (long long)S->admitted_midflight, S->workers.size());
+                  (long long)S->admitted_midflight, S->workers.size(),
+                  S->open_seconds, S->admit_seconds, S->step_seconds,
+                  S->extract_seconds, S->queue_seconds, S->service_seconds);
   });
 }
 

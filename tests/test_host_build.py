@@ -1,0 +1,45 @@
+"""Host AINV factorization through the C ABI (ainv_host_ainv_bld, no GPU):
+bit-identical to the oracle's mld_d_ainv_bld restatement for every
+biconjugation variant and for any number of host threads (the pipelined
+passes claim columns in order and wait for the finished columns they
+read)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+
+def same(x, y):
+    x, y = sp.csr_matrix(x), sp.csr_matrix(y)
+    return (np.array_equal(x.indptr, y.indptr)
+            and np.array_equal(x.indices, y.indices)
+            and np.array_equal(x.data, y.data))
+
+
+@pytest.mark.parametrize("alg,fill,thresh", [("LLK", 8, 0.1),
+                                             ("LLK", 4, 0.0),
+                                             ("SYM-LLK", 8, 5e-2),
+                                             ("LLK-NOTH", 6, 0.1),
+                                             ("MLK", 8, 0.1),
+                                             ("STAB-LLK", 8, 0.1),
+                                             ("LLK", 0, 0.1)])
+@pytest.mark.parametrize("threads", [1, 2, 3, 8])
+def test_host_ainv_matches_oracle(mld, O, alg, fill, thresh, threads):
+    a = mld.gen_pde3d(14, 1) if alg != "SYM-LLK" else mld.gen_aniso2d(40)
+    if alg == "LLK" and fill == 0:
+        a = mld.gen_pde3d(7, 1)  # no fill cap: keeps whole columns
+    w, z, d = mld.host_ainv_bld(a, alg, fill, thresh, threads)
+    wr, zr, dr = O.ainv_bld(a, alg, fill, thresh)
+    assert same(w, wr) and same(z, zr)
+    assert np.array_equal(d, dr)
+
+
+@pytest.mark.parametrize("threads", [2, 5, 8])
+def test_host_ainv_random_matrix_threads(mld, O, threads):
+    """A random banded matrix (columns within +-300 plus long-range ones):
+    the pipelined passes run far ahead of the columns they wait for."""
+    a = mld.gen_random(6000, seed=41, median=8.0, band=300, nlong=2)
+    w1, z1, d1 = mld.host_ainv_bld(a, "LLK", 8, 5e-2, 1)
+    wt, zt, dt = mld.host_ainv_bld(a, "LLK", 8, 5e-2, threads)
+    assert same(w1, wt) and same(z1, zt) and np.array_equal(d1, dt)
+    wr, zr, dr = O.ainv_bld(a, "LLK", 8, 5e-2)
+    assert same(w1, wr) and same(z1, zr) and np.array_equal(d1, dr)
