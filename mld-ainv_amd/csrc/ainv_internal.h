@@ -138,12 +138,15 @@ constexpr int kNumMaps = 4;
 inline int map_pair(int w, int z) { return w | (z << 4); }
 inline int map_w(int pair) { return pair & 15; }
 inline int map_z(int pair) { return (pair >> 4) & 15; }
+// kw (device pointer, may be NULL): active width of the block; the
+// cooperative mappings neither read nor write columns at or beyond *kw.
 void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
                     int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s,
-                    int map = AINV_MAP_SPLIT);
+                    int map = AINV_MAP_SPLIT, const int32_t *kw = nullptr);
 void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, double *y, int64_t ldy, int k,
-                    hipStream_t s, int map = AINV_MAP_SPLIT);
+                    hipStream_t s, int map = AINV_MAP_SPLIT,
+                    const int32_t *kw = nullptr);
 // Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *
 // kMaxCols + c] for pair q, block b, column c.
 constexpr int kRedBlocks = 1024;
@@ -153,7 +156,7 @@ constexpr int kMaxCols = 64;
 // (y, w), per column, as block partials.  Returns the number of blocks.
 int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
                  const double *w, int np, double *part, int map,
-                 hipStream_t s);
+                 hipStream_t s, const int32_t *kw = nullptr);
 
 // ---- GPU row inverses (invkt.hip) ----------------------------------------
 // Row-wise approximate inverse of a unit upper triangular matrix given by

@@ -594,11 +594,15 @@ template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
 }
 
 // x rows are K apart; xs points at this lane's first column of the chunk.
+// A lane whose columns are all at or beyond the active width (live false)
+// takes part in the shuffles and loads row 0 of its columns instead of the
+// gathered rows (a select, not a branch, so the loads keep their schedule;
+// all such lanes hit the same lines), and its sums are not stored.
 template <typename VT, typename XT, int K, int KC>
 __device__ __forceinline__ void coop_acc(
     int64_t p0, int32_t width, int lane, int rg,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
-    const XT *__restrict__ xs,
+    const XT *__restrict__ xs, bool live,
     typename Val<VT>::acc (&acc)[Coop<XT, KC>::L][Coop<XT, KC>::CPL]) {
   using A = typename Val<VT>::acc;
   using CP = Coop<XT, KC>;
@@ -624,7 +628,7 @@ __device__ __forceinline__ void coop_acc(
 #pragma unroll
         for (int q = 0; q < L; ++q) {
           const int32_t c = __shfl(ci[u], q * R + rg);
-          load_xrow<XT, CPL>(xs + (int64_t)c * K, xv[u][q]);
+          load_xrow<XT, CPL>(live ? xs + (int64_t)c * K : xs, xv[u][q]);
         }
       const int32_t jn = j + P;
       const bool more = jn < nfull;
@@ -661,7 +665,7 @@ __device__ __forceinline__ void coop_acc(
       const int32_t c = __shfl(c0, q * R + rg);
       const A v = Val<VT>::get(shfl_raw(v0, q * R + rg));
       A xv[CPL];
-      load_xrow<XT, CPL>(xs + (int64_t)c * K, xv);
+      load_xrow<XT, CPL>(live ? xs + (int64_t)c * K : xs, xv);
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] += v * xv[u];
     }
@@ -674,7 +678,8 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
-    OT *__restrict__ out, double alpha, double beta) {
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ kwp) {
   using A = typename Val<VT>::acc;
   using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC;
@@ -686,6 +691,9 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
   const int32_t lb = __builtin_amdgcn_readfirstlane(
       xcd_block(blockIdx.x, gridDim.x));
   const int chunk = lb / ngrp;
+  // active width: columns at or beyond it are neither read nor written
+  const int kw = kwp ? min(k, *kwp) : k;
+  if (chunk * KC >= kw) return;
   const int32_t s = lb % ngrp * kWavesPerBlock +
                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (s >= nslices) return;
@@ -697,7 +705,8 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
 #pragma unroll
     for (int u = 0; u < CP::CPL; ++u) acc[q][u] = 0;
   const int c0 = chunk * KC + sub * CP::CPL;  // first column of this lane
-  coop_acc<VT, XT, K, KC>(base, width, lane, rg, col, val, x + c0, acc);
+  coop_acc<VT, XT, K, KC>(base, width, lane, rg, col, val, x + c0, c0 < kw,
+                          acc);
 #pragma unroll
   for (int q = 0; q < CP::L; ++q) {
     const int32_t r = s * kSlice + q * CP::R + rg;
@@ -707,11 +716,11 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
       const double d = (double)dvec[r];
 #pragma unroll
       for (int u = 0; u < CP::CPL; ++u)
-        if (c0 + u < k) o[u] = (OT)(d * (double)acc[q][u]);
+        if (c0 + u < kw) o[u] = (OT)(d * (double)acc[q][u]);
     } else {
 #pragma unroll
       for (int u = 0; u < CP::CPL; ++u) {
-        if (c0 + u >= k) continue;
+        if (c0 + u >= kw) continue;
         double v = alpha * (double)acc[q][u];
         if (beta != 0.0) v += beta * (double)o[u];
         o[u] = (OT)v;
@@ -734,7 +743,8 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
                     const double *__restrict__ val,
                     const double *__restrict__ x, int k,
                     double *__restrict__ y, const double *__restrict__ w,
-                    double *__restrict__ part) {
+                    double *__restrict__ part,
+                    const int32_t *__restrict__ kwp) {
   using CH = CoopChunk<double, K, CB>;
   constexpr int KC = CH::KC, NCH = CH::NCH;
   using CP = Coop<double, KC>;
@@ -758,10 +768,13 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
   const int32_t g_stride = gridDim.x / NCH;
 #endif
   const int c0 = chunk * KC + sub * CPL;
+  // active width: a chunk wholly beyond it only writes its zero partials
+  const int kw = kwp ? min(k, *kwp) : k;
+  const int32_t g_stop = chunk * KC >= kw ? g_first : g_end;
   double d0[CPL], d1[CPL];
 #pragma unroll
   for (int u = 0; u < CPL; ++u) d0[u] = d1[u] = 0.0;
-  for (int32_t grp = g_first; grp < g_end; grp += g_stride) {
+  for (int32_t grp = g_first; grp < g_stop; grp += g_stride) {
     const int32_t s =
         __builtin_amdgcn_readfirstlane(grp * kWavesPerBlock + wave);
     if (s >= nslices) break;
@@ -773,17 +786,17 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
     coop_acc<double, double, K, KC>(base, width, lane, rg, col, val, x + c0,
-                                    acc);
+                                    c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
       const int32_t r = s * kSlice + q * R + rg;
       if (r >= n) continue;
       double wr[CPL];
-      load_xrow<double, CPL>(w + (int64_t)r * K + c0, wr);
+      load_xrow<double, CPL>(c0 < kw ? w + (int64_t)r * K + c0 : w + c0, wr);
       double *yo = y + (int64_t)r * K + c0;
 #pragma unroll
       for (int u = 0; u < CPL; ++u) {
-        if (c0 + u >= k) continue;
+        if (c0 + u >= kw) continue;
         yo[u] = acc[q][u];
         if constexpr (NP == 1) {
           d0[u] += wr[u] * acc[q][u];
@@ -825,7 +838,8 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
 
 template <typename VT, typename XT, typename DT, typename OT, int CB, int EPI>
 void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
-                 double alpha, double beta, hipStream_t s) {
+                 double alpha, double beta, hipStream_t s,
+                 const int32_t *kw) {
   if (a.nslices == 0) return;
   const int ngrp = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
   const int64_t *sp = a.sptr.as<int64_t>();
@@ -835,7 +849,8 @@ void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
   hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, CB, EPI>),        \
                      dim3(ngrp * CoopChunk<XT, KK, CB>::NCH), dim3(kBlock), 0, \
                      s,                                                       \
-                     a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha, beta)
+                     a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha, beta, \
+                     kw)
   if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)
@@ -852,13 +867,16 @@ void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
   AINV_HIP_CHECK(hipGetLastError());
 }
 
+// kw (device, may be NULL): active width of the block, honoured by the
+// cooperative mappings (the others compute every column).
 template <typename VT, typename XT, typename DT, typename OT, int EPI>
 void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
-                 OT *out, double alpha, double beta, hipStream_t s) {
+                 OT *out, double alpha, double beta, hipStream_t s,
+                 const int32_t *kw) {
   if (map == AINV_MAP_COOP)
-    launch_coop<VT, XT, DT, OT, 256, EPI>(a, x, k, d, out, alpha, beta, s);
+    launch_coop<VT, XT, DT, OT, 256, EPI>(a, x, k, d, out, alpha, beta, s, kw);
   else if (map == AINV_MAP_COOP128)
-    launch_coop<VT, XT, DT, OT, 128, EPI>(a, x, k, d, out, alpha, beta, s);
+    launch_coop<VT, XT, DT, OT, 128, EPI>(a, x, k, d, out, alpha, beta, s, kw);
   else
     launch_row<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
 }
@@ -872,7 +890,7 @@ bool aligned16(const void *p) {
 
 int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
                  const double *w, int np, double *part, int map,
-                 hipStream_t s) {
+                 hipStream_t s, const int32_t *kw) {
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
@@ -894,11 +912,11 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
     if (map == AINV_MAP_COOP)                                                \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 256, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part);                                     \
+                         k, y, w, part, kw);                                 \
     else if (map == AINV_MAP_COOP128)                                        \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part);                                     \
+                         k, y, w, part, kw);                                 \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_row<KK, NPP>), dim3(g),             \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
@@ -910,11 +928,11 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
     if (map == AINV_MAP_COOP128)                                             \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part);                                     \
+                         k, y, w, part, kw);                                 \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 256, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part);                                     \
+                         k, y, w, part, kw);                                 \
   } while (0)
 #define AINV_LRK(NPP)   \
   if (K == 2)           \
@@ -988,7 +1006,7 @@ static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,
 
 void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
                     int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s,
-                    int map) {
+                    int map, const int32_t *kw) {
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   if (x_is_f32) throw Error(AINV_ERR_INTERNAL, "scale pass reads FP64 x");
   const double *xd = static_cast<const double *>(x);
@@ -997,17 +1015,17 @@ void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
       case AINV_PREC_F64:
         launch_lane<double, double, double, double, EPI_SCALE>(
             map, a, xd, k, static_cast<const double *>(d), static_cast<double *>(out),
-            1.0, 0.0, s);
+            1.0, 0.0, s, kw);
         break;
       case AINV_PREC_F32:
         launch_lane<float, double, float, float, EPI_SCALE>(
             map, a, xd, k, static_cast<const float *>(d), static_cast<float *>(out),
-            1.0, 0.0, s);
+            1.0, 0.0, s, kw);
         break;
       default:
         launch_lane<uint16_t, double, float, float, EPI_SCALE>(
             map, a, xd, k, static_cast<const float *>(d), static_cast<float *>(out),
-            1.0, 0.0, s);
+            1.0, 0.0, s, kw);
     }
     return;
   }
@@ -1031,7 +1049,7 @@ void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
 
 void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, double *y, int64_t ldy, int k,
-                    hipStream_t s, int map) {
+                    hipStream_t s, int map, const int32_t *kw) {
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   if (use_row(map, k, ldx, ldy, x, y)) {
     switch (a.prec) {
@@ -1039,27 +1057,27 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
         if (x_is_f32) throw Error(AINV_ERR_INTERNAL, "F64 factor with FP32 t");
         launch_lane<double, double, double, double, EPI_AXPBY>(
             map, a, static_cast<const double *>(x), k, (const double *)nullptr, y,
-            alpha, beta, s);
+            alpha, beta, s, kw);
         break;
       case AINV_PREC_F32:
         if (x_is_f32)
           launch_lane<float, float, float, double, EPI_AXPBY>(
               map, a, static_cast<const float *>(x), k, (const float *)nullptr, y,
-              alpha, beta, s);
+              alpha, beta, s, kw);
         else
           launch_lane<float, double, float, double, EPI_AXPBY>(
               map, a, static_cast<const double *>(x), k, (const float *)nullptr, y,
-              alpha, beta, s);
+              alpha, beta, s, kw);
         break;
       default:
         if (x_is_f32)
           launch_lane<uint16_t, float, float, double, EPI_AXPBY>(
               map, a, static_cast<const float *>(x), k, (const float *)nullptr, y,
-              alpha, beta, s);
+              alpha, beta, s, kw);
         else
           launch_lane<uint16_t, double, float, double, EPI_AXPBY>(
               map, a, static_cast<const double *>(x), k, (const float *)nullptr, y,
-              alpha, beta, s);
+              alpha, beta, s, kw);
     }
     return;
   }

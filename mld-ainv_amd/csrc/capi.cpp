@@ -370,10 +370,11 @@ size_t solver_t_bytes(const ainv_solver *h, int k) {
 }
 
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
-                       int k, int64_t ld, void *t, hipStream_t s, int map) {
+                       int k, int64_t ld, void *t, hipStream_t s, int map,
+                       const int32_t *kw) {
   const bool f32t = h->prec != AINV_PREC_F64;
-  hll_spmm_scale(h->hw, h->dv.p, x, false, ld, t, ld, k, s, map_w(map));
-  hll_spmm_axpby(h->hz, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map));
+  hll_spmm_scale(h->hw, h->dv.p, x, false, ld, t, ld, k, s, map_w(map), kw);
+  hll_spmm_axpby(h->hz, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map), kw);
 }
 
 void solver_ensure_transposed(ainv_solver *h) {
@@ -383,11 +384,12 @@ void solver_ensure_transposed(ainv_solver *h) {
 
 // y = M^T x with the transposed layout ('T' apply, beta = 0), no allocation
 void solver_apply_into_t(const ainv_solver *h, const double *x, double *y,
-                         int k, int64_t ld, void *t, hipStream_t s, int map) {
+                         int k, int64_t ld, void *t, hipStream_t s, int map,
+                         const int32_t *kw) {
   if (!h->have_t) throw Error(AINV_ERR_INTERNAL, "transposed layout missing");
   const bool f32t = h->prec != AINV_PREC_F64;
-  hll_spmm_scale(h->hzt, h->dv.p, x, false, ld, t, ld, k, s, map_w(map));
-  hll_spmm_axpby(h->hwt, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map));
+  hll_spmm_scale(h->hzt, h->dv.p, x, false, ld, t, ld, k, s, map_w(map), kw);
+  hll_spmm_axpby(h->hwt, 1.0, t, f32t, ld, 0.0, y, ld, k, s, map_z(map), kw);
 }
 
 int log2_pow2(int K) {

@@ -321,8 +321,20 @@ __global__ __launch_bounds__(kThreads) void k_fin(
   col[c] = cs;
   if ((fin == FIN_INIT || fin == FIN_INIT_CG || fin == FIN_CONV_BICG ||
        fin == FIN_CONV_CG) &&
-      cs.active)
+      cs.active) {
     atomicAdd(nactive, 1);
+    atomicMax(nactive + 1, c + 1);  // width for the next iteration
+  }
+}
+
+// First node of an iteration: the width summed since the last roll (the
+// columns active after the previous iteration and those admitted since)
+// becomes this iteration's active width.
+__global__ void k_kw_roll(int32_t *__restrict__ na) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    na[2] = na[1];
+    na[1] = 0;
+  }
 }
 
 int pow2_at_least(int k) {
@@ -448,7 +460,9 @@ struct KrylovPlan {
   Vecs v{};
   Col *col = nullptr;
   double *pt = nullptr;
-  int32_t *na = nullptr;
+  int32_t *na = nullptr;  // [0] active count, [1] width being summed,
+                          // [2] active width of this iteration
+  const int32_t *kwu = nullptr;  // na + 2 for the one-graph methods
   int32_t *pinned = nullptr;  // host copies of nactive, two slots
   hipEvent_t ev[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
@@ -534,11 +548,14 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   }
   p->colbuf.alloc(sizeof(Col) * kMaxCols);
   p->part.alloc(sizeof(double) * kMaxPairs * kRedBlocks * kMaxCols);
-  p->nact.alloc(sizeof(int32_t));
+  p->nact.alloc(3 * sizeof(int32_t));
+  AINV_HIP_CHECK(hipMemsetAsync(p->nact.p, 0, 3 * sizeof(int32_t), s));
   if (m) p->tbuf.alloc(solver_t_bytes(m, p->K));
   p->col = p->colbuf.as<Col>();
   p->pt = p->part.as<double>();
   p->na = p->nact.as<int32_t>();
+  // the one-graph methods skip the columns beyond the active width
+  if (method <= KM_BICG) p->kwu = p->na + 2;
   AINV_HIP_CHECK(hipHostMalloc((void **)&p->pinned, 2 * sizeof(int32_t),
                                hipHostMallocDefault));
   for (auto &e : p->ev)
@@ -550,7 +567,8 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
 
 void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
   if (p->m)
-    solver_apply_into(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map);
+    solver_apply_into(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map,
+                      p->kwu);
   else
     AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->K * sizeof(double),
                                   hipMemcpyDeviceToDevice, s));
@@ -559,7 +577,8 @@ void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
 void precond_t(const KrylovPlan *p, const double *x, double *y,
                hipStream_t s) {
   if (p->m)
-    solver_apply_into_t(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map);
+    solver_apply_into_t(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map,
+                        p->kwu);
   else
     AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->K * sizeof(double),
                                   hipMemcpyDeviceToDevice, s));
@@ -654,6 +673,9 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   const int32_t n = p->n;
   const int k = p->k, K = p->K, sm = p->spmv_map;
   const Vecs &v = p->v;
+  const int32_t *kw = p->kwu;
+  hipLaunchKernelGGL(k_kw_roll, dim3(1), dim3(64), 0, s, p->na);
+  AINV_HIP_CHECK(hipGetLastError());
   if (p->method == KM_BICG) {
     // vectors: q = rtilde, f = ztilde, s = ptilde, v = A p, t = A^T ptilde
     precond(p, v.r, v.z, s);
@@ -661,8 +683,8 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     int g = vec_op<OP_DOT_ZRT>(n, k, v, p->col, p->pt, s);
     fin(FIN_RHO_BICGM, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_P_BICGM>(n, k, v, p->col, p->pt, s);
-    g = hll_spmv_dot(a->hll, v.p, v.v, k, K, v.s, 1, p->pt, sm, s);
-    hll_spmm_axpby(a->hll_t, 1.0, v.s, false, K, 0.0, v.t, K, k, s, sm);
+    g = hll_spmv_dot(a->hll, v.p, v.v, k, K, v.s, 1, p->pt, sm, s, kw);
+    hll_spmm_axpby(a->hll_t, 1.0, v.s, false, K, 0.0, v.t, K, k, s, sm, kw);
     fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_BICGM>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_CG, k, g, p->pt, p->col, p->na, s);
@@ -672,22 +694,22 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     fin(FIN_RHO_CGS, k, 0, p->pt, p->col, p->na, s);
     vec_op<OP_U_CGS>(n, k, v, p->col, p->pt, s);
     precond(p, v.p, v.f, s);
-    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s);
+    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s, kw);
     fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_Q_CGS>(n, k, v, p->col, p->pt, s);
     precond(p, v.z, v.f, s);
-    hll_spmm_axpby(a->hll, 1.0, v.f, false, K, 0.0, v.v, K, k, s, sm);
+    hll_spmm_axpby(a->hll, 1.0, v.f, false, K, 0.0, v.v, K, k, s, sm, kw);
     g = vec_op<OP_XR_CGS>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
   } else if (!p->cg) {
     fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
     vec_op<OP_P_BICG>(n, k, v, p->col, p->pt, s);
     precond(p, v.p, v.f, s);
-    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s);
+    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s, kw);
     fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_S_BICG>(n, k, v, p->col, p->pt, s);
     precond(p, v.s, v.z, s);
-    g = hll_spmv_dot(a->hll, v.z, v.t, k, K, v.s, 2, p->pt, sm, s);
+    g = hll_spmv_dot(a->hll, v.z, v.t, k, K, v.s, 2, p->pt, sm, s, kw);
     fin(FIN_OMEGA, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_BICG>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
@@ -696,7 +718,7 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     int g = vec_op<OP_DOT_RZ>(n, k, v, p->col, p->pt, s);
     fin(FIN_RHO_CG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_P_CG>(n, k, v, p->col, p->pt, s);
-    g = hll_spmv_dot(a->hll, v.p, v.q, k, K, v.p, 1, p->pt, sm, s);
+    g = hll_spmv_dot(a->hll, v.p, v.q, k, K, v.p, 1, p->pt, sm, s, kw);
     fin(FIN_ALPHA_CG, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_CG>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_CG, k, g, p->pt, p->col, p->na, s);
@@ -783,7 +805,7 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
   for (auto &c0 : init) c0 = Col{};
   AINV_HIP_CHECK(hipMemcpyAsync(p->col, init.data(), sizeof(Col) * kMaxCols,
                                 hipMemcpyHostToDevice, s));
-  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, 3 * sizeof(int32_t), s));
   AINV_HIP_CHECK(hipMemsetAsync(p->work.p, 0, p->work.bytes, s));
   AINV_HIP_CHECK(hipStreamSynchronize(s));
   capture(p, a, s);
@@ -949,7 +971,7 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
   }
   AINV_HIP_CHECK(hipMemcpyAsync(p->col, init.data(), sizeof(Col) * kMaxCols,
                                 hipMemcpyHostToDevice, s));
-  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, 3 * sizeof(int32_t), s));
   // r = b - A x
   AINV_HIP_CHECK(hipMemcpyAsync(v.r, v.b, vb, hipMemcpyDeviceToDevice, s));
   hll_spmm_axpby(a->hll, -1.0, v.x, false, K, 1.0, v.r, K, k, s);
@@ -1382,7 +1404,7 @@ void gmres_solve(ainv_matrix *a, ainv_solver *m, const double *b, double *x,
   AINV_HIP_CHECK(hipMemcpyAsync(p->gmcol.p, init.data(),
                                 sizeof(GmCol) * kMaxCols,
                                 hipMemcpyHostToDevice, s));
-  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, 3 * sizeof(int32_t), s));
   const bool eager = (flags & AINV_KRYLOV_EAGER) != 0;
   // the init step uses the tuned system-matrix mapping, as the graphs do
   tune_plan(p, a, m, s);
@@ -1822,7 +1844,7 @@ void bicgstabl_solve(ainv_matrix *a, ainv_solver *m, const double *b,
   AINV_HIP_CHECK(hipMemcpyAsync(p->gmcol.p, init.data(),
                                 sizeof(BlCol) * kMaxCols,
                                 hipMemcpyHostToDevice, s));
-  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, sizeof(int32_t), s));
+  AINV_HIP_CHECK(hipMemsetAsync(p->na, 0, 3 * sizeof(int32_t), s));
   tune_plan(p, a, m, s);
   const BlArgs g0 = bl_args(p, 0, 0);
   // r0 = b - A x ; rt = r0 ; yhat = 0 ; uhat_0 = 0

@@ -88,13 +88,14 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
 // packed pair of the two passes' mappings (map_pair).
 void solver_apply_into(const ainv_solver *h, const double *x, double *y,
                        int k, int64_t ld, void *t, hipStream_t s,
-                       int map = 0);
+                       int map = 0, const int32_t *kw = nullptr);
 // Transposed layout for 'T' applies (built on first use; call before any
 // graph capture) and y = M^T x into a caller buffer t, like
 // solver_apply_into.
 void solver_ensure_transposed(ainv_solver *h);
 void solver_apply_into_t(const ainv_solver *h, const double *x, double *y,
-                         int k, int64_t ld, void *t, hipStream_t s, int map);
+                         int k, int64_t ld, void *t, hipStream_t s, int map,
+                         const int32_t *kw = nullptr);
 // Index of K = 2^i and the mapping pair the tuner chose for it (split if
 // none, the forced mapping for both passes under APPLY_MAP).
 int log2_pow2(int K);
