@@ -55,7 +55,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_w.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_coop_k32.json")
 
 # name: (description, solver kind, parameters, method, rtol, slots, queue,
 #        right-hand sides ("Ax": x_0 = 1, x_j ~ N(0,1); "ones+N": b_0 = 1,
@@ -94,16 +94,18 @@ def make_matrix(m, name, grid):
                         cap=400, band=5000, nlong=2, dom=1.1)
 
 
-def apply_pass_bytes(lay, k, first):
+def apply_pass_bytes(lay, k, first, xb=8):
     """Algorithmic bytes of one apply pass for k columns (padding
     excluded): factor words (value + int32 column), one row-extent word per
-    row, and for the first pass d, the FP64 x read and the t write; for the
-    second pass the t read and the FP64 y write (beta = 0).  t is FP32 for
-    FP32 / BF16 factors, FP64 otherwise."""
+    row, and for the first pass d, the x read (xb bytes per value: 8, or 4
+    for the FP32 copy BiCGSTAB passes with FP32 / BF16 factors) and the t
+    write; for the second pass the t read and the FP64 y write (beta = 0).
+    t is FP32 for FP32 / BF16 factors, FP64 otherwise."""
     n, vb = lay["n"], lay["value_bytes"]
     st = 8 if vb == 8 else 4
     if first:
-        return lay["nnz_w"] * (vb + 4) + n * 4 + n * st + k * n * 8 + k * n * st
+        return (lay["nnz_w"] * (vb + 4) + n * 4 + n * st + k * n * xb
+                + k * n * st)
     return lay["nnz_z"] * (vb + 4) + n * 4 + k * n * st + k * n * 8
 
 
@@ -116,6 +118,8 @@ def main():
     p.add_argument("--c5-nmax", type=float, default=2e5,
                    help="largest random matrix of the C5 pool")
     p.add_argument("--c5-requests", type=int, default=96)
+    p.add_argument("--c5-slots", type=int, default=0,
+                   help="SLOTS of every served matrix (0: server default)")
     p.add_argument("--c5-window", type=int, default=16,
                    help="outstanding requests of the closed-loop capacity run")
     p.add_argument("--workers", type=int, default=4)
@@ -174,10 +178,12 @@ def main():
         sv.set("APPLY_MAP", args.map)
         sv.build(a)
         lay = sv.layout()
-        ms_w, ms_z = profile_apply(m, sv, n, slots, 10)
-        bw = apply_pass_bytes(lay, slots, True)
+        x32 = method == "BICGSTAB" and args.prec != "F64"
+        ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32)
+        bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
         print(json.dumps({"apply_only": True, "config": args.config,
                           "prec": args.prec, "map": args.map, "k": slots,
+                          "x": "FP32" if x32 else "FP64",
                           "ms_w": ms_w, "ms_z": ms_z, "bytes_w": bw,
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
         return
@@ -273,31 +279,53 @@ def main():
                for j in range(0, M, max(1, M // 16)))
     rep = sv.report()
     tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
-    kmap = tuned[0]["map_w"] if tuned else "split"
+    kmap_w = tuned[0]["map_w"] if tuned else "split"
+    kmap_z = tuned[0]["map_z"] if tuned else "split"
     it_red = [int(iters[j]) for j in sample]
     log(f"served {args.prec} slots={slots}: {rate:.2f} solves/s, "
         f"{elapsed / args.steps:.2f} s/step, iterations mean "
         f"{float(np.mean(iters)):.1f} max {int(np.max(iters))}, tuning {tuned}")
 
-    ms_w, ms_z = profile_apply(m, sv, n, slots, 10)
-    bw = apply_pass_bytes(lay, slots, True)
-    gbs_w = bw / (ms_w * 1e-3) / 1e9
-    kernel, extra = {"split": ("k_hll_spmm", ""),
-                     "row": ("k_hll_spmm_row", ""),
-                     "coop": ("k_hll_spmm_coop", " 256,"),
-                     "coop128": ("k_hll_spmm_coop", " 128,")}[kmap]
+    # the two apply passes as the served form runs them (BiCGSTAB with FP32 /
+    # BF16 factors gives the first pass an FP32 x); the dominant kernel is
+    # the longer of the two
+    x32 = method == "BICGSTAB" and args.prec != "F64"
+    ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32)
+    bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
+    bz = apply_pass_bytes(lay, slots, False)
     vt = {"F64": "double", "F32": "float", "BF16": "unsigned short"}[args.prec]
     tt = "double" if args.prec == "F64" else "float"
-    kname = f"{kernel}<{vt}, double, {tt}, {tt}, {slots},{extra} 0>"
-    traffic = None
+    xt = "float" if x32 else "double"
+
+    def kname(kmap, first):
+        kernel, extra = {"split": ("k_hll_spmm", ""),
+                         "row": ("k_hll_spmm_row", ""),
+                         "coop": ("k_hll_spmm_coop", " 256,"),
+                         "coop128": ("k_hll_spmm_coop", " 128,")}[kmap]
+        if first:
+            return f"{kernel}<{vt}, {xt}, {tt}, {tt}, {slots},{extra} 0>"
+        return f"{kernel}<{vt}, {tt}, {tt}, double, {slots},{extra} 1>"
+
+    passes = [{"pass": "first: t = d o (W x)", "kernel": kname(kmap_w, True),
+               "bytes_per_launch": bw, "ms_per_launch": ms_w},
+              {"pass": "second: y = Z t", "kernel": kname(kmap_z, False),
+               "bytes_per_launch": bz, "ms_per_launch": ms_z}]
+    traffic = {}
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
-        if (tr.get("kernel") == kname and tr.get("config") == args.config
-                and tr.get("n") == n):
-            traffic = tr.get("bytes_per_launch")
-    log(f"apply at k={slots}: W {ms_w:.3f} ms ({gbs_w:.0f} GB/s), "
-        f"Z {ms_z:.3f} ms")
+        if tr.get("config") == args.config and tr.get("n") == n:
+            traffic = {e["kernel"]: e["bytes_per_launch"]
+                       for e in tr.get("kernels", [])}
+    for ps in passes:
+        ps["achieved_GBs"] = ps["bytes_per_launch"] / (ps["ms_per_launch"]
+                                                       * 1e-3) / 1e9
+        ps["traffic"] = traffic.get(ps["kernel"])
+    dom = max(passes, key=lambda q: q["ms_per_launch"])
+    other = passes[1] if dom is passes[0] else passes[0]
+    log(f"apply at k={slots}: W {ms_w:.3f} ms "
+        f"({passes[0]['achieved_GBs']:.0f} GB/s), Z {ms_z:.3f} ms "
+        f"({passes[1]['achieved_GBs']:.0f} GB/s)")
 
     cpu = None
     if not args.no_cpu:
@@ -345,14 +373,15 @@ def main():
         },
         "speedup_vs_fp64_single": rate / fp64_rate,
         "roofline": {
-            "bound": "hbm", "achieved": gbs_w, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": gbs_w / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": kname,
-            "bytes_per_launch": bw, "ms_per_launch": ms_w,
+            "bound": "hbm", "achieved": dom["achieved_GBs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": dom["traffic"],
+            "kernel": dom["kernel"], "pass": dom["pass"],
+            "bytes_per_launch": dom["bytes_per_launch"],
+            "ms_per_launch": dom["ms_per_launch"],
             "achievable_copy_GBs": copy_gbs,
-            "second_pass_ms": ms_z,
-            "second_pass_GBs": apply_pass_bytes(lay, slots, False) /
-            (ms_z * 1e-3) / 1e9,
+            "other_pass": other,
         },
         "build": {"seconds": t_build},
         "cpu_baseline": cpu,
@@ -380,6 +409,9 @@ def serve_bench(m, args):
     sizes = [a.shape[0] for _, a, _, _ in pool]
     log(f"C5 pool: {len(pool)} matrices, n from {min(sizes)} to {max(sizes)}")
     srv = m.Server(hbm_budget_bytes=200 << 30, workers=args.workers)
+    if args.c5_slots:
+        for _, _, _, prm in pool:
+            prm["SLOTS"] = args.c5_slots
     ids = [srv.add_matrix(a, kind, **prm) for _, a, kind, prm in pool]
     blocks = [m.gen_normal(500 + j, (sizes[j], 64)) for j in range(len(pool))]
     t0 = time.time()
@@ -418,8 +450,9 @@ def serve_bench(m, args):
     st_burst = srv.stats()
 
     # capacity: closed loop keeping `window` requests outstanding, each new
-    # one submitted when one completes (sustained throughput)
-    closed = requests(args.c5_requests, 9)
+    # one submitted when one completes (sustained throughput), over the same
+    # request list the open loop replays
+    closed = requests(args.c5_requests, 7)
     window = args.c5_window
     lock = threading.Lock()
     nxt = [0]
@@ -512,7 +545,8 @@ def serve_bench(m, args):
                         "BiCGSTAB 1e-8, requests of U{1..64} columns, Poisson "
                         "arrivals at 0.8x the closed-loop capacity",
             "name": "C5", "requests": len(reqs), "columns": cols,
-            "workers": args.workers, "parallelism": "1 GPU"},
+            "workers": args.workers, "slots": args.c5_slots or "adaptive",
+            "parallelism": "1 GPU"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)),
                        "p99": float(np.percentile(lat, 99)),
                        "note": "submission to completion per request "
@@ -533,13 +567,15 @@ def serve_bench(m, args):
     print(json.dumps(out))
 
 
-def profile_apply(m, sv, n, k, reps):
-    x = m.DeviceArray.from_numpy(m.gen_normal(11, (n, k)) if k > 1
-                                 else m.gen_normal(11, (n,)))
+def profile_apply(m, sv, n, k, reps, x32=False):
+    """Average durations (ms) of the two apply kernels on k columns; x32:
+    FP32 x (the input the served BiCGSTAB gives FP32 / BF16 factors)."""
+    xh = m.gen_normal(11, (n, k)) if k > 1 else m.gen_normal(11, (n,))
+    x = m.DeviceArray.from_numpy(xh.astype(np.float32) if x32 else xh)
     y = m.DeviceArray((n, k) if k > 1 else (n,))
     a, b = C.c_double(), C.c_double()
     m._ck(m.lib().ainv_apply_profile(sv._h, x.ptr, y.ptr, k, reps,
-                                     C.byref(a), C.byref(b)))
+                                     1 if x32 else 0, C.byref(a), C.byref(b)))
     x.free()
     y.free()
     return a.value, b.value

@@ -291,7 +291,9 @@ typedef struct ainv_server *ainv_server_h;
 int ainv_server_create(int device, int64_t hbm_budget_bytes, int workers,
                        ainv_server_h *srv);
 /* kind: AINV_KIND_*; params: "KEY=VALUE;..." with the keys of
- * ainv_cseti/csetr/csetc plus METHOD=BICGSTAB|CG|CGS|BICG.  Host CSR is
+ * ainv_cseti/csetr/csetc plus METHOD=BICGSTAB|CG|CGS|BICG and SLOTS=s (a
+ * batch on this matrix opens with at least s column slots, rounded up to a
+ * power of two; default: as many as the pending columns).  Host CSR is
  * copied. */
 int ainv_server_add_matrix(ainv_server_h srv, int32_t n, const int64_t *irp,
                            const int32_t *ja, const double *val,
@@ -322,9 +324,12 @@ int ainv_device_sync(void);
 /* Runs `reps` applies of h on device vectors and returns the average
  * duration (ms) of each of its two kernels, timed with HIP events on the
  * stream they are launched on: ms_first = d o (W x) (or Z^T for 'T'),
- * ms_second = alpha Z t + beta y. */
-int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
-                       double *ms_first, double *ms_second);
+ * ms_second = alpha Z t + beta y.  flags AINV_PROFILE_X32: x holds FP32
+ * values (FP32 / BF16 factors), the input the Krylov engine's BiCGSTAB
+ * passes its preconditioner. */
+#define AINV_PROFILE_X32 1
+int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
+                       int flags, double *ms_first, double *ms_second);
 /* Same for the system matrix product y = A x. */
 int ainv_mat_profile(ainv_mat_h a, const double *x, double *y, int k, int reps,
                      double *ms);

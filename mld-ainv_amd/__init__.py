@@ -108,7 +108,7 @@ def lib():
         "ainv_memcpy": [vp, vp, C.c_size_t, C.c_int],
         "ainv_memset": [vp, C.c_int, C.c_size_t],
         "ainv_device_sync": [],
-        "ainv_apply_profile": [vp, vp, vp, C.c_int, C.c_int, dp, dp],
+        "ainv_apply_profile": [vp, vp, vp, C.c_int, C.c_int, C.c_int, dp, dp],
         "ainv_mat_profile": [vp, vp, vp, C.c_int, C.c_int, dp],
         "ainv_krylov_profile": [C.c_char_p, vp, vp, vp, vp, C.c_int,
                                 C.c_double, C.c_int, C.c_int, i32p, dp, dp],
@@ -690,7 +690,8 @@ class Server:
 
     def add_matrix(self, a, kind="INVK", **params):
         """params: solver keys (SUB_FILLIN=1, FACTOR_PREC='F32', ...) and
-        METHOD='BICGSTAB'|'CG'|'CGS'|'BICG'.  Returns the matrix id."""
+        METHOD='BICGSTAB'|'CG'|'CGS'|'BICG', SLOTS=s (fewest column slots of
+        a batch).  Returns the matrix id."""
         n, irp, ja, val = _csr_args(a)
         spec = ";".join(f"{k}={v}" for k, v in params.items())
         mid = C.c_int32()

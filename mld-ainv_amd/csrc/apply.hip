@@ -1008,7 +1008,29 @@ void hll_spmm_scale(const Hll &a, const void *d, const void *x, bool x_is_f32,
                     int64_t ldx, void *out, int64_t ldo, int k, hipStream_t s,
                     int map, const int32_t *kw) {
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
-  if (x_is_f32) throw Error(AINV_ERR_INTERNAL, "scale pass reads FP64 x");
+  if (x_is_f32) {
+    // FP32 x (FP32 / BF16 factors only): FP32 t
+    if (a.prec == AINV_PREC_F64)
+      throw Error(AINV_ERR_INTERNAL, "FP64 factor with FP32 x");
+    const float *xf = static_cast<const float *>(x);
+    const float *df = static_cast<const float *>(d);
+    float *of = static_cast<float *>(out);
+    if (use_row(map, k, ldx, ldo, x, out)) {
+      if (a.prec == AINV_PREC_F32)
+        launch_lane<float, float, float, float, EPI_SCALE>(map, a, xf, k, df,
+                                                           of, 1.0, 0.0, s, kw);
+      else
+        launch_lane<uint16_t, float, float, float, EPI_SCALE>(
+            map, a, xf, k, df, of, 1.0, 0.0, s, kw);
+    } else if (a.prec == AINV_PREC_F32) {
+      launch_k<float, float, float, float, EPI_SCALE>(a, xf, ldx, k, df, of,
+                                                      ldo, 1.0, 0.0, s);
+    } else {
+      launch_k<uint16_t, float, float, float, EPI_SCALE>(a, xf, ldx, k, df, of,
+                                                         ldo, 1.0, 0.0, s);
+    }
+    return;
+  }
   const double *xd = static_cast<const double *>(x);
   if (use_row(map, k, ldx, ldo, x, out)) {
     switch (a.prec) {

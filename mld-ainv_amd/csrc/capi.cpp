@@ -1164,8 +1164,8 @@ struct Events {
 };
 }  // namespace
 
-int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
-                       double *ms_first, double *ms_second) {
+int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
+                       int flags, double *ms_first, double *ms_second) {
   return guard([&] {
     check_handle(h);
     std::lock_guard<std::mutex> lk(h->mu);
@@ -1174,6 +1174,9 @@ int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
     if (reps < 1 || k < 1 || k > 64 || !x || !y || !ms_first || !ms_second)
       throw Error(AINV_ERR_ARG, "bad profile arguments");
     const bool f32t = h->prec != AINV_PREC_F64;
+    const bool x32 = (flags & AINV_PROFILE_X32) != 0;
+    if (x32 && !f32t)
+      throw Error(AINV_ERR_ARG, "FP32 x needs FP32 or BF16 factors");
     const size_t need = (size_t)h->n * k * (f32t ? sizeof(float) : sizeof(double));
     if (h->tbuf.bytes < need) h->tbuf.alloc(need);
     hipStream_t s = h->stream;
@@ -1182,11 +1185,11 @@ int ainv_apply_profile(ainv_h h, const double *x, double *y, int k, int reps,
     const int map = tuned_apply_map(h, k);
     // one untimed pass first: first touches of the buffers stay out
     const int mw = map_w(map), mz = map_z(map);
-    hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, mw);
+    hll_spmm_scale(h->hw, h->dv.p, x, x32, k, h->tbuf.p, k, k, s, mw);
     hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, mz);
     for (int r = 0; r < reps; ++r) {
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r], s));
-      hll_spmm_scale(h->hw, h->dv.p, x, false, k, h->tbuf.p, k, k, s, mw);
+      hll_spmm_scale(h->hw, h->dv.p, x, x32, k, h->tbuf.p, k, k, s, mw);
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 1], s));
       hll_spmm_axpby(h->hz, 1.0, h->tbuf.p, f32t, k, 0.0, y, k, k, s, mz);
       AINV_HIP_CHECK(hipEventRecord(e.ev[3 * r + 2], s));

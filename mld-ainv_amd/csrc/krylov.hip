@@ -42,6 +42,9 @@ struct Col {
 
 struct Vecs {
   double *x, *b, *r, *q, *p, *v, *f, *s, *z, *t;
+  // BiCGSTAB with FP32 / BF16 factors: FP32 copies of p and s, the
+  // preconditioner's inputs, written by the passes that form p and s
+  float *p32, *s32;
 };
 
 enum Op {
@@ -96,12 +99,15 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         acc[1] += v.r[e] * v.r[e];
         acc[2] += v.q[e] * v.r[e];
       } else if constexpr (OP == OP_P_BICG) {
-        if (cs.iter == 1)
-          v.p[e] = v.r[e];
-        else
-          v.p[e] = v.r[e] + cs.beta * (v.p[e] + (-cs.omega) * v.v[e]);
+        const double pn =
+            cs.iter == 1 ? v.r[e]
+                         : v.r[e] + cs.beta * (v.p[e] + (-cs.omega) * v.v[e]);
+        v.p[e] = pn;
+        if (v.p32) v.p32[e] = (float)pn;
       } else if constexpr (OP == OP_S_BICG) {
-        v.s[e] = v.r[e] + (-cs.alpha) * v.v[e];
+        const double sn = v.r[e] + (-cs.alpha) * v.v[e];
+        v.s[e] = sn;
+        if (v.s32) v.s32[e] = (float)sn;
       } else if constexpr (OP == OP_XR_BICG) {
         v.x[e] = v.x[e] + cs.alpha * v.f[e] + cs.omega * v.z[e];
         const double rn = v.s[e] + (-cs.omega) * v.t[e];
@@ -475,6 +481,7 @@ struct KrylovPlan {
   // of a cycle plus one for the cycle's end
   int irst = 0;
   DevBuf basis, gmcol, gmarr;
+  DevBuf work32;  // BiCGSTAB: FP32 p and s for FP32 / BF16 factors
   std::vector<hipGraphExec_t> gm_exec;
 
   void release_graph() {
@@ -499,6 +506,15 @@ void destroy_plan(KrylovPlan *p) { delete p; }
 
 namespace {
 
+void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
+  if (p->method != KM_BICGSTAB || !m || m->prec == AINV_PREC_F64 ||
+      p->work32.p)
+    return;
+  p->work32.alloc((size_t)2 * p->n * p->K * sizeof(float));
+  // padding columns are read by the row-mapped kernels: keep them 0
+  AINV_HIP_CHECK(hipMemsetAsync(p->work32.p, 0, p->work32.bytes, s));
+}
+
 KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
                      hipStream_t s, int irst = 0) {
   for (size_t i = 0; i < a->plans.size(); ++i) {
@@ -508,6 +524,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
         p->release_graph();  // factors changed: recapture
         if (p->tbuf.bytes < solver_t_bytes(m, p->K))
           p->tbuf.alloc(solver_t_bytes(m, p->K));
+        alloc_x32(p, m, s);
         p->gen = m->generation;
       }
       if (i != 0) std::rotate(a->plans.begin(), a->plans.begin() + i,
@@ -551,6 +568,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   p->nact.alloc(3 * sizeof(int32_t));
   AINV_HIP_CHECK(hipMemsetAsync(p->nact.p, 0, 3 * sizeof(int32_t), s));
   if (m) p->tbuf.alloc(solver_t_bytes(m, p->K));
+  alloc_x32(p, m, s);
   p->col = p->colbuf.as<Col>();
   p->pt = p->part.as<double>();
   p->na = p->nact.as<int32_t>();
@@ -572,6 +590,16 @@ void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
   else
     AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->K * sizeof(double),
                                   hipMemcpyDeviceToDevice, s));
+}
+
+// y = M x for an FP32 copy of x (FP32 / BF16 factors)
+void precond32(const KrylovPlan *p, const float *x, double *y, hipStream_t s) {
+  const ainv_solver *m = p->m;
+  const bool f32t = m->prec != AINV_PREC_F64;
+  hll_spmm_scale(m->hw, m->dv.p, x, true, p->K, p->tbuf.p, p->K, p->k, s,
+                 map_w(p->apply_map), p->kwu);
+  hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, p->K, 0.0, y, p->K, p->k, s,
+                 map_z(p->apply_map), p->kwu);
 }
 
 void precond_t(const KrylovPlan *p, const double *x, double *y,
@@ -702,13 +730,27 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     g = vec_op<OP_XR_CGS>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
   } else if (!p->cg) {
+    // with FP32 / BF16 factors the preconditioner reads FP32 copies of p
+    // and s (half the gathered bytes of its first pass)
+    Vecs w = v;
+    const bool x32 = p->m && p->m->prec != AINV_PREC_F64 && p->work32.p;
+    if (x32) {
+      w.p32 = p->work32.as<float>();
+      w.s32 = w.p32 + (int64_t)n * K;
+    }
     fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
-    vec_op<OP_P_BICG>(n, k, v, p->col, p->pt, s);
-    precond(p, v.p, v.f, s);
+    vec_op<OP_P_BICG>(n, k, w, p->col, p->pt, s);
+    if (x32)
+      precond32(p, w.p32, v.f, s);
+    else
+      precond(p, v.p, v.f, s);
     int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s, kw);
     fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
-    vec_op<OP_S_BICG>(n, k, v, p->col, p->pt, s);
-    precond(p, v.s, v.z, s);
+    vec_op<OP_S_BICG>(n, k, w, p->col, p->pt, s);
+    if (x32)
+      precond32(p, w.s32, v.z, s);
+    else
+      precond(p, v.s, v.z, s);
     g = hll_spmv_dot(a->hll, v.z, v.t, k, K, v.s, 2, p->pt, sm, s, kw);
     fin(FIN_OMEGA, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_BICG>(n, k, v, p->col, p->pt, s);

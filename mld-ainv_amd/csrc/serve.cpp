@@ -11,6 +11,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <thread>
@@ -57,6 +58,7 @@ struct ServedMatrix {
   std::vector<std::pair<std::string, std::string>> params;
   int kind = AINV_KIND_INVK;
   int method = KM_BICGSTAB;
+  int min_slots = 1;  // SLOTS: fewest column slots a batch opens with
   ainv_matrix *A = nullptr;
   ainv_solver *M = nullptr;
   int64_t bytes = 0;
@@ -187,7 +189,7 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
     std::unique_lock<std::mutex> lk(S->mu);
     int cols = 0;
     for (int64_t id : m->pending) cols += S->reqs[id]->k;
-    cols = std::min(cols, kMaxCols);
+    cols = std::min(std::max(cols, m->min_slots), kMaxCols);
     while (K < cols) K <<= 1;
   }
   ensure_resident(S, m, K);
@@ -429,6 +431,11 @@ int ainv_server_add_matrix(ainv_server_h S, int32_t n, const int64_t *irp,
         std::string v = kv.second;
         for (char &c : v) c = (char)std::toupper((unsigned char)c);
         m->method = krylov_method_id(v);
+      } else if (k == "SLOTS") {
+        const int v = std::atoi(kv.second.c_str());
+        if (v < 1 || v > kMaxCols)
+          throw Error(AINV_ERR_ARG, "SLOTS must be 1..64");
+        m->min_slots = v;
       } else {
         m->params.emplace_back(k, kv.second);
       }
