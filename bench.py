@@ -98,7 +98,7 @@ def apply_pass_bytes(lay, k, first, xb=8):
     """Algorithmic bytes of one apply pass for k columns (padding
     excluded): factor words (value + int32 column), one row-extent word per
     row, and for the first pass d, the x read (xb bytes per value: 8, or 4
-    for the FP32 copy BiCGSTAB passes with FP32 / BF16 factors) and the t
+    for the FP32 copy BiCGSTAB and CG pass with FP32 / BF16 factors) and the t
     write; for the second pass the t read and the FP64 y write (beta = 0).
     t is FP32 for FP32 / BF16 factors, FP64 otherwise."""
     n, vb = lay["n"], lay["value_bytes"]
@@ -178,7 +178,7 @@ def main():
         sv.set("APPLY_MAP", args.map)
         sv.build(a)
         lay = sv.layout()
-        x32 = method == "BICGSTAB" and args.prec != "F64"
+        x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
         ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32)
         bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
         print(json.dumps({"apply_only": True, "config": args.config,
@@ -286,10 +286,10 @@ def main():
         f"{elapsed / args.steps:.2f} s/step, iterations mean "
         f"{float(np.mean(iters)):.1f} max {int(np.max(iters))}, tuning {tuned}")
 
-    # the two apply passes as the served form runs them (BiCGSTAB with FP32 /
-    # BF16 factors gives the first pass an FP32 x); the dominant kernel is
-    # the longer of the two
-    x32 = method == "BICGSTAB" and args.prec != "F64"
+    # the two apply passes as the served form runs them (BiCGSTAB and CG with
+    # FP32 / BF16 factors give the first pass an FP32 x); the dominant kernel
+    # is the longer of the two
+    x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
     ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32)
     bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
     bz = apply_pass_bytes(lay, slots, False)

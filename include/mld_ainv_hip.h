@@ -325,8 +325,8 @@ int ainv_device_sync(void);
  * duration (ms) of each of its two kernels, timed with HIP events on the
  * stream they are launched on: ms_first = d o (W x) (or Z^T for 'T'),
  * ms_second = alpha Z t + beta y.  flags AINV_PROFILE_X32: x holds FP32
- * values (FP32 / BF16 factors), the input the Krylov engine's BiCGSTAB
- * passes its preconditioner. */
+ * values (FP32 / BF16 factors), the input the Krylov engine's BiCGSTAB and
+ * CG pass their preconditioner. */
 #define AINV_PROFILE_X32 1
 int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
                        int flags, double *ms_first, double *ms_second);

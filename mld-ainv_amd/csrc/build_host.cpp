@@ -620,11 +620,37 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
   if (threads <= 0) threads = build_threads();
   // bounded columns (a fill cap) can be built by the pipelined passes
   const bool bounded = fill_in >= 1 || var.list_sparsify;
+  // The pipelined passes pay off only when a column's dependencies lie
+  // well behind it: with a neighbour at distance 1 (stencil orderings)
+  // column i needs column i-1 and the pipeline runs one column at a time
+  // while paying for the hand-offs.  Median over sampled columns of the
+  // distance to the nearest earlier row or column coupled to them.
+  if (threads >= 2 && n > 1) {
+    std::vector<int32_t> dist;
+    // scattered sample (a regular stride can land on one grid boundary)
+    for (int64_t t = 1; t <= 1024; ++t) {
+      const int32_t i = (int32_t)((t * 2654435761LL) % n);
+      if (i == 0) continue;
+      int32_t best = i;  // none: as far as it gets
+      for (int64_t e = a.ptr[i]; e < a.ptr[i + 1]; ++e)
+        if (a.idx[e] < i) best = std::min(best, i - a.idx[e]);
+      for (int64_t e = at.ptr[i]; e < at.ptr[i + 1]; ++e)
+        if (at.idx[e] < i) best = std::min(best, i - at.idx[e]);
+      dist.push_back(best);
+    }
+    if (!dist.empty()) {
+      std::nth_element(dist.begin(), dist.begin() + dist.size() / 2,
+                       dist.end());
+      if (dist[dist.size() / 2] < 8 * threads) threads = 2;
+    }
+  }
   const bool two = alg != AINV_ALG_S_LLK;
   bool done = false;
-  if (!var.stabilized && n > 1 && threads >= 2 && bounded) {
+  if (!var.stabilized && n > 1 && threads >= 2 && bounded &&
+      (two || threads > 2)) {
     // Without stabilization the Z pass (pivots p) and the W pass (pivots q)
-    // share nothing: each runs pipelined on its share of the threads.
+    // share nothing: each runs pipelined on its share of the threads (one
+    // thread each: the plain column loop, concurrently).
     const int tz = two ? std::max(1, (threads + 1) / 2) : threads;
     const int tw = std::max(1, threads - tz);
     std::exception_ptr err;

@@ -42,9 +42,9 @@ struct Col {
 
 struct Vecs {
   double *x, *b, *r, *q, *p, *v, *f, *s, *z, *t;
-  // BiCGSTAB with FP32 / BF16 factors: FP32 copies of p and s, the
-  // preconditioner's inputs, written by the passes that form p and s
-  float *p32, *s32;
+  // FP32 / BF16 factors: FP32 copies of the preconditioner's inputs,
+  // written by the passes that form them (BiCGSTAB: p and s; CG: r)
+  float *p32, *s32, *r32;
 };
 
 enum Op {
@@ -117,6 +117,7 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
       } else if constexpr (OP == OP_INIT_CG) {
         acc[0] += v.b[e] * v.b[e];
         acc[1] += v.r[e] * v.r[e];
+        if (v.r32) v.r32[e] = (float)v.r[e];
       } else if constexpr (OP == OP_DOT_RZ) {
         acc[0] += v.r[e] * v.z[e];
       } else if constexpr (OP == OP_P_CG) {
@@ -128,6 +129,7 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         v.x[e] = v.x[e] + cs.alpha * v.p[e];
         const double rn = v.r[e] + (-cs.alpha) * v.q[e];
         v.r[e] = rn;
+        if (v.r32) v.r32[e] = (float)rn;
         acc[0] += rn * rn;
       } else if constexpr (OP == OP_U_CGS) {
         // u in s, the q of CGS in t
@@ -399,7 +401,7 @@ struct AdmitList {
 
 __global__ void k_admit(int32_t n, int K, AdmitList L, double *__restrict__ b,
                         double *__restrict__ x, double *__restrict__ r,
-                        double *__restrict__ q) {
+                        double *__restrict__ q, float *__restrict__ r32) {
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= n) return;
   for (int j = 0; j < L.cnt; ++j) {
@@ -409,6 +411,7 @@ __global__ void k_admit(int32_t n, int K, AdmitList L, double *__restrict__ b,
     x[e] = 0.0;
     r[e] = v;
     if (q) q[e] = v;
+    if (r32) r32[e] = (float)v;
   }
 }
 
@@ -507,12 +510,28 @@ void destroy_plan(KrylovPlan *p) { delete p; }
 namespace {
 
 void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
-  if (p->method != KM_BICGSTAB || !m || m->prec == AINV_PREC_F64 ||
-      p->work32.p)
+  if ((p->method != KM_BICGSTAB && p->method != KM_CG) || !m ||
+      m->prec == AINV_PREC_F64 || p->work32.p)
     return;
   p->work32.alloc((size_t)2 * p->n * p->K * sizeof(float));
   // padding columns are read by the row-mapped kernels: keep them 0
   AINV_HIP_CHECK(hipMemsetAsync(p->work32.p, 0, p->work32.bytes, s));
+}
+
+// The plan's vectors with the FP32 copies the method writes when its
+// preconditioner has FP32 / BF16 factors (none otherwise).
+Vecs x32_vecs(const KrylovPlan *p) {
+  Vecs w = p->v;
+  if (p->m && p->m->prec != AINV_PREC_F64 && p->work32.p) {
+    float *f = p->work32.as<float>();
+    if (p->method == KM_BICGSTAB) {
+      w.p32 = f;
+      w.s32 = f + (int64_t)p->n * p->K;
+    } else if (p->method == KM_CG) {
+      w.r32 = f;
+    }
+  }
+  return w;
 }
 
 KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
@@ -732,12 +751,8 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   } else if (!p->cg) {
     // with FP32 / BF16 factors the preconditioner reads FP32 copies of p
     // and s (half the gathered bytes of its first pass)
-    Vecs w = v;
-    const bool x32 = p->m && p->m->prec != AINV_PREC_F64 && p->work32.p;
-    if (x32) {
-      w.p32 = p->work32.as<float>();
-      w.s32 = w.p32 + (int64_t)n * K;
-    }
+    const Vecs w = x32_vecs(p);
+    const bool x32 = w.p32 != nullptr;
     fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
     vec_op<OP_P_BICG>(n, k, w, p->col, p->pt, s);
     if (x32)
@@ -756,13 +771,17 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     g = vec_op<OP_XR_BICG>(n, k, v, p->col, p->pt, s);
     fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
   } else {
-    precond(p, v.r, v.z, s);
+    const Vecs w = x32_vecs(p);
+    if (w.r32)
+      precond32(p, w.r32, v.z, s);
+    else
+      precond(p, v.r, v.z, s);
     int g = vec_op<OP_DOT_RZ>(n, k, v, p->col, p->pt, s);
     fin(FIN_RHO_CG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_P_CG>(n, k, v, p->col, p->pt, s);
     g = hll_spmv_dot(a->hll, v.p, v.q, k, K, v.p, 1, p->pt, sm, s, kw);
     fin(FIN_ALPHA_CG, k, g, p->pt, p->col, p->na, s);
-    g = vec_op<OP_XR_CG>(n, k, v, p->col, p->pt, s);
+    g = vec_op<OP_XR_CG>(n, k, w, p->col, p->pt, s);
     fin(FIN_CONV_CG, k, g, p->pt, p->col, p->na, s);
   }
 }
@@ -874,7 +893,7 @@ void admit(SlotBatch *b, const AdmitList &L, double eps) {
   if (L.cnt == 0) return;
   hipLaunchKernelGGL(k_admit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      b->s, n, b->K, L, p->v.b, p->v.x, p->v.r,
-                     p->cg ? nullptr : p->v.q);
+                     p->cg ? nullptr : p->v.q, x32_vecs(p).r32);
   AINV_HIP_CHECK(hipGetLastError());
   const int g = vec_blocks(n, 1);
   const int fin_id = p->cg ? FIN_INIT_CG : FIN_INIT;
@@ -1023,7 +1042,7 @@ void krylov_solve(const std::string &method, ainv_matrix *a, ainv_solver *m,
     g = vec_op<OP_INIT_BICG>(n, k, v, p->col, p->pt, s);
     fin(FIN_INIT, k, g, p->pt, p->col, p->na, s);
   } else {
-    g = vec_op<OP_INIT_CG>(n, k, v, p->col, p->pt, s);
+    g = vec_op<OP_INIT_CG>(n, k, x32_vecs(p), p->col, p->pt, s);
     fin(FIN_INIT_CG, k, g, p->pt, p->col, p->na, s);
   }
   AINV_HIP_CHECK(hipMemcpyAsync(p->pinned, p->na, sizeof(int32_t),

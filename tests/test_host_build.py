@@ -35,9 +35,11 @@ def test_host_ainv_matches_oracle(mld, O, alg, fill, thresh, threads):
 
 @pytest.mark.parametrize("threads", [2, 5, 8])
 def test_host_ainv_random_matrix_threads(mld, O, threads):
-    """A random banded matrix (columns within +-300 plus long-range ones):
-    the pipelined passes run far ahead of the columns they wait for."""
-    a = mld.gen_random(6000, seed=41, median=8.0, band=300, nlong=2)
+    """A random banded matrix (columns within +-5000 plus long-range ones,
+    so the nearest coupled column is hundreds of columns back and the
+    pipelined passes run): the passes run far ahead of the columns they
+    wait for."""
+    a = mld.gen_random(20000, seed=41, median=8.0, band=5000, nlong=2)
     w1, z1, d1 = mld.host_ainv_bld(a, "LLK", 8, 5e-2, 1)
     wt, zt, dt = mld.host_ainv_bld(a, "LLK", 8, 5e-2, threads)
     assert same(w1, wt) and same(z1, zt) and np.array_equal(d1, dt)
