@@ -94,19 +94,20 @@ def make_matrix(m, name, grid):
                         cap=400, band=5000, nlong=2, dom=1.1)
 
 
-def apply_pass_bytes(lay, k, first, xb=8):
+def apply_pass_bytes(lay, k, first, xb=8, yb=8):
     """Algorithmic bytes of one apply pass for k columns (padding
     excluded): factor words (value + int32 column), one row-extent word per
     row, and for the first pass d, the x read (xb bytes per value: 8, or 4
     for the FP32 copy BiCGSTAB and CG pass with FP32 / BF16 factors) and the t
-    write; for the second pass the t read and the FP64 y write (beta = 0).
+    write; for the second pass the t read and the y write (yb bytes per
+    value: 8, or 4 for BiCGSTAB's FP32 preconditioned vectors; beta = 0).
     t is FP32 for FP32 / BF16 factors, FP64 otherwise."""
     n, vb = lay["n"], lay["value_bytes"]
     st = 8 if vb == 8 else 4
     if first:
         return (lay["nnz_w"] * (vb + 4) + n * 4 + n * st + k * n * xb
                 + k * n * st)
-    return lay["nnz_z"] * (vb + 4) + n * 4 + k * n * st + k * n * 8
+    return lay["nnz_z"] * (vb + 4) + n * 4 + k * n * st + k * n * yb
 
 
 def main():
@@ -179,7 +180,8 @@ def main():
         sv.build(a)
         lay = sv.layout()
         x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
-        ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32)
+        y32 = method == "BICGSTAB" and args.prec != "F64"
+        ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32, y32)
         bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
         print(json.dumps({"apply_only": True, "config": args.config,
                           "prec": args.prec, "map": args.map, "k": slots,
@@ -290,9 +292,10 @@ def main():
     # FP32 / BF16 factors give the first pass an FP32 x); the dominant kernel
     # is the longer of the two
     x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
-    ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32)
+    y32 = method == "BICGSTAB" and args.prec != "F64"
+    ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32, y32)
     bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
-    bz = apply_pass_bytes(lay, slots, False)
+    bz = apply_pass_bytes(lay, slots, False, 8, 4 if y32 else 8)
     vt = {"F64": "double", "F32": "float", "BF16": "unsigned short"}[args.prec]
     tt = "double" if args.prec == "F64" else "float"
     xt = "float" if x32 else "double"
@@ -304,7 +307,8 @@ def main():
                          "coop128": ("k_hll_spmm_coop", " 128,")}[kmap]
         if first:
             return f"{kernel}<{vt}, {xt}, {tt}, {tt}, {slots},{extra} 0>"
-        return f"{kernel}<{vt}, {tt}, {tt}, double, {slots},{extra} 1>"
+        yt = "float" if y32 else "double"
+        return f"{kernel}<{vt}, {tt}, {tt}, {yt}, {slots},{extra} 1>"
 
     passes = [{"pass": "first: t = d o (W x)", "kernel": kname(kmap_w, True),
                "bytes_per_launch": bw, "ms_per_launch": ms_w},
@@ -567,15 +571,17 @@ def serve_bench(m, args):
     print(json.dumps(out))
 
 
-def profile_apply(m, sv, n, k, reps, x32=False):
+def profile_apply(m, sv, n, k, reps, x32=False, y32=False):
     """Average durations (ms) of the two apply kernels on k columns; x32:
-    FP32 x (the input the served BiCGSTAB gives FP32 / BF16 factors)."""
+    FP32 x (the input the served BiCGSTAB and CG give FP32 / BF16 factors),
+    y32: FP32 y (BiCGSTAB's preconditioned vectors)."""
     xh = m.gen_normal(11, (n, k)) if k > 1 else m.gen_normal(11, (n,))
     x = m.DeviceArray.from_numpy(xh.astype(np.float32) if x32 else xh)
     y = m.DeviceArray((n, k) if k > 1 else (n,))
     a, b = C.c_double(), C.c_double()
     m._ck(m.lib().ainv_apply_profile(sv._h, x.ptr, y.ptr, k, reps,
-                                     1 if x32 else 0, C.byref(a), C.byref(b)))
+                                     (1 if x32 else 0) | (2 if y32 else 0),
+                                     C.byref(a), C.byref(b)))
     x.free()
     y.free()
     return a.value, b.value

@@ -324,10 +324,12 @@ int ainv_device_sync(void);
 /* Runs `reps` applies of h on device vectors and returns the average
  * duration (ms) of each of its two kernels, timed with HIP events on the
  * stream they are launched on: ms_first = d o (W x) (or Z^T for 'T'),
- * ms_second = alpha Z t + beta y.  flags AINV_PROFILE_X32: x holds FP32
- * values (FP32 / BF16 factors), the input the Krylov engine's BiCGSTAB and
- * CG pass their preconditioner. */
+ * ms_second = alpha Z t + beta y.  flags (FP32 / BF16 factors only):
+ * AINV_PROFILE_X32, x holds FP32 values (the input the Krylov engine's
+ * BiCGSTAB and CG give their preconditioner); AINV_PROFILE_Y32, y gets FP32
+ * values (BiCGSTAB's preconditioned vectors). */
 #define AINV_PROFILE_X32 1
+#define AINV_PROFILE_Y32 2
 int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
                        int flags, double *ms_first, double *ms_second);
 /* Same for the system matrix product y = A x. */

@@ -147,6 +147,11 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, double *y, int64_t ldy, int k,
                     hipStream_t s, int map = AINV_MAP_SPLIT,
                     const int32_t *kw = nullptr);
+// FP32 y from FP32 x (FP32 / BF16 factors only)
+void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
+                    int64_t ldx, double beta, float *y, int64_t ldy, int k,
+                    hipStream_t s, int map = AINV_MAP_SPLIT,
+                    const int32_t *kw = nullptr);
 // Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *
 // kMaxCols + c] for pair q, block b, column c.
 constexpr int kRedBlocks = 1024;
@@ -155,6 +160,11 @@ constexpr int kMaxCols = 64;
 // products of its result: np = 1 gives (w, y), np = 2 gives (y, y) and
 // (y, w), per column, as block partials.  Returns the number of blocks.
 int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
+                 const double *w, int np, double *part, int map,
+                 hipStream_t s, const int32_t *kw = nullptr);
+// the same product of an FP32 x (the Krylov engine's FP32 preconditioned
+// vectors); y, w and the sums stay FP64
+int hll_spmv_dot(const Hll &a, const float *x, double *y, int k, int64_t ld,
                  const double *w, int np, double *part, int map,
                  hipStream_t s, const int32_t *kw = nullptr);
 

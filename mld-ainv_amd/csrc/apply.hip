@@ -418,11 +418,11 @@ void launch_k(const Hll &a, const XT *x, int64_t ldx, int k, const DT *d,
 // slices with a fixed grid (at most kRedBlocks blocks), per-column partial
 // sums reduced across the row groups of a wave by shuffles and across the
 // waves of a block in LDS, in a fixed order.
-template <int K, int NP>
+template <typename XT, int K, int NP>
 __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ rlen, const int32_t *__restrict__ col,
-    const double *__restrict__ val, const double *__restrict__ x, int k,
+    const double *__restrict__ val, const XT *__restrict__ x, int k,
     int64_t ld, double *__restrict__ y, const double *__restrict__ w,
     double *__restrict__ part) {
   constexpr int G = kSlice / K;
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
       if (r >= n) break;
       const int32_t len = rlen[r];
       const double yv =
-          row_dot<double, double>(base + ri, len, col, val, x, ld, c, col_ok);
+          row_dot<double, XT>(base + ri, len, col, val, x, ld, c, col_ok);
       if (!col_ok) continue;
       const int64_t e = (int64_t)r * ld + c;
       y[e] = yv;
@@ -481,11 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
 
 // Row mapping of the fused SpMV + dots (ld = K): each lane keeps K
 // partial sums for its rows and the wave reduces them once at the end.
-template <int K, int NP>
+template <typename XT, int K, int NP>
 __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_row(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const double *__restrict__ val,
-    const double *__restrict__ x, int k, double *__restrict__ y,
+    const XT *__restrict__ x, int k, double *__restrict__ y,
     const double *__restrict__ w, double *__restrict__ part) {
   const int lane = threadIdx.x & (kSlice - 1);
   const int wave = threadIdx.x >> 6;
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_row(
     double acc[K];
 #pragma unroll
     for (int u = 0; u < K; ++u) acc[u] = 0.0;
-    row_acc<double, double, K>(base + lane, width, col, val, x, acc);
+    row_acc<double, XT, K>(base + lane, width, col, val, x, acc);
     const int32_t r = s * kSlice + lane;
     if (r >= n) continue;
     double wr[K];
@@ -735,19 +735,19 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
 // lane chunk hold partial sums of the same columns; they are reduced by
 // shuffles across the row groups, then across the waves in LDS; a block
 // writes zero partial sums for the columns of the other chunks.
-template <int K, int CB, int NP>
-__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(double) void
+template <typename XT, int K, int CB, int NP>
+__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void
 k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
                     const int64_t *__restrict__ sptr,
                     const int32_t *__restrict__ col,
                     const double *__restrict__ val,
-                    const double *__restrict__ x, int k,
+                    const XT *__restrict__ x, int k,
                     double *__restrict__ y, const double *__restrict__ w,
                     double *__restrict__ part,
                     const int32_t *__restrict__ kwp) {
-  using CH = CoopChunk<double, K, CB>;
+  using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC, NCH = CH::NCH;
-  using CP = Coop<double, KC>;
+  using CP = Coop<XT, KC>;
   constexpr int L = CP::L, CPL = CP::CPL, R = CP::R;
   const int lane = threadIdx.x & (kSlice - 1);
   const int wave = threadIdx.x >> 6;
@@ -785,8 +785,8 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
     for (int q = 0; q < L; ++q)
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
-    coop_acc<double, double, K, KC>(base, width, lane, rg, col, val, x + c0,
-                                    c0 < kw, acc);
+    coop_acc<double, XT, K, KC>(base, width, lane, rg, col, val, x + c0,
+                                c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
       const int32_t r = s * kSlice + q * R + rg;
@@ -888,9 +888,10 @@ bool aligned16(const void *p) {
 
 }  // namespace
 
-int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
-                 const double *w, int np, double *part, int map,
-                 hipStream_t s, const int32_t *kw) {
+template <typename XT>
+int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
+                  const double *w, int np, double *part, int map,
+                  hipStream_t s, const int32_t *kw) {
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
@@ -910,15 +911,15 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
 #define AINV_LR(KK, NPP)                                                     \
   do {                                                                       \
     if (map == AINV_MAP_COOP)                                                \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 256, NPP>), dim3(g),       \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part, kw);                                 \
     else if (map == AINV_MAP_COOP128)                                        \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 128, NPP>), dim3(g),       \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part, kw);                                 \
     else                                                                     \
-      hipLaunchKernelGGL((k_hll_spmv_dot_row<KK, NPP>), dim3(g),             \
+      hipLaunchKernelGGL((k_hll_spmv_dot_row<XT, KK, NPP>), dim3(g),             \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part);                                     \
   } while (0)
@@ -926,11 +927,11 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
 #define AINV_LC(KK, NPP)                                                     \
   do {                                                                       \
     if (map == AINV_MAP_COOP128)                                             \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 128, NPP>), dim3(g),       \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part, kw);                                 \
     else                                                                     \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<KK, 256, NPP>), dim3(g),       \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part, kw);                                 \
   } while (0)
@@ -959,7 +960,7 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
     return g;
   }
 #define AINV_LD(KK, NPP)                                                     \
-  hipLaunchKernelGGL((k_hll_spmv_dot<KK, NPP>), dim3(g), dim3(kBlock), 0, s, \
+  hipLaunchKernelGGL((k_hll_spmv_dot<XT, KK, NPP>), dim3(g), dim3(kBlock), 0, s, \
                      a.n, a.nslices, sp, rl, cl, vl, x, k, ld, y, w, part)
 #define AINV_LDK(NPP)        \
   if (k <= 1)                \
@@ -985,6 +986,18 @@ int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
 #undef AINV_LD
   AINV_HIP_CHECK(hipGetLastError());
   return g;
+}
+
+int hll_spmv_dot(const Hll &a, const double *x, double *y, int k, int64_t ld,
+                 const double *w, int np, double *part, int map,
+                 hipStream_t s, const int32_t *kw) {
+  return spmv_dot_impl<double>(a, x, y, k, ld, w, np, part, map, s, kw);
+}
+
+int hll_spmv_dot(const Hll &a, const float *x, double *y, int k, int64_t ld,
+                 const double *w, int np, double *part, int map,
+                 hipStream_t s, const int32_t *kw) {
+  return spmv_dot_impl<float>(a, x, y, k, ld, w, np, part, map, s, kw);
 }
 
 // Row mapping needs ld = K (a power of two) and 16-byte aligned x and
@@ -1129,6 +1142,31 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
         launch_k<uint16_t, double, float, double, EPI_AXPBY>(
             a, static_cast<const double *>(x), ldx, k, (const float *)nullptr,
             y, ldy, alpha, beta, s);
+  }
+}
+
+// FP32 output y = alpha Z t + beta y from an FP32 t (FP32 / BF16 factors):
+// the preconditioner output the Krylov engine keeps in FP32.
+void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
+                    int64_t ldx, double beta, float *y, int64_t ldy, int k,
+                    hipStream_t s, int map, const int32_t *kw) {
+  if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
+  if (a.prec == AINV_PREC_F64 || !x_is_f32)
+    throw Error(AINV_ERR_INTERNAL, "FP32 output needs FP32 t and factors");
+  const float *xf = static_cast<const float *>(x);
+  if (use_row(map, k, ldx, ldy, x, y)) {
+    if (a.prec == AINV_PREC_F32)
+      launch_lane<float, float, float, float, EPI_AXPBY>(
+          map, a, xf, k, (const float *)nullptr, y, alpha, beta, s, kw);
+    else
+      launch_lane<uint16_t, float, float, float, EPI_AXPBY>(
+          map, a, xf, k, (const float *)nullptr, y, alpha, beta, s, kw);
+  } else if (a.prec == AINV_PREC_F32) {
+    launch_k<float, float, float, float, EPI_AXPBY>(
+        a, xf, ldx, k, (const float *)nullptr, y, ldy, alpha, beta, s);
+  } else {
+    launch_k<uint16_t, float, float, float, EPI_AXPBY>(
+        a, xf, ldx, k, (const float *)nullptr, y, ldy, alpha, beta, s);
   }
 }
 

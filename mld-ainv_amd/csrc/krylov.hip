@@ -43,8 +43,10 @@ struct Col {
 struct Vecs {
   double *x, *b, *r, *q, *p, *v, *f, *s, *z, *t;
   // FP32 / BF16 factors: FP32 copies of the preconditioner's inputs,
-  // written by the passes that form them (BiCGSTAB: p and s; CG: r)
-  float *p32, *s32, *r32;
+  // written by the passes that form them (BiCGSTAB: p and s; CG: r), and
+  // BiCGSTAB's preconditioned vectors f = M p and z = M s kept in FP32
+  // (the x update and the products A f, A z read the same rounded values)
+  float *p32, *s32, *r32, *f32, *z32;
 };
 
 enum Op {
@@ -109,7 +111,9 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         v.s[e] = sn;
         if (v.s32) v.s32[e] = (float)sn;
       } else if constexpr (OP == OP_XR_BICG) {
-        v.x[e] = v.x[e] + cs.alpha * v.f[e] + cs.omega * v.z[e];
+        const double fe = v.f32 ? (double)v.f32[e] : v.f[e];
+        const double ze = v.z32 ? (double)v.z32[e] : v.z[e];
+        v.x[e] = v.x[e] + cs.alpha * fe + cs.omega * ze;
         const double rn = v.s[e] + (-cs.omega) * v.t[e];
         v.r[e] = rn;
         acc[0] += rn * rn;
@@ -509,11 +513,17 @@ void destroy_plan(KrylovPlan *p) { delete p; }
 
 namespace {
 
+// FP32 vectors of a plan start 256 bytes apart at least (aligned loads)
+int64_t x32_stride(const KrylovPlan *p) {
+  return ((int64_t)p->n * p->K + 63) / 64 * 64;
+}
+
 void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
   if ((p->method != KM_BICGSTAB && p->method != KM_CG) || !m ||
       m->prec == AINV_PREC_F64 || p->work32.p)
     return;
-  p->work32.alloc((size_t)2 * p->n * p->K * sizeof(float));
+  const int nv = p->method == KM_BICGSTAB ? 4 : 1;
+  p->work32.alloc((size_t)nv * x32_stride(p) * sizeof(float));
   // padding columns are read by the row-mapped kernels: keep them 0
   AINV_HIP_CHECK(hipMemsetAsync(p->work32.p, 0, p->work32.bytes, s));
 }
@@ -525,8 +535,11 @@ Vecs x32_vecs(const KrylovPlan *p) {
   if (p->m && p->m->prec != AINV_PREC_F64 && p->work32.p) {
     float *f = p->work32.as<float>();
     if (p->method == KM_BICGSTAB) {
+      const int64_t nk = x32_stride(p);
       w.p32 = f;
-      w.s32 = f + (int64_t)p->n * p->K;
+      w.s32 = f + nk;
+      w.f32 = f + 2 * nk;
+      w.z32 = f + 3 * nk;
     } else if (p->method == KM_CG) {
       w.r32 = f;
     }
@@ -609,6 +622,15 @@ void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
   else
     AINV_HIP_CHECK(hipMemcpyAsync(y, x, (size_t)p->n * p->K * sizeof(double),
                                   hipMemcpyDeviceToDevice, s));
+}
+
+// y = M x, FP32 x and FP32 y (FP32 / BF16 factors)
+void precond32(const KrylovPlan *p, const float *x, float *y, hipStream_t s) {
+  const ainv_solver *m = p->m;
+  hll_spmm_scale(m->hw, m->dv.p, x, true, p->K, p->tbuf.p, p->K, p->k, s,
+                 map_w(p->apply_map), p->kwu);
+  hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, true, p->K, 0.0, y, p->K, p->k, s,
+                 map_z(p->apply_map), p->kwu);
 }
 
 // y = M x for an FP32 copy of x (FP32 / BF16 factors)
@@ -755,20 +777,25 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
     const bool x32 = w.p32 != nullptr;
     fin(FIN_RHO_BICG, k, 0, p->pt, p->col, p->na, s);
     vec_op<OP_P_BICG>(n, k, w, p->col, p->pt, s);
-    if (x32)
-      precond32(p, w.p32, v.f, s);
-    else
+    int g;
+    if (x32) {
+      precond32(p, w.p32, w.f32, s);
+      g = hll_spmv_dot(a->hll, w.f32, v.v, k, K, v.q, 1, p->pt, sm, s, kw);
+    } else {
       precond(p, v.p, v.f, s);
-    int g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s, kw);
+      g = hll_spmv_dot(a->hll, v.f, v.v, k, K, v.q, 1, p->pt, sm, s, kw);
+    }
     fin(FIN_ALPHA_BICG, k, g, p->pt, p->col, p->na, s);
     vec_op<OP_S_BICG>(n, k, w, p->col, p->pt, s);
-    if (x32)
-      precond32(p, w.s32, v.z, s);
-    else
+    if (x32) {
+      precond32(p, w.s32, w.z32, s);
+      g = hll_spmv_dot(a->hll, w.z32, v.t, k, K, v.s, 2, p->pt, sm, s, kw);
+    } else {
       precond(p, v.s, v.z, s);
-    g = hll_spmv_dot(a->hll, v.z, v.t, k, K, v.s, 2, p->pt, sm, s, kw);
+      g = hll_spmv_dot(a->hll, v.z, v.t, k, K, v.s, 2, p->pt, sm, s, kw);
+    }
     fin(FIN_OMEGA, k, g, p->pt, p->col, p->na, s);
-    g = vec_op<OP_XR_BICG>(n, k, v, p->col, p->pt, s);
+    g = vec_op<OP_XR_BICG>(n, k, w, p->col, p->pt, s);
     fin(FIN_CONV_BICG, k, g, p->pt, p->col, p->na, s);
   } else {
     const Vecs w = x32_vecs(p);
