@@ -154,7 +154,9 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     const int32_t *kw = nullptr);
 // Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *
 // kMaxCols + c] for pair q, block b, column c.
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 2048;
+// grid of the persistent (grid-stride) system-matrix kernels
+constexpr int kPersistBlocks = 1024;
 constexpr int kMaxCols = 64;
 // y = A x (FP64 HLL, k interleaved columns, ld = k) fused with the dot
 // products of its result: np = 1 gives (w, y), np = 2 gives (y, y) and

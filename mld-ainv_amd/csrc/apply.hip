@@ -67,10 +67,10 @@ __device__ __forceinline__ GroupRange xcd_groups(int32_t nslices) {
 
 int persistent_grid(int32_t nslices) {
   int g = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
-  if (g > kRedBlocks) g = kRedBlocks;
+  if (g > kPersistBlocks) g = kPersistBlocks;
   if (g < 1) g = 1;
 #if AINV_XCD_REMAP
-  g = (g + kXcds - 1) / kXcds * kXcds;  // kRedBlocks is a multiple of 8
+  g = (g + kXcds - 1) / kXcds * kXcds;  // kPersistBlocks is a multiple of 8
 #endif
   return g;
 }
@@ -415,7 +415,7 @@ void launch_k(const Hll &a, const XT *x, int64_t ldx, int k, const DT *d,
 
 // System-matrix product fused with the dot products of the Krylov step
 // that follows it, so that the product is not read back: grid-stride over
-// slices with a fixed grid (at most kRedBlocks blocks), per-column partial
+// slices with a fixed grid (at most kPersistBlocks blocks), per-column partial
 // sums reduced across the row groups of a wave by shuffles and across the
 // waves of a block in LDS, in a fixed order.
 template <typename XT, int K, int NP>

@@ -367,7 +367,11 @@ template <int OP>
 int vec_op(int32_t n, int k, const Vecs &v, const Col *col, double *part,
            hipStream_t s) {
   const int K = pow2_at_least(k);
-  const int g = vec_blocks(n, k);
+  // passes with dots run on the full grid, the plain updates on half of it
+  // (measured: more waves help the former and slow the latter)
+  const int g = NPairs<OP>::v > 0 ? vec_blocks(n, k)
+                                  : std::max(1, std::min(vec_blocks(n, k),
+                                                         kRedBlocks / 2));
 #define AINV_VL(KK)                                                        \
   hipLaunchKernelGGL((k_vec<OP, KK>), dim3(g), dim3(kThreads), 0, s, n, k, \
                      v, col, part)
