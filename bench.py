@@ -100,7 +100,8 @@ def apply_pass_bytes(lay, k, first, xb=8, yb=8):
     row, and for the first pass d, the x read (xb bytes per value: 8, or 4
     for the FP32 copy BiCGSTAB and CG pass with FP32 / BF16 factors) and the t
     write; for the second pass the t read and the y write (yb bytes per
-    value: 8, or 4 for BiCGSTAB's FP32 preconditioned vectors; beta = 0).
+    value: 8, or 4 for the FP32 preconditioned vectors of BiCGSTAB and CG;
+    beta = 0).
     t is FP32 for FP32 / BF16 factors, FP64 otherwise."""
     n, vb = lay["n"], lay["value_bytes"]
     st = 8 if vb == 8 else 4
@@ -180,7 +181,7 @@ def main():
         sv.build(a)
         lay = sv.layout()
         x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
-        y32 = method == "BICGSTAB" and args.prec != "F64"
+        y32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
         ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32, y32)
         bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
         print(json.dumps({"apply_only": True, "config": args.config,
@@ -292,7 +293,7 @@ def main():
     # FP32 / BF16 factors give the first pass an FP32 x); the dominant kernel
     # is the longer of the two
     x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
-    y32 = method == "BICGSTAB" and args.prec != "F64"
+    y32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
     ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32, y32)
     bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
     bz = apply_pass_bytes(lay, slots, False, 8, 4 if y32 else 8)
@@ -574,7 +575,7 @@ def serve_bench(m, args):
 def profile_apply(m, sv, n, k, reps, x32=False, y32=False):
     """Average durations (ms) of the two apply kernels on k columns; x32:
     FP32 x (the input the served BiCGSTAB and CG give FP32 / BF16 factors),
-    y32: FP32 y (BiCGSTAB's preconditioned vectors)."""
+    y32: FP32 y (BiCGSTAB's and CG's preconditioned vectors)."""
     xh = m.gen_normal(11, (n, k)) if k > 1 else m.gen_normal(11, (n,))
     x = m.DeviceArray.from_numpy(xh.astype(np.float32) if x32 else xh)
     y = m.DeviceArray((n, k) if k > 1 else (n,))

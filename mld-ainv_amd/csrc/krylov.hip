@@ -44,8 +44,9 @@ struct Vecs {
   double *x, *b, *r, *q, *p, *v, *f, *s, *z, *t;
   // FP32 / BF16 factors: FP32 copies of the preconditioner's inputs,
   // written by the passes that form them (BiCGSTAB: p and s; CG: r), and
-  // BiCGSTAB's preconditioned vectors f = M p and z = M s kept in FP32
-  // (the x update and the products A f, A z read the same rounded values)
+  // the preconditioned vectors kept in FP32 (BiCGSTAB: f = M p, z = M s,
+  // whose x update and products A f, A z read the same rounded values; CG:
+  // z = M r, read by (r, z) and p = z + beta p)
   float *p32, *s32, *r32, *f32, *z32;
 };
 
@@ -123,12 +124,13 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         acc[1] += v.r[e] * v.r[e];
         if (v.r32) v.r32[e] = (float)v.r[e];
       } else if constexpr (OP == OP_DOT_RZ) {
-        acc[0] += v.r[e] * v.z[e];
+        acc[0] += v.r[e] * (v.z32 ? (double)v.z32[e] : v.z[e]);
       } else if constexpr (OP == OP_P_CG) {
+        const double ze = v.z32 ? (double)v.z32[e] : v.z[e];
         if (cs.iter == 1)
-          v.p[e] = v.z[e];
+          v.p[e] = ze;
         else
-          v.p[e] = v.z[e] + cs.beta * v.p[e];
+          v.p[e] = ze + cs.beta * v.p[e];
       } else if constexpr (OP == OP_XR_CG) {
         v.x[e] = v.x[e] + cs.alpha * v.p[e];
         const double rn = v.r[e] + (-cs.alpha) * v.q[e];
@@ -526,7 +528,7 @@ void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
   if ((p->method != KM_BICGSTAB && p->method != KM_CG) || !m ||
       m->prec == AINV_PREC_F64 || p->work32.p)
     return;
-  const int nv = p->method == KM_BICGSTAB ? 4 : 1;
+  const int nv = p->method == KM_BICGSTAB ? 4 : 2;
   p->work32.alloc((size_t)nv * x32_stride(p) * sizeof(float));
   // padding columns are read by the row-mapped kernels: keep them 0
   AINV_HIP_CHECK(hipMemsetAsync(p->work32.p, 0, p->work32.bytes, s));
@@ -546,6 +548,7 @@ Vecs x32_vecs(const KrylovPlan *p) {
       w.z32 = f + 3 * nk;
     } else if (p->method == KM_CG) {
       w.r32 = f;
+      w.z32 = f + x32_stride(p);
     }
   }
   return w;
@@ -634,16 +637,6 @@ void precond32(const KrylovPlan *p, const float *x, float *y, hipStream_t s) {
   hll_spmm_scale(m->hw, m->dv.p, x, true, p->K, p->tbuf.p, p->K, p->k, s,
                  map_w(p->apply_map), p->kwu);
   hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, true, p->K, 0.0, y, p->K, p->k, s,
-                 map_z(p->apply_map), p->kwu);
-}
-
-// y = M x for an FP32 copy of x (FP32 / BF16 factors)
-void precond32(const KrylovPlan *p, const float *x, double *y, hipStream_t s) {
-  const ainv_solver *m = p->m;
-  const bool f32t = m->prec != AINV_PREC_F64;
-  hll_spmm_scale(m->hw, m->dv.p, x, true, p->K, p->tbuf.p, p->K, p->k, s,
-                 map_w(p->apply_map), p->kwu);
-  hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, p->K, 0.0, y, p->K, p->k, s,
                  map_z(p->apply_map), p->kwu);
 }
 
@@ -804,12 +797,12 @@ void enqueue_iteration(const KrylovPlan *p, ainv_matrix *a, hipStream_t s) {
   } else {
     const Vecs w = x32_vecs(p);
     if (w.r32)
-      precond32(p, w.r32, v.z, s);
+      precond32(p, w.r32, w.z32, s);
     else
       precond(p, v.r, v.z, s);
-    int g = vec_op<OP_DOT_RZ>(n, k, v, p->col, p->pt, s);
+    int g = vec_op<OP_DOT_RZ>(n, k, w, p->col, p->pt, s);
     fin(FIN_RHO_CG, k, g, p->pt, p->col, p->na, s);
-    vec_op<OP_P_CG>(n, k, v, p->col, p->pt, s);
+    vec_op<OP_P_CG>(n, k, w, p->col, p->pt, s);
     g = hll_spmv_dot(a->hll, v.p, v.q, k, K, v.p, 1, p->pt, sm, s, kw);
     fin(FIN_ALPHA_CG, k, g, p->pt, p->col, p->na, s);
     g = vec_op<OP_XR_CG>(n, k, w, p->col, p->pt, s);
