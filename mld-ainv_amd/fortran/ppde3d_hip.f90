@@ -1,20 +1,22 @@
 ! Sample driver in the style of the reference's tests/pdegen/ppde3d.f90:
 ! generate the 3-D convection-diffusion matrix, build an AINV / INVK / INVT
-! preconditioner through the Fortran solver objects, solve with BiCGSTAB
-! or CG on the GPU and print the same quantities (build time, solve time,
+! preconditioner through the Fortran solver objects, solve with a Krylov
+! method on the GPU and print the same quantities (build time, solve time,
 ! iterations, error estimate, preconditioner nonzeros).
 !
 ! stdin deck, one value per line (text after the value is ignored):
-!   method (BICGSTAB|CG), preconditioner (AINV|INVK|INVT), grid size N,
-!   eps, itmax, sub_fillin, inv_fillin, sub_iluthrs, inv_thresh,
-!   ainv_alg, factor storage (F64|F32|BF16)
+!   method (BICGSTAB|CG|CGS|BICG|RGMRES|BICGSTABL), preconditioner
+!   (AINV|INVK|INVT), grid size N, eps, itmax, sub_fillin, inv_fillin,
+!   sub_iluthrs, inv_thresh, ainv_alg, factor storage (F64|F32|BF16) and,
+!   optionally, irst (restart of RGMRES / l of BiCGStab(l), as the
+!   reference decks' IRST line, tests/pdegen/runs/mld3d.inp:10)
 program ppde3d_hip
   use iso_c_binding
   use mld_ainv_hip_cbind, only: ainv_gen_pde3d, last_error
   use mld_d_ainv_hip_mod
   implicit none
   character(len=20) :: method, ptype, alg, fprec
-  integer :: idim, itmax, fill1, fill2, info, iter, i
+  integer :: idim, itmax, fill1, fill2, info, iter, i, irst, ios
   real(c_double) :: eps, thr1, thr2, err
   integer(c_int32_t) :: n
   integer(c_int64_t) :: nnz
@@ -36,6 +38,8 @@ program ppde3d_hip
   read(*, *) thr2
   read(*, *) alg
   read(*, *) fprec
+  read(*, *, iostat=ios) irst
+  if (ios /= 0) irst = 0
 
   call mld_ainv_init(0, info)
   if (info /= 0) call fail('init')
@@ -80,7 +84,7 @@ program ppde3d_hip
   end do
   x = 0.0d0
   call mld_krylov(trim(method), int(n), irp, ja, val, prec, b, x, eps, &
-       & itmax, iter, err, info)
+       & itmax, iter, err, info, irst)
   if (info /= 0) call fail('krylov')
   call system_clock(t2)
 

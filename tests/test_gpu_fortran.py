@@ -1,7 +1,7 @@
 """The Fortran iso_c_binding layer end to end on the GPU: the sample driver
 (mld-ainv_amd/fortran/ppde3d_hip.f90, built by __graft_entry__.build())
 builds the preconditioner through the Fortran solver objects and solves with
-BiCGSTAB; its iteration count must match the oracle's within +-1."""
+a Krylov method; its iteration count must match the oracle's within +-1."""
 import os
 import re
 import subprocess
@@ -53,3 +53,25 @@ def test_fortran_driver_matches_oracle(mld, O, ptype, params):
     assert float(f["Convergence indicator on exit"]) <= 1e-8
     assert float(f["Max error |x - 1|"]) < 1e-5
     assert int(f["Total nonzeros for PREC"]) == a.shape[0] + w.nnz + z.nnz
+
+
+@pytest.mark.parametrize("method,irst", [("RGMRES", 30), ("RGMRES", 2),
+                                         ("BICGSTABL", 2), ("CGS", 0)])
+def test_fortran_driver_methods_and_irst(mld, O, method, irst):
+    """The deck's method and IRST lines reach psb_krylov's counterpart."""
+    N = 12
+    lines = [method, "INVK", str(N), "1.d-8", "800", "1", "1", "0.0", "0.0",
+             "LLK", "F64"]
+    if irst:
+        lines.append(str(irst))
+    f, text = run_driver("\n".join(lines) + "\n")
+    its = int(f["Number of iterations"])
+    a = mld.gen_pde3d(N)
+    b = a @ np.ones(a.shape[0])
+    w, z, d = O.invk_bld(a, 1, 1)
+    _, it_ref, _ = O.krylov(method, a, w, z, d, b, 1e-8, 800,
+                            irst=irst if irst else
+                            (10 if method == "RGMRES" else 2))
+    tol = irst if method == "BICGSTABL" else 1
+    assert abs(its - it_ref) <= tol, (its, it_ref, text)
+    assert float(f["Max error |x - 1|"]) < 1e-5

@@ -231,6 +231,46 @@ def test_reduced_precision_factors(gpu, O, prec, tol):
     assert np.linalg.norm(y64 - ref) < 1e-12 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("method,prec", [("BICGSTAB", "F32"),
+                                         ("BICGSTAB", "BF16"), ("CG", "F32"),
+                                         ("CG", "BF16")])
+def test_reduced_precision_krylov(gpu, O, method, prec):
+    """With FP32 / BF16 factors the Krylov engine hands the preconditioner
+    FP32 copies of its inputs and keeps the preconditioned vectors in FP32
+    (recurrences, dots and residuals stay FP64): iteration counts stay close
+    to the FP64-factor solve's (within 2 for FP32, 25 % + 2 for BF16), every
+    column reaches the tolerance in true residual, and the oracle's FP64
+    solve with the FP32 factors' values agrees on the count within 2."""
+    if method == "CG":
+        a = gpu.gen_aniso2d(48, 1e-2)
+        sv = build(gpu, "AINV", a, AINV_ALG="SYM-LLK", SUB_FILLIN=6,
+                   SUB_ILUTHRS=5e-2)
+    else:
+        a = gpu.gen_pde3d(12)
+        sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    n = a.shape[0]
+    B = gpu.gen_normal(29, (n, 5))
+    A = gpu.SystemMatrix(a)
+    _, it64, _ = gpu.krylov(method, A, sv, B, eps=1e-8)
+    sv.cnv(prec)
+    X, it, err = gpu.krylov(method, A, sv, B, eps=1e-8)
+    for c in range(5):
+        if prec == "F32":
+            assert abs(int(it[c]) - int(it64[c])) <= 2, (c, it, it64)
+        else:
+            assert it[c] <= 1.25 * it64[c] + 2, (c, it, it64)
+        assert err[c] <= 1e-8
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c])
+    if prec == "F32":
+        # the factors as stored (FP32 values) in an FP64 oracle solve
+        w = sv.export("W")
+        z = sv.export("Z")
+        d = sv.export("D")
+        _, it_ref, _ = O.krylov(method, a, w, z, d, B[:, 0], 1e-8)
+        assert abs(int(it[0]) - it_ref) <= 2, (it[0], it_ref)
+
+
 def test_krylov_bicgstab_iterations(gpu, O):
     a = gpu.gen_pde3d(14)
     n = a.shape[0]
