@@ -117,7 +117,7 @@ def main():
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--config", default="C2", choices=sorted(CONFIGS) + ["C5"])
-    p.add_argument("--c5-nmax", type=float, default=2e5,
+    p.add_argument("--c5-nmax", type=float, default=2e6,
                    help="largest random matrix of the C5 pool")
     p.add_argument("--c5-requests", type=int, default=96)
     p.add_argument("--c5-slots", type=int, default=0,
