@@ -213,7 +213,7 @@ void do_build(ainv_solver *h, const HostCsr &a) {
   std::vector<double> d;
   if (h->kind == AINV_KIND_AINV) {
     HostCsr w, z;
-    host_ainv(a, h->alg, h->fill_in, h->thresh, w, z, d);
+    host_ainv(a, h->alg, h->fill_in, h->thresh, w, z, d, h->host_threads);
     h->t_factor = now_seconds() - t0;
     csr_upload(w, h->w, h->stream);
     csr_upload(z, h->z, h->stream);

@@ -32,6 +32,7 @@ struct ainv_solver {
   // kernel mapping of the apply: -1 autotuned, else AINV_MAP_SPLIT / ROW /
   // COOP / COOP128 for both passes
   int map_mode = -1;
+  int host_threads = 0;  // host threads of the AINV build (0: the default)
 
   // ---- device / stream
   int device = 0;

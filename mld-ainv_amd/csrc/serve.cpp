@@ -82,6 +82,7 @@ struct ainv_server {
   uint64_t clock = 0;
   int64_t resident = 0;
   bool stop = false;
+  int nworkers = 0;  // set before the worker threads start
   std::vector<std::thread> workers;
   // statistics
   int64_t builds = 0, evictions = 0, completed = 0, admitted_midflight = 0;
@@ -137,6 +138,8 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
     M = new ainv_solver();
     M->kind = m->kind;
     M->device = S->device;
+    // the workers' builds share the host cores
+    M->host_threads = std::max(1, build_threads() / std::max(1, S->nworkers));
     for (auto &kv : m->params) apply_param(M, kv.first, kv.second);
     build_solver(M, m->a);
   } catch (...) {
@@ -407,6 +410,7 @@ int ainv_server_create(int device, int64_t hbm_budget, int workers,
     auto *S = new ainv_server();
     S->device = device;
     S->budget = hbm_budget;
+    S->nworkers = workers;
     for (int w = 0; w < workers; ++w) S->workers.emplace_back(worker_main, S);
     *out = S;
   });
