@@ -37,13 +37,19 @@ namespace {
 
 using namespace ainv;
 
+// A request's right-hand sides and solution live in device memory: the
+// submitting thread uploads b, the worker extracts x on the device and the
+// waiting thread copies it out, so no worker blocks on a host copy.
 struct Request {
   int64_t id = 0;
   int32_t mat = 0;
+  int32_t n = 0;
   int k = 1;
   double eps = 1e-8;
   int itmax = 1000;
-  std::vector<double> b, x;
+  double *dev_b = nullptr;       // n x k, row-interleaved
+  double *dev_x = nullptr;       // n x k, written by the worker
+  hipEvent_t ready = nullptr;    // recorded after the extraction
   std::vector<int32_t> iters;
   std::vector<double> errs;
   double t_submit = 0, t_start = 0, t_done = 0;
@@ -83,6 +89,11 @@ struct ainv_server {
   int64_t resident = 0;
   bool stop = false;
   int nworkers = 0;  // set before the worker threads start
+  hipStream_t copy = nullptr;  // uploads and downloads of request data
+  // Held while a worker may capture a graph (batch_open) and around every
+  // stream-ordered allocation or free: HIP refuses those while any stream
+  // of the process is capturing.
+  std::mutex hip_mu;
   std::vector<std::thread> workers;
   // statistics
   int64_t builds = 0, evictions = 0, completed = 0, admitted_midflight = 0;
@@ -185,8 +196,7 @@ struct SlotUse {
   int64_t first_step = 0;  // first step that runs with this request's columns
 };
 
-void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
-                  DevBuf &stage) {
+void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s) {
   int K = 1;
   {
     std::unique_lock<std::mutex> lk(S->mu);
@@ -197,10 +207,12 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
   }
   ensure_resident(S, m, K);
   const int32_t n = m->a.n;
-  const size_t need = (size_t)n * kMaxCols * sizeof(double);
   const double t_open = now_seconds();
-  if (stage.bytes < need) stage.alloc(need);
-  SlotBatch *b = batch_open(m->A, m->M, m->method, K, s);
+  SlotBatch *b = nullptr;
+  {
+    std::lock_guard<std::mutex> g(S->hip_mu);
+    b = batch_open(m->A, m->M, m->method, K, s);
+  }
   {
     std::unique_lock<std::mutex> lk(S->mu);
     S->open_seconds += now_seconds() - t_open;
@@ -238,10 +250,7 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
       for (size_t t = 0; t < adm.size(); ++t) {
         Request *r = adm[t];
         r->t_start = now_seconds();
-        AINV_HIP_CHECK(hipMemcpyAsync(stage.p, r->b.data(),
-                                      r->b.size() * sizeof(double),
-                                      hipMemcpyHostToDevice, s));
-        batch_admit(b, adm_c0[t], r->k, stage.as<double>(), r->eps);
+        batch_admit(b, adm_c0[t], r->k, r->dev_b, r->eps);
       }
       const double t1 = now_seconds();
       t_admit += t1 - t0;
@@ -275,12 +284,15 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s,
             if (x.active && x.iter < r->itmax) alldone = false;
           }
           if (!alldone) continue;
-          batch_extract(b, su.c0, su.kc, stage.as<double>());
-          r->x.resize((size_t)n * su.kc);
-          AINV_HIP_CHECK(hipMemcpyAsync(r->x.data(), stage.p,
-                                        r->x.size() * sizeof(double),
-                                        hipMemcpyDeviceToHost, s));
-          AINV_HIP_CHECK(hipStreamSynchronize(s));
+          {
+            std::lock_guard<std::mutex> g(S->hip_mu);
+            AINV_HIP_CHECK(hipMallocAsync(
+                (void **)&r->dev_x, (size_t)n * su.kc * sizeof(double), s));
+          }
+          batch_extract(b, su.c0, su.kc, r->dev_x);
+          AINV_HIP_CHECK(
+              hipEventCreateWithFlags(&r->ready, hipEventDisableTiming));
+          AINV_HIP_CHECK(hipEventRecord(r->ready, s));
           r->iters.resize(su.kc);
           r->errs.resize(su.kc);
           for (int j = 0; j < su.kc; ++j) {
@@ -320,7 +332,6 @@ void worker_main(ainv_server *S) {
   hipStream_t s = nullptr;
   if (hipSetDevice(S->device) != hipSuccess || hipStreamCreate(&s) != hipSuccess)
     return;
-  DevBuf stage;
   for (;;) {
     ServedMatrix *m = nullptr;
     {
@@ -328,7 +339,6 @@ void worker_main(ainv_server *S) {
       for (;;) {
         if (S->stop) {
           lk.unlock();
-          stage.release();
           (void)hipStreamDestroy(s);
           return;
         }
@@ -348,7 +358,7 @@ void worker_main(ainv_server *S) {
       m->busy = true;
     }
     try {
-      serve_matrix(S, m, s, stage);
+      serve_matrix(S, m, s);
     } catch (const std::exception &e) {
       // fail the requests still queued for this matrix
       std::vector<Request *> fail;
@@ -411,6 +421,11 @@ int ainv_server_create(int device, int64_t hbm_budget, int workers,
     S->device = device;
     S->budget = hbm_budget;
     S->nworkers = workers;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess) {
+      delete S;
+      throw Error(AINV_ERR_HIP, "cannot create the server's copy stream");
+    }
     for (int w = 0; w < workers; ++w) S->workers.emplace_back(worker_main, S);
     *out = S;
   });
@@ -457,17 +472,37 @@ int ainv_server_submit(ainv_server_h S, int32_t mat_id, int k, const double *b,
     if (!S || !b || !req_id) throw Error(AINV_ERR_ARG, "NULL argument");
     if (k < 1 || k > kMaxCols) throw Error(AINV_ERR_ARG, "k must be 1..64");
     if (itmax < 1) throw Error(AINV_ERR_ARG, "itmax must be >= 1");
+    int32_t n = 0;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      auto it = S->mats.find(mat_id);
+      if (it == S->mats.end()) throw Error(AINV_ERR_ARG, "unknown matrix id");
+      n = it->second->a.n;
+    }
     auto r = std::make_unique<Request>();
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->mats.find(mat_id);
-    if (it == S->mats.end()) throw Error(AINV_ERR_ARG, "unknown matrix id");
-    const int32_t n = it->second->a.n;
-    r->b.assign(b, b + (size_t)n * k);
-    r->id = S->next_req++;
+    r->n = n;
     r->mat = mat_id;
     r->k = k;
     r->eps = eps;
     r->itmax = itmax;
+    // upload outside the lock: b is the caller's until this call returns
+    AINV_HIP_CHECK(hipSetDevice(S->device));
+    const size_t bytes = (size_t)n * k * sizeof(double);
+    hipEvent_t up = nullptr;
+    {
+      std::lock_guard<std::mutex> g(S->hip_mu);
+      AINV_HIP_CHECK(hipMallocAsync((void **)&r->dev_b, bytes, S->copy));
+      AINV_HIP_CHECK(
+          hipMemcpyAsync(r->dev_b, b, bytes, hipMemcpyHostToDevice, S->copy));
+      AINV_HIP_CHECK(hipEventCreateWithFlags(&up, hipEventDisableTiming));
+      AINV_HIP_CHECK(hipEventRecord(up, S->copy));
+    }
+    const hipError_t e = hipEventSynchronize(up);
+    (void)hipEventDestroy(up);
+    AINV_HIP_CHECK(e);
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->mats.find(mat_id);
+    r->id = S->next_req++;
     r->t_submit = now_seconds();
     *req_id = r->id;
     it->second->pending.push_back(r->id);
@@ -476,29 +511,62 @@ int ainv_server_submit(ainv_server_h S, int32_t mat_id, int k, const double *b,
   });
 }
 
+namespace {
+// Device buffers and event of a request, released on the copy stream after
+// everything queued on it (the download) is done.
+void release_request(ainv_server *S, Request *r) {
+  if (r->dev_b) (void)hipFreeAsync(r->dev_b, S->copy);
+  if (r->dev_x) (void)hipFreeAsync(r->dev_x, S->copy);
+  if (r->ready) (void)hipEventDestroy(r->ready);
+  r->dev_b = r->dev_x = nullptr;
+  r->ready = nullptr;
+}
+}  // namespace
+
 int ainv_server_wait(ainv_server_h S, int64_t req_id, double *x,
                      int32_t *iters, double *errs, double *latency_ms) {
-  int code = AINV_SUCCESS;
-  int rc = guard([&] {
+  return guard([&] {
     if (!S) throw Error(AINV_ERR_ARG, "NULL server");
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(req_id);
-    if (it == S->reqs.end()) throw Error(AINV_ERR_ARG, "unknown request id");
-    Request *r = it->second.get();
-    S->cv_done.wait(lk, [&] { return r->done; });
-    if (r->status != AINV_SUCCESS) {
-      code = r->status;
-      std::string msg = r->msg;
+    std::unique_ptr<Request> r;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      auto it = S->reqs.find(req_id);
+      if (it == S->reqs.end()) throw Error(AINV_ERR_ARG, "unknown request id");
+      Request *rp = it->second.get();
+      S->cv_done.wait(lk, [&] { return rp->done; });
+      r = std::move(it->second);
       S->reqs.erase(it);
-      throw Error(code, "request failed: " + msg);
     }
-    if (x) std::memcpy(x, r->x.data(), r->x.size() * sizeof(double));
+    AINV_HIP_CHECK(hipSetDevice(S->device));
+    if (r->status != AINV_SUCCESS) {
+      // a failed batch may have left work on its stream that reads dev_b
+      std::lock_guard<std::mutex> g(S->hip_mu);
+      (void)hipDeviceSynchronize();
+      release_request(S, r.get());
+      throw Error(r->status, "request failed: " + r->msg);
+    }
+    // download outside the server lock, after the worker's extraction; the
+    // buffers are freed on the copy stream behind that point in any case
+    hipError_t e = hipSuccess;
+    hipEvent_t down = nullptr;
+    {
+      std::lock_guard<std::mutex> g(S->hip_mu);
+      if (r->ready) e = hipStreamWaitEvent(S->copy, r->ready, 0);
+      if (e == hipSuccess && x && r->dev_x)
+        e = hipMemcpyAsync(x, r->dev_x, (size_t)r->n * r->k * sizeof(double),
+                           hipMemcpyDeviceToHost, S->copy);
+      if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&down, hipEventDisableTiming);
+      if (e == hipSuccess) e = hipEventRecord(down, S->copy);
+      release_request(S, r.get());
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(down);
+    if (down) (void)hipEventDestroy(down);
+    AINV_HIP_CHECK(e);
     if (iters) std::memcpy(iters, r->iters.data(), r->k * sizeof(int32_t));
     if (errs) std::memcpy(errs, r->errs.data(), r->k * sizeof(double));
     if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
-    S->reqs.erase(it);
   });
-  return rc;
 }
 
 int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
@@ -537,6 +605,16 @@ int ainv_server_free(ainv_server_h S) {
     }
     S->cv_work.notify_all();
     for (auto &t : S->workers) t.join();
+    (void)hipSetDevice(S->device);
+    (void)hipDeviceSynchronize();  // the workers' streams are done
+    {
+      std::lock_guard<std::mutex> g(S->hip_mu);
+      for (auto &kv : S->reqs) release_request(S, kv.second.get());
+    }
+    if (S->copy) {
+      (void)hipStreamSynchronize(S->copy);
+      (void)hipStreamDestroy(S->copy);
+    }
     for (auto &kv : S->mats) {
       if (kv.second->M) ainv_free(kv.second->M);
       if (kv.second->A) ainv_mat_free(kv.second->A);
