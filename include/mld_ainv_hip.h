@@ -299,12 +299,14 @@ int ainv_server_add_matrix(ainv_server_h srv, int32_t n, const int64_t *irp,
                            const int32_t *ja, const double *val,
                            int index_base, int kind, const char *params,
                            int32_t *mat_id);
-/* b: host, n x k interleaved (1 <= k <= 64), x0 = 0, istop = 2. */
+/* b: host, n x k interleaved (1 <= k <= 64), x0 = 0, istop = 2; copied to
+ * device memory before the call returns (b may be reused at once). */
 int ainv_server_submit(ainv_server_h srv, int32_t mat_id, int k,
                        const double *b, double eps, int itmax,
                        int64_t *req_id);
-/* Blocks until the request completes; x: host n x k; latency from submit to
- * completion.  Releases the request. */
+/* Blocks until the request completes and copies its solution from device
+ * memory into x (host n x k; may be NULL); latency from submit to
+ * completion on the device.  Releases the request. */
 int ainv_server_wait(ainv_server_h srv, int64_t req_id, double *x,
                      int32_t *iters, double *errs, double *latency_ms);
 int ainv_server_stats(ainv_server_h srv, char *json, size_t len);
