@@ -1995,6 +1995,91 @@ void bicgstabl_solve(ainv_matrix *a, ainv_solver *m, const double *b,
   }
 }
 
+namespace {
+struct ColMap {
+  int cnt;
+  int from[kMaxCols];
+};
+
+// Column map.from[j] of every workspace vector of one plan to column j of
+// the same vector of a plan with fewer slots (same method and solver, so the
+// same vector list).
+__global__ void k_shrink_cols(int32_t n, int K1, int K2,
+                              const double *__restrict__ w1,
+                              double *__restrict__ w2, int nvec, int64_t vs1,
+                              int64_t vs2, const float *__restrict__ f1,
+                              float *__restrict__ f2, int nv32, int64_t fs1,
+                              int64_t fs2, ColMap map) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int j = 0; j < map.cnt; ++j) {
+    const int c = map.from[j];
+    for (int v = 0; v < nvec; ++v)
+      w2[v * vs2 + row * K2 + j] = w1[v * vs1 + row * K1 + c];
+    for (int v = 0; v < nv32; ++v)
+      f2[v * fs2 + row * K2 + j] = f1[v * fs1 + row * K1 + c];
+  }
+}
+
+// Column states to the new slots (the rest empty) and the active width the
+// next iteration rolls in.
+__global__ void k_shrink_state(const Col *__restrict__ c1, Col *__restrict__ c2,
+                               int K2, ColMap map, int32_t *__restrict__ na2) {
+  const int t = threadIdx.x;
+  if (t < K2) {
+    Col c = Col{};
+    if (t < map.cnt) c = c1[map.from[t]];
+    c2[t] = c;
+  }
+  if (t == 0) {
+    int32_t w = 0;
+    for (int j = 0; j < map.cnt; ++j)
+      if (c1[map.from[j]].active) w = j + 1;
+    na2[1] = w;
+  }
+}
+}  // namespace
+
+// A batch whose queue has run dry and whose remaining columns fit in half
+// of its slots continues on a plan with fewer slots: the columns `keep`
+// (their state and every workspace vector) move to slots 0.. of a new batch
+// on the same stream, and this batch closes.  The narrower plan's graph runs
+// every kernel at the smaller block width.
+SlotBatch *batch_shrink(SlotBatch *b, ainv_solver *m,
+                        const std::vector<int> &keep) {
+  KrylovPlan *p1 = b->p;
+  int K2 = 1;
+  while (K2 < (int)keep.size()) K2 <<= 1;
+  if ((int)keep.size() > kMaxCols || K2 >= b->K)
+    throw Error(AINV_ERR_INTERNAL, "shrink to a batch that is not narrower");
+  SlotBatch *nb = batch_open(b->a, m, p1->method, K2, b->s);
+  KrylovPlan *p2 = nb->p;
+  const int32_t n = p1->n;
+  const int64_t vs1 = (int64_t)n * p1->K, vs2 = (int64_t)n * p2->K;
+  const int nvec = (int)(p1->work.bytes / (sizeof(double) * (size_t)vs1));
+  const int nvec2 = (int)(p2->work.bytes / (sizeof(double) * (size_t)vs2));
+  const int64_t fs1 = x32_stride(p1), fs2 = x32_stride(p2);
+  const int nv32 =
+      (p1->work32.p && p2->work32.p)
+          ? (int)(p1->work32.bytes / (sizeof(float) * (size_t)fs1))
+          : 0;
+  if (nvec != nvec2)
+    throw Error(AINV_ERR_INTERNAL, "plans of one method differ in vectors");
+  ColMap map{};
+  map.cnt = (int)keep.size();
+  for (int j = 0; j < map.cnt; ++j) map.from[j] = keep[j];
+  hipLaunchKernelGGL(k_shrink_cols, dim3((unsigned)((n + 255) / 256)),
+                     dim3(256), 0, b->s, n, p1->K, p2->K,
+                     p1->work.as<double>(), p2->work.as<double>(), nvec, vs1,
+                     vs2, p1->work32.as<float>(), p2->work32.as<float>(), nv32,
+                     fs1, fs2, map);
+  hipLaunchKernelGGL(k_shrink_state, dim3(1), dim3(kMaxCols), 0, b->s, p1->col,
+                     p2->col, K2, map, p2->na);
+  AINV_HIP_CHECK(hipGetLastError());
+  batch_close(b);
+  return nb;
+}
+
 void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
                    const double *b, double *x, int nrhs, int slots, double eps,
                    int itmax, int32_t *iter, double *err, double *lat_ms,
@@ -2008,7 +2093,7 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
   if (nrhs == 0) return;
   const int64_t n = a->n;
   SlotBatch *bt = batch_open(a, m, mid, slots, s);
-  const int K = bt->K;
+  int K = bt->K;
   std::vector<int> owner(K, -1);
   std::vector<int64_t> first(K, 0);
   std::vector<double> t_adm(K, 0.0);
@@ -2043,6 +2128,28 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
         if (lat_ms) lat_ms[j] = (t - t_adm[c]) * 1e3;
         owner[c] = -1;
         ++done;
+      }
+      // queue run dry and at most half the slots busy: continue narrower
+      if (next >= nrhs && K > 1) {
+        std::vector<int> keep;
+        for (int c = 0; c < K; ++c)
+          if (owner[c] >= 0) keep.push_back(c);
+        if (!keep.empty() && (int)keep.size() <= K / 2) {
+          bt = batch_shrink(bt, m, keep);
+          K = bt->K;
+          slots = K;
+          std::vector<int> ow(K, -1);
+          std::vector<int64_t> fi(K, 0);
+          std::vector<double> ta(K, 0.0);
+          for (size_t j = 0; j < keep.size(); ++j) {
+            ow[j] = owner[keep[j]];
+            ta[j] = t_adm[keep[j]];
+            fi[j] = batch_next_step(bt);
+          }
+          owner.swap(ow);
+          first.swap(fi);
+          t_adm.swap(ta);
+        }
       }
     }
     AINV_HIP_CHECK(hipStreamSynchronize(s));
