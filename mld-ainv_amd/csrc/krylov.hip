@@ -32,7 +32,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kMaxPairs = 3;
-constexpr int kPlansPerMatrix = 8;
+constexpr int kPlansPerMatrix = 12;
 
 // per-column state of the recurrences
 struct Col {
@@ -2040,6 +2040,10 @@ __global__ void k_shrink_state(const Col *__restrict__ c1, Col *__restrict__ c2,
 }
 }  // namespace
 
+namespace {
+void move_columns(SlotBatch *b, SlotBatch *nb, const std::vector<int> &keep);
+}  // namespace
+
 // A batch whose queue has run dry and whose remaining columns fit in half
 // of its slots continues on a plan with fewer slots: the columns `keep`
 // (their state and every workspace vector) move to slots 0.. of a new batch
@@ -2053,7 +2057,21 @@ SlotBatch *batch_shrink(SlotBatch *b, ainv_solver *m,
   if ((int)keep.size() > kMaxCols || K2 >= b->K)
     throw Error(AINV_ERR_INTERNAL, "shrink to a batch that is not narrower");
   SlotBatch *nb = batch_open(b->a, m, p1->method, K2, b->s);
+  try {
+    move_columns(b, nb, keep);
+  } catch (...) {
+    batch_close(nb);  // the caller still owns b
+    throw;
+  }
+  batch_close(b);
+  return nb;
+}
+
+namespace {
+void move_columns(SlotBatch *b, SlotBatch *nb, const std::vector<int> &keep) {
+  KrylovPlan *p1 = b->p;
   KrylovPlan *p2 = nb->p;
+  const int K2 = nb->K;
   const int32_t n = p1->n;
   const int64_t vs1 = (int64_t)n * p1->K, vs2 = (int64_t)n * p2->K;
   const int nvec = (int)(p1->work.bytes / (sizeof(double) * (size_t)vs1));
@@ -2076,9 +2094,8 @@ SlotBatch *batch_shrink(SlotBatch *b, ainv_solver *m,
   hipLaunchKernelGGL(k_shrink_state, dim3(1), dim3(kMaxCols), 0, b->s, p1->col,
                      p2->col, K2, map, p2->na);
   AINV_HIP_CHECK(hipGetLastError());
-  batch_close(b);
-  return nb;
 }
+}  // namespace
 
 void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
                    const double *b, double *x, int nrhs, int slots, double eps,
