@@ -114,3 +114,36 @@ def test_krylov_stream_matches_single_solves(gpu, method, slots, nrhs):
         tr = np.linalg.norm(B[j] - a @ Xh[j]) / np.linalg.norm(B[j])
         assert tr < 1e-8, (j, tr)
         assert np.linalg.norm(Xh[j] - xs) <= 1e-6 * np.linalg.norm(xs)
+
+
+def test_failed_build_and_unwaited_requests(gpu):
+    """A matrix whose build fails fails its requests with the build's error
+    and leaves the server serving the others; requests still queued or
+    unwaited when the server is freed release their device buffers."""
+    good = gpu.gen_pde3d(8)
+    # a zero diagonal entry: ILU(0) meets a zero pivot
+    bad = good.tolil()
+    bad[0, 0] = 0.0
+    bad = bad.tocsr()
+    bad.eliminate_zeros()
+    srv = gpu.Server(hbm_budget_bytes=4 << 30, workers=2)
+    ig = srv.add_matrix(good, "INVK", SUB_FILLIN=0, INV_FILLIN=0)
+    ib = srv.add_matrix(bad, "INVK", SUB_FILLIN=0, INV_FILLIN=0)
+    n = good.shape[0]
+    tb = srv.submit(ib, gpu.gen_normal(3, (n, 2)), eps=1e-8)
+    tg = srv.submit(ig, gpu.gen_normal(4, (n, 3)), eps=1e-8)
+    with pytest.raises(gpu.AinvError) as e:
+        srv.wait(tb)
+    assert "pivot" in str(e.value).lower()
+    x, it, err, _ = srv.wait(tg)
+    assert np.all(err <= 1e-8)
+    for r in range(6):  # left unwaited
+        srv.submit(ig, gpu.gen_normal(10 + r, (n, 4)), eps=1e-8)
+    srv.free()
+    # a new server on the same device works afterwards
+    srv2 = gpu.Server(hbm_budget_bytes=4 << 30, workers=1)
+    i2 = srv2.add_matrix(good, "INVK", SUB_FILLIN=1, INV_FILLIN=1)
+    x, it, err, _ = srv2.wait(srv2.submit(i2, gpu.gen_normal(5, (n, 1)),
+                                          eps=1e-8))
+    assert np.all(err <= 1e-8)
+    srv2.free()
