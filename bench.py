@@ -280,6 +280,10 @@ def main():
     Xs = dX.to_numpy()
     true = max(float(np.linalg.norm(Bh[j] - a @ Xs[j]) / np.linalg.norm(Bh[j]))
                for j in range(0, M, max(1, M // 16)))
+    # the served solution of the last sampled right-hand side against its
+    # FP64 single-RHS solve (both at the same tolerance)
+    j64 = sample[-1]
+    xdiff = float(np.linalg.norm(Xs[j64] - x64) / np.linalg.norm(x64))
     rep = sv.report()
     tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
     kmap_w = tuned[0]["map_w"] if tuned else "split"
@@ -371,6 +375,12 @@ def main():
                        "sample": f"right-hand sides {sample[0]}.."
                                  f"{sample[-1]}"},
         "true_residual_max": true,
+        "error_vs_fp64": {"rel_diff": xdiff, "rhs": j64,
+                          "iterations_fp64": it64[-1],
+                          "iterations_served": int(iters[j64]),
+                          "note": "||x_served - x_fp64|| / ||x_fp64|| for one "
+                                  "right-hand side, both solves at the "
+                                  "stopping tolerance"},
         "fp64_single_rhs": {
             "solves_per_s": fp64_rate, "ms_mean": 1e3 * float(np.mean(t64)),
             "iterations": it64, "true_residual": true64,
