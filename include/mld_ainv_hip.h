@@ -42,6 +42,8 @@ extern "C" {
 #define AINV_KIND_AINV 1 /* mld_d_ainv_solver_type (biconjugation) */
 #define AINV_KIND_INVK 2 /* mld_d_invk_solver_type (inverse of ILU(k)) */
 #define AINV_KIND_INVT 3 /* mld_d_invt_solver_type (inverse of ILUT) */
+#define AINV_KIND_ML 4   /* multilevel preconditioner whose smoothers and
+                            coarsest-level solver are AINV-family factors */
 
 /* ---- AINV algorithm variants, in the order of the reference's ids
  * (src/mld_base_ainv_mod.F90:40-60; the absolute values there are offsets
@@ -57,6 +59,39 @@ extern "C" {
 #define AINV_PREC_F64 0  /* FP64 values, FP64 d, FP64 intermediate */
 #define AINV_PREC_F32 1  /* FP32 values and d, FP64 accumulation */
 #define AINV_PREC_BF16 2 /* BF16 values, FP32 d, FP32 accumulation */
+
+/* ---- multilevel preconditioner (kind AINV_KIND_ML) ----------------- */
+/* The reference's solvers are MLD2P4 smoothers; its deck
+ * tests/pdegen/runs/ppde.inp:10-29 runs them inside MLD2P4's multilevel
+ * preconditioner (prec%init('ML')), a framework outside the reference tree
+ * (MLD2P4 >= 2.1.1, configure.ac:283-289).  Kind AINV_KIND_ML is that
+ * configuration on one process: smoothed aggregation (greedy three-phase
+ * aggregation over the strength graph |a_ij| > theta sqrt(|a_ii a_jj|),
+ * prolongator P = (I - omega D^-1 A) T with omega = 4 / (3 ||D^-1 A||_inf),
+ * Galerkin R A P with R = P^T), AINV-family factors as the smoother of each
+ * level (AS / block Jacobi with one block) and as the coarsest-level solver,
+ * and a V-cycle.  Keys of the setters for this kind:
+ *   ML_TYPE          'MULT' | 'ADD'                  (default MULT)
+ *   SMOOTHER_POS     'PRE' | 'POST' | 'TWOSIDE'      (MULT; default POST)
+ *   SMOOTHER_SWEEPS  sweeps of the smoother per visit (default 1)
+ *   N_PREC_LEVS      levels, coarsest included       (default 3)
+ *   AGGR_KIND        'SMOOTHED' | 'NONSMOOTHED'      (default SMOOTHED)
+ *   AGGR_ALG         'DEC'                           (the only one)
+ *   AGGR_THRESH      theta of the finest level       (default 0.10)
+ *   AGGR_SCALE       theta of level l+1 = theta of level l * AGGR_SCALE
+ *                    (default 1)
+ *   SUB_SOLVE        'AINV' | 'INVK' | 'INVT': smoother factors (AINV)
+ *   COARSE_SUBSOLVE  'AINV' | 'INVK' | 'INVT': coarsest-level factors (AINV)
+ *   COARSE_SWEEPS    sweeps u += M (b - A u) of the coarsest level (15)
+ * and the factor keys below (SUB_FILLIN, AINV_ALG, SUB_ILUTHRS, INV_FILLIN,
+ * INV_THRESH, FACTOR_PREC, APPLY_MAP), which hold for the factors of every
+ * level.  Coarsening stops before N_PREC_LEVS when a step no longer reduces
+ * the size.  Applies only with trans 'N'; Krylov methods other than BICG. */
+#define AINV_ML_MULT 0
+#define AINV_ML_ADD 1
+#define AINV_ML_PRE 1
+#define AINV_ML_POST 2
+#define AINV_ML_TWOSIDE 3
 
 /* ---- flags --------------------------------------------------------- */
 #define AINV_HOST_PTRS 1 /* x/y (or b/x) are host pointers */
@@ -175,6 +210,33 @@ int ainv_import_factors(ainv_h h, int32_t n, const int64_t *w_irp,
                         const int32_t *w_ja, const double *w_val,
                         const int64_t *z_irp, const int32_t *z_ja,
                         const double *z_val, const double *d, int index_base);
+
+/* Multilevel hierarchy of a built AINV_KIND_ML handle (no counterpart in
+ * the reference, whose hierarchy lives in MLD2P4): the number of levels, and
+ * level l's (0 = finest) 'A' (level matrix), 'P' (prolongator, nrows x
+ * ncols), 'R' (restriction P^T), 'W', 'Z' (factors) as 0-based CSR, 'D'
+ * (pivots, nnz = nrows values) or 'G' (aggregate of each row, nnz = nrows
+ * entries in ja).  Call with NULL arrays to get the sizes first. */
+int ainv_ml_levels(ainv_h h, int *nlev);
+int ainv_ml_export(ainv_h h, int level, char which, int32_t *nrows,
+                   int32_t *ncols, int64_t *nnz, int64_t *irp, int32_t *ja,
+                   double *val);
+
+/* One coarsening step of the multilevel build on the host alone (no device
+ * involved; no counterpart in the reference): aggregates agg[n] and their
+ * count, the prolongator P (n x nagg, SMOOTHED if smoothed != 0, else the
+ * tentative one) and the Galerkin matrix R (A P) (nagg x nagg), as 0-based
+ * CSR with row pointers of n+1 (P) and nagg+1 (R A P) entries and at most
+ * `cap` entries each (larger: AINV_ERR_ARG with the sizes in *p_nnz,
+ * *c_nnz), and omega of the smoothing step; on `threads` host threads (0:
+ * the default count), bit-identical results for any count. */
+int ainv_ml_host_level(int32_t n, const int64_t *irp, const int32_t *ja,
+                       const double *val, int index_base, double theta,
+                       int smoothed, int threads, int64_t cap, int32_t *agg,
+                       int32_t *nagg, int64_t *p_irp, int32_t *p_ja,
+                       double *p_val, int64_t *p_nnz, int64_t *c_irp,
+                       int32_t *c_ja, double *c_val, int64_t *c_nnz,
+                       double *omega);
 
 /* MatrixMarket readers (the input path of tests/fileread/df_sample.f90:
  * 156-175): a square coordinate file (real, integer or pattern; general,

@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmldainv_hip.so")
 # info codes (include/mld_ainv_hip.h)
 SUCCESS, ERR_ARG, ERR_ALG, ERR_ALLOC, ERR_INTERNAL, ERR_PIVOT, ERR_STATE, \
     ERR_HIP, ERR_CAPACITY = range(9)
-KIND_AINV, KIND_INVK, KIND_INVT = 1, 2, 3
+KIND_AINV, KIND_INVK, KIND_INVT, KIND_ML = 1, 2, 3, 4
 ALG_LLK, ALG_S_LLK, ALG_S_FT_LLK, ALG_LLK_NOTH, ALG_MLK = 1, 2, 3, 4, 5
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 HOST_PTRS = 1
@@ -77,6 +77,13 @@ def lib():
         "ainv_clone_settings": [vp, vp],
         "ainv_clear_data": [vp],
         "ainv_export": [vp, C.c_char, i32p, i64p, i64p, i32p, dp],
+        "ainv_ml_levels": [vp, P(C.c_int)],
+        "ainv_ml_host_level": [C.c_int32, i64p, i32p, dp, C.c_int, C.c_double,
+                               C.c_int, C.c_int, C.c_int64, i32p, i32p, i64p,
+                               i32p, dp, P(C.c_int64), i64p, i32p, dp,
+                               P(C.c_int64), dp],
+        "ainv_ml_export": [vp, C.c_int, C.c_char, i32p, i32p, i64p, i64p,
+                           i32p, dp],
         "ainv_report": [vp, C.c_char_p, C.c_size_t],
         "ainv_mat_create": [C.c_int32, i64p, i32p, dp, C.c_int, P(vp)],
         "ainv_mat_free": [vp],
@@ -477,8 +484,52 @@ class MldDInvtSolver(_BaseAinvSolver):
     KIND = KIND_INVT
 
 
+class MldMlPrec(_BaseAinvSolver):
+    """Multilevel preconditioner (prec%init('ML') of MLD2P4 as the deck
+    tests/pdegen/runs/ppde.inp:10-29 configures it): smoothed aggregation,
+    AINV-family factors as smoother and coarsest-level solver, V-cycle.
+    Keys besides the factor keys: ML_TYPE, SMOOTHER_POS, SMOOTHER_SWEEPS,
+    N_PREC_LEVS, AGGR_KIND, AGGR_ALG, AGGR_THRESH, AGGR_SCALE, SUB_SOLVE,
+    COARSE_SUBSOLVE, COARSE_SWEEPS."""
+    KIND = KIND_ML
+
+    def levels(self):
+        n = C.c_int()
+        _ck(lib().ainv_ml_levels(self._h, C.byref(n)))
+        return n.value
+
+    def level(self, level, which):
+        """Level data: 'A', 'P', 'R', 'W', 'Z' as scipy CSR, 'D' (pivots)
+        as a float64 vector, 'G' (aggregates) as an int32 vector."""
+        import scipy.sparse as sp
+        w = which.upper()
+        nr, nc, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+        _ck(lib().ainv_ml_export(self._h, level, _b(w), C.byref(nr),
+                                 C.byref(nc), C.byref(nnz), None, None, None))
+        if w == "D":
+            d = np.empty(max(nnz.value, 1))
+            _ck(lib().ainv_ml_export(self._h, level, b"D", None, None, None,
+                                     None, None,
+                                     d.ctypes.data_as(C.POINTER(C.c_double))))
+            return d[:nnz.value]
+        if w == "G":
+            g = np.empty(max(nnz.value, 1), dtype=np.int32)
+            _ck(lib().ainv_ml_export(self._h, level, b"G", None, None, None,
+                                     None,
+                                     g.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     None))
+            return g[:nnz.value]
+        irp = np.empty(nr.value + 1, dtype=np.int64)
+        ja = np.empty(max(nnz.value, 1), dtype=np.int32)
+        val = np.empty(max(nnz.value, 1), dtype=np.float64)
+        _ck(lib().ainv_ml_export(self._h, level, _b(w), None, None, None,
+                                 *_ptrs(irp, ja, val)))
+        return sp.csr_matrix((val[:nnz.value], ja[:nnz.value], irp),
+                             shape=(nr.value, nc.value))
+
+
 SOLVERS = {"AINV": MldDAinvSolver, "INVK": MldDInvkSolver,
-           "INVT": MldDInvtSolver}
+           "INVT": MldDInvtSolver, "ML": MldMlPrec}
 
 
 # ---------------------------------------------------- system matrix, Krylov
@@ -636,6 +687,40 @@ def host_ainv_bld(a, alg="LLK", fill_in=8, thresh=0.1, threads=0):
 
 
 # --------------------------------------------------------------- generators
+def ml_host_level(a, theta=0.1, smoothed=True, threads=0):
+    """One coarsening step of the multilevel build on the host (no device):
+    (agg, P, R A P, omega)."""
+    import scipy.sparse as sp
+    n, irp, ja, val = _csr_args(a)
+    cap = max(16 * len(val), 1024)
+    while True:
+        agg = np.empty(max(n, 1), dtype=np.int32)
+        nagg = C.c_int32()
+        p_irp = np.empty(n + 1, dtype=np.int64)
+        c_irp = np.empty(n + 1, dtype=np.int64)
+        p_ja = np.empty(cap, dtype=np.int32)
+        c_ja = np.empty(cap, dtype=np.int32)
+        p_val = np.empty(cap)
+        c_val = np.empty(cap)
+        pn, cn, om = C.c_int64(), C.c_int64(), C.c_double()
+        rc = lib().ainv_ml_host_level(
+            n, *_ptrs(irp, ja, val), 0, theta, int(bool(smoothed)),
+            threads, cap, agg.ctypes.data_as(C.POINTER(C.c_int32)),
+            C.byref(nagg), *_ptrs(p_irp, p_ja, p_val), C.byref(pn),
+            *_ptrs(c_irp, c_ja, c_val), C.byref(cn), C.byref(om))
+        if rc == ERR_ARG and max(pn.value, cn.value) > cap:
+            cap = max(pn.value, cn.value)
+            continue
+        _ck(rc)
+        break
+    nc = nagg.value
+    P = sp.csr_matrix((p_val[:pn.value], p_ja[:pn.value], p_irp),
+                      shape=(n, nc))
+    Ac = sp.csr_matrix((c_val[:cn.value], c_ja[:cn.value], c_irp[:nc + 1]),
+                       shape=(nc, nc))
+    return agg[:n], P, Ac, om.value
+
+
 def _gen(fn, *args):
     import scipy.sparse as sp
     n = C.c_int32()

@@ -111,8 +111,11 @@ HostCsr mm_read_csr(const std::string &path);
 std::vector<double> mm_read_vector(const std::string &path);
 // Transpose of a device CSR (rows sorted by column in the result).
 void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s);
-// HLL from device CSR, values converted to `prec`.
-void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s);
+// HLL from device CSR, values converted to `prec`.  ncols: column count of
+// a rectangular matrix (padding entries point at a column below it); < 0
+// for a square one.
+void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
+                  int32_t ncols = -1);
 // Plain 16-byte-vector copy kernel (bandwidth reference).
 void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Convert n doubles to the storage type of prec (d vector).
@@ -150,6 +153,12 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
 // FP32 y from FP32 x (FP32 / BF16 factors only)
 void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, float *y, int64_t ldy, int k,
+                    hipStream_t s, int map = AINV_MAP_SPLIT,
+                    const int32_t *kw = nullptr);
+// y = alpha A x + beta z (FP64 A; x, z, y distinct with leading dimension
+// ld): the residual b - A u of the multilevel cycle in one pass
+void hll_spmm_resid(const Hll &a, double alpha, const double *x, double beta,
+                    const double *z, double *y, int64_t ld, int k,
                     hipStream_t s, int map = AINV_MAP_SPLIT,
                     const int32_t *kw = nullptr);
 // Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *

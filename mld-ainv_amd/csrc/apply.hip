@@ -162,7 +162,9 @@ __device__ __forceinline__ typename Val<VT>::acc row_dot(
   return acc0 + acc1;
 }
 
-enum { EPI_SCALE = 0, EPI_AXPBY = 1 };
+// RESID: out[r, c] = alpha * acc + beta * z[r, c], z passed as the d
+// argument with the leading dimension of out
+enum { EPI_SCALE = 0, EPI_AXPBY = 1, EPI_RESID = 2 };
 
 template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(
@@ -190,6 +192,9 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(
     if (!col_ok) continue;
     if constexpr (EPI == EPI_SCALE) {
       out[(int64_t)r * ldo + c] = (OT)((double)dvec[r] * (double)acc);
+    } else if constexpr (EPI == EPI_RESID) {
+      out[(int64_t)r * ldo + c] =
+          (OT)(alpha * (double)acc + beta * (double)dvec[(int64_t)r * ldo + c]);
     } else {
       double v = alpha * (double)acc;
       OT *yp = out + (int64_t)r * ldo + c;
@@ -345,6 +350,11 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_row(
 #pragma unroll
     for (int u = 0; u < K; ++u)
       if (u < k) o[u] = (OT)(d * (double)acc[u]);
+  } else if constexpr (EPI == EPI_RESID) {
+    const DT *zr = dvec + (int64_t)r * K;
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      if (u < k) o[u] = (OT)(alpha * (double)acc[u] + beta * (double)zr[u]);
   } else {
 #pragma unroll
     for (int u = 0; u < K; ++u) {
@@ -717,6 +727,12 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
 #pragma unroll
       for (int u = 0; u < CP::CPL; ++u)
         if (c0 + u < kw) o[u] = (OT)(d * (double)acc[q][u]);
+    } else if constexpr (EPI == EPI_RESID) {
+      const DT *zr = dvec + (int64_t)r * K + c0;
+#pragma unroll
+      for (int u = 0; u < CP::CPL; ++u)
+        if (c0 + u < kw)
+          o[u] = (OT)(alpha * (double)acc[q][u] + beta * (double)zr[u]);
     } else {
 #pragma unroll
       for (int u = 0; u < CP::CPL; ++u) {
@@ -1143,6 +1159,22 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
             a, static_cast<const double *>(x), ldx, k, (const float *)nullptr,
             y, ldy, alpha, beta, s);
   }
+}
+
+// y = alpha A x + beta z (FP64 matrix and vectors; z and y distinct, both
+// with the leading dimension ld of x)
+void hll_spmm_resid(const Hll &a, double alpha, const double *x, double beta,
+                    const double *z, double *y, int64_t ld, int k,
+                    hipStream_t s, int map, const int32_t *kw) {
+  if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
+  if (a.prec != AINV_PREC_F64)
+    throw Error(AINV_ERR_INTERNAL, "residual product needs an FP64 matrix");
+  if (use_row(map, k, ld, ld, x, y) && aligned16(z))
+    launch_lane<double, double, double, double, EPI_RESID>(map, a, x, k, z, y,
+                                                           alpha, beta, s, kw);
+  else
+    launch_k<double, double, double, double, EPI_RESID>(a, x, ld, k, z, y, ld,
+                                                        alpha, beta, s);
 }
 
 // FP32 output y = alpha Z t + beta y from an FP32 t (FP32 / BF16 factors):

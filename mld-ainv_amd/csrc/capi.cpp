@@ -58,10 +58,32 @@ void apply_defaults(ainv_solver *h) {
   h->inv_fill = 0;
   h->thresh = 0.0;
   h->inv_thresh = 0.0;
+  // multilevel: tests/pdegen/runs/ppde.inp:17-29 (3 levels, SMOOTHED DEC
+  // aggregation with threshold 0.10, MULT / POST, one sweep, AINV factors)
+  h->ml_nlev = 3;
+  h->ml_type = AINV_ML_MULT;
+  h->ml_pos = AINV_ML_POST;
+  h->ml_sweeps = 1;
+  h->ml_csweeps = 15;
+  h->ml_smoothed = 1;
+  h->ml_thresh = 0.1;
+  h->ml_scale = 1.0;
+  h->ml_sub = AINV_KIND_AINV;
+  h->ml_coarse = AINV_KIND_AINV;
 }
+
 
 // mld_check_def semantics: an illegal value is replaced by the default
 void check_params(ainv_solver *h) {
+  if (h->kind == AINV_KIND_ML) {
+    // factor parameters are checked per level, by the kind of each level
+    if (!(h->ml_nlev >= 1 && h->ml_nlev <= 16)) h->ml_nlev = 3;
+    if (!(h->ml_sweeps >= 1)) h->ml_sweeps = 1;
+    if (!(h->ml_csweeps >= 1)) h->ml_csweeps = 1;
+    if (!(h->ml_thresh >= 0.0)) h->ml_thresh = 0.0;
+    if (!(h->ml_scale > 0.0)) h->ml_scale = 1.0;
+    return;
+  }
   if (h->kind == AINV_KIND_AINV) {
     if (!(h->fill_in >= 1)) h->fill_in = 1;          // is_positive_nz_min
     if (!(h->thresh >= 0.0)) h->thresh = 0.0;        // is_legal_d_fact_thrs
@@ -179,6 +201,7 @@ void clear_built(ainv_solver *h) {
   h->tbuf = DevBuf();
   h->stage_x = DevBuf();
   h->stage_y = DevBuf();
+  h->ml.reset();
   h->inv_report.clear();
 }
 
@@ -209,6 +232,13 @@ void do_build(ainv_solver *h, const HostCsr &a) {
   const double t0 = now_seconds();
   clear_built(h);
   ensure_stream(h);
+  if (h->kind == AINV_KIND_ML) {
+    ml_build(h, a);
+    h->generation = next_generation();
+    h->built = true;
+    h->t_total = now_seconds() - t0;
+    return;
+  }
   h->n = a.n;
   std::vector<double> d;
   if (h->kind == AINV_KIND_AINV) {
@@ -258,6 +288,36 @@ void copy_settings(const ainv_solver *src, ainv_solver *dst) {
   dst->inv_thresh = src->inv_thresh;
   dst->prec = src->prec;
   dst->map_mode = src->map_mode;
+  dst->ml_nlev = src->ml_nlev;
+  dst->ml_type = src->ml_type;
+  dst->ml_pos = src->ml_pos;
+  dst->ml_sweeps = src->ml_sweeps;
+  dst->ml_csweeps = src->ml_csweeps;
+  dst->ml_smoothed = src->ml_smoothed;
+  dst->ml_thresh = src->ml_thresh;
+  dst->ml_scale = src->ml_scale;
+  dst->ml_sub = src->ml_sub;
+  dst->ml_coarse = src->ml_coarse;
+}
+
+// AINV / INVK / INVT from a name (SUB_SOLVE, COARSE_SUBSOLVE)
+int kind_from_string(const std::string &v) {
+  if (v == "AINV") return AINV_KIND_AINV;
+  if (v == "INVK") return AINV_KIND_INVK;
+  if (v == "INVT") return AINV_KIND_INVT;
+  throw Error(AINV_ERR_ARG, "unknown factor kind '" + v +
+                                "' (AINV, INVK, INVT)");
+}
+
+const char *kind_name(int k) {
+  return k == AINV_KIND_AINV ? "AINV" : k == AINV_KIND_INVK ? "INVK" : "INVT";
+}
+
+void not_ml(const ainv_solver *h, const char *what) {
+  if (h->kind == AINV_KIND_ML)
+    throw Error(AINV_ERR_STATE,
+                std::string(what) +
+                    " is not offered for the multilevel preconditioner");
 }
 
 int map_from_string(const std::string &v) {
@@ -355,7 +415,13 @@ void apply_param(ainv_solver *h, const std::string &key,
     throw Error(rc, "parameter " + key + "=" + val + ": " + last_error());
 }
 
+void solver_remold(ainv_solver *h, int prec) {
+  h->prec = prec;
+  if (h->built) build_layout(h);
+}
+
 int64_t solver_device_bytes(const ainv_solver *h) {
+  if (h->kind == AINV_KIND_ML) return h->ml ? ml_device_bytes(h) : 0;
   int64_t b = 0;
   b += (int64_t)(h->w.ptr.bytes + h->w.idx.bytes + h->w.val.bytes);
   b += (int64_t)(h->z.ptr.bytes + h->z.idx.bytes + h->z.val.bytes);
@@ -365,6 +431,7 @@ int64_t solver_device_bytes(const ainv_solver *h) {
 }
 
 size_t solver_t_bytes(const ainv_solver *h, int k) {
+  if (h->kind == AINV_KIND_ML) return ml_ws_bytes(h, k);
   return (size_t)h->n * k *
          (h->prec != AINV_PREC_F64 ? sizeof(float) : sizeof(double));
 }
@@ -378,6 +445,10 @@ void solver_apply_into(const ainv_solver *h, const double *x, double *y,
 }
 
 void solver_ensure_transposed(ainv_solver *h) {
+  if (h->kind == AINV_KIND_ML)
+    throw Error(AINV_ERR_ARG,
+                "BICG applies M^T, which the multilevel preconditioner does "
+                "not offer");
   ensure_transposed(h);
   AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
 }
@@ -416,6 +487,10 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
                   double *y, char trans, int k, int64_t ldx, int64_t ldy,
                   hipStream_t s) {
   if (!h->built) throw Error(AINV_ERR_STATE, "apply before build");
+  if (h->kind == AINV_KIND_ML) {
+    ml_apply_abi(h, alpha, x, beta, y, trans, k, ldx, ldy, s);
+    return;
+  }
   const char t = (char)std::toupper((unsigned char)trans);
   if (t != 'N' && t != 'T' && t != 'C')
     throw Error(AINV_ERR_ARG,
@@ -472,7 +547,7 @@ int ainv_create(int kind, ainv_h *out) {
   return guard([&] {
     if (!out) throw Error(AINV_ERR_ARG, "NULL output handle");
     if (kind != AINV_KIND_AINV && kind != AINV_KIND_INVK &&
-        kind != AINV_KIND_INVT)
+        kind != AINV_KIND_INVT && kind != AINV_KIND_ML)
       throw Error(AINV_ERR_ARG, "unknown solver kind");
     auto *h = new ainv_solver();
     h->kind = kind;
@@ -518,12 +593,31 @@ int ainv_cseti(ainv_h h, const char *what, int val) {
     check_handle(h);
     std::lock_guard<std::mutex> lk(h->mu);
     const std::string w = upper_trim(what);
+    const bool ml = h->kind == AINV_KIND_ML;
     if (w == "SUB_FILLIN") {
       h->fill_in = val;
-    } else if (w == "AINV_ALG" && h->kind == AINV_KIND_AINV) {
+    } else if (w == "AINV_ALG" && (h->kind == AINV_KIND_AINV || ml)) {
       h->alg = val;
-    } else if (w == "INV_FILLIN" && h->kind != AINV_KIND_AINV) {
+    } else if (w == "INV_FILLIN" && (h->kind != AINV_KIND_AINV || ml)) {
       h->inv_fill = val;
+    } else if (ml && w == "N_PREC_LEVS") {
+      h->ml_nlev = val;
+    } else if (ml && w == "SMOOTHER_SWEEPS") {
+      h->ml_sweeps = val;
+    } else if (ml && w == "COARSE_SWEEPS") {
+      h->ml_csweeps = val;
+    } else if (ml && w == "ML_TYPE") {
+      if (val != AINV_ML_MULT && val != AINV_ML_ADD)
+        throw Error(AINV_ERR_ARG, "ML_TYPE out of range");
+      h->ml_type = val;
+    } else if (ml && w == "SMOOTHER_POS") {
+      if (val < AINV_ML_PRE || val > AINV_ML_TWOSIDE)
+        throw Error(AINV_ERR_ARG, "SMOOTHER_POS out of range");
+      h->ml_pos = val;
+    } else if (ml && (w == "SUB_SOLVE" || w == "COARSE_SUBSOLVE")) {
+      if (val < AINV_KIND_AINV || val > AINV_KIND_INVT)
+        throw Error(AINV_ERR_ARG, w + " out of range");
+      (w == "SUB_SOLVE" ? h->ml_sub : h->ml_coarse) = val;
     } else if (w == "FACTOR_PREC") {
       if (val < AINV_PREC_F64 || val > AINV_PREC_BF16)
         throw Error(AINV_ERR_ARG, "FACTOR_PREC out of range");
@@ -540,10 +634,15 @@ int ainv_csetr(ainv_h h, const char *what, double val) {
     check_handle(h);
     std::lock_guard<std::mutex> lk(h->mu);
     const std::string w = upper_trim(what);
-    if (w == "SUB_ILUTHRS" && h->kind != AINV_KIND_INVK) {
+    const bool ml = h->kind == AINV_KIND_ML;
+    if (w == "SUB_ILUTHRS" && (h->kind != AINV_KIND_INVK || ml)) {
       h->thresh = val;
-    } else if (w == "INV_THRESH" && h->kind == AINV_KIND_INVT) {
+    } else if (w == "INV_THRESH" && (h->kind == AINV_KIND_INVT || ml)) {
       h->inv_thresh = val;
+    } else if (ml && w == "AGGR_THRESH") {
+      h->ml_thresh = val;
+    } else if (ml && w == "AGGR_SCALE") {
+      h->ml_scale = val;
     }
   });
 }
@@ -554,10 +653,43 @@ int ainv_csetc(ainv_h h, const char *what, const char *val) {
     std::lock_guard<std::mutex> lk(h->mu);
     const std::string w = upper_trim(what);
     const std::string v = upper_trim(val);
-    if (w == "AINV_ALG" && h->kind == AINV_KIND_AINV) {
+    const bool ml = h->kind == AINV_KIND_ML;
+    if (w == "AINV_ALG" && (h->kind == AINV_KIND_AINV || ml)) {
       const int a = stringval(v);
       // an unknown name yields an id no variant has; build reports it
       h->alg = a < 0 ? 0 : a;
+    } else if (ml && w == "ML_TYPE") {
+      if (v == "MULT")
+        h->ml_type = AINV_ML_MULT;
+      else if (v == "ADD")
+        h->ml_type = AINV_ML_ADD;
+      else
+        throw Error(AINV_ERR_ARG, "unknown ML_TYPE '" + v + "' (MULT, ADD)");
+    } else if (ml && w == "SMOOTHER_POS") {
+      if (v == "PRE")
+        h->ml_pos = AINV_ML_PRE;
+      else if (v == "POST")
+        h->ml_pos = AINV_ML_POST;
+      else if (v == "TWOSIDE")
+        h->ml_pos = AINV_ML_TWOSIDE;
+      else
+        throw Error(AINV_ERR_ARG,
+                    "unknown SMOOTHER_POS '" + v + "' (PRE, POST, TWOSIDE)");
+    } else if (ml && w == "AGGR_KIND") {
+      if (v == "SMOOTHED")
+        h->ml_smoothed = 1;
+      else if (v == "NONSMOOTHED")
+        h->ml_smoothed = 0;
+      else
+        throw Error(AINV_ERR_ARG,
+                    "unknown AGGR_KIND '" + v + "' (SMOOTHED, NONSMOOTHED)");
+    } else if (ml && w == "AGGR_ALG") {
+      if (v != "DEC")
+        throw Error(AINV_ERR_ARG, "unknown AGGR_ALG '" + v + "' (DEC)");
+    } else if (ml && w == "SUB_SOLVE") {
+      h->ml_sub = kind_from_string(v);
+    } else if (ml && w == "COARSE_SUBSOLVE") {
+      h->ml_coarse = kind_from_string(v);
     } else if (w == "FACTOR_PREC") {
       h->prec = prec_from_string(v);
     } else if (w == "APPLY_MAP") {
@@ -581,6 +713,22 @@ int ainv_geti(ainv_h h, const char *what, int *val) {
       *val = h->prec;
     else if (w == "APPLY_MAP")
       *val = h->map_mode;
+    else if (h->kind == AINV_KIND_ML && w == "N_PREC_LEVS")
+      *val = h->ml_nlev;
+    else if (h->kind == AINV_KIND_ML && w == "SMOOTHER_SWEEPS")
+      *val = h->ml_sweeps;
+    else if (h->kind == AINV_KIND_ML && w == "COARSE_SWEEPS")
+      *val = h->ml_csweeps;
+    else if (h->kind == AINV_KIND_ML && w == "ML_TYPE")
+      *val = h->ml_type;
+    else if (h->kind == AINV_KIND_ML && w == "SMOOTHER_POS")
+      *val = h->ml_pos;
+    else if (h->kind == AINV_KIND_ML && w == "AGGR_KIND")
+      *val = h->ml_smoothed;
+    else if (h->kind == AINV_KIND_ML && w == "SUB_SOLVE")
+      *val = h->ml_sub;
+    else if (h->kind == AINV_KIND_ML && w == "COARSE_SUBSOLVE")
+      *val = h->ml_coarse;
     else
       throw Error(AINV_ERR_ARG, "unknown integer key " + w);
   });
@@ -595,6 +743,10 @@ int ainv_getr(ainv_h h, const char *what, double *val) {
       *val = h->thresh;
     else if (w == "INV_THRESH")
       *val = h->inv_thresh;
+    else if (h->kind == AINV_KIND_ML && w == "AGGR_THRESH")
+      *val = h->ml_thresh;
+    else if (h->kind == AINV_KIND_ML && w == "AGGR_SCALE")
+      *val = h->ml_scale;
     else
       throw Error(AINV_ERR_ARG, "unknown real key " + w);
   });
@@ -614,7 +766,30 @@ int ainv_descr(ainv_h h, char *buf, size_t len) {
     if (!buf || len == 0) throw Error(AINV_ERR_ARG, "NULL buffer");
     char tmp[1024];
     int m = 0;
-    if (h->kind == AINV_KIND_AINV) {
+    if (h->kind == AINV_KIND_ML) {
+      m = std::snprintf(
+          tmp, sizeof tmp,
+          "  Multilevel preconditioner (smoothed aggregation) \n"
+          "  Levels                 : %d\n"
+          "  Aggregation            : %s DEC, threshold %.16g, scale %.16g\n"
+          "  Cycle                  : %s%s, %d sweep(s)\n"
+          "  Smoother factors       : %s\n"
+          "  Coarsest-level factors : %s, %d sweep(s)\n"
+          "  Fill level             : %d\n"
+          "  Fill threshold         : %.16g\n"
+          "  Inverse fill level     : %d\n"
+          "  Inverse fill threshold : %.16g\n",
+          h->ml_nlev, h->ml_smoothed ? "SMOOTHED" : "NONSMOOTHED",
+          h->ml_thresh, h->ml_scale,
+          h->ml_type == AINV_ML_ADD ? "ADD" : "MULT",
+          h->ml_type == AINV_ML_ADD
+              ? ""
+              : (h->ml_pos == AINV_ML_PRE    ? " PRE"
+                 : h->ml_pos == AINV_ML_POST ? " POST"
+                                             : " TWOSIDE"),
+          h->ml_sweeps, kind_name(h->ml_sub), kind_name(h->ml_coarse),
+          h->ml_csweeps, h->fill_in, h->thresh, h->inv_fill, h->inv_thresh);
+    } else if (h->kind == AINV_KIND_AINV) {
       m = std::snprintf(
           tmp, sizeof tmp,
           "  AINV: Approximate Inverse with sparse biconjugation \n"
@@ -677,6 +852,14 @@ int ainv_cnv(ainv_h h, int prec) {
     set_device(h->device);
     if (prec < AINV_PREC_F64 || prec > AINV_PREC_BF16)
       throw Error(AINV_ERR_ARG, "unknown factor precision");
+    if (h->kind == AINV_KIND_ML) {
+      h->prec = prec;
+      if (h->built) {
+        ml_remold(h, prec);
+        h->generation = next_generation();
+      }
+      return;
+    }
     h->prec = prec;
     if (h->built) build_layout(h);
   });
@@ -715,8 +898,12 @@ int ainv_get_nzeros(ainv_h h, int64_t *nz) {
   return guard([&] {
     check_handle(h);
     if (!nz) throw Error(AINV_ERR_ARG, "NULL output");
-    // n + nnz(W) + nnz(Z) (mld_d_base_ainv_mod.F90:179-192)
-    *nz = h->built ? (int64_t)h->n + h->w.nnz + h->z.nnz : 0;
+    // n + nnz(W) + nnz(Z) (mld_d_base_ainv_mod.F90:179-192); multilevel:
+    // that of every level's factors plus the coarse matrices, P and R
+    if (h->built && h->kind == AINV_KIND_ML)
+      *nz = ml_nzeros(h);
+    else
+      *nz = h->built ? (int64_t)h->n + h->w.nnz + h->z.nnz : 0;
   });
 }
 
@@ -725,7 +912,9 @@ int ainv_sizeof(ainv_h h, int64_t *bytes) {
     check_handle(h);
     if (!bytes) throw Error(AINV_ERR_ARG, "NULL output");
     int64_t b = (int64_t)sizeof(ainv_solver);
-    if (h->built) {
+    if (h->built && h->kind == AINV_KIND_ML) {
+      b += ml_device_bytes(h) + (int64_t)h->tbuf.bytes;
+    } else if (h->built) {
       b += (int64_t)(h->w.ptr.bytes + h->w.idx.bytes + h->w.val.bytes);
       b += (int64_t)(h->z.ptr.bytes + h->z.idx.bytes + h->z.val.bytes);
       b += (int64_t)h->d64.bytes + (int64_t)h->dv.bytes;
@@ -764,6 +953,7 @@ int ainv_clone(ainv_h h, ainv_h *out) {
       c->device = h->device;
       ensure_stream(c);
       if (h->built) {
+        not_ml(h, "cloning a built hierarchy");
         c->n = h->n;
         copy_csr_dev(h->w, c->w, c->stream);
         copy_csr_dev(h->z, c->z, c->stream);
@@ -814,6 +1004,7 @@ int ainv_export(ainv_h h, char which, int32_t *n, int64_t *nnz, int64_t *irp,
     check_handle(h);
     std::lock_guard<std::mutex> lk(h->mu);
     if (!h->built) throw Error(AINV_ERR_STATE, "export before build");
+    not_ml(h, "ainv_export (ainv_ml_export gives the levels)");
     set_device(h->device);
     const char w = (char)std::toupper((unsigned char)which);
     if (w == 'D') {
@@ -847,6 +1038,26 @@ int ainv_dump(ainv_h h, const char *prefix, int level, int solver,
     if (level < 0 || level > 999) throw Error(AINV_ERR_ARG, "level out of range");
     set_device(h->device);
     const std::string pre = prefix && *prefix ? prefix : "dump_ainv_d";
+    if (h->kind == AINV_KIND_ML) {
+      // one set of files per level, numbered from `level` up
+      const int nl = ml_levels(h);
+      if (level + nl - 1 > 999) throw Error(AINV_ERR_ARG, "level out of range");
+      for (int l = 0; l < nl; ++l) {
+        char tl[64];
+        std::snprintf(tl, sizeof tl, "_p000_l%03d_", level + l);
+        HostCsr w, z, unused;
+        int32_t nc = 0;
+        std::vector<double> d;
+        std::vector<int32_t> iv;
+        ml_export(h, l, 'W', w, nc, d, iv);
+        ml_export(h, l, 'Z', z, nc, d, iv);
+        ml_export(h, l, 'D', unused, nc, d, iv);
+        mm_write_csr(pre + tl + "wmat.mtx", w, head);
+        mm_write_vector(pre + tl + "diag.mtx", d, head);
+        mm_write_csr(pre + tl + "zmat.mtx", z, head);
+      }
+      return;
+    }
     char tail[64];
     std::snprintf(tail, sizeof tail, "_p000_l%03d_", level);
     HostCsr w, z;
@@ -868,6 +1079,7 @@ int ainv_import_factors(ainv_h h, int32_t n, const int64_t *w_irp,
   return guard([&] {
     check_handle(h);
     if (!d) throw Error(AINV_ERR_ARG, "NULL d");
+    not_ml(h, "importing factors");
     HostCsr w = canonical_csr(n, w_irp, w_ja, w_val, index_base);
     HostCsr z = canonical_csr(n, z_irp, z_ja, z_val, index_base);
     std::lock_guard<std::mutex> lk(h->mu);
@@ -895,6 +1107,7 @@ int ainv_update_a(ainv_h h, const double *x, int64_t nx, int flags) {
     if (!x) throw Error(AINV_ERR_ARG, "NULL x");
     std::lock_guard<std::mutex> lk(h->mu);
     if (!h->built) throw Error(AINV_ERR_STATE, "update_a before build");
+    not_ml(h, "update_a");
     if (nx < h->n)
       throw Error(AINV_ERR_ARG, "wrong size in update: " + std::to_string(nx) +
                                     " < " + std::to_string(h->n));
@@ -922,7 +1135,12 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
     check_handle(h);
     if (!buf || len == 0) throw Error(AINV_ERR_ARG, "NULL buffer");
     std::string r = "{\"built\": " + std::string(h->built ? "true" : "false");
-    if (h->built) {
+    if (h->built && h->kind == AINV_KIND_ML) {
+      char tb[128];
+      std::snprintf(tb, sizeof tb, ", \"t_total\": %.6f", h->t_total);
+      r += ", \"n\": " + std::to_string(h->n) + tb + ", \"prec\": \"" +
+           prec_name(h->prec) + "\", \"multilevel\": " + ml_report(h);
+    } else if (h->built) {
       auto hll_json = [](const Hll &x) {
         return "{\"nnz\": " + std::to_string(x.nnz) +
                ", \"padded\": " + std::to_string(x.padded) +
@@ -964,6 +1182,83 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
     }
     r += "}";
     std::snprintf(buf, len, "%s", r.c_str());
+  });
+}
+
+int ainv_ml_levels(ainv_h h, int *nlev) {
+  return guard([&] {
+    check_handle(h);
+    if (!nlev) throw Error(AINV_ERR_ARG, "NULL output");
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->kind != AINV_KIND_ML) throw Error(AINV_ERR_ARG, "not a multilevel handle");
+    *nlev = h->built ? ml_levels(h) : 0;
+  });
+}
+
+int ainv_ml_export(ainv_h h, int level, char which, int32_t *nrows,
+                   int32_t *ncols, int64_t *nnz, int64_t *irp, int32_t *ja,
+                   double *val) {
+  return guard([&] {
+    check_handle(h);
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (h->kind != AINV_KIND_ML) throw Error(AINV_ERR_ARG, "not a multilevel handle");
+    if (!h->built) throw Error(AINV_ERR_STATE, "export before build");
+    set_device(h->device);
+    const char w = (char)std::toupper((unsigned char)which);
+    HostCsr m;
+    int32_t nc = 0;
+    std::vector<double> dv;
+    std::vector<int32_t> iv;
+    ml_export(h, level, w, m, nc, dv, iv);
+    if (w == 'D' || w == 'G') {
+      const int64_t cnt = w == 'D' ? (int64_t)dv.size() : (int64_t)iv.size();
+      if (nrows) *nrows = (int32_t)cnt;
+      if (ncols) *ncols = 1;
+      if (nnz) *nnz = cnt;
+      if (w == 'D' && val) std::copy(dv.begin(), dv.end(), val);
+      if (w == 'G' && ja) std::copy(iv.begin(), iv.end(), ja);
+      return;
+    }
+    if (nrows) *nrows = m.n;
+    if (ncols) *ncols = nc;
+    if (nnz) *nnz = m.nnz();
+    if (irp) std::copy(m.ptr.begin(), m.ptr.end(), irp);
+    if (ja) std::copy(m.idx.begin(), m.idx.end(), ja);
+    if (val) std::copy(m.val.begin(), m.val.end(), val);
+  });
+}
+
+int ainv_ml_host_level(int32_t n, const int64_t *irp, const int32_t *ja,
+                       const double *val, int index_base, double theta,
+                       int smoothed, int threads, int64_t cap, int32_t *agg,
+                       int32_t *nagg, int64_t *p_irp, int32_t *p_ja,
+                       double *p_val, int64_t *p_nnz, int64_t *c_irp,
+                       int32_t *c_ja, double *c_val, int64_t *c_nnz,
+                       double *omega) {
+  return guard([&] {
+    if (!agg || !nagg || !p_irp || !p_ja || !p_val || !p_nnz || !c_irp ||
+        !c_ja || !c_val || !c_nnz || !omega)
+      throw Error(AINV_ERR_ARG, "NULL output");
+    if (threads < 0) throw Error(AINV_ERR_ARG, "threads must be >= 0");
+    HostCsr a = canonical_csr(n, irp, ja, val, index_base);
+    std::vector<int32_t> g;
+    int32_t nc = 0;
+    HostCsr p, ac;
+    double om = 0.0;
+    ml_host_level(a, theta, smoothed != 0, threads, g, nc, p, ac, om);
+    *p_nnz = p.nnz();
+    *c_nnz = ac.nnz();
+    *nagg = nc;
+    *omega = om;
+    if (p.nnz() > cap || ac.nnz() > cap)
+      throw Error(AINV_ERR_ARG, "output capacity too small");
+    std::copy(g.begin(), g.end(), agg);
+    std::copy(p.ptr.begin(), p.ptr.end(), p_irp);
+    std::copy(p.idx.begin(), p.idx.end(), p_ja);
+    std::copy(p.val.begin(), p.val.end(), p_val);
+    std::copy(ac.ptr.begin(), ac.ptr.end(), c_irp);
+    std::copy(ac.idx.begin(), ac.idx.end(), c_ja);
+    std::copy(ac.val.begin(), ac.val.end(), c_val);
   });
 }
 
@@ -1171,6 +1466,7 @@ int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
     std::lock_guard<std::mutex> lk(h->mu);
     set_device(h->device);
     if (!h->built) throw Error(AINV_ERR_STATE, "profile before build");
+    not_ml(h, "ainv_apply_profile");
     if (reps < 1 || k < 1 || k > 64 || !x || !y || !ms_first || !ms_second)
       throw Error(AINV_ERR_ARG, "bad profile arguments");
     const bool f32t = h->prec != AINV_PREC_F64;
@@ -1289,6 +1585,7 @@ int ainv_layout(ainv_h h, int64_t *n, int64_t *nnz_w, int64_t *nnz_z,
   return guard([&] {
     check_handle(h);
     if (!h->built) throw Error(AINV_ERR_STATE, "layout before build");
+    not_ml(h, "ainv_layout");
     if (n) *n = h->n;
     if (nnz_w) *nnz_w = h->hw.nnz;
     if (nnz_z) *nnz_z = h->hz.nnz;

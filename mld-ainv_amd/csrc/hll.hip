@@ -192,7 +192,8 @@ __global__ void k_hll_fill(int32_t n, int32_t nslices,
                            const int32_t *__restrict__ idx,
                            const double *__restrict__ val,
                            const int64_t *__restrict__ sptr,
-                           int32_t *__restrict__ col, VT *__restrict__ hv) {
+                           int32_t *__restrict__ col, VT *__restrict__ hv,
+                           int32_t ncols) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int32_t s = (int32_t)(r / kSlice);
   if (s >= nslices) return;
@@ -202,6 +203,7 @@ __global__ void k_hll_fill(int32_t n, int32_t nslices,
   int64_t beg = 0;
   int32_t len = 0;
   int32_t self = (int32_t)(r < n ? r : n - 1);
+  if (self >= ncols) self = ncols - 1;  // rectangular: stay inside x
   if (r < n) {
     beg = ptr[r];
     len = (int32_t)(ptr[r + 1] - beg);
@@ -227,7 +229,10 @@ __global__ void k_convert(int64_t n, const double *__restrict__ src,
 
 }  // namespace
 
-void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s) {
+void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
+                  int32_t ncols) {
+  if (ncols < 0) ncols = a.n;
+  if (ncols < 1) ncols = 1;
   const int vb = value_bytes(prec);
   h = Hll();
   h.n = a.n;
@@ -270,19 +275,19 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s) {
         hipLaunchKernelGGL(k_hll_fill<double>, g, dim3(kThreads), 0, s, a.n,
                            h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
                            a.val.as<double>(), h.sptr.as<int64_t>(),
-                           h.col.as<int32_t>(), h.val.as<double>());
+                           h.col.as<int32_t>(), h.val.as<double>(), ncols);
         break;
       case AINV_PREC_F32:
         hipLaunchKernelGGL(k_hll_fill<float>, g, dim3(kThreads), 0, s, a.n,
                            h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
                            a.val.as<double>(), h.sptr.as<int64_t>(),
-                           h.col.as<int32_t>(), h.val.as<float>());
+                           h.col.as<int32_t>(), h.val.as<float>(), ncols);
         break;
       default:
         hipLaunchKernelGGL(k_hll_fill<uint16_t>, g, dim3(kThreads), 0, s, a.n,
                            h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
                            a.val.as<double>(), h.sptr.as<int64_t>(),
-                           h.col.as<int32_t>(), h.val.as<uint16_t>());
+                           h.col.as<int32_t>(), h.val.as<uint16_t>(), ncols);
     }
     AINV_HIP_CHECK(hipGetLastError());
   }

@@ -526,7 +526,7 @@ int64_t x32_stride(const KrylovPlan *p) {
 
 void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
   if ((p->method != KM_BICGSTAB && p->method != KM_CG) || !m ||
-      m->prec == AINV_PREC_F64 || p->work32.p)
+      m->prec == AINV_PREC_F64 || m->kind == AINV_KIND_ML || p->work32.p)
     return;
   const int nv = p->method == KM_BICGSTAB ? 4 : 2;
   p->work32.alloc((size_t)nv * x32_stride(p) * sizeof(float));
@@ -534,11 +534,22 @@ void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
   AINV_HIP_CHECK(hipMemsetAsync(p->work32.p, 0, p->work32.bytes, s));
 }
 
+// Preconditioner workspace of a plan; the multilevel cycle's is cleared
+// (its vectors' padding columns are read by the row-mapped kernels).
+void alloc_tbuf(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
+  const size_t need = solver_t_bytes(m, p->K);
+  if (p->tbuf.bytes >= need) return;
+  p->tbuf.alloc(need);
+  if (m->kind == AINV_KIND_ML)
+    AINV_HIP_CHECK(hipMemsetAsync(p->tbuf.p, 0, need, s));
+}
+
 // The plan's vectors with the FP32 copies the method writes when its
 // preconditioner has FP32 / BF16 factors (none otherwise).
 Vecs x32_vecs(const KrylovPlan *p) {
   Vecs w = p->v;
-  if (p->m && p->m->prec != AINV_PREC_F64 && p->work32.p) {
+  if (p->m && p->m->prec != AINV_PREC_F64 && p->m->kind != AINV_KIND_ML &&
+      p->work32.p) {
     float *f = p->work32.as<float>();
     if (p->method == KM_BICGSTAB) {
       const int64_t nk = x32_stride(p);
@@ -561,8 +572,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
     if (p->method == method && p->m == m && p->k == k && p->irst == irst) {
       if (m && p->gen != m->generation) {
         p->release_graph();  // factors changed: recapture
-        if (p->tbuf.bytes < solver_t_bytes(m, p->K))
-          p->tbuf.alloc(solver_t_bytes(m, p->K));
+        alloc_tbuf(p, m, s);
         alloc_x32(p, m, s);
         p->gen = m->generation;
       }
@@ -606,7 +616,7 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
   p->part.alloc(sizeof(double) * kMaxPairs * kRedBlocks * kMaxCols);
   p->nact.alloc(3 * sizeof(int32_t));
   AINV_HIP_CHECK(hipMemsetAsync(p->nact.p, 0, 3 * sizeof(int32_t), s));
-  if (m) p->tbuf.alloc(solver_t_bytes(m, p->K));
+  if (m) alloc_tbuf(p, m, s);
   alloc_x32(p, m, s);
   p->col = p->colbuf.as<Col>();
   p->pt = p->part.as<double>();
@@ -623,7 +633,9 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
 }
 
 void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
-  if (p->m)
+  if (p->m && p->m->kind == AINV_KIND_ML)
+    ml_apply_into(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->kwu);
+  else if (p->m)
     solver_apply_into(p->m, x, y, p->k, p->K, p->tbuf.p, s, p->apply_map,
                       p->kwu);
   else
@@ -701,7 +713,9 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
     return;
   }
   const Vecs &v = p->v;
-  if (m) {
+  if (m && m->kind == AINV_KIND_ML) {
+    ml_tune(m, K, p->k, v.b, v.z, p->tbuf.p, s);
+  } else if (m) {
     if (m->apply_map[li] < 0) {
       // the two passes are timed separately; t keeps leading dimension K
       const bool f32t = m->prec != AINV_PREC_F64;

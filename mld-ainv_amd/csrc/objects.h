@@ -17,6 +17,12 @@ void destroy_plan(KrylovPlan *p);
 struct PlanDeleter {
   void operator()(KrylovPlan *p) const { destroy_plan(p); }
 };
+// Multilevel hierarchy of a solver of kind AINV_KIND_ML (ml.hip)
+struct MlHier;
+void destroy_ml(MlHier *m);
+struct MlDeleter {
+  void operator()(MlHier *m) const { destroy_ml(m); }
+};
 }  // namespace ainv
 
 struct ainv_solver {
@@ -33,6 +39,18 @@ struct ainv_solver {
   // COOP / COOP128 for both passes
   int map_mode = -1;
   int host_threads = 0;  // host threads of the AINV build (0: the default)
+  // multilevel parameters (kind AINV_KIND_ML); the factor parameters above
+  // apply to the factors of every level
+  int ml_nlev = 3;
+  int ml_type = AINV_ML_MULT;
+  int ml_pos = AINV_ML_POST;
+  int ml_sweeps = 1;
+  int ml_csweeps = 15;
+  int ml_smoothed = 1;
+  double ml_thresh = 0.1;
+  double ml_scale = 1.0;
+  int ml_sub = AINV_KIND_AINV;
+  int ml_coarse = AINV_KIND_AINV;
 
   // ---- device / stream
   int device = 0;
@@ -56,6 +74,9 @@ struct ainv_solver {
   // (packed pair: first and second pass, map_pair)
   int16_t apply_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   double apply_ms[8][2][ainv::kNumMaps] = {};  // [K][pass][map]
+
+  // ---- multilevel hierarchy (kind AINV_KIND_ML)
+  std::unique_ptr<ainv::MlHier, ainv::MlDeleter> ml;
 
   // ---- build report
   double t_factor = 0, t_inverse = 0, t_layout = 0, t_total = 0;
@@ -103,6 +124,51 @@ int log2_pow2(int K);
 int tuned_apply_map(const ainv_solver *h, int K);
 void reset_tuning(ainv_solver *h);
 size_t solver_t_bytes(const ainv_solver *h, int k);
+// Builds the factors of h (its kind and parameters) for a (capi.cpp).
+void build_solver(ainv_solver *h, const HostCsr &a);
+int64_t solver_device_bytes(const ainv_solver *h);
+
+// ---- multilevel preconditioner (ml.hip) ------------------------------------
+// Builds the smoothed-aggregation hierarchy of h (kind AINV_KIND_ML) for a:
+// level matrices, prolongators, restrictions and the factors of each level.
+void ml_build(ainv_solver *h, const HostCsr &a);
+// Device workspace of one cycle over k columns (leading dimension k).
+size_t ml_ws_bytes(const ainv_solver *h, int k);
+// y = V-cycle(x): x, y with leading dimension K = pow2 >= k (16-byte
+// aligned), ws of ml_ws_bytes(h, K) bytes; no allocation, stream-ordered, so
+// it may be captured in a graph.  kw as for the apply kernels.
+void ml_apply_into(const ainv_solver *h, const double *x, double *y, int k,
+                   int64_t K, void *ws, hipStream_t s,
+                   const int32_t *kw = nullptr);
+// y = alpha * V-cycle(x) + beta * y for the C ABI (any leading dimensions;
+// trans 'N' only), workspace in h->tbuf.
+void ml_apply_abi(ainv_solver *h, double alpha, const double *x, double beta,
+                  double *y, char trans, int k, int64_t ldx, int64_t ldy,
+                  hipStream_t s);
+// Times the kernel mappings of every level's products for block width K and
+// keeps the fastest; x0 (read) and y0 (overwritten) are n x K vectors of the
+// finest level, ws the cycle's workspace (cleared first).  Before a capture.
+void ml_tune(ainv_solver *h, int K, int k, const double *x0, double *y0,
+             void *ws, hipStream_t s);
+// One coarsening step on the host alone: aggregates, prolongator P and
+// Galerkin matrix R (A P), on `threads` threads (0: the default count).
+void ml_host_level(const HostCsr &a, double theta, bool smoothed, int threads,
+                   std::vector<int32_t> &agg, int32_t &nagg, HostCsr &p,
+                   HostCsr &ac, double &omega);
+// Re-molds the factors of every level to another storage precision.
+void ml_remold(ainv_solver *h, int prec);
+// Storage precision change of a built solver's factors (capi.cpp).
+void solver_remold(ainv_solver *h, int prec);
+int ml_levels(const ainv_solver *h);
+int64_t ml_device_bytes(const ainv_solver *h);
+int64_t ml_nzeros(const ainv_solver *h);
+std::string ml_report(const ainv_solver *h);
+// Host copies of level l: 'A', 'P', 'R' (CSR), 'W', 'Z' (factors, CSR),
+// 'D' (pivots) and 'G' (aggregates of the rows); ncols of the CSR ones.
+void ml_export(const ainv_solver *h, int l, char which, HostCsr &m,
+               int32_t &ncols, std::vector<double> &dv,
+               std::vector<int32_t> &iv);
+const ainv_solver *ml_level_solver(const ainv_solver *h, int l);
 
 // Continuous batching over K column slots of one (matrix, preconditioner):
 // admit right-hand sides into free slots between iterations, step one

@@ -211,8 +211,13 @@ static double spge_dot(int64_t nz, const int32_t *ja, const double *v,
 
 /* ------------------------------------------------------------------ */
 int or_transpose(const or_mat *a, or_mat *t) {
-  int32_t n = a->n;
-  int64_t nnz = a->ptr[n];
+  return or_transpose_rect(a, a->n, t);
+}
+
+/* transpose of an a->n x ncols matrix (result: ncols rows) */
+int or_transpose_rect(const or_mat *a, int32_t ncols, or_mat *t) {
+  int32_t n = ncols, nr = a->n;
+  int64_t nnz = a->ptr[nr];
   int64_t *tp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   int32_t *ti = (int32_t *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int32_t));
   double *tv = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
@@ -232,7 +237,7 @@ int or_transpose(const or_mat *a, or_mat *t) {
     return OR_ERR_ALLOC;
   }
   memcpy(pos, tp, (size_t)n * sizeof(int64_t));
-  for (int32_t i = 0; i < n; i++)
+  for (int32_t i = 0; i < nr; i++)
     for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++) {
       int64_t q = pos[a->idx[k]]++;
       ti[q] = i;
@@ -1193,6 +1198,319 @@ int or_update_a(const or_mat *w, const or_mat *z, const double *x, double *d) {
   return OR_OK;
 }
 
+/* ------------------------------------------------------------------ */
+/* Multilevel use of the AINV-family factors: smoothed aggregation.
+ *
+ * The multilevel framework the plugin's solvers plug into is MLD2P4
+ * (>= 2.1.1: configure.ac:283-289, README:13-14), which is not part of the
+ * reference tree.  The reference's deck tests/pdegen/runs/ppde.inp:10-29
+ * names the configuration: ML, AS smoother with AINV subdomain solver, one
+ * sweep, 3 levels, SMOOTHED DEC aggregation with threshold 0.10, MULT
+ * correction with POST smoothing.  Restated here from the published
+ * algorithms MLD2P4 implements (one process, so "decoupled" aggregation and
+ * AS / block Jacobi act on the whole matrix):
+ *  - greedy three-phase aggregation (Vanek, Mandel, Brezina, Computing 56
+ *    (1996) 179-196) over the strength graph |a_ij| > theta sqrt(|a_ii a_jj|);
+ *  - tentative prolongator T (row i: a unit entry in column agg[i]) and one
+ *    damped Jacobi step P = (I - omega D^-1 A) T with omega = 4 / (3 rho),
+ *    rho = ||D^-1 A||_inf;
+ *  - Galerkin coarse matrix R (A P) with R = P^T;
+ *  - V-cycle, multiplicative (PRE / POST / TWOSIDE smoothing) or additive.
+ * Every sum runs in a fixed order (Gustavson row products in stored order,
+ * no contraction) so that a hierarchy can be compared bit for bit. */
+
+static int strong_link(const or_mat *a, const double *dg, double theta,
+                       int32_t i, int64_t k) {
+  const int32_t j = a->idx[k];
+  if (j == i) return 0;
+  return fabs(a->val[k]) > theta * sqrt(fabs(dg[i] * dg[j]));
+}
+
+static double *diag_of(const or_mat *a) {
+  double *dg = (double *)calloc((size_t)a->n + 1, sizeof(double));
+  if (!dg) return NULL;
+  for (int32_t i = 0; i < a->n; i++)
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
+      if (a->idx[k] == i) dg[i] = a->val[k];
+  return dg;
+}
+
+int or_aggregate(const or_mat *a, double theta, int32_t *agg, int32_t *nagg) {
+  const int32_t n = a->n;
+  double *dg = diag_of(a);
+  int32_t *a1 = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  if (!dg || !a1) {
+    free(dg);
+    free(a1);
+    return OR_ERR_ALLOC;
+  }
+  for (int32_t i = 0; i < n; i++) agg[i] = -1;
+  int32_t na = 0;
+  /* phase 1: a node whose strong neighbours are all free roots a new
+   * aggregate made of it and them */
+  for (int32_t i = 0; i < n; i++) {
+    if (agg[i] >= 0) continue;
+    int ok = 1;
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1] && ok; k++)
+      if (strong_link(a, dg, theta, i, k) && agg[a->idx[k]] >= 0) ok = 0;
+    if (!ok) continue;
+    agg[i] = na;
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
+      if (strong_link(a, dg, theta, i, k)) agg[a->idx[k]] = na;
+    na++;
+  }
+  /* phase 2: a free node joins the phase-1 aggregate of its most strongly
+   * coupled neighbour (largest |a_ij|, first in column order on ties) */
+  memcpy(a1, agg, (size_t)n * sizeof(int32_t));
+  for (int32_t i = 0; i < n; i++) {
+    if (a1[i] >= 0) continue;
+    double bv = -1.0;
+    int32_t best = -1;
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++) {
+      const int32_t j = a->idx[k];
+      if (strong_link(a, dg, theta, i, k) && a1[j] >= 0 &&
+          fabs(a->val[k]) > bv) {
+        bv = fabs(a->val[k]);
+        best = a1[j];
+      }
+    }
+    if (best >= 0) agg[i] = best;
+  }
+  /* phase 3: the remaining nodes form aggregates with their free strong
+   * neighbours */
+  for (int32_t i = 0; i < n; i++) {
+    if (agg[i] >= 0) continue;
+    agg[i] = na;
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
+      if (strong_link(a, dg, theta, i, k) && agg[a->idx[k]] < 0)
+        agg[a->idx[k]] = na;
+    na++;
+  }
+  *nagg = na;
+  free(dg);
+  free(a1);
+  return OR_OK;
+}
+
+/* C = A B (B with bcols columns): row i of C accumulates a_ik * B(k, :) for
+ * the entries of row i of A in stored order, each sum starting from 0.0;
+ * the pattern is the symbolic product (cancelled entries are kept). */
+int or_spgemm(const or_mat *a, const or_mat *b, int32_t bcols, or_mat *c) {
+  const int32_t n = a->n;
+  double *acc = (double *)calloc((size_t)bcols + 1, sizeof(double));
+  int32_t *mark = (int32_t *)malloc(((size_t)bcols + 1) * sizeof(int32_t));
+  int32_t *list = (int32_t *)malloc(((size_t)bcols + 1) * sizeof(int32_t));
+  int64_t *cp = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
+  int64_t cap = 1024, nz = 0;
+  int32_t *ci = (int32_t *)malloc((size_t)cap * sizeof(int32_t));
+  double *cv = (double *)malloc((size_t)cap * sizeof(double));
+  int rc = OR_OK;
+  if (!acc || !mark || !list || !cp || !ci || !cv) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  for (int32_t j = 0; j < bcols; j++) mark[j] = -1;
+  for (int32_t i = 0; i < n; i++) {
+    int32_t cnt = 0;
+    for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++) {
+      const double aik = a->val[k];
+      const int32_t r = a->idx[k];
+      for (int64_t q = b->ptr[r]; q < b->ptr[r + 1]; q++) {
+        const int32_t j = b->idx[q];
+        if (mark[j] != i) {
+          mark[j] = i;
+          list[cnt++] = j;
+          acc[j] = 0.0;
+        }
+        acc[j] = acc[j] + aik * b->val[q];
+      }
+    }
+    qsort(list, (size_t)cnt, sizeof(int32_t), cmp_i32);
+    if (nz + cnt > cap) {
+      while (nz + cnt > cap) cap *= 2;
+      int32_t *ni = (int32_t *)realloc(ci, (size_t)cap * sizeof(int32_t));
+      if (ni) ci = ni;
+      double *nv = (double *)realloc(cv, (size_t)cap * sizeof(double));
+      if (nv) cv = nv;
+      if (!ni || !nv) {
+        rc = OR_ERR_ALLOC;
+        goto done;
+      }
+    }
+    for (int32_t t = 0; t < cnt; t++) {
+      ci[nz] = list[t];
+      cv[nz] = acc[list[t]];
+      nz++;
+    }
+    cp[i + 1] = nz;
+  }
+  c->n = n;
+  c->nnz = nz;
+  c->ptr = cp;
+  c->idx = ci;
+  c->val = cv;
+  cp = NULL;
+  ci = NULL;
+  cv = NULL;
+done:
+  free(acc);
+  free(mark);
+  free(list);
+  free(cp);
+  free(ci);
+  free(cv);
+  return rc;
+}
+
+int or_ml_level(const or_mat *a, double theta, int smoothed, int32_t *agg,
+                int32_t *nagg, or_mat *p, or_mat *r, or_mat *ac,
+                double *omega) {
+  const int32_t n = a->n;
+  int rc = or_aggregate(a, theta, agg, nagg);
+  if (rc) return rc;
+  const int32_t nc = *nagg;
+  or_mat t = {0}, at = {0}, ap = {0};
+  t.n = n;
+  t.nnz = n;
+  t.ptr = (int64_t *)malloc(((size_t)n + 1) * sizeof(int64_t));
+  t.idx = (int32_t *)malloc(((size_t)n + 1) * sizeof(int32_t));
+  t.val = (double *)malloc(((size_t)n + 1) * sizeof(double));
+  double *dg = diag_of(a);
+  if (!t.ptr || !t.idx || !t.val || !dg) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  for (int32_t i = 0; i <= n; i++) t.ptr[i] = i;
+  for (int32_t i = 0; i < n; i++) {
+    t.idx[i] = agg[i];
+    t.val[i] = 1.0;
+  }
+  *omega = 0.0;
+  if (!smoothed) {
+    *p = t;
+    t.ptr = NULL;
+    t.idx = NULL;
+    t.val = NULL;
+  } else {
+    double rho = 0.0;
+    for (int32_t i = 0; i < n; i++) {
+      if (dg[i] == 0.0) {
+        rc = OR_ERR_PIVOT;
+        goto done;
+      }
+      double s = 0.0;
+      for (int64_t k = a->ptr[i]; k < a->ptr[i + 1]; k++)
+        s = s + fabs(a->val[k]);
+      s = s / fabs(dg[i]);
+      if (s > rho) rho = s;
+    }
+    const double om = 4.0 / (3.0 * rho);
+    *omega = om;
+    if ((rc = or_spgemm(a, &t, nc, &at))) goto done;
+    /* P = T - omega D^-1 (A T): the pattern of A T holds column agg[i] */
+    for (int32_t i = 0; i < n; i++) {
+      const double sc = om / dg[i];
+      for (int64_t k = at.ptr[i]; k < at.ptr[i + 1]; k++) {
+        const double tv = (at.idx[k] == agg[i]) ? 1.0 : 0.0;
+        at.val[k] = tv - sc * at.val[k];
+      }
+    }
+    *p = at;
+    at.ptr = NULL;
+    at.idx = NULL;
+    at.val = NULL;
+  }
+  if ((rc = or_transpose_rect(p, nc, r))) goto done;
+  if ((rc = or_spgemm(a, p, nc, &ap))) goto done;
+  rc = or_spgemm(r, &ap, nc, ac);
+done:
+  or_mat_free(&t);
+  or_mat_free(&at);
+  or_mat_free(&ap);
+  free(dg);
+  return rc;
+}
+
+/* u <- u + S (b - A u) with S = Z diag(d) W^T of level l; t: scratch */
+static void ml_sweep(const or_ml *ml, int32_t l, const double *b, double *u,
+                     double *t) {
+  const or_mat *a = &ml->a[l];
+  memcpy(t, b, (size_t)a->n * sizeof(double));
+  or_spmv('N', -1.0, a, u, 1.0, t);
+  or_apply('N', 1.0, &ml->w[l], &ml->z[l], ml->d[l], t, 1.0, u);
+}
+
+/* u = S b followed by nsw - 1 sweeps */
+static void ml_smooth0(const or_ml *ml, int32_t l, int32_t nsw,
+                       const double *b, double *u, double *t) {
+  or_apply('N', 1.0, &ml->w[l], &ml->z[l], ml->d[l], b, 0.0, u);
+  for (int32_t s = 1; s < nsw; s++) ml_sweep(ml, l, b, u, t);
+}
+
+static int ml_cycle(const or_ml *ml, int32_t l, const double *b, double *u) {
+  const int32_t n = ml->a[l].n;
+  double *t = (double *)malloc(((size_t)n + 1) * sizeof(double));
+  if (!t) return OR_ERR_ALLOC;
+  if (l == ml->nlev - 1) {
+    ml_smooth0(ml, l, ml->csweeps, b, u, t);
+    free(t);
+    return OR_OK;
+  }
+  const int32_t nc = ml->a[l + 1].n;
+  double *bc = (double *)malloc(((size_t)nc + 1) * sizeof(double));
+  double *uc = (double *)malloc(((size_t)nc + 1) * sizeof(double));
+  int rc = OR_OK;
+  if (!bc || !uc) {
+    rc = OR_ERR_ALLOC;
+    goto done;
+  }
+  if (ml->type == OR_ML_ADD) {
+    /* u = S^(nu) b + P V(R b) */
+    ml_smooth0(ml, l, ml->sweeps, b, u, t);
+    or_spmv('N', 1.0, &ml->r[l], b, 0.0, bc);
+    if ((rc = ml_cycle(ml, l + 1, bc, uc))) goto done;
+    or_spmv('N', 1.0, &ml->p[l], uc, 1.0, u);
+  } else {
+    const int pre = ml->pos & OR_ML_PRE, post = ml->pos & OR_ML_POST;
+    if (pre) {
+      ml_smooth0(ml, l, ml->sweeps, b, u, t);
+      memcpy(t, b, (size_t)n * sizeof(double));
+      or_spmv('N', -1.0, &ml->a[l], u, 1.0, t);
+      or_spmv('N', 1.0, &ml->r[l], t, 0.0, bc);
+    } else {
+      or_spmv('N', 1.0, &ml->r[l], b, 0.0, bc);
+    }
+    if ((rc = ml_cycle(ml, l + 1, bc, uc))) goto done;
+    or_spmv('N', 1.0, &ml->p[l], uc, pre ? 1.0 : 0.0, u);
+    if (post)
+      for (int32_t s = 0; s < ml->sweeps; s++) ml_sweep(ml, l, b, u, t);
+  }
+done:
+  free(t);
+  free(bc);
+  free(uc);
+  return rc;
+}
+
+int or_ml_apply(const or_ml *ml, const double *x, double *y) {
+  if (!ml || ml->nlev < 1 || ml->sweeps < 1 || ml->csweeps < 1 ||
+      (ml->type == OR_ML_MULT && (ml->pos < 1 || ml->pos > 3)))
+    return OR_ERR_ARG;
+  return ml_cycle(ml, 0, x, y);
+}
+
+/* preconditioner of the Krylov routines: the multilevel cycle while
+ * or_ml_krylov runs on this thread, else the factors w, z, d */
+static _Thread_local const or_ml *ml_prec = NULL;
+
+static void pc_apply(const or_mat *w, const or_mat *z, const double *d,
+                     const double *x, double *y) {
+  if (ml_prec)
+    (void)or_ml_apply(ml_prec, x, y);
+  else
+    or_apply('N', 1.0, w, z, d, x, 0.0, y);
+}
+
 static double dot(int32_t n, const double *a, const double *b) {
   double s = 0.0;
   _Pragma("omp parallel for schedule(static) reduction(+ : s) num_threads(or_threads) if (or_threads > 1)")
@@ -1235,14 +1553,14 @@ int or_bicgstab(const or_mat *a, const or_mat *w, const or_mat *z,
       OR_PAR
       for (int32_t i = 0; i < n; i++) p[i] = r[i] + beta * p[i];
     }
-    or_apply('N', 1.0, w, z, d, p, 0.0, f);
+    pc_apply(w, z, d, p, f);
     or_spmv('N', 1.0, a, f, 0.0, v);
     sigma = dot(n, q, v);
     if (sigma == 0.0) break;
     alpha = rho / sigma;
     OR_PAR
     for (int32_t i = 0; i < n; i++) s[i] = r[i] + (-alpha) * v[i];
-    or_apply('N', 1.0, w, z, d, s, 0.0, zz);
+    pc_apply(w, z, d, s, zz);
     or_spmv('N', 1.0, a, zz, 0.0, t);
     sigma = dot(n, t, t);
     if (sigma == 0.0) break;
@@ -1310,14 +1628,14 @@ int or_cgs(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
       for (int32_t i = 0; i < n; i++) u[i] = r[i] + beta * qq[i];
       for (int32_t i = 0; i < n; i++) p[i] = u[i] + beta * (qq[i] + beta * p[i]);
     }
-    or_apply('N', 1.0, w, z, d, p, 0.0, ph);
+    pc_apply(w, z, d, p, ph);
     or_spmv('N', 1.0, a, ph, 0.0, vh);
     sigma = dot(n, rt, vh);
     if (sigma == 0.0) break;
     alpha = rho / sigma;
     for (int32_t i = 0; i < n; i++) qq[i] = u[i] + (-alpha) * vh[i];
     for (int32_t i = 0; i < n; i++) uq[i] = u[i] + qq[i];
-    or_apply('N', 1.0, w, z, d, uq, 0.0, uh);
+    pc_apply(w, z, d, uq, uh);
     or_spmv('N', 1.0, a, uh, 0.0, qh);
     for (int32_t i = 0; i < n; i++) x[i] = x[i] + alpha * uh[i];
     for (int32_t i = 0; i < n; i++) r[i] = r[i] + (-alpha) * qh[i];
@@ -1366,7 +1684,7 @@ int or_bicg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
   if (*err <= eps) goto out;
   while (it < itmax) {
     it++;
-    or_apply('N', 1.0, w, z, d, r, 0.0, zz);
+    pc_apply(w, z, d, r, zz);
     or_apply('T', 1.0, w, z, d, rt, 0.0, zt);
     rho_old = rho;
     rho = dot(n, zz, rt);
@@ -1445,7 +1763,7 @@ int or_gmres(const or_mat *a, const or_mat *w, const or_mat *z,
     int32_t len = 0, conv = 0, brk = 0;
     for (int32_t i = 0; i < m && it < itmax; i++) {
       it++;
-      or_apply('N', 1.0, w, z, d, VV(i), 0.0, zv);
+      pc_apply(w, z, d, VV(i), zv);
       or_spmv('N', 1.0, a, zv, 0.0, wv);
       for (int32_t j = 0; j <= i; j++) {
         const double hj = dot(n, VV(j), wv);
@@ -1494,7 +1812,7 @@ int or_gmres(const or_mat *a, const or_mat *w, const or_mat *z,
         for (int32_t j = 0; j < len; j++) u = u + y[j] * VV(j)[e];
         wv[e] = u;
       }
-      or_apply('N', 1.0, w, z, d, wv, 0.0, zv);
+      pc_apply(w, z, d, wv, zv);
       for (int32_t e = 0; e < n; e++) x[e] = x[e] + zv[e];
     }
     if (conv || brk) break;
@@ -1555,7 +1873,7 @@ int or_bicgstabl(const or_mat *a, const or_mat *w, const or_mat *z,
       rho0 = rho1;
       for (int32_t i = 0; i <= j; i++)
         for (int32_t e = 0; e < n; e++) UU(i)[e] = RR(i)[e] - beta * UU(i)[e];
-      or_apply('N', 1.0, w, z, d, UU(j), 0.0, tz);
+      pc_apply(w, z, d, UU(j), tz);
       or_spmv('N', 1.0, a, tz, 0.0, UU(j + 1));
       const double gam = dot(n, UU(j + 1), rt);
       if (gam == 0.0) { brk = 1; break; }
@@ -1563,7 +1881,7 @@ int or_bicgstabl(const or_mat *a, const or_mat *w, const or_mat *z,
       for (int32_t i = 0; i <= j; i++)
         for (int32_t e = 0; e < n; e++)
           RR(i)[e] = RR(i)[e] - alpha * UU(i + 1)[e];
-      or_apply('N', 1.0, w, z, d, RR(j), 0.0, tz);
+      pc_apply(w, z, d, RR(j), tz);
       or_spmv('N', 1.0, a, tz, 0.0, RR(j + 1));
       for (int32_t e = 0; e < n; e++) yh[e] = yh[e] + alpha * UU(0)[e];
     }
@@ -1609,7 +1927,7 @@ int or_bicgstabl(const or_mat *a, const or_mat *w, const or_mat *z,
     *err = (bn2 > 0.0) ? rn2 / bn2 : rn2;
   }
   /* x = x0 + M yhat */
-  or_apply('N', 1.0, w, z, d, yh, 0.0, tz);
+  pc_apply(w, z, d, yh, tz);
   for (int32_t e = 0; e < n; e++) x[e] = x[e] + tz[e];
 #undef RR
 #undef UU
@@ -1638,7 +1956,7 @@ int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
   if (*err <= eps) goto out;
   while (it < itmax) {
     it++;
-    or_apply('N', 1.0, w, z, d, r, 0.0, zz);
+    pc_apply(w, z, d, r, zz);
     rho_old = rho;
     rho = dot(n, r, zz);
     if (it == 1) {
@@ -1667,4 +1985,26 @@ out:
   free(p);
   free(q);
   return OR_OK;
+}
+
+int or_ml_krylov(const char *method, const or_mat *a, const or_ml *ml,
+                 const double *b, double *x, double eps, int32_t itmax,
+                 int32_t irst, int32_t *iter, double *err) {
+  if (!ml || !method) return OR_ERR_ARG;
+  int rc;
+  ml_prec = ml;
+  if (!strcmp(method, "BICGSTAB"))
+    rc = or_bicgstab(a, NULL, NULL, NULL, b, x, eps, itmax, iter, err);
+  else if (!strcmp(method, "CG"))
+    rc = or_cg(a, NULL, NULL, NULL, b, x, eps, itmax, iter, err);
+  else if (!strcmp(method, "CGS"))
+    rc = or_cgs(a, NULL, NULL, NULL, b, x, eps, itmax, iter, err);
+  else if (!strcmp(method, "RGMRES"))
+    rc = or_gmres(a, NULL, NULL, NULL, b, x, eps, itmax, irst, iter, err);
+  else if (!strcmp(method, "BICGSTABL"))
+    rc = or_bicgstabl(a, NULL, NULL, NULL, b, x, eps, itmax, irst, iter, err);
+  else
+    rc = OR_ERR_ARG; /* BiCG needs M^T, which the cycle does not offer */
+  ml_prec = NULL;
+  return rc;
 }

@@ -152,6 +152,47 @@ int or_cg(const or_mat *a, const or_mat *w, const or_mat *z, const double *d,
           const double *b, double *x, double eps, int32_t itmax,
           int32_t *iter, double *err);
 
+
+/* ---- multilevel (smoothed aggregation; MLD2P4's published algorithm, the
+ * framework itself being absent from the reference tree) ---------------- */
+/* rectangular transpose: a has a->n rows and ncols columns */
+int or_transpose_rect(const or_mat *a, int32_t ncols, or_mat *t);
+/* greedy three-phase aggregation; agg[n] receives the aggregate of each
+ * row, *nagg their count */
+int or_aggregate(const or_mat *a, double theta, int32_t *agg, int32_t *nagg);
+/* C = A B, B with bcols columns (Gustavson, stored order) */
+int or_spgemm(const or_mat *a, const or_mat *b, int32_t bcols, or_mat *c);
+/* one coarsening step: aggregates, prolongator P (n x nagg; smoothed or
+ * tentative), R = P^T, Ac = R (A P); omega of the smoothing step */
+int or_ml_level(const or_mat *a, double theta, int smoothed, int32_t *agg,
+                int32_t *nagg, or_mat *p, or_mat *r, or_mat *ac,
+                double *omega);
+#define OR_ML_MULT 0
+#define OR_ML_ADD 1
+#define OR_ML_PRE 1
+#define OR_ML_POST 2
+#define OR_ML_TWOSIDE 3
+typedef struct {
+  int32_t nlev;
+  const or_mat *a; /* [nlev] level matrices */
+  const or_mat *p; /* [nlev - 1] prolongators */
+  const or_mat *r; /* [nlev - 1] restrictions */
+  const or_mat *w; /* [nlev] smoother / coarse factors: stored W */
+  const or_mat *z; /* [nlev] Z */
+  const double *const *d; /* [nlev] pivots */
+  int32_t type;    /* OR_ML_MULT or OR_ML_ADD */
+  int32_t pos;     /* OR_ML_PRE, OR_ML_POST or OR_ML_TWOSIDE (MULT) */
+  int32_t sweeps;  /* smoother sweeps per visit */
+  int32_t csweeps; /* coarsest-level sweeps */
+} or_ml;
+/* y = V-cycle(x) */
+int or_ml_apply(const or_ml *ml, const double *x, double *y);
+/* Krylov method (BICGSTAB, CG, CGS, RGMRES, BICGSTABL) preconditioned by
+ * the cycle; irst as for or_gmres / or_bicgstabl */
+int or_ml_krylov(const char *method, const or_mat *a, const or_ml *ml,
+                 const double *b, double *x, double eps, int32_t itmax,
+                 int32_t irst, int32_t *iter, double *err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,6 +26,15 @@ class OrMat(C.Structure):
                 ("val", C.POINTER(C.c_double))]
 
 
+class OrMl(C.Structure):
+    _fields_ = [("nlev", C.c_int32), ("a", C.POINTER(OrMat)),
+                ("p", C.POINTER(OrMat)), ("r", C.POINTER(OrMat)),
+                ("w", C.POINTER(OrMat)), ("z", C.POINTER(OrMat)),
+                ("d", C.POINTER(C.POINTER(C.c_double))),
+                ("type", C.c_int32), ("pos", C.c_int32),
+                ("sweeps", C.c_int32), ("csweeps", C.c_int32)]
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
@@ -84,6 +93,21 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = C.c_int32 if name == "or_sparsify" else C.c_int
+        L.or_aggregate.argtypes = [P(OrMat), C.c_double, P(C.c_int32),
+                                   P(C.c_int32)]
+        L.or_spgemm.argtypes = [P(OrMat), P(OrMat), C.c_int32, P(OrMat)]
+        L.or_transpose_rect.argtypes = [P(OrMat), C.c_int32, P(OrMat)]
+        L.or_ml_level.argtypes = [P(OrMat), C.c_double, C.c_int,
+                                  P(C.c_int32), P(C.c_int32), P(OrMat),
+                                  P(OrMat), P(OrMat), P(C.c_double)]
+        L.or_ml_apply.argtypes = [P(OrMl), P(C.c_double), P(C.c_double)]
+        L.or_ml_krylov.argtypes = [C.c_char_p, P(OrMat), P(OrMl),
+                                   P(C.c_double), P(C.c_double), C.c_double,
+                                   C.c_int32, C.c_int32, P(C.c_int32),
+                                   P(C.c_double)]
+        for f in ("or_aggregate", "or_spgemm", "or_transpose_rect",
+                  "or_ml_level", "or_ml_apply", "or_ml_krylov"):
+            getattr(L, f).restype = C.c_int
         L.or_spmv.argtypes = [C.c_char, C.c_double, P(OrMat), P(C.c_double),
                               C.c_double, P(C.c_double)]
         L.or_spmv.restype = None
@@ -280,4 +304,155 @@ def krylov(method, a, w, z, d, b, eps, itmax=1000, x0=None, irst=10):
          "CGS": lib().or_cgs, "BICG": lib().or_bicg}[meth]
     _check(f(A.ref, W.ref, Z.ref, _dp(d), _dp(b), _dp(x), eps, itmax,
              C.byref(it), C.byref(err)), method)
+    return x, it.value, err.value
+
+
+# ---- multilevel (smoothed aggregation) -----------------------------------
+ML_TYPE = {"MULT": 0, "ADD": 1}
+ML_POS = {"PRE": 1, "POST": 2, "TWOSIDE": 3}
+
+
+def _out_rect(m, ncols):
+    n, nnz = m.n, m.nnz
+    ptr = np.ctypeslib.as_array(m.ptr, shape=(n + 1,)).copy()
+    idx = np.ctypeslib.as_array(m.idx, shape=(max(nnz, 1),))[:nnz].copy()
+    val = np.ctypeslib.as_array(m.val, shape=(max(nnz, 1),))[:nnz].copy()
+    lib().or_mat_free(C.byref(m))
+    return sp.csr_matrix((val, idx, ptr), shape=(n, ncols))
+
+
+class _InRect(_In):
+    """A rectangular CSR input (the OrMat carries only the row count)."""
+
+    def __init__(self, a):
+        a = sp.csr_matrix(a)
+        a.sort_indices()
+        self.ptr = np.ascontiguousarray(a.indptr, dtype=np.int64)
+        self.idx = np.ascontiguousarray(a.indices, dtype=np.int32)
+        self.val = np.ascontiguousarray(a.data, dtype=np.float64)
+        self.m = OrMat(a.shape[0], len(self.val),
+                       self.ptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                       _ip(self.idx), _dp(self.val))
+
+
+def aggregate(a, theta):
+    """Returns (agg, nagg): the aggregate of each row and their count."""
+    A = wrap(a)
+    agg = np.zeros(A.m.n, dtype=np.int32)
+    na = C.c_int32(0)
+    _check(lib().or_aggregate(A.ref, theta, _ip(agg), C.byref(na)),
+           "aggregate")
+    return agg, na.value
+
+
+def spgemm(a, b):
+    """A @ B in the oracle's fixed summation order (explicit zeros kept)."""
+    A, B = _InRect(a), _InRect(b)
+    c = OrMat()
+    _check(lib().or_spgemm(A.ref, B.ref, b.shape[1], C.byref(c)), "spgemm")
+    return _out_rect(c, b.shape[1])
+
+
+def ml_level(a, theta, smoothed=True):
+    """One coarsening step: (agg, P, R, Ac, omega)."""
+    A = wrap(a)
+    n = A.m.n
+    agg = np.zeros(n, dtype=np.int32)
+    na = C.c_int32(0)
+    p, r, ac = OrMat(), OrMat(), OrMat()
+    om = C.c_double(0.0)
+    _check(lib().or_ml_level(A.ref, theta, int(bool(smoothed)), _ip(agg),
+                             C.byref(na), C.byref(p), C.byref(r), C.byref(ac),
+                             C.byref(om)), "ml_level")
+    nc = na.value
+    return (agg, _out_rect(p, nc), _out_rect(r, n), _out_to_csr(ac),
+            om.value)
+
+
+def factor_bld(a, kind, fill_in=0, inv_fill=0, thresh=0.0, inv_thresh=0.0,
+               alg="LLK"):
+    """(W, Z, d) of one AINV-family factorization with the library's
+    parameter names and checks (AINV: fill >= 1)."""
+    kind = kind.upper()
+    if kind == "AINV":
+        return ainv_bld(a, alg, max(fill_in, 1), max(thresh, 0.0))
+    if kind == "INVK":
+        return invk_bld(a, max(fill_in, 0), max(inv_fill, 0))
+    if kind == "INVT":
+        return invt_bld(a, max(fill_in, 0), max(thresh, 0.0),
+                        max(inv_fill, 0), max(inv_thresh, 0.0))
+    raise ValueError(kind)
+
+
+class MlHierarchy:
+    """Levels of a smoothed-aggregation hierarchy with AINV-family smoothers
+    (sub) and coarsest-level factors (coarse).  Coarsening stops early when
+    a step no longer reduces the size."""
+
+    def __init__(self, a, nlev=3, theta=0.1, smoothed=True, sub=None,
+                 coarse=None, ml_type="MULT", pos="POST", sweeps=1,
+                 csweeps=1, scale=1.0):
+        """theta of level l + 1 = theta of level l * scale."""
+        sub = dict(sub or {"kind": "INVK"})
+        coarse = dict(coarse or sub)
+        self.A, self.P, self.R, self.agg, self.omega = [sp.csr_matrix(a)], [], [], [], []
+        th = float(theta)
+        while len(self.A) < nlev and self.A[-1].shape[0] > 1:
+            agg, P, R, Ac, om = ml_level(self.A[-1], th, smoothed)
+            th = th * scale
+            if Ac.shape[0] >= self.A[-1].shape[0] or Ac.shape[0] == 0:
+                break
+            self.agg.append(agg)
+            self.P.append(P)
+            self.R.append(R)
+            self.A.append(Ac)
+            self.omega.append(om)
+        L = len(self.A)
+        self.F = [factor_bld(self.A[l], **(sub if l < L - 1 else coarse))
+                  for l in range(L)]
+        self.type, self.pos = ML_TYPE[ml_type.upper()], ML_POS[pos.upper()]
+        self.sweeps, self.csweeps = sweeps, csweeps
+        self._pack()
+
+    def _pack(self):
+        L = len(self.A)
+        self._keep = [[_In(m) for m in self.A], [_InRect(m) for m in self.P],
+                      [_InRect(m) for m in self.R],
+                      [_In(f[0]) for f in self.F], [_In(f[1]) for f in self.F]]
+        self._d = [np.ascontiguousarray(f[2], dtype=np.float64) for f in self.F]
+
+        def arr(objs):
+            t = (OrMat * max(len(objs), 1))()
+            for i, o in enumerate(objs):
+                t[i] = o.m
+            return t
+        self._arrs = [arr(k) for k in self._keep]
+        dp = (C.POINTER(C.c_double) * L)(*[_dp(d) for d in self._d])
+        self._dp = dp
+        self.m = OrMl(L, self._arrs[0], self._arrs[1], self._arrs[2],
+                      self._arrs[3], self._arrs[4], dp, self.type, self.pos,
+                      self.sweeps, self.csweeps)
+
+    @property
+    def nlev(self):
+        return len(self.A)
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        _check(lib().or_ml_apply(C.byref(self.m), _dp(x), _dp(y)), "ml_apply")
+        return y
+
+
+def ml_krylov(method, a, ml, b, eps, itmax=1000, x0=None, irst=10):
+    """Krylov method preconditioned by the V-cycle of ml (MlHierarchy):
+    (x, iterations, last ||r||/||b||)."""
+    A = wrap(a)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int32(0)
+    err = C.c_double(0.0)
+    _check(lib().or_ml_krylov(method.upper().encode(), A.ref,
+                              C.byref(ml.m), _dp(b), _dp(x), eps, itmax, irst,
+                              C.byref(it), C.byref(err)), method)
     return x, it.value, err.value
