@@ -79,6 +79,22 @@ CONFIGS = {
 }
 
 
+# Multilevel leg (C1, C2): the shape of the reference's deck
+# tests/pdegen/runs/ppde.inp (ML, 3 levels, SMOOTHED DEC aggregation,
+# MULT / POST, one smoother sweep) with factors of the configuration's kind
+# as smoother and coarsest-level solver.  Threshold 0 instead of the deck's
+# 0.10: the smoothed Galerkin operator's relative couplings fall below 0.10,
+# which stops coarsening after one step, and a threshold halved per level
+# leaves a coarsest level five times larger for the same iteration count.
+# The coarsest level gets 2 sweeps (more do not reduce the iteration count).
+ML_PARAMS = dict(N_PREC_LEVS=3, AGGR_KIND="SMOOTHED", AGGR_THRESH=0.0,
+                 AGGR_SCALE=1.0, ML_TYPE="MULT", SMOOTHER_POS="POST",
+                 SMOOTHER_SWEEPS=1, COARSE_SWEEPS=2)
+# factors of every level: INVK with ILU(0) and inverse fill 1 (the Galerkin
+# levels are denser than A, and ILU(1) on them fills heavily)
+ML_FACTORS = dict(SUB_FILLIN=0, INV_FILLIN=1)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -111,6 +127,142 @@ def apply_pass_bytes(lay, k, first, xb=8, yb=8):
     return lay["nnz_z"] * (vb + 4) + n * 4 + k * n * st + k * n * yb
 
 
+def ml_cycle_bytes(levels, k, sweeps=1, csweeps=2, eb=8):
+    """Algorithmic bytes of one POST V-cycle over k columns: per product the
+    matrix (value + int32 column per entry, one row-extent word per row) and
+    each vector row read or written once (eb bytes per vector value: 8, or 4
+    for the FP32 cycle, whose level matrices are FP32; the smoother's
+    intermediate t in the factors' intermediate type)."""
+    tot = 0
+    mb = 4 if eb == 4 else 8  # A, P, R value bytes
+
+    def prod(nnz, rows, reads, writes):
+        return nnz * (mb + 4) + rows * 4 + (reads + writes) * rows * k * eb
+
+    def smoother(L, beta):
+        vb = L["value_bytes"]
+        st = 8 if vb == 8 else 4
+        n = L["n"]
+        w = L["nnz_w"] * (vb + 4) + n * 4 + n * st + n * k * eb + n * k * st
+        z = (L["nnz_z"] * (vb + 4) + n * 4 + n * k * st
+             + (2 if beta else 1) * n * k * eb)
+        return w + z
+
+    nl = len(levels)
+    for i, L in enumerate(levels):
+        n = L["n"]
+        resid = prod(L["nnz"], n, 2, 1)
+        if i == nl - 1:
+            tot += smoother(L, False) + (csweeps - 1) * (resid +
+                                                         smoother(L, True))
+            break
+        nc = levels[i + 1]["n"]
+        tot += L["nnz_r"] * (mb + 4) + nc * 4 + (n + nc) * k * eb  # R b
+        tot += L["nnz_p"] * (mb + 4) + n * 4 + (nc + n) * k * eb  # P u_c
+        tot += sweeps * (resid + smoother(L, True))
+    return tot
+
+
+def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
+           M, fp64_rate, it64):
+    """The multilevel preconditioner on the same problem and right-hand
+    sides: single-RHS FP64 solves, the served form with --prec level
+    factors, and one cycle's duration at the served block width."""
+    n = a.shape[0]
+    pv = m.MldMlPrec()
+    for key, v in (ML_FACTORS if kind == "INVK" else params).items():
+        pv.set(key, v)
+    pv.set("SUB_SOLVE", kind)
+    pv.set("COARSE_SUBSOLVE", kind)
+    for key, v in ML_PARAMS.items():
+        pv.set(key, v)
+    t0 = time.time()
+    pv.build(a)
+    t_build = time.time() - t0
+    levels = pv.report()["multilevel"]
+    log(f"multilevel build {t_build:.2f}s: "
+        f"{[(L['n'], L['nnz']) for L in levels['levels']]}")
+    dx1 = m.DeviceArray((n,))
+
+    def single(j):
+        db1 = m.DeviceArray.from_numpy(Bh[j])
+        dx1.zero()
+        m.synchronize()
+        ts = time.time()
+        _, it, err = m.krylov(method, A, pv, db1, dx1, eps=eps, itmax=5000)
+        m.synchronize()
+        dt = time.time() - ts
+        db1.free()
+        return dt, int(it[0]), float(err[0])
+
+    single(sample[0])
+    ts, its = [], []
+    for j in sample:
+        dt, it, _ = single(j)
+        ts.append(dt)
+        its.append(it)
+    x = dx1.to_numpy()
+    true1 = float(np.linalg.norm(Bh[sample[-1]] - a @ x) /
+                  np.linalg.norm(Bh[sample[-1]]))
+    rate1 = len(ts) / sum(ts)
+    log(f"multilevel FP64 single: {rate1:.2f} solves/s, iterations {its}")
+    # served form with --prec level factors (tuned, captured graphs)
+    pv.cnv(args.prec)
+    dX = m.DeviceArray((M, n))
+    warm = m.DeviceArray.from_numpy(Bh[:min(slots, M)])
+    m.krylov_stream(method, A, pv, warm, slots=slots, eps=eps, itmax=5000)
+    m.synchronize()
+    warm.free()
+    t_start = time.time()
+    _, iters, errs, lat = m.krylov_stream(method, A, pv, dB, dX, slots=slots,
+                                          eps=eps, itmax=5000)
+    m.synchronize()
+    el = time.time() - t_start
+    Xs = dX.to_numpy()
+    dX.free()
+    true = max(float(np.linalg.norm(Bh[j] - a @ Xs[j]) /
+                     np.linalg.norm(Bh[j])) for j in range(0, M,
+                                                           max(1, M // 16)))
+    rate = M / el
+    # one cycle at the served block width, on the vectors the served form
+    # gives it (FP32 for BiCGSTAB / CG with FP32 / BF16 factors) and with
+    # the mappings it tuned, timed with HIP events on the handle's stream
+    x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
+    ms, _ = profile_apply(m, pv, n, slots, 10, x32, x32)
+    lv = pv.report()["multilevel"]["levels"]
+    byt = ml_cycle_bytes(lv, slots, 1, ML_PARAMS["COARSE_SWEEPS"],
+                         4 if x32 else 8)
+    cyc = {"factor_prec": args.prec, "vectors": "FP32" if x32 else "FP64",
+           "ms": ms, "bytes": byt, "GBs": byt / (ms * 1e-3) / 1e9,
+           "roofline_frac": byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"multilevel served {args.prec}: {rate:.2f} solves/s, iterations "
+        f"mean {float(np.mean(iters)):.1f}; cycle {cyc}")
+    return {
+        "params": ML_PARAMS,
+        "factors": {"kind": kind,
+                    **(ML_FACTORS if kind == "INVK" else params)},
+        "build_s": t_build,
+        "levels": [{k: L[k] for k in ("n", "nnz", "aggregates", "nnz_w",
+                                      "nnz_z", "theta", "omega")}
+                   for L in levels["levels"]],
+        "build_phases_s": {k: levels[k] for k in
+                           ("t_aggregation", "t_galerkin", "t_factors",
+                            "t_upload")},
+        "fp64_single_rhs": {"solves_per_s": rate1, "iterations": its,
+                            "true_residual": true1,
+                            "vs_one_level_fp64_single": rate1 / fp64_rate,
+                            "one_level_iterations": it64},
+        "served": {"solves_per_s": rate, "factor_prec": args.prec,
+                   "slots": slots, "iterations_mean": float(np.mean(iters)),
+                   "iterations_max": int(np.max(iters)),
+                   "latency_ms_p50": float(np.percentile(lat, 50)),
+                   "latency_ms_p99": float(np.percentile(lat, 99)),
+                   "true_residual_max": true,
+                   "vs_one_level_fp64_single": rate / fp64_rate},
+        "cycle_k": slots, "cycle": cyc,
+    }
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -134,6 +286,8 @@ def main():
                         "in FP64 (the baseline sample)")
     p.add_argument("--prec", default="F32")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-ml", action="store_true",
+                   help="skip the multilevel leg (C1, C2)")
     p.add_argument("--cpu-iters", type=int, default=None,
                    help="oracle iterations per timed run (default: ~2 s)")
     p.add_argument("--apply-only", action="store_true",
@@ -336,6 +490,11 @@ def main():
         f"({passes[0]['achieved_GBs']:.0f} GB/s), Z {ms_z:.3f} ms "
         f"({passes[1]['achieved_GBs']:.0f} GB/s)")
 
+    ml = None
+    if not args.no_ml and args.config in ("C1", "C2"):
+        ml = ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps,
+                    slots, M, fp64_rate, it64)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sv, a, Bh[sample[0]], method, args.cpu_iters,
@@ -400,6 +559,7 @@ def main():
         },
         "build": {"seconds": t_build},
         "cpu_baseline": cpu,
+        "multilevel": ml,
     }
     print(json.dumps(out))
 

@@ -391,7 +391,9 @@ int ainv_device_sync(void);
  * ms_second = alpha Z t + beta y.  flags (FP32 / BF16 factors only):
  * AINV_PROFILE_X32, x holds FP32 values (the input the Krylov engine's
  * BiCGSTAB and CG give their preconditioner); AINV_PROFILE_Y32, y gets FP32
- * values (their preconditioned vectors). */
+ * values (their preconditioned vectors).  A multilevel handle times whole
+ * cycles instead (ms_first, ms_second = 0; k a power of two; X32 and Y32
+ * together for the FP32 cycle). */
 #define AINV_PROFILE_X32 1
 #define AINV_PROFILE_Y32 2
 int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,

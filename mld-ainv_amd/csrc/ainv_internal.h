@@ -155,10 +155,15 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
                     int64_t ldx, double beta, float *y, int64_t ldy, int k,
                     hipStream_t s, int map = AINV_MAP_SPLIT,
                     const int32_t *kw = nullptr);
-// y = alpha A x + beta z (FP64 A; x, z, y distinct with leading dimension
-// ld): the residual b - A u of the multilevel cycle in one pass
+// y = alpha A x + beta z (x, z, y distinct with leading dimension ld): the
+// residual b - A u of the multilevel cycle in one pass.  FP64 vectors with
+// an FP64 or FP32 A, or FP32 vectors with an FP32 A.
 void hll_spmm_resid(const Hll &a, double alpha, const double *x, double beta,
                     const double *z, double *y, int64_t ld, int k,
+                    hipStream_t s, int map = AINV_MAP_SPLIT,
+                    const int32_t *kw = nullptr);
+void hll_spmm_resid(const Hll &a, double alpha, const float *x, double beta,
+                    const float *z, float *y, int64_t ld, int k,
                     hipStream_t s, int map = AINV_MAP_SPLIT,
                     const int32_t *kw = nullptr);
 // Block partial sums of the Krylov reductions: part[(q*kRedBlocks + b) *

@@ -1161,20 +1161,45 @@ void hll_spmm_axpby(const Hll &a, double alpha, const void *x, bool x_is_f32,
   }
 }
 
-// y = alpha A x + beta z (FP64 matrix and vectors; z and y distinct, both
-// with the leading dimension ld of x)
+// y = alpha A x + beta z with FP64 vectors (A FP64 or FP32; z and y
+// distinct, all three with the leading dimension ld)
 void hll_spmm_resid(const Hll &a, double alpha, const double *x, double beta,
                     const double *z, double *y, int64_t ld, int k,
                     hipStream_t s, int map, const int32_t *kw) {
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
-  if (a.prec != AINV_PREC_F64)
-    throw Error(AINV_ERR_INTERNAL, "residual product needs an FP64 matrix");
+  const bool row = use_row(map, k, ld, ld, x, y) && aligned16(z);
+  if (a.prec == AINV_PREC_F64) {
+    if (row)
+      launch_lane<double, double, double, double, EPI_RESID>(
+          map, a, x, k, z, y, alpha, beta, s, kw);
+    else
+      launch_k<double, double, double, double, EPI_RESID>(a, x, ld, k, z, y,
+                                                          ld, alpha, beta, s);
+  } else if (a.prec == AINV_PREC_F32) {
+    if (row)
+      launch_lane<float, double, double, double, EPI_RESID>(
+          map, a, x, k, z, y, alpha, beta, s, kw);
+    else
+      launch_k<float, double, double, double, EPI_RESID>(a, x, ld, k, z, y,
+                                                         ld, alpha, beta, s);
+  } else {
+    throw Error(AINV_ERR_INTERNAL, "residual product of a BF16 matrix");
+  }
+}
+
+// the same with FP32 vectors (FP32 matrix, FP64 accumulation)
+void hll_spmm_resid(const Hll &a, double alpha, const float *x, double beta,
+                    const float *z, float *y, int64_t ld, int k,
+                    hipStream_t s, int map, const int32_t *kw) {
+  if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
+  if (a.prec != AINV_PREC_F32)
+    throw Error(AINV_ERR_INTERNAL, "FP32 residual product needs an FP32 matrix");
   if (use_row(map, k, ld, ld, x, y) && aligned16(z))
-    launch_lane<double, double, double, double, EPI_RESID>(map, a, x, k, z, y,
-                                                           alpha, beta, s, kw);
+    launch_lane<float, float, float, float, EPI_RESID>(map, a, x, k, z, y,
+                                                       alpha, beta, s, kw);
   else
-    launch_k<double, double, double, double, EPI_RESID>(a, x, ld, k, z, y, ld,
-                                                        alpha, beta, s);
+    launch_k<float, float, float, float, EPI_RESID>(a, x, ld, k, z, y, ld,
+                                                    alpha, beta, s);
 }
 
 // FP32 output y = alpha Z t + beta y from an FP32 t (FP32 / BF16 factors):

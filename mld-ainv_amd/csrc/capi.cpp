@@ -1466,7 +1466,6 @@ int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
     std::lock_guard<std::mutex> lk(h->mu);
     set_device(h->device);
     if (!h->built) throw Error(AINV_ERR_STATE, "profile before build");
-    not_ml(h, "ainv_apply_profile");
     if (reps < 1 || k < 1 || k > 64 || !x || !y || !ms_first || !ms_second)
       throw Error(AINV_ERR_ARG, "bad profile arguments");
     const bool f32t = h->prec != AINV_PREC_F64;
@@ -1475,6 +1474,34 @@ int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
     if ((x32 || y32) && !f32t)
       throw Error(AINV_ERR_ARG, "FP32 x or y needs FP32 or BF16 factors");
     hipStream_t s = h->stream;
+    if (h->kind == AINV_KIND_ML) {
+      // whole cycles on FP32 (x32 and y32) or FP64 vectors, ld = k
+      if (x32 != y32)
+        throw Error(AINV_ERR_ARG, "the cycle takes FP32 x and y together");
+      if (k & (k - 1)) throw Error(AINV_ERR_ARG, "k must be a power of two");
+      const size_t need = ml_ws_bytes(h, k);
+      if (h->tbuf.bytes < need) {
+        h->tbuf.alloc(need);
+        AINV_HIP_CHECK(hipMemsetAsync(h->tbuf.p, 0, need, s));
+      }
+      auto cycle = [&] {
+        if (x32)
+          ml_apply_into32(h, static_cast<const float *>(x),
+                          reinterpret_cast<float *>(y), k, k, h->tbuf.p, s);
+        else
+          ml_apply_into(h, static_cast<const double *>(x), y, k, k,
+                        h->tbuf.p, s);
+      };
+      Events e(2);
+      cycle();
+      AINV_HIP_CHECK(hipEventRecord(e.ev[0], s));
+      for (int r = 0; r < reps; ++r) cycle();
+      AINV_HIP_CHECK(hipEventRecord(e.ev[1], s));
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+      *ms_first = e.ms(0, 1) / reps;
+      *ms_second = 0.0;
+      return;
+    }
     // the second pass, into y as FP64 or FP32
     auto second = [&](int mz) {
       if (y32)

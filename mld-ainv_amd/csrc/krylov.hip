@@ -526,7 +526,7 @@ int64_t x32_stride(const KrylovPlan *p) {
 
 void alloc_x32(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
   if ((p->method != KM_BICGSTAB && p->method != KM_CG) || !m ||
-      m->prec == AINV_PREC_F64 || m->kind == AINV_KIND_ML || p->work32.p)
+      m->prec == AINV_PREC_F64 || p->work32.p)
     return;
   const int nv = p->method == KM_BICGSTAB ? 4 : 2;
   p->work32.alloc((size_t)nv * x32_stride(p) * sizeof(float));
@@ -548,8 +548,7 @@ void alloc_tbuf(KrylovPlan *p, const ainv_solver *m, hipStream_t s) {
 // preconditioner has FP32 / BF16 factors (none otherwise).
 Vecs x32_vecs(const KrylovPlan *p) {
   Vecs w = p->v;
-  if (p->m && p->m->prec != AINV_PREC_F64 && p->m->kind != AINV_KIND_ML &&
-      p->work32.p) {
+  if (p->m && p->m->prec != AINV_PREC_F64 && p->work32.p) {
     float *f = p->work32.as<float>();
     if (p->method == KM_BICGSTAB) {
       const int64_t nk = x32_stride(p);
@@ -646,6 +645,10 @@ void precond(const KrylovPlan *p, const double *x, double *y, hipStream_t s) {
 // y = M x, FP32 x and FP32 y (FP32 / BF16 factors)
 void precond32(const KrylovPlan *p, const float *x, float *y, hipStream_t s) {
   const ainv_solver *m = p->m;
+  if (m->kind == AINV_KIND_ML) {
+    ml_apply_into32(m, x, y, p->k, p->K, p->tbuf.p, s, p->kwu);
+    return;
+  }
   hll_spmm_scale(m->hw, m->dv.p, x, true, p->K, p->tbuf.p, p->K, p->k, s,
                  map_w(p->apply_map), p->kwu);
   hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, true, p->K, 0.0, y, p->K, p->k, s,
@@ -714,7 +717,14 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   }
   const Vecs &v = p->v;
   if (m && m->kind == AINV_KIND_ML) {
-    ml_tune(m, K, p->k, v.b, v.z, p->tbuf.p, s);
+    // tuned on the vectors the cycle will run on
+    const Vecs w = x32_vecs(p);
+    if (w.p32)
+      ml_tune(m, K, p->k, w.p32, w.f32, true, p->tbuf.p, s);
+    else if (w.r32)
+      ml_tune(m, K, p->k, w.r32, w.z32, true, p->tbuf.p, s);
+    else
+      ml_tune(m, K, p->k, v.b, v.z, false, p->tbuf.p, s);
   } else if (m) {
     if (m->apply_map[li] < 0) {
       // the two passes are timed separately; t keeps leading dimension K

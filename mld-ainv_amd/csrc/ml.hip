@@ -47,6 +47,7 @@ struct MlLevel {
 
 struct MlHier {
   std::vector<std::unique_ptr<MlLevel>> lv;
+  int mprec = AINV_PREC_F64;  // storage of the level A, P, R: F64 or F32
   double t_aggr = 0, t_galerkin = 0, t_factors = 0, t_upload = 0;
 };
 
@@ -262,38 +263,58 @@ void prolongator(const HostCsr &a, const std::vector<int32_t> &agg,
   });
 }
 
-void upload_hll(const HostCsr &m, int32_t ncols, Hll &h, hipStream_t s) {
+void upload_hll(const HostCsr &m, int32_t ncols, int prec, Hll &h,
+                hipStream_t s) {
   DevCsr d;
   csr_upload(m, d, s);
-  hll_from_csr(d, AINV_PREC_F64, h, s, ncols);
+  hll_from_csr(d, prec, h, s, ncols);
+}
+
+// A, P and R of every level in FP64 (FP64 factors) or FP32 (FP32 / BF16
+// factors: the cycle may then run on FP32 vectors)
+void upload_levels(MlHier &H, int factor_prec, hipStream_t s) {
+  const int mp = factor_prec == AINV_PREC_F64 ? AINV_PREC_F64 : AINV_PREC_F32;
+  for (auto &Lp : H.lv) {
+    MlLevel &L = *Lp;
+    upload_hll(L.a, L.n, mp, L.ha, s);
+    if (L.nc > 0) {
+      upload_hll(L.p, L.nc, mp, L.hp, s);
+      upload_hll(L.r, L.n, mp, L.hr, s);
+    }
+    for (int i = 0; i < 8; ++i) L.map_a[i] = L.map_p[i] = L.map_r[i] = -1;
+  }
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  H.mprec = mp;
 }
 
 int64_t hll_bytes(const Hll &h) { return h.bytes(); }
 
 // ---- the cycle --------------------------------------------------------------
 
-int64_t vec_bytes(int32_t n, int64_t K) {
-  return ((int64_t)n * K * (int64_t)sizeof(double) + 255) / 256 * 256;
+int64_t vec_bytes(int32_t n, int64_t K, int64_t elem = sizeof(double)) {
+  return ((int64_t)n * K * elem + 255) / 256 * 256;
 }
 
-struct Ws {
-  double *b[kMaxLevels] = {}, *u[kMaxLevels] = {}, *t[kMaxLevels] = {};
+// the cycle's vectors of each level (V: double or float) and the smoother's
+// intermediate, carved from one workspace (FP64 sizes cover FP32 ones)
+template <typename V> struct Ws {
+  V *b[kMaxLevels] = {}, *u[kMaxLevels] = {}, *t[kMaxLevels] = {};
   void *tb = nullptr;
 };
 
-Ws carve(const MlHier &H, int64_t K, void *base) {
-  Ws w;
+template <typename V> Ws<V> carve(const MlHier &H, int64_t K, void *base) {
+  Ws<V> w;
   char *p = static_cast<char *>(base);
   for (size_t l = 0; l < H.lv.size(); ++l) {
     const int32_t n = H.lv[l]->n;
     if (l > 0) {
-      w.b[l] = reinterpret_cast<double *>(p);
-      p += vec_bytes(n, K);
-      w.u[l] = reinterpret_cast<double *>(p);
-      p += vec_bytes(n, K);
+      w.b[l] = reinterpret_cast<V *>(p);
+      p += vec_bytes(n, K, sizeof(V));
+      w.u[l] = reinterpret_cast<V *>(p);
+      p += vec_bytes(n, K, sizeof(V));
     }
-    w.t[l] = reinterpret_cast<double *>(p);
-    p += vec_bytes(n, K);
+    w.t[l] = reinterpret_cast<V *>(p);
+    p += vec_bytes(n, K, sizeof(V));
   }
   w.tb = p;
   return w;
@@ -305,65 +326,71 @@ int pick(const ainv_solver *h, const int8_t *tuned, int K) {
   return m < 0 ? AINV_MAP_SPLIT : m;
 }
 
-struct Cycle {
+// One cycle on vectors of type V (double, or float with FP32 / BF16 factors
+// and FP32 level matrices).
+template <typename V> struct Cycle {
+  static constexpr bool kF32 = sizeof(V) == sizeof(float);
   const ainv_solver *h;
   const MlHier *H;
   int k;
   int64_t K;
   hipStream_t s;
   const int32_t *kw;
-  Ws w;
+  Ws<V> w;
 
   // y = M_l x + beta y
-  void smooth(int l, const double *x, double *y, double beta) const {
+  void smooth(int l, const V *x, V *y, double beta) const {
     const ainv_solver *sv = H->lv[l]->sv.get();
     const int mp = tuned_apply_map(sv, (int)K);
     const bool f32t = sv->prec != AINV_PREC_F64;
-    hll_spmm_scale(sv->hw, sv->dv.p, x, false, K, w.tb, K, k, s, map_w(mp),
+    hll_spmm_scale(sv->hw, sv->dv.p, x, kF32, K, w.tb, K, k, s, map_w(mp),
                    kw);
     hll_spmm_axpby(sv->hz, 1.0, w.tb, f32t, K, beta, y, K, k, s, map_z(mp),
                    kw);
   }
+  // y = M x + beta y for M = R_l or P_l
+  void prod(const Hll &m, const int8_t *maps, const V *x, V *y,
+            double beta) const {
+    hll_spmm_axpby(m, 1.0, x, kF32, K, beta, y, K, k, s, pick(h, maps, K), kw);
+  }
   // t = b - A_l u
-  void resid(int l, const double *b, const double *u, double *t) const {
+  void resid(int l, const V *b, const V *u, V *t) const {
     const MlLevel &L = *H->lv[l];
     hll_spmm_resid(L.ha, -1.0, u, 1.0, b, t, K, k, s, pick(h, L.map_a, K), kw);
   }
   // u = M_l b, then nsw - 1 sweeps u += M_l (b - A_l u)
-  void smooth0(int l, int nsw, const double *b, double *u) const {
+  void smooth0(int l, int nsw, const V *b, V *u) const {
     smooth(l, b, u, 0.0);
     for (int i = 1; i < nsw; ++i) {
       resid(l, b, u, w.t[l]);
       smooth(l, w.t[l], u, 1.0);
     }
   }
-  void run(int l, const double *b, double *u) const {
+  void run(int l, const V *b, V *u) const {
     const int nl = (int)H->lv.size();
     if (l == nl - 1) {
       smooth0(l, h->ml_csweeps, b, u);
       return;
     }
     const MlLevel &L = *H->lv[l];
-    const int mr = pick(h, L.map_r, K), mp = pick(h, L.map_p, K);
-    double *bc = w.b[l + 1], *uc = w.u[l + 1];
+    V *bc = w.b[l + 1], *uc = w.u[l + 1];
     if (h->ml_type == AINV_ML_ADD) {
       smooth0(l, h->ml_sweeps, b, u);
-      hll_spmm_axpby(L.hr, 1.0, b, false, K, 0.0, bc, K, k, s, mr, kw);
+      prod(L.hr, L.map_r, b, bc, 0.0);
       run(l + 1, bc, uc);
-      hll_spmm_axpby(L.hp, 1.0, uc, false, K, 1.0, u, K, k, s, mp, kw);
+      prod(L.hp, L.map_p, uc, u, 1.0);
       return;
     }
     const bool pre = h->ml_pos & AINV_ML_PRE, post = h->ml_pos & AINV_ML_POST;
     if (pre) {
       smooth0(l, h->ml_sweeps, b, u);
       resid(l, b, u, w.t[l]);
-      hll_spmm_axpby(L.hr, 1.0, w.t[l], false, K, 0.0, bc, K, k, s, mr, kw);
+      prod(L.hr, L.map_r, w.t[l], bc, 0.0);
     } else {
-      hll_spmm_axpby(L.hr, 1.0, b, false, K, 0.0, bc, K, k, s, mr, kw);
+      prod(L.hr, L.map_r, b, bc, 0.0);
     }
     run(l + 1, bc, uc);
-    hll_spmm_axpby(L.hp, 1.0, uc, false, K, pre ? 1.0 : 0.0, u, K, k, s, mp,
-                   kw);
+    prod(L.hp, L.map_p, uc, u, pre ? 1.0 : 0.0);
     if (post)
       for (int i = 0; i < h->ml_sweeps; ++i) {
         resid(l, b, u, w.t[l]);
@@ -496,15 +523,7 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
   }
   H->t_factors = now_seconds() - t0;
   t0 = now_seconds();
-  for (int l = 0; l < nl; ++l) {
-    MlLevel &L = *H->lv[l];
-    upload_hll(L.a, L.n, L.ha, h->stream);
-    if (L.nc > 0) {
-      upload_hll(L.p, L.nc, L.hp, h->stream);
-      upload_hll(L.r, L.n, L.hr, h->stream);
-    }
-  }
-  AINV_HIP_CHECK(hipStreamSynchronize(h->stream));
+  upload_levels(*H, h->prec, h->stream);
   H->t_upload = now_seconds() - t0;
   h->n = a.n;
   h->ml = std::move(H);
@@ -525,7 +544,16 @@ size_t ml_ws_bytes(const ainv_solver *h, int k) {
 void ml_apply_into(const ainv_solver *h, const double *x, double *y, int k,
                    int64_t K, void *ws, hipStream_t s, const int32_t *kw) {
   const MlHier &H = hier(h);
-  Cycle c{h, &H, k, K, s, kw, carve(H, K, ws)};
+  Cycle<double> c{h, &H, k, K, s, kw, carve<double>(H, K, ws)};
+  c.run(0, x, y);
+}
+
+void ml_apply_into32(const ainv_solver *h, const float *x, float *y, int k,
+                     int64_t K, void *ws, hipStream_t s, const int32_t *kw) {
+  const MlHier &H = hier(h);
+  if (H.mprec != AINV_PREC_F32)
+    throw Error(AINV_ERR_INTERNAL, "FP32 cycle needs FP32 level matrices");
+  Cycle<float> c{h, &H, k, K, s, kw, carve<float>(H, K, ws)};
   c.run(0, x, y);
 }
 
@@ -572,19 +600,20 @@ void ml_apply_abi(ainv_solver *h, double alpha, const double *x, double beta,
   }
 }
 
-void ml_tune(ainv_solver *h, int K, int k, const double *x0, double *y0,
-             void *ws, hipStream_t s) {
-  if (K == 1 || k < 2 || h->map_mode >= 0) return;
+template <typename V>
+void tune_levels(ainv_solver *h, int K, int k, const V *x0, V *y0, void *ws,
+                 hipStream_t s) {
+  constexpr bool f32 = sizeof(V) == sizeof(float);
   MlHier &H = *h->ml;
   const int li = log2_pow2(K);
   AINV_HIP_CHECK(hipMemsetAsync(ws, 0, ml_ws_bytes(h, K), s));
-  const Ws w = carve(H, K, ws);
+  const Ws<V> w = carve<V>(H, K, ws);
   const int nl = (int)H.lv.size();
   for (int l = 0; l < nl; ++l) {
     MlLevel &L = *H.lv[l];
     // x is only read, y and the workspace only written
-    const double *x = l > 0 ? w.b[l] : x0;
-    double *y = l > 0 ? w.u[l] : y0;
+    const V *x = l > 0 ? w.b[l] : x0;
+    V *y = l > 0 ? w.u[l] : y0;
     if (L.map_a[li] < 0)
       L.map_a[li] = (int8_t)best_map(s, [&](int map) {
         hll_spmm_resid(L.ha, -1.0, y, 1.0, x, w.t[l], K, k, s, map);
@@ -592,20 +621,18 @@ void ml_tune(ainv_solver *h, int K, int k, const double *x0, double *y0,
     if (L.nc > 0) {
       if (L.map_r[li] < 0)
         L.map_r[li] = (int8_t)best_map(s, [&](int map) {
-          hll_spmm_axpby(L.hr, 1.0, x, false, K, 0.0, w.b[l + 1], K, k, s,
-                         map);
+          hll_spmm_axpby(L.hr, 1.0, x, f32, K, 0.0, w.b[l + 1], K, k, s, map);
         });
       if (L.map_p[li] < 0)
         L.map_p[li] = (int8_t)best_map(s, [&](int map) {
-          hll_spmm_axpby(L.hp, 1.0, w.u[l + 1], false, K, 0.0, y, K, k, s,
-                         map);
+          hll_spmm_axpby(L.hp, 1.0, w.u[l + 1], f32, K, 0.0, y, K, k, s, map);
         });
     }
     ainv_solver *sv = L.sv.get();
     if (sv->apply_map[li] < 0) {
       const bool f32t = sv->prec != AINV_PREC_F64;
       const int bw = best_map(s, [&](int map) {
-        hll_spmm_scale(sv->hw, sv->dv.p, x, false, K, w.tb, K, k, s, map);
+        hll_spmm_scale(sv->hw, sv->dv.p, x, f32, K, w.tb, K, k, s, map);
       });
       const int bz = best_map(s, [&](int map) {
         hll_spmm_axpby(sv->hz, 1.0, w.tb, f32t, K, 0.0, y, K, k, s, map);
@@ -614,6 +641,17 @@ void ml_tune(ainv_solver *h, int K, int k, const double *x0, double *y0,
     }
   }
   AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void ml_tune(ainv_solver *h, int K, int k, const void *x0, void *y0,
+             bool x32, void *ws, hipStream_t s) {
+  if (K == 1 || k < 2 || h->map_mode >= 0) return;
+  if (x32)
+    tune_levels<float>(h, K, k, static_cast<const float *>(x0),
+                       static_cast<float *>(y0), ws, s);
+  else
+    tune_levels<double>(h, K, k, static_cast<const double *>(x0),
+                        static_cast<double *>(y0), ws, s);
 }
 
 // ---- queries -----------------------------------------------------------------
@@ -629,6 +667,8 @@ const ainv_solver *ml_level_solver(const ainv_solver *h, int l) {
 
 void ml_remold(ainv_solver *h, int prec) {
   for (auto &L : h->ml->lv) solver_remold(L->sv.get(), prec);
+  const int mp = prec == AINV_PREC_F64 ? AINV_PREC_F64 : AINV_PREC_F32;
+  if (mp != h->ml->mprec) upload_levels(*h->ml, prec, h->stream);
 }
 
 int64_t ml_device_bytes(const ainv_solver *h) {
@@ -660,12 +700,15 @@ std::string ml_report(const ainv_solver *h) {
     const MlLevel &L = *H.lv[l];
     std::snprintf(buf, sizeof buf,
                   "%s{\"n\": %d, \"nnz\": %lld, \"aggregates\": %d, "
+                  "\"nnz_p\": %lld, \"nnz_r\": %lld, "
                   "\"theta\": %.6g, \"omega\": %.17g, \"kind\": %d, "
-                  "\"nnz_w\": %lld, \"nnz_z\": %lld, \"t_factor\": %.4f, "
-                  "\"t_inverse\": %.4f}",
-                  l ? ", " : "", L.n, (long long)L.a.nnz(), L.nc, L.theta,
+                  "\"nnz_w\": %lld, \"nnz_z\": %lld, \"value_bytes\": %d, "
+                  "\"t_factor\": %.4f, \"t_inverse\": %.4f}",
+                  l ? ", " : "", L.n, (long long)L.a.nnz(), L.nc,
+                  (long long)L.p.nnz(), (long long)L.r.nnz(), L.theta,
                   L.omega, L.sv->kind, (long long)L.sv->w.nnz,
-                  (long long)L.sv->z.nnz, L.sv->t_factor, L.sv->t_inverse);
+                  (long long)L.sv->z.nnz, value_bytes(L.sv->prec),
+                  L.sv->t_factor, L.sv->t_inverse);
     s += buf;
   }
   std::snprintf(buf, sizeof buf,

@@ -145,11 +145,17 @@ void ml_apply_into(const ainv_solver *h, const double *x, double *y, int k,
 void ml_apply_abi(ainv_solver *h, double alpha, const double *x, double beta,
                   double *y, char trans, int k, int64_t ldx, int64_t ldy,
                   hipStream_t s);
+// The same cycle on FP32 vectors (FP32 / BF16 factors: the level matrices
+// are then FP32 too).
+void ml_apply_into32(const ainv_solver *h, const float *x, float *y, int k,
+                     int64_t K, void *ws, hipStream_t s,
+                     const int32_t *kw = nullptr);
 // Times the kernel mappings of every level's products for block width K and
 // keeps the fastest; x0 (read) and y0 (overwritten) are n x K vectors of the
-// finest level, ws the cycle's workspace (cleared first).  Before a capture.
-void ml_tune(ainv_solver *h, int K, int k, const double *x0, double *y0,
-             void *ws, hipStream_t s);
+// finest level, FP32 if x32 (the cycle the caller will run), ws the cycle's
+// workspace (cleared first).  Before a capture.
+void ml_tune(ainv_solver *h, int K, int k, const void *x0, void *y0,
+             bool x32, void *ws, hipStream_t s);
 // One coarsening step on the host alone: aggregates, prolongator P and
 // Galerkin matrix R (A P), on `threads` threads (0: the default count).
 void ml_host_level(const HostCsr &a, double theta, bool smoothed, int threads,

@@ -257,3 +257,32 @@ def test_hierarchy_without_reduction_is_one_level(gpu, O):
     x = gpu.gen_normal(2, (a.shape[0],))
     yr = ref.apply(x)
     assert np.linalg.norm(pv.apply(x) - yr) <= 1e-12 * np.linalg.norm(yr)
+
+
+@pytest.mark.parametrize("prec,tol", [("F32", 2e-6), ("BF16", 1e-5)])
+def test_fp32_cycle(gpu, O, prec, tol):
+    """The cycle on FP32 vectors (what BiCGSTAB and CG give FP32 / BF16
+    factors), left in y by the profiling entry point, against the FP64
+    cycle on the same factors and FP32 level matrices."""
+    import ctypes as C
+    a = gpu.gen_pde3d(12, 1)
+    n, k = a.shape[0], 8
+    pv, _ = build_ml(gpu, O, a, "INVK")
+    pv.cnv(prec)
+    X = gpu.gen_normal(4, (n, k)).astype(np.float32)
+    y64 = pv.apply(X.astype(np.float64))
+    dx = gpu.DeviceArray.from_numpy(X)
+    dy = gpu.DeviceArray((n, k), dtype=np.float32)
+    ms1, ms2 = C.c_double(), C.c_double()
+    gpu._ck(gpu.lib().ainv_apply_profile(pv._h, dx.ptr, dy.ptr, k, 2, 3,
+                                         C.byref(ms1), C.byref(ms2)))
+    y32 = dy.to_numpy().astype(np.float64)
+    assert ms1.value > 0 and ms2.value == 0
+    assert np.linalg.norm(y32 - y64) <= tol * np.linalg.norm(y64)
+    # and the served BiCGSTAB on that cycle converges like the FP64 one
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(12, (n, 4))
+    _, it32, err32 = gpu.krylov("BICGSTAB", A, pv, B, eps=1e-8)
+    pv.cnv("F64")
+    _, it64, _ = gpu.krylov("BICGSTAB", A, pv, B, eps=1e-8)
+    assert np.all(err32 <= 1e-8) and np.all(it32 <= it64 + 2), (it32, it64)

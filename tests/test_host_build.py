@@ -45,3 +45,28 @@ def test_host_ainv_random_matrix_threads(mld, O, threads):
     assert same(w1, wt) and same(z1, zt) and np.array_equal(d1, dt)
     wr, zr, dr = O.ainv_bld(a, "LLK", 8, 5e-2)
     assert same(w1, wr) and same(z1, zr) and np.array_equal(d1, dr)
+
+
+def same_bits(x, y):
+    x, y = sp.csr_matrix(x), sp.csr_matrix(y)
+    return (x.shape == y.shape and np.array_equal(x.indptr, y.indptr)
+            and np.array_equal(x.indices, y.indices)
+            and np.array_equal(x.data.view(np.int64), y.data.view(np.int64)))
+
+
+@pytest.mark.parametrize("mat", ["pde3d", "aniso2d", "random"])
+@pytest.mark.parametrize("theta,smoothed", [(0.0, True), (0.1, True),
+                                            (0.05, False)])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_host_ml_level_matches_oracle(mld, O, mat, theta, smoothed, threads):
+    """One coarsening step of the multilevel build through the C ABI
+    (ainv_ml_host_level, no GPU): aggregates, prolongator, Galerkin matrix
+    and omega bit-identical to the oracle for any host thread count."""
+    a = {"pde3d": lambda: mld.gen_pde3d(20, 1),
+         "aniso2d": lambda: mld.gen_aniso2d(64, 1e-3),
+         "random": lambda: mld.gen_random(20000, seed=3, band=3000)}[mat]()
+    agg, P, Ac, om = mld.ml_host_level(a, theta, smoothed, threads)
+    ragg, rP, rR, rAc, rom = O.ml_level(a, theta, smoothed)
+    assert np.array_equal(agg, ragg)
+    assert same_bits(P, rP) and same_bits(Ac, rAc)
+    assert om == rom
