@@ -136,7 +136,14 @@ constexpr int AINV_MAP_SPLIT = 0;
 constexpr int AINV_MAP_ROW = 1;
 constexpr int AINV_MAP_COOP = 2;
 constexpr int AINV_MAP_COOP128 = 3;
-constexpr int kNumMaps = 4;
+// the cooperative mappings with four waves sharing each slice (more waves
+// for matrices of few slices; the fused system-matrix product runs them as
+// COOP / COOP128)
+constexpr int AINV_MAP_COOP_Q = 4;
+constexpr int AINV_MAP_COOP128_Q = 5;
+constexpr int kNumMaps = 6;
+// mappings the fused system-matrix product distinguishes
+constexpr int kNumSpmvMaps = 4;
 // The two apply passes may use different mappings: a packed pair.
 inline int map_pair(int w, int z) { return w | (z << 4); }
 inline int map_w(int pair) { return pair & 15; }

@@ -608,15 +608,17 @@ template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
 // takes part in the shuffles and loads row 0 of its columns instead of the
 // gathered rows (a select, not a branch, so the loads keep their schedule;
 // all such lanes hit the same lines), and its sums are not stored.
-template <typename VT, typename XT, int K, int KC>
+// NQ of the L row passes, from pass q0 on (a wave may share its slice with
+// other waves, each taking some of the passes).
+template <typename VT, typename XT, int K, int KC, int NQ = Coop<XT, KC>::L>
 __device__ __forceinline__ void coop_acc(
-    int64_t p0, int32_t width, int lane, int rg,
+    int64_t p0, int32_t width, int lane, int rg, int q0,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ xs, bool live,
-    typename Val<VT>::acc (&acc)[Coop<XT, KC>::L][Coop<XT, KC>::CPL]) {
+    typename Val<VT>::acc (&acc)[NQ][Coop<XT, KC>::CPL]) {
   using A = typename Val<VT>::acc;
   using CP = Coop<XT, KC>;
-  constexpr int L = CP::L, CPL = CP::CPL, R = CP::R;
+  constexpr int L = NQ, CPL = CP::CPL, R = CP::R;
   constexpr int P = RowGroup<A, KC>::v;
   const int32_t nfull = width / P * P;
   const int64_t pl = p0 + lane;
@@ -637,7 +639,7 @@ __device__ __forceinline__ void coop_acc(
       for (int u = 0; u < P; ++u)
 #pragma unroll
         for (int q = 0; q < L; ++q) {
-          const int32_t c = __shfl(ci[u], q * R + rg);
+          const int32_t c = __shfl(ci[u], (q0 + q) * R + rg);
           load_xrow<XT, CPL>(live ? xs + (int64_t)c * K : xs, xv[u][q]);
         }
       const int32_t jn = j + P;
@@ -654,7 +656,7 @@ __device__ __forceinline__ void coop_acc(
       for (int u = 0; u < P; ++u)
 #pragma unroll
         for (int q = 0; q < L; ++q) {
-          const A v = Val<VT>::get(shfl_raw(vi[u], q * R + rg));
+          const A v = Val<VT>::get(shfl_raw(vi[u], (q0 + q) * R + rg));
 #pragma unroll
           for (int c = 0; c < CPL; ++c) acc[q][c] += v * xv[u][q][c];
         }
@@ -672,8 +674,8 @@ __device__ __forceinline__ void coop_acc(
     const VT v0 = val[pl + (int64_t)j * kSlice];
 #pragma unroll
     for (int q = 0; q < L; ++q) {
-      const int32_t c = __shfl(c0, q * R + rg);
-      const A v = Val<VT>::get(shfl_raw(v0, q * R + rg));
+      const int32_t c = __shfl(c0, (q0 + q) * R + rg);
+      const A v = Val<VT>::get(shfl_raw(v0, (q0 + q) * R + rg));
       A xv[CPL];
       load_xrow<XT, CPL>(live ? xs + (int64_t)c * K : xs, xv);
 #pragma unroll
@@ -682,9 +684,20 @@ __device__ __forceinline__ void coop_acc(
   }
 }
 
+// Waves of a block that share one slice in the quarter-slice variant
+// (each takes L / QW of the slice's row passes): more waves for matrices of
+// few slices, whose waves are otherwise bound by the latency of walking
+// long rows.  QW = 1: each wave has a slice of its own.
+template <typename XT, int K, int CB, int QS> struct CoopSplit {
+  static constexpr int L = Coop<XT, CoopChunk<XT, K, CB>::KC>::L;
+  static constexpr int QW = QS < L ? QS : L;
+  static constexpr int NQ = L / QW;
+  static constexpr int SPB = kWavesPerBlock / QW;  // slices per block
+};
+
 template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
-          int EPI>
-__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
+          int EPI, int QS>
+__device__ __forceinline__ void coop_spmm_body(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
@@ -694,32 +707,35 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
   using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC;
   using CP = Coop<XT, KC>;
+  using SP = CoopSplit<XT, K, CB, QS>;
+  constexpr int NQ = SP::NQ;
   const int lane = threadIdx.x & (kSlice - 1);
   const int sub = lane % CP::L, rg = lane / CP::L;
   // grid = NCH x slice groups; each XCD runs one chunk of a slice range
-  const int32_t ngrp = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int32_t ngrp = (nslices + SP::SPB - 1) / SP::SPB;
   const int32_t lb = __builtin_amdgcn_readfirstlane(
       xcd_block(blockIdx.x, gridDim.x));
   const int chunk = lb / ngrp;
   // active width: columns at or beyond it are neither read nor written
   const int kw = kwp ? min(k, *kwp) : k;
   if (chunk * KC >= kw) return;
-  const int32_t s = lb % ngrp * kWavesPerBlock +
-                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int32_t s = lb % ngrp * SP::SPB + wv / SP::QW;
   if (s >= nslices) return;
+  const int q0 = (wv % SP::QW) * NQ;  // first row pass of this wave
   const int64_t base = sptr[s];
   const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
-  A acc[CP::L][CP::CPL];
+  A acc[NQ][CP::CPL];
 #pragma unroll
-  for (int q = 0; q < CP::L; ++q)
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
     for (int u = 0; u < CP::CPL; ++u) acc[q][u] = 0;
   const int c0 = chunk * KC + sub * CP::CPL;  // first column of this lane
-  coop_acc<VT, XT, K, KC>(base, width, lane, rg, col, val, x + c0, c0 < kw,
-                          acc);
+  coop_acc<VT, XT, K, KC, NQ>(base, width, lane, rg, q0, col, val, x + c0,
+                              c0 < kw, acc);
 #pragma unroll
-  for (int q = 0; q < CP::L; ++q) {
-    const int32_t r = s * kSlice + q * CP::R + rg;
+  for (int q = 0; q < NQ; ++q) {
+    const int32_t r = s * kSlice + (q0 + q) * CP::R + rg;
     if (r >= n) continue;
     OT *o = out + (int64_t)r * K + c0;
     if constexpr (EPI == EPI_SCALE) {
@@ -743,6 +759,33 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
       }
     }
   }
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
+          int EPI>
+__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
+    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ kwp) {
+  coop_spmm_body<VT, XT, DT, OT, K, CB, EPI, 1>(n, nslices, sptr, col, val, x,
+                                                k, dvec, out, alpha, beta,
+                                                kwp);
+}
+
+// quarter-slice variant (AINV_MAP_COOP_Q / AINV_MAP_COOP128_Q)
+template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
+          int EPI>
+__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coopq(
+    int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ kwp) {
+  coop_spmm_body<VT, XT, DT, OT, K, CB, EPI, 4>(n, nslices, sptr, col, val, x,
+                                                k, dvec, out, alpha, beta,
+                                                kwp);
 }
 
 // Fused system-matrix product + dots in the cooperative mapping (ld = K).
@@ -801,7 +844,7 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
     for (int q = 0; q < L; ++q)
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
-    coop_acc<double, XT, K, KC>(base, width, lane, rg, col, val, x + c0,
+    coop_acc<double, XT, K, KC>(base, width, lane, rg, 0, col, val, x + c0,
                                 c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
@@ -852,21 +895,29 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
   }
 }
 
-template <typename VT, typename XT, typename DT, typename OT, int CB, int EPI>
+template <typename VT, typename XT, typename DT, typename OT, int CB, int EPI,
+          int QS = 1>
 void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
                  double alpha, double beta, hipStream_t s,
                  const int32_t *kw) {
   if (a.nslices == 0) return;
-  const int ngrp = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
   const int64_t *sp = a.sptr.as<int64_t>();
   const int32_t *cl = a.col.as<int32_t>();
   const VT *vl = a.val.as<VT>();
 #define AINV_LAUNCH(KK)                                                     \
-  hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, CB, EPI>),        \
-                     dim3(ngrp * CoopChunk<XT, KK, CB>::NCH), dim3(kBlock), 0, \
-                     s,                                                       \
-                     a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha, beta, \
-                     kw)
+  do {                                                                      \
+    constexpr int spb = CoopSplit<XT, KK, CB, QS>::SPB;                     \
+    const int ngrp = (a.nslices + spb - 1) / spb;                           \
+    const dim3 grid(ngrp * CoopChunk<XT, KK, CB>::NCH);                     \
+    if (QS == 1)                                                            \
+      hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, CB, EPI>),    \
+                         grid, dim3(kBlock), 0, s, a.n, a.nslices, sp, cl,  \
+                         vl, x, k, d, out, alpha, beta, kw);                \
+    else                                                                    \
+      hipLaunchKernelGGL((k_hll_spmm_coopq<VT, XT, DT, OT, KK, CB, EPI>),   \
+                         grid, dim3(kBlock), 0, s, a.n, a.nslices, sp, cl,  \
+                         vl, x, k, d, out, alpha, beta, kw);                \
+  } while (0)
   if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)
@@ -893,6 +944,12 @@ void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
     launch_coop<VT, XT, DT, OT, 256, EPI>(a, x, k, d, out, alpha, beta, s, kw);
   else if (map == AINV_MAP_COOP128)
     launch_coop<VT, XT, DT, OT, 128, EPI>(a, x, k, d, out, alpha, beta, s, kw);
+  else if (map == AINV_MAP_COOP_Q)
+    launch_coop<VT, XT, DT, OT, 256, EPI, 4>(a, x, k, d, out, alpha, beta, s,
+                                             kw);
+  else if (map == AINV_MAP_COOP128_Q)
+    launch_coop<VT, XT, DT, OT, 128, EPI, 4>(a, x, k, d, out, alpha, beta, s,
+                                             kw);
   else
     launch_row<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
 }
@@ -910,6 +967,9 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
                   hipStream_t s, const int32_t *kw) {
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
+  // the fused product + dots has no quarter-slice variant
+  if (map == AINV_MAP_COOP_Q) map = AINV_MAP_COOP;
+  if (map == AINV_MAP_COOP128_Q) map = AINV_MAP_COOP128;
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   const int g = persistent_grid(a.nslices);
   const int64_t *sp = a.sptr.as<int64_t>();
@@ -1019,7 +1079,8 @@ int hll_spmv_dot(const Hll &a, const float *x, double *y, int k, int64_t ld,
 // Row mapping needs ld = K (a power of two) and 16-byte aligned x and
 // output.
 static bool is_coop(int map) {
-  return map == AINV_MAP_COOP || map == AINV_MAP_COOP128;
+  return map == AINV_MAP_COOP || map == AINV_MAP_COOP128 ||
+         map == AINV_MAP_COOP_Q || map == AINV_MAP_COOP128_Q;
 }
 
 static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,

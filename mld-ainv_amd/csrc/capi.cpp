@@ -326,14 +326,19 @@ int map_from_string(const std::string &v) {
   if (v == "ROW") return AINV_MAP_ROW;
   if (v == "COOP") return AINV_MAP_COOP;
   if (v == "COOP128") return AINV_MAP_COOP128;
+  if (v == "COOP_Q") return AINV_MAP_COOP_Q;
+  if (v == "COOP128_Q") return AINV_MAP_COOP128_Q;
   throw Error(AINV_ERR_ARG,
-              "unknown APPLY_MAP '" + v + "' (AUTO, SPLIT, ROW, COOP, COOP128)");
+              "unknown APPLY_MAP '" + v +
+                  "' (AUTO, SPLIT, ROW, COOP, COOP128, COOP_Q, COOP128_Q)");
 }
 
 const char *map_name(int m) {
   return m == AINV_MAP_ROW       ? "row"
          : m == AINV_MAP_COOP    ? "coop"
          : m == AINV_MAP_COOP128 ? "coop128"
+         : m == AINV_MAP_COOP_Q    ? "coop_q"
+         : m == AINV_MAP_COOP128_Q ? "coop128_q"
                                  : "split";
 }
 

@@ -409,20 +409,22 @@ struct AdmitList {
   const double *src[kMaxCols];
 };
 
+// One thread per (row, admitted column): the lanes of a wave write the
+// entries of a few rows, each row's admitted columns by neighbouring lanes.
 __global__ void k_admit(int32_t n, int K, AdmitList L, double *__restrict__ b,
                         double *__restrict__ x, double *__restrict__ r,
                         double *__restrict__ q, float *__restrict__ r32) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  for (int j = 0; j < L.cnt; ++j) {
-    const double v = L.src[j][row * L.stride[j]];
-    const int64_t e = row * K + L.slot[j];
-    b[e] = v;
-    x[e] = 0.0;
-    r[e] = v;
-    if (q) q[e] = v;
-    if (r32) r32[e] = (float)v;
-  }
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * L.cnt) return;
+  const int64_t row = t / L.cnt;
+  const int j = (int)(t - row * L.cnt);
+  const double v = L.src[j][row * L.stride[j]];
+  const int64_t e = row * K + L.slot[j];
+  b[e] = v;
+  x[e] = 0.0;
+  r[e] = v;
+  if (q) q[e] = v;
+  if (r32) r32[e] = (float)v;
 }
 
 // With x0 = 0 every dot of the init step of an admitted column is ||b||^2:
@@ -748,7 +750,7 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   }
   if (a->spmv_map[li] < 0) {
     int best = 0;
-    for (int map = 0; map < kNumMaps; ++map) {
+    for (int map = 0; map < kNumSpmvMaps; ++map) {
       a->spmv_ms[li][map] = time_ms(s, [&] {
         hll_spmv_dot(a->hll, v.b, v.z, p->k, K, v.r, 1, p->pt, map, s);
       }, 3);
@@ -939,7 +941,9 @@ void admit(SlotBatch *b, const AdmitList &L, double eps) {
   KrylovPlan *p = b->p;
   const int32_t n = p->n;
   if (L.cnt == 0) return;
-  hipLaunchKernelGGL(k_admit, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(k_admit,
+                     dim3((unsigned)(((int64_t)n * L.cnt + 255) / 256)),
+                     dim3(256), 0,
                      b->s, n, b->K, L, p->v.b, p->v.x, p->v.r,
                      p->cg ? nullptr : p->v.q, x32_vecs(p).r32);
   AINV_HIP_CHECK(hipGetLastError());
@@ -2034,15 +2038,17 @@ __global__ void k_shrink_cols(int32_t n, int K1, int K2,
                               int64_t vs2, const float *__restrict__ f1,
                               float *__restrict__ f2, int nv32, int64_t fs1,
                               int64_t fs2, ColMap map) {
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  for (int j = 0; j < map.cnt; ++j) {
-    const int c = map.from[j];
-    for (int v = 0; v < nvec; ++v)
-      w2[v * vs2 + row * K2 + j] = w1[v * vs1 + row * K1 + c];
-    for (int v = 0; v < nv32; ++v)
-      f2[v * fs2 + row * K2 + j] = f1[v * fs1 + row * K1 + c];
-  }
+  // one thread per (row, kept column): neighbouring lanes move the columns
+  // of one row, so each load and store instruction covers a few rows
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * map.cnt) return;
+  const int64_t row = t / map.cnt;
+  const int j = (int)(t - row * map.cnt);
+  const int c = map.from[j];
+  for (int v = 0; v < nvec; ++v)
+    w2[v * vs2 + row * K2 + j] = w1[v * vs1 + row * K1 + c];
+  for (int v = 0; v < nv32; ++v)
+    f2[v * fs2 + row * K2 + j] = f1[v * fs1 + row * K1 + c];
 }
 
 // Column states to the new slots (the rest empty) and the active width the
@@ -2110,7 +2116,8 @@ void move_columns(SlotBatch *b, SlotBatch *nb, const std::vector<int> &keep) {
   ColMap map{};
   map.cnt = (int)keep.size();
   for (int j = 0; j < map.cnt; ++j) map.from[j] = keep[j];
-  hipLaunchKernelGGL(k_shrink_cols, dim3((unsigned)((n + 255) / 256)),
+  hipLaunchKernelGGL(k_shrink_cols,
+                     dim3((unsigned)(((int64_t)n * map.cnt + 255) / 256)),
                      dim3(256), 0, b->s, n, p1->K, p2->K,
                      p1->work.as<double>(), p2->work.as<double>(), nvec, vs1,
                      vs2, p1->work32.as<float>(), p2->work32.as<float>(), nv32,

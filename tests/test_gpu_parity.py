@@ -207,7 +207,8 @@ def test_autotuner_report(gpu):
     (t,) = [e for e in rep["tuning"] if e["K"] == 8]
     for pass_, key in (("w_ms", "map_w"), ("z_ms", "map_z")):
         ms = t[pass_]
-        assert set(ms) == {"split", "row", "coop", "coop128"}
+        assert set(ms) == {"split", "row", "coop", "coop128", "coop_q",
+                           "coop128_q"}
         assert min(ms.values()) > 0
         assert t[key] == min(ms, key=ms.get)
 
