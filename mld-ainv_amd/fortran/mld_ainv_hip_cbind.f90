@@ -6,7 +6,7 @@ module mld_ainv_hip_cbind
 
   integer(c_int), parameter :: ainv_success = 0
   integer(c_int), parameter :: ainv_kind_ainv = 1, ainv_kind_invk = 2, &
-       & ainv_kind_invt = 3
+       & ainv_kind_invt = 3, ainv_kind_ml = 4
   integer(c_int), parameter :: ainv_alg_llk = 1, ainv_alg_s_llk = 2, &
        & ainv_alg_s_ft_llk = 3, ainv_alg_llk_noth = 4, ainv_alg_mlk = 5
   integer(c_int), parameter :: ainv_prec_f64 = 0, ainv_prec_f32 = 1, &

@@ -4,6 +4,10 @@
 !   mld_d_ainv_solver_type       (src/mld_d_ainv_solver.F90:46-67)
 !   mld_d_invk_solver_type       (src/mld_d_invk_solver.f90:47-57)
 !   mld_d_invt_solver_type       (src/mld_d_invt_solver.f90:46-58)
+! and mld_d_ml_prec_type, the multilevel preconditioner of MLD2P4
+! (prec%init('ML'), configured as tests/pdegen/runs/ppde.inp:10-29 does) with
+! these factors as smoother and coarsest-level solver, set through the same
+! set / build / apply procedures (ML_TYPE, SMOOTHER_POS, N_PREC_LEVS, ...).
 ! Each procedure is one call through iso_c_binding; info follows the
 ! reference convention (0 = success).  The local matrix block is passed as
 ! 1-based CSR arrays (what psb_dspmat_type holds for a CSR local block);
@@ -18,7 +22,8 @@ module mld_d_ainv_hip_mod
   private
 
   public :: mld_d_base_ainv_solver_type, mld_d_ainv_solver_type, &
-       & mld_d_invk_solver_type, mld_d_invt_solver_type, mld_krylov, &
+       & mld_d_invk_solver_type, mld_d_invt_solver_type, &
+       & mld_d_ml_prec_type, mld_krylov, &
        & mld_ainv_init
 
   type, extends(mld_d_base_solver_type) :: mld_d_base_ainv_solver_type
@@ -60,6 +65,10 @@ module mld_d_ainv_hip_mod
   contains
     procedure, pass(sv) :: kind_id => invt_kind_id
   end type
+  type, extends(mld_d_base_ainv_solver_type) :: mld_d_ml_prec_type
+  contains
+    procedure, pass(sv) :: kind_id => ml_kind_id
+  end type
 
 contains
 
@@ -84,6 +93,10 @@ contains
   integer function invt_kind_id(sv)
     class(mld_d_invt_solver_type), intent(in) :: sv
     invt_kind_id = ainv_kind_invt
+  end function
+  integer function ml_kind_id(sv)
+    class(mld_d_ml_prec_type), intent(in) :: sv
+    ml_kind_id = ainv_kind_ml
   end function
 
   ! creates the C object on first use (applies the defaults)

@@ -6,16 +6,24 @@
 !
 ! stdin deck, one value per line (text after the value is ignored):
 !   method (BICGSTAB|CG|CGS|BICG|RGMRES|BICGSTABL), preconditioner
-!   (AINV|INVK|INVT), grid size N, eps, itmax, sub_fillin, inv_fillin,
+!   (AINV|INVK|INVT|ML), grid size N, eps, itmax, sub_fillin, inv_fillin,
 !   sub_iluthrs, inv_thresh, ainv_alg, factor storage (F64|F32|BF16) and,
 !   optionally, irst (restart of RGMRES / l of BiCGStab(l), as the
-!   reference decks' IRST line, tests/pdegen/runs/mld3d.inp:10)
+!   reference decks' IRST line, tests/pdegen/runs/mld3d.inp:10); for ML
+!   then, optionally, the multilevel fields of tests/pdegen/runs/ppde.inp:
+!   smoother factors (AINV|INVK|INVT), smoother sweeps, levels,
+!   aggregation kind (SMOOTHED|NONSMOOTHED), correction (MULT|ADD), side
+!   (PRE|POST|TWOSIDE), coarsest-level factors, coarsest-level sweeps,
+!   aggregation threshold
 program ppde3d_hip
   use iso_c_binding
   use mld_ainv_hip_cbind, only: ainv_gen_pde3d, last_error
   use mld_d_ainv_hip_mod
   implicit none
   character(len=20) :: method, ptype, alg, fprec
+  character(len=20) :: mlsub, mlkind, mltype, mlpos, mlcoarse
+  integer :: mlsweeps, mlnlev, mlcsweeps
+  real(c_double) :: mlthr
   integer :: idim, itmax, fill1, fill2, info, iter, i, irst, ios
   real(c_double) :: eps, thr1, thr2, err
   integer(c_int32_t) :: n
@@ -40,6 +48,19 @@ program ppde3d_hip
   read(*, *) fprec
   read(*, *, iostat=ios) irst
   if (ios /= 0) irst = 0
+  ! multilevel fields (defaults: the ppde.inp deck, coarse sweeps aside)
+  mlsub = 'AINV'; mlsweeps = 1; mlnlev = 3; mlkind = 'SMOOTHED'
+  mltype = 'MULT'; mlpos = 'POST'; mlcoarse = 'AINV'; mlcsweeps = 15
+  mlthr = 0.1d0
+  if (ios == 0) read(*, *, iostat=ios) mlsub
+  if (ios == 0) read(*, *, iostat=ios) mlsweeps
+  if (ios == 0) read(*, *, iostat=ios) mlnlev
+  if (ios == 0) read(*, *, iostat=ios) mlkind
+  if (ios == 0) read(*, *, iostat=ios) mltype
+  if (ios == 0) read(*, *, iostat=ios) mlpos
+  if (ios == 0) read(*, *, iostat=ios) mlcoarse
+  if (ios == 0) read(*, *, iostat=ios) mlcsweeps
+  if (ios == 0) read(*, *, iostat=ios) mlthr
 
   call mld_ainv_init(0, info)
   if (info /= 0) call fail('init')
@@ -59,6 +80,8 @@ program ppde3d_hip
     allocate(mld_d_invk_solver_type :: prec)
   case ('INVT')
     allocate(mld_d_invt_solver_type :: prec)
+  case ('ML')
+    allocate(mld_d_ml_prec_type :: prec)
   case default
     allocate(mld_d_ainv_solver_type :: prec)
   end select
@@ -66,7 +89,20 @@ program ppde3d_hip
   call prec%set('INV_FILLIN', fill2, info)
   call prec%set('SUB_ILUTHRS', thr1, info)
   call prec%set('INV_THRESH', thr2, info)
-  if (trim(ptype) == 'AINV') call prec%set('AINV_ALG', trim(alg), info)
+  if (trim(ptype) == 'AINV' .or. trim(ptype) == 'ML') &
+       & call prec%set('AINV_ALG', trim(alg), info)
+  if (trim(ptype) == 'ML') then
+    call prec%set('SUB_SOLVE', trim(mlsub), info)
+    call prec%set('SMOOTHER_SWEEPS', mlsweeps, info)
+    call prec%set('N_PREC_LEVS', mlnlev, info)
+    call prec%set('AGGR_KIND', trim(mlkind), info)
+    call prec%set('ML_TYPE', trim(mltype), info)
+    call prec%set('SMOOTHER_POS', trim(mlpos), info)
+    call prec%set('COARSE_SUBSOLVE', trim(mlcoarse), info)
+    call prec%set('COARSE_SWEEPS', mlcsweeps, info)
+    call prec%set('AGGR_THRESH', mlthr, info)
+    if (info /= 0) call fail('multilevel settings')
+  end if
   call prec%set('FACTOR_PREC', trim(fprec), info)
   call prec%check(info)
 

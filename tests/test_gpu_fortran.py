@@ -75,3 +75,30 @@ def test_fortran_driver_methods_and_irst(mld, O, method, irst):
     tol = irst if method == "BICGSTABL" else 1
     assert abs(its - it_ref) <= tol, (its, it_ref, text)
     assert float(f["Max error |x - 1|"]) < 1e-5
+
+
+@pytest.mark.parametrize("sub,pos,mltype", [("INVK", "POST", "MULT"),
+                                            ("AINV", "TWOSIDE", "MULT"),
+                                            ("INVK", "POST", "ADD")])
+def test_fortran_driver_multilevel(mld, O, sub, pos, mltype):
+    """The ML deck fields of tests/pdegen/runs/ppde.inp (smoother factors,
+    sweeps, levels, aggregation, correction, side, coarsest-level factors and
+    sweeps, threshold) reach the multilevel preconditioner through the
+    Fortran type mld_d_ml_prec_type; iterations match the oracle's cycle."""
+    N = 14
+    fill1, fill2, thr = (1, 1, 0.0) if sub == "INVK" else (6, 0, 5e-2)
+    lines = ["BICGSTAB", "ML", str(N), "1.d-8", "300", str(fill1),
+             str(fill2), repr(thr), "0.0", "LLK", "F64", "0",
+             sub, "1", "3", "SMOOTHED", mltype, pos, sub, "4", "0.05"]
+    f, text = run_driver("\n".join(lines) + "\n")
+    its = int(f["Number of iterations"])
+    a = mld.gen_pde3d(N)
+    b = a @ np.ones(a.shape[0])
+    spec = ({"kind": "INVK", "fill_in": 1, "inv_fill": 1} if sub == "INVK"
+            else {"kind": "AINV", "alg": "LLK", "fill_in": 6, "thresh": 5e-2})
+    H = O.MlHierarchy(a, nlev=3, theta=0.05, sub=spec, coarse=spec,
+                      ml_type=mltype, pos=pos, sweeps=1, csweeps=4)
+    _, it_ref, _ = O.ml_krylov("BICGSTAB", a, H, b, 1e-8, 300)
+    assert abs(its - it_ref) <= 1, (its, it_ref, text)
+    assert float(f["Max error |x - 1|"]) < 1e-5
+    assert "Multilevel" in text
