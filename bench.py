@@ -163,12 +163,7 @@ def ml_cycle_bytes(levels, k, sweeps=1, csweeps=2, eb=8):
     return tot
 
 
-def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
-           M, fp64_rate, it64):
-    """The multilevel preconditioner on the same problem and right-hand
-    sides: single-RHS FP64 solves, the served form with --prec level
-    factors, and one cycle's duration at the served block width."""
-    n = a.shape[0]
+def new_ml(m, kind, params):
     pv = m.MldMlPrec()
     for key, v in (ML_FACTORS if kind == "INVK" else params).items():
         pv.set(key, v)
@@ -176,6 +171,42 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
     pv.set("COARSE_SUBSOLVE", kind)
     for key, v in ML_PARAMS.items():
         pv.set(key, v)
+    return pv
+
+
+def ml_cycle_only(m, args, a, A, kind, params, method, slots):
+    """--ml-cycle: the cycle's kernels without graph replays (for kernel
+    traces): build, a two-iteration eager solve at the served width that
+    tunes the mappings on the vectors the served form uses, 10 cycles."""
+    n = a.shape[0]
+    pv = new_ml(m, kind, params)
+    t0 = time.time()
+    pv.build(a)
+    t_build = time.time() - t0
+    pv.cnv(args.prec)
+    B = m.gen_normal(13, (n, slots))
+    m.krylov(method, A, pv, B, eps=1e-8, itmax=2, eager=True)
+    x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
+    ms, _ = profile_apply(m, pv, n, slots, 10, x32, x32)
+    lv = pv.report()["multilevel"]["levels"]
+    byt = ml_cycle_bytes(lv, slots, 1, ML_PARAMS["COARSE_SWEEPS"],
+                         4 if x32 else 8)
+    print(json.dumps({"ml_cycle": True, "config": args.config, "k": slots,
+                      "factor_prec": args.prec,
+                      "vectors": "FP32" if x32 else "FP64",
+                      "build_s": t_build, "ms": ms, "bytes": byt,
+                      "GBs": byt / (ms * 1e-3) / 1e9,
+                      "levels": [(L["n"], L["nnz"], L["nnz_w"], L["nnz_p"])
+                                 for L in lv]}))
+
+
+def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
+           M, fp64_rate, it64):
+    """The multilevel preconditioner on the same problem and right-hand
+    sides: single-RHS FP64 solves, the served form with --prec level
+    factors, and one cycle's duration at the served block width."""
+    n = a.shape[0]
+    pv = new_ml(m, kind, params)
     t0 = time.time()
     pv.build(a)
     t_build = time.time() - t0
@@ -288,6 +319,10 @@ def main():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-ml", action="store_true",
                    help="skip the multilevel leg (C1, C2)")
+    p.add_argument("--ml-cycle", action="store_true",
+                   help="time the multilevel cycle alone (C1, C2): build, "
+                        "tune the mappings with a short eager solve, then "
+                        "10 cycles at the served block width; no graphs")
     p.add_argument("--cpu-iters", type=int, default=None,
                    help="oracle iterations per timed run (default: ~2 s)")
     p.add_argument("--apply-only", action="store_true",
@@ -345,6 +380,9 @@ def main():
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
         return
     A = m.SystemMatrix(a)
+    if args.ml_cycle:
+        ml_cycle_only(m, args, a, A, kind, params, method, slots)
+        return
 
     # ---- right-hand sides, resident on the device: M n-vectors
     t0 = time.time()
