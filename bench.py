@@ -334,9 +334,10 @@ def main():
                         "COOP128)")
     args = p.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}): every "
-              "GPU box of this setup has a single GPU; DESIGN.md describes "
-              "how the path shards", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the "
+              "sharded path (a row block per GPU, halo exchange and "
+              "all-reduced dots in the Krylov loop, no communication in the "
+              "apply) is not built", file=sys.stderr)
         sys.exit(2)
 
     import importlib
