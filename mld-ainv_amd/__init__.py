@@ -763,7 +763,8 @@ class Server:
     requests of 1..64 right-hand sides batched per matrix with continuous
     admission into freed column slots."""
 
-    KINDS = {"AINV": KIND_AINV, "INVK": KIND_INVK, "INVT": KIND_INVT}
+    KINDS = {"AINV": KIND_AINV, "INVK": KIND_INVK, "INVT": KIND_INVT,
+             "ML": KIND_ML}
 
     def __init__(self, hbm_budget_bytes=64 << 30, workers=2, device=0):
         _ensure_init()

@@ -300,6 +300,25 @@ void copy_settings(const ainv_solver *src, ainv_solver *dst) {
   dst->ml_coarse = src->ml_coarse;
 }
 
+// Real-valued keys of csetr (cseti ignores them, as the reference's
+// integer setters pass unknown keys to the base solver).  Returns whether
+// the key is one of them.
+bool set_real(ainv_solver *h, const std::string &w, double val) {
+  const bool ml = h->kind == AINV_KIND_ML;
+  if (w == "SUB_ILUTHRS") {
+    if (h->kind != AINV_KIND_INVK || ml) h->thresh = val;
+  } else if (w == "INV_THRESH") {
+    if (h->kind == AINV_KIND_INVT || ml) h->inv_thresh = val;
+  } else if (w == "AGGR_THRESH") {
+    if (ml) h->ml_thresh = val;
+  } else if (w == "AGGR_SCALE") {
+    if (ml) h->ml_scale = val;
+  } else {
+    return false;
+  }
+  return true;
+}
+
 // AINV / INVK / INVT from a name (SUB_SOLVE, COARSE_SUBSOLVE)
 int kind_from_string(const std::string &v) {
   if (v == "AINV") return AINV_KIND_AINV;
@@ -406,7 +425,11 @@ void apply_param(ainv_solver *h, const std::string &key,
   char *end = nullptr;
   const long iv = std::strtol(val.c_str(), &end, 10);
   int rc;
-  if (end && *end == '\0' && !val.empty()) {
+  // real-valued keys take an integer-looking value as a real
+  const std::string ku = upper_trim(key.c_str());
+  const bool real_key = ku == "SUB_ILUTHRS" || ku == "INV_THRESH" ||
+                        ku == "AGGR_THRESH" || ku == "AGGR_SCALE";
+  if (end && *end == '\0' && !val.empty() && !real_key) {
     rc = ainv_cseti(h, key.c_str(), (int)iv);
   } else {
     end = nullptr;
@@ -638,17 +661,7 @@ int ainv_csetr(ainv_h h, const char *what, double val) {
   return guard([&] {
     check_handle(h);
     std::lock_guard<std::mutex> lk(h->mu);
-    const std::string w = upper_trim(what);
-    const bool ml = h->kind == AINV_KIND_ML;
-    if (w == "SUB_ILUTHRS" && (h->kind != AINV_KIND_INVK || ml)) {
-      h->thresh = val;
-    } else if (w == "INV_THRESH" && (h->kind == AINV_KIND_INVT || ml)) {
-      h->inv_thresh = val;
-    } else if (ml && w == "AGGR_THRESH") {
-      h->ml_thresh = val;
-    } else if (ml && w == "AGGR_SCALE") {
-      h->ml_scale = val;
-    }
+    set_real(h, upper_trim(what), val);
   });
 }
 

@@ -165,6 +165,9 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
   m->bytes = A->hll.bytes() + (int64_t)(A->csr.ptr.bytes + A->csr.idx.bytes +
                                         A->csr.val.bytes) +
              solver_device_bytes(M) + plan_bytes(m->a.n, K, m->method);
+  // the multilevel cycle's workspace instead of one apply intermediate
+  if (M->kind == AINV_KIND_ML)
+    m->bytes += (int64_t)solver_t_bytes(M, K) - (int64_t)m->a.n * K * 8;
   // BiCG builds the transposed layouts of A and of both factors on first use
   if (m->method == KM_BICG)
     m->bytes += A->hll.bytes() + M->hw.bytes() + M->hz.bytes();
@@ -438,7 +441,7 @@ int ainv_server_add_matrix(ainv_server_h S, int32_t n, const int64_t *irp,
   return guard([&] {
     if (!S || !mat_id) throw Error(AINV_ERR_ARG, "NULL argument");
     if (kind != AINV_KIND_AINV && kind != AINV_KIND_INVK &&
-        kind != AINV_KIND_INVT)
+        kind != AINV_KIND_INVT && kind != AINV_KIND_ML)
       throw Error(AINV_ERR_ARG, "unknown solver kind");
     auto m = std::make_unique<ServedMatrix>();
     m->a = canonical_csr_public(n, irp, ja, val, index_base);
@@ -459,6 +462,10 @@ int ainv_server_add_matrix(ainv_server_h S, int32_t n, const int64_t *irp,
         m->params.emplace_back(k, kv.second);
       }
     }
+    if (kind == AINV_KIND_ML && m->method == KM_BICG)
+      throw Error(AINV_ERR_ARG,
+                  "BICG applies M^T, which the multilevel preconditioner does "
+                  "not offer");
     std::unique_lock<std::mutex> lk(S->mu);
     m->id = S->next_mat++;
     *mat_id = m->id;

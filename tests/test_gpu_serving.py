@@ -147,3 +147,52 @@ def test_failed_build_and_unwaited_requests(gpu):
                                           eps=1e-8))
     assert np.all(err <= 1e-8)
     srv2.free()
+
+
+def test_multilevel_matrix_in_the_pool(gpu):
+    """A matrix served with the multilevel preconditioner (kind ML, FP32
+    level factors, so the served BiCGSTAB runs the FP32 cycle) beside a
+    one-level one: results as the single solves give them; a real key given
+    an integer-looking value still reaches its setter; BiCG is refused when
+    the matrix is registered."""
+    a_ml = gpu.gen_pde3d(14, 1)
+    p_ml = dict(SUB_SOLVE="INVK", COARSE_SUBSOLVE="INVK", SUB_FILLIN=0,
+                INV_FILLIN=1, N_PREC_LEVS=3, AGGR_THRESH=0,
+                COARSE_SWEEPS=2, FACTOR_PREC="F32")
+    a_1 = gpu.gen_pde3d(10)
+    p_1 = dict(SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC="F32")
+    srv = gpu.Server(hbm_budget_bytes=8 << 30, workers=2)
+    mid = srv.add_matrix(a_ml, "ML", **p_ml)
+    oid = srv.add_matrix(a_1, "INVK", **p_1)
+    with pytest.raises(gpu.AinvError) as e:
+        srv.add_matrix(a_ml, "ML", METHOD="BICG", **p_ml)
+    assert "M^T" in str(e.value)
+    tickets = []
+    for r in range(8):
+        a, m = (a_ml, mid) if r % 2 == 0 else (a_1, oid)
+        b = gpu.gen_normal(300 + r, (a.shape[0], 1 + r % 5))
+        tickets.append((a, r % 2, b, srv.submit(m, b, eps=1e-9)))
+    # the reference object gets the threshold as a real (an integer goes to
+    # cseti, which ignores real keys as the reference's setters do; the
+    # server's "KEY=value" parsing sends real keys to csetr)
+    ref = gpu.MldMlPrec()
+    for k, v in p_ml.items():
+        ref.set(k, float(v) if k == "AGGR_THRESH" else v)
+    ref.build(a_ml)
+    assert ref.getr("AGGR_THRESH") == 0.0
+    assert ref.report()["multilevel"]["levels"][0]["theta"] == 0.0
+    for a, one, b, t in tickets:
+        x, it, err, lat = srv.wait(t)
+        assert np.all(err <= 1e-9)
+        if one:
+            _, ita = alone(gpu, a, "INVK", p_1, b, 1e-9)
+        else:
+            _, ita, _ = gpu.krylov("BICGSTAB", gpu.SystemMatrix(a), ref, b,
+                                   eps=1e-9)
+        for c in range(b.shape[1]):
+            assert abs(int(it[c]) - int(ita[c])) <= 1, (one, c, it, ita)
+            tr = np.linalg.norm(b[:, c] - a @ x[:, c]) / np.linalg.norm(b[:, c])
+            assert tr < 1e-7, (one, c, tr)
+    st = srv.stats()
+    assert st["completed"] == 8
+    srv.free()
