@@ -502,7 +502,9 @@ def main():
         kernel, extra = {"split": ("k_hll_spmm", ""),
                          "row": ("k_hll_spmm_row", ""),
                          "coop": ("k_hll_spmm_coop", " 256,"),
-                         "coop128": ("k_hll_spmm_coop", " 128,")}[kmap]
+                         "coop128": ("k_hll_spmm_coop", " 128,"),
+                         "coop_q": ("k_hll_spmm_coopq", " 256,"),
+                         "coop128_q": ("k_hll_spmm_coopq", " 128,")}[kmap]
         if first:
             return f"{kernel}<{vt}, {xt}, {tt}, {tt}, {slots},{extra} 0>"
         yt = "float" if y32 else "double"
