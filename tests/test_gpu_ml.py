@@ -286,3 +286,30 @@ def test_fp32_cycle(gpu, O, prec, tol):
     pv.cnv("F64")
     _, it64, _ = gpu.krylov("BICGSTAB", A, pv, B, eps=1e-8)
     assert np.all(err32 <= 1e-8) and np.all(it32 <= it64 + 2), (it32, it64)
+
+
+def test_cg_on_the_fp32_cycle(gpu, O):
+    """CG hands FP32 / BF16 factors the FP32 copy of r and keeps z FP32: the
+    multilevel cycle then runs on FP32 vectors; counts close to the FP64
+    cycle's, graph replays bit-identical.  (POST: with these undamped
+    smoothers the TWOSIDE cycle, though symmetric, needs some 340 CG
+    iterations on this matrix in FP64, too close to failure to compare.)"""
+    a = gpu.gen_aniso2d(48, 1e-2)
+    n = a.shape[0]
+    pv, ref = build_ml(gpu, O, a, "AINV", pos="POST")
+    pv.set("AINV_ALG", "SYM-LLK")
+    pv.build(a)
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(14, (n, 4))
+    _, it64, err64 = gpu.krylov("CG", A, pv, B, eps=1e-8, itmax=400)
+    pv.cnv("F32")
+    x32, it32, err32 = gpu.krylov("CG", A, pv, B, eps=1e-8, itmax=400)
+    x32e, it32e, _ = gpu.krylov("CG", A, pv, B, eps=1e-8, itmax=400,
+                                eager=True)
+    assert np.all(err64 <= 1e-8) and np.all(err32 <= 1e-8)
+    assert np.all(np.abs(it32.astype(int) - it64.astype(int)) <= 2), \
+        (it32, it64)
+    assert np.array_equal(it32, it32e) and np.array_equal(x32, x32e)
+    for j in range(4):
+        r = B[:, j] - a @ x32[:, j]
+        assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(B[:, j])
