@@ -39,6 +39,13 @@ N(0,1) right-hand sides (seed 3); C4 random nonsymmetric n = 10^6
 (LogNormal(ln 12, 1.0) off-diagonals capped at 400, band 5000, 2
 long-range columns, diagonal 1.1 x row sum) with AINV-LLK (fill 8, thresh
 5e-2), BiCGSTAB 1e-8, 16 slots, N(0,1) right-hand sides (seed 4).
+
+C1, C2 and C3 also report, as `multilevel`, the multilevel preconditioner
+(smoothed-aggregation V-cycle with factors of the configuration's kind as
+smoother and coarsest-level solver; ML_CONFIGS below) on the same matrix and
+right-hand sides: its FP64 single-RHS solves, its served form with --prec
+level factors and one cycle's duration at the served block width
+(--no-ml skips it; --ml-cycle times the cycle alone).
 """
 import argparse
 import ctypes as C
@@ -93,6 +100,20 @@ ML_PARAMS = dict(N_PREC_LEVS=3, AGGR_KIND="SMOOTHED", AGGR_THRESH=0.0,
 # factors of every level: INVK with ILU(0) and inverse fill 1 (the Galerkin
 # levels are denser than A, and ILU(1) on them fills heavily)
 ML_FACTORS = dict(SUB_FILLIN=0, INV_FILLIN=1)
+# per configuration: (cycle parameters, factor parameters or None for the
+# configuration's own, Krylov method or None for the configuration's).
+# C3, the anisotropic problem: threshold 0.10 as the deck (threshold 0
+# aggregates across the weak direction and the cycle stops converging), 6
+# levels (aggregates of about 3 rows: the coarsest is then ~17,000 rows),
+# and BiCGSTAB: the POST cycle is not symmetric and CG fails with it, while
+# the symmetric TWOSIDE cycle makes a poor preconditioner with these
+# undamped smoothers (it fails under CG too)
+ML_CONFIGS = {
+    "C1": (ML_PARAMS, None, None),
+    "C2": (ML_PARAMS, ML_FACTORS, None),
+    "C3": (dict(ML_PARAMS, N_PREC_LEVS=6, AGGR_THRESH=0.1), None,
+           "BICGSTAB"),
+}
 
 
 def log(*a):
@@ -163,14 +184,22 @@ def ml_cycle_bytes(levels, k, sweeps=1, csweeps=2, eb=8):
     return tot
 
 
-def new_ml(m, kind, params):
+def ml_setup(config, kind, params, method):
+    """(cycle parameters, factor parameters, Krylov method) of the
+    multilevel leg of a configuration."""
+    mp, fp, meth = ML_CONFIGS[config]
+    return mp, dict(fp if fp is not None else params), meth or method
+
+
+def new_ml(m, config, kind, params, method):
+    mp, fp, _ = ml_setup(config, kind, params, method)
     pv = m.MldMlPrec()
-    for key, v in (ML_FACTORS if kind == "INVK" else params).items():
+    for key, v in fp.items():
         pv.set(key, v)
     pv.set("SUB_SOLVE", kind)
     pv.set("COARSE_SUBSOLVE", kind)
-    for key, v in ML_PARAMS.items():
-        pv.set(key, v)
+    for key, v in mp.items():
+        pv.set(key, float(v) if key in ("AGGR_THRESH", "AGGR_SCALE") else v)
     return pv
 
 
@@ -179,7 +208,8 @@ def ml_cycle_only(m, args, a, A, kind, params, method, slots):
     traces): build, a two-iteration eager solve at the served width that
     tunes the mappings on the vectors the served form uses, 10 cycles."""
     n = a.shape[0]
-    pv = new_ml(m, kind, params)
+    mp, _, method = ml_setup(args.config, kind, params, method)
+    pv = new_ml(m, args.config, kind, params, method)
     t0 = time.time()
     pv.build(a)
     t_build = time.time() - t0
@@ -189,7 +219,7 @@ def ml_cycle_only(m, args, a, A, kind, params, method, slots):
     x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
     ms, _ = profile_apply(m, pv, n, slots, 10, x32, x32)
     lv = pv.report()["multilevel"]["levels"]
-    byt = ml_cycle_bytes(lv, slots, 1, ML_PARAMS["COARSE_SWEEPS"],
+    byt = ml_cycle_bytes(lv, slots, 1, mp["COARSE_SWEEPS"],
                          4 if x32 else 8)
     print(json.dumps({"ml_cycle": True, "config": args.config, "k": slots,
                       "factor_prec": args.prec,
@@ -206,7 +236,8 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
     sides: single-RHS FP64 solves, the served form with --prec level
     factors, and one cycle's duration at the served block width."""
     n = a.shape[0]
-    pv = new_ml(m, kind, params)
+    mp, fp, method = ml_setup(args.config, kind, params, method)
+    pv = new_ml(m, args.config, kind, params, method)
     t0 = time.time()
     pv.build(a)
     t_build = time.time() - t0
@@ -261,7 +292,7 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
     x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
     ms, _ = profile_apply(m, pv, n, slots, 10, x32, x32)
     lv = pv.report()["multilevel"]["levels"]
-    byt = ml_cycle_bytes(lv, slots, 1, ML_PARAMS["COARSE_SWEEPS"],
+    byt = ml_cycle_bytes(lv, slots, 1, mp["COARSE_SWEEPS"],
                          4 if x32 else 8)
     cyc = {"factor_prec": args.prec, "vectors": "FP32" if x32 else "FP64",
            "ms": ms, "bytes": byt, "GBs": byt / (ms * 1e-3) / 1e9,
@@ -269,9 +300,8 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
     log(f"multilevel served {args.prec}: {rate:.2f} solves/s, iterations "
         f"mean {float(np.mean(iters)):.1f}; cycle {cyc}")
     return {
-        "params": ML_PARAMS,
-        "factors": {"kind": kind,
-                    **(ML_FACTORS if kind == "INVK" else params)},
+        "params": mp, "method": method,
+        "factors": {"kind": kind, **fp},
         "build_s": t_build,
         "levels": [{k: L[k] for k in ("n", "nnz", "aggregates", "nnz_w",
                                       "nnz_z", "theta", "omega")}
@@ -318,7 +348,7 @@ def main():
     p.add_argument("--prec", default="F32")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-ml", action="store_true",
-                   help="skip the multilevel leg (C1, C2)")
+                   help="skip the multilevel leg (C1, C2, C3)")
     p.add_argument("--ml-cycle", action="store_true",
                    help="time the multilevel cycle alone (C1, C2): build, "
                         "tune the mappings with a short eager solve, then "
@@ -532,7 +562,7 @@ def main():
         f"({passes[1]['achieved_GBs']:.0f} GB/s)")
 
     ml = None
-    if not args.no_ml and args.config in ("C1", "C2"):
+    if not args.no_ml and args.config in ML_CONFIGS:
         ml = ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps,
                     slots, M, fp64_rate, it64)
 
