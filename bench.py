@@ -299,6 +299,11 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
            "roofline_frac": byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     log(f"multilevel served {args.prec}: {rate:.2f} solves/s, iterations "
         f"mean {float(np.mean(iters)):.1f}; cycle {cyc}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = ml_cpu_baseline(pv, a, Bh[sample[0]], method, mp,
+                              float(np.mean(its)))
+        log(f"multilevel cpu baseline {cpu}")
     return {
         "params": mp, "method": method,
         "factors": {"kind": kind, **fp},
@@ -321,6 +326,7 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
                    "true_residual_max": true,
                    "vs_one_level_fp64_single": rate / fp64_rate},
         "cycle_k": slots, "cycle": cyc,
+        "cpu_baseline": cpu,
     }
 
 
@@ -840,6 +846,76 @@ def _host_info():
     except OSError:
         pass
     return model, len(os.sched_getaffinity(0))
+
+
+def _timed_legs(run, label, gpu_iters, threads):
+    """One warm-up iteration (which sizes the runs to about 2 s), then 5
+    timed runs; median seconds per iteration extrapolated to one solve of
+    gpu_iters iterations; single-thread figure with the threaded one beside
+    it."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    model, nproc = _host_info()
+    if threads is None:
+        threads = min(16, nproc)
+
+    def timed(nt, mm):
+        O.set_threads(nt)
+        t0 = time.perf_counter()
+        it = run(mm)
+        return (time.perf_counter() - t0) / max(it, 1), it
+
+    def leg(nt):
+        per0, _ = timed(nt, 1)
+        mm = max(1, int(round(2.0 / max(per0, 1e-6))))
+        runs = [timed(nt, mm) for _ in range(5)]
+        per = float(np.median([r[0] for r in runs]))
+        return {"value": 1.0 / (per * gpu_iters), "unit": "solves/s",
+                "cores": nt, "kind": "port",
+                "sample": f"5 runs of {runs[0][1]} {label}, after one "
+                          f"warm-up; median {per:.4f} s per iteration, "
+                          f"extrapolated to {gpu_iters:.1f} iterations per "
+                          f"solve",
+                "seconds_per_iteration": per}
+
+    try:
+        one = leg(1)
+        one["host"] = {"cpu_model": model, "nproc": nproc,
+                       "compiler": O.build_info()}
+        if threads > 1:
+            mt = leg(threads)
+            mt["note"] = ("the same loops row-parallel with OpenMP")
+            one["threaded"] = mt
+    finally:
+        O.set_threads(1)
+    return one
+
+
+def ml_cpu_baseline(pv, a, b, method, mp, gpu_iters, threads=None):
+    """The oracle's cycle inside its Krylov method on the same hierarchy
+    (every level's A, P, R and FP64 factors exported from the library, which
+    builds them bit-identical to the oracle's own build), timed as
+    cpu_baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nl = pv.levels()
+    H = O.MlHierarchy.from_levels(
+        [pv.level(l, "A") for l in range(nl)],
+        [pv.level(l, "P") for l in range(nl - 1)],
+        [pv.level(l, "R") for l in range(nl - 1)],
+        [(pv.level(l, "W"), pv.level(l, "Z"), pv.level(l, "D"))
+         for l in range(nl)],
+        ml_type=mp["ML_TYPE"], pos=mp["SMOOTHER_POS"],
+        sweeps=mp["SMOOTHER_SWEEPS"], csweeps=mp["COARSE_SWEEPS"])
+    A = O.wrap(a)
+
+    def run(mm):
+        _, it, _ = O.ml_krylov(method, A, H, b, 1e-30, itmax=mm)
+        return it
+
+    return _timed_legs(run, f"{method} iterations of the oracle's cycle on "
+                       "the same hierarchy (FP64 factors) and one right-hand "
+                       "side of the queue", gpu_iters, threads)
 
 
 def cpu_baseline(sv, a, b, method, iters, gpu_iters, threads=None):

@@ -414,6 +414,23 @@ class MlHierarchy:
         self.sweeps, self.csweeps = sweeps, csweeps
         self._pack()
 
+    @classmethod
+    def from_levels(cls, A, P, R, F, ml_type="MULT", pos="POST", sweeps=1,
+                    csweeps=1):
+        """A hierarchy given level by level (scipy CSR A_l, P_l, R_l and
+        factors F_l = (W, Z, d)), for instance exported from the library."""
+        self = cls.__new__(cls)
+        self.A = [sp.csr_matrix(x) for x in A]
+        self.P = [sp.csr_matrix(x) for x in P]
+        self.R = [sp.csr_matrix(x) for x in R]
+        self.F = [(sp.csr_matrix(w), sp.csr_matrix(z), np.asarray(d))
+                  for w, z, d in F]
+        self.agg, self.omega = [], []
+        self.type, self.pos = ML_TYPE[ml_type.upper()], ML_POS[pos.upper()]
+        self.sweeps, self.csweeps = sweeps, csweeps
+        self._pack()
+        return self
+
     def _pack(self):
         L = len(self.A)
         self._keep = [[_In(m) for m in self.A], [_InRect(m) for m in self.P],

@@ -335,3 +335,16 @@ def test_ml_krylov_bicg_refused():
     H = O.MlHierarchy(a, nlev=2, theta=0.0)
     with pytest.raises(O.OracleError):
         O.ml_krylov("BICG", a, H, np.ones(a.shape[0]), 1e-6)
+
+
+def test_hierarchy_from_levels_is_the_same_cycle():
+    """A hierarchy rebuilt from its levels (how the benchmark's CPU
+    baseline takes the library's exported hierarchy) applies the same
+    cycle bit for bit."""
+    a = grid2d(20, conv=0.3)
+    H = O.MlHierarchy(a, nlev=3, theta=0.0,
+                      sub={"kind": "INVK", "fill_in": 0, "inv_fill": 1},
+                      csweeps=2)
+    H2 = O.MlHierarchy.from_levels(H.A, H.P, H.R, H.F, csweeps=2)
+    x = np.random.default_rng(5).standard_normal(a.shape[0])
+    assert np.array_equal(H.apply(x), H2.apply(x))
