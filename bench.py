@@ -184,10 +184,16 @@ def ml_cycle_bytes(levels, k, sweeps=1, csweeps=2, eb=8):
     return tot
 
 
+ML_LEVELS = 0  # --ml-levels
+
+
 def ml_setup(config, kind, params, method):
     """(cycle parameters, factor parameters, Krylov method) of the
     multilevel leg of a configuration."""
     mp, fp, meth = ML_CONFIGS[config]
+    mp = dict(mp)
+    if ML_LEVELS:
+        mp["N_PREC_LEVS"] = ML_LEVELS
     return mp, dict(fp if fp is not None else params), meth or method
 
 
@@ -316,7 +322,8 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
                             "t_upload")},
         "fp64_single_rhs": {"solves_per_s": rate1, "iterations": its,
                             "true_residual": true1,
-                            "vs_one_level_fp64_single": rate1 / fp64_rate,
+                            "vs_one_level_fp64_single":
+                                rate1 / fp64_rate if fp64_rate else None,
                             "one_level_iterations": it64},
         "served": {"solves_per_s": rate, "factor_prec": args.prec,
                    "slots": slots, "iterations_mean": float(np.mean(iters)),
@@ -324,7 +331,8 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
                    "latency_ms_p50": float(np.percentile(lat, 50)),
                    "latency_ms_p99": float(np.percentile(lat, 99)),
                    "true_residual_max": true,
-                   "vs_one_level_fp64_single": rate / fp64_rate},
+                   "vs_one_level_fp64_single":
+                       rate / fp64_rate if fp64_rate else None},
         "cycle_k": slots, "cycle": cyc,
         "cpu_baseline": cpu,
     }
@@ -355,6 +363,10 @@ def main():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-ml", action="store_true",
                    help="skip the multilevel leg (C1, C2, C3)")
+    p.add_argument("--ml-levels", type=int, default=0,
+                   help="levels of the multilevel leg (0: the configuration's)")
+    p.add_argument("--ml-only", action="store_true",
+                   help="the multilevel leg alone (no one-level solves)")
     p.add_argument("--ml-cycle", action="store_true",
                    help="time the multilevel cycle alone (C1, C2): build, "
                         "tune the mappings with a short eager solve, then "
@@ -376,6 +388,8 @@ def main():
               "apply) is not built", file=sys.stderr)
         sys.exit(2)
 
+    global ML_LEVELS
+    ML_LEVELS = args.ml_levels
     import importlib
     m = importlib.import_module("mld-ainv_amd")
     m.init(0)
@@ -444,6 +458,14 @@ def main():
         dB = m.DeviceArray.from_numpy(Bn)
         del Bn
     log(f"{M} right-hand sides on the device ({time.time() - t0:.1f}s)")
+
+    if args.ml_only:
+        Bh = dB.to_numpy()
+        sample = list(range(1, min(args.single, M - 1) + 1))
+        print(json.dumps({"config": args.config, "multilevel": ml_leg(
+            m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
+            M, None, None)}))
+        return
 
     sv = new_solver("F64")
     t0 = time.time()
