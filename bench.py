@@ -228,6 +228,8 @@ def ml_cycle_only(m, args, a, A, kind, params, method, slots):
     byt = ml_cycle_bytes(lv, slots, 1, mp["COARSE_SWEEPS"],
                          4 if x32 else 8)
     print(json.dumps({"ml_cycle": True, "config": args.config, "k": slots,
+                      "maps": [e for e in pv.report()["multilevel"]["maps"]
+                               if e["K"] == slots],
                       "factor_prec": args.prec,
                       "vectors": "FP32" if x32 else "FP64",
                       "build_s": t_build, "ms": ms, "bytes": byt,

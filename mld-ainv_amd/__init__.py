@@ -373,7 +373,7 @@ class _BaseAinvSolver:
 
     def report(self):
         import json
-        buf = C.create_string_buffer(8192)
+        buf = C.create_string_buffer(65536)
         _ck(lib().ainv_report(self._h, buf, len(buf)))
         return json.loads(buf.value.decode())
 

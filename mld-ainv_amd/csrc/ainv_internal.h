@@ -89,8 +89,12 @@ struct Hll {
   DevBuf rlen;  // int32 [n]
   DevBuf col;   // int32 [padded]
   DevBuf val;   // VT    [padded]
+  // optional CSR copy (row pointers, columns, VT values) for the
+  // one-wavefront-per-row mapping AINV_MAP_WIDE (long rows on few rows)
+  DevBuf cptr, cidx, cval;
   int64_t bytes() const {
-    return (int64_t)(sptr.bytes + rlen.bytes + col.bytes + val.bytes);
+    return (int64_t)(sptr.bytes + rlen.bytes + col.bytes + val.bytes +
+                     cptr.bytes + cidx.bytes + cval.bytes);
   }
 };
 
@@ -116,6 +120,9 @@ void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s);
 // for a square one.
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
                   int32_t ncols = -1);
+// Attaches the CSR copy of `a` (host, 0-based, rows sorted) to h, values in
+// h's precision, for AINV_MAP_WIDE.
+void hll_attach_csr(const HostCsr &a, Hll &h, hipStream_t s);
 // Plain 16-byte-vector copy kernel (bandwidth reference).
 void stream_copy(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Convert n doubles to the storage type of prec (d vector).
@@ -141,7 +148,12 @@ constexpr int AINV_MAP_COOP128 = 3;
 // COOP / COOP128)
 constexpr int AINV_MAP_COOP_Q = 4;
 constexpr int AINV_MAP_COOP128_Q = 5;
-constexpr int kNumMaps = 6;
+// one wavefront per row over the matrix's CSR copy (Hll::cptr), for the
+// multilevel products whose rows are long and few
+constexpr int AINV_MAP_WIDE = 6;
+constexpr int kNumMaps = 7;
+// mappings of the factor applies (no CSR copy)
+constexpr int kNumApplyMaps = 6;
 // mappings the fused system-matrix product distinguishes
 constexpr int kNumSpmvMaps = 4;
 // The two apply passes may use different mappings: a packed pair.

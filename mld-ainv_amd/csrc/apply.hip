@@ -934,8 +934,93 @@ void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
   AINV_HIP_CHECK(hipGetLastError());
 }
 
+// One wavefront per row over the CSR copy (AINV_MAP_WIDE), for matrices of
+// few, long rows (the small levels of a multilevel hierarchy): lane (g, c)
+// of the G = 64 / K groups of K lanes takes column c of entries g, g + G,
+// ... of the row; the wave loads 64 (column, value) words at a time,
+// coalesced, and hands them out by shuffles (uniform trip counts: the row
+// is the wave's); each group's K lanes gather one contiguous x row.  The
+// groups' sums are combined by xor shuffles in a fixed order.  Rows of
+// ld = K, x and out with ld = K.
+template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
+__global__ __launch_bounds__(kBlock) void k_csr_wide(
+    int32_t n, const int64_t *__restrict__ ptr,
+    const int32_t *__restrict__ idx, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ kwp) {
+  using A = typename Val<VT>::acc;
+  constexpr int G = kSlice / K;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int c = lane % K, g = lane / K;
+  const int32_t r = __builtin_amdgcn_readfirstlane(
+      (int32_t)blockIdx.x * kWavesPerBlock + (int32_t)(threadIdx.x >> 6));
+  if (r >= n) return;
+  const int kw = kwp ? min(k, *kwp) : k;
+  const bool live = c < kw;
+  const int64_t beg = ptr[r], end = ptr[r + 1];
+  A acc = 0;
+  for (int64_t j0 = beg; j0 < end; j0 += kSlice) {
+    const int64_t jl = j0 + lane;
+    const int32_t cl = jl < end ? idx[jl] : 0;
+    const VT vl = jl < end ? val[jl] : VT(0);
+    const int cnt = (int)min((int64_t)kSlice, end - j0);
+    for (int t = 0; t < cnt; t += G) {
+      const int te = t + g;  // < 64: t < 64 and g < G
+      const int32_t cc = __shfl(cl, te);
+      const A v = Val<VT>::get(shfl_raw(vl, te));
+      const A xv = cvt<A>(x[(int64_t)cc * K + (live ? c : 0)]);
+      if (te < cnt) acc += v * xv;
+    }
+  }
+#pragma unroll
+  for (int o = K; o < kSlice; o <<= 1) acc += __shfl_xor(acc, o);
+  if (g != 0 || !live) return;
+  OT *o = out + (int64_t)r * K + c;
+  if constexpr (EPI == EPI_SCALE) {
+    *o = (OT)((double)dvec[r] * (double)acc);
+  } else if constexpr (EPI == EPI_RESID) {
+    *o = (OT)(alpha * (double)acc + beta * (double)dvec[(int64_t)r * K + c]);
+  } else {
+    double v = alpha * (double)acc;
+    if (beta != 0.0) v += beta * (double)(*o);
+    *o = (OT)v;
+  }
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int EPI>
+void launch_wide(const Hll &a, const XT *x, int k, const DT *d, OT *out,
+                 double alpha, double beta, hipStream_t s,
+                 const int32_t *kw) {
+  if (!a.cptr.p)
+    throw Error(AINV_ERR_INTERNAL, "the wide mapping needs the CSR copy");
+  if (a.n == 0) return;
+  const dim3 grid((unsigned)((a.n + kWavesPerBlock - 1) / kWavesPerBlock));
+  const int64_t *cp = a.cptr.as<int64_t>();
+  const int32_t *ci = a.cidx.as<int32_t>();
+  const VT *cv = a.cval.as<VT>();
+#define AINV_LAUNCH(KK)                                                     \
+  hipLaunchKernelGGL((k_csr_wide<VT, XT, DT, OT, KK, EPI>), grid,           \
+                     dim3(kBlock), 0, s, a.n, cp, ci, cv, x, k, d, out,     \
+                     alpha, beta, kw)
+  if (k <= 2)
+    AINV_LAUNCH(2);
+  else if (k <= 4)
+    AINV_LAUNCH(4);
+  else if (k <= 8)
+    AINV_LAUNCH(8);
+  else if (k <= 16)
+    AINV_LAUNCH(16);
+  else if (k <= 32)
+    AINV_LAUNCH(32);
+  else
+    AINV_LAUNCH(64);
+#undef AINV_LAUNCH
+  AINV_HIP_CHECK(hipGetLastError());
+}
+
 // kw (device, may be NULL): active width of the block, honoured by the
-// cooperative mappings (the others compute every column).
+// cooperative and wide mappings (the others compute every column).
 template <typename VT, typename XT, typename DT, typename OT, int EPI>
 void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
                  OT *out, double alpha, double beta, hipStream_t s,
@@ -950,6 +1035,8 @@ void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
   else if (map == AINV_MAP_COOP128_Q)
     launch_coop<VT, XT, DT, OT, 128, EPI, 4>(a, x, k, d, out, alpha, beta, s,
                                              kw);
+  else if (map == AINV_MAP_WIDE)
+    launch_wide<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s, kw);
   else
     launch_row<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s);
 }
@@ -967,9 +1054,10 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
                   hipStream_t s, const int32_t *kw) {
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
-  // the fused product + dots has no quarter-slice variant
+  // the fused product + dots has no quarter-slice or wide variant
   if (map == AINV_MAP_COOP_Q) map = AINV_MAP_COOP;
-  if (map == AINV_MAP_COOP128_Q) map = AINV_MAP_COOP128;
+  if (map == AINV_MAP_COOP128_Q || map == AINV_MAP_WIDE)
+    map = AINV_MAP_COOP128;
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
   const int g = persistent_grid(a.nslices);
   const int64_t *sp = a.sptr.as<int64_t>();
@@ -1085,9 +1173,10 @@ static bool is_coop(int map) {
 
 static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,
                     const void *o) {
-  // row: k <= 32; coop: k <= 64 (column chunks)
-  if ((map != AINV_MAP_ROW && !is_coop(map)) || k < 2 ||
-      k > (is_coop(map) ? 64 : 32))
+  // row: k <= 32; coop and wide: k <= 64 (column chunks)
+  const bool wide = map == AINV_MAP_WIDE;
+  if ((map != AINV_MAP_ROW && !is_coop(map) && !wide) || k < 2 ||
+      k > (is_coop(map) || wide ? 64 : 32))
     return false;
   int K = 2;
   while (K < k) K <<= 1;

@@ -347,9 +347,11 @@ int map_from_string(const std::string &v) {
   if (v == "COOP128") return AINV_MAP_COOP128;
   if (v == "COOP_Q") return AINV_MAP_COOP_Q;
   if (v == "COOP128_Q") return AINV_MAP_COOP128_Q;
+  if (v == "WIDE") return AINV_MAP_WIDE;
   throw Error(AINV_ERR_ARG,
               "unknown APPLY_MAP '" + v +
-                  "' (AUTO, SPLIT, ROW, COOP, COOP128, COOP_Q, COOP128_Q)");
+                  "' (AUTO, SPLIT, ROW, COOP, COOP128, COOP_Q, COOP128_Q, "
+                  "WIDE)");
 }
 
 const char *map_name(int m) {
@@ -358,14 +360,17 @@ const char *map_name(int m) {
          : m == AINV_MAP_COOP128 ? "coop128"
          : m == AINV_MAP_COOP_Q    ? "coop_q"
          : m == AINV_MAP_COOP128_Q ? "coop128_q"
+         : m == AINV_MAP_WIDE      ? "wide"
                                  : "split";
 }
 
 void set_map_mode(ainv_solver *h, int mode) {
-  if (mode < -1 || mode >= kNumMaps)
-    throw Error(AINV_ERR_ARG, "APPLY_MAP out of range");
+  const bool wide_ok = h->kind == AINV_KIND_ML && mode == AINV_MAP_WIDE;
+  if (mode < -1 || (mode >= kNumApplyMaps && !wide_ok))
+    throw Error(AINV_ERR_ARG, "APPLY_MAP out of range (WIDE: multilevel only)");
   if (mode == h->map_mode) return;
   h->map_mode = mode;
+  if (h->kind == AINV_KIND_ML && h->built) ml_set_map_mode(h, mode);
   // cached Krylov graphs hold the old kernels
   if (h->built) h->generation = next_generation();
 }
@@ -1182,7 +1187,7 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
                         "{\"K\": " + std::to_string(1 << i);
         for (int pass = 0; pass < 2; ++pass) {
           e += pass ? ", \"z_ms\": {" : ", \"w_ms\": {";
-          for (int m = 0; m < kNumMaps; ++m) {
+          for (int m = 0; m < kNumApplyMaps; ++m) {
             char b2[64];
             std::snprintf(b2, sizeof b2, "%s\"%s\": %.6f", m ? ", " : "",
                           map_name(m), h->apply_ms[i][pass][m]);

@@ -229,6 +229,33 @@ __global__ void k_convert(int64_t n, const double *__restrict__ src,
 
 }  // namespace
 
+void hll_attach_csr(const HostCsr &a, Hll &h, hipStream_t s) {
+  if (a.n != h.n) throw Error(AINV_ERR_INTERNAL, "CSR copy of another matrix");
+  if (h.prec == AINV_PREC_BF16)
+    throw Error(AINV_ERR_INTERNAL, "CSR copy of a BF16 matrix");
+  const int64_t nnz = a.nnz();
+  h.cptr.alloc((size_t)(a.n + 1) * sizeof(int64_t));
+  h.cidx.alloc((size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t));
+  const int vb = value_bytes(h.prec);
+  h.cval.alloc((size_t)std::max<int64_t>(nnz, 1) * vb);
+  AINV_HIP_CHECK(hipMemcpyAsync(h.cptr.p, a.ptr.data(), h.cptr.bytes,
+                                hipMemcpyHostToDevice, s));
+  if (nnz) {
+    AINV_HIP_CHECK(hipMemcpyAsync(h.cidx.p, a.idx.data(),
+                                  (size_t)nnz * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, s));
+    DevBuf v64;
+    v64.alloc((size_t)nnz * sizeof(double));
+    AINV_HIP_CHECK(hipMemcpyAsync(v64.p, a.val.data(), v64.bytes,
+                                  hipMemcpyHostToDevice, s));
+    DevBuf conv;
+    convert_vector(v64.as<double>(), nnz, h.prec, conv, s);
+    AINV_HIP_CHECK(hipMemcpyAsync(h.cval.p, conv.p, (size_t)nnz * vb,
+                                  hipMemcpyDeviceToDevice, s));
+    AINV_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
                   int32_t ncols) {
   if (ncols < 0) ncols = a.n;

@@ -732,7 +732,7 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
       // the two passes are timed separately; t keeps leading dimension K
       const bool f32t = m->prec != AINV_PREC_F64;
       int bw = 0, bz = 0;
-      for (int map = 0; map < kNumMaps; ++map) {
+      for (int map = 0; map < kNumApplyMaps; ++map) {
         m->apply_ms[li][0][map] = time_ms(s, [&] {
           hll_spmm_scale(m->hw, m->dv.p, v.b, false, K, p->tbuf.p, K, p->k, s,
                          map);

@@ -270,16 +270,24 @@ void upload_hll(const HostCsr &m, int32_t ncols, int prec, Hll &h,
   hll_from_csr(d, prec, h, s, ncols);
 }
 
+// A matrix whose rows average 16 entries or more also gets its CSR copy,
+// for the one-wavefront-per-row mapping.
+void upload_hll_wide(const HostCsr &m, int32_t ncols, int prec, Hll &h,
+                     hipStream_t s) {
+  upload_hll(m, ncols, prec, h, s);
+  if (m.n > 0 && m.nnz() >= 16 * (int64_t)m.n) hll_attach_csr(m, h, s);
+}
+
 // A, P and R of every level in FP64 (FP64 factors) or FP32 (FP32 / BF16
 // factors: the cycle may then run on FP32 vectors)
 void upload_levels(MlHier &H, int factor_prec, hipStream_t s) {
   const int mp = factor_prec == AINV_PREC_F64 ? AINV_PREC_F64 : AINV_PREC_F32;
   for (auto &Lp : H.lv) {
     MlLevel &L = *Lp;
-    upload_hll(L.a, L.n, mp, L.ha, s);
+    upload_hll_wide(L.a, L.n, mp, L.ha, s);
     if (L.nc > 0) {
-      upload_hll(L.p, L.nc, mp, L.hp, s);
-      upload_hll(L.r, L.n, mp, L.hr, s);
+      upload_hll_wide(L.p, L.nc, mp, L.hp, s);
+      upload_hll_wide(L.r, L.n, mp, L.hr, s);
     }
     for (int i = 0; i < 8; ++i) L.map_a[i] = L.map_p[i] = L.map_r[i] = -1;
   }
@@ -320,10 +328,14 @@ template <typename V> Ws<V> carve(const MlHier &H, int64_t K, void *base) {
   return w;
 }
 
-int pick(const ainv_solver *h, const int8_t *tuned, int K) {
+// mapping of a level product: forced (APPLY_MAP; WIDE where the matrix has
+// its CSR copy, COOP elsewhere), else tuned, else split
+int pick(const ainv_solver *h, const Hll &m, const int8_t *tuned, int K) {
+  if (h->map_mode == AINV_MAP_WIDE)
+    return m.cptr.p ? AINV_MAP_WIDE : AINV_MAP_COOP;
   if (h->map_mode >= 0) return h->map_mode;
-  const int m = tuned[log2_pow2((int)K)];
-  return m < 0 ? AINV_MAP_SPLIT : m;
+  const int t = tuned[log2_pow2((int)K)];
+  return t < 0 ? AINV_MAP_SPLIT : t;
 }
 
 // One cycle on vectors of type V (double, or float with FP32 / BF16 factors
@@ -351,12 +363,14 @@ template <typename V> struct Cycle {
   // y = M x + beta y for M = R_l or P_l
   void prod(const Hll &m, const int8_t *maps, const V *x, V *y,
             double beta) const {
-    hll_spmm_axpby(m, 1.0, x, kF32, K, beta, y, K, k, s, pick(h, maps, K), kw);
+    hll_spmm_axpby(m, 1.0, x, kF32, K, beta, y, K, k, s, pick(h, m, maps, K),
+                   kw);
   }
   // t = b - A_l u
   void resid(int l, const V *b, const V *u, V *t) const {
     const MlLevel &L = *H->lv[l];
-    hll_spmm_resid(L.ha, -1.0, u, 1.0, b, t, K, k, s, pick(h, L.map_a, K), kw);
+    hll_spmm_resid(L.ha, -1.0, u, 1.0, b, t, K, k, s,
+                   pick(h, L.ha, L.map_a, K), kw);
   }
   // u = M_l b, then nsw - 1 sweeps u += M_l (b - A_l u)
   void smooth0(int l, int nsw, const V *b, V *u) const {
@@ -399,6 +413,11 @@ template <typename V> struct Cycle {
   }
 };
 
+// the factors have no CSR copy: a forced WIDE runs them cooperatively
+int smoother_map(int mode) {
+  return mode == AINV_MAP_WIDE ? AINV_MAP_COOP : mode;
+}
+
 const MlHier &hier(const ainv_solver *h) {
   if (h->kind != AINV_KIND_ML || !h->ml)
     throw Error(AINV_ERR_STATE, "multilevel preconditioner not built");
@@ -421,10 +440,12 @@ double time_ms(hipStream_t s, const std::function<void()> &f, int reps) {
   return (double)ms;
 }
 
-int best_map(hipStream_t s, const std::function<void(int)> &f) {
+// nmaps: kNumApplyMaps, or kNumMaps for a matrix with its CSR copy
+int best_map(hipStream_t s, const std::function<void(int)> &f,
+             int nmaps = kNumApplyMaps) {
   int best = 0;
   double bt = 0.0;
-  for (int map = 0; map < kNumMaps; ++map) {
+  for (int map = 0; map < nmaps; ++map) {
     const double t = time_ms(s, [&] { f(map); }, 3);
     if (map == 0 || t < bt) {
       bt = t;
@@ -515,7 +536,7 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
     sv->thresh = h->thresh;
     sv->inv_thresh = h->inv_thresh;
     sv->prec = h->prec;
-    sv->map_mode = h->map_mode;
+    sv->map_mode = smoother_map(h->map_mode);
     sv->host_threads = h->host_threads;
     sv->device = h->device;
     sv->stream = h->stream;
@@ -614,19 +635,20 @@ void tune_levels(ainv_solver *h, int K, int k, const V *x0, V *y0, void *ws,
     // x is only read, y and the workspace only written
     const V *x = l > 0 ? w.b[l] : x0;
     V *y = l > 0 ? w.u[l] : y0;
+    auto nm = [](const Hll &h) { return h.cptr.p ? kNumMaps : kNumApplyMaps; };
     if (L.map_a[li] < 0)
       L.map_a[li] = (int8_t)best_map(s, [&](int map) {
         hll_spmm_resid(L.ha, -1.0, y, 1.0, x, w.t[l], K, k, s, map);
-      });
+      }, nm(L.ha));
     if (L.nc > 0) {
       if (L.map_r[li] < 0)
         L.map_r[li] = (int8_t)best_map(s, [&](int map) {
           hll_spmm_axpby(L.hr, 1.0, x, f32, K, 0.0, w.b[l + 1], K, k, s, map);
-        });
+        }, nm(L.hr));
       if (L.map_p[li] < 0)
         L.map_p[li] = (int8_t)best_map(s, [&](int map) {
           hll_spmm_axpby(L.hp, 1.0, w.u[l + 1], f32, K, 0.0, y, K, k, s, map);
-        });
+        }, nm(L.hp));
     }
     ainv_solver *sv = L.sv.get();
     if (sv->apply_map[li] < 0) {
@@ -663,6 +685,15 @@ const ainv_solver *ml_level_solver(const ainv_solver *h, int l) {
   if (l < 0 || l >= (int)H.lv.size())
     throw Error(AINV_ERR_ARG, "level out of range");
   return H.lv[l]->sv.get();
+}
+
+void ml_set_map_mode(ainv_solver *h, int mode) {
+  if (!h->ml) return;
+  for (auto &L : h->ml->lv) {
+    L->sv->map_mode = smoother_map(mode);
+    reset_tuning(L->sv.get());
+    for (int i = 0; i < 8; ++i) L->map_a[i] = L->map_p[i] = L->map_r[i] = -1;
+  }
 }
 
 void ml_remold(ainv_solver *h, int prec) {
@@ -711,6 +742,26 @@ std::string ml_report(const ainv_solver *h) {
                   L.sv->t_factor, L.sv->t_inverse);
     s += buf;
   }
+  // tuned mappings of the level products per block width
+  s += "], \"maps\": [";
+  static const char *names[kNumMaps] = {"split",  "row",       "coop", "coop128",
+                                        "coop_q", "coop128_q", "wide"};
+  auto nm = [&](int8_t m) { return m < 0 ? "-" : names[m]; };
+  bool first = true;
+  for (int i = 0; i < 8; ++i)
+    for (size_t l = 0; l < H.lv.size(); ++l) {
+      const MlLevel &L = *H.lv[l];
+      if (L.map_a[i] < 0 && L.map_p[i] < 0 && L.map_r[i] < 0) continue;
+      std::snprintf(buf, sizeof buf,
+                    "%s{\"K\": %d, \"level\": %zu, \"a\": \"%s\", "
+                    "\"p\": \"%s\", \"r\": \"%s\", \"wide\": %s}",
+                    first ? "" : ", ", 1 << i, l, nm(L.map_a[i]),
+                    nm(L.map_p[i]), nm(L.map_r[i]),
+                    L.ha.cptr.p || L.hp.cptr.p || L.hr.cptr.p ? "true"
+                                                              : "false");
+      s += buf;
+      first = false;
+    }
   std::snprintf(buf, sizeof buf,
                 "], \"t_aggregation\": %.4f, \"t_galerkin\": %.4f, "
                 "\"t_factors\": %.4f, \"t_upload\": %.4f}",

@@ -163,6 +163,8 @@ void ml_host_level(const HostCsr &a, double theta, bool smoothed, int threads,
                    HostCsr &ac, double &omega);
 // Re-molds the factors of every level to another storage precision.
 void ml_remold(ainv_solver *h, int prec);
+// A new APPLY_MAP for every level (tuning forgotten).
+void ml_set_map_mode(ainv_solver *h, int mode);
 // Storage precision change of a built solver's factors (capi.cpp).
 void solver_remold(ainv_solver *h, int prec);
 int ml_levels(const ainv_solver *h);

@@ -313,3 +313,54 @@ def test_cg_on_the_fp32_cycle(gpu, O):
     for j in range(4):
         r = B[:, j] - a @ x32[:, j]
         assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(B[:, j])
+
+
+@pytest.mark.parametrize("k", [8, 32])
+def test_wide_mapping_parity(gpu, O, k):
+    """APPLY_MAP=WIDE (multilevel only) runs every level product that has a
+    CSR copy (rows of 16 entries or more on average: the anisotropic
+    problem's Galerkin levels) one wavefront per row, the rest
+    cooperatively; the cycle and the solve match the oracle."""
+    a = gpu.gen_aniso2d(64, 1e-3)
+    n = a.shape[0]
+    pv, ref = build_ml(gpu, O, a, "AINV", nlev=5, theta=0.1, csweeps=2)
+    # SYM-LLK 8 / 5e-2 (21 iterations here; LLK 6 needs ~150, too erratic a
+    # convergence to compare counts)
+    sub = {"kind": "AINV", "alg": "SYM-LLK", "fill_in": 8, "thresh": 5e-2}
+    pv.set("AINV_ALG", "SYM-LLK")
+    pv.set("SUB_FILLIN", 8)
+    pv.build(a)
+    ref = O.MlHierarchy(a, nlev=5, theta=0.1, sub=sub, csweeps=2)
+    pv.set("APPLY_MAP", "WIDE")
+    assert pv.geti("APPLY_MAP") == 6
+    X = gpu.gen_normal(23, (n, k))
+    Y = pv.apply(X)
+    for j in range(0, k, 3):
+        yr = ref.apply(X[:, j])
+        assert np.linalg.norm(Y[:, j] - yr) <= 1e-11 * np.linalg.norm(yr), j
+    assert any(L["n"] > 0 for L in pv.report()["multilevel"]["levels"])
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(24, (n, k))
+    x, it, err = gpu.krylov("BICGSTAB", A, pv, B, eps=1e-8, itmax=300)
+    for j in range(0, k, 5):
+        _, itr, _ = O.ml_krylov("BICGSTAB", a, ref, B[:, j], 1e-8, 300)
+        assert abs(int(it[j]) - itr) <= 1, (j, it[j], itr)
+    # the one-level kinds refuse it (their factors have no CSR copy)
+    sv = gpu.MldDInvkSolver()
+    with pytest.raises(gpu.AinvError):
+        sv.set("APPLY_MAP", "WIDE")
+
+
+def test_tuner_report_lists_level_mappings(gpu, O):
+    a = gpu.gen_aniso2d(64, 1e-3)
+    n = a.shape[0]
+    pv, _ = build_ml(gpu, O, a, "AINV", nlev=5, theta=0.1, csweeps=2)
+    A = gpu.SystemMatrix(a)
+    gpu.krylov("BICGSTAB", A, pv, gpu.gen_normal(25, (n, 16)), eps=1e-6,
+               itmax=50)
+    maps = [e for e in pv.report()["multilevel"]["maps"] if e["K"] == 16]
+    assert len(maps) == pv.levels()
+    names = {"split", "row", "coop", "coop128", "coop_q", "coop128_q", "wide"}
+    for e in maps:
+        assert e["a"] in names
+    assert any(e["wide"] for e in maps)
