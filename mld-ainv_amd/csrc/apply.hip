@@ -1003,7 +1003,9 @@ void launch_wide(const Hll &a, const XT *x, int k, const DT *d, OT *out,
   hipLaunchKernelGGL((k_csr_wide<VT, XT, DT, OT, KK, EPI>), grid,           \
                      dim3(kBlock), 0, s, a.n, cp, ci, cv, x, k, d, out,     \
                      alpha, beta, kw)
-  if (k <= 2)
+  if (k <= 1)
+    AINV_LAUNCH(1);
+  else if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)
     AINV_LAUNCH(4);
@@ -1173,12 +1175,12 @@ static bool is_coop(int map) {
 
 static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,
                     const void *o) {
-  // row: k <= 32; coop and wide: k <= 64 (column chunks)
+  // row: k <= 32; coop and wide: k <= 64 (column chunks); wide also k = 1
   const bool wide = map == AINV_MAP_WIDE;
-  if ((map != AINV_MAP_ROW && !is_coop(map) && !wide) || k < 2 ||
+  if ((map != AINV_MAP_ROW && !is_coop(map) && !wide) || k < (wide ? 1 : 2) ||
       k > (is_coop(map) || wide ? 64 : 32))
     return false;
-  int K = 2;
+  int K = wide && k == 1 ? 1 : 2;
   while (K < k) K <<= 1;
   return ldx == K && ldo == K && aligned16(x) && aligned16(o);
 }

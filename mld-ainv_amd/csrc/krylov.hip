@@ -710,6 +710,17 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   if (K == 1 || p->k < 2) {
     p->apply_map = map_pair(AINV_MAP_SPLIT, AINV_MAP_SPLIT);
     p->spmv_map = AINV_MAP_SPLIT;
+    // one column: the multilevel products may still run one wavefront per
+    // long row
+    if (K == 1 && m && m->kind == AINV_KIND_ML) {
+      const Vecs w = x32_vecs(p);
+      if (w.p32)
+        ml_tune(m, 1, p->k, w.p32, w.f32, true, p->tbuf.p, s);
+      else if (w.r32)
+        ml_tune(m, 1, p->k, w.r32, w.z32, true, p->tbuf.p, s);
+      else
+        ml_tune(m, 1, p->k, p->v.b, p->v.z, false, p->tbuf.p, s);
+    }
     return;
   }
   if (forced >= 0) {

@@ -665,9 +665,55 @@ void tune_levels(ainv_solver *h, int K, int k, const V *x0, V *y0, void *ws,
   AINV_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// One column: split or, for the products with a CSR copy, wide.
+template <typename V>
+void tune_levels_1(ainv_solver *h, const V *x0, V *y0, void *ws,
+                   hipStream_t s) {
+  constexpr bool f32 = sizeof(V) == sizeof(float);
+  MlHier &H = *h->ml;
+  AINV_HIP_CHECK(hipMemsetAsync(ws, 0, ml_ws_bytes(h, 1), s));
+  const Ws<V> w = carve<V>(H, 1, ws);
+  auto two = [&](const Hll &m, const std::function<void(int)> &f) {
+    if (!m.cptr.p) return (int8_t)AINV_MAP_SPLIT;
+    const double ts = time_ms(s, [&] { f(AINV_MAP_SPLIT); }, 3);
+    const double tw = time_ms(s, [&] { f(AINV_MAP_WIDE); }, 3);
+    return (int8_t)(tw < ts ? AINV_MAP_WIDE : AINV_MAP_SPLIT);
+  };
+  for (size_t l = 0; l < H.lv.size(); ++l) {
+    MlLevel &L = *H.lv[l];
+    const V *x = l > 0 ? w.b[l] : x0;
+    V *y = l > 0 ? w.u[l] : y0;
+    if (L.map_a[0] < 0)
+      L.map_a[0] = two(L.ha, [&](int map) {
+        hll_spmm_resid(L.ha, -1.0, y, 1.0, x, w.t[l], 1, 1, s, map);
+      });
+    if (L.nc > 0) {
+      if (L.map_r[0] < 0)
+        L.map_r[0] = two(L.hr, [&](int map) {
+          hll_spmm_axpby(L.hr, 1.0, x, f32, 1, 0.0, w.b[l + 1], 1, 1, s, map);
+        });
+      if (L.map_p[0] < 0)
+        L.map_p[0] = two(L.hp, [&](int map) {
+          hll_spmm_axpby(L.hp, 1.0, w.u[l + 1], f32, 1, 0.0, y, 1, 1, s, map);
+        });
+    }
+  }
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
 void ml_tune(ainv_solver *h, int K, int k, const void *x0, void *y0,
              bool x32, void *ws, hipStream_t s) {
-  if (K == 1 || k < 2 || h->map_mode >= 0) return;
+  if (h->map_mode >= 0) return;
+  if (K == 1) {
+    if (x32)
+      tune_levels_1<float>(h, static_cast<const float *>(x0),
+                           static_cast<float *>(y0), ws, s);
+    else
+      tune_levels_1<double>(h, static_cast<const double *>(x0),
+                            static_cast<double *>(y0), ws, s);
+    return;
+  }
+  if (k < 2) return;
   if (x32)
     tune_levels<float>(h, K, k, static_cast<const float *>(x0),
                        static_cast<float *>(y0), ws, s);

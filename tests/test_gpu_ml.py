@@ -315,7 +315,7 @@ def test_cg_on_the_fp32_cycle(gpu, O):
         assert np.linalg.norm(r) <= 1e-6 * np.linalg.norm(B[:, j])
 
 
-@pytest.mark.parametrize("k", [8, 32])
+@pytest.mark.parametrize("k", [1, 8, 32])
 def test_wide_mapping_parity(gpu, O, k):
     """APPLY_MAP=WIDE (multilevel only) runs every level product that has a
     CSR copy (rows of 16 entries or more on average: the anisotropic
