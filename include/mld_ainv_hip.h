@@ -127,7 +127,8 @@ int ainv_get_kind(ainv_h h, int *kind);
  * -1/0/1/2/3/4/5: kernel mapping of block applies with a power-of-two
  * leading dimension, the _Q ones with four waves per 64-row slice; AUTO
  * times each for each of the two passes once per block width and keeps the
- * fastest).  Unknown keys are passed to the base solver, which
+ * fastest; multilevel handles also take 'WIDE' or 6: one wavefront per row
+ * for the level products that keep a CSR copy, COOP for the rest).  Unknown keys are passed to the base solver, which
  * ignores them (info = 0). */
 int ainv_cseti(ainv_h h, const char *what, int val);
 int ainv_csetr(ainv_h h, const char *what, double val);
