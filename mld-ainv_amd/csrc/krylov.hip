@@ -2158,6 +2158,7 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
   std::vector<double> t_adm(K, 0.0);
   std::vector<SlotState> st;
   int next = 0, done = 0;
+  int64_t done_iters = 0;
   try {
     while (done < nrhs) {
       // fill every free slot (up to the requested count) from the queue
@@ -2174,7 +2175,12 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
       }
       batch_admit_cols(bt, adm, src, eps);
       batch_step(bt);
-      const int64_t seen = batch_poll(bt, false, st);
+      // Reading the states one step behind keeps the device busy but leaves
+      // a converged column's slot idle for one step: for solves of few
+      // iterations (a multilevel preconditioner) that step costs more than
+      // the host's turnaround, so wait for the step just launched.
+      const bool short_solves = done >= 8 && done_iters < 24 * (int64_t)done;
+      const int64_t seen = batch_poll(bt, short_solves, st);
       if (seen < 0) continue;
       const double t = now_seconds();
       for (int c = 0; c < K; ++c) {
@@ -2187,6 +2193,7 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
         if (lat_ms) lat_ms[j] = (t - t_adm[c]) * 1e3;
         owner[c] = -1;
         ++done;
+        done_iters += st[c].iter;
       }
       // queue run dry and at most half the slots busy: continue narrower
       if (next >= nrhs && K > 1) {
