@@ -701,9 +701,21 @@ void tune_levels_1(ainv_solver *h, const V *x0, V *y0, void *ws,
   AINV_HIP_CHECK(hipStreamSynchronize(s));
 }
 
+// every product of every level has a mapping for block width index li
+// (the one-column tuning sets only the level products)
+static bool all_tuned(const MlHier &H, int li, bool with_smoothers) {
+  for (const auto &L : H.lv) {
+    if (L->map_a[li] < 0) return false;
+    if (L->nc > 0 && (L->map_p[li] < 0 || L->map_r[li] < 0)) return false;
+    if (with_smoothers && L->sv->apply_map[li] < 0) return false;
+  }
+  return true;
+}
+
 void ml_tune(ainv_solver *h, int K, int k, const void *x0, void *y0,
              bool x32, void *ws, hipStream_t s) {
   if (h->map_mode >= 0) return;
+  if (all_tuned(*h->ml, log2_pow2(K), K > 1)) return;
   if (K == 1) {
     if (x32)
       tune_levels_1<float>(h, static_cast<const float *>(x0),
