@@ -454,6 +454,22 @@ __global__ __launch_bounds__(kThreads) void k_admit_norms(
           red[0];
 }
 
+// Columns slot[j] of a K-column vector to the contiguous n-vectors dst[j],
+// one thread per (row, column) so each wave reads a few rows once.
+struct ExtractList {
+  int cnt;
+  int slot[kMaxCols];
+  double *dst[kMaxCols];
+};
+__global__ void k_extract_cols(int32_t n, int K, ExtractList L,
+                               const double *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * L.cnt) return;
+  const int64_t row = t / L.cnt;
+  const int j = (int)(t - row * L.cnt);
+  L.dst[j][row] = x[row * K + L.slot[j]];
+}
+
 // Copy columns [c0, c0+kc) of a K-column vector to dst (n x kc).
 __global__ void k_extract(int32_t n, int K, int c0, int kc,
                           const double *__restrict__ x,
@@ -1043,6 +1059,25 @@ int64_t batch_poll(SlotBatch *b, bool wait_last, std::vector<SlotState> &out) {
     out[j].valid = true;
   }
   return which;
+}
+
+void batch_extract_cols(SlotBatch *b, const std::vector<int> &slots,
+                        const std::vector<double *> &dsts) {
+  if (slots.size() != dsts.size() || slots.size() > (size_t)kMaxCols)
+    throw Error(AINV_ERR_INTERNAL, "bad extraction list");
+  if (slots.empty()) return;
+  ExtractList L{};
+  L.cnt = (int)slots.size();
+  for (int j = 0; j < L.cnt; ++j) {
+    if (slots[j] < 0 || slots[j] >= b->K)
+      throw Error(AINV_ERR_INTERNAL, "extraction outside the slot range");
+    L.slot[j] = slots[j];
+    L.dst[j] = dsts[j];
+  }
+  const int64_t tot = (int64_t)b->p->n * L.cnt;
+  hipLaunchKernelGGL(k_extract_cols, dim3((unsigned)((tot + 255) / 256)),
+                     dim3(256), 0, b->s, b->p->n, b->K, L, b->p->v.x);
+  AINV_HIP_CHECK(hipGetLastError());
 }
 
 void batch_extract(SlotBatch *b, int c0, int kc, double *dst) {
@@ -2183,11 +2218,14 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
       const int64_t seen = batch_poll(bt, short_solves, st);
       if (seen < 0) continue;
       const double t = now_seconds();
+      std::vector<int> ext;
+      std::vector<double *> dst;
       for (int c = 0; c < K; ++c) {
         if (owner[c] < 0 || seen < first[c]) continue;
         if (st[c].active && st[c].iter < itmax) continue;
         const int j = owner[c];
-        batch_extract(bt, c, 1, x + (int64_t)j * n);
+        ext.push_back(c);
+        dst.push_back(x + (int64_t)j * n);
         iter[j] = st[c].iter;
         err[j] = st[c].err;
         if (lat_ms) lat_ms[j] = (t - t_adm[c]) * 1e3;
@@ -2195,6 +2233,10 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
         ++done;
         done_iters += st[c].iter;
       }
+      // one launch for the step's finished columns, ordered on the stream
+      // before the admissions that reuse their slots and any move to a
+      // narrower batch
+      batch_extract_cols(bt, ext, dst);
       // queue run dry and at most half the slots busy: continue narrower
       if (next >= nrhs && K > 1) {
         std::vector<int> keep;

@@ -215,6 +215,9 @@ int64_t batch_next_step(const SlotBatch *b);
 int64_t batch_poll(SlotBatch *b, bool wait_last, std::vector<SlotState> &out);
 // dst: device, n x kc interleaved
 void batch_extract(SlotBatch *b, int c0, int kc, double *dst);
+// slots[j] to the contiguous device n-vector dsts[j], one launch
+void batch_extract_cols(SlotBatch *b, const std::vector<int> &slots,
+                        const std::vector<double *> &dsts);
 
 // Continuous batching over a device-resident queue: b and x hold nrhs
 // n-vectors one after another; `slots` columns run at a time and a slot is
