@@ -525,6 +525,8 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
   }
   const int nl = (int)H->lv.size();
   double t0 = now_seconds();
+  // the levels' factors are independent: built concurrently, one host
+  // thread and one HIP stream each, the host threads split between them
   for (int l = 0; l < nl; ++l) {
     MlLevel &L = *H->lv[l];
     L.sv.reset(new ainv_solver());
@@ -537,10 +539,33 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
     sv->inv_thresh = h->inv_thresh;
     sv->prec = h->prec;
     sv->map_mode = smoother_map(h->map_mode);
-    sv->host_threads = h->host_threads;
+    sv->host_threads = std::max(1, nt / nl);
     sv->device = h->device;
-    sv->stream = h->stream;
-    build_solver(sv, L.a);
+  }
+  {
+    std::vector<std::thread> th;
+    std::exception_ptr err;
+    std::mutex emu;
+    for (int l = 0; l < nl; ++l)
+      th.emplace_back([&, l] {
+        ainv_solver *sv = H->lv[l]->sv.get();
+        try {
+          AINV_HIP_CHECK(hipSetDevice(h->device));
+          AINV_HIP_CHECK(hipStreamCreate(&sv->stream));
+          build_solver(sv, H->lv[l]->a);
+        } catch (...) {
+          std::lock_guard<std::mutex> lk(emu);
+          if (!err) err = std::current_exception();
+        }
+        // the level keeps the handle's stream from here on
+        if (sv->stream && sv->stream != h->stream) {
+          (void)hipStreamSynchronize(sv->stream);
+          (void)hipStreamDestroy(sv->stream);
+        }
+        sv->stream = h->stream;
+      });
+    for (auto &x : th) x.join();
+    if (err) std::rethrow_exception(err);
   }
   H->t_factors = now_seconds() - t0;
   t0 = now_seconds();
