@@ -580,12 +580,15 @@ template <typename XT, int K> struct Coop {
 // x rows an XCD gathers between the first and the last use of a row (the
 // matrix bandwidth times the row width) then fit in its L2, at the price of
 // reading the factor words once per chunk.
+#ifndef AINV_COOP_WIDE_WAVES
+#define AINV_COOP_WIDE_WAVES 2
+#endif
 template <typename XT, int K, int CB> struct CoopChunk {
   static constexpr int cap = CB / (int)sizeof(XT);
   static constexpr int KC = K < cap ? K : cap;
   static constexpr int NCH = K / KC;
-  // KC >= 32 accumulators per lane: 2 waves per SIMD instead of spilling
-  static constexpr int waves = KC >= 32 ? 2 : AINV_ROW_WAVES;
+  // KC >= 32 accumulators per lane: fewer waves per SIMD instead of spilling
+  static constexpr int waves = KC >= 32 ? AINV_COOP_WIDE_WAVES : AINV_ROW_WAVES;
 };
 #if AINV_ROW_WAVES > 0
 #define AINV_COOP_ATTR(XT)                                                  \
@@ -594,6 +597,16 @@ template <typename XT, int K, int CB> struct CoopChunk {
 #else
 #define AINV_COOP_ATTR(XT)
 #endif
+
+// Slice positions per trip of the cooperative loop.  For chunks of 32
+// columns one position puts 8 KB (FP32 x) of loads per wave in flight; two
+// positions per trip double that at 2 waves per SIMD, which measured 3-4 %
+// faster than one position at 2 or 4 waves (4 waves per SIMD spill, and so
+// do two positions of 64-column chunks).  The fused system-matrix product +
+// dots keeps one position per trip: two measured 5 % slower there.
+template <typename A, int KC> struct CoopGroup {
+  static constexpr int v = KC == 32 ? 2 : RowGroup<A, KC>::v;
+};
 
 template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
   if constexpr (sizeof(T) == 2) {
@@ -610,7 +623,8 @@ template <typename T> __device__ __forceinline__ T shfl_raw(T v, int src) {
 // all such lanes hit the same lines), and its sums are not stored.
 // NQ of the L row passes, from pass q0 on (a wave may share its slice with
 // other waves, each taking some of the passes).
-template <typename VT, typename XT, int K, int KC, int NQ = Coop<XT, KC>::L>
+template <typename VT, typename XT, int K, int KC, int NQ = Coop<XT, KC>::L,
+          int P = CoopGroup<typename Val<VT>::acc, KC>::v>
 __device__ __forceinline__ void coop_acc(
     int64_t p0, int32_t width, int lane, int rg, int q0,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
@@ -619,7 +633,6 @@ __device__ __forceinline__ void coop_acc(
   using A = typename Val<VT>::acc;
   using CP = Coop<XT, KC>;
   constexpr int L = NQ, CPL = CP::CPL, R = CP::R;
-  constexpr int P = RowGroup<A, KC>::v;
   const int32_t nfull = width / P * P;
   const int64_t pl = p0 + lane;
   if (nfull > 0) {
@@ -844,8 +857,8 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
     for (int q = 0; q < L; ++q)
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
-    coop_acc<double, XT, K, KC>(base, width, lane, rg, 0, col, val, x + c0,
-                                c0 < kw, acc);
+    coop_acc<double, XT, K, KC, Coop<XT, KC>::L, RowGroup<double, KC>::v>(
+        base, width, lane, rg, 0, col, val, x + c0, c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
       const int32_t r = s * kSlice + q * R + rg;
