@@ -602,8 +602,9 @@ template <typename XT, int K, int CB> struct CoopChunk {
 // columns one position puts 8 KB (FP32 x) of loads per wave in flight; two
 // positions per trip double that at 2 waves per SIMD, which measured 3-4 %
 // faster than one position at 2 or 4 waves (4 waves per SIMD spill, and so
-// do two positions of 64-column chunks).  The fused system-matrix product +
-// dots keeps one position per trip: two measured 5 % slower there.
+// do two positions of 64-column chunks; 16-column chunks ran 5-33 % slower
+// with two).  The fused system-matrix product + dots keeps one position per
+// trip: two measured 2 % slower there.
 template <typename A, int KC> struct CoopGroup {
   static constexpr int v = KC == 32 ? 2 : RowGroup<A, KC>::v;
 };
