@@ -1,0 +1,204 @@
+"""Properties that hold at any size, checked at the full sizes of the
+benchmark workloads: pde3d 160^3 with INVK(1,1) (n = 4,096,000), the 2048^2
+anisotropic Laplacian with SYM-LLK AINV (n = 4,194,304), and the random
+nonsymmetric matrix of n = 10^6 with AINV-LLK.
+
+The oracle needs minutes for these factorizations, so the bit-exact factor
+comparisons stay at the small sizes of test_gpu_parity.py.  Here:
+
+* factor structure: W (stored lower) and Z unit triangular with an exact 1.0
+  diagonal, finite nonzero d; SYM-LLK gives W = Z^T bit for bit and d > 0 on
+  an SPD matrix; INVK(1,1) on the structurally symmetric 7-point stencil
+  gives W the pattern of Z^T;
+* the FP64 apply against a host evaluation of Z diag(d) W x (scipy's CSR
+  products on the exported factors, another summation order) within 1e-12,
+  and the 'T' apply as its adjoint (y . M x = M^T y . x);
+* linearity, and a block of columns equal to the columns applied alone;
+* FP32 / BF16 factor storage within 2e-6 / 2e-2 of the FP64 apply;
+* Krylov solves whose true residuals are recomputed on the host; a solve
+  repeated gives bit-identical iterates; a right-hand side solved alone, in
+  a block and in the served mix reaches the same solution within the
+  stopping tolerance (A (x1 - x2) small against b; the solutions
+  themselves differ by up to cond(A) times that: 2.4e-5 relative seen at
+  160^3).  The +-1 iteration bar of the small sizes does not carry over:
+  the k = 1 and k = 4 plans sum in different orders, and at n = 4 x 10^6
+  BiCGSTAB's trajectory amplifies that rounding (65 against 71 iterations
+  seen for one column), so counts are held to 10 % here.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(mld, kind, a, **params):
+    sv = mld.SOLVERS[kind]()
+    for k, v in params.items():
+        sv.set(k, v)
+    sv.build(a)
+    return sv
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _same_solution(a, x1, x2, b):
+    assert np.linalg.norm(a @ (x1 - x2)) <= 2e-7 * np.linalg.norm(b)
+    assert _rel(x1, x2) <= 1e-3
+
+
+def _close_counts(i, j):
+    i, j = int(i), int(j)
+    assert abs(i - j) <= max(2, (max(i, j) + 9) // 10), (i, j)
+
+
+def _check_structure(w, z, d):
+    assert sp.triu(w, 1).nnz == 0, "W is not lower triangular"
+    assert sp.tril(z, -1).nnz == 0, "Z is not upper triangular"
+    assert np.all(w.diagonal() == 1.0) and np.all(z.diagonal() == 1.0)
+    assert np.all(np.isfinite(d)) and np.all(d != 0.0)
+
+
+def _check_apply(mld, sv, w, z, d, n, seed):
+    x = mld.gen_normal(seed, (n,))
+    y = mld.gen_normal(seed + 1, (n,))
+    mx = sv.apply(x)
+    assert _rel(mx, z @ (d * (w @ x))) < 1e-12
+    mty = sv.apply(y, trans="T")
+    assert _rel(mty, w.T @ (d * (z.T @ y))) < 1e-12
+    lhs, rhs = float(y @ mx), float(mty @ x)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(y) * np.linalg.norm(mx)
+    my = sv.apply(y)
+    assert _rel(sv.apply(2.0 * x - 3.0 * y), 2.0 * mx - 3.0 * my) < 1e-12
+    X = np.ascontiguousarray(np.stack([x, y, x - y, 0.5 * x], axis=1))
+    MX = sv.apply(X)
+    for c in range(X.shape[1]):
+        ref = sv.apply(np.ascontiguousarray(X[:, c]))
+        assert _rel(MX[:, c], ref) < 1e-12, c
+    return x, mx
+
+
+def _check_precisions(sv, x, mx):
+    for prec, tol in (("F32", 2e-6), ("BF16", 2e-2)):
+        sv.cnv(prec)
+        assert _rel(sv.apply(x), mx) < tol, prec
+
+
+def test_pde3d160_invk(gpu):
+    a = gpu.gen_pde3d(160)
+    n = a.shape[0]
+    assert n == 4096000 and a.nnz == 28518400
+    sv = _build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_structure(w, z, d)
+    pw = (w != 0).astype(np.int8)
+    pz = (z.T != 0).astype(np.int8)
+    assert (pw != pz).nnz == 0, "W and Z^T patterns differ"
+    x, mx = _check_apply(gpu, sv, w, z, d, n, 41)
+    del w, z
+
+    # BiCGSTAB 1e-8, right-hand sides A x_j: a block of 4, then two of its
+    # columns alone
+    A = gpu.SystemMatrix(a)
+    B = np.ascontiguousarray(a @ gpu.gen_normal(43, (n, 4)))
+    X, it, err = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
+    assert np.all(err <= 1e-8)
+    for c in range(4):
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c]), c
+    X2, it2, _ = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
+    assert np.array_equal(it2, it) and np.array_equal(X2, X)
+    for c in (0, 3):
+        xs, its, _ = gpu.krylov("BICGSTAB", A, sv,
+                                np.ascontiguousarray(B[:, c]), eps=1e-8)
+        _close_counts(its[0], it[c])
+        _same_solution(a, X[:, c], xs, B[:, c])
+
+    # FP32 factors: the served form over 6 right-hand sides in 4 slots gives
+    # each the iterations and solution of solving it alone
+    _check_precisions(sv, x, mx)
+    sv.cnv("F32")
+    Q = np.ascontiguousarray(B.T[[0, 1, 2, 3, 0, 2]])
+    Xq, itq, errq, lat = gpu.krylov_stream(
+        "BICGSTAB", A, sv, gpu.DeviceArray.from_numpy(Q), slots=4, eps=1e-8)
+    Xq = Xq.to_numpy()
+    assert np.all(errq <= 1e-8) and np.all(lat > 0)
+    for j in (0, 3, 4):
+        xs, its, _ = gpu.krylov("BICGSTAB", A, sv, Q[j], eps=1e-8)
+        _close_counts(itq[j], its[0])
+        assert np.linalg.norm(Q[j] - a @ Xq[j]) <= 1e-7 * np.linalg.norm(Q[j])
+        _same_solution(a, Xq[j], xs, Q[j])
+
+
+def test_aniso2048_sym_llk(gpu):
+    a = gpu.gen_aniso2d(2048, 1e-3)
+    n = a.shape[0]
+    assert n == 4194304 and a.nnz == 20963328
+    sv = _build(gpu, "AINV", a, AINV_ALG="SYM-LLK", SUB_FILLIN=8,
+                SUB_ILUTHRS=5e-2)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_structure(w, z, d)
+    zt = z.T.tocsr()
+    zt.sort_indices()
+    w.sort_indices()
+    assert (np.array_equal(w.indptr, zt.indptr)
+            and np.array_equal(w.indices, zt.indices)
+            and np.array_equal(w.data, zt.data)), "W != Z^T"
+    assert np.all(d > 0.0), "nonpositive pivot on an SPD matrix"
+    x, mx = _check_apply(gpu, sv, w, z, d, n, 51)
+    # symmetric preconditioner: x . M y = y . M x
+    y = gpu.gen_normal(53, (n,))
+    assert abs(float(y @ mx) - float(x @ sv.apply(y))) <= \
+        1e-12 * np.linalg.norm(y) * np.linalg.norm(mx)
+    del w, z, zt
+
+    # CG 1e-8 on N(0,1) right-hand sides with FP32 factors: a block of 2 and
+    # one column alone
+    _check_precisions(sv, x, mx)
+    sv.cnv("F32")
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(3, (n, 2))
+    X, it, err = gpu.krylov("CG", A, sv, B, eps=1e-8, itmax=5000)
+    assert np.all(err <= 1e-8)
+    for c in range(2):
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c]), c
+    xs, its, _ = gpu.krylov("CG", A, sv, np.ascontiguousarray(B[:, 1]),
+                            eps=1e-8, itmax=5000)
+    _close_counts(its[0], it[1])
+    _same_solution(a, X[:, 1], xs, B[:, 1])
+
+
+def test_random1m_ainv_llk(gpu):
+    a = gpu.gen_random(1000000, seed=40, median=12.0, sigma=1.0, cap=400,
+                       band=5000, nlong=2, dom=1.1)
+    n = a.shape[0]
+    assert n == 1000000
+    sv = _build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
+                SUB_ILUTHRS=5e-2)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_structure(w, z, d)
+    # at most SUB_FILLIN entries per column of Z and per row of W
+    assert np.diff(z.tocsc().indptr).max() <= 8
+    assert np.diff(w.indptr).max() <= 8
+    x, mx = _check_apply(gpu, sv, w, z, d, n, 61)
+    del w, z
+
+    # BiCGSTAB 1e-8 on 16 N(0,1) right-hand sides with FP32 and with BF16
+    # factors; two columns alone
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(4, (n, 16))
+    _check_precisions(sv, x, mx)
+    for prec in ("F32", "BF16"):
+        sv.cnv(prec)
+        X, it, err = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
+        assert np.all(err <= 1e-8), prec
+        for c in (0, 15):
+            assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+                1e-7 * np.linalg.norm(B[:, c]), (prec, c)
+            xs, its, _ = gpu.krylov("BICGSTAB", A, sv,
+                                    np.ascontiguousarray(B[:, c]), eps=1e-8)
+            _close_counts(its[0], it[c])
+            _same_solution(a, X[:, c], xs, B[:, c])
