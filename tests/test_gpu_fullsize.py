@@ -202,3 +202,79 @@ def test_random1m_ainv_llk(gpu):
                                     np.ascontiguousarray(B[:, c]), eps=1e-8)
             _close_counts(its[0], it[c])
             _same_solution(a, X[:, c], xs, B[:, c])
+
+
+def test_pde3d160_multilevel(gpu):
+    """The multilevel preconditioner at 160^3 with the settings of the
+    benchmark's multilevel leg (3 levels, smoothed aggregation, threshold 0,
+    MULT / POST, one sweep, 2 coarsest-level sweeps, INVK(0,1) factors)."""
+    a = gpu.gen_pde3d(160)
+    n = a.shape[0]
+    pv = gpu.MldMlPrec()
+    for k, v in dict(SUB_FILLIN=0, INV_FILLIN=1, SUB_SOLVE="INVK",
+                     COARSE_SUBSOLVE="INVK", N_PREC_LEVS=3,
+                     AGGR_KIND="SMOOTHED", AGGR_THRESH=0.0, AGGR_SCALE=1.0,
+                     ML_TYPE="MULT", SMOOTHER_POS="POST", SMOOTHER_SWEEPS=1,
+                     COARSE_SWEEPS=2).items():
+        pv.set(k, v)
+    pv.build(a)
+    assert pv.levels() == 3
+    for lev in range(2):
+        al, p, r = pv.level(lev, "A"), pv.level(lev, "P"), pv.level(lev, "R")
+        g = pv.level(lev, "G")
+        nl, nc = p.shape
+        assert al.shape == (nl, nl) and g.shape == (nl,)
+        # the aggregates partition the level's rows into nc nonempty sets
+        assert g.min() == 0 and g.max() == nc - 1
+        assert np.all(np.bincount(g, minlength=nc) > 0)
+        # R = P^T bit for bit
+        pt = p.T.tocsr()
+        pt.sort_indices()
+        r.sort_indices()
+        assert (np.array_equal(r.indptr, pt.indptr)
+                and np.array_equal(r.indices, pt.indices)
+                and np.array_equal(r.data, pt.data))
+        # P = (I - omega D^-1 A) T with T's unit entries and
+        # omega = 4 / (3 ||D^-1 A||_inf): P 1 = 1 - omega D^-1 A 1
+        dg = al.diagonal()
+        da = abs(al).sum(axis=1).A1 / abs(dg)
+        omega = 4.0 / (3.0 * da.max())
+        ref = 1.0 - omega * (al @ np.ones(nl)) / dg
+        assert np.abs(p @ np.ones(nc) - ref).max() <= 1e-12
+        # Galerkin: A_{l+1} = R A_l P (another summation order)
+        ac = pv.level(lev + 1, "A")
+        diff = ac - r @ (al @ p)
+        assert ac.shape == (nc, nc)
+        assert (np.abs(diff.data).max() if diff.nnz else 0.0) <= \
+            1e-12 * np.abs(ac.data).max()
+    del al, p, r, pt, ac, diff
+
+    # the cycle: linear, a block equal to its columns, FP32 within 2e-6
+    x = gpu.gen_normal(71, (n,))
+    y = gpu.gen_normal(72, (n,))
+    mx, my = pv.apply(x), pv.apply(y)
+    assert _rel(pv.apply(2.0 * x - 3.0 * y), 2.0 * mx - 3.0 * my) < 1e-12
+    X = np.ascontiguousarray(np.stack([x, y, x - y, 0.5 * x], axis=1))
+    MX = pv.apply(X)
+    for c in range(4):
+        assert _rel(MX[:, c], pv.apply(np.ascontiguousarray(X[:, c]))) \
+            < 1e-12, c
+
+    # BiCGSTAB 1e-8 with the cycle: a block of 4, repeated bit for bit, and
+    # one column alone
+    A = gpu.SystemMatrix(a)
+    B = np.ascontiguousarray(a @ gpu.gen_normal(73, (n, 4)))
+    Xs, it, err = gpu.krylov("BICGSTAB", A, pv, B, eps=1e-8)
+    assert np.all(err <= 1e-8) and np.all(it <= 12), it
+    for c in range(4):
+        assert np.linalg.norm(B[:, c] - a @ Xs[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c]), c
+    X2, it2, _ = gpu.krylov("BICGSTAB", A, pv, B, eps=1e-8)
+    assert np.array_equal(it2, it) and np.array_equal(X2, Xs)
+    xs, its, _ = gpu.krylov("BICGSTAB", A, pv, np.ascontiguousarray(B[:, 2]),
+                            eps=1e-8)
+    _close_counts(its[0], it[2])
+    _same_solution(a, Xs[:, 2], xs, B[:, 2])
+
+    pv.cnv("F32")
+    assert _rel(pv.apply(x), mx) < 2e-6
