@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <functional>
+#include <system_error>
 #include <thread>
 
 #include "objects.h"
@@ -75,14 +76,20 @@ void parallel_rows(int32_t n, int nt,
   for (int t = 0; t < nt; ++t) {
     const int32_t lo = (int32_t)((int64_t)n * t / nt);
     const int32_t hi = (int32_t)((int64_t)n * (t + 1) / nt);
-    th.emplace_back([&, lo, hi] {
+    auto range = [&, lo, hi] {
       try {
         f(lo, hi);
       } catch (...) {
         std::lock_guard<std::mutex> lk(emu);
         if (!err) err = std::current_exception();
       }
-    });
+    };
+    // a range whose thread cannot be started runs on this thread
+    try {
+      th.emplace_back(range);
+    } catch (const std::system_error &) {
+      range();
+    }
   }
   for (auto &x : th) x.join();
   if (err) std::rethrow_exception(err);
@@ -546,24 +553,31 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
     std::vector<std::thread> th;
     std::exception_ptr err;
     std::mutex emu;
-    for (int l = 0; l < nl; ++l)
-      th.emplace_back([&, l] {
-        ainv_solver *sv = H->lv[l]->sv.get();
-        try {
-          AINV_HIP_CHECK(hipSetDevice(h->device));
-          AINV_HIP_CHECK(hipStreamCreate(&sv->stream));
-          build_solver(sv, H->lv[l]->a);
-        } catch (...) {
-          std::lock_guard<std::mutex> lk(emu);
-          if (!err) err = std::current_exception();
-        }
-        // the level keeps the handle's stream from here on
-        if (sv->stream && sv->stream != h->stream) {
-          (void)hipStreamSynchronize(sv->stream);
-          (void)hipStreamDestroy(sv->stream);
-        }
-        sv->stream = h->stream;
-      });
+    auto level_build = [&](int l) {
+      ainv_solver *sv = H->lv[l]->sv.get();
+      try {
+        AINV_HIP_CHECK(hipSetDevice(h->device));
+        AINV_HIP_CHECK(hipStreamCreate(&sv->stream));
+        build_solver(sv, H->lv[l]->a);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(emu);
+        if (!err) err = std::current_exception();
+      }
+      // the level keeps the handle's stream from here on
+      if (sv->stream && sv->stream != h->stream) {
+        (void)hipStreamSynchronize(sv->stream);
+        (void)hipStreamDestroy(sv->stream);
+      }
+      sv->stream = h->stream;
+    };
+    // a level whose thread cannot be started is built on this thread
+    for (int l = 0; l < nl; ++l) {
+      try {
+        th.emplace_back(level_build, l);
+      } catch (const std::system_error &) {
+        level_build(l);
+      }
+    }
     for (auto &x : th) x.join();
     if (err) std::rethrow_exception(err);
   }
