@@ -37,6 +37,7 @@
 #include <memory>
 #include <mutex>
 #include <queue>
+#include <system_error>
 #include <thread>
 
 #include "ainv_internal.h"
@@ -561,7 +562,13 @@ void pipelined_pass(int32_t n, const HostCsr &dotm, const HostCsr &seedm,
     }
   };
   std::vector<std::thread> pool;
-  for (int t = 1; t < threads; ++t) pool.emplace_back(worker);
+  for (int t = 1; t < threads; ++t) {
+    try {
+      pool.emplace_back(worker);
+    } catch (const std::system_error &) {
+      break;  // columns are claimed in order by whichever threads run
+    }
+  }
   worker();
   for (auto &t : pool) t.join();
   if (err) std::rethrow_exception(err);
@@ -655,14 +662,21 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
     const int tw = std::max(1, threads - tz);
     std::exception_ptr err;
     std::thread wth;
-    if (two)
-      wth = std::thread([&] {
-        try {
-          pipelined_pass(n, at, as, q, fill_in, thresh, var, wc, tw, wc);
-        } catch (...) {
-          err = std::current_exception();
-        }
-      });
+    auto w_pass = [&] {
+      try {
+        pipelined_pass(n, at, as, q, fill_in, thresh, var, wc, tw, wc);
+      } catch (...) {
+        err = std::current_exception();
+      }
+    };
+    bool w_here = false;  // the W pass runs after the Z pass on this thread
+    if (two) {
+      try {
+        wth = std::thread(w_pass);
+      } catch (const std::system_error &) {
+        w_here = true;
+      }
+    }
     try {
       pipelined_pass(n, as, at, p, fill_in, thresh, var, zc, tz, zc);
     } catch (...) {
@@ -670,6 +684,7 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
       throw;
     }
     if (wth.joinable()) wth.join();
+    if (w_here) w_pass();
     if (err) std::rethrow_exception(err);
     done = true;
   }

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 
@@ -429,7 +430,16 @@ int ainv_server_create(int device, int64_t hbm_budget, int workers,
       delete S;
       throw Error(AINV_ERR_HIP, "cannot create the server's copy stream");
     }
-    for (int w = 0; w < workers; ++w) S->workers.emplace_back(worker_main, S);
+    for (int w = 0; w < workers; ++w) {
+      try {
+        S->workers.emplace_back(worker_main, S);
+      } catch (const std::system_error &) {
+        // serve with the workers that started
+        if (S->workers.empty())
+          throw Error(AINV_ERR_INTERNAL, "cannot start a server worker thread");
+        break;
+      }
+    }
     *out = S;
   });
 }
