@@ -267,6 +267,24 @@ int ainv_host_ainv_bld(int32_t n, const int64_t *irp, const int32_t *ja,
                        int32_t *z_ja, double *z_val, int64_t *z_nnz,
                        double *d);
 
+/* The incomplete factorization of the INVK / INVT builds alone, on the
+ * host (no device involved): ILU(fill) when kind = AINV_KIND_INVK (the
+ * mld_iluk_fact call of src/impl/mld_d_invk_bld.f90:104-105), ILUT(fill,
+ * thresh) with maxval scaling when kind = AINV_KIND_INVT (the
+ * mld_ilut_fact call of src/impl/mld_d_invt_bld.f90:108-109).  Outputs
+ * (0-based CSR, FP64): L strictly lower (unit diagonal implied), U strictly
+ * upper (unit diagonal implied, rows scaled by the inverted pivots), with
+ * row pointers of n+1 entries and at most `cap` entries each (larger:
+ * AINV_ERR_ARG with the sizes in *l_nnz, *u_nnz), and the n inverted
+ * pivots d, so that A = L diag(1/d) (I + U).  A zero pivot returns
+ * AINV_ERR_PIVOT. */
+int ainv_host_ilu_fact(int32_t n, const int64_t *irp, const int32_t *ja,
+                       const double *val, int index_base, int kind,
+                       int32_t fill, double thresh, int64_t cap,
+                       int64_t *l_irp, int32_t *l_ja, double *l_val,
+                       int64_t *l_nnz, int64_t *u_irp, int32_t *u_ja,
+                       double *u_val, int64_t *u_nnz, double *d);
+
 /* sv%update_a (mld_d_base_ainv_update_a.f90:53-79): adapt the built
  * preconditioner to the shifted operator A + diag(x) by updating the
  * inverted pivots, d_i = 1 / (1/d_i + sum_k W(i,k) x_k Z(k,i)) (stored W by

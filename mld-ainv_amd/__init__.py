@@ -100,6 +100,10 @@ def lib():
                                C.c_int32, C.c_double, C.c_int, C.c_int64,
                                i64p, i32p, dp, P(C.c_int64), i64p, i32p, dp,
                                P(C.c_int64), dp],
+        "ainv_host_ilu_fact": [C.c_int32, i64p, i32p, dp, C.c_int, C.c_int,
+                               C.c_int32, C.c_double, C.c_int64, i64p, i32p,
+                               dp, P(C.c_int64), i64p, i32p, dp,
+                               P(C.c_int64), dp],
         "ainv_krylov_stream": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_int,
                                C.c_double, C.c_int, vp, vp, vp, vp],
         "ainv_krylov": [C.c_char_p, vp, vp, vp, vp, C.c_int, C.c_double,
@@ -684,6 +688,36 @@ def host_ainv_bld(a, alg="LLK", fill_in=8, thresh=0.1, threads=0):
     w = sp.csr_matrix((wv[:wn.value], wj[:wn.value], wi), shape=(n, n))
     z = sp.csr_matrix((zv[:zn.value], zj[:zn.value], zi), shape=(n, n))
     return w, z, d
+
+
+def host_ilu_fact(a, kind="ILUT", fill=1, thresh=0.0):
+    """The INVK / INVT builds' incomplete factorization on the host
+    (ainv_host_ilu_fact): kind "ILUK" (mld_iluk_fact) or "ILUT"
+    (mld_ilut_fact, maxval scaling).  Returns (L, U, d) as scipy CSR /
+    numpy: L strictly lower, U strictly upper (unit diagonals implied),
+    d the inverted pivots."""
+    import scipy.sparse as sp
+    n, irp, ja, val = _csr_args(a)
+    kid = {"ILUK": KIND_INVK, "ILUT": KIND_INVT}[kind.upper()]
+    cap = max(4 * len(val), 1024)
+    while True:
+        outs = [(np.empty(n + 1, dtype=np.int64), np.empty(cap, dtype=np.int32),
+                 np.empty(cap, dtype=np.float64)) for _ in range(2)]
+        ln, un = C.c_int64(), C.c_int64()
+        d = np.empty(n, dtype=np.float64)
+        (li, lj, lv), (ui, uj, uv) = outs
+        rc = lib().ainv_host_ilu_fact(
+            n, *_ptrs(irp, ja, val), 0, kid, int(fill), float(thresh), cap,
+            *_ptrs(li, lj, lv), C.byref(ln), *_ptrs(ui, uj, uv), C.byref(un),
+            d.ctypes.data_as(C.POINTER(C.c_double)))
+        if rc == ERR_ARG and max(ln.value, un.value) > cap:
+            cap = max(ln.value, un.value)
+            continue
+        _ck(rc)
+        break
+    L = sp.csr_matrix((lv[:ln.value], lj[:ln.value], li), shape=(n, n))
+    U = sp.csr_matrix((uv[:un.value], uj[:un.value], ui), shape=(n, n))
+    return L, U, d
 
 
 # --------------------------------------------------------------- generators

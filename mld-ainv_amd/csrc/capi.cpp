@@ -1364,6 +1364,45 @@ int ainv_host_ainv_bld(int32_t n, const int64_t *irp, const int32_t *ja,
   });
 }
 
+int ainv_host_ilu_fact(int32_t n, const int64_t *irp, const int32_t *ja,
+                       const double *val, int index_base, int kind,
+                       int32_t fill, double thresh, int64_t cap,
+                       int64_t *l_irp, int32_t *l_ja, double *l_val,
+                       int64_t *l_nnz, int64_t *u_irp, int32_t *u_ja,
+                       double *u_val, int64_t *u_nnz, double *d) {
+  return guard([&] {
+    if (!l_irp || !l_ja || !l_val || !l_nnz || !u_irp || !u_ja || !u_val ||
+        !u_nnz || !d)
+      throw Error(AINV_ERR_ARG, "NULL output");
+    if (kind != AINV_KIND_INVK && kind != AINV_KIND_INVT)
+      throw Error(AINV_ERR_ARG, "kind must be AINV_KIND_INVK (ILU(k)) or "
+                                "AINV_KIND_INVT (ILUT)");
+    if (fill < 0) throw Error(AINV_ERR_ARG, "fill must be >= 0");
+    if (kind == AINV_KIND_INVT && !(thresh >= 0.0))
+      throw Error(AINV_ERR_ARG, "thresh must be >= 0");
+    HostCsr a = canonical_csr(n, irp, ja, val, index_base);
+    HostCsr l, u;
+    std::vector<double> dv;
+    if (kind == AINV_KIND_INVK)
+      host_iluk(a, fill, l, u, dv);
+    else
+      host_ilut(a, fill, thresh, l, u, dv);
+    *l_nnz = l.nnz();
+    *u_nnz = u.nnz();
+    if (l.nnz() > cap || u.nnz() > cap)
+      throw Error(AINV_ERR_ARG, "factor larger than the output capacity (" +
+                                    std::to_string(std::max(l.nnz(), u.nnz())) +
+                                    " entries)");
+    std::copy(l.ptr.begin(), l.ptr.end(), l_irp);
+    std::copy(l.idx.begin(), l.idx.end(), l_ja);
+    std::copy(l.val.begin(), l.val.end(), l_val);
+    std::copy(u.ptr.begin(), u.ptr.end(), u_irp);
+    std::copy(u.idx.begin(), u.idx.end(), u_ja);
+    std::copy(u.val.begin(), u.val.end(), u_val);
+    std::copy(dv.begin(), dv.end(), d);
+  });
+}
+
 int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
                 double *x, int k, double eps, int itmax, int istop, int irst,
                 int32_t *iter, double *err, int flags, void *stream) {

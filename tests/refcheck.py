@@ -254,3 +254,124 @@ def dense_iluk_levels(a, fill):
                 if lev[k, j] <= fill:
                     lev[i, j] = min(lev[i, j], lev[i, k] + lev[k, j] + 1)
     return lev <= fill
+
+
+def dense_ilut(a, fill, thresh, mutate=None):
+    """ILUT(p, tau) of Saad, Iterative Methods for Sparse Linear Systems,
+    2nd ed., Alg. 10.6, restated with a dense work row scanned in full
+    (no heap), dense pattern flags and numpy selection, with the
+    conventions the INVT build relies on (mld_d_invt_bld.f90:108-109 calls
+    mld_ilut_fact with iscale = maxval):
+
+    * A is scaled by 1 / max|a_ij| first; tau_i = tau * ||scaled row i||_2
+      (reference-BLAS dnrm2 over the stored entries in stored order);
+    * elimination (lines 3-8): for k < i in ascending order, if k is in the
+      row's pattern, the multiplier L(i,k) = w_k * d_k (d_k the inverted
+      pivot of row k) is dropped when |L(i,k)| < tau_i; otherwise
+      w_j -= w_k * U(k,j) over the stored unit-upper row k, and a position
+      j outside the pattern whose new value is below tau_i is set to 0 and
+      stays outside (entries of A's own pattern are only tested at the
+      end);
+    * dropping at copy-out (line 10): entries below tau_i go; L keeps the
+      nlw + p and U the nup + p largest |values| (nlw, nup the row's own
+      counts in A; ties to the smaller column); the pivot always stays;
+    * U is stored unit upper (scaled by the row's inverted pivot), d holds
+      the inverted pivots times the scale factor (A = L diag(1/d) U).
+
+    `mutate` plants one plausible mistake for the tests that check this
+    restatement's power: "keep_smallest", "unscaled_norm" (tau_i from the
+    unscaled row), "drop_before_scale" (the drop test on w_k instead of
+    w_k d_k).  Returns (L rows, U rows, d, stats) with rows as
+    {column: value} dicts and stats counting threshold and cap drops."""
+    a = sp.csr_matrix(a)
+    n = a.shape[0]
+    scale = 1.0 / np.abs(a.data).max()
+    Lr, Ur = [], []
+    d = np.zeros(n)
+    stats = {"thresh_drops": 0, "cap_cuts": 0}
+    for i in range(n):
+        lo, hi = a.indptr[i], a.indptr[i + 1]
+        cols = a.indices[lo:hi]
+        vals = [float(v) * scale for v in a.data[lo:hi]]
+        w = np.zeros(n)
+        pat = np.zeros(n, dtype=bool)
+        for c, v in zip(cols, vals):
+            w[c] = v
+            pat[c] = True
+        nlw = int(np.sum(cols < i))
+        nup = int(np.sum(cols > i))
+        nv = dnrm2(a.data[lo:hi]) if mutate == "unscaled_norm" else \
+            dnrm2(vals)
+        tau = thresh * nv
+        for k in range(i):
+            if not pat[k]:
+                continue
+            wk = w[k]
+            lik = wk * d[k]
+            test = wk if mutate == "drop_before_scale" else lik
+            if abs(test) < tau:
+                w[k] = 0.0
+                pat[k] = False
+                stats["thresh_drops"] += 1
+                continue
+            w[k] = lik
+            for j, ukj in Ur[k].items():
+                w[j] = w[j] - wk * ukj
+                if not pat[j]:
+                    if abs(w[j]) < tau:
+                        w[j] = 0.0
+                        stats["thresh_drops"] += 1
+                    else:
+                        pat[j] = True
+        piv = w[i] if pat[i] else 0.0
+        if piv == 0.0:
+            raise ZeroDivisionError(f"zero pivot at row {i}")
+
+        def keep(js, cap):
+            js = np.asarray(js, dtype=np.int64)
+            big = js[np.abs(w[js]) >= tau] if len(js) else js
+            stats["thresh_drops"] += len(js) - len(big)
+            if len(big) > cap:
+                stats["cap_cuts"] += len(big) - cap
+                mag = np.abs(w[big])
+                order = np.lexsort((big, mag if mutate == "keep_smallest"
+                                    else -mag))
+                big = big[order[:max(cap, 0)]]
+            return np.sort(big)
+
+        lj = keep(np.nonzero(pat[:i])[0], nlw + fill)
+        Lr.append({int(j): float(w[j]) for j in lj})
+        d[i] = 1.0 / piv
+        uj = keep(np.nonzero(pat[i + 1:])[0] + i + 1, nup + fill)
+        Ur.append({int(j): float(w[j] * d[i]) for j in uj})
+    return Lr, Ur, d * scale, stats
+
+
+def csr_digest(m, values=True):
+    """SHA-256 of a CSR matrix's row pointers (int64), column indices
+    (int32) and, with values, its FP64 values, little-endian, columns
+    sorted within rows: equal digests mean bit-identical matrices."""
+    import hashlib
+    m = sp.csr_matrix(m)
+    m.sort_indices()
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(m.indptr, dtype="<i8").tobytes())
+    h.update(np.ascontiguousarray(m.indices, dtype="<i4").tobytes())
+    if values:
+        h.update(np.ascontiguousarray(m.data, dtype="<f8").tobytes())
+    return h.hexdigest()
+
+
+def vec_digest(v):
+    import hashlib
+    return hashlib.sha256(
+        np.ascontiguousarray(v, dtype="<f8").tobytes()).hexdigest()
+
+
+def factor_digests(w, z, d):
+    """Digests of stored W, Z (patterns and full) and d, plus the entry
+    counts, as committed for the full-size factor checks."""
+    return {"w_nnz": int(w.nnz), "z_nnz": int(z.nnz),
+            "w_pattern": csr_digest(w, values=False),
+            "z_pattern": csr_digest(z, values=False),
+            "w": csr_digest(w), "z": csr_digest(z), "d": vec_digest(d)}

@@ -3,8 +3,10 @@ benchmark workloads: pde3d 160^3 with INVK(1,1) (n = 4,096,000), the 2048^2
 anisotropic Laplacian with SYM-LLK AINV (n = 4,194,304), and the random
 nonsymmetric matrix of n = 10^6 with AINV-LLK.
 
-The oracle needs minutes for these factorizations, so the bit-exact factor
-comparisons stay at the small sizes of test_gpu_parity.py.  Here:
+The factors are compared bit for bit with the oracle's through SHA-256
+digests committed in tests/golden/fullsize_hashes.json (the oracle takes
+10-40 s per factorization at these sizes, too long to run inside a GPU
+test); beyond that:
 
 * factor structure: W (stored lower) and Z unit triangular with an exact 1.0
   diagonal, finite nonzero d; SYM-LLK gives W = Z^T bit for bit and d > 0 on
@@ -25,11 +27,33 @@ comparisons stay at the small sizes of test_gpu_parity.py.  Here:
   BiCGSTAB's trajectory amplifies that rounding (65 against 71 iterations
   seen for one column), so counts are held to 10 % here.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
+import refcheck as R
+
 pytestmark = pytest.mark.gpu
+
+with open(os.path.join(os.path.dirname(__file__), "golden",
+                       "fullsize_hashes.json")) as _f:
+    HASHES = json.load(_f)
+
+
+def _check_digests(name, a, w, z, d):
+    """The matrix and the exported W, Z, d bit for bit equal to the
+    oracle's (digests from tests/golden/make_fullsize_hashes.py); the
+    pattern digests are compared first so that a failure says whether the
+    sparsity pattern or only values differ."""
+    ref = HASHES[name]
+    assert a.nnz == ref["a_nnz"] and R.csr_digest(a) == ref["a"], \
+        "generated matrix differs from the one the oracle factored"
+    got = R.factor_digests(w, z, d)
+    for key in ("w_nnz", "z_nnz", "w_pattern", "z_pattern", "w", "z", "d"):
+        assert got[key] == ref[key], f"{name}: {key} differs from the oracle"
 
 
 def _build(mld, kind, a, **params):
@@ -92,6 +116,7 @@ def test_pde3d160_invk(gpu):
     assert n == 4096000 and a.nnz == 28518400
     sv = _build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1)
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_digests("pde3d160_invk_1_1", a, w, z, d)
     _check_structure(w, z, d)
     pw = (w != 0).astype(np.int8)
     pz = (z.T != 0).astype(np.int8)
@@ -132,6 +157,35 @@ def test_pde3d160_invk(gpu):
         _same_solution(a, Xq[j], xs, Q[j])
 
 
+def test_pde3d160_invt(gpu):
+    """INVT at 160^3 with the settings of the benchmark's INVT leg: ILUT
+    fill 10, drop 1e-2 (maxval scaling), inverse fill 10, inverse drop
+    1e-2; factors bit for bit the oracle's."""
+    a = gpu.gen_pde3d(160)
+    n = a.shape[0]
+    sv = _build(gpu, "INVT", a, SUB_FILLIN=10, SUB_ILUTHRS=1e-2,
+                INV_FILLIN=10, INV_THRESH=1e-2)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_digests("pde3d160_invt_10_1e-2_10_1e-2", a, w, z, d)
+    _check_structure(w, z, d)
+    x, mx = _check_apply(gpu, sv, w, z, d, n, 81)
+    del w, z
+    _check_precisions(sv, x, mx)
+    # BiCGSTAB 1e-8 with FP32 factors: a block of 2 and one column alone
+    sv.cnv("F32")
+    A = gpu.SystemMatrix(a)
+    B = np.ascontiguousarray(a @ gpu.gen_normal(83, (n, 2)))
+    X, it, err = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
+    assert np.all(err <= 1e-8)
+    for c in range(2):
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c]), c
+    xs, its, _ = gpu.krylov("BICGSTAB", A, sv, np.ascontiguousarray(B[:, 1]),
+                            eps=1e-8)
+    _close_counts(its[0], it[1])
+    _same_solution(a, X[:, 1], xs, B[:, 1])
+
+
 def test_aniso2048_sym_llk(gpu):
     a = gpu.gen_aniso2d(2048, 1e-3)
     n = a.shape[0]
@@ -139,6 +193,7 @@ def test_aniso2048_sym_llk(gpu):
     sv = _build(gpu, "AINV", a, AINV_ALG="SYM-LLK", SUB_FILLIN=8,
                 SUB_ILUTHRS=5e-2)
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_digests("aniso2048_symllk_8_5e-2", a, w, z, d)
     _check_structure(w, z, d)
     zt = z.T.tocsr()
     zt.sort_indices()
@@ -179,6 +234,7 @@ def test_random1m_ainv_llk(gpu):
     sv = _build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
                 SUB_ILUTHRS=5e-2)
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    _check_digests("random1e6_llk_8_5e-2", a, w, z, d)
     _check_structure(w, z, d)
     # at most SUB_FILLIN entries per column of Z and per row of W
     assert np.diff(z.tocsc().indptr).max() <= 8

@@ -70,3 +70,43 @@ def test_host_ml_level_matches_oracle(mld, O, mat, theta, smoothed, threads):
     assert np.array_equal(agg, ragg)
     assert same_bits(P, rP) and same_bits(Ac, rAc)
     assert om == rom
+
+
+def _digests():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "fullsize_hashes.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("name", ["random1e6_llk_8_5e-2",
+                                  "aniso2048_symllk_8_5e-2"])
+def test_host_ainv_fullsize_matches_oracle_digests(mld, name):
+    """The library's host biconjugation at the benchmark sizes (the random
+    n = 10^6 matrix, pipelined, and the 2048^2 anisotropic Laplacian)
+    bit for bit equal to the oracle's factors, through the digests of
+    tests/golden/make_fullsize_hashes.py."""
+    import refcheck as R
+    ref = _digests()[name]
+    if ref["matrix"] == "random":
+        a = mld.gen_random(ref["size"], seed=40, median=12.0, sigma=1.0,
+                           cap=400, band=5000, nlong=2, dom=1.1)
+    else:
+        a = mld.gen_aniso2d(ref["size"], 1e-3)
+    assert R.csr_digest(a) == ref["a"]
+    p = ref["params"]
+    w, z, d = mld.host_ainv_bld(a, p["alg"], p["fill_in"], p["thresh"], 0)
+    got = R.factor_digests(w, z, d)
+    for key in ("w_nnz", "z_nnz", "w_pattern", "z_pattern", "w", "z", "d"):
+        assert got[key] == ref[key], key
+
+
+def test_host_ilu_fullsize_digest_inputs(mld):
+    """The INVK / INVT digests at 160^3 are of factors of this generator's
+    matrix: the generator at full size against the recorded digest (the
+    GPU tests compare the factors)."""
+    import refcheck as R
+    ref = _digests()["pde3d160_invk_1_1"]
+    a = mld.gen_pde3d(160)
+    assert a.nnz == ref["a_nnz"] and R.csr_digest(a) == ref["a"]

@@ -565,3 +565,127 @@ def test_golden_fixtures(O):
     w, z, d = O.invk_bld(a, 1, 1)
     _, it, _ = O.krylov("BICGSTAB", a, w, z, d, a @ np.ones(n), 1e-8)
     assert abs(it - int(g["scipy_bicgstab_iters_invk"])) <= 1
+
+
+# ------------------------------------------------- ILUT against Saad 10.6
+def ilut_case_matrix(n, density, seed, mag=250.0):
+    """Nonsymmetric, rows of very different magnitude (so tau_i differs from
+    row to row and the pivots from 1 after maxval scaling), max|a| far from
+    1, diagonally dominant by a random factor in [1.05, 3]."""
+    rng = np.random.default_rng(seed)
+    m = sp.random(n, n, density=density, random_state=rng,
+                  data_rvs=lambda k: rng.standard_normal(k)).tocsr()
+    m = m - sp.diags(m.diagonal())
+    rowabs = np.asarray(abs(m).sum(axis=1)).ravel()
+    dom = rng.uniform(1.05, 3.0, n)
+    m = (m + sp.diags(dom * rowabs + 0.1)).tocsr()
+    rs = 10.0 ** rng.uniform(-1.0, 1.0, n)
+    m = (sp.diags(rs * mag) @ m).tocsr()
+    m.sort_indices()
+    return m
+
+
+def ilut_rows(l, u):
+    return R.rows_of(l), R.rows_of(u)
+
+
+# (fill, thresh) binding: the threshold alone (fill never caps), the caps
+# alone (thresh 0 drops nothing), and both
+ILUT_CASES = [("thresh", 10 ** 6, 0.05), ("caps", 0, 0.0), ("caps", 1, 0.0),
+              ("both", 1, 0.02), ("both", 2, 0.01)]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("what,fill,thresh", ILUT_CASES)
+def test_ilut_matches_dense_saad(O, seed, what, fill, thresh):
+    a = ilut_case_matrix(48, 0.12, 300 + seed)
+    fill = min(fill, a.shape[0])
+    l, u, d = O.ilut(a, fill, thresh)
+    Lr, Ur, dr, st = R.dense_ilut(a, fill, thresh)
+    assert ilut_rows(l, u) == (Lr, Ur)
+    assert np.array_equal(d, dr)
+    # the case binds what it claims to
+    if what in ("thresh", "both"):
+        assert st["thresh_drops"] > 0
+    if what in ("caps", "both"):
+        assert st["cap_cuts"] > 0
+    if what == "thresh":
+        assert st["cap_cuts"] == 0
+    if what == "caps":
+        assert st["thresh_drops"] == 0
+
+
+@pytest.mark.parametrize("mutate", ["keep_smallest", "unscaled_norm",
+                                    "drop_before_scale"])
+def test_ilut_restatement_detects_planted_mistakes(O, mutate):
+    """Each plausible mistake, planted in the dense restatement, changes
+    the factors on at least one of the cases above, so agreement with the
+    oracle rules it out there."""
+    differs = False
+    for seed in (0, 1, 2):
+        a = ilut_case_matrix(48, 0.12, 300 + seed)
+        for _, fill, thresh in ILUT_CASES:
+            l, u, d = O.ilut(a, fill, thresh)
+            try:
+                Lr, Ur, dr, _ = R.dense_ilut(a, fill, thresh, mutate=mutate)
+            except ZeroDivisionError:
+                differs = True
+                continue
+            if ilut_rows(l, u) != (Lr, Ur) or not np.array_equal(d, dr):
+                differs = True
+    assert differs
+
+
+def test_ilut_tie_goes_to_smaller_column(O):
+    """Equal magnitudes at the cap: the smaller column stays (the tie rule
+    the oracle fixes for the PSBLAS heap)."""
+    # row 0 has two equal upper entries under a cap of nup + 1 = 3: nothing
+    # to cut; row 1 (no upper entry of its own) gains equal fill at columns
+    # 2 and 3 from row 0 and keeps nup + fill = 1 of them
+    a = sp.csr_matrix(np.array([[4.0, 0.0, 1.0, 1.0],
+                                [1.0, 4.0, 0.0, 0.0],
+                                [0.0, 0.0, 4.0, 0.0],
+                                [0.0, 0.0, 0.0, 4.0]]))
+    l, u, d = O.ilut(a, 1, 0.0)
+    Lr, Ur, dr, st = R.dense_ilut(a, 1, 0.0)
+    assert ilut_rows(l, u) == (Lr, Ur)
+    assert st["cap_cuts"] == 1
+    assert list(Ur[1].keys()) == [2]
+
+
+@pytest.mark.parametrize("what,fill,thresh", ILUT_CASES)
+def test_library_host_ilut_matches_saad_and_oracle(mld, O, what, fill, thresh):
+    """The library's own ILUT (ainv_host_ilu_fact, the factorization the
+    INVT build runs) against the dense restatement, bit for bit."""
+    a = ilut_case_matrix(60, 0.1, 310)
+    l, u, d = mld.host_ilu_fact(a, "ILUT", fill, thresh)
+    Lr, Ur, dr, _ = R.dense_ilut(a, fill, thresh)
+    assert ilut_rows(l, u) == (Lr, Ur)
+    assert np.array_equal(d, dr)
+    lo, uo, do = O.ilut(a, fill, thresh)
+    assert (l != lo).nnz == 0 and (u != uo).nnz == 0
+
+
+@pytest.mark.parametrize("fill", [0, 1, 2])
+def test_library_host_iluk_matches_oracle_and_levels(mld, O, fill):
+    a = rand_sparse(50, 0.06, 40 + fill)
+    l, u, d = mld.host_ilu_fact(a, "ILUK", fill)
+    lo, uo, do = O.iluk(a, fill)
+    assert (l != lo).nnz == 0 and (u != uo).nnz == 0
+    assert np.array_equal(d, do)
+    keep = R.dense_iluk_levels(a, fill)
+    np.fill_diagonal(keep, False)
+    lu = (l + u).tocoo()
+    pat = np.zeros_like(keep)
+    pat[lu.row, lu.col] = True
+    assert np.array_equal(pat, keep)
+
+
+def test_library_host_ilu_zero_pivot_and_args(mld):
+    a = sp.csr_matrix(np.array([[0.0, 1.0], [1.0, 1.0]]))
+    with pytest.raises(mld.AinvError) as e:
+        mld.host_ilu_fact(a, "ILUT", 1, 0.0)
+    assert e.value.code == mld.ERR_PIVOT
+    b = rand_sparse(10, 0.2, 1)
+    with pytest.raises(mld.AinvError):
+        mld.host_ilu_fact(b, "ILUT", -1, 0.0)
