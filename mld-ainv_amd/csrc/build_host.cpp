@@ -562,6 +562,7 @@ void pipelined_pass(int32_t n, const HostCsr &dotm, const HostCsr &seedm,
     }
   };
   std::vector<std::thread> pool;
+  pool.reserve(threads > 1 ? threads - 1 : 0);
   for (int t = 1; t < threads; ++t) {
     try {
       pool.emplace_back(worker);

@@ -493,6 +493,7 @@ struct KrylovPlan {
   int apply_map = 0;  // packed pair of the two passes' mappings
   int spmv_map = AINV_MAP_SPLIT;
   int32_t n = 0;
+  int holders = 0;  // open slot batches using this plan (never evicted)
   DevBuf work, colbuf, part, nact, tbuf;
   Vecs v{};
   Col *col = nullptr;
@@ -582,6 +583,27 @@ Vecs x32_vecs(const KrylovPlan *p) {
   return w;
 }
 
+// The buffers the host reads after an event (pinned) must be host
+// allocations and the ones kernels write (col, na) device allocations;
+// checked once per plan, when they are made.
+void check_buffer(const void *ptr, bool host, const char *what) {
+  if (!ptr) return;
+  hipPointerAttribute_t at{};
+  AINV_HIP_CHECK(hipPointerGetAttributes(&at, ptr));
+  const bool is_host = at.type == hipMemoryTypeHost;
+  if (is_host != host)
+    throw Error(AINV_ERR_INTERNAL, std::string("Krylov plan: ") + what +
+                                       (host ? " is not pinned host memory"
+                                             : " is not device memory"));
+}
+
+void check_host_device(const KrylovPlan *p) {
+  check_buffer(p->pinned, true, "pinned count");
+  check_buffer(p->pinned_cols, true, "pinned column states");
+  check_buffer(p->col, false, "column states");
+  check_buffer(p->na, false, "active counts");
+}
+
 KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
                      hipStream_t s, int irst = 0) {
   for (size_t i = 0; i < a->plans.size(); ++i) {
@@ -644,8 +666,18 @@ KrylovPlan *get_plan(ainv_matrix *a, int method, const ainv_solver *m, int k,
                                hipHostMallocDefault));
   for (auto &e : p->ev)
     AINV_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  check_host_device(p);
   a->plans.insert(a->plans.begin(), std::move(up));
-  if ((int)a->plans.size() > kPlansPerMatrix) a->plans.pop_back();
+  // evict the least recently used plan that no open slot batch holds (a
+  // batch keeps launching its plan's graph and copying its column states
+  // until it is closed)
+  if ((int)a->plans.size() > kPlansPerMatrix) {
+    for (size_t i = a->plans.size(); i-- > 1;)
+      if (a->plans[i]->holders == 0) {
+        a->plans.erase(a->plans.begin() + i);
+        break;
+      }
+  }
   return p;
 }
 
@@ -933,10 +965,12 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
   KrylovPlan *p = get_plan(a, method, m, K, s);
   prepare_method(p, a, m);
-  if (!p->pinned_cols)
+  if (!p->pinned_cols) {
     AINV_HIP_CHECK(hipHostMalloc((void **)&p->pinned_cols,
                                  2 * kMaxCols * sizeof(Col),
                                  hipHostMallocDefault));
+    check_buffer(p->pinned_cols, true, "pinned column states");
+  }
   if (!p->mask.p) p->mask.alloc(kMaxCols * sizeof(int32_t));
   // every slot empty: inactive columns are skipped by every kernel
   std::vector<Col> init(kMaxCols);
@@ -950,6 +984,7 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
   auto *b = new SlotBatch();
   b->a = a;
   b->p = p;
+  ++p->holders;
   b->s = s;
   b->K = p->K;  // slot count and leading dimension (pow2 >= K)
   b->hmask.assign(kMaxCols, 0);
@@ -958,6 +993,7 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
 
 void batch_close(SlotBatch *b) {
   if (b && b->s) (void)hipStreamSynchronize(b->s);
+  if (b && b->p) --b->p->holders;
   delete b;
 }
 

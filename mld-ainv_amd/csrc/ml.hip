@@ -71,6 +71,7 @@ void parallel_rows(int32_t n, int nt,
     return;
   }
   std::vector<std::thread> th;
+  th.reserve(nt);
   std::exception_ptr err;
   std::mutex emu;
   for (int t = 0; t < nt; ++t) {
@@ -551,12 +552,17 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
   }
   {
     std::vector<std::thread> th;
+    th.reserve(nl);
     std::exception_ptr err;
     std::mutex emu;
     auto level_build = [&](int l) {
       ainv_solver *sv = H->lv[l]->sv.get();
+      // run on the handle's device; a level built on the calling thread
+      // leaves that thread's current device as it found it
+      int prev = -1;
       try {
-        AINV_HIP_CHECK(hipSetDevice(h->device));
+        AINV_HIP_CHECK(hipGetDevice(&prev));
+        if (prev != h->device) AINV_HIP_CHECK(hipSetDevice(h->device));
         AINV_HIP_CHECK(hipStreamCreate(&sv->stream));
         build_solver(sv, H->lv[l]->a);
       } catch (...) {
@@ -569,6 +575,7 @@ void ml_build(ainv_solver *h, const HostCsr &a) {
         (void)hipStreamDestroy(sv->stream);
       }
       sv->stream = h->stream;
+      if (prev >= 0 && prev != h->device) (void)hipSetDevice(prev);
     };
     // a level whose thread cannot be started is built on this thread
     for (int l = 0; l < nl; ++l) {

