@@ -564,7 +564,8 @@ def main():
                          "coop": ("k_hll_spmm_coop", " 256,"),
                          "coop128": ("k_hll_spmm_coop", " 128,"),
                          "coop_q": ("k_hll_spmm_coopq", " 256,"),
-                         "coop128_q": ("k_hll_spmm_coopq", " 128,")}[kmap]
+                         "coop128_q": ("k_hll_spmm_coopq", " 128,"),
+                         "run": ("k_hll_spmm_run", " 256,")}[kmap]
         if first:
             return f"{kernel}<{vt}, {xt}, {tt}, {tt}, {slots},{extra} 0>"
         yt = "float" if y32 else "double"

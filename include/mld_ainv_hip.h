@@ -123,12 +123,14 @@ int ainv_get_kind(ainv_h h, int *kind);
  * mld_d_invt_solver_cseti.f90:55-62, mld_d_invt_solver_csetr.f90:55-62).
  * Keys: SUB_FILLIN, AINV_ALG, SUB_ILUTHRS, INV_FILLIN, INV_THRESH, plus
  * FACTOR_PREC ('F64'|'F32'|'BF16' or AINV_PREC_*) and APPLY_MAP
- * ('AUTO'|'SPLIT'|'ROW'|'COOP'|'COOP128'|'COOP_Q'|'COOP128_Q' or
- * -1/0/1/2/3/4/5: kernel mapping of block applies with a power-of-two
- * leading dimension, the _Q ones with four waves per 64-row slice; AUTO
- * times each for each of the two passes once per block width and keeps the
- * fastest; multilevel handles also take 'WIDE' or 6: one wavefront per row
- * for the level products that keep a CSR copy, COOP for the rest).  Unknown keys are passed to the base solver, which
+ * ('AUTO'|'SPLIT'|'ROW'|'COOP'|'COOP128'|'COOP_Q'|'COOP128_Q'|'RUN' or
+ * -1/0/1/2/3/4/5/6: kernel mapping of block applies with a power-of-two
+ * leading dimension, the _Q ones with four waves per 64-row slice, RUN the
+ * one whose row groups hand gathered x rows on along runs of consecutive
+ * diagonals and gather only the rows they do not hold; AUTO times each for
+ * each of the two passes once per block width and keeps the fastest;
+ * multilevel handles also take 'WIDE' or 7: one wavefront per row for the
+ * level products that keep a CSR copy, COOP for the rest).  Unknown keys are passed to the base solver, which
  * ignores them (info = 0). */
 int ainv_cseti(ainv_h h, const char *what, int val);
 int ainv_csetr(ainv_h h, const char *what, double val);

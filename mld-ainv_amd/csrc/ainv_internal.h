@@ -80,6 +80,7 @@ struct DevCsr {
 constexpr int kSlice = 64;
 struct Hll {
   int32_t n = 0;
+  int32_t ncols = 0;  // columns (rows of the x it multiplies)
   int32_t nslices = 0;
   int64_t nnz = 0;     // true entries
   int64_t padded = 0;  // stored entries including padding
@@ -148,12 +149,15 @@ constexpr int AINV_MAP_COOP128 = 3;
 // COOP / COOP128)
 constexpr int AINV_MAP_COOP_Q = 4;
 constexpr int AINV_MAP_COOP128_Q = 5;
+// row groups of consecutive rows that hand gathered x rows on along runs of
+// consecutive diagonals, gathering only what they do not hold
+constexpr int AINV_MAP_RUN = 6;
 // one wavefront per row over the matrix's CSR copy (Hll::cptr), for the
 // multilevel products whose rows are long and few
-constexpr int AINV_MAP_WIDE = 6;
-constexpr int kNumMaps = 7;
+constexpr int AINV_MAP_WIDE = 7;
+constexpr int kNumMaps = 8;
 // mappings of the factor applies (no CSR copy)
-constexpr int kNumApplyMaps = 6;
+constexpr int kNumApplyMaps = 7;
 // mappings the fused system-matrix product distinguishes
 constexpr int kNumSpmvMaps = 4;
 // The two apply passes may use different mappings: a packed pair.

@@ -80,9 +80,13 @@ template <> struct Val<double> {
   using acc = double;
   __device__ static double get(double v) { return v; }
 };
+// FP32 factors accumulate in FP32 (the factors carry FP32 rounding
+// already; the Krylov recurrences around the apply stay FP64).  FP32
+// accumulators take half the registers of FP64 ones, which lets the
+// 32-column gather kernels run 3 waves per SIMD.
 template <> struct Val<float> {
-  using acc = double;
-  __device__ static double get(float v) { return (double)v; }
+  using acc = float;
+  __device__ static float get(float v) { return v; }
 };
 template <> struct Val<uint16_t> {  // BF16 value, FP32 accumulation
   using acc = float;
@@ -581,7 +585,7 @@ template <typename XT, int K> struct Coop {
 // matrix bandwidth times the row width) then fit in its L2, at the price of
 // reading the factor words once per chunk.
 #ifndef AINV_COOP_WIDE_WAVES
-#define AINV_COOP_WIDE_WAVES 2
+#define AINV_COOP_WIDE_WAVES 3
 #endif
 template <typename XT, int K, int CB> struct CoopChunk {
   static constexpr int cap = CB / (int)sizeof(XT);
@@ -709,6 +713,37 @@ template <typename XT, int K, int CB, int QS> struct CoopSplit {
   static constexpr int SPB = kWavesPerBlock / QW;  // slices per block
 };
 
+// Epilogue of one lane's CPL columns c0.. of row r (columns at or beyond
+// the active width kw are not written).
+template <typename DT, typename OT, int K, int CPL, int EPI, typename A>
+__device__ __forceinline__ void lane_store(int32_t r, int c0, int kw,
+                                           const A (&acc)[CPL],
+                                           const DT *__restrict__ dvec,
+                                           OT *__restrict__ out, double alpha,
+                                           double beta) {
+  OT *o = out + (int64_t)r * K + c0;
+  if constexpr (EPI == EPI_SCALE) {
+    const double d = (double)dvec[r];
+#pragma unroll
+    for (int u = 0; u < CPL; ++u)
+      if (c0 + u < kw) o[u] = (OT)(d * (double)acc[u]);
+  } else if constexpr (EPI == EPI_RESID) {
+    const DT *zr = dvec + (int64_t)r * K + c0;
+#pragma unroll
+    for (int u = 0; u < CPL; ++u)
+      if (c0 + u < kw)
+        o[u] = (OT)(alpha * (double)acc[u] + beta * (double)zr[u]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+      if (c0 + u >= kw) continue;
+      double v = alpha * (double)acc[u];
+      if (beta != 0.0) v += beta * (double)o[u];
+      o[u] = (OT)v;
+    }
+  }
+}
+
 template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
           int EPI, int QS>
 __device__ __forceinline__ void coop_spmm_body(
@@ -750,28 +785,9 @@ __device__ __forceinline__ void coop_spmm_body(
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int32_t r = s * kSlice + (q0 + q) * CP::R + rg;
-    if (r >= n) continue;
-    OT *o = out + (int64_t)r * K + c0;
-    if constexpr (EPI == EPI_SCALE) {
-      const double d = (double)dvec[r];
-#pragma unroll
-      for (int u = 0; u < CP::CPL; ++u)
-        if (c0 + u < kw) o[u] = (OT)(d * (double)acc[q][u]);
-    } else if constexpr (EPI == EPI_RESID) {
-      const DT *zr = dvec + (int64_t)r * K + c0;
-#pragma unroll
-      for (int u = 0; u < CP::CPL; ++u)
-        if (c0 + u < kw)
-          o[u] = (OT)(alpha * (double)acc[q][u] + beta * (double)zr[u]);
-    } else {
-#pragma unroll
-      for (int u = 0; u < CP::CPL; ++u) {
-        if (c0 + u >= kw) continue;
-        double v = alpha * (double)acc[q][u];
-        if (beta != 0.0) v += beta * (double)o[u];
-        o[u] = (OT)v;
-      }
-    }
+    if (r < n)
+      lane_store<DT, OT, K, CP::CPL, EPI>(r, c0, kw, acc[q], dvec, out, alpha,
+                                          beta);
   }
 }
 
@@ -800,6 +816,201 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coopq(
   coop_spmm_body<VT, XT, DT, OT, K, CB, EPI, 4>(n, nslices, sptr, col, val, x,
                                                 k, dvec, out, alpha, beta,
                                                 kwp);
+}
+
+// ---------------------------------------------------------------- runs
+// Run-reuse mapping (AINV_MAP_RUN).  Factors of stencil-like matrices hold
+// runs of consecutive diagonals: entry j + 1 of row r reads the x row that
+// entry j of row r + 1 read (col[r, j+1] = col[r+1, j]).  Here the L lanes
+// of a row group own L consecutive rows of the slice (row group g: rows
+// gL .. gL + L - 1, one pass each), so the x row a pass needs at entry
+// j + 1 is the one its next pass (or, for the group's last pass, the next
+// group's first pass: one shuffle) held at entry j.  Each pass compares the
+// column it needs with the column held there and takes the held values
+// when they match; only the others are gathered.  The gathers go through a
+// raw buffer resource over x: a lane that takes held values gets an
+// offset beyond the buffer, reads nothing and makes no memory request, so
+// inside a run only the slice's last row gathers.  Sums are formed in the
+// same entry order from the same values as in the cooperative mapping.
+constexpr uint32_t kNoLoad = 0xfffffff0u;  // out-of-range buffer offset
+// factor words fetched RC entries ahead
+constexpr int kRunChunk = 8;
+
+template <int VB>
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                         uint32_t (&w)[VB / 4]) {
+  if constexpr (VB == 16) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    w[0] = v[0];
+    w[1] = v[1];
+    w[2] = v[2];
+    w[3] = v[3];
+  } else if constexpr (VB == 8) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    w[0] = v[0];
+    w[1] = v[1];
+  } else {
+    w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+  }
+}
+
+// Largest byte size of an x block the run mapping addresses (32-bit buffer
+// offsets, kNoLoad beyond it).
+constexpr int64_t kRunMaxBytes = (int64_t)kNoLoad;
+
+template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
+          int EPI>
+__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_run(
+    int32_t n, int32_t ncols, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ kwp) {
+  using A = typename Val<VT>::acc;
+  using CH = CoopChunk<XT, K, CB>;
+  constexpr int KC = CH::KC;
+  using CP = Coop<XT, KC>;
+  constexpr int L = CP::L, CPL = CP::CPL, VB = CP::VB;
+  constexpr int NW = VB / 4;  // dwords per lane load
+  static_assert(CPL * (int)sizeof(XT) == VB, "lane load holds CPL values");
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int sub = lane % L, rg = lane / L;
+  const int rb = rg * L;                  // first slice row of the group
+  const bool last_group = rg == CP::R - 1;  // rows up to 63: no next group
+  const int kw = kwp ? min(k, *kwp) : k;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int32_t ngrp = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int32_t lb = __builtin_amdgcn_readfirstlane(
+      xcd_block(blockIdx.x, gridDim.x));
+  const int chunk = lb / ngrp;
+  if (chunk * KC >= kw) return;
+  const int32_t s = lb % ngrp * kWavesPerBlock + wv;
+  if (s >= nslices) return;
+  const int64_t base = sptr[s];
+  const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+  const int c0 = chunk * KC + sub * CPL;  // first column of this lane
+  const bool live = c0 < kw;
+  // x has ncols rows of K values (the launcher keeps this below
+  // kRunMaxBytes)
+  const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<XT *>(x), 0, (int)((int64_t)ncols * K * (int64_t)sizeof(XT)),
+      0x00020000);
+  const uint32_t lane_off = (uint32_t)(c0 * (int)sizeof(XT));
+  A acc[L][CPL];
+  uint32_t xv[L][NW];  // x values of each pass at the current entry
+  int32_t held[L];     // their columns (-1: nothing held)
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    held[q] = -1;
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) acc[q][u] = 0;
+#pragma unroll
+    for (int u = 0; u < NW; ++u) xv[q][u] = 0;
+  }
+  const int64_t pl = base + lane;
+  // factor words of RC entries at a time, the next RC requested while the
+  // current ones are used (entries past the width re-read the last one)
+  constexpr int RC = kRunChunk;
+  const int32_t last = width > 0 ? width - 1 : 0;
+  int32_t cwc[RC];
+  VT vwc[RC];
+#pragma unroll
+  for (int t = 0; t < RC; ++t) {
+    const int64_t q = pl + (int64_t)min(t, last) * kSlice;
+    cwc[t] = width > 0 ? col[q] : 0;
+    vwc[t] = width > 0 ? val[q] : VT(0);
+  }
+  for (int32_t j0 = 0; j0 < width; j0 += RC) {
+    int32_t cwn[RC];
+    VT vwn[RC];
+#pragma unroll
+    for (int t = 0; t < RC; ++t) {
+      const int64_t q = pl + (int64_t)min(j0 + RC + t, last) * kSlice;
+      cwn[t] = col[q];
+      vwn[t] = val[q];
+    }
+#pragma unroll
+    for (int t = 0; t < RC; ++t) {
+    if (j0 + t >= width) break;
+    const int32_t cw = cwc[t];
+    const VT vw = vwc[t];
+    // candidates: pass q + 1's values (the next group's pass 0 for the
+    // last pass)
+    const int32_t hn = __shfl(held[0], lane + L);
+    uint32_t xn[NW];
+#pragma unroll
+    for (int u = 0; u < NW; ++u) xn[u] = __shfl(xv[0][u], lane + L);
+    int32_t need[L];
+    bool take[L];
+    uint32_t ld[L][NW];
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+      need[q] = __shfl(cw, rb + q);
+      const int32_t cand = q + 1 < L ? held[q + 1] : (last_group ? -1 : hn);
+      take[q] = need[q] == cand;
+      const bool load = !take[q] && live;
+      const uint32_t off =
+          load ? (uint32_t)need[q] * (uint32_t)(K * (int)sizeof(XT)) + lane_off
+               : kNoLoad;
+      buf_load<VB>(xr, off, ld[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < L; ++q) {
+#pragma unroll
+      for (int u = 0; u < NW; ++u) {
+        const uint32_t c = q + 1 < L ? xv[q + 1][u] : xn[u];
+        xv[q][u] = take[q] ? c : ld[q][u];
+      }
+      held[q] = need[q];
+      const A v = Val<VT>::get(shfl_raw(vw, rb + q));
+      XT e[CPL];
+      __builtin_memcpy(e, xv[q], VB);
+#pragma unroll
+      for (int u = 0; u < CPL; ++u) acc[q][u] += v * cvt<A>(e[u]);
+    }
+    }
+#pragma unroll
+    for (int t = 0; t < RC; ++t) {
+      cwc[t] = cwn[t];
+      vwc[t] = vwn[t];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < L; ++q) {
+    const int32_t r = s * kSlice + rb + q;
+    if (r < n)
+      lane_store<DT, OT, K, CPL, EPI>(r, c0, kw, acc[q], dvec, out, alpha,
+                                      beta);
+  }
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int EPI>
+void launch_run(const Hll &a, const XT *x, int k, const DT *d, OT *out,
+                double alpha, double beta, hipStream_t s, const int32_t *kw) {
+  if (a.nslices == 0) return;
+  const int64_t *sp = a.sptr.as<int64_t>();
+  const int32_t *cl = a.col.as<int32_t>();
+  const VT *vl = a.val.as<VT>();
+  const int ngrp = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+#define AINV_LAUNCH(KK)                                                     \
+  hipLaunchKernelGGL((k_hll_spmm_run<VT, XT, DT, OT, KK, 256, EPI>),        \
+                     dim3(ngrp * CoopChunk<XT, KK, 256>::NCH), dim3(kBlock), \
+                     0, s, a.n, a.ncols, a.nslices, sp, cl, vl, x, k, d, out, \
+                     alpha, beta, kw)
+  if (k <= 2)
+    AINV_LAUNCH(2);
+  else if (k <= 4)
+    AINV_LAUNCH(4);
+  else if (k <= 8)
+    AINV_LAUNCH(8);
+  else if (k <= 16)
+    AINV_LAUNCH(16);
+  else if (k <= 32)
+    AINV_LAUNCH(32);
+  else
+    AINV_LAUNCH(64);
+#undef AINV_LAUNCH
+  AINV_HIP_CHECK(hipGetLastError());
 }
 
 // Fused system-matrix product + dots in the cooperative mapping (ld = K).
@@ -1051,6 +1262,11 @@ void launch_lane(int map, const Hll &a, const XT *x, int k, const DT *d,
   else if (map == AINV_MAP_COOP128_Q)
     launch_coop<VT, XT, DT, OT, 128, EPI, 4>(a, x, k, d, out, alpha, beta, s,
                                              kw);
+  else if (map == AINV_MAP_RUN && a.ncols > 0 &&
+           (int64_t)a.ncols * 64 * (int64_t)sizeof(XT) < kRunMaxBytes)
+    launch_run<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s, kw);
+  else if (map == AINV_MAP_RUN)  // x too large for 32-bit buffer offsets
+    launch_coop<VT, XT, DT, OT, 256, EPI>(a, x, k, d, out, alpha, beta, s, kw);
   else if (map == AINV_MAP_WIDE)
     launch_wide<VT, XT, DT, OT, EPI>(a, x, k, d, out, alpha, beta, s, kw);
   else
@@ -1071,7 +1287,7 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
   // the fused product + dots has no quarter-slice or wide variant
-  if (map == AINV_MAP_COOP_Q) map = AINV_MAP_COOP;
+  if (map == AINV_MAP_COOP_Q || map == AINV_MAP_RUN) map = AINV_MAP_COOP;
   if (map == AINV_MAP_COOP128_Q || map == AINV_MAP_WIDE)
     map = AINV_MAP_COOP128;
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
@@ -1184,7 +1400,8 @@ int hll_spmv_dot(const Hll &a, const float *x, double *y, int k, int64_t ld,
 // output.
 static bool is_coop(int map) {
   return map == AINV_MAP_COOP || map == AINV_MAP_COOP128 ||
-         map == AINV_MAP_COOP_Q || map == AINV_MAP_COOP128_Q;
+         map == AINV_MAP_COOP_Q || map == AINV_MAP_COOP128_Q ||
+         map == AINV_MAP_RUN;
 }
 
 static bool use_row(int map, int k, int64_t ldx, int64_t ldo, const void *x,

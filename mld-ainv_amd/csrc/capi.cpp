@@ -347,11 +347,12 @@ int map_from_string(const std::string &v) {
   if (v == "COOP128") return AINV_MAP_COOP128;
   if (v == "COOP_Q") return AINV_MAP_COOP_Q;
   if (v == "COOP128_Q") return AINV_MAP_COOP128_Q;
+  if (v == "RUN") return AINV_MAP_RUN;
   if (v == "WIDE") return AINV_MAP_WIDE;
   throw Error(AINV_ERR_ARG,
               "unknown APPLY_MAP '" + v +
                   "' (AUTO, SPLIT, ROW, COOP, COOP128, COOP_Q, COOP128_Q, "
-                  "WIDE)");
+                  "RUN, WIDE)");
 }
 
 const char *map_name(int m) {
@@ -360,6 +361,7 @@ const char *map_name(int m) {
          : m == AINV_MAP_COOP128 ? "coop128"
          : m == AINV_MAP_COOP_Q    ? "coop_q"
          : m == AINV_MAP_COOP128_Q ? "coop128_q"
+         : m == AINV_MAP_RUN       ? "run"
          : m == AINV_MAP_WIDE      ? "wide"
                                  : "split";
 }

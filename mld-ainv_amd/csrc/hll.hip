@@ -263,6 +263,7 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
   const int vb = value_bytes(prec);
   h = Hll();
   h.n = a.n;
+  h.ncols = ncols;
   h.prec = prec;
   h.nnz = a.nnz;
   h.nslices = (a.n + kSlice - 1) / kSlice;

@@ -848,8 +848,9 @@ std::string ml_report(const ainv_solver *h) {
   }
   // tuned mappings of the level products per block width
   s += "], \"maps\": [";
-  static const char *names[kNumMaps] = {"split",  "row",       "coop", "coop128",
-                                        "coop_q", "coop128_q", "wide"};
+  static const char *names[kNumMaps] = {"split",  "row",       "coop",
+                                        "coop128", "coop_q",   "coop128_q",
+                                        "run",     "wide"};
   auto nm = [&](int8_t m) { return m < 0 ? "-" : names[m]; };
   bool first = true;
   for (int i = 0; i < 8; ++i)

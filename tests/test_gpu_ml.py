@@ -104,6 +104,27 @@ def test_cycle_matches_oracle(gpu, O, ml_type, pos, sweeps, smoothed):
             assert err < 1e-11, (k, j, err)
 
 
+@pytest.mark.parametrize("mapping", ["ROW", "COOP", "COOP128", "COOP_Q",
+                                     "RUN"])
+def test_cycle_every_forced_mapping(gpu, O, mapping):
+    """Every level product (the square level matrices and factors, and the
+    rectangular P, whose x has fewer rows than P, and R, whose x has more)
+    in each block mapping: the cycle matches the oracle's.  A mapping that
+    sized its reads of x by the product's row count would read past x
+    (P) or miss rows of it (R)."""
+    a = gpu.gen_pde3d(16, 1)
+    n = a.shape[0]
+    pv, ref = build_ml(gpu, O, a, "INVK", csweeps=3)
+    pv.set("APPLY_MAP", mapping)
+    X = gpu.gen_normal(15, (n, 32))
+    for k in (4, 16, 32):
+        Y = pv.apply(np.ascontiguousarray(X[:, :k])).reshape(n, k)
+        for j in (0, k - 1):
+            yr = ref.apply(X[:, j])
+            err = np.linalg.norm(Y[:, j] - yr) / np.linalg.norm(yr)
+            assert err < 1e-11, (mapping, k, j, err)
+
+
 def test_cycle_alpha_beta_and_device_arrays(gpu, O):
     a = gpu.gen_pde3d(10)
     n = a.shape[0]
@@ -332,7 +353,7 @@ def test_wide_mapping_parity(gpu, O, k):
     pv.build(a)
     ref = O.MlHierarchy(a, nlev=5, theta=0.1, sub=sub, csweeps=2)
     pv.set("APPLY_MAP", "WIDE")
-    assert pv.geti("APPLY_MAP") == 6
+    assert pv.geti("APPLY_MAP") == 7
     X = gpu.gen_normal(23, (n, k))
     Y = pv.apply(X)
     for j in range(0, k, 3):
@@ -360,7 +381,8 @@ def test_tuner_report_lists_level_mappings(gpu, O):
                itmax=50)
     maps = [e for e in pv.report()["multilevel"]["maps"] if e["K"] == 16]
     assert len(maps) == pv.levels()
-    names = {"split", "row", "coop", "coop128", "coop_q", "coop128_q", "wide"}
+    names = {"split", "row", "coop", "coop128", "coop_q", "coop128_q",
+             "run", "wide"}
     for e in maps:
         assert e["a"] in names
     assert any(e["wide"] for e in maps)

@@ -141,7 +141,7 @@ def test_block_apply_matches_columns(gpu, O, k):
 @pytest.mark.parametrize("prec,tol", [("F64", 1e-12), ("F32", 2e-6),
                                       ("BF16", 2e-2)])
 @pytest.mark.parametrize("k", [2, 4, 8, 16, 32, 64])
-@pytest.mark.parametrize("mapping", ["ROW", "COOP", "COOP128"])
+@pytest.mark.parametrize("mapping", ["ROW", "COOP", "COOP128", "RUN"])
 def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
     """The row-per-lane and cooperative kernels (leading dimension a
     power of two) against the oracle column by column, alpha / beta
@@ -149,7 +149,8 @@ def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
     a = gpu.gen_pde3d(10)
     sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC=prec,
                APPLY_MAP=mapping)
-    assert sv.geti("APPLY_MAP") == {"ROW": 1, "COOP": 2, "COOP128": 3}[mapping]
+    assert sv.geti("APPLY_MAP") == {"ROW": 1, "COOP": 2, "COOP128": 3,
+                                    "RUN": 6}[mapping]
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
     n = a.shape[0]
     x = gpu.gen_normal(7, (n, k))
@@ -164,11 +165,50 @@ def test_row_mapping_apply(gpu, O, prec, tol, k, mapping):
     assert np.linalg.norm(ys - y) <= 4 * tol * np.linalg.norm(ys)
 
 
+@pytest.mark.parametrize("prec", ["F64", "F32", "BF16"])
+@pytest.mark.parametrize("k", [1, 8, 32, 64])
+def test_run_mapping_bit_identical(gpu, O, prec, k):
+    """RUN (row groups of consecutive rows hand gathered x rows on along
+    runs of consecutive diagonals and gather only the others) on the 7-point
+    stencil's INVK factors, whose rows hold runs of 2-3 diagonals: the same
+    products summed in the same order as the cooperative mapping, so the
+    results agree bit for bit (both transposes, alpha / beta), and with the
+    oracle to the storage precision.  Rows near the grid faces break the
+    runs (held columns differ), so both branches of the hand-on run."""
+    a = gpu.gen_pde3d(24)
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC=prec,
+               APPLY_MAP="RUN")
+    n = a.shape[0]
+    x = gpu.gen_normal(9, (n, k))
+    y0 = gpu.gen_normal(10, (n, k))
+    if k == 1:
+        x, y0 = x[:, 0].copy(), y0[:, 0].copy()
+    outs = {}
+    for mapping in ("RUN", "COOP"):
+        sv.set("APPLY_MAP", mapping)
+        outs[mapping] = [sv.apply(x, y=y0.copy(), alpha=0.75, beta=-0.5,
+                                  trans=t) for t in ("N", "T")]
+    for yr, yc in zip(outs["RUN"], outs["COOP"]):
+        assert np.array_equal(yr, yc)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    tol = {"F64": 1e-12, "F32": 2e-6, "BF16": 2e-2}[prec]
+    X = x.reshape(n, -1)
+    Y0 = y0.reshape(n, -1)
+    Y = outs["RUN"][0].reshape(n, -1)
+    YT = outs["RUN"][1].reshape(n, -1)
+    for c in {0, X.shape[1] - 1}:
+        ref = O.apply(w, z, d, X[:, c], alpha=0.75, beta=-0.5, y=Y0[:, c])
+        assert np.linalg.norm(Y[:, c] - ref) <= tol * np.linalg.norm(ref)
+        ref = O.apply(w, z, d, X[:, c], alpha=0.75, beta=-0.5, y=Y0[:, c],
+                      trans="T")
+        assert np.linalg.norm(YT[:, c] - ref) <= tol * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("method,k,mapping", [
     ("BICGSTAB", 8, "ROW"), ("BICGSTAB", 5, "ROW"), ("CG", 16, "ROW"),
     ("BICGSTAB", 8, "COOP"), ("BICGSTAB", 3, "COOP"), ("CG", 32, "COOP"),
     ("BICGSTAB", 40, "COOP"), ("BICGSTAB", 64, "COOP128"),
-    ("CG", 32, "COOP128")])
+    ("CG", 32, "COOP128"), ("BICGSTAB", 32, "RUN"), ("CG", 16, "RUN")])
 def test_row_mapping_krylov(gpu, O, method, k, mapping):
     """Krylov solves with the row-mapped apply and system-matrix product
     (a padded leading dimension when k is not a power of two): iteration
@@ -208,7 +248,7 @@ def test_autotuner_report(gpu):
     for pass_, key in (("w_ms", "map_w"), ("z_ms", "map_z")):
         ms = t[pass_]
         assert set(ms) == {"split", "row", "coop", "coop128", "coop_q",
-                           "coop128_q"}
+                           "coop128_q", "run"}
         assert min(ms.values()) > 0
         assert t[key] == min(ms, key=ms.get)
 
