@@ -584,22 +584,33 @@ template <typename XT, int K> struct Coop {
 // x rows an XCD gathers between the first and the last use of a row (the
 // matrix bandwidth times the row width) then fit in its L2, at the price of
 // reading the factor words once per chunk.
-#ifndef AINV_COOP_WIDE_WAVES
-#define AINV_COOP_WIDE_WAVES 3
-#endif
 template <typename XT, int K, int CB> struct CoopChunk {
   static constexpr int cap = CB / (int)sizeof(XT);
   static constexpr int KC = K < cap ? K : cap;
   static constexpr int NCH = K / KC;
-  // KC >= 32 accumulators per lane: fewer waves per SIMD instead of spilling
-  static constexpr int waves = KC >= 32 ? AINV_COOP_WIDE_WAVES : AINV_ROW_WAVES;
 };
+// Waves per SIMD of the gather kernels: chunks of 32 columns or more hold
+// 32+ accumulators per lane, so fewer waves instead of spilling -- 3 with
+// FP32 sums (FP32 / BF16 factors), 2 with FP64 sums.
+template <typename VT, typename XT, int K, int CB> struct CoopWaves {
+  static constexpr int v =
+      CoopChunk<XT, K, CB>::KC >= 32
+          ? (sizeof(typename Val<VT>::acc) == 4 ? 3 : 2)
+          : AINV_ROW_WAVES;
+};
+// The fused system-matrix product keeps FP64 sums of its FP64 matrix: at
+// 32-column chunks it needs the registers of 2 waves per SIMD (measured at
+// 3: 1.04 against 0.80 ms per launch at 160^3).
 #if AINV_ROW_WAVES > 0
 #define AINV_COOP_ATTR(XT)                                                  \
-  __attribute__((amdgpu_waves_per_eu(CoopChunk<XT, K, CB>::waves,           \
-                                     CoopChunk<XT, K, CB>::waves)))
+  __attribute__((amdgpu_waves_per_eu(CoopWaves<VT, XT, K, CB>::v,           \
+                                     CoopWaves<VT, XT, K, CB>::v)))
+#define AINV_DOT_ATTR(XT)                                                   \
+  __attribute__((amdgpu_waves_per_eu(CoopWaves<double, XT, K, CB>::v,       \
+                                     CoopWaves<double, XT, K, CB>::v)))
 #else
 #define AINV_COOP_ATTR(XT)
+#define AINV_DOT_ATTR(XT)
 #endif
 
 // Slice positions per trip of the cooperative loop.  For chunks of 32
@@ -1020,7 +1031,7 @@ void launch_run(const Hll &a, const XT *x, int k, const DT *d, OT *out,
 // shuffles across the row groups, then across the waves in LDS; a block
 // writes zero partial sums for the columns of the other chunks.
 template <typename XT, int K, int CB, int NP>
-__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void
+__global__ __launch_bounds__(kBlock) AINV_DOT_ATTR(XT) void
 k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
                     const int64_t *__restrict__ sptr,
                     const int32_t *__restrict__ col,

@@ -788,17 +788,31 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
       ml_tune(m, K, p->k, v.b, v.z, false, p->tbuf.p, s);
   } else if (m) {
     if (m->apply_map[li] < 0) {
-      // the two passes are timed separately; t keeps leading dimension K
+      // the two passes are timed separately, on the vectors the iteration
+      // hands the preconditioner: with FP32 / BF16 factors BiCGSTAB and CG
+      // apply it to FP32 copies (p / s, r) into FP32 outputs (f / z);
+      // t keeps leading dimension K
       const bool f32t = m->prec != AINV_PREC_F64;
+      const Vecs w32 = x32_vecs(p);
+      const float *x32 = w32.p32 ? w32.p32 : w32.r32;
+      float *y32 = w32.p32 ? w32.f32 : w32.z32;
       int bw = 0, bz = 0;
       for (int map = 0; map < kNumApplyMaps; ++map) {
         m->apply_ms[li][0][map] = time_ms(s, [&] {
-          hll_spmm_scale(m->hw, m->dv.p, v.b, false, K, p->tbuf.p, K, p->k, s,
-                         map);
+          if (x32)
+            hll_spmm_scale(m->hw, m->dv.p, x32, true, K, p->tbuf.p, K, p->k,
+                           s, map);
+          else
+            hll_spmm_scale(m->hw, m->dv.p, v.b, false, K, p->tbuf.p, K, p->k,
+                           s, map);
         }, 3);
         m->apply_ms[li][1][map] = time_ms(s, [&] {
-          hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, K, 0.0, v.z, K, p->k, s,
-                         map);
+          if (x32)
+            hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, true, K, 0.0, y32, K, p->k,
+                           s, map);
+          else
+            hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, K, 0.0, v.z, K, p->k,
+                           s, map);
         }, 3);
         if (m->apply_ms[li][0][map] < m->apply_ms[li][0][bw]) bw = map;
         if (m->apply_ms[li][1][map] < m->apply_ms[li][1][bz]) bz = map;
@@ -808,10 +822,16 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
     p->apply_map = m->apply_map[li];
   }
   if (a->spmv_map[li] < 0) {
+    // BiCGSTAB with FP32 / BF16 factors multiplies the FP32 preconditioned
+    // vectors (f, z): time that form
+    const float *f32 = p->method == KM_BICGSTAB ? x32_vecs(p).f32 : nullptr;
     int best = 0;
     for (int map = 0; map < kNumSpmvMaps; ++map) {
       a->spmv_ms[li][map] = time_ms(s, [&] {
-        hll_spmv_dot(a->hll, v.b, v.z, p->k, K, v.r, 1, p->pt, map, s);
+        if (f32)
+          hll_spmv_dot(a->hll, f32, v.z, p->k, K, v.r, 1, p->pt, map, s);
+        else
+          hll_spmv_dot(a->hll, v.b, v.z, p->k, K, v.r, 1, p->pt, map, s);
       }, 3);
       if (a->spmv_ms[li][map] < a->spmv_ms[li][best]) best = map;
     }
