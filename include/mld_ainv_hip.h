@@ -295,6 +295,24 @@ int ainv_host_ilu_fact(int32_t n, const int64_t *irp, const int32_t *ja,
  * host pointer with flags = AINV_HOST_PTRS. */
 int ainv_update_a(ainv_h h, const double *x, int64_t nx, int flags);
 
+/* The apply of ainv_apply without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as ainv_apply with device
+ * pointers; the intermediate t of the two passes lives in a 64 MiB device
+ * array the library holds once per GPU until the process ends (its address
+ * taken at ainv_init or at the first call on a device), so n x K x 4 bytes
+ * (FP32 / BF16 factors; x 8 with FP64 factors; K = k rounded up to a power
+ * of two when ldx = ldy = K, else k) must fit in it (larger:
+ * AINV_ERR_CAPACITY).  Refused (AINV_ERR_ARG / AINV_ERR_STATE) where
+ * ainv_apply would allocate: host pointers, the multilevel kind, and a
+ * 'T' / 'C' apply before ainv_apply has built the transposed layout.
+ * Every call on one GPU shares its array: calls that overlap on a GPU
+ * (other threads or streams) race and must be serialized by the caller. */
+int ainv_apply_noalloc(ainv_h h, double alpha, const double *x, double beta,
+                       double *y, char trans, int k, int64_t ldx, int64_t ldy,
+                       void *stream);
+
 /* Timing and layout report of the last build (JSON text). */
 int ainv_report(ainv_h h, char *buf, size_t len);
 

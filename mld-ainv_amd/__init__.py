@@ -69,6 +69,8 @@ def lib():
         "ainv_cnv": [vp, C.c_int],
         "ainv_apply": [vp, C.c_double, vp, C.c_double, vp, C.c_char, C.c_int,
                        C.c_int64, C.c_int64, C.c_int, vp],
+        "ainv_apply_noalloc": [vp, C.c_double, vp, C.c_double, vp, C.c_char,
+                               C.c_int, C.c_int64, C.c_int64, vp],
         "ainv_get_nzeros": [vp, i64p],
         "ainv_sizeof": [vp, i64p],
         "ainv_get_wrksz": [vp, P(C.c_int)],

@@ -575,6 +575,12 @@ int ainv_init(int device) {
       throw Error(AINV_ERR_ARG, "device index out of range");
     set_device(device);
     g_device = device;
+    // the no-allocation apply's scratch address; a failure here is reported
+    // by that entry point's first call instead
+    try {
+      noalloc_setup(device);
+    } catch (const Error &) {
+    }
   });
 }
 

@@ -121,6 +121,9 @@ void solver_apply_into_t(const ainv_solver *h, const double *x, double *y,
 // Index of K = 2^i and the mapping pair the tuner chose for it (split if
 // none, the forced mapping for both passes under APPLY_MAP).
 int log2_pow2(int K);
+// The device address of this GPU's scratch array of the no-allocation
+// apply, taken once per device (noalloc.hip); the device must be current.
+void *noalloc_setup(int device);
 int tuned_apply_map(const ainv_solver *h, int K);
 void reset_tuning(ainv_solver *h);
 size_t solver_t_bytes(const ainv_solver *h, int k);
