@@ -365,6 +365,10 @@ def main():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-ml", action="store_true",
                    help="skip the multilevel leg (C1, C2, C3)")
+    p.add_argument("--no-invt", action="store_true",
+                   help="skip C2's INVT leg")
+    p.add_argument("--no-c5", action="store_true",
+                   help="skip the serving-stream leg of the default line")
     p.add_argument("--ml-levels", type=int, default=0,
                    help="levels of the multilevel leg (0: the configuration's)")
     p.add_argument("--ml-only", action="store_true",
@@ -596,12 +600,23 @@ def main():
     if not args.no_ml and args.config in ML_CONFIGS:
         ml = ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps,
                     slots, M, fp64_rate, it64)
+    invt = None
+    if args.config == "C2" and not args.no_invt:
+        invt = invt_leg(m, args, a, A, dB, Bh, sample, method, eps, slots, M)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sv, a, Bh[sample[0]], method, args.cpu_iters,
                            float(np.mean(it64)))
         log(f"cpu baseline {cpu}")
+
+    # the serving stream over the 24-matrix pool (its own matrices; the
+    # queue above is released first)
+    c5 = None
+    if args.config == "C2" and not args.no_c5:
+        dX.free()
+        dB.free()
+        c5 = c5_compact(c5_run(m, args, burst=False))
 
     lat_ms = np.array(lat)
     out = {
@@ -662,11 +677,20 @@ def main():
         "build": {"seconds": t_build},
         "cpu_baseline": cpu,
         "multilevel": ml,
+        "invt": invt,
+        "c5": c5,
     }
     print(json.dumps(out))
 
 
 def serve_bench(m, args):
+    print(json.dumps(c5_run(m, args, burst=True)))
+
+
+def c5_run(m, args, burst=True):
+    """The serving stream: first-touch builds of the 24-matrix pool, an
+    optional burst, the closed-loop capacity and the open loop (Poisson
+    arrivals at 0.8x that capacity); returns the record."""
     import threading
     rng = np.random.default_rng(5)
     pool = []
@@ -715,15 +739,19 @@ def serve_bench(m, args):
             raise SystemExit(f"C5: a request ended at ||r||/||b|| = {worst}")
 
     # burst: every request submitted at once (the most batching there is)
-    burst = requests(max(24, args.c5_requests // 2), 6)
-    t0 = time.time()
-    tk = [srv.submit(ids[j], b, eps=1e-8, itmax=2000) for j, _, b in burst]
-    res = [srv.wait(t) for t in tk]
-    t_burst = time.time() - t0
-    check(res)
-    burst_req = len(burst) / t_burst
-    burst_cols = sum(k for _, k, _ in burst) / t_burst
-    log(f"burst: {burst_req:.2f} requests/s, {burst_cols:.1f} columns/s")
+    burst_req = burst_cols = None
+    nburst = 0
+    if burst:
+        bl = requests(max(24, args.c5_requests // 2), 6)
+        nburst = len(bl)
+        t0 = time.time()
+        tk = [srv.submit(ids[j], b, eps=1e-8, itmax=2000) for j, _, b in bl]
+        res = [srv.wait(t) for t in tk]
+        t_burst = time.time() - t0
+        check(res)
+        burst_req = len(bl) / t_burst
+        burst_cols = sum(k for _, k, _ in bl) / t_burst
+        log(f"burst: {burst_req:.2f} requests/s, {burst_cols:.1f} columns/s")
     st_burst = srv.stats()
 
     # capacity: closed loop keeping `window` requests outstanding, each new
@@ -832,7 +860,7 @@ def serve_bench(m, args):
                      "note": f"closed loop, {window} requests outstanding, "
                              f"{len(closed)} requests"},
         "burst": {"requests_per_s": burst_req, "columns_per_s": burst_cols,
-                  "requests": len(burst)},
+                  "requests": nburst},
         "arrival_rate_per_s": lam,
         "first_touch_seconds": t_warm,
         "server_stats": st,
@@ -840,8 +868,123 @@ def serve_bench(m, args):
                    "closed_loop": phase(st_burst, st_closed),
                    "open_loop": phase(st_closed, st)},
         "roofline": None, "cpu_baseline": None,
+        "true_residual_max": max(float(np.max(e)) for _, _, e, _ in results),
     }
-    print(json.dumps(out))
+    return out
+
+
+def c5_compact(rec):
+    """The serving stream's figures for the default benchmark line."""
+    return {
+        "workload": rec["config"]["workload"],
+        "requests": rec["config"]["requests"],
+        "columns": rec["config"]["columns"],
+        "first_touch_seconds": rec["first_touch_seconds"],
+        "capacity": rec["capacity"],
+        "open_loop": {"solves_per_s": rec["value"],
+                      "unit": "right-hand sides per second",
+                      "arrival_rate_requests_per_s":
+                          rec["arrival_rate_per_s"]},
+        "latency_ms": rec["latency_ms"],
+        "true_residual_max": rec["true_residual_max"],
+        "peak_resident_bytes": rec["server_stats"]["peak_resident_bytes"],
+        "evictions": rec["server_stats"]["evictions"],
+    }
+
+
+INVT_PARAMS = dict(SUB_FILLIN=10, SUB_ILUTHRS=1e-2, INV_FILLIN=10,
+                   INV_THRESH=1e-2)
+
+
+def invt_leg(m, args, a, A, dB, Bh, sample, method, eps, slots, M):
+    """C2's INVT preconditioner (ILUT fill 10, drop 1e-2 with maxval
+    scaling; inverse fill 10, inverse drop 1e-2) on the same matrix and
+    queue: FP64 single-RHS solves, the served form with --prec factors and
+    its apply passes against the roofline."""
+    n = a.shape[0]
+    sv = m.SOLVERS["INVT"]()
+    for key, v in INVT_PARAMS.items():
+        sv.set(key, v)
+    sv.set("FACTOR_PREC", "F64")
+    t0 = time.time()
+    sv.build(a)
+    t_build = time.time() - t0
+    dx1 = m.DeviceArray((n,))
+
+    def single(j):
+        db1 = m.DeviceArray.from_numpy(Bh[j])
+        dx1.zero()
+        m.synchronize()
+        ts = time.time()
+        _, it, _ = m.krylov(method, A, sv, db1, dx1, eps=eps, itmax=5000)
+        m.synchronize()
+        dt = time.time() - ts
+        db1.free()
+        return dt, int(it[0])
+
+    sub = sample[:3]
+    single(sub[0])
+    ts, its = [], []
+    for j in sub:
+        dt, it = single(j)
+        ts.append(dt)
+        its.append(it)
+    x64 = dx1.to_numpy()
+    dx1.free()
+    rate64 = len(ts) / sum(ts)
+    true64 = float(np.linalg.norm(Bh[sub[-1]] - a @ x64) /
+                   np.linalg.norm(Bh[sub[-1]]))
+    sv.cnv(args.prec)
+    lay = sv.layout()
+    dX = m.DeviceArray((M, n))
+    warm = m.DeviceArray.from_numpy(Bh[:min(slots, M)])
+    m.krylov_stream(method, A, sv, warm, slots=slots, eps=eps, itmax=5000)
+    m.synchronize()
+    warm.free()
+    t_start = time.time()
+    _, iters, _, lat = m.krylov_stream(method, A, sv, dB, dX, slots=slots,
+                                       eps=eps, itmax=5000)
+    m.synchronize()
+    el = time.time() - t_start
+    Xs = dX.to_numpy()
+    dX.free()
+    rate = M / el
+    true = max(float(np.linalg.norm(Bh[j] - a @ Xs[j]) /
+                     np.linalg.norm(Bh[j])) for j in range(0, M,
+                                                           max(1, M // 16)))
+    xdiff = float(np.linalg.norm(Xs[sub[-1]] - x64) / np.linalg.norm(x64))
+    x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
+    ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32, x32)
+    bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
+    bz = apply_pass_bytes(lay, slots, False, 8, 4 if x32 else 8)
+    rep = sv.report()
+    tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
+    dom_w = ms_w >= ms_z
+    gbs = (bw / (ms_w * 1e-3) if dom_w else bz / (ms_z * 1e-3)) / 1e9
+    log(f"INVT: build {t_build:.2f}s, FP64 single {rate64:.2f} solves/s "
+        f"({its}), served {rate:.2f} solves/s, apply {ms_w:.3f}+{ms_z:.3f}"
+        f" ms")
+    return {
+        "params": INVT_PARAMS, "method": method, "build_s": t_build,
+        "nnz_w": lay["nnz_w"], "nnz_z": lay["nnz_z"],
+        "fp64_single_rhs": {"solves_per_s": rate64, "iterations": its,
+                            "true_residual": true64},
+        "served": {"solves_per_s": rate, "factor_prec": args.prec,
+                   "slots": slots, "queue": M,
+                   "iterations_mean": float(np.mean(iters)),
+                   "iterations_max": int(np.max(iters)),
+                   "latency_ms_p50": float(np.percentile(lat, 50)),
+                   "latency_ms_p99": float(np.percentile(lat, 99)),
+                   "true_residual_max": true,
+                   "error_vs_fp64": xdiff},
+        "speedup_vs_fp64_single": rate / rate64,
+        "apply": {"ms_first": ms_w, "ms_second": ms_z,
+                  "bytes_first": bw, "bytes_second": bz,
+                  "maps": tuned[0] if tuned else None},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "pass": "first" if dom_w else "second"},
+    }
 
 
 def profile_apply(m, sv, n, k, reps, x32=False, y32=False):
