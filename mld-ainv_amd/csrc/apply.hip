@@ -119,7 +119,7 @@ __device__ __forceinline__ typename Val<VT>::acc row_dot(
   // outstanding load at the block boundary): lanes of an idle column read
   // column 0 and discard the sum, and the last group re-reads its own words
   const int cc = col_ok ? c : 0;
-  A acc0 = 0, acc1 = 0;
+  A acc0 = 0;
   int32_t j = 0;
   if (len >= kGroup) {
     int64_t q = p0;
@@ -145,11 +145,10 @@ __device__ __forceinline__ typename Val<VT>::acc row_dot(
         cn[u] = col[q + u * kSlice];
         vn[u] = Val<VT>::get(val[q + u * kSlice]);
       }
+      // one sum in entry order, as every other mapping forms it: a
+      // column's result does not depend on the mapping or block width
 #pragma unroll
-      for (int u = 0; u < kGroup; u += 2) {
-        acc0 += vi[u] * xi[u];
-        if (u + 1 < kGroup) acc1 += vi[u + 1] * xi[u + 1];
-      }
+      for (int u = 0; u < kGroup; ++u) acc0 += vi[u] * xi[u];
       j = jn;
       if (!more) break;
 #pragma unroll
@@ -163,7 +162,7 @@ __device__ __forceinline__ typename Val<VT>::acc row_dot(
     const int64_t q = p0 + (int64_t)j * kSlice;
     acc0 += Val<VT>::get(val[q]) * cvt<A>(x[(int64_t)col[q] * ldx + cc]);
   }
-  return acc0 + acc1;
+  return acc0;
 }
 
 // RESID: out[r, c] = alpha * acc + beta * z[r, c], z passed as the d

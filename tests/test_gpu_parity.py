@@ -190,6 +190,14 @@ def test_run_mapping_bit_identical(gpu, O, prec, k):
                                   trans=t) for t in ("N", "T")]
     for yr, yc in zip(outs["RUN"], outs["COOP"]):
         assert np.array_equal(yr, yc)
+    if k > 1:
+        # every mapping sums a row's entries in entry order: a column alone
+        # (the split mapping at k = 1) equals the same column in the block
+        for c in (0, k - 1):
+            y1 = sv.apply(np.ascontiguousarray(x[:, c]),
+                          y=np.ascontiguousarray(y0[:, c]), alpha=0.75,
+                          beta=-0.5)
+            assert np.array_equal(y1, outs["COOP"][0][:, c]), c
     w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
     tol = {"F64": 1e-12, "F32": 2e-6, "BF16": 2e-2}[prec]
     X = x.reshape(n, -1)
