@@ -12,6 +12,7 @@ module mld_ainv_hip_cbind
   integer(c_int), parameter :: ainv_prec_f64 = 0, ainv_prec_f32 = 1, &
        & ainv_prec_bf16 = 2
   integer(c_int), parameter :: ainv_host_ptrs = 1
+  integer(c_int), parameter :: ainv_h2d = 1, ainv_d2h = 2, ainv_d2d = 3
   integer(c_int), parameter :: ainv_err_state = 6
 
   interface
@@ -89,12 +90,13 @@ module mld_ainv_hip_cbind
       integer(c_int), value :: prec
       integer(c_int) :: ainv_cnv
     end function
+    ! x, y: device addresses, or host ones (c_loc) with ainv_host_ptrs
     function ainv_apply(h, alpha, x, beta, y, trans, k, ldx, ldy, flags, &
          & stream) bind(c, name='ainv_apply')
       import :: c_int, c_ptr, c_double, c_char, c_int64_t
       type(c_ptr), value :: h
       real(c_double), value :: alpha, beta
-      real(c_double) :: x(*), y(*)
+      type(c_ptr), value :: x, y
       character(kind=c_char), value :: trans
       integer(c_int), value :: k, flags
       integer(c_int64_t), value :: ldx, ldy
@@ -138,7 +140,7 @@ module mld_ainv_hip_cbind
     function ainv_update_a(h, x, nx, flags) bind(c, name='ainv_update_a')
       import :: c_int, c_ptr, c_double, c_int64_t
       type(c_ptr), value :: h
-      real(c_double) :: x(*)
+      type(c_ptr), value :: x
       integer(c_int64_t), value :: nx
       integer(c_int), value :: flags
       integer(c_int) :: ainv_update_a
@@ -172,13 +174,37 @@ module mld_ainv_hip_cbind
       import :: c_int, c_ptr, c_char, c_double, c_int32_t
       character(kind=c_char) :: method(*)
       type(c_ptr), value :: a, prec
-      real(c_double) :: b(*), x(*)
+      type(c_ptr), value :: b, x
       integer(c_int), value :: k, itmax, istop, irst, flags
       real(c_double), value :: eps
       integer(c_int32_t) :: iter(*)
       real(c_double) :: err(*)
       type(c_ptr), value :: stream
       integer(c_int) :: ainv_krylov
+    end function
+    ! device memory for vectors kept on the GPU (no counterpart in the
+    ! reference, where PSBLAS vectors own their storage)
+    function ainv_dev_alloc(bytes, p) bind(c, name='ainv_dev_alloc')
+      import :: c_int, c_ptr, c_size_t
+      integer(c_size_t), value :: bytes
+      type(c_ptr) :: p
+      integer(c_int) :: ainv_dev_alloc
+    end function
+    function ainv_dev_free(p) bind(c, name='ainv_dev_free')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: p
+      integer(c_int) :: ainv_dev_free
+    end function
+    function ainv_memcpy(dst, src, bytes, kind) bind(c, name='ainv_memcpy')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: dst, src
+      integer(c_size_t), value :: bytes
+      integer(c_int), value :: kind
+      integer(c_int) :: ainv_memcpy
+    end function
+    function ainv_device_sync() bind(c, name='ainv_device_sync')
+      import :: c_int
+      integer(c_int) :: ainv_device_sync
     end function
     ! irp / ja / val: c_loc of the arrays, or c_null_ptr to query sizes
     function ainv_gen_pde3d(nn, hconv, n, nnz, irp, ja, val) &

@@ -1,8 +1,11 @@
 ! Sample driver in the style of the reference's tests/pdegen/ppde3d.f90:
-! generate the 3-D convection-diffusion matrix, build an AINV / INVK / INVT
-! preconditioner through the Fortran solver objects, solve with a Krylov
-! method on the GPU and print the same quantities (build time, solve time,
-! iterations, error estimate, preconditioner nonzeros).
+! generate the 3-D convection-diffusion matrix as the local block of a
+! psb_dspmat_type with its descriptor, build an AINV / INVK / INVT
+! preconditioner through the solver objects' build(a, desc_a, info), solve
+! with a Krylov method on the GPU with the right-hand side and solution in
+! device storage (psb_d_vect_type with the mld_d_vect_hip_type mold), and
+! print the same quantities (build time, solve time, iterations, error
+! estimate, preconditioner nonzeros).
 !
 ! stdin deck, one value per line (text after the value is ignored):
 !   method (BICGSTAB|CG|CGS|BICG|RGMRES|BICGSTABL), preconditioner
@@ -17,6 +20,7 @@
 !   aggregation threshold
 program ppde3d_hip
   use iso_c_binding
+  use psb_base_mod
   use mld_ainv_hip_cbind, only: ainv_gen_pde3d, last_error
   use mld_d_ainv_hip_mod
   implicit none
@@ -31,8 +35,12 @@ program ppde3d_hip
   integer(c_int64_t), allocatable, target :: irp(:)
   integer(c_int32_t), allocatable, target :: ja(:)
   real(c_double), allocatable, target :: val(:)
-  real(c_double), allocatable :: b(:), x(:), ones(:)
+  real(c_double), allocatable :: bh(:), xh(:), ones(:)
   class(mld_d_base_ainv_solver_type), allocatable :: prec
+  type(psb_dspmat_type) :: a
+  type(psb_desc_type) :: desc_a
+  type(psb_d_vect_type) :: b, x
+  type(mld_d_vect_hip_type) :: vmold
   integer(8) :: t0, t1, t2, rate
 
   read(*, *) method
@@ -74,6 +82,10 @@ program ppde3d_hip
   ! the library generates 0-based CSR; the Fortran side works 1-based
   irp = irp + 1
   ja = ja + 1
+  call a%csr_bld(n, n, irp, ja, val, info)
+  if (info /= 0) call fail('matrix')
+  desc_a%n_row = n
+  desc_a%n_col = n
 
   select case (trim(ptype))
   case ('INVK')
@@ -107,22 +119,27 @@ program ppde3d_hip
   call prec%check(info)
 
   call system_clock(t0, rate)
-  call prec%build(int(n), irp, ja, val, info)
+  call prec%build(a, desc_a, info, vmold=vmold)
   if (info /= 0) call fail('build')
   call system_clock(t1)
 
-  ! b = A * 1 (the df_sample.f90 convention when no rhs is given)
-  allocate(ones(n), b(n), x(n))
+  ! b = A * 1 (the df_sample.f90 convention when no rhs is given), b and x
+  ! in device storage
+  allocate(ones(n), bh(n), xh(n))
   ones = 1.0d0
-  b = 0.0d0
+  bh = 0.0d0
   do i = 1, n
-    b(i) = sum(val(irp(i):irp(i + 1) - 1) * ones(ja(irp(i):irp(i + 1) - 1)))
+    bh(i) = sum(val(irp(i):irp(i + 1) - 1) * ones(ja(irp(i):irp(i + 1) - 1)))
   end do
-  x = 0.0d0
-  call mld_krylov(trim(method), int(n), irp, ja, val, prec, b, x, eps, &
-       & itmax, iter, err, info, irst)
+  xh = 0.0d0
+  call b%bld(bh, info, mold=vmold)
+  if (info == 0) call x%bld(xh, info, mold=vmold)
+  if (info /= 0) call fail('vectors')
+  call mld_krylov(trim(method), a, prec, b, x, eps, desc_a, info, &
+       & itmax=itmax, iter=iter, err=err, irst=irst, istop=2)
   if (info /= 0) call fail('krylov')
   call system_clock(t2)
+  xh = x%get_vect()
 
   call prec%descr(info)
   write(*, '(a,i0)') 'Matrix order                 : ', n
@@ -133,7 +150,9 @@ program ppde3d_hip
   write(*, '(a,i0)') 'Number of iterations         : ', iter
   write(*, '(a,es12.5)') 'Convergence indicator on exit: ', err
   write(*, '(a,i0)') 'Total nonzeros for PREC      : ', prec%get_nzeros()
-  write(*, '(a,es12.5)') 'Max error |x - 1|            : ', maxval(abs(x - 1.0d0))
+  write(*, '(a,es12.5)') 'Max error |x - 1|            : ', maxval(abs(xh - 1.0d0))
+  call b%free(info)
+  call x%free(info)
   call prec%free(info)
 
 contains

@@ -102,3 +102,66 @@ def test_fortran_driver_multilevel(mld, O, sub, pos, mltype):
     assert abs(its - it_ref) <= 1, (its, it_ref, text)
     assert float(f["Max error |x - 1|"]) < 1e-5
     assert "Multilevel" in text
+
+
+CHECK = os.path.join(ROOT, "mld-ainv_amd", "fortran", "build",
+                     "apply_check_hip")
+
+
+@pytest.mark.parametrize("ptype,prec", [("INVK", "F64"), ("AINV", "F64"),
+                                        ("INVT", "F64"), ("INVK", "F32")])
+def test_fortran_objects_through_reference_interfaces(mld, O, ptype, prec):
+    """The solver objects through the reference's type-bound interfaces
+    (build on a psb_dspmat_type block with its descriptor, apply_a on host
+    arrays, apply_v on device-stored psb_d_vect_type, clone,
+    clone_settings, set with idx, descr with coarse, update, stringval,
+    algname), checked against the oracle."""
+    if not os.path.exists(CHECK):
+        pytest.fail(f"{CHECK} not built (run __graft_entry__.build())")
+    N = 8
+    out = subprocess.run([CHECK], input=f"{ptype}\n{N}\n{prec}\n",
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    f = {}
+    for line in lines:
+        m = re.match(r"\s*(.+?)\s*:\s*(.+?)\s*$", line)
+        if m:
+            f[m.group(1)] = m.group(2)
+    # the host and device paths run the same kernels
+    assert float(f["apply_a vs apply_v N"]) == 0.0
+    assert float(f["apply_a vs apply_v T"]) == 0.0
+    assert int(f["apply_v with 1 wv info"]) != 0
+    assert float(f["clone apply diff"]) == 0.0
+    assert f["clone type"] == ptype
+    assert int(f["clone_settings info"]) == 0
+    assert int(f["clone_settings other"]) == 1
+    assert int(f["get_wrksz"]) == 2
+    assert int(f["sizeof positive"]) == 1
+    assert int(f["stringval SYM-LLK"]) == 2
+    assert f["algname 1"] == "Left-looking"
+
+    def block(tag):
+        i = lines.index(tag)
+        return np.array([float(v) for v in lines[i + 1:i + 1 + N ** 3]])
+
+    a = mld.gen_pde3d(N)
+    n = a.shape[0]
+    if ptype == "INVK":
+        w, z, d = O.invk_bld(a, 1, 1)
+    elif ptype == "INVT":
+        w, z, d = O.invt_bld(a, 4, 1e-2, 4, 1e-2)
+    else:
+        w, z, d = O.ainv_bld(a, "LLK", 8, 0.1)
+    assert int(f["get_nzeros"]) == n + w.nnz + z.nnz
+    idx = np.arange(1, n + 1, dtype=np.float64)
+    x, y0 = np.sin(idx), np.cos(idx)
+    tol = 1e-12 if prec == "F64" else 2e-6
+    for tag, trans in (("Y_N", "N"), ("Y_T", "T")):
+        ref = O.apply(w, z, d, x, alpha=0.75, beta=-0.5, y=y0, trans=trans)
+        got = block(tag)
+        assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref), tag
+    d2 = O.update_a(w, z, d, np.full(n, 0.5))
+    ref = O.apply(w, z, d2, x, alpha=0.75, beta=-0.5, y=y0)
+    got = block("Y_U")
+    assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref)
