@@ -62,7 +62,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_coop_k32.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_k32.json")
 
 # name: (description, solver kind, parameters, method, rtol, slots, queue,
 #        right-hand sides ("Ax": x_0 = 1, x_j ~ N(0,1); "ones+N": b_0 = 1,
@@ -383,6 +383,11 @@ def main():
                    help="profiling mode: build, then only the two apply "
                         "kernels at the served block width (and the copy "
                         "kernel that calibrates the traffic counters)")
+    p.add_argument("--eager-iteration", action="store_true",
+                   help="profiling mode: build, then two 12-iteration "
+                        "solves of `slots` right-hand sides with the "
+                        "iteration's kernels launched one by one (no graph "
+                        "replays), for kernel traces")
     p.add_argument("--map", default="AUTO",
                    help="APPLY_MAP for --apply-only (AUTO, SPLIT, ROW, COOP, "
                         "COOP128)")
@@ -437,6 +442,20 @@ def main():
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
         return
     A = m.SystemMatrix(a)
+    if args.eager_iteration:
+        sv = new_solver(args.prec)
+        sv.build(a)
+        B = m.DeviceArray.from_numpy(np.ascontiguousarray(
+            m.gen_normal(3, (n, slots))))
+        for _ in range(2):
+            t0 = time.time()
+            m.krylov(method, A, sv, B, eps=1e-30, itmax=12, eager=True)
+            m.synchronize()
+            log(f"eager: 12 iterations at k={slots} in "
+                f"{time.time() - t0:.3f}s")
+        print(json.dumps({"eager_iteration": True, "config": args.config,
+                          "k": slots, "prec": args.prec}))
+        return
     if args.ml_cycle:
         ml_cycle_only(m, args, a, A, kind, params, method, slots)
         return
