@@ -64,25 +64,27 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_k32.json")
 
-# name: (description, solver kind, parameters, method, rtol, slots, queue,
+# name: (description, solver kind, parameters, method, rtol, slots (0: the
+#        served form times 8, 16, 32 and 64 slots on its first call and keeps
+#        the fastest), queue,
 #        right-hand sides ("Ax": x_0 = 1, x_j ~ N(0,1); "ones+N": b_0 = 1,
 #        b_j ~ N(0,1); "N": all N(0,1)), seed)
 CONFIGS = {
     "C1": ("pde3d 40^3 convection-diffusion, AINV-LLK fill 8 thresh 0.1, "
            "BiCGSTAB rtol 1e-9", "AINV",
            dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=0.1), "BICGSTAB",
-           1e-9, 32, 192, "ones+N", 1),
+           1e-9, 0, 192, "ones+N", 1),
     "C2": ("pde3d 160^3 convection-diffusion, INVK(1,1), BiCGSTAB rtol 1e-8",
-           "INVK", dict(SUB_FILLIN=1, INV_FILLIN=1), "BICGSTAB", 1e-8, 32,
+           "INVK", dict(SUB_FILLIN=1, INV_FILLIN=1), "BICGSTAB", 1e-8, 0,
            192, "Ax", 5),
     "C3": ("aniso2d 2048^2 eps 1e-3, AINV SYM-LLK fill 8 thresh 5e-2, "
            "CG rtol 1e-8", "AINV",
            dict(AINV_ALG="SYM-LLK", SUB_FILLIN=8, SUB_ILUTHRS=5e-2), "CG",
-           1e-8, 32, 96, "N", 3),
+           1e-8, 0, 96, "N", 3),
     "C4": ("random nonsymmetric n=10^6 (LogNormal sigma 1.0), AINV-LLK fill 8 "
            "thresh 5e-2, BiCGSTAB rtol 1e-8", "AINV",
            dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=5e-2), "BICGSTAB",
-           1e-8, 16, 96, "N", 4),
+           1e-8, 0, 96, "N", 4),
 }
 
 
@@ -238,6 +240,25 @@ def ml_cycle_only(m, args, a, A, kind, params, method, slots):
                                  for L in lv]}))
 
 
+def served_warm(m, method, A, pv, Bh, slots, M, eps, times=1):
+    """Warm the served form on the first right-hand sides; with slots 0 its
+    first call times 8, 16, 32 and 64 slots.  Returns the slot count it runs
+    and the ms per column-iteration of each width tried (None when given)."""
+    warm = m.DeviceArray.from_numpy(Bh[:min(slots or 64, M)])
+    for _ in range(times):
+        m.krylov_stream(method, A, pv, warm, slots=slots, eps=eps,
+                        itmax=5000)
+    m.synchronize()
+    warm.free()
+    if slots:
+        return slots, None
+    e = [t for t in pv.report().get("served_slots", [])
+         if t["method"] == method][0]
+    log(f"served block width {e['slots']} (ms per column-iteration "
+        f"{e['ms_per_col_iter']})")
+    return e["slots"], e["ms_per_col_iter"]
+
+
 def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
            M, fp64_rate, it64):
     """The multilevel preconditioner on the same problem and right-hand
@@ -279,10 +300,7 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
     # served form with --prec level factors (tuned, captured graphs)
     pv.cnv(args.prec)
     dX = m.DeviceArray((M, n))
-    warm = m.DeviceArray.from_numpy(Bh[:min(slots, M)])
-    m.krylov_stream(method, A, pv, warm, slots=slots, eps=eps, itmax=5000)
-    m.synchronize()
-    warm.free()
+    slots, width_ms = served_warm(m, method, A, pv, Bh, slots, M, eps)
     t_start = time.time()
     _, iters, errs, lat = m.krylov_stream(method, A, pv, dB, dX, slots=slots,
                                           eps=eps, itmax=5000)
@@ -328,7 +346,8 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
                                 rate1 / fp64_rate if fp64_rate else None,
                             "one_level_iterations": it64},
         "served": {"solves_per_s": rate, "factor_prec": args.prec,
-                   "slots": slots, "iterations_mean": float(np.mean(iters)),
+                   "slots": slots, "slots_ms_per_col_iter": width_ms,
+                   "iterations_mean": float(np.mean(iters)),
                    "iterations_max": int(np.max(iters)),
                    "latency_ms_p50": float(np.percentile(lat, 50)),
                    "latency_ms_p99": float(np.percentile(lat, 99)),
@@ -411,6 +430,9 @@ def main():
     desc, kind, params, method, eps, slots, M, rhs_kind, seed = \
         CONFIGS[args.config]
     slots = args.slots or slots
+    # the modes that time one pass or cycle take a fixed block width
+    kfix = slots or 32
+    slots_cfg = slots
     M = args.queue or M
     copy_gbs, _ = m.stream_copy_gbs(1 << 30, 10)
     t0 = time.time()
@@ -433,10 +455,10 @@ def main():
         lay = sv.layout()
         x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
         y32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
-        ms_w, ms_z = profile_apply(m, sv, n, slots, 10, x32, y32)
-        bw = apply_pass_bytes(lay, slots, True, 4 if x32 else 8)
+        ms_w, ms_z = profile_apply(m, sv, n, kfix, 10, x32, y32)
+        bw = apply_pass_bytes(lay, kfix, True, 4 if x32 else 8)
         print(json.dumps({"apply_only": True, "config": args.config,
-                          "prec": args.prec, "map": args.map, "k": slots,
+                          "prec": args.prec, "map": args.map, "k": kfix,
                           "x": "FP32" if x32 else "FP64",
                           "ms_w": ms_w, "ms_z": ms_z, "bytes_w": bw,
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
@@ -446,18 +468,18 @@ def main():
         sv = new_solver(args.prec)
         sv.build(a)
         B = m.DeviceArray.from_numpy(np.ascontiguousarray(
-            m.gen_normal(3, (n, slots))))
+            m.gen_normal(3, (n, kfix))))
         for _ in range(2):
             t0 = time.time()
             m.krylov(method, A, sv, B, eps=1e-30, itmax=12, eager=True)
             m.synchronize()
-            log(f"eager: 12 iterations at k={slots} in "
+            log(f"eager: 12 iterations at k={kfix} in "
                 f"{time.time() - t0:.3f}s")
         print(json.dumps({"eager_iteration": True, "config": args.config,
-                          "k": slots, "prec": args.prec}))
+                          "k": kfix, "prec": args.prec}))
         return
     if args.ml_cycle:
-        ml_cycle_only(m, args, a, A, kind, params, method, slots)
+        ml_cycle_only(m, args, a, A, kind, params, method, kfix)
         return
 
     # ---- right-hand sides, resident on the device: M n-vectors
@@ -539,11 +561,8 @@ def main():
     sv.cnv(args.prec)
     lay = sv.layout()
     dX = m.DeviceArray((M, n))
-    warm = m.DeviceArray.from_numpy(Bh[:min(slots, M)])
-    for _ in range(args.warmup):
-        m.krylov_stream(method, A, sv, warm, slots=slots, eps=eps, itmax=5000)
-    m.synchronize()
-    warm.free()
+    slots, width_ms = served_warm(m, method, A, sv, Bh, slots, M, eps,
+                                  max(1, args.warmup))
     lat, iters, errs = [], None, None
     t_start = time.time()
     for _ in range(args.steps):
@@ -618,10 +637,11 @@ def main():
     ml = None
     if not args.no_ml and args.config in ML_CONFIGS:
         ml = ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps,
-                    slots, M, fp64_rate, it64)
+                    slots_cfg, M, fp64_rate, it64)
     invt = None
     if args.config == "C2" and not args.no_invt:
-        invt = invt_leg(m, args, a, A, dB, Bh, sample, method, eps, slots, M)
+        invt = invt_leg(m, args, a, A, dB, Bh, sample, method, eps,
+                        slots_cfg, M)
 
     cpu = None
     if not args.no_cpu:
@@ -657,7 +677,8 @@ def main():
                         f"right-hand sides through {slots} slots, "
                         f"{args.prec}-stored factors, FP64 Krylov",
             "name": args.config, "n": n, "nnz": int(a.nnz), "queue": M,
-            "slots": slots, "factor_prec": args.prec, "parallelism": "1 GPU",
+            "slots": slots, "slots_ms_per_col_iter": width_ms,
+            "factor_prec": args.prec, "parallelism": "1 GPU",
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
                        "p99": float(np.percentile(lat_ms, 99)),
@@ -740,8 +761,19 @@ def c5_run(m, args, burst=True):
     for t in warm:
         srv.wait(t)
     t_warm = time.time() - t0
-    log(f"first-touch builds + one solve per matrix: {t_warm:.1f}s")
     st_warm = srv.stats()
+    # builds run on the worker that takes the matrix, while the other workers
+    # keep their batches running on the device
+    first_touch = {
+        "seconds": t_warm, "builds": st_warm["builds"],
+        "build_seconds": st_warm["build_seconds"],
+        "build_overlap_seconds": st_warm["build_overlap_seconds"],
+        "overlap_frac": st_warm["build_overlap_seconds"] /
+                        max(st_warm["build_seconds"], 1e-9),
+        "note": "build_seconds summed over workers; overlap: the build "
+                "seconds during which another worker ran a slot batch"}
+    log(f"first-touch builds + one solve per matrix: {t_warm:.1f}s "
+        f"({first_touch})")
 
     def requests(count, seed):
         rr = np.random.default_rng(seed)
@@ -882,6 +914,7 @@ def c5_run(m, args, burst=True):
                   "requests": nburst},
         "arrival_rate_per_s": lam,
         "first_touch_seconds": t_warm,
+        "first_touch": first_touch,
         "server_stats": st,
         "phases": {"burst": phase(st_warm, st_burst),
                    "closed_loop": phase(st_burst, st_closed),
@@ -899,6 +932,7 @@ def c5_compact(rec):
         "requests": rec["config"]["requests"],
         "columns": rec["config"]["columns"],
         "first_touch_seconds": rec["first_touch_seconds"],
+        "first_touch": rec["first_touch"],
         "capacity": rec["capacity"],
         "open_loop": {"solves_per_s": rec["value"],
                       "unit": "right-hand sides per second",
@@ -956,10 +990,7 @@ def invt_leg(m, args, a, A, dB, Bh, sample, method, eps, slots, M):
     sv.cnv(args.prec)
     lay = sv.layout()
     dX = m.DeviceArray((M, n))
-    warm = m.DeviceArray.from_numpy(Bh[:min(slots, M)])
-    m.krylov_stream(method, A, sv, warm, slots=slots, eps=eps, itmax=5000)
-    m.synchronize()
-    warm.free()
+    slots, width_ms = served_warm(m, method, A, sv, Bh, slots, M, eps)
     t_start = time.time()
     _, iters, _, lat = m.krylov_stream(method, A, sv, dB, dX, slots=slots,
                                        eps=eps, itmax=5000)
@@ -989,8 +1020,8 @@ def invt_leg(m, args, a, A, dB, Bh, sample, method, eps, slots, M):
         "fp64_single_rhs": {"solves_per_s": rate64, "iterations": its,
                             "true_residual": true64},
         "served": {"solves_per_s": rate, "factor_prec": args.prec,
-                   "slots": slots, "queue": M,
-                   "iterations_mean": float(np.mean(iters)),
+                   "slots": slots, "slots_ms_per_col_iter": width_ms,
+                   "queue": M, "iterations_mean": float(np.mean(iters)),
                    "iterations_max": int(np.max(iters)),
                    "latency_ms_p50": float(np.percentile(lat, 50)),
                    "latency_ms_p99": float(np.percentile(lat, 99)),

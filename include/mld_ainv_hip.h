@@ -349,8 +349,14 @@ int ainv_krylov(const char *method, ainv_mat_h a, ainv_h prec, const double *b,
  * Krylov driver takes one right-hand side at a time): a device-resident
  * queue of nrhs right-hand sides b (n-vectors one after another) is solved
  * through `slots` (1..64) column slots of one captured Krylov
- * iteration (any method above); a slot is refilled from the queue between two iterations as
- * soon as its column stops (continuous batching).  x (device, nrhs
+ * iteration (BICGSTAB, CG, CGS or BICG); a slot is refilled from the queue
+ * between two iterations as soon as its column stops (continuous batching).
+ * slots = 0 chooses the width: the first such call on a preconditioner
+ * times six iterations at 8, 16, 32 and 64 slots (no wider than the queue
+ * rounded up to a power of two) on the queue's first columns and keeps the
+ * width with the least time per column-iteration for that preconditioner
+ * and method until its factors change (ainv_report "served_slots"; with
+ * prec NULL the trial runs on every call).  x (device, nrhs
  * n-vectors) gets the solutions, x0 = 0.  iter, err, lat_ms are host arrays
  * of nrhs: iterations, final ||r||/||b|| and milliseconds from admission to
  * completion (lat_ms may be NULL).  istop = 2 semantics as above. */
@@ -412,6 +418,12 @@ int ainv_server_submit(ainv_server_h srv, int32_t mat_id, int k,
  * completion on the device.  Releases the request. */
 int ainv_server_wait(ainv_server_h srv, int64_t req_id, double *x,
                      int32_t *iters, double *errs, double *latency_ms);
+/* Counters as JSON: builds, evictions, completions, batches, steps, host
+ * seconds per phase; build_seconds (first-touch builds, summed over
+ * workers), batch_busy_seconds (wall seconds with at least one worker inside
+ * a slot batch) and build_overlap_seconds (the build seconds that fell
+ * inside those: builds on one worker hidden behind other workers' device
+ * work). */
 int ainv_server_stats(ainv_server_h srv, char *json, size_t len);
 int ainv_server_free(ainv_server_h srv);
 

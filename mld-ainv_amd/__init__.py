@@ -610,12 +610,15 @@ def krylov(method, a, prec, b, x=None, eps=1e-8, itmax=1000, istop=2,
     return x, iters, errs
 
 
-def krylov_stream(method, a, prec, b, x=None, slots=32, eps=1e-8,
+def krylov_stream(method, a, prec, b, x=None, slots=0, eps=1e-8,
                   itmax=1000):
     """Served form: solve a device-resident queue of right-hand sides
     (DeviceArray b of shape (nrhs, n), one right-hand side per row) through
-    `slots` continuously refilled column slots.  Returns (x, iters[nrhs],
-    errs[nrhs], latency_ms[nrhs]) with x a DeviceArray like b."""
+    `slots` continuously refilled column slots (0: the width among 8, 16,
+    32, 64 with the least time per column-iteration in a trial on the
+    queue, kept per preconditioner and method; report()["served_slots"]).
+    Returns (x, iters[nrhs], errs[nrhs], latency_ms[nrhs]) with x a
+    DeviceArray like b."""
     if not isinstance(b, DeviceArray) or len(b.shape) != 2:
         raise AinvError(ERR_ARG, "krylov_stream takes a (nrhs, n) DeviceArray")
     nrhs = b.shape[0]

@@ -223,9 +223,15 @@ void host_ilut(const HostCsr &a, int32_t fill, double thresh, HostCsr &l,
 // Host threads of the build phase: AINV_BUILD_THREADS, else the CPUs this
 // process may run on (at most 16).
 int build_threads();
+// How a biconjugation ran: the threads it used and whether its Z and W
+// passes ran pipelined over several threads (else the column loop).
+struct AinvBuildInfo {
+  int threads = 1;
+  bool pipelined = false;
+};
 void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
                HostCsr &w, HostCsr &z, std::vector<double> &d,
-               int threads = 0);
+               int threads = 0, AinvBuildInfo *info = nullptr);
 
 double now_seconds();
 

@@ -591,7 +591,8 @@ int build_threads() {
 }
 
 void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
-               HostCsr &w, HostCsr &z, std::vector<double> &d, int threads) {
+               HostCsr &w, HostCsr &z, std::vector<double> &d, int threads,
+               AinvBuildInfo *info) {
   // the dispatcher's select (mld_d_biconjg_mod.F90:107-122)
   if (alg < AINV_ALG_LLK || alg > AINV_ALG_MLK)
     throw Error(AINV_ERR_ALG, "Invalid alg " + std::to_string(alg));
@@ -688,6 +689,10 @@ void host_ainv(const HostCsr &a, int alg, int32_t fill_in, double thresh,
     if (w_here) w_pass();
     if (err) std::rethrow_exception(err);
     done = true;
+  }
+  if (info) {
+    info->threads = done ? threads : 1;
+    info->pipelined = done;
   }
   SeqCols zs{zc}, ws{wc};
   for (int32_t i = 1; i < n && !done; ++i) {

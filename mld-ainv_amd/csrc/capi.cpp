@@ -243,7 +243,8 @@ void do_build(ainv_solver *h, const HostCsr &a) {
   std::vector<double> d;
   if (h->kind == AINV_KIND_AINV) {
     HostCsr w, z;
-    host_ainv(a, h->alg, h->fill_in, h->thresh, w, z, d, h->host_threads);
+    host_ainv(a, h->alg, h->fill_in, h->thresh, w, z, d, h->host_threads,
+              &h->bicg);
     h->t_factor = now_seconds() - t0;
     csr_upload(w, h->w, h->stream);
     csr_upload(z, h->z, h->stream);
@@ -560,6 +561,36 @@ void solver_apply(ainv_solver *h, double alpha, const double *x, double beta,
 }  // namespace ainv
 
 using namespace ainv;
+
+namespace {
+
+// served block width (krylov_stream with slots 0): per method, the chosen
+// slot count and ms per column-iteration of each width tried
+std::string served_slots_json(const ainv_solver *h) {
+  static const char *const kMethods[4] = {"BICGSTAB", "CG", "CGS", "BICG"};
+  std::string ss;
+  for (int mi = 0; mi < 4; ++mi) {
+    if (h->auto_slots[mi] <= 0 || h->auto_slots_gen[mi] != h->generation)
+      continue;
+    std::string e = std::string(ss.empty() ? "" : ", ") + "{\"method\": \"" +
+                    kMethods[mi] + "\", \"slots\": " +
+                    std::to_string(h->auto_slots[mi]) +
+                    ", \"ms_per_col_iter\": {";
+    bool first = true;
+    for (int i = 0; i < 8; ++i) {
+      if (h->slot_ms[mi][i] <= 0.0) continue;
+      char b2[64];
+      std::snprintf(b2, sizeof b2, "%s\"%d\": %.6f", first ? "" : ", ", 1 << i,
+                    h->slot_ms[mi][i]);
+      e += b2;
+      first = false;
+    }
+    ss += e + "}}";
+  }
+  return ", \"served_slots\": [" + ss + "]";
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1187,6 +1218,10 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
            ", \"prec\": \"" + prec_name(h->prec) + "\"" +
            ", \"w\": " + hll_json(h->hw) + ", \"z\": " + hll_json(h->hz);
       if (!h->inv_report.empty()) r += ", \"inverse\": " + h->inv_report;
+      if (h->kind == AINV_KIND_AINV)
+        r += ", \"biconjugation\": {\"threads\": " +
+             std::to_string(h->bicg.threads) + ", \"pipelined\": " +
+             (h->bicg.pipelined ? "true" : "false") + "}";
       // autotuner: per block width K, ms of each pass in each mapping
       std::string tu;
       for (int i = 0; i < 8; ++i) {
@@ -1211,6 +1246,7 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
            std::string(h->map_mode < 0 ? "auto" : map_name(h->map_mode)) +
            "\", \"tuning\": [" + tu + "]";
     }
+    if (h->built) r += served_slots_json(h);
     r += "}";
     std::snprintf(buf, len, "%s", r.c_str());
   });

@@ -2230,17 +2230,80 @@ void move_columns(SlotBatch *b, SlotBatch *nb, const std::vector<int> &keep) {
 }
 }  // namespace
 
+// Block-width autotuner of the served form: for each candidate slot count K
+// (8, 16, 32, 64, no wider than the queue needs) one batch is opened (its
+// mappings tuned and its iteration captured), K columns of the queue are
+// admitted and, after two steps, four replayed steps are timed; the width
+// with the least time per column-iteration is kept per (preconditioner,
+// method) until the factors change.  The plans the trial made for widths
+// above the chosen one are dropped again (each holds the iteration's
+// vectors at its width; the narrower ones serve the queue's tail).
+int tune_slots(ainv_matrix *a, ainv_solver *m, int mid, const double *b,
+               int nrhs, double eps, hipStream_t s) {
+  if (m && m->auto_slots[mid] > 0 && m->auto_slots_gen[mid] == m->generation)
+    return m->auto_slots[mid];
+  std::vector<const KrylovPlan *> before;
+  for (auto &q : a->plans) before.push_back(q.get());
+  if (m) std::fill(m->slot_ms[mid], m->slot_ms[mid] + 8, 0.0);
+  int cap = 1;
+  while (cap < nrhs && cap < kMaxCols) cap <<= 1;
+  const int64_t n = a->n;
+  int best = 0;
+  double best_ms = 0.0;
+  for (int K = 8; K <= kMaxCols; K <<= 1) {
+    if (K > cap && K > 8) break;
+    SlotBatch *bt = batch_open(a, m, mid, K, s);
+    double ms = 0.0;
+    try {
+      std::vector<int> slot(K);
+      std::vector<const double *> src(K);
+      for (int c = 0; c < K; ++c) {
+        slot[c] = c;
+        src[c] = b + (int64_t)(c % nrhs) * n;
+      }
+      batch_admit_cols(bt, slot, src, eps);
+      for (int r = 0; r < 2; ++r) batch_step(bt);
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+      const double t0 = now_seconds();
+      for (int r = 0; r < 4; ++r) batch_step(bt);
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+      ms = (now_seconds() - t0) * 1e3 / 4.0 / K;
+    } catch (...) {
+      batch_close(bt);
+      throw;
+    }
+    batch_close(bt);
+    if (m) m->slot_ms[mid][log2_pow2(K)] = ms;
+    if (best == 0 || ms < best_ms) {
+      best = K;
+      best_ms = ms;
+    }
+  }
+  for (size_t i = a->plans.size(); i-- > 0;) {
+    const KrylovPlan *q = a->plans[i].get();
+    if (q->k > best && q->holders == 0 &&
+        std::find(before.begin(), before.end(), q) == before.end())
+      a->plans.erase(a->plans.begin() + i);
+  }
+  if (m) {
+    m->auto_slots[mid] = best;
+    m->auto_slots_gen[mid] = m->generation;
+  }
+  return best;
+}
+
 void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
                    const double *b, double *x, int nrhs, int slots, double eps,
                    int itmax, int32_t *iter, double *err, double *lat_ms,
                    hipStream_t s) {
   if (nrhs < 0) throw Error(AINV_ERR_ARG, "nrhs must be >= 0");
-  if (slots < 1 || slots > kMaxCols)
-    throw Error(AINV_ERR_ARG, "slots must be in 1..64");
+  if (slots < 0 || slots > kMaxCols)
+    throw Error(AINV_ERR_ARG, "slots must be in 0..64 (0: tuned)");
   if (m && m->n != a->n)
     throw Error(AINV_ERR_ARG, "preconditioner and matrix sizes differ");
   const int mid = krylov_method_id(method);
   if (nrhs == 0) return;
+  if (slots == 0) slots = tune_slots(a, m, mid, b, nrhs, eps, s);
   const int64_t n = a->n;
   SlotBatch *bt = batch_open(a, m, mid, slots, s);
   int K = bt->K;

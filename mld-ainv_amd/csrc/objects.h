@@ -39,6 +39,7 @@ struct ainv_solver {
   // COOP / COOP128 for both passes
   int map_mode = -1;
   int host_threads = 0;  // host threads of the AINV build (0: the default)
+  ainv::AinvBuildInfo bicg;  // how the last AINV build's biconjugation ran
   // multilevel parameters (kind AINV_KIND_ML); the factor parameters above
   // apply to the factors of every level
   int ml_nlev = 3;
@@ -74,6 +75,13 @@ struct ainv_solver {
   // (packed pair: first and second pass, map_pair)
   int16_t apply_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   double apply_ms[8][2][ainv::kNumMaps] = {};  // [K][pass][map]
+  // ---- served block width chosen by a timed trial (krylov_stream with
+  // slots = 0): per Krylov method, the slot count (0: not tuned), the
+  // factors' generation it was tuned on, and ms per column-iteration of
+  // each candidate width K = 2^i
+  int auto_slots[8] = {};
+  uint64_t auto_slots_gen[8] = {};
+  double slot_ms[8][8] = {};  // [method][log2 K]
 
   // ---- multilevel hierarchy (kind AINV_KIND_ML)
   std::unique_ptr<ainv::MlHier, ainv::MlDeleter> ml;

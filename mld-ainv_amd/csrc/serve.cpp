@@ -101,6 +101,11 @@ struct ainv_server {
   int64_t builds = 0, evictions = 0, completed = 0, admitted_midflight = 0;
   int64_t steps = 0, batches = 0;
   double build_seconds = 0;
+  // overlap of first-touch builds with device work: workers inside a slot
+  // batch, the seconds during which at least one was (integrated at every
+  // change), and the build seconds that fell inside them
+  int in_batch = 0;
+  double busy_mark = 0, busy_seconds = 0, build_overlap_seconds = 0;
   // host seconds per phase summed over workers, and request waits
   double open_seconds = 0, admit_seconds = 0, step_seconds = 0,
          extract_seconds = 0, queue_seconds = 0, service_seconds = 0;
@@ -108,6 +113,30 @@ struct ainv_server {
 };
 
 namespace {
+
+// Seconds so far during which some worker was inside a slot batch (the
+// caller holds the lock).
+double busy_now_locked(ainv_server *S) {
+  const double t = now_seconds();
+  if (S->in_batch > 0) S->busy_seconds += t - S->busy_mark;
+  S->busy_mark = t;
+  return S->busy_seconds;
+}
+
+// A worker inside a slot batch from construction to destruction.
+struct InBatch {
+  ainv_server *S;
+  explicit InBatch(ainv_server *s) : S(s) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    busy_now_locked(S);
+    S->in_batch++;
+  }
+  ~InBatch() {
+    std::unique_lock<std::mutex> lk(S->mu);
+    busy_now_locked(S);
+    S->in_batch--;
+  }
+};
 
 int64_t plan_bytes(int32_t n, int K, int method) {
   return (int64_t)n * K * (method == KM_CG ? 6 : 10) * 8 + (int64_t)n * K * 8;
@@ -143,6 +172,11 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
       evict_locked(S, victim);
     }
   }
+  double busy0;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    busy0 = busy_now_locked(S);
+  }
   const double t0 = now_seconds();
   ainv_solver *M = nullptr;
   ainv_matrix *A = nullptr;
@@ -177,6 +211,7 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
   S->peak_resident = std::max(S->peak_resident, S->resident);
   S->builds++;
   S->build_seconds += dt;
+  S->build_overlap_seconds += busy_now_locked(S) - busy0;
   m->builds++;
 }
 
@@ -222,6 +257,7 @@ void serve_matrix(ainv_server *S, ServedMatrix *m, hipStream_t s) {
     std::unique_lock<std::mutex> lk(S->mu);
     S->open_seconds += now_seconds() - t_open;
   }
+  InBatch busy(S);
   double t_admit = 0, t_step = 0, t_extract = 0;
   std::vector<SlotUse> uses;  // admitted requests
   std::vector<int> owner(K, -1);
@@ -634,7 +670,8 @@ int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
                   "{\"matrices\": %zu, \"resident_matrices\": %d, "
                   "\"resident_bytes\": %lld, \"peak_resident_bytes\": %lld, "
                   "\"budget_bytes\": %lld, \"builds\": %lld, "
-                  "\"build_seconds\": %.6f, \"evictions\": %lld, "
+                  "\"build_seconds\": %.6f, \"build_overlap_seconds\": %.6f, "
+                  "\"batch_busy_seconds\": %.6f, \"evictions\": %lld, "
                   "\"completed\": %lld, \"batches\": %lld, \"steps\": %lld, "
                   "\"admitted_midflight\": %lld, \"workers\": %zu, "
                   "\"open_seconds\": %.6f, \"admit_seconds\": %.6f, "
@@ -643,6 +680,7 @@ int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
                   S->mats.size(), resident_mats, (long long)S->resident,
                   (long long)S->peak_resident, (long long)S->budget,
                   (long long)S->builds, S->build_seconds,
+                  S->build_overlap_seconds, busy_now_locked(S),
                   (long long)S->evictions, (long long)S->completed,
                   (long long)S->batches, (long long)S->steps,
                   (long long)S->admitted_midflight, S->workers.size(),

@@ -90,6 +90,32 @@ def test_ainv_sym_llk_bit_exact(gpu, O):
     check_factors(sv, O.ainv_bld(a, "SYM-LLK", 8, 5e-2))
 
 
+@pytest.mark.parametrize("alg", ["LLK", "LLK-NOTH", "MLK", "SYM-LLK"])
+def test_ainv_pipelined_build_bit_exact(gpu, O, monkeypatch, alg):
+    """The biconjugation pipelined over several host threads (taken when a
+    column's dependencies lie far back, as in this random banded matrix,
+    whose median distance to the nearest earlier coupling is 96 columns)
+    gives the oracle's factors bit for bit, and its applies on the GPU
+    match the oracle's."""
+    monkeypatch.setenv("AINV_BUILD_THREADS", "4")
+    a = gpu.gen_random(6000, seed=23, median=8.0, band=2500)
+    sv = build(gpu, "AINV", a, AINV_ALG=alg, SUB_ILUTHRS=5e-2, SUB_FILLIN=8)
+    bc = sv.report()["biconjugation"]
+    assert bc["pipelined"] and bc["threads"] == 4, bc
+    w, z, d = O.ainv_bld(a, alg, 8, 5e-2)
+    check_factors(sv, (w, z, d))
+    x = gpu.gen_normal(6, (a.shape[0],))
+    for trans in ("N", "T"):
+        y = sv.apply(x, trans=trans)
+        ref = O.apply(w, z, d, x, trans=trans)
+        assert np.linalg.norm(y - ref) <= 1e-12 * np.linalg.norm(ref)
+    # one thread: the column loop, the same factors
+    monkeypatch.setenv("AINV_BUILD_THREADS", "1")
+    sv1 = build(gpu, "AINV", a, AINV_ALG=alg, SUB_ILUTHRS=5e-2, SUB_FILLIN=8)
+    assert not sv1.report()["biconjugation"]["pipelined"]
+    check_factors(sv1, (w, z, d))
+
+
 def test_tiny_and_diagonal_matrices(gpu, O):
     for a in [sp.csr_matrix(np.array([[4.0]])),
               sp.csr_matrix(np.diag([2.0, -3.0, 5.0])),

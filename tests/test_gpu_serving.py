@@ -116,6 +116,54 @@ def test_krylov_stream_matches_single_solves(gpu, method, slots, nrhs):
         assert np.linalg.norm(Xh[j] - xs) <= 1e-6 * np.linalg.norm(xs)
 
 
+def test_krylov_stream_tuned_width(gpu):
+    """slots = 0: the first call times 8, 16, 32 and 64 slots (no wider than
+    the queue rounded up to a power of two), reports the chosen width and
+    every width's time, keeps the choice for later calls and tunes again
+    after the factors change; the results are those of solving alone."""
+    a = gpu.gen_pde3d(12)
+    n = a.shape[0]
+    sv = gpu.SOLVERS["INVK"]()
+    for k, v in dict(SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC="F32").items():
+        sv.set(k, v)
+    sv.build(a)
+    A = gpu.SystemMatrix(a)
+    assert sv.report()["served_slots"] == []
+    B = gpu.gen_normal(41, (40, n))
+    dB = gpu.DeviceArray.from_numpy(B)
+    X, it, err, lat = gpu.krylov_stream("BICGSTAB", A, sv, dB, slots=0,
+                                        eps=1e-9)
+    ss = sv.report()["served_slots"]
+    assert len(ss) == 1 and ss[0]["method"] == "BICGSTAB"
+    assert ss[0]["slots"] in (8, 16, 32, 64)
+    t = ss[0]["ms_per_col_iter"]
+    assert sorted(t, key=int) == ["8", "16", "32", "64"]
+    assert all(v > 0 for v in t.values())
+    assert ss[0]["slots"] == int(min(t, key=t.get))
+    Xh = X.to_numpy()
+    assert np.all(err <= 1e-9) and np.all(lat > 0)
+    for j in (0, 17, 39):
+        xs, its, _ = gpu.krylov("BICGSTAB", A, sv, B[j], eps=1e-9)
+        assert abs(int(it[j]) - int(its[0])) <= 1, (j, it[j], its)
+        assert np.linalg.norm(Xh[j] - xs) <= 1e-6 * np.linalg.norm(xs)
+    # kept: a second call runs no trial (the timings are unchanged)
+    gpu.krylov_stream("BICGSTAB", A, sv, dB, slots=0, eps=1e-9)
+    assert sv.report()["served_slots"] == ss
+    # a short queue on another method tries only 8 slots
+    X5, it5, err5, _ = gpu.krylov_stream(
+        "BICG", A, sv, gpu.DeviceArray.from_numpy(B[:5]), slots=0, eps=1e-9)
+    assert np.all(err5 <= 1e-9)
+    e5 = [e for e in sv.report()["served_slots"] if e["method"] == "BICG"]
+    assert e5[0]["slots"] == 8 and list(e5[0]["ms_per_col_iter"]) == ["8"]
+    # new factors: no width kept until the next call tunes again
+    sv.cnv("F64")
+    assert sv.report()["served_slots"] == []
+    X2, it2, err2, _ = gpu.krylov_stream("BICGSTAB", A, sv, dB, slots=0,
+                                         eps=1e-9)
+    assert np.all(err2 <= 1e-9)
+    assert sv.report()["served_slots"][0]["slots"] in (8, 16, 32, 64)
+
+
 def test_failed_build_and_unwaited_requests(gpu):
     """A matrix whose build fails fails its requests with the build's error
     and leaves the server serving the others; requests still queued or
