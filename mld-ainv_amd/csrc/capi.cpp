@@ -587,7 +587,17 @@ std::string served_slots_json(const ainv_solver *h) {
     }
     ss += e + "}}";
   }
-  return ", \"served_slots\": [" + ss + "]";
+  const auto &ls = h->last_stream;
+  char b3[512];
+  std::snprintf(b3, sizeof b3,
+                ", \"served_last\": {\"steps\": %lld, \"slot_steps\": %lld, "
+                "\"busy_slot_steps\": %lld, \"shrinks\": %lld, \"wall_s\": "
+                "%.6f, \"launch_s\": %.6f, \"admit_s\": %.6f, \"wait_s\": "
+                "%.6f, \"extract_s\": %.6f}",
+                (long long)ls.steps, (long long)ls.slot_steps,
+                (long long)ls.busy_slot_steps, (long long)ls.shrinks, ls.wall,
+                ls.launch, ls.admit, ls.wait, ls.extract);
+  return ", \"served_slots\": [" + ss + "]" + b3;
 }
 
 }  // namespace

@@ -38,6 +38,15 @@ constexpr int kPlansPerMatrix = 12;
 struct Col {
   double rho, rho_old, rho_next, alpha, omega, beta, bn, err, eps;
   int32_t active, iter;
+  // A BiCGSTAB column admitted into the served form (x0 = 0) starts with
+  // only r~ written into the workspace: until its first x / r update
+  // (lazy = 1) the passes read r = b from its right-hand side (src, row
+  // stride sstride), take x = 0, and an extraction gives zeros, so that an
+  // admission writes one interleaved column instead of four (scattered
+  // 8-byte writes into n rows, each a partial-line update in HBM).
+  int32_t lazy, pad_;
+  const double *src;
+  int64_t sstride;
 };
 
 struct Vecs {
@@ -102,19 +111,22 @@ __global__ __launch_bounds__(kThreads) void k_vec(int32_t n, int k, Vecs v,
         acc[1] += v.r[e] * v.r[e];
         acc[2] += v.q[e] * v.r[e];
       } else if constexpr (OP == OP_P_BICG) {
+        const double re = cs.lazy ? cs.src[r * cs.sstride] : v.r[e];
         const double pn =
-            cs.iter == 1 ? v.r[e]
-                         : v.r[e] + cs.beta * (v.p[e] + (-cs.omega) * v.v[e]);
+            cs.iter == 1 ? re
+                         : re + cs.beta * (v.p[e] + (-cs.omega) * v.v[e]);
         v.p[e] = pn;
         if (v.p32) v.p32[e] = (float)pn;
       } else if constexpr (OP == OP_S_BICG) {
-        const double sn = v.r[e] + (-cs.alpha) * v.v[e];
+        const double re = cs.lazy ? cs.src[r * cs.sstride] : v.r[e];
+        const double sn = re + (-cs.alpha) * v.v[e];
         v.s[e] = sn;
         if (v.s32) v.s32[e] = (float)sn;
       } else if constexpr (OP == OP_XR_BICG) {
         const double fe = v.f32 ? (double)v.f32[e] : v.f[e];
         const double ze = v.z32 ? (double)v.z32[e] : v.z[e];
-        v.x[e] = v.x[e] + cs.alpha * fe + cs.omega * ze;
+        const double xe = cs.lazy ? 0.0 : v.x[e];
+        v.x[e] = xe + cs.alpha * fe + cs.omega * ze;
         const double rn = v.s[e] + (-cs.omega) * v.t[e];
         v.r[e] = rn;
         acc[0] += rn * rn;
@@ -294,6 +306,7 @@ __global__ __launch_bounds__(kThreads) void k_fin(
         }
         break;
       case FIN_CONV_BICG: {
+        cs.lazy = 0;  // the x / r update of this iteration wrote x and r
         const double rn = sqrt(s0);
         cs.err = (cs.bn > 0.0) ? rn / cs.bn : rn;
         cs.rho_next = s1;
@@ -411,20 +424,49 @@ struct AdmitList {
 
 // One thread per (row, admitted column): the lanes of a wave write the
 // entries of a few rows, each row's admitted columns by neighbouring lanes.
-__global__ void k_admit(int32_t n, int K, AdmitList L, double *__restrict__ b,
-                        double *__restrict__ x, double *__restrict__ r,
-                        double *__restrict__ q, float *__restrict__ r32) {
+// lazy (BiCGSTAB): r~ = b only (x, r and b are taken from the right-hand
+// side until the first update).
+__global__ void k_admit(int32_t n, int K, AdmitList L, int lazy,
+                        double *__restrict__ b, double *__restrict__ x,
+                        double *__restrict__ r, double *__restrict__ q,
+                        float *__restrict__ r32) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * L.cnt) return;
   const int64_t row = t / L.cnt;
   const int j = (int)(t - row * L.cnt);
   const double v = L.src[j][row * L.stride[j]];
   const int64_t e = row * K + L.slot[j];
+  if (lazy) {
+    q[e] = v;
+    return;
+  }
   b[e] = v;
   x[e] = 0.0;
   r[e] = v;
   if (q) q[e] = v;
   if (r32) r32[e] = (float)v;
+}
+
+// Column states and admission mask of an admission, written on the device
+// from the list passed by value (kMaxCols threads), so that admitting needs
+// no host staging and no wait for the step in flight.
+__global__ void k_admit_state(AdmitList L, double eps, int lazy,
+                              Col *__restrict__ col,
+                              int32_t *__restrict__ mask) {
+  const int t = threadIdx.x;
+  int adm = -1;
+  for (int j = 0; j < L.cnt; ++j)
+    if (L.slot[j] == t) adm = j;
+  mask[t] = adm >= 0;
+  if (adm >= 0) {
+    Col c{};
+    c.active = 1;
+    c.eps = eps;
+    c.lazy = lazy;
+    c.src = L.src[adm];
+    c.sstride = L.stride[adm];
+    col[t] = c;
+  }
 }
 
 // With x0 = 0 every dot of the init step of an admitted column is ||b||^2:
@@ -461,24 +503,27 @@ struct ExtractList {
   int slot[kMaxCols];
   double *dst[kMaxCols];
 };
+// (a column that never reached its first update holds x0 = 0)
 __global__ void k_extract_cols(int32_t n, int K, ExtractList L,
-                               const double *__restrict__ x) {
+                               const double *__restrict__ x,
+                               const Col *__restrict__ col) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * L.cnt) return;
   const int64_t row = t / L.cnt;
   const int j = (int)(t - row * L.cnt);
-  L.dst[j][row] = x[row * K + L.slot[j]];
+  L.dst[j][row] = col[L.slot[j]].lazy ? 0.0 : x[row * K + L.slot[j]];
 }
 
 // Copy columns [c0, c0+kc) of a K-column vector to dst (n x kc).
 __global__ void k_extract(int32_t n, int K, int c0, int kc,
                           const double *__restrict__ x,
-                          double *__restrict__ dst) {
+                          double *__restrict__ dst,
+                          const Col *__restrict__ col) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * kc) return;
   const int64_t row = t / kc;
   const int j = (int)(t % kc);
-  dst[t] = x[row * K + c0 + j];
+  dst[t] = col[c0 + j].lazy ? 0.0 : x[row * K + c0 + j];
 }
 
 }  // namespace
@@ -971,7 +1016,6 @@ struct SlotBatch {
   hipStream_t s = nullptr;
   int K = 0;
   int64_t steps = 0;
-  std::vector<int32_t> hmask;
 };
 
 SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
@@ -1007,7 +1051,6 @@ SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
   ++p->holders;
   b->s = s;
   b->K = p->K;  // slot count and leading dimension (pow2 >= K)
-  b->hmask.assign(kMaxCols, 0);
   return b;
 }
 
@@ -1024,10 +1067,13 @@ void admit(SlotBatch *b, const AdmitList &L, double eps) {
   KrylovPlan *p = b->p;
   const int32_t n = p->n;
   if (L.cnt == 0) return;
+  // BiCGSTAB reads r = b and x = 0 from the right-hand side until its
+  // first update; the other methods get their columns written here
+  const int lazy = p->method == KM_BICGSTAB && !p->cg;
   hipLaunchKernelGGL(k_admit,
                      dim3((unsigned)(((int64_t)n * L.cnt + 255) / 256)),
                      dim3(256), 0,
-                     b->s, n, b->K, L, p->v.b, p->v.x, p->v.r,
+                     b->s, n, b->K, L, lazy, p->v.b, p->v.x, p->v.r,
                      p->cg ? nullptr : p->v.q, x32_vecs(p).r32);
   AINV_HIP_CHECK(hipGetLastError());
   const int g = vec_blocks(n, 1);
@@ -1035,22 +1081,10 @@ void admit(SlotBatch *b, const AdmitList &L, double eps) {
   hipLaunchKernelGGL(k_admit_norms, dim3(g, L.cnt), dim3(kThreads), 0, b->s,
                      n, L, fin_pairs(fin_id), p->pt);
   AINV_HIP_CHECK(hipGetLastError());
-  Col c1 = Col{};
-  c1.active = 1;
-  c1.eps = eps;
-  std::vector<Col> cs(L.cnt, c1);
-  std::fill(b->hmask.begin(), b->hmask.end(), 0);
-  for (int j = 0; j < L.cnt; ++j) {
-    b->hmask[L.slot[j]] = 1;
-    AINV_HIP_CHECK(hipMemcpyAsync(p->col + L.slot[j], &cs[j], sizeof(Col),
-                                  hipMemcpyHostToDevice, b->s));
-  }
-  AINV_HIP_CHECK(hipMemcpyAsync(p->mask.p, b->hmask.data(),
-                                sizeof(int32_t) * kMaxCols,
-                                hipMemcpyHostToDevice, b->s));
+  hipLaunchKernelGGL(k_admit_state, dim3(1), dim3(kMaxCols), 0, b->s, L, eps,
+                     lazy, p->col, p->mask.as<int32_t>());
+  AINV_HIP_CHECK(hipGetLastError());
   fin(fin_id, b->K, g, p->pt, p->col, p->na, b->s, p->mask.as<int32_t>());
-  // the host buffers above must outlive the copies
-  AINV_HIP_CHECK(hipStreamSynchronize(b->s));
 }
 }  // namespace
 
@@ -1132,14 +1166,16 @@ void batch_extract_cols(SlotBatch *b, const std::vector<int> &slots,
   }
   const int64_t tot = (int64_t)b->p->n * L.cnt;
   hipLaunchKernelGGL(k_extract_cols, dim3((unsigned)((tot + 255) / 256)),
-                     dim3(256), 0, b->s, b->p->n, b->K, L, b->p->v.x);
+                     dim3(256), 0, b->s, b->p->n, b->K, L, b->p->v.x,
+                     b->p->col);
   AINV_HIP_CHECK(hipGetLastError());
 }
 
 void batch_extract(SlotBatch *b, int c0, int kc, double *dst) {
   const int64_t tot = (int64_t)b->p->n * kc;
   hipLaunchKernelGGL(k_extract, dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                     0, b->s, b->p->n, b->K, c0, kc, b->p->v.x, dst);
+                     0, b->s, b->p->n, b->K, c0, kc, b->p->v.x, dst,
+                     b->p->col);
   AINV_HIP_CHECK(hipGetLastError());
 }
 
@@ -2305,6 +2341,8 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
   if (nrhs == 0) return;
   if (slots == 0) slots = tune_slots(a, m, mid, b, nrhs, eps, s);
   const int64_t n = a->n;
+  ainv_solver::StreamStats ss;
+  const double t_begin = now_seconds();
   SlotBatch *bt = batch_open(a, m, mid, slots, s);
   int K = bt->K;
   std::vector<int> owner(K, -1);
@@ -2327,16 +2365,25 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
         src.push_back(b + (int64_t)next * n);
         ++next;
       }
+      const double t0 = now_seconds();
       batch_admit_cols(bt, adm, src, eps);
+      const double t1 = now_seconds();
       batch_step(bt);
+      const double t2 = now_seconds();
+      ss.admit += t1 - t0;
+      ss.launch += t2 - t1;
+      ss.steps++;
+      ss.slot_steps += K;
+      for (int c = 0; c < K; ++c) ss.busy_slot_steps += owner[c] >= 0;
       // Reading the states one step behind keeps the device busy but leaves
       // a converged column's slot idle for one step: for solves of few
       // iterations (a multilevel preconditioner) that step costs more than
       // the host's turnaround, so wait for the step just launched.
       const bool short_solves = done >= 8 && done_iters < 24 * (int64_t)done;
       const int64_t seen = batch_poll(bt, short_solves, st);
-      if (seen < 0) continue;
       const double t = now_seconds();
+      ss.wait += t - t2;
+      if (seen < 0) continue;
       std::vector<int> ext;
       std::vector<double *> dst;
       for (int c = 0; c < K; ++c) {
@@ -2356,12 +2403,14 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
       // before the admissions that reuse their slots and any move to a
       // narrower batch
       batch_extract_cols(bt, ext, dst);
+      ss.extract += now_seconds() - t;
       // queue run dry and at most half the slots busy: continue narrower
       if (next >= nrhs && K > 1) {
         std::vector<int> keep;
         for (int c = 0; c < K; ++c)
           if (owner[c] >= 0) keep.push_back(c);
         if (!keep.empty() && (int)keep.size() <= K / 2) {
+          ss.shrinks++;
           bt = batch_shrink(bt, m, keep);
           K = bt->K;
           slots = K;
@@ -2385,6 +2434,8 @@ void krylov_stream(const std::string &method, ainv_matrix *a, ainv_solver *m,
     throw;
   }
   batch_close(bt);
+  ss.wall = now_seconds() - t_begin;
+  if (m) m->last_stream = ss;
 }
 
 }  // namespace ainv

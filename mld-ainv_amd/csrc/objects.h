@@ -82,6 +82,14 @@ struct ainv_solver {
   int auto_slots[8] = {};
   uint64_t auto_slots_gen[8] = {};
   double slot_ms[8][8] = {};  // [method][log2 K]
+  // the last krylov_stream on this preconditioner: steps launched, slot
+  // steps (the batch width summed over steps) and the slot steps of
+  // admitted columns, narrowings, and host seconds spent launching steps,
+  // admitting, waiting for column states and extracting solutions
+  struct StreamStats {
+    int64_t steps = 0, slot_steps = 0, busy_slot_steps = 0, shrinks = 0;
+    double wall = 0, launch = 0, admit = 0, wait = 0, extract = 0;
+  } last_stream;
 
   // ---- multilevel hierarchy (kind AINV_KIND_ML)
   std::unique_ptr<ainv::MlHier, ainv::MlDeleter> ml;

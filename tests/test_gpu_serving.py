@@ -116,6 +116,46 @@ def test_krylov_stream_matches_single_solves(gpu, method, slots, nrhs):
         assert np.linalg.norm(Xh[j] - xs) <= 1e-6 * np.linalg.norm(xs)
 
 
+@pytest.mark.parametrize("method", ["BICGSTAB", "CG", "CGS", "BICG"])
+def test_krylov_stream_zero_rhs_and_slot_reuse(gpu, method):
+    """A zero right-hand side stops at admission with x = 0 (iteration 0),
+    also in a slot whose previous occupant left its iterate behind; every
+    other column's iterations (+-1: the queue's tail narrows at other
+    steps, and narrower plans sum the dots in another order) and solution
+    match those of the same queue without the zero column."""
+    if method == "CG":
+        a = gpu.gen_aniso2d(30, 1e-2)
+        kind, params = "AINV", dict(AINV_ALG="SYM-LLK", SUB_FILLIN=8,
+                                    SUB_ILUTHRS=5e-2, FACTOR_PREC="F32")
+    else:
+        a = gpu.gen_pde3d(10)
+        kind, params = "INVK", dict(SUB_FILLIN=1, INV_FILLIN=1,
+                                    FACTOR_PREC="F32")
+    n = a.shape[0]
+    sv = gpu.SOLVERS[kind]()
+    for k, v in params.items():
+        sv.set(k, v)
+    sv.build(a)
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(77, (12, n))
+    Bz = np.insert(B, 9, 0.0, axis=0)  # admitted after 9 others: a reused slot
+    X, it, err, _ = gpu.krylov_stream(method, A, sv,
+                                      gpu.DeviceArray.from_numpy(B), slots=4,
+                                      eps=1e-9)
+    Xz, itz, errz, _ = gpu.krylov_stream(method, A, sv,
+                                         gpu.DeviceArray.from_numpy(Bz),
+                                         slots=4, eps=1e-9)
+    Xz = Xz.to_numpy()
+    assert itz[9] == 0 and errz[9] == 0.0
+    assert np.all(Xz[9] == 0.0)
+    keep = [j for j in range(13) if j != 9]
+    assert np.all(np.abs(itz[keep] - it) <= 1), (itz, it)
+    assert np.all(errz[keep] <= 1e-9)
+    Xh = X.to_numpy()
+    for j, jz in enumerate(keep):
+        assert np.linalg.norm(Xz[jz] - Xh[j]) <= 1e-6 * np.linalg.norm(Xh[j])
+
+
 def test_krylov_stream_tuned_width(gpu):
     """slots = 0: the first call times 8, 16, 32 and 64 slots (no wider than
     the queue rounded up to a power of two), reports the chosen width and
