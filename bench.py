@@ -607,7 +607,7 @@ def main():
                          "coop128": ("k_hll_spmm_coop", " 128,"),
                          "coop_q": ("k_hll_spmm_coopq", " 256,"),
                          "coop128_q": ("k_hll_spmm_coopq", " 128,"),
-                         "run": ("k_hll_spmm_run", " 256,")}[kmap]
+                         "run": ("k_hll_spmm_run", " 128,")}[kmap]
         if first:
             return f"{kernel}<{vt}, {xt}, {tt}, {tt}, {slots},{extra} 0>"
         yt = "float" if y32 else "double"

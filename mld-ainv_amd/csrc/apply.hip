@@ -868,9 +868,32 @@ __device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off,
 // offsets, kNoLoad beyond it).
 constexpr int64_t kRunMaxBytes = (int64_t)kNoLoad;
 
+// Column chunks of the run mapping: at most 128 bytes of an x row per
+// workgroup (8 lanes per row), so a lane holds the values, gathers and sums
+// of 8 rows x 16 bytes; wider rows (64 FP32 / 32 FP64 columns) run as 2-4
+// chunks of the same kernel instead of spilling 16-lane row groups.
+constexpr int kRunChunkBytes = 128;
+// Waves per SIMD: rows of 128 bytes hold 8 x 16-byte values, gathers and
+// sums per lane -- 3 waves with FP32 sums, 2 with FP64 sums; narrower rows
+// fit the row kernels' budget.
+template <typename VT, typename XT, int K> struct RunWaves {
+  static constexpr int rb =
+      CoopChunk<XT, K, kRunChunkBytes>::KC * (int)sizeof(XT);
+  static constexpr int v =
+      rb >= 128 ? (sizeof(typename Val<VT>::acc) == 4 ? 3 : 2)
+                : AINV_ROW_WAVES;
+};
+#if AINV_ROW_WAVES > 0
+#define AINV_RUN_ATTR                                                       \
+  __attribute__((amdgpu_waves_per_eu(RunWaves<VT, XT, K>::v,                \
+                                     RunWaves<VT, XT, K>::v)))
+#else
+#define AINV_RUN_ATTR
+#endif
+
 template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
           int EPI>
-__global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_run(
+__global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
     int32_t n, int32_t ncols, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
@@ -1003,8 +1026,10 @@ void launch_run(const Hll &a, const XT *x, int k, const DT *d, OT *out,
   const VT *vl = a.val.as<VT>();
   const int ngrp = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
 #define AINV_LAUNCH(KK)                                                     \
-  hipLaunchKernelGGL((k_hll_spmm_run<VT, XT, DT, OT, KK, 256, EPI>),        \
-                     dim3(ngrp * CoopChunk<XT, KK, 256>::NCH), dim3(kBlock), \
+  hipLaunchKernelGGL((k_hll_spmm_run<VT, XT, DT, OT, KK, kRunChunkBytes,   \
+                                     EPI>),                                 \
+                     dim3(ngrp * CoopChunk<XT, KK, kRunChunkBytes>::NCH),   \
+                     dim3(kBlock),                                          \
                      0, s, a.n, a.ncols, a.nslices, sp, cl, vl, x, k, d, out, \
                      alpha, beta, kw)
   if (k <= 2)
@@ -1324,15 +1349,26 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part, kw);                                 \
+    else if constexpr (KK >= 16 && NPP == 2)                                 \
+      /* 3K FP64 sums per lane do not fit: the cooperative kernel instead */ \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part, kw);                                 \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_row<XT, KK, NPP>), dim3(g),             \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part);                                     \
   } while (0)
-// K = 64 exists only in the cooperative mapping
+// K = 64 exists only in the cooperative mapping; FP32 rows of 64 columns
+// (256 bytes) run as two 128-byte chunks in both variants (16 lanes per row
+// would spill the FP64 sums)
 #define AINV_LC(KK, NPP)                                                     \
   do {                                                                       \
-    if (map == AINV_MAP_COOP128)                                             \
+    if constexpr (sizeof(XT) == 4)                                           \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
+                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
+                         k, y, w, part, kw);                                 \
+    else if (map == AINV_MAP_COOP128)                                        \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),       \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
                          k, y, w, part, kw);                                 \
