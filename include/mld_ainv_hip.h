@@ -454,6 +454,20 @@ int ainv_apply_profile(ainv_h h, const void *x, double *y, int k, int reps,
 /* Same for the system matrix product y = A x. */
 int ainv_mat_profile(ainv_mat_h a, const double *x, double *y, int k, int reps,
                      double *ms);
+/* The Krylov step's fused system-matrix product in one mapping (map:
+ * AINV_MAP_SPLIT, _ROW, _COOP, _COOP128 or _RUN as numbered for APPLY_MAP):
+ * y = A x for k columns interleaved with leading dimension K (the power of
+ * two >= k; x FP32 when x_is_f32), and per column c the dots
+ * dots[c] = (w_c, y_c) (np = 1, BiCGSTAB's (r~, v)) or dots[c] = (y_c, y_c),
+ * dots[k + c] = (y_c, w_c) (np = 2, its (t, t) and (t, s)), summed over the
+ * blocks in the order the Krylov step sums them (host array of np * k).
+ * x, y, w device pointers; ms (may be NULL): average device time of ten
+ * more launches.  No counterpart in the reference (psb_spmm and psb_gedot
+ * there, tests/pdegen/ppde3d.f90:280 through psb_krylov); for checking the
+ * mappings against each other. */
+int ainv_mat_spmv_dot(ainv_mat_h a, const void *x, int x_is_f32, double *y,
+                      int k, const double *w, int np, int map, double *dots,
+                      double *ms);
 /* Average device time (ms) of `reps` Krylov solves from x0 = 0 with the
  * last solve's iteration counts and errors (timed with HIP events). */
 int ainv_krylov_profile(const char *method, ainv_mat_h a, ainv_h prec,

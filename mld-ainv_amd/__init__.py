@@ -123,6 +123,8 @@ def lib():
         "ainv_device_sync": [],
         "ainv_apply_profile": [vp, vp, vp, C.c_int, C.c_int, C.c_int, dp, dp],
         "ainv_mat_profile": [vp, vp, vp, C.c_int, C.c_int, dp],
+        "ainv_mat_spmv_dot": [vp, vp, C.c_int, vp, C.c_int, vp, C.c_int,
+                              C.c_int, dp, dp],
         "ainv_krylov_profile": [C.c_char_p, vp, vp, vp, vp, C.c_int,
                                 C.c_double, C.c_int, C.c_int, i32p, dp, dp],
         "ainv_layout": [vp, i64p, i64p, i64p, i64p, i64p, P(C.c_int)],
@@ -557,6 +559,30 @@ class SystemMatrix:
                                 None))
         synchronize()
         return y
+
+    def spmv_dot(self, x, w, np_=1, mapping="COOP", timed=False):
+        """The Krylov step's fused product in one mapping: y = A x for the
+        columns of x (numpy (n, K), K a power of two; FP32 or FP64) and the
+        per-column dots (w, y) (np_ = 1) or (y, y), (y, w) (np_ = 2).
+        Returns (y, dots[np_, K], ms or None)."""
+        maps = {"SPLIT": 0, "ROW": 1, "COOP": 2, "COOP128": 3, "RUN": 6}
+        x = np.ascontiguousarray(x)
+        n, K = x.shape
+        f32 = x.dtype == np.float32
+        xd = DeviceArray.from_numpy(x if f32 else x.astype(np.float64))
+        wd = DeviceArray.from_numpy(np.ascontiguousarray(w, dtype=np.float64))
+        yd = DeviceArray((n, K))
+        yd.zero()
+        dots = np.zeros(np_ * K, dtype=np.float64)
+        ms = C.c_double(0.0)
+        _ck(lib().ainv_mat_spmv_dot(
+            self._h, xd.ptr, int(f32), yd.ptr, K, wd.ptr, np_,
+            maps[mapping.upper()], dots.ctypes.data_as(C.POINTER(C.c_double)),
+            C.byref(ms) if timed else None))
+        y = yd.to_numpy()
+        for d in (xd, wd, yd):
+            d.free()
+        return y, dots.reshape(np_, K), (ms.value if timed else None)
 
     def free(self):
         if self._h:

@@ -158,8 +158,12 @@ constexpr int AINV_MAP_WIDE = 7;
 constexpr int kNumMaps = 8;
 // mappings of the factor applies (no CSR copy)
 constexpr int kNumApplyMaps = 7;
-// mappings the fused system-matrix product distinguishes
-constexpr int kNumSpmvMaps = 4;
+// mappings the fused system-matrix product distinguishes (timed by the
+// autotuner in this order)
+constexpr int kNumSpmvMaps = 5;
+constexpr int kSpmvMaps[kNumSpmvMaps] = {AINV_MAP_SPLIT, AINV_MAP_ROW,
+                                         AINV_MAP_COOP, AINV_MAP_COOP128,
+                                         AINV_MAP_RUN};
 // The two apply passes may use different mappings: a packed pair.
 inline int map_pair(int w, int z) { return w | (z << 4); }
 inline int map_w(int pair) { return pair & 15; }

@@ -599,17 +599,13 @@ template <typename VT, typename XT, int K, int CB> struct CoopWaves {
 };
 // The fused system-matrix product keeps FP64 sums of its FP64 matrix: at
 // 32-column chunks it needs the registers of 2 waves per SIMD (measured at
-// 3: 1.04 against 0.80 ms per launch at 160^3).
+// 3: 1.04 against 0.80 ms per launch at 160^3); its budget is DotWaves.
 #if AINV_ROW_WAVES > 0
 #define AINV_COOP_ATTR(XT)                                                  \
   __attribute__((amdgpu_waves_per_eu(CoopWaves<VT, XT, K, CB>::v,           \
                                      CoopWaves<VT, XT, K, CB>::v)))
-#define AINV_DOT_ATTR(XT)                                                   \
-  __attribute__((amdgpu_waves_per_eu(CoopWaves<double, XT, K, CB>::v,       \
-                                     CoopWaves<double, XT, K, CB>::v)))
 #else
 #define AINV_COOP_ATTR(XT)
-#define AINV_DOT_ATTR(XT)
 #endif
 
 // Slice positions per trip of the cooperative loop.  For chunks of 32
@@ -891,45 +887,25 @@ template <typename VT, typename XT, int K> struct RunWaves {
 #define AINV_RUN_ATTR
 #endif
 
-template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
-          int EPI>
-__global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
-    int32_t n, int32_t ncols, int32_t nslices, const int64_t *__restrict__ sptr,
+// Sums of rows rb .. rb + L - 1 of one slice (entries of lane l at
+// base + 64 j + l, rb = L * (lane / L)) for this lane's CPL columns of a
+// chunk, x rows handed on along runs of consecutive diagonals.  x is read
+// through the buffer resource xr (lane_off: the lane's byte offset in a
+// row); live false: the lane's columns are beyond the active width and it
+// only takes part in the shuffles.
+template <typename VT, typename XT, int K, int KC>
+__device__ __forceinline__ void run_acc(
+    int64_t base, int32_t width, int lane, bool live,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
-    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
-    OT *__restrict__ out, double alpha, double beta,
-    const int32_t *__restrict__ kwp) {
+    __amdgpu_buffer_rsrc_t xr, uint32_t lane_off,
+    typename Val<VT>::acc (&acc)[Coop<XT, KC>::L][Coop<XT, KC>::CPL]) {
   using A = typename Val<VT>::acc;
-  using CH = CoopChunk<XT, K, CB>;
-  constexpr int KC = CH::KC;
   using CP = Coop<XT, KC>;
   constexpr int L = CP::L, CPL = CP::CPL, VB = CP::VB;
   constexpr int NW = VB / 4;  // dwords per lane load
   static_assert(CPL * (int)sizeof(XT) == VB, "lane load holds CPL values");
-  const int lane = threadIdx.x & (kSlice - 1);
-  const int sub = lane % L, rg = lane / L;
-  const int rb = rg * L;                  // first slice row of the group
-  const bool last_group = rg == CP::R - 1;  // rows up to 63: no next group
-  const int kw = kwp ? min(k, *kwp) : k;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int32_t ngrp = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int32_t lb = __builtin_amdgcn_readfirstlane(
-      xcd_block(blockIdx.x, gridDim.x));
-  const int chunk = lb / ngrp;
-  if (chunk * KC >= kw) return;
-  const int32_t s = lb % ngrp * kWavesPerBlock + wv;
-  if (s >= nslices) return;
-  const int64_t base = sptr[s];
-  const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
-  const int c0 = chunk * KC + sub * CPL;  // first column of this lane
-  const bool live = c0 < kw;
-  // x has ncols rows of K values (the launcher keeps this below
-  // kRunMaxBytes)
-  const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<XT *>(x), 0, (int)((int64_t)ncols * K * (int64_t)sizeof(XT)),
-      0x00020000);
-  const uint32_t lane_off = (uint32_t)(c0 * (int)sizeof(XT));
-  A acc[L][CPL];
+  const int rb = lane / L * L;                  // first slice row of the group
+  const bool last_group = lane / L == CP::R - 1;  // rows up to 63: no next
   uint32_t xv[L][NW];  // x values of each pass at the current entry
   int32_t held[L];     // their columns (-1: nothing held)
 #pragma unroll
@@ -1008,6 +984,50 @@ __global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
       vwc[t] = vwn[t];
     }
   }
+}
+
+// Buffer resource over an x block of ncols rows of K values (callers keep
+// its byte size below kRunMaxBytes).
+template <typename XT, int K>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t run_rsrc(const XT *x,
+                                                           int32_t ncols) {
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<XT *>(x), 0, (int)((int64_t)ncols * K * (int64_t)sizeof(XT)),
+      0x00020000);
+}
+
+template <typename VT, typename XT, typename DT, typename OT, int K, int CB,
+          int EPI>
+__global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
+    int32_t n, int32_t ncols, int32_t nslices, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ col, const VT *__restrict__ val,
+    const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ kwp) {
+  using A = typename Val<VT>::acc;
+  using CH = CoopChunk<XT, K, CB>;
+  constexpr int KC = CH::KC;
+  using CP = Coop<XT, KC>;
+  constexpr int L = CP::L, CPL = CP::CPL;
+  const int lane = threadIdx.x & (kSlice - 1);
+  const int sub = lane % L;
+  const int rb = lane / L * L;  // first slice row of the row group
+  const int kw = kwp ? min(k, *kwp) : k;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int32_t ngrp = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int32_t lb = __builtin_amdgcn_readfirstlane(
+      xcd_block(blockIdx.x, gridDim.x));
+  const int chunk = lb / ngrp;
+  if (chunk * KC >= kw) return;
+  const int32_t s = lb % ngrp * kWavesPerBlock + wv;
+  if (s >= nslices) return;
+  const int64_t base = sptr[s];
+  const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+  const int c0 = chunk * KC + sub * CPL;  // first column of this lane
+  A acc[L][CPL];
+  run_acc<VT, XT, K, KC>(base, width, lane, c0 < kw, col, val,
+                         run_rsrc<XT, K>(x, ncols),
+                         (uint32_t)(c0 * (int)sizeof(XT)), acc);
 #pragma unroll
   for (int q = 0; q < L; ++q) {
     const int32_t r = s * kSlice + rb + q;
@@ -1054,9 +1074,23 @@ void launch_run(const Hll &a, const XT *x, int k, const DT *d, OT *out,
 // lane chunk hold partial sums of the same columns; they are reduced by
 // shuffles across the row groups, then across the waves in LDS; a block
 // writes zero partial sums for the columns of the other chunks.
-template <typename XT, int K, int CB, int NP>
-__global__ __launch_bounds__(kBlock) AINV_DOT_ATTR(XT) void
-k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
+// RUN: the run mapping's lane layout and hand-on of x rows along runs of
+// consecutive diagonals (run_acc) instead of the cooperative one; the same
+// row sums, the dots summed over a lane's rows in another order.
+// (run: FP64 sums of 8 rows per lane at 128-byte rows, of 4 rows at 64)
+template <typename XT, int K, int CB, int RUN> struct DotWaves {
+  static constexpr int rb = RunWaves<double, XT, K>::rb;
+  static constexpr int v =
+      RUN ? (rb >= 128 ? 2 : rb == 64 ? 3 : AINV_ROW_WAVES)
+          : CoopWaves<double, XT, K, CB>::v;
+};
+template <typename XT, int K, int CB, int NP, int RUN = 0>
+__global__ __launch_bounds__(kBlock)
+#if AINV_ROW_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(DotWaves<XT, K, CB, RUN>::v,
+                                   DotWaves<XT, K, CB, RUN>::v)))
+#endif
+void k_hll_spmv_dot_coop(int32_t n, int32_t ncols, int32_t nslices,
                     const int64_t *__restrict__ sptr,
                     const int32_t *__restrict__ col,
                     const double *__restrict__ val,
@@ -1104,11 +1138,16 @@ k_hll_spmv_dot_coop(int32_t n, int32_t nslices,
     for (int q = 0; q < L; ++q)
 #pragma unroll
       for (int u = 0; u < CPL; ++u) acc[q][u] = 0.0;
-    coop_acc<double, XT, K, KC, Coop<XT, KC>::L, RowGroup<double, KC>::v>(
-        base, width, lane, rg, 0, col, val, x + c0, c0 < kw, acc);
+    if constexpr (RUN)
+      run_acc<double, XT, K, KC>(base, width, lane, c0 < kw, col, val,
+                                 run_rsrc<XT, K>(x, ncols),
+                                 (uint32_t)(c0 * (int)sizeof(XT)), acc);
+    else
+      coop_acc<double, XT, K, KC, Coop<XT, KC>::L, RowGroup<double, KC>::v>(
+          base, width, lane, rg, 0, col, val, x + c0, c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
-      const int32_t r = s * kSlice + q * R + rg;
+      const int32_t r = RUN ? s * kSlice + rg * L + q : s * kSlice + q * R + rg;
       if (r >= n) continue;
       double wr[CPL];
       load_xrow<double, CPL>(c0 < kw ? w + (int64_t)r * K + c0 : w + c0, wr);
@@ -1321,8 +1360,12 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
                   hipStream_t s, const int32_t *kw) {
   if (a.prec != AINV_PREC_F64)
     throw Error(AINV_ERR_INTERNAL, "system matrix must be FP64");
-  // the fused product + dots has no quarter-slice or wide variant
-  if (map == AINV_MAP_COOP_Q || map == AINV_MAP_RUN) map = AINV_MAP_COOP;
+  // the fused product + dots has no quarter-slice or wide variant; the run
+  // mapping addresses x through 32-bit buffer offsets
+  if (map == AINV_MAP_COOP_Q ||
+      (map == AINV_MAP_RUN &&
+       (int64_t)a.ncols * 64 * (int64_t)sizeof(XT) >= kRunMaxBytes))
+    map = AINV_MAP_COOP;
   if (map == AINV_MAP_COOP128_Q || map == AINV_MAP_WIDE)
     map = AINV_MAP_COOP128;
   if (k < 1 || k > 64) throw Error(AINV_ERR_ARG, "k must be in 1..64");
@@ -1331,7 +1374,8 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
   const int32_t *rl = a.rlen.as<int32_t>();
   const int32_t *cl = a.col.as<int32_t>();
   const double *vl = a.val.as<double>();
-  const bool coop = map == AINV_MAP_COOP || map == AINV_MAP_COOP128;
+  const bool coop = map == AINV_MAP_COOP || map == AINV_MAP_COOP128 ||
+                    map == AINV_MAP_RUN;
   if ((map == AINV_MAP_ROW && k > 1 && k <= 32) ||
       (coop && k > 1 && k <= 64)) {
     int K = 2;
@@ -1341,19 +1385,24 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
                   "row mapping needs ld = K and 16-byte aligned vectors");
 #define AINV_LR(KK, NPP)                                                     \
   do {                                                                       \
-    if (map == AINV_MAP_COOP)                                                \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),       \
-                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, kw);                                 \
+    if (map == AINV_MAP_RUN)                                                 \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, kRunChunkBytes, NPP,   \
+                                              1>),                           \
+                         dim3(g), dim3(kBlock), 0, s, a.n, a.ncols,          \
+                         a.nslices, sp, cl, vl, x, k, y, w, part, kw);       \
+    else if (map == AINV_MAP_COOP)                                           \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),   \
+                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
+                         cl, vl, x, k, y, w, part, kw);                      \
     else if (map == AINV_MAP_COOP128)                                        \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),       \
-                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, kw);                                 \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
+                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
+                         cl, vl, x, k, y, w, part, kw);                      \
     else if constexpr (KK >= 16 && NPP == 2)                                 \
       /* 3K FP64 sums per lane do not fit: the cooperative kernel instead */ \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, kw);                                 \
+                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
+                         cl, vl, x, k, y, w, part, kw);                      \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_row<XT, KK, NPP>), dim3(g),             \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
@@ -1364,18 +1413,23 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
 // would spill the FP64 sums)
 #define AINV_LC(KK, NPP)                                                     \
   do {                                                                       \
-    if constexpr (sizeof(XT) == 4)                                           \
+    if (map == AINV_MAP_RUN)                                                 \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, kRunChunkBytes, NPP,   \
+                                              1>),                           \
+                         dim3(g), dim3(kBlock), 0, s, a.n, a.ncols,          \
+                         a.nslices, sp, cl, vl, x, k, y, w, part, kw);       \
+    else if constexpr (sizeof(XT) == 4)                                      \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, kw);                                 \
+                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
+                         cl, vl, x, k, y, w, part, kw);                      \
     else if (map == AINV_MAP_COOP128)                                        \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),       \
-                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, kw);                                 \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
+                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
+                         cl, vl, x, k, y, w, part, kw);                      \
     else                                                                     \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),       \
-                         dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, kw);                                 \
+      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),   \
+                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
+                         cl, vl, x, k, y, w, part, kw);                      \
   } while (0)
 #define AINV_LRK(NPP)   \
   if (K == 2)           \

@@ -1670,6 +1670,57 @@ int ainv_mat_profile(ainv_mat_h a, const double *x, double *y, int k, int reps,
   });
 }
 
+int ainv_mat_spmv_dot(ainv_mat_h a, const void *x, int x_is_f32, double *y,
+                      int k, const double *w, int np, int map, double *dots,
+                      double *ms) {
+  return guard([&] {
+    if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
+    if (!x || !y || !w || !dots) throw Error(AINV_ERR_ARG, "NULL vector");
+    if (k < 1 || k > kMaxCols) throw Error(AINV_ERR_ARG, "k must be in 1..64");
+    if (np != 1 && np != 2) throw Error(AINV_ERR_ARG, "np must be 1 or 2");
+    if (map != AINV_MAP_SPLIT && map != AINV_MAP_ROW &&
+        map != AINV_MAP_COOP && map != AINV_MAP_COOP128 &&
+        map != AINV_MAP_RUN)
+      throw Error(AINV_ERR_ARG, "not a mapping of the fused product");
+    int K = 1;
+    while (K < k) K <<= 1;
+    set_device(a->device);
+    DevBuf part;
+    part.alloc((size_t)2 * kRedBlocks * kMaxCols * sizeof(double));
+    auto run = [&] {
+      return x_is_f32
+                 ? hll_spmv_dot(a->hll, static_cast<const float *>(x), y, k,
+                                K, w, np, part.as<double>(), map, a->stream,
+                                nullptr)
+                 : hll_spmv_dot(a->hll, static_cast<const double *>(x), y, k,
+                                K, w, np, part.as<double>(), map, a->stream,
+                                nullptr);
+    };
+    int g = run();
+    if (ms) {
+      Events e(2);
+      AINV_HIP_CHECK(hipEventRecord(e.ev[0], a->stream));
+      for (int r = 0; r < 10; ++r) g = run();
+      AINV_HIP_CHECK(hipEventRecord(e.ev[1], a->stream));
+      AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
+      *ms = e.ms(0, 1) / 10;
+    }
+    std::vector<double> h((size_t)2 * kRedBlocks * kMaxCols);
+    AINV_HIP_CHECK(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, a->stream));
+    AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
+    // the blocks' partial sums in block order, as the Krylov step's
+    // column-state kernel adds them
+    for (int q = 0; q < np; ++q)
+      for (int c = 0; c < k; ++c) {
+        double t = 0.0;
+        for (int b = 0; b < g; ++b)
+          t += h[((size_t)q * kRedBlocks + b) * kMaxCols + c];
+        dots[(size_t)q * k + c] = t;
+      }
+  });
+}
+
 int ainv_krylov_profile(const char *method, ainv_mat_h a, ainv_h prec,
                         const double *b, double *x, int k, double eps,
                         int itmax, int reps, int32_t *iter, double *err,

@@ -774,8 +774,9 @@ void prepare_method(KrylovPlan *p, ainv_matrix *a, ainv_solver *m) {
   }
 }
 
-// Autotuner: time the kernel mappings (split, row, coop, coop128) of each
-// apply pass and of the fused system-matrix product on this plan's buffers
+// Autotuner: time the kernel mappings (split, row, coop, coop128, their
+// quarter-slice variants and run) of each apply pass and of the fused
+// system-matrix product (split, row, coop, coop128, run) on this plan's buffers
 // and keep the fastest of each; results are cached on the solver / matrix
 // per K.
 double time_ms(hipStream_t s, const std::function<void()> &f, int reps) {
@@ -870,8 +871,8 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
     // BiCGSTAB with FP32 / BF16 factors multiplies the FP32 preconditioned
     // vectors (f, z): time that form
     const float *f32 = p->method == KM_BICGSTAB ? x32_vecs(p).f32 : nullptr;
-    int best = 0;
-    for (int map = 0; map < kNumSpmvMaps; ++map) {
+    int best = kSpmvMaps[0];
+    for (const int map : kSpmvMaps) {
       a->spmv_ms[li][map] = time_ms(s, [&] {
         if (f32)
           hll_spmv_dot(a->hll, f32, v.z, p->k, K, v.r, 1, p->pt, map, s);
@@ -2271,9 +2272,14 @@ void move_columns(SlotBatch *b, SlotBatch *nb, const std::vector<int> &keep) {
 // mappings tuned and its iteration captured), K columns of the queue are
 // admitted and, after two steps, four replayed steps are timed; the width
 // with the least time per column-iteration is kept per (preconditioner,
-// method) until the factors change.  The plans the trial made for widths
+// method) until the factors change, a wider width only when it is at least
+// kWiderGain faster than the narrower ones: a step of twice the width takes
+// about twice as long, so every solve's latency grows with the width, and
+// the trial's own timings vary by 1-2 %.  The plans the trial made for widths
 // above the chosen one are dropped again (each holds the iteration's
 // vectors at its width; the narrower ones serve the queue's tail).
+constexpr double kWiderGain = 0.03;
+
 int tune_slots(ainv_matrix *a, ainv_solver *m, int mid, const double *b,
                int nrhs, double eps, hipStream_t s) {
   if (m && m->auto_slots[mid] > 0 && m->auto_slots_gen[mid] == m->generation)
@@ -2310,7 +2316,7 @@ int tune_slots(ainv_matrix *a, ainv_solver *m, int mid, const double *b,
     }
     batch_close(bt);
     if (m) m->slot_ms[mid][log2_pow2(K)] = ms;
-    if (best == 0 || ms < best_ms) {
+    if (best == 0 || ms < best_ms * (1.0 - kWiderGain)) {
       best = K;
       best_ms = ms;
     }

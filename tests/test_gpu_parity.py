@@ -238,6 +238,48 @@ def test_run_mapping_bit_identical(gpu, O, prec, k):
         assert np.linalg.norm(YT[:, c] - ref) <= tol * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("kind", ["pde3d", "aniso2d", "random"])
+@pytest.mark.parametrize("xprec", ["F64", "F32"])
+@pytest.mark.parametrize("k", [2, 8, 16, 32, 64])
+def test_fused_product_mappings(gpu, kind, xprec, k):
+    """The Krylov step's fused system-matrix product v = A f and its dots
+    in every mapping: the products agree bit for bit across mappings (each
+    sums a row's entries in entry order), including the run mapping that
+    hands x rows on along the stencils' runs of diagonals (the random
+    matrix has almost none, so its rows gather everything), and with the
+    FP64 CSR product to rounding; the dots (BiCGSTAB's (r~, v), and (t, t),
+    (t, s)) agree with numpy's to rounding."""
+    a = {"pde3d": lambda: gpu.gen_pde3d(14),
+         "aniso2d": lambda: gpu.gen_aniso2d(50, 1e-3),
+         "random": lambda: gpu.gen_random(3000, seed=5)}[kind]()
+    n = a.shape[0]
+    A = gpu.SystemMatrix(a)
+    x = gpu.gen_normal(21, (n, k))
+    if xprec == "F32":
+        x = x.astype(np.float32)
+    w = gpu.gen_normal(22, (n, k))
+    y_ref = a @ x.astype(np.float64)
+    for np_ in (1, 2):
+        ys = {}
+        for mapping in ("SPLIT", "ROW", "COOP", "COOP128", "RUN"):
+            y, dots, _ = A.spmv_dot(x, w, np_=np_, mapping=mapping)
+            ys[mapping] = y
+            assert np.linalg.norm(y - y_ref) <= 1e-14 * np.linalg.norm(y_ref)
+            ny = np.linalg.norm(y, axis=0)
+            nw = np.linalg.norm(w, axis=0)
+            if np_ == 1:
+                ref = np.einsum("ij,ij->j", w, y)[None, :]
+                scale = (nw * ny)[None, :]
+            else:
+                ref = np.stack([np.einsum("ij,ij->j", y, y),
+                                np.einsum("ij,ij->j", y, w)])
+                scale = np.stack([ny * ny, ny * nw])
+            assert np.all(np.abs(dots - ref) <= 1e-13 * scale), mapping
+        for mapping, y in ys.items():
+            assert np.array_equal(y, ys["SPLIT"]), mapping
+    A.free()
+
+
 @pytest.mark.parametrize("method,k,mapping", [
     ("BICGSTAB", 8, "ROW"), ("BICGSTAB", 5, "ROW"), ("CG", 16, "ROW"),
     ("BICGSTAB", 8, "COOP"), ("BICGSTAB", 3, "COOP"), ("CG", 32, "COOP"),

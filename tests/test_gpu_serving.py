@@ -158,9 +158,10 @@ def test_krylov_stream_zero_rhs_and_slot_reuse(gpu, method):
 
 def test_krylov_stream_tuned_width(gpu):
     """slots = 0: the first call times 8, 16, 32 and 64 slots (no wider than
-    the queue rounded up to a power of two), reports the chosen width and
-    every width's time, keeps the choice for later calls and tunes again
-    after the factors change; the results are those of solving alone."""
+    the queue rounded up to a power of two), keeps the narrowest width that
+    no wider one beats by 3 % per column-iteration, reports it and every
+    width's time, keeps the choice for later calls and tunes again after
+    the factors change; the results are those of solving alone."""
     a = gpu.gen_pde3d(12)
     n = a.shape[0]
     sv = gpu.SOLVERS["INVK"]()
@@ -179,7 +180,11 @@ def test_krylov_stream_tuned_width(gpu):
     t = ss[0]["ms_per_col_iter"]
     assert sorted(t, key=int) == ["8", "16", "32", "64"]
     assert all(v > 0 for v in t.values())
-    assert ss[0]["slots"] == int(min(t, key=t.get))
+    best = 8
+    for w in (16, 32, 64):
+        if t[str(w)] < t[str(best)] * (1 - 0.03):
+            best = w
+    assert ss[0]["slots"] == best
     Xh = X.to_numpy()
     assert np.all(err <= 1e-9) and np.all(lat > 0)
     for j in (0, 17, 39):
