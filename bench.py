@@ -62,7 +62,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_k32.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_pde3d160.json")
 
 # name: (description, solver kind, parameters, method, rtol, slots (0: the
 #        served form times 8, 16, 32 and 64 slots on its first call and keeps
@@ -646,7 +646,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sv, a, Bh[sample[0]], method, args.cpu_iters,
-                           float(np.mean(it64)))
+                           float(np.mean(it64)), eps=eps)
         log(f"cpu baseline {cpu}")
 
     # the serving stream over the 24-matrix pool (its own matrices; the
@@ -712,6 +712,7 @@ def main():
             "bytes_per_launch": dom["bytes_per_launch"],
             "ms_per_launch": dom["ms_per_launch"],
             "achievable_copy_GBs": copy_gbs,
+            "achievable_copy_runtime_GBs": m.runtime_copy_gbs(1 << 30, 10),
             "other_pass": other,
         },
         "build": {"seconds": t_build},
@@ -1100,6 +1101,21 @@ def _timed_legs(run, label, gpu_iters, threads):
         one = leg(1)
         one["host"] = {"cpu_model": model, "nproc": nproc,
                        "compiler": O.build_info()}
+        expect = one["seconds_per_iteration"] * gpu_iters
+        if eps is not None and expect <= CPU_FULL_SOLVE_MAX_S:
+            O.set_threads(1)
+            t0 = time.perf_counter()
+            _, it, err = O.krylov(method, A, W, Z, d, b, eps, itmax=100000)
+            dt = time.perf_counter() - t0
+            one["full_solve"] = {
+                "seconds": dt, "solves_per_s": 1.0 / dt, "iterations": it,
+                "relative_residual": err, "cores": 1,
+                "note": f"one whole {method} solve of the same right-hand "
+                        f"side to {eps:g} on one thread"}
+        elif eps is not None:
+            one["full_solve"] = {
+                "skipped": f"predicted {expect:.0f} s > "
+                           f"{CPU_FULL_SOLVE_MAX_S:.0f} s"}
         if threads > 1:
             mt = leg(threads)
             mt["note"] = ("the same loops row-parallel with OpenMP")
@@ -1136,13 +1152,20 @@ def ml_cpu_baseline(pv, a, b, method, mp, gpu_iters, threads=None):
                        "side of the queue", gpu_iters, threads)
 
 
-def cpu_baseline(sv, a, b, method, iters, gpu_iters, threads=None):
+# a whole CPU solve is timed beside the per-iteration sample when it is
+# expected to take at most this long
+CPU_FULL_SOLVE_MAX_S = 30.0
+
+
+def cpu_baseline(sv, a, b, method, iters, gpu_iters, threads=None, eps=None):
     """The oracle's Krylov method on the same system with the same FP64
     factors: one warm-up, then 5 timed runs of `iters` iterations each
     (`iters` = None: as many as take about 2 s), median time per iteration,
     extrapolated to one solve at the FP64 GPU path's mean iteration count.
     Returns the single-thread figure (the reference's one rank, no threads)
-    and, beside it, the same loops row-parallel on `threads` host cores."""
+    and, beside it, the same loops row-parallel on `threads` host cores.
+    With eps, one whole single-thread solve to that tolerance is timed too
+    (`full_solve`) when the sample predicts at most CPU_FULL_SOLVE_MAX_S."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     sv.cnv("F64")

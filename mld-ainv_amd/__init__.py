@@ -240,6 +240,28 @@ def stream_copy_gbs(nbytes=1 << 30, reps=20):
     return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
 
+def runtime_copy_gbs(nbytes=1 << 30, reps=10):
+    """A second reference for the achievable bandwidth: the HIP runtime's
+    own device-to-device copy (hipMemcpy) of nbytes, timed on the host over
+    reps copies after one warm-up, with a device synchronization on both
+    sides (a device-to-device hipMemcpy may return before the copy ends);
+    read + write bytes over the time, GB/s."""
+    import time
+    src = DeviceArray((nbytes // 8,))
+    dst = DeviceArray((nbytes // 8,))
+    src.zero()
+    _ck(lib().ainv_memcpy(dst.ptr, src.ptr, nbytes, 3))
+    _ck(lib().ainv_device_sync())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _ck(lib().ainv_memcpy(dst.ptr, src.ptr, nbytes, 3))
+    _ck(lib().ainv_device_sync())
+    dt = (time.perf_counter() - t0) / reps
+    src.free()
+    dst.free()
+    return 2.0 * nbytes / dt / 1e9
+
+
 # ------------------------------------------------------------------- CSR in
 def _csr_args(a):
     """(n, irp int64, ja int32, val f64) from scipy CSR or a tuple."""
