@@ -328,7 +328,7 @@ def ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps, slots,
     cpu = None
     if not args.no_cpu:
         cpu = ml_cpu_baseline(pv, a, Bh[sample[0]], method, mp,
-                              float(np.mean(its)))
+                              float(np.mean(its)), eps=eps)
         log(f"multilevel cpu baseline {cpu}")
     return {
         "params": mp, "method": method,
@@ -1067,11 +1067,37 @@ def _host_info():
     return model, len(os.sched_getaffinity(0))
 
 
-def _timed_legs(run, label, gpu_iters, threads):
+# a whole CPU solve is timed beside the per-iteration sample when it is
+# expected to take at most this long
+CPU_FULL_SOLVE_MAX_S = 30.0
+
+
+def _full_solve(one, full, gpu_iters, O):
+    """full: (label, callable running one whole single-thread solve and
+    returning (iterations, final relative residual)), or None.  Timed when
+    the per-iteration sample in `one` predicts at most CPU_FULL_SOLVE_MAX_S."""
+    if full is None:
+        return
+    label, solve = full
+    expect = one["seconds_per_iteration"] * gpu_iters
+    if expect > CPU_FULL_SOLVE_MAX_S:
+        one["full_solve"] = {"skipped": f"predicted {expect:.0f} s > "
+                                        f"{CPU_FULL_SOLVE_MAX_S:.0f} s"}
+        return
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    it, err = solve()
+    dt = time.perf_counter() - t0
+    one["full_solve"] = {"seconds": dt, "solves_per_s": 1.0 / dt,
+                         "iterations": int(it), "relative_residual": err,
+                         "cores": 1, "note": label}
+
+
+def _timed_legs(run, label, gpu_iters, threads, full=None):
     """One warm-up iteration (which sizes the runs to about 2 s), then 5
     timed runs; median seconds per iteration extrapolated to one solve of
     gpu_iters iterations; single-thread figure with the threaded one beside
-    it."""
+    it (and one whole solve, see _full_solve)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     model, nproc = _host_info()
@@ -1101,21 +1127,7 @@ def _timed_legs(run, label, gpu_iters, threads):
         one = leg(1)
         one["host"] = {"cpu_model": model, "nproc": nproc,
                        "compiler": O.build_info()}
-        expect = one["seconds_per_iteration"] * gpu_iters
-        if eps is not None and expect <= CPU_FULL_SOLVE_MAX_S:
-            O.set_threads(1)
-            t0 = time.perf_counter()
-            _, it, err = O.krylov(method, A, W, Z, d, b, eps, itmax=100000)
-            dt = time.perf_counter() - t0
-            one["full_solve"] = {
-                "seconds": dt, "solves_per_s": 1.0 / dt, "iterations": it,
-                "relative_residual": err, "cores": 1,
-                "note": f"one whole {method} solve of the same right-hand "
-                        f"side to {eps:g} on one thread"}
-        elif eps is not None:
-            one["full_solve"] = {
-                "skipped": f"predicted {expect:.0f} s > "
-                           f"{CPU_FULL_SOLVE_MAX_S:.0f} s"}
+        _full_solve(one, full, gpu_iters, O)
         if threads > 1:
             mt = leg(threads)
             mt["note"] = ("the same loops row-parallel with OpenMP")
@@ -1125,7 +1137,8 @@ def _timed_legs(run, label, gpu_iters, threads):
     return one
 
 
-def ml_cpu_baseline(pv, a, b, method, mp, gpu_iters, threads=None):
+def ml_cpu_baseline(pv, a, b, method, mp, gpu_iters, threads=None,
+                    eps=None):
     """The oracle's cycle inside its Krylov method on the same hierarchy
     (every level's A, P, R and FP64 factors exported from the library, which
     builds them bit-identical to the oracle's own build), timed as
@@ -1147,14 +1160,16 @@ def ml_cpu_baseline(pv, a, b, method, mp, gpu_iters, threads=None):
         _, it, _ = O.ml_krylov(method, A, H, b, 1e-30, itmax=mm)
         return it
 
+    def whole():
+        _, it, err = O.ml_krylov(method, A, H, b, eps, itmax=100000)
+        return it, err
+
+    full = None if eps is None else (
+        f"one whole {method} solve with the oracle's cycle, same right-hand "
+        f"side, to {eps:g} on one thread", whole)
     return _timed_legs(run, f"{method} iterations of the oracle's cycle on "
                        "the same hierarchy (FP64 factors) and one right-hand "
-                       "side of the queue", gpu_iters, threads)
-
-
-# a whole CPU solve is timed beside the per-iteration sample when it is
-# expected to take at most this long
-CPU_FULL_SOLVE_MAX_S = 30.0
+                       "side of the queue", gpu_iters, threads, full)
 
 
 def cpu_baseline(sv, a, b, method, iters, gpu_iters, threads=None, eps=None):
@@ -1195,10 +1210,18 @@ def cpu_baseline(sv, a, b, method, iters, gpu_iters, threads=None, eps=None):
                           f"to {gpu_iters:.1f} iterations per solve",
                 "seconds_per_iteration": per}
 
+    def whole():
+        _, it, err = O.krylov(method, A, W, Z, d, b, eps, itmax=100000)
+        return it, err
+
+    full = None if eps is None else (
+        f"one whole {method} solve of the same right-hand side to {eps:g} "
+        f"on one thread", whole)
     try:
         one = leg(1)
         one["host"] = {"cpu_model": model, "nproc": nproc,
                        "compiler": O.build_info()}
+        _full_solve(one, full, gpu_iters, O)
         if threads > 1:
             mt = leg(threads)
             mt["note"] = ("beyond the reference (one rank, no threads): the "
