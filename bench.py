@@ -409,7 +409,12 @@ def main():
                         "replays), for kernel traces")
     p.add_argument("--map", default="AUTO",
                    help="APPLY_MAP for --apply-only (AUTO, SPLIT, ROW, COOP, "
-                        "COOP128)")
+                        "COOP128, COOP_Q, COOP128_Q, RUN)")
+    p.add_argument("--order", default="",
+                   help="--apply-only: slice-group order of the run mapping "
+                        "as PERIOD,BAND (groups of 4 slices; the served "
+                        "run's tuned choice is in roofline.slice_order); "
+                        "default linear")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the "
@@ -451,6 +456,12 @@ def main():
     if args.apply_only:
         sv = new_solver(args.prec)
         sv.set("APPLY_MAP", args.map)
+        if args.order:
+            op, ob = (int(v) for v in args.order.split(","))
+            sv.set("ORDER_PERIOD", op)
+            sv.set("ORDER_BAND", ob)
+        else:
+            sv.set("ORDER_PERIOD", -1)
         sv.build(a)
         lay = sv.layout()
         x32 = method in ("BICGSTAB", "CG") and args.prec != "F64"
@@ -459,6 +470,7 @@ def main():
         bw = apply_pass_bytes(lay, kfix, True, 4 if x32 else 8)
         print(json.dumps({"apply_only": True, "config": args.config,
                           "prec": args.prec, "map": args.map, "k": kfix,
+                          "order": sv.report()["slice_order"]["w"],
                           "x": "FP32" if x32 else "FP64",
                           "ms_w": ms_w, "ms_z": ms_z, "bytes_w": bw,
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
@@ -583,6 +595,9 @@ def main():
     tuned = [t for t in rep.get("tuning", []) if t["K"] == slots]
     kmap_w = tuned[0]["map_w"] if tuned else "split"
     kmap_z = tuned[0]["map_z"] if tuned else "split"
+    so = rep.get("slice_order", {})
+    order_w = [so.get("w", {}).get("period", 0), so.get("w", {}).get("band", 0)]
+    order_z = [so.get("z", {}).get("period", 0), so.get("z", {}).get("band", 0)]
     it_red = [int(iters[j]) for j in sample]
     log(f"served {args.prec} slots={slots}: {rate:.2f} solves/s, "
         f"{elapsed / args.steps:.2f} s/step, iterations mean "
@@ -613,21 +628,25 @@ def main():
         yt = "float" if y32 else "double"
         return f"{kernel}<{vt}, {tt}, {tt}, {yt}, {slots},{extra} 1>"
 
+    # slice-group order ([period, band] in groups of 4 slices; [0, 0]:
+    # linear) of the run mapping's passes, as the tuner left them
     passes = [{"pass": "first: t = d o (W x)", "kernel": kname(kmap_w, True),
-               "bytes_per_launch": bw, "ms_per_launch": ms_w},
+               "bytes_per_launch": bw, "ms_per_launch": ms_w,
+               "slice_order": order_w if kmap_w == "run" else [0, 0]},
               {"pass": "second: y = Z t", "kernel": kname(kmap_z, False),
-               "bytes_per_launch": bz, "ms_per_launch": ms_z}]
+               "bytes_per_launch": bz, "ms_per_launch": ms_z,
+               "slice_order": order_z if kmap_z == "run" else [0, 0]}]
     traffic = {}
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f)
         if tr.get("config") == args.config and tr.get("n") == n:
-            traffic = {e["kernel"]: e["bytes_per_launch"]
-                       for e in tr.get("kernels", [])}
+            traffic = {(e["kernel"], tuple(e.get("slice_order", [0, 0]))):
+                       e["bytes_per_launch"] for e in tr.get("kernels", [])}
     for ps in passes:
         ps["achieved_GBs"] = ps["bytes_per_launch"] / (ps["ms_per_launch"]
                                                        * 1e-3) / 1e9
-        ps["traffic"] = traffic.get(ps["kernel"])
+        ps["traffic"] = traffic.get((ps["kernel"], tuple(ps["slice_order"])))
     dom = max(passes, key=lambda q: q["ms_per_launch"])
     other = passes[1] if dom is passes[0] else passes[0]
     log(f"apply at k={slots}: W {ms_w:.3f} ms "
@@ -709,6 +728,8 @@ def main():
             "frac": dom["achieved_GBs"] / HBM_PEAK_GBS,
             "traffic": dom["traffic"],
             "kernel": dom["kernel"], "pass": dom["pass"],
+            "slice_order": dom["slice_order"],
+            "slice_order_trial": so.get("trial"),
             "bytes_per_launch": dom["bytes_per_launch"],
             "ms_per_launch": dom["ms_per_launch"],
             "achievable_copy_GBs": copy_gbs,

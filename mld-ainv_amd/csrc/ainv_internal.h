@@ -78,6 +78,10 @@ struct DevCsr {
 // wave read consecutive words.  Padding entries carry value 0 and the row's
 // own index as column.  rlen holds each row's true length.
 constexpr int kSlice = 64;
+// slices per workgroup of the apply kernels (one wavefront each), and the
+// XCDs a launch's workgroups are dealt to round-robin
+constexpr int kSliceGroup = 4;
+constexpr int kXcdCount = 8;
 struct Hll {
   int32_t n = 0;
   int32_t ncols = 0;  // columns (rows of the x it multiplies)
@@ -93,9 +97,15 @@ struct Hll {
   // optional CSR copy (row pointers, columns, VT values) for the
   // one-wavefront-per-row mapping AINV_MAP_WIDE (long rows on few rows)
   DevBuf cptr, cidx, cval;
+  // optional order in which the run mapping's workgroups take their groups
+  // of 4 slices (int32 [ceil(nslices / 4)], a permutation within each XCD's
+  // range; empty: linear), set by hll_set_order: strips of `band` groups
+  // taken across `period`-group planes
+  DevBuf gorder;
+  int32_t order_period = 0, order_band = 0;
   int64_t bytes() const {
     return (int64_t)(sptr.bytes + rlen.bytes + col.bytes + val.bytes +
-                     cptr.bytes + cidx.bytes + cval.bytes);
+                     cptr.bytes + cidx.bytes + cval.bytes + gorder.bytes);
   }
 };
 
@@ -119,6 +129,16 @@ void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s);
 // HLL from device CSR, values converted to `prec`.  ncols: column count of
 // a rectangular matrix (padding entries point at a column below it); < 0
 // for a square one.
+// Median over slices of a middle row's reach (the larger of its distance
+// to its first and to its last column), in groups of 4 slices (hll.hip).
+int32_t hll_reach_groups(const Hll &h, hipStream_t s);
+// Slice-group order of the run mapping: the groups of each of the 8 XCD
+// ranges taken in strips -- `band` consecutive groups of every
+// `period`-group plane of the range, then the next band -- so that the x
+// rows a band's planes share (the factor's clusters of diagonals one and
+// two planes apart) are gathered while still in the XCD's L2.  period or
+// band <= 0: linear order (hll.hip).
+void hll_set_order(Hll &h, int32_t period, int32_t band, hipStream_t s);
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
                   int32_t ncols = -1);
 // Attaches the CSR copy of `a` (host, 0-based, rows sorted) to h, values in

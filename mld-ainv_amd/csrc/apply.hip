@@ -23,7 +23,7 @@
 namespace ainv {
 namespace {
 
-constexpr int kWavesPerBlock = 4;
+constexpr int kWavesPerBlock = kSliceGroup;
 constexpr int kBlock = kWavesPerBlock * kSlice;
 
 // Workgroups are dispatched round-robin over the XCDs (workgroup i runs on
@@ -34,7 +34,7 @@ constexpr int kBlock = kWavesPerBlock * kSlice;
 #ifndef AINV_XCD_REMAP
 #define AINV_XCD_REMAP 1
 #endif
-constexpr int kXcds = 8;
+constexpr int kXcds = kXcdCount;
 
 __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t g) {
 #if AINV_XCD_REMAP
@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
     OT *__restrict__ out, double alpha, double beta,
-    const int32_t *__restrict__ kwp) {
+    const int32_t *__restrict__ kwp, const int32_t *__restrict__ gorder) {
   using A = typename Val<VT>::acc;
   using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC;
@@ -1019,7 +1019,9 @@ __global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
       xcd_block(blockIdx.x, gridDim.x));
   const int chunk = lb / ngrp;
   if (chunk * KC >= kw) return;
-  const int32_t s = lb % ngrp * kWavesPerBlock + wv;
+  // slice group of this workgroup: linear, or the factor's strip order
+  const int32_t grp = gorder ? gorder[lb % ngrp] : lb % ngrp;
+  const int32_t s = grp * kWavesPerBlock + wv;
   if (s >= nslices) return;
   const int64_t base = sptr[s];
   const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
@@ -1051,7 +1053,7 @@ void launch_run(const Hll &a, const XT *x, int k, const DT *d, OT *out,
                      dim3(ngrp * CoopChunk<XT, KK, kRunChunkBytes>::NCH),   \
                      dim3(kBlock),                                          \
                      0, s, a.n, a.ncols, a.nslices, sp, cl, vl, x, k, d, out, \
-                     alpha, beta, kw)
+                     alpha, beta, kw, a.gorder.as<int32_t>())
   if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)

@@ -205,12 +205,34 @@ void clear_built(ainv_solver *h) {
   h->inv_report.clear();
 }
 
+// A set slice-group order (ORDER_PERIOD, ORDER_BAND > 0) on both factor
+// layouts, or linear order with ORDER_PERIOD -1 (0 leaves the layouts to
+// the timed trial of the Krylov tuner).
+void apply_fixed_order(ainv_solver *h) {
+  h->order_tuned = false;
+  h->order_report.clear();
+  if (h->kind == AINV_KIND_ML || !h->built) return;
+  if (h->order_period > 0 && h->order_band > 0) {
+    hll_set_order(h->hw, h->order_period, h->order_band, h->stream);
+    hll_set_order(h->hz, h->order_period, h->order_band, h->stream);
+  } else {
+    hll_set_order(h->hw, 0, 0, h->stream);
+    hll_set_order(h->hz, 0, 0, h->stream);
+  }
+}
+
 void build_layout(ainv_solver *h) {
   const double t0 = now_seconds();
   hll_from_csr(h->w, h->prec, h->hw, h->stream);
   hll_from_csr(h->z, h->prec, h->hz, h->stream);
   convert_vector(h->d64.as<double>(), h->n, h->prec, h->dv, h->stream);
   reset_tuning(h);
+  if (h->order_period > 0 && h->order_band > 0) {
+    hll_set_order(h->hw, h->order_period, h->order_band, h->stream);
+    hll_set_order(h->hz, h->order_period, h->order_band, h->stream);
+  }
+  h->order_tuned = false;
+  h->order_report.clear();
   h->generation = next_generation();
   h->have_t = false;
   h->hwt = Hll();
@@ -289,6 +311,8 @@ void copy_settings(const ainv_solver *src, ainv_solver *dst) {
   dst->inv_thresh = src->inv_thresh;
   dst->prec = src->prec;
   dst->map_mode = src->map_mode;
+  dst->order_period = src->order_period;
+  dst->order_band = src->order_band;
   dst->ml_nlev = src->ml_nlev;
   dst->ml_type = src->ml_type;
   dst->ml_pos = src->ml_pos;
@@ -706,6 +730,14 @@ int ainv_cseti(ainv_h h, const char *what, int val) {
       h->prec = val;
     } else if (w == "APPLY_MAP") {
       set_map_mode(h, val);
+    } else if (w == "ORDER_PERIOD" || w == "ORDER_BAND") {
+      if (val < (w == "ORDER_PERIOD" ? -1 : 0))
+        throw Error(AINV_ERR_ARG, w + " out of range");
+      (w == "ORDER_PERIOD" ? h->order_period : h->order_band) = val;
+      if (h->built) {
+        apply_fixed_order(h);
+        h->generation = next_generation();
+      }
     }
     // other keys go to the base solver, which ignores them
   });
@@ -785,6 +817,10 @@ int ainv_geti(ainv_h h, const char *what, int *val) {
       *val = h->prec;
     else if (w == "APPLY_MAP")
       *val = h->map_mode;
+    else if (w == "ORDER_PERIOD")
+      *val = h->order_period;
+    else if (w == "ORDER_BAND")
+      *val = h->order_band;
     else if (h->kind == AINV_KIND_ML && w == "N_PREC_LEVS")
       *val = h->ml_nlev;
     else if (h->kind == AINV_KIND_ML && w == "SMOOTHER_SWEEPS")
@@ -1255,6 +1291,14 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
       r += ", \"apply_map\": \"" +
            std::string(h->map_mode < 0 ? "auto" : map_name(h->map_mode)) +
            "\", \"tuning\": [" + tu + "]";
+      // slice-group order of the run mapping on each factor, and its trial
+      r += ", \"slice_order\": {\"w\": {\"period\": " +
+           std::to_string(h->hw.order_period) + ", \"band\": " +
+           std::to_string(h->hw.order_band) + "}, \"z\": {\"period\": " +
+           std::to_string(h->hz.order_period) + ", \"band\": " +
+           std::to_string(h->hz.order_band) + "}, \"trial\": " +
+           (h->order_report.empty() ? std::string("null")
+                                    : h->order_report) + "}";
     }
     if (h->built) r += served_slots_json(h);
     r += "}";

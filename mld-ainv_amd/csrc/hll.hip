@@ -4,6 +4,9 @@
 // kernels.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <vector>
+
 #include "ainv_internal.h"
 
 namespace ainv {
@@ -320,6 +323,89 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
     AINV_HIP_CHECK(hipGetLastError());
   }
   AINV_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace {
+// reach (rows) of each slice's middle row -- its last row for a short last
+// slice -- to its first and its last column (entries sorted by column)
+__global__ void k_slice_reach(int32_t n, int32_t nslices,
+                              const int64_t *__restrict__ sptr,
+                              const int32_t *__restrict__ rlen,
+                              const int32_t *__restrict__ col,
+                              int32_t *__restrict__ out) {
+  const int32_t s = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (s >= nslices) return;
+  const int32_t r0 = s * kSlice;
+  const int32_t lane = min(kSlice / 2, n - 1 - r0);
+  const int32_t r = r0 + lane;
+  const int32_t len = rlen[r];
+  int32_t reach = 0;
+  if (len > 0) {
+    const int32_t first = col[sptr[s] + lane];
+    const int32_t last = col[sptr[s] + (int64_t)(len - 1) * kSlice + lane];
+    reach = max(max(r - first, last - r), 0);
+  }
+  out[s] = reach;
+}
+}  // namespace
+
+int32_t hll_reach_groups(const Hll &h, hipStream_t s) {
+  if (h.nslices == 0 || h.padded == 0) return 0;
+  DevBuf out;
+  out.alloc((size_t)h.nslices * sizeof(int32_t));
+  hipLaunchKernelGGL(k_slice_reach, dim3(grid_for(h.nslices)), dim3(kThreads),
+                     0, s, h.n, h.nslices, h.sptr.as<int64_t>(),
+                     h.rlen.as<int32_t>(), h.col.as<int32_t>(),
+                     out.as<int32_t>());
+  AINV_HIP_CHECK(hipGetLastError());
+  std::vector<int32_t> r(h.nslices);
+  AINV_HIP_CHECK(hipMemcpyAsync(r.data(), out.p, out.bytes,
+                                hipMemcpyDeviceToHost, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  std::nth_element(r.begin(), r.begin() + r.size() / 2, r.end());
+  return r[r.size() / 2] / (kSliceGroup * kSlice);
+}
+
+void hll_set_order(Hll &h, int32_t period, int32_t band, hipStream_t s) {
+  const int32_t ngrp = (h.nslices + kSliceGroup - 1) / kSliceGroup;
+  const bool strips = period > 0 && band > 0 && ngrp >= kXcdCount;
+  // once allocated the table stays (captured graphs hold its address);
+  // linear order is then the identity
+  if (!strips && !h.gorder.p) {
+    h.order_period = h.order_band = 0;
+    return;
+  }
+  if (!h.gorder.p) h.gorder.alloc((size_t)ngrp * sizeof(int32_t));
+  std::vector<int32_t> ord(ngrp);
+  if (!strips) {
+    for (int32_t g = 0; g < ngrp; ++g) ord[g] = g;
+  } else {
+    band = std::min(band, period);
+    // the XCD ranges of a launch of ngrp workgroups (the apply kernels'
+    // renumbering: XCD x runs a contiguous range, the first ngrp % 8 ranges
+    // one group longer)
+    const int32_t per = ngrp / kXcdCount, rem = ngrp % kXcdCount;
+    int32_t pos = 0;
+    for (int32_t x = 0; x < kXcdCount; ++x) {
+      const int32_t a = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+      const int32_t len = per + (x < rem ? 1 : 0);
+      const int32_t planes = (len + period - 1) / period;
+      for (int32_t b0 = 0; b0 < period; b0 += band)
+        for (int32_t p = 0; p < planes; ++p)
+          for (int32_t o = b0; o < std::min(b0 + band, period); ++o) {
+            const int32_t g = p * period + o;
+            if (g < len) ord[pos++] = a + g;
+          }
+    }
+    if (pos != ngrp)
+      throw Error(AINV_ERR_INTERNAL, "slice-group order is not a permutation");
+  }
+  AINV_HIP_CHECK(hipMemcpyAsync(h.gorder.p, ord.data(),
+                                (size_t)ngrp * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+  AINV_HIP_CHECK(hipStreamSynchronize(s));
+  h.order_period = strips ? period : 0;
+  h.order_band = strips ? band : 0;
 }
 
 namespace {

@@ -795,6 +795,48 @@ double time_ms(hipStream_t s, const std::function<void()> &f, int reps) {
   return (double)ms / reps;
 }
 
+// Timed trial of the run mapping's slice-group order on one factor layout
+// (f runs its pass): linear order against strips of P / 20, P / 10 and
+// P / 5 groups over planes of P = R / 2 and R groups, R the factor's
+// median reach (rows from a row to its farthest column, in groups of 4
+// slices: the stencil factors' far cluster of diagonals lies two planes
+// back).  A strip order is kept when it beats linear order by 2 % (the
+// trial's own spread); results are the same bits in every order.
+// Returns the trial as JSON ("name": {...}).
+std::string tune_order(Hll &h, const std::function<void()> &f,
+                       const char *name, hipStream_t s) {
+  const int32_t R = hll_reach_groups(h, s);
+  std::vector<std::pair<int32_t, int32_t>> cand{{0, 0}};
+  for (const int32_t P : {R / 2, R}) {
+    if (P < 4) continue;
+    for (const int32_t div : {20, 10, 5}) {
+      const std::pair<int32_t, int32_t> c{P, std::max<int32_t>(1, P / div)};
+      if (std::find(cand.begin(), cand.end(), c) == cand.end())
+        cand.push_back(c);
+    }
+  }
+  std::string js = std::string("\"") + name + "\": {\"reach_groups\": " +
+                   std::to_string(R) + ", \"ms\": [";
+  double lin = 0.0, best = 0.0;
+  std::pair<int32_t, int32_t> pick{0, 0};
+  for (size_t i = 0; i < cand.size(); ++i) {
+    hll_set_order(h, cand[i].first, cand[i].second, s);
+    const double t = time_ms(s, f, 3);
+    char b[96];
+    std::snprintf(b, sizeof b, "%s[%d, %d, %.6f]", i ? ", " : "",
+                  cand[i].first, cand[i].second, t);
+    js += b;
+    if (i == 0) {
+      lin = best = t;
+    } else if (t < best && t < lin * 0.98) {
+      best = t;
+      pick = cand[i];
+    }
+  }
+  hll_set_order(h, pick.first, pick.second, s);
+  return js + "]}";
+}
+
 void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   const int K = p->K;
   const int li = log2_pow2(K);
@@ -820,7 +862,8 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
   if (forced >= 0) {
     p->apply_map = map_pair(forced, forced);
     p->spmv_map = forced;
-    return;
+    // a forced run mapping still has its slice-group order tried
+    if (forced != AINV_MAP_RUN || !m || m->kind == AINV_KIND_ML) return;
   }
   const Vecs &v = p->v;
   if (m && m->kind == AINV_KIND_ML) {
@@ -833,40 +876,58 @@ void tune_plan(KrylovPlan *p, ainv_matrix *a, ainv_solver *m, hipStream_t s) {
     else
       ml_tune(m, K, p->k, v.b, v.z, false, p->tbuf.p, s);
   } else if (m) {
-    if (m->apply_map[li] < 0) {
-      // the two passes are timed separately, on the vectors the iteration
-      // hands the preconditioner: with FP32 / BF16 factors BiCGSTAB and CG
-      // apply it to FP32 copies (p / s, r) into FP32 outputs (f / z);
-      // t keeps leading dimension K
-      const bool f32t = m->prec != AINV_PREC_F64;
-      const Vecs w32 = x32_vecs(p);
-      const float *x32 = w32.p32 ? w32.p32 : w32.r32;
-      float *y32 = w32.p32 ? w32.f32 : w32.z32;
+    // the two passes are timed separately, on the vectors the iteration
+    // hands the preconditioner: with FP32 / BF16 factors BiCGSTAB and CG
+    // apply it to FP32 copies (p / s, r) into FP32 outputs (f / z); t keeps
+    // leading dimension K
+    const bool f32t = m->prec != AINV_PREC_F64;
+    const Vecs w32 = x32_vecs(p);
+    const float *x32 = w32.p32 ? w32.p32 : w32.r32;
+    float *y32 = w32.p32 ? w32.f32 : w32.z32;
+    auto pass_w = [&](int map) {
+      if (x32)
+        hll_spmm_scale(m->hw, m->dv.p, x32, true, K, p->tbuf.p, K, p->k, s,
+                       map);
+      else
+        hll_spmm_scale(m->hw, m->dv.p, v.b, false, K, p->tbuf.p, K, p->k, s,
+                       map);
+    };
+    auto pass_z = [&](int map) {
+      if (x32)
+        hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, true, K, 0.0, y32, K, p->k, s,
+                       map);
+      else
+        hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, K, 0.0, v.z, K, p->k, s,
+                       map);
+    };
+    if (forced < 0 && m->apply_map[li] < 0) {
       int bw = 0, bz = 0;
       for (int map = 0; map < kNumApplyMaps; ++map) {
-        m->apply_ms[li][0][map] = time_ms(s, [&] {
-          if (x32)
-            hll_spmm_scale(m->hw, m->dv.p, x32, true, K, p->tbuf.p, K, p->k,
-                           s, map);
-          else
-            hll_spmm_scale(m->hw, m->dv.p, v.b, false, K, p->tbuf.p, K, p->k,
-                           s, map);
-        }, 3);
-        m->apply_ms[li][1][map] = time_ms(s, [&] {
-          if (x32)
-            hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, true, K, 0.0, y32, K, p->k,
-                           s, map);
-          else
-            hll_spmm_axpby(m->hz, 1.0, p->tbuf.p, f32t, K, 0.0, v.z, K, p->k,
-                           s, map);
-        }, 3);
+        m->apply_ms[li][0][map] = time_ms(s, [&] { pass_w(map); }, 3);
+        m->apply_ms[li][1][map] = time_ms(s, [&] { pass_z(map); }, 3);
         if (m->apply_ms[li][0][map] < m->apply_ms[li][0][bw]) bw = map;
         if (m->apply_ms[li][1][map] < m->apply_ms[li][1][bz]) bz = map;
       }
       m->apply_map[li] = (int16_t)map_pair(bw, bz);
     }
-    p->apply_map = m->apply_map[li];
+    if (forced < 0) p->apply_map = m->apply_map[li];
+    // the run mapping's slice-group order, once per layout, on the first
+    // plan of 32 or more columns that runs it
+    const bool run_w = map_w(p->apply_map) == AINV_MAP_RUN;
+    const bool run_z = map_z(p->apply_map) == AINV_MAP_RUN;
+    if (K >= 32 && m->order_period == 0 && !m->order_tuned &&
+        (run_w || run_z)) {
+      std::string js;
+      if (run_w)
+        js += tune_order(m->hw, [&] { pass_w(AINV_MAP_RUN); }, "w", s);
+      if (run_z)
+        js += std::string(js.empty() ? "" : ", ") +
+              tune_order(m->hz, [&] { pass_z(AINV_MAP_RUN); }, "z", s);
+      m->order_report = "{\"K\": " + std::to_string(K) + ", " + js + "}";
+      m->order_tuned = true;
+    }
   }
+  if (forced >= 0) return;
   if (a->spmv_map[li] < 0) {
     // BiCGSTAB with FP32 / BF16 factors multiplies the FP32 preconditioned
     // vectors (f, z): time that form

@@ -38,6 +38,13 @@ struct ainv_solver {
   // kernel mapping of the apply: -1 autotuned, else AINV_MAP_SPLIT / ROW /
   // COOP / COOP128 for both passes
   int map_mode = -1;
+  // slice-group order of the run mapping's passes over W and Z (period and
+  // band in groups of 4 slices; 0: chosen by a timed trial at the first
+  // Krylov plan of 32 or more columns; period -1: linear), whether that
+  // trial has run on the current layout, and its timings (JSON)
+  int32_t order_period = 0, order_band = 0;
+  bool order_tuned = false;
+  std::string order_report;
   int host_threads = 0;  // host threads of the AINV build (0: the default)
   ainv::AinvBuildInfo bicg;  // how the last AINV build's biconjugation ran
   // multilevel parameters (kind AINV_KIND_ML); the factor parameters above

@@ -238,6 +238,74 @@ def test_run_mapping_bit_identical(gpu, O, prec, k):
         assert np.linalg.norm(YT[:, c] - ref) <= tol * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("prec", ["F64", "F32"])
+@pytest.mark.parametrize("k", [8, 32, 64])
+def test_run_slice_order_bit_identical(gpu, O, prec, k):
+    """The run mapping's slice-group order (strips of ORDER_BAND groups of
+    4 slices taken across ORDER_PERIOD-group planes of each XCD's range)
+    changes only the order in which workgroups take their slices: the
+    applies are the linear order's bits for periods and bands that do and
+    do not divide the ranges (64 FP32 / 32 FP64 columns run as two column
+    chunks), and agree with the oracle to the storage precision."""
+    a = gpu.gen_pde3d(40)  # 1,000 slices: 250 groups, 31-32 per XCD range
+    sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1, FACTOR_PREC=prec,
+               APPLY_MAP="RUN", ORDER_PERIOD=-1)
+    n = a.shape[0]
+    x = gpu.gen_normal(31, (n, k))
+    y0 = gpu.gen_normal(32, (n, k))
+    ylin = sv.apply(x, y=y0.copy(), alpha=0.75, beta=-0.5)
+    assert sv.report()["slice_order"]["w"]["period"] == 0
+    for period, band in [(25, 2), (7, 3), (10, 10), (3, 1), (1000, 7)]:
+        sv.set("ORDER_PERIOD", period)
+        sv.set("ORDER_BAND", band)
+        so = sv.report()["slice_order"]
+        assert so["w"] == {"period": period, "band": min(band, period)}
+        assert so["z"] == so["w"]
+        y = sv.apply(x, y=y0.copy(), alpha=0.75, beta=-0.5)
+        assert np.array_equal(y, ylin), (period, band)
+    sv.set("ORDER_PERIOD", -1)
+    assert sv.report()["slice_order"]["w"]["period"] == 0
+    assert np.array_equal(sv.apply(x, y=y0.copy(), alpha=0.75, beta=-0.5),
+                          ylin)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    tol = {"F64": 1e-12, "F32": 2e-6}[prec]
+    for c in (0, k - 1):
+        ref = O.apply(w, z, d, x[:, c], alpha=0.75, beta=-0.5, y=y0[:, c])
+        assert np.linalg.norm(ylin[:, c] - ref) <= tol * np.linalg.norm(ref)
+
+
+def test_run_slice_order_trial(gpu):
+    """ORDER_PERIOD 0 (the default): the first Krylov plan of 32 or more
+    columns that runs the run mapping times linear order against strip
+    orders over planes of R / 2 and R groups (R the factor's median reach)
+    and keeps one, reported with the trial; the solve is the same bits as
+    with linear order."""
+    a = gpu.gen_pde3d(40)
+    n = a.shape[0]
+    A = gpu.SystemMatrix(a)
+    B = gpu.gen_normal(33, (n, 32))
+    out = {}
+    for period in (-1, 0):
+        sv = build(gpu, "INVK", a, SUB_FILLIN=1, INV_FILLIN=1,
+                   FACTOR_PREC="F32", APPLY_MAP="RUN", ORDER_PERIOD=period)
+        x, it, err = gpu.krylov("BICGSTAB", A, sv, B, eps=1e-8)
+        out[period] = (x, it, sv.report()["slice_order"])
+    assert np.array_equal(out[0][0], out[-1][0])
+    assert np.array_equal(out[0][1], out[-1][1])
+    assert out[-1][2]["trial"] is None
+    tr = out[0][2]["trial"]
+    assert tr["K"] == 32
+    for f in ("w", "z"):
+        reach = tr[f]["reach_groups"]
+        # 40^3 INVK(1,1): the far cluster lies 2 x 1,600 rows back
+        assert 10 <= reach <= 15, reach
+        ms = tr[f]["ms"]
+        assert ms[0][:2] == [0, 0] and all(m[2] > 0 for m in ms)
+        kept = [out[0][2][f]["period"], out[0][2][f]["band"]]
+        assert kept == [0, 0] or kept in [m[:2] for m in ms[1:]]
+    A.free()
+
+
 @pytest.mark.parametrize("kind", ["pde3d", "aniso2d", "random"])
 @pytest.mark.parametrize("xprec", ["F64", "F32"])
 @pytest.mark.parametrize("k", [2, 8, 16, 32, 64])
