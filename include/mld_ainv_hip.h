@@ -130,8 +130,13 @@ int ainv_get_kind(ainv_h h, int *kind);
  * diagonals and gather only the rows they do not hold; AUTO times each for
  * each of the two passes once per block width and keeps the fastest;
  * multilevel handles also take 'WIDE' or 7: one wavefront per row for the
- * level products that keep a CSR copy, COOP for the rest).  Unknown keys are passed to the base solver, which
- * ignores them (info = 0). */
+ * level products that keep a CSR copy, COOP for the rest), ORDER_PERIOD and
+ * ORDER_BAND (integers: the order in which the RUN mapping's workgroups take
+ * their groups of 4 slices within each XCD's range -- strips of ORDER_BAND
+ * groups across planes of ORDER_PERIOD groups; ORDER_PERIOD 0, the default,
+ * lets the Krylov tuner time linear and strip orders once per layout,
+ * -1 keeps linear order; results are the same in every order).  Unknown
+ * keys are passed to the base solver, which ignores them (info = 0). */
 int ainv_cseti(ainv_h h, const char *what, int val);
 int ainv_csetr(ainv_h h, const char *what, double val);
 int ainv_csetc(ainv_h h, const char *what, const char *val);
