@@ -415,6 +415,11 @@ def main():
                         "as PERIOD,BAND (groups of 4 slices; the served "
                         "run's tuned choice is in roofline.slice_order); "
                         "default linear")
+    p.add_argument("--slice-sort", type=int, default=-1,
+                   help="row sort of the system matrix and factor layouts: "
+                        "-1 the window picked from the row lengths "
+                        "(default), 0 rows in order, else a window of that "
+                        "many rows (a multiple of 64)")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the "
@@ -451,6 +456,7 @@ def main():
         for key, v in params.items():
             sv.set(key, v)
         sv.set("FACTOR_PREC", prec)
+        sv.set("SLICE_SORT", args.slice_sort)
         return sv
 
     if args.apply_only:
@@ -476,6 +482,8 @@ def main():
                           "GBs_w": bw / (ms_w * 1e-3) / 1e9}))
         return
     A = m.SystemMatrix(a)
+    if args.slice_sort != -1:
+        A.set_slice_sort(args.slice_sort)
     if args.eager_iteration:
         sv = new_solver(args.prec)
         sv.build(a)
@@ -737,12 +745,29 @@ def main():
             "other_pass": other,
         },
         "build": {"seconds": t_build},
+        "layout": layout_record(A, rep),
         "cpu_baseline": cpu,
         "multilevel": ml,
         "invt": invt,
         "c5": c5,
     }
     print(json.dumps(out))
+
+
+def layout_record(A, rep):
+    """Sliced layouts of the system matrix and the factors: stored entries
+    with the row sort and with the rows in order, the sort window, and the
+    slice widths (entries per row slot) whose spread is the load balance of
+    the slices' waves."""
+    out = {"a": A.report()["a"]}
+    for f in ("w", "z"):
+        if f in rep:
+            out[f] = rep[f]
+    for v in out.values():
+        v["padding_ratio"] = v["padded"] / max(v["nnz"], 1)
+        v["unsorted_padding_ratio"] = v["unsorted_padded"] / max(v["nnz"], 1)
+        v["max_over_mean_width"] = v["max_width"] / max(v["width_mean"], 1e-9)
+    return out
 
 
 def serve_bench(m, args):

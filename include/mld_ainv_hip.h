@@ -135,7 +135,12 @@ int ainv_get_kind(ainv_h h, int *kind);
  * their groups of 4 slices within each XCD's range -- strips of ORDER_BAND
  * groups across planes of ORDER_PERIOD groups; ORDER_PERIOD 0, the default,
  * lets the Krylov tuner time linear and strip orders once per layout,
- * -1 keeps linear order; results are the same in every order).  Unknown
+ * -1 keeps linear order; results are the same in every order), SLICE_SORT
+ * (integer: the row sort of the factor layouts as in
+ * ainv_mat_set_slice_sort -- -1, the default, picks the window from the row
+ * lengths, 0 keeps rows in order, else a window of that many rows, a
+ * multiple of 64; set on a built handle it rebuilds the layouts; every
+ * apply's result is the same in each).  Unknown
  * keys are passed to the base solver, which ignores them (info = 0). */
 int ainv_cseti(ainv_h h, const char *what, int val);
 int ainv_csetr(ainv_h h, const char *what, double val);
@@ -326,6 +331,35 @@ int ainv_report(ainv_h h, char *buf, size_t len);
 int ainv_mat_create(int32_t n, const int64_t *irp, const int32_t *ja,
                     const double *val, int index_base, ainv_mat_h *a);
 int ainv_mat_free(ainv_mat_h a);
+/* Row sort of A's sliced layout: rows sorted by length (longest first,
+ * ties in row order) within windows of `window` rows before they are cut
+ * into 64-row slices, so that a slice pads little (SELL-C-sigma); window
+ * -1 (the default at ainv_mat_create) picks the window from the row
+ * lengths -- none when the rows in order pad by less than a quarter or the
+ * sort saves less than a fifth, else the smallest of 256, 1024, 4096 and
+ * 16384 rows within 5 % of the best -- 0 keeps the rows in order, else a
+ * multiple of 64.  Products are the same in every layout; dot products
+ * over rows are summed in another order.  Rebuilds the layout and drops the
+ * matrix's Krylov plans and tuned mappings (AINV_ERR_STATE while a served
+ * stream holds one).  No counterpart in the reference, whose matrices stay
+ * in PSBLAS CSR. */
+int ainv_mat_set_slice_sort(ainv_mat_h a, int window);
+/* Layout of A (JSON text): {"n", "slice_sort", "a": {"nnz", "padded"
+ * (stored entries), "unsorted_padded" (those of the rows in order),
+ * "sort_window", "slices", "max_width", "width_mean", "width_p50",
+ * "width_p99" (slice widths in entries per row slot: the load balance of
+ * the slices' waves)}, "a_t": the same for A^T once BiCG has built it}.
+ * AINV_ERR_ARG when len is too small. */
+int ainv_mat_report(ainv_mat_h a, char *buf, size_t len);
+/* The row sort on the host, without a device: for the CSR row pointers irp
+ * (n + 1 entries) and window as in ainv_mat_set_slice_sort, perm (may be
+ * NULL; 64 * ceil(n / 64) entries) gets the stored order -- stored row i
+ * holds row perm[i], n past the last row -- padded the stored entries of
+ * that layout, unsorted_padded those of the rows in order and window_used
+ * the window applied (0: none).  No counterpart in the reference. */
+int ainv_slice_sort_plan(int32_t n, const int64_t *irp, int window,
+                         int32_t *perm, int64_t *padded,
+                         int64_t *unsorted_padded, int *window_used);
 /* y = alpha*A*x + beta*y for k interleaved columns (device pointers). */
 int ainv_mat_spmm(ainv_mat_h a, double alpha, const double *x, double beta,
                   double *y, int k, int64_t ldx, int64_t ldy, void *stream);

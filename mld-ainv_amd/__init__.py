@@ -89,6 +89,10 @@ def lib():
         "ainv_report": [vp, C.c_char_p, C.c_size_t],
         "ainv_mat_create": [C.c_int32, i64p, i32p, dp, C.c_int, P(vp)],
         "ainv_mat_free": [vp],
+        "ainv_mat_set_slice_sort": [vp, C.c_int],
+        "ainv_mat_report": [vp, C.c_char_p, C.c_size_t],
+        "ainv_slice_sort_plan": [C.c_int32, i64p, C.c_int, P(C.c_int32),
+                                 i64p, i64p, P(C.c_int)],
         "ainv_mat_spmm": [vp, C.c_double, vp, C.c_double, vp, C.c_int,
                           C.c_int64, C.c_int64, vp],
         "ainv_dump": [vp, C.c_char_p, C.c_int, C.c_int, C.c_char_p],
@@ -582,6 +586,20 @@ class SystemMatrix:
         synchronize()
         return y
 
+    def set_slice_sort(self, window):
+        """Row sort of A's sliced layout: -1 picks the window from the row
+        lengths (the default), 0 keeps rows in order, else a window of that
+        many rows (a multiple of 64).  Drops the matrix's Krylov plans."""
+        _ck(lib().ainv_mat_set_slice_sort(self._h, int(window)))
+
+    def report(self):
+        """Layout of A: stored entries with and without the row sort, the
+        sort window and the slice widths (JSON as a dict)."""
+        import json
+        buf = C.create_string_buffer(4096)
+        _ck(lib().ainv_mat_report(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
     def spmv_dot(self, x, w, np_=1, mapping="COOP", timed=False):
         """The Krylov step's fused product in one mapping: y = A x for the
         columns of x (numpy (n, K), K a power of two; FP32 or FP64) and the
@@ -833,6 +851,22 @@ def gen_random(n, seed=40, median=12.0, sigma=1.0, cap=400, band=5000,
                nlong=2, dom=1.1):
     return _gen(lib().ainv_gen_random, int(n), int(seed), float(median),
                 float(sigma), int(cap), int(band), int(nlong), float(dom))
+
+
+def slice_sort_plan(indptr, window=-1):
+    """The row sort of the sliced layouts computed on the host (no device):
+    for CSR row pointers `indptr` and a window as in
+    SystemMatrix.set_slice_sort, returns (perm, padded, unsorted_padded,
+    window_used): stored row i holds row perm[i] (n past the last row)."""
+    irp = np.ascontiguousarray(indptr, dtype=np.int64)
+    n = len(irp) - 1
+    perm = np.empty(64 * ((n + 63) // 64), dtype=np.int32)
+    pad, upad, used = C.c_int64(0), C.c_int64(0), C.c_int(0)
+    _ck(lib().ainv_slice_sort_plan(
+        n, irp.ctypes.data_as(C.POINTER(C.c_int64)), int(window),
+        perm.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(pad),
+        C.byref(upad), C.byref(used)))
+    return perm, pad.value, upad.value, used.value
 
 
 def gen_normal(seed, shape):

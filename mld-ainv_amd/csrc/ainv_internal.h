@@ -76,7 +76,14 @@ struct DevCsr {
 // wavefront; inside a slice entries are stored column-major
 // (entry j of row 64*s + l at sptr[s] + 64*j + l) so that the 64 lanes of a
 // wave read consecutive words.  Padding entries carry value 0 and the row's
-// own index as column.  rlen holds each row's true length.
+// own index as column.  rlen holds each stored row's true length.
+// Optionally the rows are sorted by length within windows of `sigma` rows
+// (a multiple of 64) before they are cut into slices, so that a slice holds
+// rows of like length and pads little (SELL-C-sigma, Kreutzer et al., SIAM
+// J. Sci. Comput. 36(5), 2014): perm then maps stored row i (slice i / 64,
+// lane i % 64) to the matrix row it holds, n for the rows past n of the last
+// slice.  Entries keep their order within a row, so every row sum is formed
+// as in the unsorted layout.
 constexpr int kSlice = 64;
 // slices per workgroup of the apply kernels (one wavefront each), and the
 // XCDs a launch's workgroups are dealt to round-robin
@@ -103,9 +110,19 @@ struct Hll {
   // taken across `period`-group planes
   DevBuf gorder;
   int32_t order_period = 0, order_band = 0;
+  // row order of sorted slices (int32 [nslices * 64]; empty: rows in order)
+  // and its window in rows (0: unsorted)
+  DevBuf perm;
+  int32_t sigma = 0;
+  int64_t unsorted_padded = 0;  // stored entries the unsorted layout needs
+  // slice widths (entries per row slot): mean and percentiles, the load
+  // balance of the slices' waves
+  double width_mean = 0;
+  int32_t width_p50 = 0, width_p99 = 0;
   int64_t bytes() const {
     return (int64_t)(sptr.bytes + rlen.bytes + col.bytes + val.bytes +
-                     cptr.bytes + cidx.bytes + cval.bytes + gorder.bytes);
+                     cptr.bytes + cidx.bytes + cval.bytes + gorder.bytes +
+                     perm.bytes);
   }
 };
 
@@ -139,8 +156,25 @@ int32_t hll_reach_groups(const Hll &h, hipStream_t s);
 // two planes apart) are gathered while still in the XCD's L2.  period or
 // band <= 0: linear order (hll.hip).
 void hll_set_order(Hll &h, int32_t period, int32_t band, hipStream_t s);
+// sigma: window of the row sort in rows (a multiple of 64; 0: rows in
+// order; kSortAuto: the window hll_pick_sigma picks from the row lengths).
+constexpr int32_t kSortAuto = -1;
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
-                  int32_t ncols = -1);
+                  int32_t ncols = -1, int32_t sigma = 0);
+// Window of the row sort for rows of the given lengths: 0 when sorting
+// saves less than a fifth of the unsorted layout's stored entries or the
+// layout pads less than a quarter; else the smallest of 256, 1024, 4096 and
+// 16384 rows whose stored entries are within 5 % of the largest window's.
+int32_t hll_pick_sigma(const std::vector<int32_t> &len);
+// Stored entries of slices of 64 rows of these lengths, the rows sorted by
+// length (longest first, ties in row order) within windows of sigma rows
+// (0: in order); perm (optional) receives the stored order, n past the
+// last row.
+int64_t hll_sorted_padding(const std::vector<int32_t> &len, int32_t sigma,
+                           std::vector<int32_t> *perm = nullptr);
+// {"nnz", "padded", "unsorted_padded", "sort_window", "slices",
+//  "max_width", "width_mean", "width_p50", "width_p99"} of a layout.
+std::string hll_layout_json(const Hll &h);
 // Attaches the CSR copy of `a` (host, 0-based, rows sorted) to h, values in
 // h's precision, for AINV_MAP_WIDE.
 void hll_attach_csr(const HostCsr &a, Hll &h, hipStream_t s);

@@ -169,13 +169,21 @@ __device__ __forceinline__ typename Val<VT>::acc row_dot(
 // argument with the leading dimension of out
 enum { EPI_SCALE = 0, EPI_AXPBY = 1, EPI_RESID = 2 };
 
+// Matrix row held by stored row i of a layout whose rows are sorted by
+// length within windows (perm: stored row -> row, n past the last row;
+// null: rows in order).  Kernels walk stored rows and write matrix rows.
+__device__ __forceinline__ int32_t hll_row(const int32_t *__restrict__ perm,
+                                           int32_t i) {
+  return perm ? perm[i] : i;
+}
+
 template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ rlen, const int32_t *__restrict__ col,
     const VT *__restrict__ val, const XT *__restrict__ x, int64_t ldx, int k,
     const DT *__restrict__ dvec, OT *__restrict__ out, int64_t ldo,
-    double alpha, double beta) {
+    double alpha, double beta, const int32_t *__restrict__ perm) {
   using A = typename Val<VT>::acc;
   constexpr int G = kSlice / K;  // rows per pass
   const int lane = threadIdx.x & (kSlice - 1);
@@ -193,14 +201,15 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmm(
     const int32_t len = rlen[r];
     const A acc = row_dot<VT, XT>(base + ri, len, col, val, x, ldx, c, col_ok);
     if (!col_ok) continue;
+    const int32_t o = hll_row(perm, r);
     if constexpr (EPI == EPI_SCALE) {
-      out[(int64_t)r * ldo + c] = (OT)((double)dvec[r] * (double)acc);
+      out[(int64_t)o * ldo + c] = (OT)((double)dvec[o] * (double)acc);
     } else if constexpr (EPI == EPI_RESID) {
-      out[(int64_t)r * ldo + c] =
-          (OT)(alpha * (double)acc + beta * (double)dvec[(int64_t)r * ldo + c]);
+      out[(int64_t)o * ldo + c] =
+          (OT)(alpha * (double)acc + beta * (double)dvec[(int64_t)o * ldo + c]);
     } else {
       double v = alpha * (double)acc;
-      OT *yp = out + (int64_t)r * ldo + c;
+      OT *yp = out + (int64_t)o * ldo + c;
       if (beta != 0.0) v += beta * (double)(*yp);
       *yp = (OT)v;
     }
@@ -332,7 +341,8 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_row(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
-    OT *__restrict__ out, double alpha, double beta) {
+    OT *__restrict__ out, double alpha, double beta,
+    const int32_t *__restrict__ perm) {
   using A = typename Val<VT>::acc;
   const int lane = threadIdx.x & (kSlice - 1);
   // wave-uniform: scalar loads of the slice bounds, scalar loop control
@@ -345,8 +355,8 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmm_row(
 #pragma unroll
   for (int u = 0; u < K; ++u) acc[u] = 0;
   row_acc<VT, XT, K>(base + lane, width, col, val, x, acc);
-  const int32_t r = s * kSlice + lane;
-  if (r >= n) return;
+  if (s * kSlice + lane >= n) return;
+  const int32_t r = hll_row(perm, s * kSlice + lane);
   OT *o = out + (int64_t)r * K;
   if constexpr (EPI == EPI_SCALE) {
     const double d = (double)dvec[r];
@@ -380,7 +390,7 @@ void launch_row(const Hll &a, const XT *x, int k, const DT *d, OT *out,
 #define AINV_LAUNCH(KK)                                                      \
   hipLaunchKernelGGL((k_hll_spmm_row<VT, XT, DT, OT, KK, EPI>), grid, block, \
                      0, s, a.n, a.nslices, sp, cl, vl, x, k, d, out, alpha,  \
-                     beta)
+                     beta, a.perm.as<int32_t>())
   if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)
@@ -407,7 +417,7 @@ void launch_k(const Hll &a, const XT *x, int64_t ldx, int k, const DT *d,
 #define AINV_LAUNCH(KK)                                                    \
   hipLaunchKernelGGL((k_hll_spmm<VT, XT, DT, OT, KK, EPI>), grid, block, 0, \
                      s, a.n, a.nslices, sp, rl, cl, vl, x, ldx, k, d, out,  \
-                     ldo, alpha, beta)
+                     ldo, alpha, beta, a.perm.as<int32_t>())
   if (k <= 1)
     AINV_LAUNCH(1);
   else if (k <= 2)
@@ -437,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
     const int32_t *__restrict__ rlen, const int32_t *__restrict__ col,
     const double *__restrict__ val, const XT *__restrict__ x, int k,
     int64_t ld, double *__restrict__ y, const double *__restrict__ w,
-    double *__restrict__ part) {
+    double *__restrict__ part, const int32_t *__restrict__ perm) {
   constexpr int G = kSlice / K;
   const int lane = threadIdx.x & (kSlice - 1);
   const int wave = threadIdx.x >> 6;
@@ -458,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_hll_spmv_dot(
       const double yv =
           row_dot<double, XT>(base + ri, len, col, val, x, ld, c, col_ok);
       if (!col_ok) continue;
-      const int64_t e = (int64_t)r * ld + c;
+      const int64_t e = (int64_t)hll_row(perm, r) * ld + c;
       y[e] = yv;
       if constexpr (NP == 1) {
         d0 += w[e] * yv;
@@ -499,7 +509,8 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_row(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ col, const double *__restrict__ val,
     const XT *__restrict__ x, int k, double *__restrict__ y,
-    const double *__restrict__ w, double *__restrict__ part) {
+    const double *__restrict__ w, double *__restrict__ part,
+    const int32_t *__restrict__ perm) {
   const int lane = threadIdx.x & (kSlice - 1);
   const int wave = threadIdx.x >> 6;
   double d0[K], d1[K];
@@ -516,8 +527,8 @@ __global__ __launch_bounds__(kBlock) AINV_ROW_ATTR void k_hll_spmv_dot_row(
 #pragma unroll
     for (int u = 0; u < K; ++u) acc[u] = 0.0;
     row_acc<double, XT, K>(base + lane, width, col, val, x, acc);
-    const int32_t r = s * kSlice + lane;
-    if (r >= n) continue;
+    if (s * kSlice + lane >= n) continue;
+    const int32_t r = hll_row(perm, s * kSlice + lane);
     double wr[K];
     load_xrow<double, K>(w + (int64_t)r * K, wr);
     double *yo = y + (int64_t)r * K;
@@ -757,7 +768,7 @@ __device__ __forceinline__ void coop_spmm_body(
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
     OT *__restrict__ out, double alpha, double beta,
-    const int32_t *__restrict__ kwp) {
+    const int32_t *__restrict__ kwp, const int32_t *__restrict__ perm) {
   using A = typename Val<VT>::acc;
   using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC;
@@ -792,8 +803,8 @@ __device__ __forceinline__ void coop_spmm_body(
   for (int q = 0; q < NQ; ++q) {
     const int32_t r = s * kSlice + (q0 + q) * CP::R + rg;
     if (r < n)
-      lane_store<DT, OT, K, CP::CPL, EPI>(r, c0, kw, acc[q], dvec, out, alpha,
-                                          beta);
+      lane_store<DT, OT, K, CP::CPL, EPI>(hll_row(perm, r), c0, kw, acc[q],
+                                          dvec, out, alpha, beta);
   }
 }
 
@@ -804,10 +815,10 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coop(
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
     OT *__restrict__ out, double alpha, double beta,
-    const int32_t *__restrict__ kwp) {
+    const int32_t *__restrict__ kwp, const int32_t *__restrict__ perm) {
   coop_spmm_body<VT, XT, DT, OT, K, CB, EPI, 1>(n, nslices, sptr, col, val, x,
                                                 k, dvec, out, alpha, beta,
-                                                kwp);
+                                                kwp, perm);
 }
 
 // quarter-slice variant (AINV_MAP_COOP_Q / AINV_MAP_COOP128_Q)
@@ -818,10 +829,10 @@ __global__ __launch_bounds__(kBlock) AINV_COOP_ATTR(XT) void k_hll_spmm_coopq(
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
     OT *__restrict__ out, double alpha, double beta,
-    const int32_t *__restrict__ kwp) {
+    const int32_t *__restrict__ kwp, const int32_t *__restrict__ perm) {
   coop_spmm_body<VT, XT, DT, OT, K, CB, EPI, 4>(n, nslices, sptr, col, val, x,
                                                 k, dvec, out, alpha, beta,
-                                                kwp);
+                                                kwp, perm);
 }
 
 // ---------------------------------------------------------------- runs
@@ -1003,7 +1014,8 @@ __global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
     const int32_t *__restrict__ col, const VT *__restrict__ val,
     const XT *__restrict__ x, int k, const DT *__restrict__ dvec,
     OT *__restrict__ out, double alpha, double beta,
-    const int32_t *__restrict__ kwp, const int32_t *__restrict__ gorder) {
+    const int32_t *__restrict__ kwp, const int32_t *__restrict__ gorder,
+    const int32_t *__restrict__ perm) {
   using A = typename Val<VT>::acc;
   using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC;
@@ -1034,8 +1046,8 @@ __global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
   for (int q = 0; q < L; ++q) {
     const int32_t r = s * kSlice + rb + q;
     if (r < n)
-      lane_store<DT, OT, K, CPL, EPI>(r, c0, kw, acc[q], dvec, out, alpha,
-                                      beta);
+      lane_store<DT, OT, K, CPL, EPI>(hll_row(perm, r), c0, kw, acc[q], dvec,
+                                      out, alpha, beta);
   }
 }
 
@@ -1053,7 +1065,8 @@ void launch_run(const Hll &a, const XT *x, int k, const DT *d, OT *out,
                      dim3(ngrp * CoopChunk<XT, KK, kRunChunkBytes>::NCH),   \
                      dim3(kBlock),                                          \
                      0, s, a.n, a.ncols, a.nslices, sp, cl, vl, x, k, d, out, \
-                     alpha, beta, kw, a.gorder.as<int32_t>())
+                     alpha, beta, kw, a.gorder.as<int32_t>(),               \
+                     a.perm.as<int32_t>())
   if (k <= 2)
     AINV_LAUNCH(2);
   else if (k <= 4)
@@ -1099,7 +1112,8 @@ void k_hll_spmv_dot_coop(int32_t n, int32_t ncols, int32_t nslices,
                     const XT *__restrict__ x, int k,
                     double *__restrict__ y, const double *__restrict__ w,
                     double *__restrict__ part,
-                    const int32_t *__restrict__ kwp) {
+                    const int32_t *__restrict__ kwp,
+                    const int32_t *__restrict__ perm) {
   using CH = CoopChunk<XT, K, CB>;
   constexpr int KC = CH::KC, NCH = CH::NCH;
   using CP = Coop<XT, KC>;
@@ -1149,8 +1163,9 @@ void k_hll_spmv_dot_coop(int32_t n, int32_t ncols, int32_t nslices,
           base, width, lane, rg, 0, col, val, x + c0, c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
-      const int32_t r = RUN ? s * kSlice + rg * L + q : s * kSlice + q * R + rg;
-      if (r >= n) continue;
+      const int32_t i = RUN ? s * kSlice + rg * L + q : s * kSlice + q * R + rg;
+      if (i >= n) continue;
+      const int32_t r = hll_row(perm, i);
       double wr[CPL];
       load_xrow<double, CPL>(c0 < kw ? w + (int64_t)r * K + c0 : w + c0, wr);
       double *yo = y + (int64_t)r * K + c0;
@@ -1213,11 +1228,13 @@ void launch_coop(const Hll &a, const XT *x, int k, const DT *d, OT *out,
     if (QS == 1)                                                            \
       hipLaunchKernelGGL((k_hll_spmm_coop<VT, XT, DT, OT, KK, CB, EPI>),    \
                          grid, dim3(kBlock), 0, s, a.n, a.nslices, sp, cl,  \
-                         vl, x, k, d, out, alpha, beta, kw);                \
+                         vl, x, k, d, out, alpha, beta, kw,                 \
+                         a.perm.as<int32_t>());                             \
     else                                                                    \
       hipLaunchKernelGGL((k_hll_spmm_coopq<VT, XT, DT, OT, KK, CB, EPI>),   \
                          grid, dim3(kBlock), 0, s, a.n, a.nslices, sp, cl,  \
-                         vl, x, k, d, out, alpha, beta, kw);                \
+                         vl, x, k, d, out, alpha, beta, kw,                 \
+                         a.perm.as<int32_t>());                             \
   } while (0)
   if (k <= 2)
     AINV_LAUNCH(2);
@@ -1376,6 +1393,7 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
   const int32_t *rl = a.rlen.as<int32_t>();
   const int32_t *cl = a.col.as<int32_t>();
   const double *vl = a.val.as<double>();
+  const int32_t *pm = a.perm.as<int32_t>();
   const bool coop = map == AINV_MAP_COOP || map == AINV_MAP_COOP128 ||
                     map == AINV_MAP_RUN;
   if ((map == AINV_MAP_ROW && k > 1 && k <= 32) ||
@@ -1391,24 +1409,24 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, kRunChunkBytes, NPP,   \
                                               1>),                           \
                          dim3(g), dim3(kBlock), 0, s, a.n, a.ncols,          \
-                         a.nslices, sp, cl, vl, x, k, y, w, part, kw);       \
+                         a.nslices, sp, cl, vl, x, k, y, w, part, kw, pm);       \
     else if (map == AINV_MAP_COOP)                                           \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),   \
                          dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw);                      \
+                         cl, vl, x, k, y, w, part, kw, pm);                      \
     else if (map == AINV_MAP_COOP128)                                        \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
                          dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw);                      \
+                         cl, vl, x, k, y, w, part, kw, pm);                      \
     else if constexpr (KK >= 16 && NPP == 2)                                 \
       /* 3K FP64 sums per lane do not fit: the cooperative kernel instead */ \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
                          dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw);                      \
+                         cl, vl, x, k, y, w, part, kw, pm);                      \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_row<XT, KK, NPP>), dim3(g),             \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part);                                     \
+                         k, y, w, part, a.perm.as<int32_t>());               \
   } while (0)
 // K = 64 exists only in the cooperative mapping; FP32 rows of 64 columns
 // (256 bytes) run as two 128-byte chunks in both variants (16 lanes per row
@@ -1419,19 +1437,19 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, kRunChunkBytes, NPP,   \
                                               1>),                           \
                          dim3(g), dim3(kBlock), 0, s, a.n, a.ncols,          \
-                         a.nslices, sp, cl, vl, x, k, y, w, part, kw);       \
+                         a.nslices, sp, cl, vl, x, k, y, w, part, kw, pm);       \
     else if constexpr (sizeof(XT) == 4)                                      \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
                          dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw);                      \
+                         cl, vl, x, k, y, w, part, kw, pm);                      \
     else if (map == AINV_MAP_COOP128)                                        \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
                          dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw);                      \
+                         cl, vl, x, k, y, w, part, kw, pm);                      \
     else                                                                     \
       hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),   \
                          dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw);                      \
+                         cl, vl, x, k, y, w, part, kw, pm);                      \
   } while (0)
 #define AINV_LRK(NPP)   \
   if (K == 2)           \
@@ -1459,7 +1477,8 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
   }
 #define AINV_LD(KK, NPP)                                                     \
   hipLaunchKernelGGL((k_hll_spmv_dot<XT, KK, NPP>), dim3(g), dim3(kBlock), 0, s, \
-                     a.n, a.nslices, sp, rl, cl, vl, x, k, ld, y, w, part)
+                     a.n, a.nslices, sp, rl, cl, vl, x, k, ld, y, w, part, \
+                     a.perm.as<int32_t>())
 #define AINV_LDK(NPP)        \
   if (k <= 1)                \
     AINV_LD(1, NPP);         \

@@ -223,8 +223,8 @@ void apply_fixed_order(ainv_solver *h) {
 
 void build_layout(ainv_solver *h) {
   const double t0 = now_seconds();
-  hll_from_csr(h->w, h->prec, h->hw, h->stream);
-  hll_from_csr(h->z, h->prec, h->hz, h->stream);
+  hll_from_csr(h->w, h->prec, h->hw, h->stream, -1, h->slice_sort);
+  hll_from_csr(h->z, h->prec, h->hz, h->stream, -1, h->slice_sort);
   convert_vector(h->d64.as<double>(), h->n, h->prec, h->dv, h->stream);
   reset_tuning(h);
   if (h->order_period > 0 && h->order_band > 0) {
@@ -245,8 +245,8 @@ void ensure_transposed(ainv_solver *h) {
   DevCsr wt, zt;
   csr_transpose(h->w, wt, h->stream);
   csr_transpose(h->z, zt, h->stream);
-  hll_from_csr(wt, h->prec, h->hwt, h->stream);
-  hll_from_csr(zt, h->prec, h->hzt, h->stream);
+  hll_from_csr(wt, h->prec, h->hwt, h->stream, -1, h->slice_sort);
+  hll_from_csr(zt, h->prec, h->hzt, h->stream, -1, h->slice_sort);
   h->have_t = true;
 }
 
@@ -313,6 +313,7 @@ void copy_settings(const ainv_solver *src, ainv_solver *dst) {
   dst->map_mode = src->map_mode;
   dst->order_period = src->order_period;
   dst->order_band = src->order_band;
+  dst->slice_sort = src->slice_sort;
   dst->ml_nlev = src->ml_nlev;
   dst->ml_type = src->ml_type;
   dst->ml_pos = src->ml_pos;
@@ -441,7 +442,8 @@ ainv_matrix *make_matrix(const HostCsr &a, int device) {
     AINV_HIP_CHECK(hipStreamCreate(&m->stream));
     m->n = a.n;
     csr_upload(a, m->csr, m->stream);
-    hll_from_csr(m->csr, AINV_PREC_F64, m->hll, m->stream);
+    hll_from_csr(m->csr, AINV_PREC_F64, m->hll, m->stream, -1,
+                 m->slice_sort);
   } catch (...) {
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
@@ -738,6 +740,12 @@ int ainv_cseti(ainv_h h, const char *what, int val) {
         apply_fixed_order(h);
         h->generation = next_generation();
       }
+    } else if (w == "SLICE_SORT") {
+      if (val < kSortAuto || (val > 0 && val % kSlice))
+        throw Error(AINV_ERR_ARG,
+                    "SLICE_SORT must be -1, 0 or a multiple of 64");
+      h->slice_sort = val;
+      if (h->built && !ml) build_layout(h);  // new layouts, new generation
     }
     // other keys go to the base solver, which ignores them
   });
@@ -821,6 +829,8 @@ int ainv_geti(ainv_h h, const char *what, int *val) {
       *val = h->order_period;
     else if (w == "ORDER_BAND")
       *val = h->order_band;
+    else if (w == "SLICE_SORT")
+      *val = h->slice_sort;
     else if (h->kind == AINV_KIND_ML && w == "N_PREC_LEVS")
       *val = h->ml_nlev;
     else if (h->kind == AINV_KIND_ML && w == "SMOOTHER_SWEEPS")
@@ -1249,12 +1259,7 @@ int ainv_report(ainv_h h, char *buf, size_t len) {
       r += ", \"n\": " + std::to_string(h->n) + tb + ", \"prec\": \"" +
            prec_name(h->prec) + "\", \"multilevel\": " + ml_report(h);
     } else if (h->built) {
-      auto hll_json = [](const Hll &x) {
-        return "{\"nnz\": " + std::to_string(x.nnz) +
-               ", \"padded\": " + std::to_string(x.padded) +
-               ", \"slices\": " + std::to_string(x.nslices) +
-               ", \"max_width\": " + std::to_string(x.max_width) + "}";
-      };
+      auto hll_json = [](const Hll &x) { return hll_layout_json(x); };
       char tb[256];
       std::snprintf(tb, sizeof tb,
                     ", \"t_factor\": %.6f, \"t_inverse\": %.6f, \"t_layout\": "
@@ -1395,7 +1400,8 @@ int ainv_mat_create(int32_t n, const int64_t *irp, const int32_t *ja,
       AINV_HIP_CHECK(hipStreamCreate(&m->stream));
       m->n = n;
       csr_upload(a, m->csr, m->stream);
-      hll_from_csr(m->csr, AINV_PREC_F64, m->hll, m->stream);
+      hll_from_csr(m->csr, AINV_PREC_F64, m->hll, m->stream, -1,
+                   m->slice_sort);
     } catch (...) {
       if (m->stream) (void)hipStreamDestroy(m->stream);
       delete m;
@@ -1414,6 +1420,59 @@ int ainv_mat_free(ainv_mat_h a) {
       (void)hipStreamDestroy(a->stream);
     }
     delete a;
+  });
+}
+
+int ainv_mat_set_slice_sort(ainv_mat_h a, int window) {
+  return guard([&] {
+    if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
+    if (window < kSortAuto || (window > 0 && window % kSlice))
+      throw Error(AINV_ERR_ARG, "window must be -1, 0 or a multiple of 64");
+    set_device(a->device);
+    AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
+    // plans hold graphs captured on the old layout; the tuned mappings were
+    // timed on it
+    matrix_drop_plans(a);
+    a->slice_sort = window;
+    hll_from_csr(a->csr, AINV_PREC_F64, a->hll, a->stream, -1, window);
+    a->have_t = false;
+    a->hll_t = Hll();
+  });
+}
+
+int ainv_slice_sort_plan(int32_t n, const int64_t *irp, int window,
+                         int32_t *perm, int64_t *padded,
+                         int64_t *unsorted_padded, int *window_used) {
+  return guard([&] {
+    if (n < 0 || (n > 0 && !irp)) throw Error(AINV_ERR_ARG, "bad matrix");
+    if (window < kSortAuto || (window > 0 && window % kSlice))
+      throw Error(AINV_ERR_ARG, "window must be -1, 0 or a multiple of 64");
+    std::vector<int32_t> len(n);
+    for (int32_t i = 0; i < n; ++i) {
+      if (irp[i + 1] < irp[i]) throw Error(AINV_ERR_ARG, "bad row pointers");
+      len[i] = (int32_t)(irp[i + 1] - irp[i]);
+    }
+    const int32_t w = window == kSortAuto ? hll_pick_sigma(len) : window;
+    std::vector<int32_t> pm;
+    const int64_t p = hll_sorted_padding(len, w, &pm);
+    if (perm) std::copy(pm.begin(), pm.end(), perm);
+    if (padded) *padded = p;
+    if (unsorted_padded) *unsorted_padded = hll_sorted_padding(len, 0);
+    if (window_used) *window_used = w;
+  });
+}
+
+int ainv_mat_report(ainv_mat_h a, char *buf, size_t len) {
+  return guard([&] {
+    if (!a) throw Error(AINV_ERR_ARG, "NULL matrix handle");
+    if (!buf || len == 0) throw Error(AINV_ERR_ARG, "NULL buffer");
+    std::string r = "{\"n\": " + std::to_string(a->n) +
+                    ", \"slice_sort\": " + std::to_string(a->slice_sort) +
+                    ", \"a\": " + hll_layout_json(a->hll);
+    if (a->have_t) r += ", \"a_t\": " + hll_layout_json(a->hll_t);
+    r += "}";
+    if (r.size() + 1 > len) throw Error(AINV_ERR_ARG, "buffer too small");
+    std::memcpy(buf, r.c_str(), r.size() + 1);
   });
 }
 

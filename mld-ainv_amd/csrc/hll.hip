@@ -5,6 +5,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdio>
+#include <numeric>
 #include <vector>
 
 #include "ainv_internal.h"
@@ -152,9 +154,11 @@ void csr_transpose(const DevCsr &a, DevCsr &t, hipStream_t s) {
 
 namespace {
 
-// per slice: width = max row length (one thread per slice); rlen per row
+// per slice: width = max row length (one thread per slice); rlen per
+// stored row (perm: the row each stored row holds, null: in order)
 __global__ void k_hll_width(int32_t n, int32_t nslices,
                             const int64_t *__restrict__ ptr,
+                            const int32_t *__restrict__ perm,
                             int32_t *__restrict__ rlen,
                             int64_t *__restrict__ w64) {
   int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,7 +171,8 @@ __global__ void k_hll_width(int32_t n, int32_t nslices,
   for (int l = 0; l < kSlice; ++l) {
     int32_t r = s * kSlice + l;
     if (r >= n) break;
-    int32_t len = (int32_t)(ptr[r + 1] - ptr[r]);
+    const int32_t pr = perm ? perm[r] : r;
+    int32_t len = (int32_t)(ptr[pr + 1] - ptr[pr]);
     rlen[r] = len;
     w = len > w ? len : w;
   }
@@ -188,13 +193,15 @@ __device__ inline void store_val(uint16_t *p, double v) {
   *p = (uint16_t)(u >> 16);
 }
 
-// one thread per (row) of the padded row space: writes its row column-major
+// one thread per stored row of the padded row space: writes the row it
+// holds (perm, or itself) column-major
 template <typename VT>
 __global__ void k_hll_fill(int32_t n, int32_t nslices,
                            const int64_t *__restrict__ ptr,
                            const int32_t *__restrict__ idx,
                            const double *__restrict__ val,
                            const int64_t *__restrict__ sptr,
+                           const int32_t *__restrict__ perm,
                            int32_t *__restrict__ col, VT *__restrict__ hv,
                            int32_t ncols) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,11 +212,12 @@ __global__ void k_hll_fill(int32_t n, int32_t nslices,
   int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
   int64_t beg = 0;
   int32_t len = 0;
-  int32_t self = (int32_t)(r < n ? r : n - 1);
+  const int32_t pr = r < n ? (perm ? perm[r] : (int32_t)r) : n;
+  int32_t self = pr < n ? pr : n - 1;
   if (self >= ncols) self = ncols - 1;  // rectangular: stay inside x
-  if (r < n) {
-    beg = ptr[r];
-    len = (int32_t)(ptr[r + 1] - beg);
+  if (pr < n) {
+    beg = ptr[pr];
+    len = (int32_t)(ptr[pr + 1] - beg);
   }
   for (int32_t j = 0; j < width; ++j) {
     int64_t q = base + (int64_t)j * kSlice + lane;
@@ -259,10 +267,100 @@ void hll_attach_csr(const HostCsr &a, Hll &h, hipStream_t s) {
   }
 }
 
+int64_t hll_sorted_padding(const std::vector<int32_t> &len, int32_t sigma,
+                           std::vector<int32_t> *perm) {
+  const int32_t n = (int32_t)len.size();
+  const int32_t ns = (n + kSlice - 1) / kSlice;
+  std::vector<int32_t> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  if (sigma > 0)
+    for (int64_t b = 0; b < n; b += sigma) {
+      const int64_t e = std::min<int64_t>(b + sigma, n);
+      std::stable_sort(ord.begin() + b, ord.begin() + e,
+                       [&](int32_t x, int32_t y) { return len[x] > len[y]; });
+    }
+  std::vector<int32_t> sw(ns);  // slice widths
+  int64_t padded = 0;
+  for (int32_t t = 0; t < ns; ++t) {
+    int32_t w = 0;
+    for (int32_t i = t * kSlice; i < std::min(n, (t + 1) * kSlice); ++i)
+      w = std::max(w, len[ord[i]]);
+    sw[t] = w;
+    padded += (int64_t)w * kSlice;
+  }
+  if (!perm) return padded;
+  perm->assign((size_t)ns * kSlice, n);
+  if (sigma <= 0) {
+    std::copy(ord.begin(), ord.end(), perm->begin());
+    return padded;
+  }
+  // Sorting packs each window's long rows into its first slices, so the
+  // heavy slices recur with the window's period, which can alias with the
+  // stride of the persistent kernels' block loops (one block taking every
+  // heavy group).  The groups of 4 slices (a workgroup's) of each of the 8
+  // contiguous ranges the XCDs take are therefore stored heaviest first:
+  // a launch starts its longest groups first and a persistent block's
+  // stride takes one group of each band of weights.  The last group, which
+  // holds the rows past n, stays last.
+  constexpr int32_t G = kSliceGroup;
+  const int32_t ng = (ns + G - 1) / G;
+  const int32_t nfull = n / (G * kSlice);  // groups of 256 rows
+  std::vector<int64_t> gw(ng, 0);
+  for (int32_t t = 0; t < ns; ++t) gw[t / G] += sw[t];
+  std::vector<int32_t> go(ng);
+  std::iota(go.begin(), go.end(), 0);
+  for (int32_t x = 0; x < kXcdCount; ++x) {
+    const int32_t b = (int32_t)((int64_t)ng * x / kXcdCount);
+    const int32_t e = std::min<int32_t>(
+        (int32_t)((int64_t)ng * (x + 1) / kXcdCount), nfull);
+    if (e > b)
+      std::stable_sort(go.begin() + b, go.begin() + e,
+                       [&](int32_t p, int32_t q) { return gw[p] > gw[q]; });
+  }
+  for (int32_t g = 0, i = 0; g < ng; ++g)
+    for (int32_t r = go[g] * G * kSlice;
+         r < std::min<int64_t>((int64_t)(go[g] + 1) * G * kSlice, n); ++r)
+      (*perm)[i++] = ord[r];
+  return padded;
+}
+
+std::string hll_layout_json(const Hll &h) {
+  char b[96];
+  std::snprintf(b, sizeof b, "%.3f", h.width_mean);
+  return "{\"nnz\": " + std::to_string(h.nnz) +
+         ", \"padded\": " + std::to_string(h.padded) +
+         ", \"unsorted_padded\": " + std::to_string(h.unsorted_padded) +
+         ", \"sort_window\": " + std::to_string(h.sigma) +
+         ", \"slices\": " + std::to_string(h.nslices) +
+         ", \"max_width\": " + std::to_string(h.max_width) +
+         ", \"width_mean\": " + b +
+         ", \"width_p50\": " + std::to_string(h.width_p50) +
+         ", \"width_p99\": " + std::to_string(h.width_p99) + "}";
+}
+
+int32_t hll_pick_sigma(const std::vector<int32_t> &len) {
+  int64_t nnz = 0;
+  for (const int32_t l : len) nnz += l;
+  const int64_t p0 = hll_sorted_padding(len, 0);
+  if (nnz == 0 || p0 * 4 < nnz * 5) return 0;  // pads < 25 %
+  constexpr int32_t win[] = {256, 1024, 4096, 16384};
+  int64_t pw[4], best = p0;
+  for (int i = 0; i < 4; ++i) {
+    pw[i] = hll_sorted_padding(len, win[i]);
+    best = std::min(best, pw[i]);
+  }
+  if (best * 5 > p0 * 4) return 0;  // saves < 20 %
+  for (int i = 0; i < 4; ++i)
+    if (pw[i] * 100 <= best * 105) return win[i];
+  return win[3];
+}
+
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
-                  int32_t ncols) {
+                  int32_t ncols, int32_t sigma) {
   if (ncols < 0) ncols = a.n;
   if (ncols < 1) ncols = 1;
+  if (sigma != 0 && sigma != kSortAuto && (sigma < 0 || sigma % kSlice))
+    throw Error(AINV_ERR_ARG, "row-sort window must be a multiple of 64");
   const int vb = value_bytes(prec);
   h = Hll();
   h.n = a.n;
@@ -270,13 +368,35 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
   h.prec = prec;
   h.nnz = a.nnz;
   h.nslices = (a.n + kSlice - 1) / kSlice;
+  if (sigma != 0 && a.n > 0) {
+    // row lengths on the host: the sort and its window are host work
+    std::vector<int64_t> ptr(a.n + 1);
+    AINV_HIP_CHECK(hipMemcpyAsync(ptr.data(), a.ptr.p,
+                                  ptr.size() * sizeof(int64_t),
+                                  hipMemcpyDeviceToHost, s));
+    AINV_HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<int32_t> len(a.n);
+    for (int32_t i = 0; i < a.n; ++i) len[i] = (int32_t)(ptr[i + 1] - ptr[i]);
+    h.unsorted_padded = hll_sorted_padding(len, 0);
+    if (sigma == kSortAuto) sigma = hll_pick_sigma(len);
+    if (sigma > 0) {
+      std::vector<int32_t> perm;
+      hll_sorted_padding(len, sigma, &perm);
+      h.perm.alloc(perm.size() * sizeof(int32_t));
+      AINV_HIP_CHECK(hipMemcpyAsync(h.perm.p, perm.data(), h.perm.bytes,
+                                    hipMemcpyHostToDevice, s));
+      AINV_HIP_CHECK(hipStreamSynchronize(s));
+      h.sigma = sigma;
+    }
+  }
   h.sptr.alloc((size_t)(h.nslices + 1) * sizeof(int64_t));
   h.rlen.alloc((size_t)std::max(a.n, 1) * sizeof(int32_t));
   DevBuf w64;
   w64.alloc((size_t)(h.nslices + 1) * sizeof(int64_t));
   hipLaunchKernelGGL(k_hll_width, dim3(grid_for(h.nslices + 1)),
                      dim3(kThreads), 0, s, a.n, h.nslices, a.ptr.as<int64_t>(),
-                     h.rlen.as<int32_t>(), w64.as<int64_t>());
+                     h.perm.as<int32_t>(), h.rlen.as<int32_t>(),
+                     w64.as<int64_t>());
   AINV_HIP_CHECK(hipGetLastError());
   size_t tmp_bytes = 0;
   AINV_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(
@@ -294,9 +414,17 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
   AINV_HIP_CHECK(hipMemcpyAsync(hs.data(), h.sptr.p, hs.size() * sizeof(int64_t),
                                 hipMemcpyDeviceToHost, s));
   AINV_HIP_CHECK(hipStreamSynchronize(s));
-  for (int32_t i = 0; i < h.nslices; ++i)
-    h.max_width = std::max<int32_t>(h.max_width,
-                                    (int32_t)((hs[i + 1] - hs[i]) / kSlice));
+  if (!h.perm.p) h.unsorted_padded = h.padded;
+  if (h.nslices > 0) {
+    std::vector<int32_t> wd(h.nslices);
+    for (int32_t i = 0; i < h.nslices; ++i)
+      wd[i] = (int32_t)((hs[i + 1] - hs[i]) / kSlice);
+    h.width_mean = (double)h.padded / kSlice / h.nslices;
+    std::sort(wd.begin(), wd.end());
+    h.max_width = wd.back();
+    h.width_p50 = wd[wd.size() / 2];
+    h.width_p99 = wd[std::min(wd.size() - 1, wd.size() * 99 / 100)];
+  }
   h.col.alloc((size_t)h.padded * sizeof(int32_t));
   h.val.alloc((size_t)h.padded * vb);
   if (h.padded > 0) {
@@ -306,19 +434,22 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
         hipLaunchKernelGGL(k_hll_fill<double>, g, dim3(kThreads), 0, s, a.n,
                            h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
                            a.val.as<double>(), h.sptr.as<int64_t>(),
-                           h.col.as<int32_t>(), h.val.as<double>(), ncols);
+                           h.perm.as<int32_t>(), h.col.as<int32_t>(),
+                           h.val.as<double>(), ncols);
         break;
       case AINV_PREC_F32:
         hipLaunchKernelGGL(k_hll_fill<float>, g, dim3(kThreads), 0, s, a.n,
                            h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
                            a.val.as<double>(), h.sptr.as<int64_t>(),
-                           h.col.as<int32_t>(), h.val.as<float>(), ncols);
+                           h.perm.as<int32_t>(), h.col.as<int32_t>(),
+                           h.val.as<float>(), ncols);
         break;
       default:
         hipLaunchKernelGGL(k_hll_fill<uint16_t>, g, dim3(kThreads), 0, s, a.n,
                            h.nslices, a.ptr.as<int64_t>(), a.idx.as<int32_t>(),
                            a.val.as<double>(), h.sptr.as<int64_t>(),
-                           h.col.as<int32_t>(), h.val.as<uint16_t>(), ncols);
+                           h.perm.as<int32_t>(), h.col.as<int32_t>(),
+                           h.val.as<uint16_t>(), ncols);
     }
     AINV_HIP_CHECK(hipGetLastError());
   }
@@ -330,6 +461,7 @@ namespace {
 // slice -- to its first and its last column (entries sorted by column)
 __global__ void k_slice_reach(int32_t n, int32_t nslices,
                               const int64_t *__restrict__ sptr,
+                              const int32_t *__restrict__ perm,
                               const int32_t *__restrict__ rlen,
                               const int32_t *__restrict__ col,
                               int32_t *__restrict__ out) {
@@ -337,8 +469,8 @@ __global__ void k_slice_reach(int32_t n, int32_t nslices,
   if (s >= nslices) return;
   const int32_t r0 = s * kSlice;
   const int32_t lane = min(kSlice / 2, n - 1 - r0);
-  const int32_t r = r0 + lane;
-  const int32_t len = rlen[r];
+  const int32_t len = rlen[r0 + lane];
+  const int32_t r = perm ? perm[r0 + lane] : r0 + lane;
   int32_t reach = 0;
   if (len > 0) {
     const int32_t first = col[sptr[s] + lane];
@@ -355,7 +487,8 @@ int32_t hll_reach_groups(const Hll &h, hipStream_t s) {
   out.alloc((size_t)h.nslices * sizeof(int32_t));
   hipLaunchKernelGGL(k_slice_reach, dim3(grid_for(h.nslices)), dim3(kThreads),
                      0, s, h.n, h.nslices, h.sptr.as<int64_t>(),
-                     h.rlen.as<int32_t>(), h.col.as<int32_t>(),
+                     h.perm.as<int32_t>(), h.rlen.as<int32_t>(),
+                     h.col.as<int32_t>(),
                      out.as<int32_t>());
   AINV_HIP_CHECK(hipGetLastError());
   std::vector<int32_t> r(h.nslices);

@@ -760,6 +760,19 @@ void precond_t(const KrylovPlan *p, const double *x, double *y,
                                   hipMemcpyDeviceToDevice, s));
 }
 
+}  // namespace
+
+void matrix_drop_plans(ainv_matrix *a) {
+  for (const auto &p : a->plans)
+    if (p->holders > 0)
+      throw Error(AINV_ERR_STATE, "matrix has Krylov work in flight");
+  a->plans.clear();
+  std::fill(std::begin(a->spmv_map), std::end(a->spmv_map), (int8_t)-1);
+  std::memset(a->spmv_ms, 0, sizeof a->spmv_ms);
+}
+
+namespace {
+
 // Layouts a method needs besides the plan's own (built outside any capture):
 // BiCG applies M^T and A^T.
 void prepare_method(KrylovPlan *p, ainv_matrix *a, ainv_solver *m) {
@@ -768,7 +781,7 @@ void prepare_method(KrylovPlan *p, ainv_matrix *a, ainv_solver *m) {
   if (!a->have_t) {
     DevCsr at;
     csr_transpose(a->csr, at, a->stream);
-    hll_from_csr(at, AINV_PREC_F64, a->hll_t, a->stream);
+    hll_from_csr(at, AINV_PREC_F64, a->hll_t, a->stream, -1, a->slice_sort);
     AINV_HIP_CHECK(hipStreamSynchronize(a->stream));
     a->have_t = true;
   }

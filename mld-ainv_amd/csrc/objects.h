@@ -43,6 +43,9 @@ struct ainv_solver {
   // Krylov plan of 32 or more columns; period -1: linear), whether that
   // trial has run on the current layout, and its timings (JSON)
   int32_t order_period = 0, order_band = 0;
+  // row sort of the factor layouts (Hll::perm): window in rows (a multiple
+  // of 64), 0 rows in order, kSortAuto picked from the row lengths
+  int32_t slice_sort = ainv::kSortAuto;
   bool order_tuned = false;
   std::string order_report;
   int host_threads = 0;  // host threads of the AINV build (0: the default)
@@ -112,6 +115,7 @@ struct ainv_matrix {
   int32_t n = 0;
   ainv::DevCsr csr;
   ainv::Hll hll;
+  int32_t slice_sort = ainv::kSortAuto;  // row sort of hll and hll_t
   bool have_t = false;  // A^T layout (BiCG), built on first use
   ainv::Hll hll_t;
   int8_t spmv_map[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
@@ -227,6 +231,10 @@ int krylov_method_id(const std::string &name);  // throws for others
 SlotBatch *batch_open(ainv_matrix *a, ainv_solver *m, int method, int K,
                       hipStream_t s);
 void batch_close(SlotBatch *b);
+// Drops the matrix's Krylov plans (their captured graphs and workspaces) and
+// its tuned product mappings, after a change of its layout; throws
+// AINV_ERR_STATE while a slot batch holds a plan.
+void matrix_drop_plans(ainv_matrix *a);
 // src: device, n x kc interleaved; x0 = 0
 void batch_admit(SlotBatch *b, int c0, int kc, const double *src, double eps);
 // one right-hand side (a contiguous device n-vector) per listed slot

@@ -306,6 +306,135 @@ def test_run_slice_order_trial(gpu):
     A.free()
 
 
+@pytest.mark.parametrize("prec", ["F64", "F32", "BF16"])
+@pytest.mark.parametrize("k", [1, 8, 32, 64])
+def test_slice_sort_factor_applies_bit_identical(gpu, O, prec, k):
+    """Factor layouts whose rows are sorted by length within windows
+    (SLICE_SORT) store each row's entries in the same order and write each
+    row where it belongs: every mapping's apply, both transposes, is the
+    rows-in-order layout's bits.  The automatic window is the host plan's
+    for the exported factor, and the sort cuts the random matrix's padding;
+    the applies agree with the oracle to the storage precision."""
+    a = gpu.gen_random(5000, seed=17, median=12.0, band=1500)
+    sv = build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
+               SUB_ILUTHRS=5e-2, FACTOR_PREC=prec, SLICE_SORT=0)
+    n = a.shape[0]
+    x = gpu.gen_normal(41, (n, k))
+    y0 = gpu.gen_normal(42, (n, k))
+    if k == 1:
+        x, y0 = x[:, 0].copy(), y0[:, 0].copy()
+    maps = ["SPLIT"] if k == 1 else ["SPLIT", "ROW", "COOP", "COOP128",
+                                     "COOP_Q", "COOP128_Q", "RUN"]
+
+    def applies():
+        out = {}
+        for mapping in maps:
+            sv.set("APPLY_MAP", mapping)
+            out[mapping] = [sv.apply(x, y=y0.copy(), alpha=0.75, beta=-0.5,
+                                     trans=t) for t in ("N", "T")]
+        return out
+
+    rep = sv.report()
+    assert rep["w"]["sort_window"] == 0
+    assert rep["w"]["padded"] == rep["w"]["unsorted_padded"]
+    ref = applies()
+    for window in (256, -1):
+        sv.set("SLICE_SORT", window)
+        rep = sv.report()
+        for f in ("w", "z"):
+            fac = sv.export(f.upper())
+            _, pad, upad, used = gpu.slice_sort_plan(fac.indptr, window)
+            lay = rep[f]
+            assert lay["sort_window"] == used
+            assert lay["padded"] == pad and lay["unsorted_padded"] == upad
+            assert lay["max_width"] >= lay["width_p99"] >= lay["width_p50"]
+        if window == 256:
+            assert rep["z"]["padded"] < rep["z"]["unsorted_padded"]
+        out = applies()
+        for mapping in maps:
+            for t in (0, 1):
+                assert np.array_equal(out[mapping][t], ref[mapping][t]), \
+                    (window, mapping, t)
+    w, z, d = sv.export("W"), sv.export("Z"), sv.export("D")
+    tol = {"F64": 1e-12, "F32": 2e-6, "BF16": 2e-2}[prec]
+    X, Y0 = x.reshape(n, -1), y0.reshape(n, -1)
+    for t in (0, 1):
+        Y = ref[maps[0]][t].reshape(n, -1)
+        c = X.shape[1] - 1
+        r = O.apply(w, z, d, X[:, c], alpha=0.75, beta=-0.5, y=Y0[:, c],
+                    trans="NT"[t])
+        assert np.linalg.norm(Y[:, c] - r) <= tol * np.linalg.norm(r)
+
+
+@pytest.mark.parametrize("xprec", ["F64", "F32"])
+@pytest.mark.parametrize("k", [1, 8, 64])
+def test_slice_sort_system_matrix(gpu, xprec, k):
+    """The system matrix's layout sorts rows by length within windows by
+    default (the random matrix's rows in order pad ~6x): the products of
+    every mapping of the fused product + dots, and the plain product, are
+    the rows-in-order layout's bits; the dots, summed over rows in another
+    order, agree to rounding; BiCG (which adds the A^T layout) takes the
+    same iterations and reaches the same solution to the tolerance."""
+    a = gpu.gen_random(3000, seed=5)
+    n = a.shape[0]
+    A = gpu.SystemMatrix(a)
+    rep = A.report()
+    _, pad, upad, used = gpu.slice_sort_plan(a.indptr, -1)
+    assert rep["slice_sort"] == -1 and rep["a"]["sort_window"] == used > 0
+    assert rep["a"]["padded"] == pad < upad == rep["a"]["unsorted_padded"]
+    assert pad < 1.5 * a.nnz
+    K = max(k, 2)
+    x = gpu.gen_normal(43, (n, K))
+    if xprec == "F32":
+        x = x.astype(np.float32)
+    w = gpu.gen_normal(44, (n, K))
+    maps = ("SPLIT", "ROW", "COOP", "COOP128", "RUN")
+
+    def products():
+        out = {m: [A.spmv_dot(x, w, np_=p, mapping=m) for p in (1, 2)]
+               for m in maps}
+        X = gpu.DeviceArray.from_numpy(np.ascontiguousarray(
+            x[:, :k].astype(np.float64)))
+        Y = gpu.DeviceArray((n, k) if k > 1 else (n,))
+        Y.zero()
+        A.spmm(X, Y)
+        out["spmm"] = Y.to_numpy()
+        X.free()
+        Y.free()
+        return out
+
+    srt = products()
+    A.set_slice_sort(0)
+    rep = A.report()
+    assert rep["slice_sort"] == 0 and rep["a"]["sort_window"] == 0
+    assert rep["a"]["padded"] == upad
+    ord_ = products()
+    assert np.array_equal(srt["spmm"], ord_["spmm"])
+    y_ref = a @ x[:, :k].astype(np.float64)
+    assert np.linalg.norm(srt["spmm"].reshape(n, -1) - y_ref) <= \
+        1e-14 * np.linalg.norm(y_ref)
+    nw = np.linalg.norm(w, axis=0)
+    for m in maps:
+        for p, ((ys, ds, _), (yo, do, _)) in enumerate(zip(srt[m], ord_[m])):
+            assert np.array_equal(ys, yo), m
+            ny = np.linalg.norm(ys, axis=0)
+            # dots to rounding of the sums' magnitudes: |y| |w|, |y|^2
+            scale = (np.stack([ny * ny, ny * nw]) if p else
+                     (ny * nw)[None, :])
+            assert np.all(np.abs(ds - do) <= 1e-13 * scale), m
+    # BiCG on both layouts (A^T is built on first use, in the same sort)
+    sv = build(gpu, "AINV", a, AINV_ALG="LLK", SUB_FILLIN=8,
+               SUB_ILUTHRS=5e-2)
+    b = gpu.gen_normal(45, (n, k)) if k > 1 else gpu.gen_normal(45, (n,))
+    xo, ito, _ = gpu.krylov("BICG", A, sv, b, eps=1e-8)
+    A.set_slice_sort(-1)
+    xs, its, _ = gpu.krylov("BICG", A, sv, b, eps=1e-8)
+    assert A.report()["a_t"]["sort_window"] > 0
+    assert np.all(np.abs(np.asarray(its) - np.asarray(ito)) <= 1)
+    assert np.linalg.norm(xs - xo) <= 1e-6 * np.linalg.norm(xo)
+    A.free()
+
+
 @pytest.mark.parametrize("kind", ["pde3d", "aniso2d", "random"])
 @pytest.mark.parametrize("xprec", ["F64", "F32"])
 @pytest.mark.parametrize("k", [2, 8, 16, 32, 64])
