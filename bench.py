@@ -38,7 +38,7 @@ thresh 0.1), BiCGSTAB 1e-9, b = 1; C3 2-D anisotropic diffusion 2048^2
 N(0,1) right-hand sides (seed 3); C4 random nonsymmetric n = 10^6
 (LogNormal(ln 12, 1.0) off-diagonals capped at 400, band 5000, 2
 long-range columns, diagonal 1.1 x row sum) with AINV-LLK (fill 8, thresh
-5e-2), BiCGSTAB 1e-8, 16 slots, N(0,1) right-hand sides (seed 4).
+5e-2), BiCGSTAB 1e-8, N(0,1) right-hand sides (seed 4).
 
 C1, C2 and C3 also report, as `multilevel`, the multilevel preconditioner
 (smoothed-aggregation V-cycle with factors of the configuration's kind as
@@ -46,6 +46,14 @@ smoother and coarsest-level solver; ML_CONFIGS below) on the same matrix and
 right-hand sides: its FP64 single-RHS solves, its served form with --prec
 level factors and one cycle's duration at the served block width
 (--no-ml skips it; --ml-cycle times the cycle alone).
+
+Every configuration reports, as `layout`, the sliced layouts of the system
+matrix and the factors: stored entries with the rows sorted by length
+within windows (the default where the rows in order pad by more than a
+quarter; --slice-sort sets the window, 0 keeps rows in order) and in row
+order, the window, and the slice widths whose spread is the load balance of
+the slices' waves.  C4 also reports, as `bf16`, the served run with
+BF16-value / FP32-accumulate factors.
 """
 import argparse
 import ctypes as C
@@ -661,6 +669,11 @@ def main():
         f"({passes[0]['achieved_GBs']:.0f} GB/s), Z {ms_z:.3f} ms "
         f"({passes[1]['achieved_GBs']:.0f} GB/s)")
 
+    bf16 = None
+    if args.config == "C4" and args.prec == "F32":
+        bf16 = bf16_leg(m, method, A, sv, a, dB, dX, Bh, M, eps, args.steps,
+                        x64, j64)
+
     ml = None
     if not args.no_ml and args.config in ML_CONFIGS:
         ml = ml_leg(m, args, a, A, dB, Bh, sample, kind, params, method, eps,
@@ -746,12 +759,53 @@ def main():
         },
         "build": {"seconds": t_build},
         "layout": layout_record(A, rep),
+        "bf16": bf16,
         "cpu_baseline": cpu,
         "multilevel": ml,
         "invt": invt,
         "c5": c5,
     }
     print(json.dumps(out))
+
+
+def bf16_leg(m, method, A, sv, a, dB, dX, Bh, M, eps, steps, x64, j64):
+    """The served run again with BF16-value / FP32-accumulate factors (the
+    same build's factors rounded to BF16), its own block-width trial, for
+    the comparison with FP32-stored factors."""
+    n = a.shape[0]
+    sv.cnv("BF16")
+    slots, width_ms = served_warm(m, method, A, sv, Bh, 0, M, eps, 1)
+    lat, iters = [], None
+    t0 = time.time()
+    for _ in range(steps):
+        _, iters, _, l = m.krylov_stream(method, A, sv, dB, dX, slots=slots,
+                                         eps=eps, itmax=5000)
+        lat.extend(l.tolist())
+    m.synchronize()
+    el = time.time() - t0
+    Xs = dX.to_numpy()
+    true = max(float(np.linalg.norm(Bh[j] - a @ Xs[j]) / np.linalg.norm(Bh[j]))
+               for j in range(0, M, max(1, M // 16)))
+    ms_w, ms_z = profile_apply(m, sv, n, slots, 10, True, True)
+    lay = sv.layout()
+    out = {"solves_per_s": steps * M / el, "slots": slots,
+           "slots_ms_per_col_iter": width_ms,
+           "iterations_mean": float(np.mean(iters)),
+           "iterations_max": int(np.max(iters)),
+           "true_residual_max": true,
+           "error_vs_fp64": float(np.linalg.norm(Xs[j64] - x64) /
+                                  np.linalg.norm(x64)),
+           "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                          "p99": float(np.percentile(lat, 99))},
+           "apply_ms": {"first": ms_w, "second": ms_z},
+           "apply_GBs": {
+               "first": apply_pass_bytes(lay, slots, True, 4) /
+               (ms_w * 1e-3) / 1e9,
+               "second": apply_pass_bytes(lay, slots, False, 8, 4) /
+               (ms_z * 1e-3) / 1e9}}
+    log(f"served BF16 slots={slots}: {out['solves_per_s']:.1f} solves/s, "
+        f"apply {ms_w:.3f}+{ms_z:.3f} ms")
+    return out
 
 
 def layout_record(A, rep):
