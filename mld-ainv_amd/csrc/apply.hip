@@ -177,6 +177,20 @@ __device__ __forceinline__ int32_t hll_row(const int32_t *__restrict__ perm,
   return perm ? perm[i] : i;
 }
 
+// The same for the kernels whose lanes store rows of their slice other than
+// their own: lane l loads the row of slice row l once per slice (pv, one
+// coalesced word, requested before the slice's entries) and a store takes
+// slice row j's by a shuffle; rows in order take no load or shuffle.  Call
+// with the whole wave active.
+__device__ __forceinline__ int32_t slice_rows(const int32_t *__restrict__ perm,
+                                              int32_t s, int lane) {
+  return perm ? perm[(int64_t)s * kSlice + lane] : 0;
+}
+__device__ __forceinline__ int32_t slice_row(const int32_t *__restrict__ perm,
+                                             int32_t pv, int32_t s, int j) {
+  return perm ? __shfl(pv, j) : s * kSlice + j;
+}
+
 template <typename VT, typename XT, typename DT, typename OT, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_hll_spmm(
     int32_t n, int32_t nslices, const int64_t *__restrict__ sptr,
@@ -791,6 +805,7 @@ __device__ __forceinline__ void coop_spmm_body(
   const int q0 = (wv % SP::QW) * NQ;  // first row pass of this wave
   const int64_t base = sptr[s];
   const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+  const int32_t pv = slice_rows(perm, s, lane);
   A acc[NQ][CP::CPL];
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
@@ -801,10 +816,11 @@ __device__ __forceinline__ void coop_spmm_body(
                               c0 < kw, acc);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int32_t r = s * kSlice + (q0 + q) * CP::R + rg;
-    if (r < n)
-      lane_store<DT, OT, K, CP::CPL, EPI>(hll_row(perm, r), c0, kw, acc[q],
-                                          dvec, out, alpha, beta);
+    const int j = (q0 + q) * CP::R + rg;  // slice row
+    const int32_t r = slice_row(perm, pv, s, j);
+    if (s * kSlice + j < n)
+      lane_store<DT, OT, K, CP::CPL, EPI>(r, c0, kw, acc[q], dvec, out, alpha,
+                                          beta);
   }
 }
 
@@ -1038,16 +1054,17 @@ __global__ __launch_bounds__(kBlock) AINV_RUN_ATTR void k_hll_spmm_run(
   const int64_t base = sptr[s];
   const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
   const int c0 = chunk * KC + sub * CPL;  // first column of this lane
+  const int32_t pv = slice_rows(perm, s, lane);
   A acc[L][CPL];
   run_acc<VT, XT, K, KC>(base, width, lane, c0 < kw, col, val,
                          run_rsrc<XT, K>(x, ncols),
                          (uint32_t)(c0 * (int)sizeof(XT)), acc);
 #pragma unroll
   for (int q = 0; q < L; ++q) {
-    const int32_t r = s * kSlice + rb + q;
-    if (r < n)
-      lane_store<DT, OT, K, CPL, EPI>(hll_row(perm, r), c0, kw, acc[q], dvec,
-                                      out, alpha, beta);
+    const int32_t r = slice_row(perm, pv, s, rb + q);
+    if (s * kSlice + rb + q < n)
+      lane_store<DT, OT, K, CPL, EPI>(r, c0, kw, acc[q], dvec, out, alpha,
+                                      beta);
   }
 }
 
@@ -1099,7 +1116,9 @@ template <typename XT, int K, int CB, int RUN> struct DotWaves {
       RUN ? (rb >= 128 ? 2 : rb == 64 ? 3 : AINV_ROW_WAVES)
           : CoopWaves<double, XT, K, CB>::v;
 };
-template <typename XT, int K, int CB, int NP, int RUN = 0>
+// SORT: a row-sorted layout (perm set); rows in order compile without the
+// row map, which costs the short rows of a stencil matrix ~4 % here.
+template <typename XT, int K, int CB, int NP, int RUN = 0, int SORT = 0>
 __global__ __launch_bounds__(kBlock)
 #if AINV_ROW_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(DotWaves<XT, K, CB, RUN>::v,
@@ -1149,6 +1168,7 @@ void k_hll_spmv_dot_coop(int32_t n, int32_t ncols, int32_t nslices,
     if (s >= nslices) break;
     const int64_t base = sptr[s];
     const int32_t width = (int32_t)((sptr[s + 1] - base) / kSlice);
+    const int32_t pv = SORT ? perm[(int64_t)s * kSlice + lane] : 0;
     double acc[L][CPL];
 #pragma unroll
     for (int q = 0; q < L; ++q)
@@ -1163,9 +1183,9 @@ void k_hll_spmv_dot_coop(int32_t n, int32_t ncols, int32_t nslices,
           base, width, lane, rg, 0, col, val, x + c0, c0 < kw, acc);
 #pragma unroll
     for (int q = 0; q < L; ++q) {
-      const int32_t i = RUN ? s * kSlice + rg * L + q : s * kSlice + q * R + rg;
-      if (i >= n) continue;
-      const int32_t r = hll_row(perm, i);
+      const int j = RUN ? rg * L + q : q * R + rg;  // slice row
+      const int32_t r = SORT ? __shfl(pv, j) : s * kSlice + j;
+      if (s * kSlice + j >= n) continue;
       double wr[CPL];
       load_xrow<double, CPL>(c0 < kw ? w + (int64_t)r * K + c0 : w + c0, wr);
       double *yo = y + (int64_t)r * K + c0;
@@ -1373,6 +1393,23 @@ bool aligned16(const void *p) {
 
 }  // namespace
 
+template <typename XT, int K, int CB, int NP, int RUN = 0>
+void launch_dot_coop(int g, hipStream_t s, const Hll &a, const XT *x, int k,
+                     double *y, const double *w, double *part,
+                     const int32_t *kw) {
+  const int64_t *sp = a.sptr.as<int64_t>();
+  const int32_t *cl = a.col.as<int32_t>();
+  const double *vl = a.val.as<double>();
+  if (a.perm.p)
+    hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, K, CB, NP, RUN, 1>), dim3(g),
+                       dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp, cl,
+                       vl, x, k, y, w, part, kw, a.perm.as<int32_t>());
+  else
+    hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, K, CB, NP, RUN, 0>), dim3(g),
+                       dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp, cl,
+                       vl, x, k, y, w, part, kw, nullptr);
+}
+
 template <typename XT>
 int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
                   const double *w, int np, double *part, int map,
@@ -1406,27 +1443,19 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
 #define AINV_LR(KK, NPP)                                                     \
   do {                                                                       \
     if (map == AINV_MAP_RUN)                                                 \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, kRunChunkBytes, NPP,   \
-                                              1>),                           \
-                         dim3(g), dim3(kBlock), 0, s, a.n, a.ncols,          \
-                         a.nslices, sp, cl, vl, x, k, y, w, part, kw, pm);       \
+      launch_dot_coop<XT, KK, kRunChunkBytes, NPP, 1>(g, s, a, x, k, y, w,   \
+                                                      part, kw);             \
     else if (map == AINV_MAP_COOP)                                           \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw, pm);                      \
+      launch_dot_coop<XT, KK, 256, NPP>(g, s, a, x, k, y, w, part, kw);      \
     else if (map == AINV_MAP_COOP128)                                        \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw, pm);                      \
+      launch_dot_coop<XT, KK, 128, NPP>(g, s, a, x, k, y, w, part, kw);      \
     else if constexpr (KK >= 16 && NPP == 2)                                 \
       /* 3K FP64 sums per lane do not fit: the cooperative kernel instead */ \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw, pm);                      \
+      launch_dot_coop<XT, KK, 128, NPP>(g, s, a, x, k, y, w, part, kw);      \
     else                                                                     \
-      hipLaunchKernelGGL((k_hll_spmv_dot_row<XT, KK, NPP>), dim3(g),             \
+      hipLaunchKernelGGL((k_hll_spmv_dot_row<XT, KK, NPP>), dim3(g),         \
                          dim3(kBlock), 0, s, a.n, a.nslices, sp, cl, vl, x,  \
-                         k, y, w, part, a.perm.as<int32_t>());               \
+                         k, y, w, part, pm);                                 \
   } while (0)
 // K = 64 exists only in the cooperative mapping; FP32 rows of 64 columns
 // (256 bytes) run as two 128-byte chunks in both variants (16 lanes per row
@@ -1434,22 +1463,14 @@ int spmv_dot_impl(const Hll &a, const XT *x, double *y, int k, int64_t ld,
 #define AINV_LC(KK, NPP)                                                     \
   do {                                                                       \
     if (map == AINV_MAP_RUN)                                                 \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, kRunChunkBytes, NPP,   \
-                                              1>),                           \
-                         dim3(g), dim3(kBlock), 0, s, a.n, a.ncols,          \
-                         a.nslices, sp, cl, vl, x, k, y, w, part, kw, pm);       \
+      launch_dot_coop<XT, KK, kRunChunkBytes, NPP, 1>(g, s, a, x, k, y, w,   \
+                                                      part, kw);             \
     else if constexpr (sizeof(XT) == 4)                                      \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw, pm);                      \
+      launch_dot_coop<XT, KK, 128, NPP>(g, s, a, x, k, y, w, part, kw);      \
     else if (map == AINV_MAP_COOP128)                                        \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 128, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw, pm);                      \
+      launch_dot_coop<XT, KK, 128, NPP>(g, s, a, x, k, y, w, part, kw);      \
     else                                                                     \
-      hipLaunchKernelGGL((k_hll_spmv_dot_coop<XT, KK, 256, NPP>), dim3(g),   \
-                         dim3(kBlock), 0, s, a.n, a.ncols, a.nslices, sp,    \
-                         cl, vl, x, k, y, w, part, kw, pm);                      \
+      launch_dot_coop<XT, KK, 256, NPP>(g, s, a, x, k, y, w, part, kw);      \
   } while (0)
 #define AINV_LRK(NPP)   \
   if (K == 2)           \
