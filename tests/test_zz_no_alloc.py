@@ -32,10 +32,16 @@ def noalloc(gpu, sv, x, y, alpha, beta, trans="N"):
 def test_noalloc_matches_apply(gpu, prec):
     cases = [(gpu.gen_pde3d(20), "INVK", dict(SUB_FILLIN=1, INV_FILLIN=1), 32),
              (gpu.gen_pde3d(9), "AINV", dict(AINV_ALG="LLK", SUB_FILLIN=8,
-                                             SUB_ILUTHRS=0.1), 5)]
+                                             SUB_ILUTHRS=0.1), 5),
+             # factors with rows sorted by length within 256-row windows
+             (gpu.gen_random(600, seed=5, band=200), "AINV",
+              dict(AINV_ALG="LLK", SUB_FILLIN=8, SUB_ILUTHRS=5e-2,
+                   SLICE_SORT=256), 3)]
     for a, kind, params, k in cases:  # the second call is the smaller one
         n = a.shape[0]
         sv = build(gpu, kind, a, prec, **params)
+        if "SLICE_SORT" in params:
+            assert sv.report()["z"]["sort_window"] == 256
         x = gpu.gen_normal(31, (n, k))
         y0 = gpu.gen_normal(32, (n, k))
         dx = gpu.DeviceArray.from_numpy(x)
