@@ -863,16 +863,17 @@ def c5_run(m, args, burst=True):
         srv.wait(t)
     t_warm = time.time() - t0
     st_warm = srv.stats()
-    # builds run on the worker that takes the matrix, while the other workers
-    # keep their batches running on the device
+    # builds run on the server's builder threads while the workers keep
+    # their batches on resident matrices running on the device
     first_touch = {
         "seconds": t_warm, "builds": st_warm["builds"],
         "build_seconds": st_warm["build_seconds"],
         "build_overlap_seconds": st_warm["build_overlap_seconds"],
         "overlap_frac": st_warm["build_overlap_seconds"] /
                         max(st_warm["build_seconds"], 1e-9),
-        "note": "build_seconds summed over workers; overlap: the build "
-                "seconds during which another worker ran a slot batch"}
+        "builders": st_warm.get("builders"),
+        "note": "build_seconds summed over the builder threads; overlap: "
+                "the build seconds during which a worker ran a slot batch"}
     log(f"first-touch builds + one solve per matrix: {t_warm:.1f}s "
         f"({first_touch})")
 

@@ -431,8 +431,10 @@ int ainv_gen_normal(uint64_t seed, int64_t count, double *out);
  * A server keeps each registered matrix's factors resident on the device
  * after first touch, under an HBM budget with LRU eviction of idle matrices.
  * Worker threads (one HIP stream each) batch the pending right-hand sides of
- * one matrix into up to 64 column slots; a converged column's slot is
- * refilled by the next pending request between two Krylov iterations.
+ * one resident matrix into up to 64 column slots; a converged column's slot
+ * is refilled by the next pending request between two Krylov iterations.
+ * First-touch builds and rebuilds after an eviction run on builder threads
+ * (as many as workers), so a build never holds a worker.
  * No counterpart in the reference (its drivers solve one system at a time);
  * serving is the deployment form of the apply + Krylov path. */
 typedef struct ainv_server *ainv_server_h;

@@ -70,9 +70,10 @@ struct ServedMatrix {
   ainv_solver *M = nullptr;
   int64_t bytes = 0;
   uint64_t last_use = 0;
-  bool busy = false;
+  bool busy = false;  // owned by a worker (served) or a builder (built)
   std::deque<int64_t> pending;
   int builds = 0;
+  bool resident() const { return A && M; }
 };
 
 }  // namespace
@@ -89,14 +90,18 @@ struct ainv_server {
   uint64_t clock = 0;
   int64_t resident = 0;
   bool stop = false;
-  int nworkers = 0;  // worker threads started (shares of the host cores)
+  int nworkers = 0;  // worker threads started
   int alive = 0;     // workers that may still serve (startup not failed)
+  // builder threads: first-touch builds (and rebuilds after eviction) run
+  // there, on the host cores and their own streams, while the workers keep
+  // serving resident matrices; with no builder alive the workers build
+  int nbuilders = 0, builders_alive = 0;
   hipStream_t copy = nullptr;  // uploads and downloads of request data
   // Held while a worker may capture a graph (batch_open) and around every
   // stream-ordered allocation or free: HIP refuses those while any stream
   // of the process is capturing.
   std::mutex hip_mu;
-  std::vector<std::thread> workers;
+  std::vector<std::thread> workers, builders;
   // statistics
   int64_t builds = 0, evictions = 0, completed = 0, admitted_midflight = 0;
   int64_t steps = 0, batches = 0;
@@ -185,8 +190,13 @@ void ensure_resident(ainv_server *S, ServedMatrix *m, int K) {
     M = new ainv_solver();
     M->kind = m->kind;
     M->device = S->device;
-    // the workers' builds share the host cores
-    M->host_threads = std::max(1, build_threads() / std::max(1, S->nworkers));
+    // concurrent builds share the host cores
+    int builders;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      builders = S->builders_alive > 0 ? S->nbuilders : S->nworkers;
+    }
+    M->host_threads = std::max(1, build_threads() / std::max(1, builders));
     for (auto &kv : m->params) apply_param(M, kv.first, kv.second);
     build_solver(M, m->a);
   } catch (...) {
@@ -387,6 +397,82 @@ void worker_startup_failed(ainv_server *S) {
     finish_request(S, r, AINV_ERR_HIP, "no server worker could start");
 }
 
+// The idle matrix whose oldest pending request is the oldest: among the
+// resident ones (which = 1), the non-resident ones (which = -1) or all (0).
+// The caller holds the lock.
+ServedMatrix *oldest_pending_locked(ainv_server *S, int which) {
+  ServedMatrix *m = nullptr;
+  double oldest = 0;
+  for (auto &kv : S->mats) {
+    ServedMatrix *c = kv.second.get();
+    if (c->busy || c->pending.empty()) continue;
+    if ((which > 0 && !c->resident()) || (which < 0 && c->resident()))
+      continue;
+    const double t = S->reqs[c->pending.front()]->t_submit;
+    if (!m || t < oldest) {
+      m = c;
+      oldest = t;
+    }
+  }
+  return m;
+}
+
+// Fails the requests still queued for m (after its build or batch threw).
+void fail_pending(ainv_server *S, ServedMatrix *m, int status,
+                  const std::string &msg) {
+  std::vector<Request *> fail;
+  {
+    std::unique_lock<std::mutex> lk(S->mu);
+    while (!m->pending.empty()) {
+      fail.push_back(S->reqs[m->pending.front()].get());
+      m->pending.pop_front();
+    }
+  }
+  for (Request *r : fail) finish_request(S, r, status, msg);
+}
+
+// A builder makes resident the non-resident matrix with the oldest pending
+// request, then hands it to the workers.
+void builder_main(ainv_server *S) {
+  if (hipSetDevice(S->device) != hipSuccess) {
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      --S->builders_alive;  // at zero the workers build themselves
+    }
+    S->cv_work.notify_all();
+    return;
+  }
+  for (;;) {
+    ServedMatrix *m = nullptr;
+    int K = 1;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      for (;;) {
+        if (S->stop) return;
+        m = oldest_pending_locked(S, -1);
+        if (m) break;
+        S->cv_work.wait(lk);
+      }
+      m->busy = true;
+      int cols = 0;
+      for (int64_t id : m->pending) cols += S->reqs[id]->k;
+      cols = std::min(std::max(cols, m->min_slots), kMaxCols);
+      while (K < cols) K <<= 1;
+    }
+    try {
+      ensure_resident(S, m, K);
+    } catch (const std::exception &e) {
+      fail_pending(S, m, AINV_ERR_INTERNAL, e.what());
+    }
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      m->busy = false;
+      m->last_use = ++S->clock;
+    }
+    S->cv_work.notify_all();
+  }
+}
+
 void worker_main(ainv_server *S) {
   hipStream_t s = nullptr;
   if (hipSetDevice(S->device) != hipSuccess ||
@@ -404,16 +490,8 @@ void worker_main(ainv_server *S) {
           (void)hipStreamDestroy(s);
           return;
         }
-        double oldest = 0;
-        for (auto &kv : S->mats) {
-          ServedMatrix *c = kv.second.get();
-          if (c->busy || c->pending.empty()) continue;
-          const double t = S->reqs[c->pending.front()]->t_submit;
-          if (!m || t < oldest) {
-            m = c;
-            oldest = t;
-          }
-        }
+        // resident matrices only, unless no builder is left to build
+        m = oldest_pending_locked(S, S->builders_alive > 0 ? 1 : 0);
         if (m) break;
         S->cv_work.wait(lk);
       }
@@ -422,16 +500,7 @@ void worker_main(ainv_server *S) {
     try {
       serve_matrix(S, m, s);
     } catch (const std::exception &e) {
-      // fail the requests still queued for this matrix
-      std::vector<Request *> fail;
-      {
-        std::unique_lock<std::mutex> lk(S->mu);
-        while (!m->pending.empty()) {
-          fail.push_back(S->reqs[m->pending.front()].get());
-          m->pending.pop_front();
-        }
-      }
-      for (Request *r : fail) finish_request(S, r, AINV_ERR_INTERNAL, e.what());
+      fail_pending(S, m, AINV_ERR_INTERNAL, e.what());
     }
     {
       std::unique_lock<std::mutex> lk(S->mu);
@@ -500,8 +569,24 @@ int ainv_server_create(int device, int64_t hbm_budget, int workers,
         }
       }
       S->nworkers = S->alive = (int)S->workers.size();
+      // as many builders as workers: the builds' host threads are shared
+      S->builders.reserve(S->workers.size());
+      for (size_t w = 0; w < S->workers.size(); ++w) {
+        try {
+          S->builders.emplace_back(builder_main, S);
+        } catch (const std::system_error &) {
+          break;  // the workers build what no builder takes
+        }
+      }
+      S->nbuilders = S->builders_alive = (int)S->builders.size();
     }
     if (S->workers.empty()) {
+      {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->stop = true;
+      }
+      S->cv_work.notify_all();
+      for (auto &t : S->builders) t.join();
       (void)hipStreamDestroy(S->copy);
       delete S;
       throw Error(AINV_ERR_INTERNAL, "cannot start a server worker thread");
@@ -598,7 +683,7 @@ int ainv_server_submit(ainv_server_h S, int32_t mat_id, int k, const double *b,
     *req_id = r->id;
     it->second->pending.push_back(r->id);
     S->reqs[r->id] = std::move(r);
-    S->cv_work.notify_one();
+    S->cv_work.notify_all();  // a worker or, for a matrix to build, a builder
   });
 }
 
@@ -674,6 +759,7 @@ int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
                   "\"batch_busy_seconds\": %.6f, \"evictions\": %lld, "
                   "\"completed\": %lld, \"batches\": %lld, \"steps\": %lld, "
                   "\"admitted_midflight\": %lld, \"workers\": %zu, "
+                  "\"builders\": %zu, "
                   "\"open_seconds\": %.6f, \"admit_seconds\": %.6f, "
                   "\"step_seconds\": %.6f, \"extract_seconds\": %.6f, "
                   "\"queue_seconds\": %.6f, \"service_seconds\": %.6f}",
@@ -684,6 +770,7 @@ int ainv_server_stats(ainv_server_h S, char *buf, size_t len) {
                   (long long)S->evictions, (long long)S->completed,
                   (long long)S->batches, (long long)S->steps,
                   (long long)S->admitted_midflight, S->workers.size(),
+                  S->builders.size(),
                   S->open_seconds, S->admit_seconds, S->step_seconds,
                   S->extract_seconds, S->queue_seconds, S->service_seconds);
   });
@@ -698,6 +785,7 @@ int ainv_server_free(ainv_server_h S) {
     }
     S->cv_work.notify_all();
     for (auto &t : S->workers) t.join();
+    for (auto &t : S->builders) t.join();
     (void)hipSetDevice(S->device);
     (void)hipDeviceSynchronize();  // the workers' streams are done
     {

@@ -34,6 +34,14 @@ def test_alone_vs_mix_and_midflight_admission(gpu):
     mats = pool(gpu)
     srv = gpu.Server(hbm_budget_bytes=8 << 30, workers=2)
     ids = [srv.add_matrix(a, kind, **p) for a, kind, p in mats]
+    # first touch: the builder threads build all three; the requests below
+    # then arrive while the workers' batches run on resident matrices
+    for m, (a, kind, p) in enumerate(mats):
+        b = gpu.gen_normal(90 + m, (a.shape[0], 1))
+        x, it, err, _ = srv.wait(srv.submit(ids[m], b, eps=1e-9))
+        assert np.all(err <= 1e-9)
+    st = srv.stats()
+    assert st["builds"] == 3 and st["builders"] == 2
     rng = np.random.default_rng(0)
     tickets = []
     for r in range(30):
@@ -57,7 +65,7 @@ def test_alone_vs_mix_and_midflight_admission(gpu):
             assert rel < 2e-5, (m, c, rel)
         assert lat > 0
     st = srv.stats()
-    assert st["completed"] == 30
+    assert st["completed"] == 33
     assert st["admitted_midflight"] >= 1
     assert st["builds"] == 3 and st["evictions"] == 0
     srv.free()
@@ -288,4 +296,41 @@ def test_multilevel_matrix_in_the_pool(gpu):
             assert tr < 1e-7, (one, c, tr)
     st = srv.stats()
     assert st["completed"] == 8
+    srv.free()
+
+
+def test_first_touch_build_does_not_block_serving(gpu):
+    """A first-touch build runs on a builder thread: with a single worker,
+    requests on a resident matrix submitted after a request on an unbuilt
+    one are served while it builds (the build overlaps their batches and
+    they finish first), and both matrices' results are right."""
+    big = gpu.gen_random(100000, seed=12, median=12.0, band=5000)
+    small = gpu.gen_pde3d(16)
+    srv = gpu.Server(hbm_budget_bytes=8 << 30, workers=1)
+    id_big = srv.add_matrix(big, "AINV", AINV_ALG="LLK", SUB_FILLIN=8,
+                            SUB_ILUTHRS=5e-2, FACTOR_PREC="F32")
+    id_small = srv.add_matrix(small, "INVK", SUB_FILLIN=1, INV_FILLIN=1,
+                              FACTOR_PREC="F32")
+    n = small.shape[0]
+    srv.wait(srv.submit(id_small, gpu.gen_normal(60, (n, 1)), eps=1e-9))
+    b_big = gpu.gen_normal(61, (big.shape[0], 4))
+    t_big = srv.submit(id_big, b_big, eps=1e-8)
+    tickets = [srv.submit(id_small, gpu.gen_normal(62 + r, (n, 8)), eps=1e-10)
+               for r in range(8)]
+    lat_small = []
+    for t in tickets:
+        x, it, err, lat = srv.wait(t)
+        assert np.all(err <= 1e-10)
+        lat_small.append(lat)
+    x, it, err, lat_big = srv.wait(t_big)
+    assert np.all(err <= 1e-8)
+    for c in range(4):
+        tr = (np.linalg.norm(b_big[:, c] - big @ x[:, c]) /
+              np.linalg.norm(b_big[:, c]))
+        assert tr < 1e-7
+    st = srv.stats()
+    assert st["builds"] == 2 and st["builders"] == 1
+    # the small matrix's first request was done while the big one built
+    assert lat_small[0] < lat_big, (lat_small, lat_big)
+    assert st["build_overlap_seconds"] > 0
     srv.free()
