@@ -87,6 +87,23 @@ def test_argument_errors_without_device(mld):
     assert e.value.code == mld.ERR_STATE
 
 
+def test_slice_sort_key_without_device(mld):
+    # SLICE_SORT: -1 (window from the row lengths), 0 (rows in order) or a
+    # multiple of 64; set before a build it touches no device
+    a = mld.MldDAinvSolver()
+    assert a.geti("SLICE_SORT") == -1
+    a.set("SLICE_SORT", 256)
+    assert a.geti("SLICE_SORT") == 256
+    c = mld.MldDAinvSolver()
+    a.clone_settings(c)
+    assert c.geti("SLICE_SORT") == 256
+    for bad in (-2, 1, 100):
+        with pytest.raises(mld.AinvError) as e:
+            a.set("SLICE_SORT", bad)
+        assert e.value.code == mld.ERR_ARG
+    assert a.geti("SLICE_SORT") == 256
+
+
 def test_fortran_layer_is_built_and_binds_the_abi():
     """amdflang builds the iso_c_binding module and the sample driver;
     the driver references the C entry points it binds."""
