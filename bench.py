@@ -70,7 +70,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM traffic of the dominant kernel from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_apply_pde3d160.json")
+TRAFFIC_FILES = {
+    "C2": os.path.join(ROOT, "profiles", "traffic_apply_pde3d160.json"),
+    "C4": os.path.join(ROOT, "profiles", "traffic_apply_random1m.json")}
 
 # name: (description, solver kind, parameters, method, rtol, slots (0: the
 #        served form times 8, 16, 32 and 64 slots on its first call and keeps
@@ -653,8 +655,9 @@ def main():
                "bytes_per_launch": bz, "ms_per_launch": ms_z,
                "slice_order": order_z if kmap_z == "run" else [0, 0]}]
     traffic = {}
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
+    tfile = TRAFFIC_FILES.get(args.config, "")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
             tr = json.load(f)
         if tr.get("config") == args.config and tr.get("n") == n:
             traffic = {(e["kernel"], tuple(e.get("slice_order", [0, 0]))):
