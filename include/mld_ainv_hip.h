@@ -336,8 +336,8 @@ int ainv_mat_free(ainv_mat_h a);
  * into 64-row slices, so that a slice pads little (SELL-C-sigma); window
  * -1 (the default at ainv_mat_create) picks the window from the row
  * lengths -- none when the rows in order pad by less than a quarter or the
- * sort saves less than a fifth, else the smallest of 256, 1024, 4096 and
- * 16384 rows within 5 % of the best -- 0 keeps the rows in order, else a
+ * sort saves less than a fifth, else the smallest of 256, 1024 and 4096
+ * rows within 5 % of the best of them -- 0 keeps the rows in order, else a
  * multiple of 64.  Products are the same in every layout; dot products
  * over rows are summed in another order.  Rebuilds the layout and drops the
  * matrix's Krylov plans and tuned mappings (AINV_ERR_STATE while a served

@@ -163,8 +163,8 @@ void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
                   int32_t ncols = -1, int32_t sigma = 0);
 // Window of the row sort for rows of the given lengths: 0 when sorting
 // saves less than a fifth of the unsorted layout's stored entries or the
-// layout pads less than a quarter; else the smallest of 256, 1024, 4096 and
-// 16384 rows whose stored entries are within 5 % of the largest window's.
+// layout pads less than a quarter; else the smallest of 256, 1024 and 4096
+// rows whose stored entries are within 5 % of the best of the three.
 int32_t hll_pick_sigma(const std::vector<int32_t> &len);
 // Stored entries of slices of 64 rows of these lengths, the rows sorted by
 // length (longest first, ties in row order) within windows of sigma rows

@@ -343,16 +343,20 @@ int32_t hll_pick_sigma(const std::vector<int32_t> &len) {
   for (const int32_t l : len) nnz += l;
   const int64_t p0 = hll_sorted_padding(len, 0);
   if (nnz == 0 || p0 * 4 < nnz * 5) return 0;  // pads < 25 %
-  constexpr int32_t win[] = {256, 1024, 4096, 16384};
-  int64_t pw[4], best = p0;
-  for (int i = 0; i < 4; ++i) {
+  // windows beyond 4096 rows pad a little less but spread a slice's rows,
+  // and so its gathers, further (C4: 16384 rows padded 1.04x against 1.19x
+  // and served 2% fewer solves/s, the single-column path 8% fewer)
+  constexpr int32_t win[] = {256, 1024, 4096};
+  constexpr int nwin = 3;
+  int64_t pw[nwin], best = p0;
+  for (int i = 0; i < nwin; ++i) {
     pw[i] = hll_sorted_padding(len, win[i]);
     best = std::min(best, pw[i]);
   }
   if (best * 5 > p0 * 4) return 0;  // saves < 20 %
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < nwin; ++i)
     if (pw[i] * 100 <= best * 105) return win[i];
-  return win[3];
+  return win[nwin - 1];
 }
 
 void hll_from_csr(const DevCsr &a, int prec, Hll &h, hipStream_t s,
