@@ -72,6 +72,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs, scaled by the same counters on a copy kernel of known size
 TRAFFIC_FILES = {
     "C2": os.path.join(ROOT, "profiles", "traffic_apply_pde3d160.json"),
+    "C3": os.path.join(ROOT, "profiles", "traffic_apply_aniso2048.json"),
     "C4": os.path.join(ROOT, "profiles", "traffic_apply_random1m.json")}
 
 # name: (description, solver kind, parameters, method, rtol, slots (0: the
